@@ -1,0 +1,1414 @@
+/*
+ * tpsc_oracle.c -- FP64 CPU restatement of the TPSC hot path (test infrastructure only;
+ * see tpsc_oracle.h).  Stage by stage it follows dbergeron1/TPSC_conductivity:
+ *   hamiltonien.cpp:14-34,49-68,119-155   dispersion, real-space hoppings
+ *   green0.cpp:182-262                    finite-lattice mu0
+ *   chi0.cpp:279-718                      chi0(q,tau) slabs, chi0(q,iqn) pencils
+ *   chi.cpp:37-206,210-417,738-876        V(r,tau), Usp/Uch sum rules
+ *   green.cpp:1673-2031,2929-3282         Sigma(k,iwn), traces, interacting mu
+ *   cond_opt.cpp:2537-4290                Lambda_jj bubble + Maki-Thompson + kx0
+ * Reductions use a fixed block decomposition so results do not depend on the thread count.
+ */
+#include "tpsc_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define PI 3.14159265358979323846
+#define NBLK 64
+
+void orc_spline_coeffs_rel_ws(const double *x0, const double *V, int N0, double *coeffs,
+                              double *B, double *r);
+
+static void *xmalloc(size_t n)
+{
+    void *p = malloc(n ? n : 1);
+    if (!p) {
+        fprintf(stderr, "tpsc_oracle: out of memory (%zu bytes)\n", n);
+        abort();
+    }
+    return p;
+}
+static double *dalloc(size_t n) { return (double *)xmalloc(n * sizeof(double)); }
+static double *dcalloc(size_t n)
+{
+    double *p = dalloc(n);
+    memset(p, 0, n * sizeof(double));
+    return p;
+}
+static void dfree(double **p)
+{
+    free(*p);
+    *p = NULL;
+}
+
+static inline long widx(long l, long m) { return m + (l * (l + 1)) / 2; }
+/* fold a lattice point onto the 1/8 wedge (green.cpp:1467-1476, cond_opt.cpp:2908-2917) */
+static inline long fold_idx(long x, long y, long N)
+{
+    if (x > N / 2) x = N - x;
+    if (y > N / 2) y = N - y;
+    if (y > x) {
+        long t = x;
+        x = y;
+        y = t;
+    }
+    return widx(x, y);
+}
+/* spline workspace sized for N0 points */
+typedef struct {
+    double *B, *r;
+} sws;
+static sws sws_new(int N0)
+{
+    sws w;
+    long N = 3L * (N0 - 1) - 1;
+    w.B = dalloc((size_t)N * 9);
+    w.r = dalloc((size_t)N);
+    return w;
+}
+static void sws_free(sws *w)
+{
+    free(w->B);
+    free(w->r);
+}
+
+/* ---------------------------------------------------------------- model (a1) */
+double orc_dispk(const orc *o, double kx, double ky)
+{
+    const double tp = o->p.tp, tpp = o->p.tpp, t3 = o->p.t3;
+    return -2.0 * (cos(kx) + cos(ky)) - 4.0 * tp * cos(kx) * cos(ky) - 2.0 * tpp * (cos(2 * kx) + cos(2 * ky)) -
+           4 * t3 * (cos(2 * kx) * cos(ky) + cos(2 * ky) * cos(kx));
+}
+double orc_grad_ek_x(const orc *o, double kx, double ky)
+{
+    const double tp = o->p.tp, tpp = o->p.tpp, t3 = o->p.t3;
+    return 2.0 * sin(kx) + 4.0 * tp * sin(kx) * cos(ky) + 4.0 * tpp * sin(2 * kx) +
+           4 * t3 * (2.0 * sin(2 * kx) * cos(ky) + cos(2 * ky) * sin(kx));
+}
+double orc_hess_ek_xx(const orc *o, double kx, double ky)
+{
+    const double tp = o->p.tp, tpp = o->p.tpp, t3 = o->p.t3;
+    return 2.0 * cos(kx) + 4.0 * tp * cos(kx) * cos(ky) + 8.0 * tpp * cos(2 * kx) +
+           4 * t3 * (4.0 * cos(2 * kx) * cos(ky) + cos(2 * ky) * cos(kx));
+}
+
+/* hamiltonien.cpp:49-68 (calc_ek) and :167-184 (set_system_size) */
+static void calc_ek(orc *o)
+{
+    const int nbk = o->nbk;
+    free(o->ek);
+    free(o->er);
+    o->ek = dalloc((size_t)nbk * nbk);
+    for (int l = 0; l < nbk; l++) {
+        const double kx = l * PI / (nbk - 1);
+        for (int m = 0; m < nbk; m++) {
+            const double ky = m * PI / (nbk - 1);
+            o->ek[m + l * nbk] = orc_dispk(o, kx, ky);
+        }
+    }
+    const long nk8 = (long)nbk * (nbk + 1) / 2;
+    o->er = dcalloc((size_t)(nk8 > 5 ? nk8 : 5));
+    o->er[1] = -1;
+    o->er[2] = -o->p.tp;
+    o->er[3] = -o->p.tpp;
+    o->er[4] = -o->p.t3;
+}
+
+/* green0.cpp:182-203 */
+double orc_ddensity_mu_PBC(orc *o, double mup)
+{
+    const int nbk = o->nbk;
+    const double Nk2 = 4.0 * (nbk - 1) * (nbk - 1);
+    double dens = 0;
+    for (int l = 0; l < nbk; l++)
+        for (int m = 0; m <= l; m++) {
+            double w = 8;
+            if (l == 0 || l == nbk - 1) w = w / 2;
+            if (m == 0 || m == nbk - 1) w = w / 2;
+            if (m == l) w = w / 2;
+            dens += 2 * w / ((exp((o->ek[m + l * nbk] - mup) / o->p.T) + 1.0) * Nk2);
+        }
+    return dens - o->p.n;
+}
+static double fz_density(void *ctx, double x) { return orc_ddensity_mu_PBC((orc *)ctx, x); }
+
+/* green0.cpp:240-262 */
+int orc_find_mu0(orc *o)
+{
+    if (o->p.n == 1.0 && o->p.tp == 0 && o->p.tpp == 0) {
+        o->mu0 = 0.0;
+        return 1;
+    }
+    const double init[3] = {0.0, -8.0, 8.0};
+    double root = 0.0;
+    if (orc_find_zero(fz_density, o, init, &root, NULL, NULL)) {
+        o->mu0 = root;
+        return 1;
+    }
+    return 0;
+}
+
+/* green0.h:44 */
+void orc_set_grid(orc *o, int nk0, int Nw0)
+{
+    o->nbk = nk0 / 2 + 1;
+    calc_ek(o);
+    o->nbw = Nw0;
+    o->nwmax = Nw0 / 2;
+    orc_find_mu0(o);
+}
+
+orc *orc_create(const orc_params *p)
+{
+    orc *o = (orc *)xmalloc(sizeof(orc));
+    memset(o, 0, sizeof(*o));
+    o->p = *p;
+    o->nsr = (p->n > 1.0) ? 2.0 - p->n : p->n; /* chi.cpp:18-19 */
+    o->N_theta = 1;
+    orc_set_grid(o, p->nk0, p->Nw0);
+    return o;
+}
+
+void orc_destroy(orc *o)
+{
+    if (!o) return;
+    free(o->ek);
+    free(o->er);
+    free(o->chiqw);
+    free(o->chiqt);
+    free(o->chirt);
+    free(o->dtau_chirt0);
+    free(o->dtau_Grt0p);
+    free(o->dtau_chiqt0);
+    free(o->chi0_inf);
+    free(o->chi0_inf2);
+    free(o->V);
+    free(o->Sigma_tau);
+    free(o->dtau_Sigma);
+    free(o->Self);
+    free(o->Sigma_inf2);
+    free(o->Sigma_inf3);
+    free(o->Sigma_inf4);
+    free(o->chijj1);
+    free(o->chijj2);
+    free(o->chijj1_k);
+    free(o->chijj2_k);
+    free(o);
+}
+
+/* fill a square nk x nk slab symmetrically from a packed-wedge array */
+static void wedge_to_square(const double *w, double *sq, int nk)
+{
+    for (int l = 0; l < nk; l++)
+        for (int m = 0; m <= l; m++) {
+            const double v = w[widx(l, m)];
+            sq[m + l * nk] = v;
+            sq[l + m * nk] = v;
+        }
+}
+static void square_to_wedge(const double *sq, double *w, int nk)
+{
+    for (int l = 0; l < nk; l++)
+        for (int m = 0; m <= l; m++) w[widx(l, m)] = sq[m + l * nk];
+}
+
+/* ---------------------------------------------------------------- chi0 (a3, a4) */
+int orc_chi0(orc *o, int nq0, int nw0)
+{
+    const int nk = nq0 / 2 + 1;
+    const double nk02 = (double)nq0 * nq0;
+    const int nk2 = nk * nk;
+    const long nk8 = (long)nk * (nk + 1) / 2;
+    const int nw = nw0 / 2 + 1;
+    const double T = o->p.T;
+
+    orc_set_grid(o, nq0, nw0); /* chi0.cpp:297 */
+    const double mu0 = o->mu0;
+
+    free(o->chiqw);
+    free(o->chiqt);
+    free(o->chirt);
+    free(o->dtau_chirt0);
+    free(o->dtau_Grt0p);
+    free(o->dtau_chiqt0);
+    free(o->chi0_inf);
+    free(o->chi0_inf2);
+    o->chiqw = dalloc((size_t)nw * nk8);
+    o->chirt = dalloc((size_t)nw * nk8);
+    o->dtau_chirt0 = dalloc(nk8);
+    o->dtau_Grt0p = dalloc(nk8);
+    o->dtau_chiqt0 = dalloc(nk8);
+    o->chi0_inf = dalloc(nk8);
+    o->chi0_inf2 = dalloc(nk8);
+
+    double *ekmu = dalloc(nk8), *Gkt = dalloc(nk8), *Grt0 = dalloc(nk8), *dGm = dalloc(nk8);
+    double *sq = dalloc(nk2), *wv = dalloc(nk8);
+    for (int l = 0; l < nk; l++)
+        for (int m = 0; m <= l; m++) {
+            const long i = widx(l, m);
+            ekmu[i] = o->ek[m + l * nk] - mu0;
+            Gkt[i] = -1.0 / (exp(-ekmu[i] / T) + 1.0);
+        }
+    /* d/dtau G(r,tau) at 0+ and 0- (chi0.cpp:323-367) */
+    dGm[0] = -mu0;
+    for (long i = 1; i < nk8; i++) dGm[i] = o->er[i];
+    for (long i = 0; i < nk8; i++) wv[i] = -ekmu[i] * Gkt[i] / nk02;
+    wedge_to_square(wv, sq, nk);
+    orc_r2r_2d(sq, nk, nk, 0, 0);
+    square_to_wedge(sq, wv, nk);
+    for (long i = 0; i < nk8; i++) {
+        dGm[i] -= wv[i];
+        o->dtau_Grt0p[i] = wv[i];
+    }
+    /* tau = 0 slice (chi0.cpp:375-418) */
+    for (long i = 0; i < nk8; i++) wv[i] = Gkt[i] / nk02;
+    wedge_to_square(wv, sq, nk);
+    orc_r2r_2d(sq, nk, nk, 0, 0);
+    square_to_wedge(sq, Grt0, nk);
+    for (long i = 0; i < nk8; i++) wv[i] = -2.0 * Grt0[i] * Grt0[i];
+    wv[0] = -2.0 * Grt0[0] * (Grt0[0] + 1.0);
+    memcpy(o->chirt, wv, sizeof(double) * nk8);
+    wedge_to_square(wv, sq, nk);
+    orc_r2r_2d(sq, nk, nk, 0, 0);
+    square_to_wedge(sq, o->chiqw, nk);
+    /* d/dtau chi0 at tau=0 (chi0.cpp:420-447) */
+    for (long i = 0; i < nk8; i++) wv[i] = -2.0 * (o->dtau_Grt0p[i] * Grt0[i] + dGm[i] * Grt0[i]);
+    wv[0] = -2.0 * (o->dtau_Grt0p[0] * (1.0 + Grt0[0]) + dGm[0] * Grt0[0]);
+    memcpy(o->dtau_chirt0, wv, sizeof(double) * nk8);
+    wedge_to_square(wv, sq, nk);
+    orc_r2r_2d(sq, nk, nk, 0, 0);
+    square_to_wedge(sq, o->dtau_chiqt0, nk);
+
+    /* 0 < tau < beta/2 (chi0.cpp:454-523) */
+#pragma omp parallel
+    {
+        double *s = dalloc(nk2), *w1 = dalloc(nk8), *w2 = dalloc(nk8);
+#pragma omp for schedule(static)
+        for (int j = 1; j < nw - 1; j++) {
+            const double t = j / (nw0 * T);
+            for (long i = 0; i < nk8; i++) {
+                const double x = ekmu[i];
+                if (x >= 0.0)
+                    w1[i] = exp(-t * x) * Gkt[i] / nk02;
+                else
+                    w1[i] = -exp((1.0 / T - t) * x) / (exp(x / T) + 1.0) / nk02;
+            }
+            wedge_to_square(w1, s, nk);
+            orc_r2r_2d(s, nk, nk, 0, 0);
+            square_to_wedge(s, w1, nk); /* G(r,t) */
+            for (long i = 0; i < nk8; i++) {
+                const double x = ekmu[i];
+                if (x >= 0.0)
+                    w2[i] = -exp((t - 1.0 / T) * x) * Gkt[i] / nk02;
+                else
+                    w2[i] = exp(t * x) / ((exp(x / T) + 1.0) * nk02);
+            }
+            wedge_to_square(w2, s, nk);
+            orc_r2r_2d(s, nk, nk, 0, 0);
+            square_to_wedge(s, w2, nk); /* G(r,-t) */
+            for (long i = 0; i < nk8; i++) w2[i] = -2.0 * w2[i] * w1[i];
+            memcpy(o->chirt + (size_t)j * nk8, w2, sizeof(double) * nk8);
+            wedge_to_square(w2, s, nk);
+            orc_r2r_2d(s, nk, nk, 0, 0);
+            square_to_wedge(s, o->chiqw + (size_t)j * nk8, nk);
+        }
+        free(s);
+        free(w1);
+        free(w2);
+    }
+    /* tau = beta/2 (chi0.cpp:527-559) */
+    {
+        const int j = nw - 1;
+        const double t = 1.0 / (2.0 * T);
+        for (long i = 0; i < nk8; i++) {
+            const double x = ekmu[i];
+            if (x >= 0.0)
+                wv[i] = exp(-t * x) * Gkt[i] / nk02;
+            else
+                wv[i] = -exp((1.0 / T - t) * x) / ((exp(x / T) + 1.0) * nk02);
+        }
+        wedge_to_square(wv, sq, nk);
+        orc_r2r_2d(sq, nk, nk, 0, 0);
+        for (int i = 0; i < nk2; i++) sq[i] = 2.0 * sq[i] * sq[i];
+        square_to_wedge(sq, o->chirt + (size_t)j * nk8, nk);
+        orc_r2r_2d(sq, nk, nk, 0, 0);
+        square_to_wedge(sq, o->chiqw + (size_t)j * nk8, nk);
+    }
+    free(ekmu);
+    free(Gkt);
+    free(Grt0);
+    free(dGm);
+    free(sq);
+    free(wv);
+
+    o->chiqt = dalloc((size_t)nw * nk8);
+    memcpy(o->chiqt, o->chiqw, sizeof(double) * (size_t)nw * nk8);
+
+    /* tau -> iqn by clamped spline + DFT (chi0.cpp:566-698) */
+    const int NS0 = nw;
+#pragma omp parallel
+    {
+        double *tau = dalloc(NS0), *val = dalloc(NS0), *cf = dalloc(4 * (size_t)(NS0 - 1));
+        double complex *z = (double complex *)xmalloc(sizeof(double complex) * (size_t)nw0);
+        sws ws = sws_new(NS0);
+#pragma omp for schedule(dynamic, 16)
+        for (long q = 0; q < nk8; q++) {
+            for (int j = 0; j < NS0; j++) {
+                tau[j] = j / (nw0 * T);
+                val[j] = o->chiqw[q + (size_t)j * nk8];
+            }
+            cf[0] = o->dtau_chiqt0[q];
+            cf[1] = 0;
+            orc_spline_coeffs_rel_ws(tau, val, NS0, cf, ws.B, ws.r);
+            for (int j = 0; j < nw0 / 2; j++) z[j] = 6.0 * cf[4 * j];
+            for (int j = nw0 / 2; j < nw0; j++) z[j] = -z[nw0 - j - 1];
+            orc_dft(z, nw0, +1);
+            double integ = 0;
+            for (int j = 0; j < nw0 / 2; j++) {
+                const double a = cf[4 * j], b = cf[4 * j + 1], c = cf[4 * j + 2], d = cf[4 * j + 3];
+                const double x = tau[j + 1] - tau[j];
+                integ += a * x * x * x * x / 4.0 + b * x * x * x / 3.0 + c * x * x / 2.0 + d * x;
+            }
+            o->chiqw[q] = 2 * integ;
+            if (q != 0) {
+                o->chi0_inf[q] = -2.0 * o->dtau_chiqt0[q];
+                for (int j = 1; j < nw; j++) {
+                    const double wn = 2 * j * PI * T;
+                    o->chiqw[q + (size_t)j * nk8] =
+                        o->chi0_inf[q] / (wn * wn) +
+                        ((1.0 - cos(2 * j * PI / nw0)) * creal(z[j]) + sin(2 * j * PI / nw0) * cimag(z[j])) /
+                            (wn * wn * wn * wn);
+                }
+                o->chi0_inf2[q] = 2.0 * creal(z[nw - 1]);
+            } else {
+                o->chi0_inf[0] = 0;
+                o->chi0_inf2[0] = 0;
+                for (int j = 1; j < nw; j++) o->chiqw[(size_t)j * nk8] = 0;
+            }
+        }
+        free(tau);
+        free(val);
+        free(cf);
+        free(z);
+        sws_free(&ws);
+    }
+    /* chi0max (chi0.cpp:703-717) */
+    o->chi0max = 0.0;
+    o->qmax[0] = o->qmax[1] = 0;
+    for (int l = nk - 1; l >= 0; l--)
+        for (int m = l; m >= 0; m--)
+            if (o->chiqw[widx(l, m)] > o->chi0max) {
+                o->chi0max = o->chiqw[widx(l, m)];
+                o->qmax[0] = l;
+                o->qmax[1] = m;
+            }
+    return 0;
+}
+
+/* chi0.cpp:1588-1651 */
+static double chi0_integrand(orc *o, double kx, double ky, double qx, double qy, double wn)
+{
+    const double T = o->p.T, mu0 = o->mu0;
+    const double e = orc_dispk(o, kx, ky), ekq = orc_dispk(o, kx + qx, ky + qy);
+    double de = e - ekq;
+    if (qx == 0.0 && qy == 0.0) de = 0.0;
+    const double expk = exp((e - mu0) / T);
+    if (wn != 0.0) {
+        if (de != 0.0) return (1.0 / (expk + 1.0) - 1.0 / (exp((ekq - mu0) / T) + 1.0)) * de / (wn * wn + de * de);
+        return 0;
+    } else if (fabs(de) > 1e-12)
+        return (1.0 / (expk + 1.0) - 1.0 / (exp((ekq - mu0) / T) + 1.0)) / de;
+    if ((e - mu0) < 0) return -(1.0 / T) * expk / ((expk + 1.0) * (expk + 1.0));
+    const double expkm = exp(-(e - mu0) / T);
+    return -(1.0 / T) * expkm / ((expkm + 1.0) * (expkm + 1.0));
+}
+double orc_chiqw_discret(orc *o, int qx, int qy, int n)
+{
+    const int nq0 = 2 * (o->nbk - 1);
+    const double Qx = qx * 2 * PI / nq0, Qy = qy * 2 * PI / nq0, wn = 2 * n * PI * o->p.T;
+    double s = 0;
+    for (int j = -nq0 / 2; j < nq0 / 2; j++) {
+        const double kx = j * 2 * PI / nq0;
+        for (int l = -nq0 / 2; l < nq0 / 2; l++) s -= 2 * chi0_integrand(o, kx, l * 2 * PI / nq0, Qx, Qy, wn) / ((double)nq0 * nq0);
+    }
+    return s;
+}
+
+/* chi0.cpp:1498-1584 (nwr = 0) */
+double orc_trace_chi0(orc *o, double *tr_corr)
+{
+    const int nbq = o->nbk, nwmax = o->nwmax;
+    const long nk8 = (long)nbq * (nbq + 1) / 2;
+    const double Nq = 4.0 * (nbq - 1) * (nbq - 1), T = o->p.T;
+    const int N1 = nwmax + 1;
+    double tr = 0.0;
+    for (int j = 0; j < N1; j++)
+        for (int l = 0; l < nbq; l++)
+            for (int m = 0; m <= l; m++) {
+                int w = 16;
+                if (j == 0 || j == nwmax) w /= 2;
+                if (l == 0 || l == nbq - 1) w /= 2;
+                if (m == 0 || m == nbq - 1) w /= 2;
+                if (m == l) w /= 2;
+                tr += w * T * o->chiqw[widx(l, m) + (size_t)j * nk8] / Nq;
+            }
+    const double tr1 = tr;
+    for (int j = nwmax + 1; j < 2 * nwmax; j++)
+        for (int l = 0; l < nbq; l++)
+            for (int m = 0; m <= l; m++) {
+                int w = 16;
+                if (l == 0 || l == nbq - 1) w /= 2;
+                if (m == 0 || m == nbq - 1) w /= 2;
+                if (m == l) w /= 2;
+                const double wn = 2 * PI * j * T;
+                const long i = widx(l, m);
+                const double c = o->chi0_inf[i] / (wn * wn) + o->chi0_inf2[i] / (wn * wn * wn * wn);
+                tr += w * T * c / Nq;
+            }
+    if (tr_corr) *tr_corr = tr + N1 * (tr1 - tr) / (N1 - 2 * nwmax);
+    return tr;
+}
+
+/* ---------------------------------------------------------------- sum rules (a5) */
+/* chi.cpp:738-843: channel 0 = spin (sum_rule_sp), 1 = charge (sum_rule_ch) */
+double orc_sum_rule(orc *o, int channel, double Ux)
+{
+    const int nbq = o->nbk, nwmax = o->nwmax;
+    const long nk8 = (long)nbq * (nbq + 1) / 2;
+    const double Nq = 4.0 * (nbq - 1) * (nbq - 1), T = o->p.T;
+    const double s = channel == 0 ? -1.0 : 1.0; /* chi/(1 + s*U*chi/2) */
+    double part[NBLK];
+    const long nterm = (long)(2 * nwmax + 1) * nk8;
+#pragma omp parallel for schedule(static)
+    for (int b = 0; b < NBLK; b++) {
+        const long i0 = nterm * b / NBLK, i1 = nterm * (b + 1) / NBLK;
+        double acc = 0;
+        long l = 0, m = 0;
+        /* decode (j, idx) from the flat index i = j*nk8 + idx */
+        for (long i = i0; i < i1; i++) {
+            const long j = i / nk8, idx = i % nk8;
+            if (i == i0 || idx == 0) {
+                /* recover (l,m) from idx */
+                l = (long)((sqrt(8.0 * idx + 1.0) - 1.0) / 2.0);
+                while (widx(l + 1, 0) <= idx) l++;
+                while (widx(l, 0) > idx) l--;
+                m = idx - widx(l, 0);
+            } else {
+                m++;
+                if (m > l) {
+                    l++;
+                    m = 0;
+                }
+            }
+            int w = 16;
+            if (j == 0) w /= 2;
+            if (l == 0 || l == nbq - 1) w /= 2;
+            if (m == 0 || m == nbq - 1) w /= 2;
+            if (m == l) w /= 2;
+            double c;
+            if (j <= nwmax)
+                c = o->chiqw[idx + j * nk8];
+            else {
+                const double wn = 2 * j * PI * T;
+                c = o->chi0_inf[idx] / (wn * wn) + o->chi0_inf2[idx] / (wn * wn * wn * wn);
+            }
+            acc += w * T / Nq * (c / (1 + s * Ux * c / 2.0) - c);
+        }
+        part[b] = acc;
+    }
+    double tr = 0;
+    for (int b = 0; b < NBLK; b++) tr += part[b];
+    const double n = o->p.n, nsr = o->nsr, U = o->p.U;
+    if (channel == 0) return tr + n - n * n / 2.0 - nsr + Ux * nsr * nsr / (2.0 * U);
+    return tr + n - n * n / 2.0 - (nsr + o->Usp * nsr * nsr / (2.0 * U) - nsr * nsr);
+}
+static double fz_sp(void *ctx, double x) { return orc_sum_rule((orc *)ctx, 0, x); }
+static double fz_ch(void *ctx, double x) { return orc_sum_rule((orc *)ctx, 1, x); }
+
+/* chi.cpp:846-876 */
+int orc_find_Usp_Uch_grid(orc *o, int nq0, int nw0, double Usc[2])
+{
+    orc_chi0(o, nq0, nw0);
+    const double U = o->p.U;
+    double root = 0.0;
+    double init[3] = {U / 2.0, 0.0, 2.0 / o->chi0max - 2 * DBL_EPSILON};
+    int ok = 1;
+    if (orc_find_zero(fz_sp, o, init, &root, NULL, &o->nevals_sp)) {
+        Usc[0] = root;
+        o->Usp = root;
+    } else
+        ok = 0;
+    root = 0.0;
+    init[0] = 0;
+    init[2] = 1000.0 * U;
+    if (orc_find_zero(fz_ch, o, init, &root, NULL, &o->nevals_ch))
+        Usc[1] = root;
+    else
+        ok = 0;
+    return ok;
+}
+
+/* chi.cpp:210-417 */
+int orc_find_Usp_Uch(orc *o)
+{
+    double Usc[2] = {0.0, 0.0};
+    const int N = o->p.nk0, Nw = o->p.Nw0;
+    o->Usp_coarse = o->Uch_coarse = 0.0;
+    if (o->p.coarse_pass) {
+        orc_find_Usp_Uch_grid(o, N / 2, Nw / 2, Usc);
+        o->Usp_coarse = Usc[0];
+        o->Uch_coarse = Usc[1];
+    }
+    int ok = orc_find_Usp_Uch_grid(o, N, Nw, Usc);
+    o->difRelsp = fabs((Usc[0] - o->Usp_coarse) / Usc[0]);
+    o->difRelch = fabs((Usc[1] - o->Uch_coarse) / Usc[1]);
+    o->Usp = Usc[0];
+    o->Uch = Usc[1];
+    const double U = o->p.U, nsr = o->nsr;
+    if (o->p.n > 1.0)
+        o->dblOcc = o->Usp / U * nsr * nsr / 4.0 + 1.0 - nsr;
+    else
+        o->dblOcc = o->Usp / U * nsr * nsr / 4.0;
+    return ok;
+}
+
+/* ---------------------------------------------------------------- vertex (a6) */
+/* chi.cpp:37-206 */
+int orc_vertex(orc *o)
+{
+    const int nbq = o->nbk, nw = o->nwmax + 1, nq0 = 2 * (nbq - 1);
+    const long nk8 = (long)nbq * (nbq + 1) / 2;
+    const double U = o->p.U, Usp = o->Usp, Uch = o->Uch, T = o->p.T;
+    const double normFact = T / ((double)nq0 * nq0);
+    free(o->V);
+    o->V = dalloc((size_t)nw * nk8);
+#pragma omp parallel
+    {
+        double *s = dalloc((size_t)nbq * nbq), *w = dalloc(nk8);
+#pragma omp for schedule(static)
+        for (int j = 0; j < nw; j++) {
+            for (long i = 0; i < nk8; i++) {
+                const double c = o->chiqw[i + (size_t)j * nk8];
+                const double csp = c / (1.0 - Usp * c / 2.0), cch = c / (1.0 + Uch * c / 2.0);
+                w[i] = normFact * U * (3.0 * Usp * (csp - c) + Uch * (cch - c)) / 8.0;
+            }
+            wedge_to_square(w, s, nbq);
+            orc_r2r_2d(s, nbq, nbq, 0, 0);
+            square_to_wedge(s, o->V + (size_t)j * nk8, nbq);
+        }
+        free(s);
+        free(w);
+    }
+#pragma omp parallel
+    {
+        double *v = dalloc(nw);
+#pragma omp for schedule(static)
+        for (long i = 0; i < nk8; i++) {
+            for (int j = 0; j < nw; j++) v[j] = o->V[i + (size_t)j * nk8];
+            orc_redft00_1d(v, nw);
+            for (int j = 0; j < nw; j++) o->V[i + (size_t)j * nk8] = v[j];
+        }
+        free(v);
+    }
+    const double fact = U * (3.0 * Usp + Uch) / 8.0;
+    for (long i = 0; i < (long)nw * nk8; i++) o->V[i] += fact * o->chirt[i];
+    return 0;
+}
+
+/* ---------------------------------------------------------------- self-energy (a7, a8) */
+/* green.cpp:1737-2031 */
+int orc_self_energy(orc *o)
+{
+    const int nk = o->nbk, nk2 = nk * nk, nk0 = 2 * (nk - 1);
+    const double nk02 = (double)nk0 * nk0;
+    const long nk8 = (long)nk * (nk + 1) / 2;
+    const int nw = o->nwmax + 1, nbw = 2 * (nw - 1);
+    const double T = o->p.T, mu0 = o->mu0, U = o->p.U;
+    const double fact = U * (3.0 * o->Usp + o->Uch) / 8.0;
+
+    double *ekmu = dalloc(nk2);
+    for (int i = 0; i < nk2; i++) ekmu[i] = o->ek[i] - mu0;
+    free(o->dtau_Sigma);
+    o->dtau_Sigma = dalloc(2 * (size_t)nk8);
+    double *sq = dalloc(nk2), *g = dalloc(nk8), *w = dalloc(nk8);
+    /* tau = 0 (green.cpp:1745-1777) */
+    for (int i = 0; i < nk2; i++) sq[i] = -1.0 / (exp(-ekmu[i] / T) + 1.0) / nk02;
+    orc_r2r_2d(sq, nk, nk, 0, 0);
+    square_to_wedge(sq, g, nk);
+    for (long i = 0; i < nk8; i++) w[i] = fact * o->dtau_chirt0[i] * g[i] + o->V[i] * o->dtau_Grt0p[i];
+    wedge_to_square(w, sq, nk);
+    orc_r2r_2d(sq, nk, nk, 0, 0);
+    square_to_wedge(sq, o->dtau_Sigma, nk);
+    /* tau = beta (green.cpp:1780-1817) */
+    for (int i = 0; i < nk2; i++) sq[i] = -1.0 / (exp(ekmu[i] / T) + 1.0) / nk02;
+    orc_r2r_2d(sq, nk, nk, 0, 0);
+    square_to_wedge(sq, g, nk);
+    for (long i = 0; i < nk8; i++) {
+        const double xi = (i == 0) ? -mu0 : o->er[i];
+        w[i] = -fact * o->dtau_chirt0[i] * g[i] + o->V[i] * (-o->dtau_Grt0p[i] + xi);
+    }
+    wedge_to_square(w, sq, nk);
+    orc_r2r_2d(sq, nk, nk, 0, 0);
+    square_to_wedge(sq, o->dtau_Sigma + nk8, nk);
+    free(sq);
+    free(g);
+    free(w);
+
+    /* Sigma(k,tau), tau_j = j beta/Nw, j = 0..Nw (green.cpp:1822-1892) */
+    free(o->Sigma_tau);
+    o->Sigma_tau = dalloc((size_t)(nbw + 1) * nk8);
+#pragma omp parallel
+    {
+        double *s = dalloc(nk2), *wv = dalloc(nk8);
+#pragma omp for schedule(static)
+        for (int j = 0; j <= nbw; j++) {
+            const double t = j / (nbw * T);
+            for (int i = 0; i < nk2; i++) {
+                const double x = ekmu[i];
+                if (x > 0)
+                    s[i] = -exp(-t * x) / (exp(-x / T) + 1.0) / nk02;
+                else
+                    s[i] = -exp((1.0 / T - t) * x) / (exp(x / T) + 1.0) / nk02;
+            }
+            orc_r2r_2d(s, nk, nk, 0, 0);
+            const int jv = (j < nw) ? j : nbw - j; /* V(r, beta - tau) for tau > beta/2 */
+            square_to_wedge(s, wv, nk);
+            for (long i = 0; i < nk8; i++) wv[i] *= o->V[i + (size_t)jv * nk8];
+            wedge_to_square(wv, s, nk);
+            orc_r2r_2d(s, nk, nk, 0, 0);
+            square_to_wedge(s, o->Sigma_tau + (size_t)j * nk8, nk);
+        }
+        free(s);
+        free(wv);
+    }
+    free(ekmu);
+
+    /* tau -> iwn by clamped spline + twiddled DFT (green.cpp:1899-2018) */
+    free(o->Self);
+    free(o->Sigma_inf2);
+    free(o->Sigma_inf3);
+    free(o->Sigma_inf4);
+    o->Self = (double complex *)xmalloc(sizeof(double complex) * (size_t)nw * nk8);
+    memset(o->Self, 0, sizeof(double complex) * (size_t)nw * nk8);
+    o->Sigma_inf2 = dalloc(nk8);
+    o->Sigma_inf3 = dalloc(nk8);
+    o->Sigma_inf4 = dalloc(nk8);
+    const int NS0 = nbw + 1;
+    double sig_inf = 0;
+#pragma omp parallel
+    {
+        double *tau = dalloc(NS0), *val = dalloc(NS0), *cf = dalloc(4 * (size_t)nbw);
+        double complex *z = (double complex *)xmalloc(sizeof(double complex) * (size_t)nbw);
+        sws ws = sws_new(NS0);
+#pragma omp for schedule(dynamic, 16)
+        for (long l = 0; l < nk8; l++) {
+            for (int j = 0; j <= nbw; j++) {
+                tau[j] = j / (nbw * T);
+                val[j] = o->Sigma_tau[l + (size_t)j * nk8];
+            }
+            cf[0] = o->dtau_Sigma[l];
+            cf[1] = o->dtau_Sigma[l + nk8];
+            orc_spline_coeffs_rel_ws(tau, val, NS0, cf, ws.B, ws.r);
+            const double d2S0 = 2 * cf[1];
+            const double d2Sb = 6.0 * cf[4 * nbw - 4] / (nbw * T) + 2.0 * cf[4 * nbw - 3];
+            for (int j = 0; j < nbw; j++) {
+                const double ang = j * PI / nbw;
+                z[j] = (cos(ang) + I * sin(ang)) * (6.0 * cf[4 * j]);
+            }
+            orc_dft(z, nbw, +1);
+            const double c1 = val[0] + val[nbw];
+            if (l == 0) sig_inf = -c1;
+            o->Sigma_inf2[l] = o->dtau_Sigma[l] + o->dtau_Sigma[l + nk8];
+            o->Sigma_inf3[l] = -(d2S0 + d2Sb);
+            for (int j = 0; j < nbw / 2; j++) {
+                const double ang = (2 * j + 1) * PI / nbw;
+                o->Self[l + (size_t)j * nk8] = (1.0 - (cos(ang) + I * sin(ang))) * z[j];
+            }
+            o->Sigma_inf4[l] = creal(o->Self[l + (size_t)(nbw / 2 - 1) * nk8]);
+        }
+        free(tau);
+        free(val);
+        free(cf);
+        free(z);
+        sws_free(&ws);
+    }
+    o->Sigma_inf = sig_inf;
+    return 0;
+}
+
+/* brute-force Sigma(k,iwn), green.cpp:1401-1494 (needs chi0(q,iqn) kept) */
+double complex orc_self_kw_brute(orc *o, int kx, int ky, int n)
+{
+    const int nbk = o->nbk, nk0 = 2 * (nbk - 1), nbw = o->nbw;
+    const double Nk = (double)nk0 * nk0, T = o->p.T, U = o->p.U;
+    const long nk8 = (long)nbk * (nbk + 1) / 2;
+    const double kn = (2 * n + 1) * PI * T;
+    double complex s = 0.0;
+    for (long j = 0; j < 2L * nbw - 1; j++) {
+        double qn;
+        long wn;
+        if (j < nbw) {
+            qn = 2.0 * j * PI * T;
+            wn = j;
+        } else {
+            qn = 2.0 * (j - 2 * nbw + 1) * PI * T;
+            wn = 2 * nbw - j - 1;
+        }
+        for (long l = 0; l < nk0; l++) {
+            long p = l + kx;
+            if (p > nk0) p -= nk0;
+            if (p > nbk - 1) p = nk0 - p;
+            for (long m = 0; m < nk0; m++) {
+                long q = m + ky;
+                if (q > nk0) q -= nk0;
+                if (q > nbk - 1) q = nk0 - q;
+                const double ekq = o->ek[q + p * nbk];
+                const long i = fold_idx(l, m, nk0);
+                double c;
+                if (wn <= nbw / 2)
+                    c = o->chiqw[i + wn * nk8];
+                else
+                    c = o->chi0_inf[i] / (qn * qn) + o->chi0_inf2[i] / (qn * qn * qn * qn);
+                const double csp = c / (1.0 - o->Usp * c / 2.0), cch = c / (1.0 + o->Uch * c / 2.0);
+                const double complex G = 1.0 / ((-ekq + o->mu0) + I * (kn + qn));
+                s += (U / 8.0) * (T / Nk) * (3.0 * o->Usp * csp + o->Uch * cch) * G;
+            }
+        }
+    }
+    return s;
+}
+
+/* Sigma(k, iwn) from its moment representation, n >= 0 (green.cpp:3039) */
+static inline double complex sigma_at(const orc *o, long i, long j, double wn)
+{
+    const double complex z = I * wn;
+    const long nk8 = (long)o->nbk * (o->nbk + 1) / 2;
+    return o->Sigma_inf / z + o->Sigma_inf2[i] / (z * z) + o->Sigma_inf3[i] / (z * z * z) +
+           o->Self[i + j * nk8] / (z * z * z * z);
+}
+
+static inline int wedge_w8(int l, int m, int nk)
+{
+    int w = 8;
+    if (l == 0 || l == nk - 1) w /= 2;
+    if (m == 0 || m == nk - 1) w /= 2;
+    if (m == l) w /= 2;
+    return w;
+}
+
+/* ---------------------------------------------------------------- traces, mu (a9) */
+/* green.cpp:3001-3104 */
+int orc_trace_self_G(orc *o)
+{
+    const int nk = o->nbk, nbw = o->nbw;
+    const double nk02 = 4.0 * (nk - 1) * (nk - 1), T = o->p.T, mu0 = o->mu0;
+    const int wl = nbw / 2;
+    double sum = 0, sum2 = 0;
+    for (int l = 0; l < nk; l++)
+        for (int m = 0; m <= l; m++) {
+            const double e = o->ek[m + l * nk];
+            const int w = wedge_w8(l, m, nk);
+            const long i = widx(l, m);
+            for (int j = 0; j < wl; j++) {
+                const double wn = (2 * j + 1) * PI * T;
+                const double complex s = sigma_at(o, i, j, wn);
+                const double tmp = w * T * 2 / (nk02 * (wn * wn + (e - mu0) * (e - mu0)));
+                sum += tmp * (creal(s) * (mu0 - e) + cimag(s) * wn);
+            }
+        }
+    for (int l = 0; l < nk; l++)
+        for (int m = 0; m <= l; m++) {
+            const double e = o->ek[m + l * nk];
+            const int w = wedge_w8(l, m, nk);
+            for (int j = 0; j < wl; j++) {
+                const double wn = (2 * j + 1) * PI * T;
+                sum2 += w * o->Sigma_inf * T * 2 / (nk02 * (wn * wn + (e - mu0) * (e - mu0)));
+            }
+            if (fabs(e - mu0) > 1.0e-12)
+                sum2 += w * o->Sigma_inf * (1.0 / (exp((e - mu0) / T) + 1.0) - 0.5) / (e - mu0) / nk02;
+            else {
+                const double ek = (e - mu0) < 0 ? exp((e - mu0) / T) : exp(-(e - mu0) / T);
+                sum2 -= w * o->Sigma_inf * ek / ((ek + 1.0) * (ek + 1.0) * nk02 * T);
+            }
+        }
+    o->trSG = sum;
+    o->trSG_corr = sum + sum2;
+    return 0;
+}
+
+/* green.cpp:3108-3182 */
+double orc_traceDiffG(orc *o, double pmu)
+{
+    const int nk = o->nbk, nbw = o->nbw;
+    const double T = o->p.T, mu0 = o->mu0;
+    double part[NBLK];
+#pragma omp parallel for schedule(static)
+    for (int b = 0; b < NBLK; b++) {
+        double acc = 0;
+        for (int l = b; l < nk; l += NBLK)
+            for (int m = 0; m <= l; m++) {
+                const double e = o->ek[m + l * nk];
+                const double w = 2.0 * wedge_w8(l, m, nk);
+                const long i = widx(l, m);
+                for (int j = nbw; j >= nbw / 2; j--) {
+                    const double wn = (2 * j + 1) * PI * T;
+                    const double rS = -o->Sigma_inf2[i] / (wn * wn) + o->Sigma_inf4[i] / (wn * wn * wn * wn);
+                    const double iS = -o->Sigma_inf / wn + o->Sigma_inf3[i] / (wn * wn * wn);
+                    const double tmp = w / (((wn - iS) * (wn - iS) + (e - pmu + rS) * (e - pmu + rS)) *
+                                            (wn * wn + (e - mu0) * (e - mu0)));
+                    acc += tmp * ((mu0 - pmu + rS) * (e - pmu + rS) * (e - mu0) + iS * (iS - 2 * wn) * (e - mu0) -
+                                  wn * wn * (mu0 - pmu + rS));
+                }
+                for (int j = nbw / 2 - 1; j >= 0; j--) {
+                    const double wn = (2 * j + 1) * PI * T;
+                    const double complex s = sigma_at(o, i, j, wn);
+                    const double rS = creal(s), iS = cimag(s);
+                    const double tmp = w / (((wn - iS) * (wn - iS) + (e - pmu + rS) * (e - pmu + rS)) *
+                                            (wn * wn + (e - mu0) * (e - mu0)));
+                    acc += tmp * ((mu0 - pmu + rS) * (e - pmu + rS) * (e - mu0) + iS * (iS - 2 * wn) * (e - mu0) -
+                                  wn * wn * (mu0 - pmu + rS));
+                }
+            }
+        part[b] = acc;
+    }
+    double s = 0;
+    for (int b = 0; b < NBLK; b++) s += part[b];
+    return s;
+}
+static double fz_mu(void *ctx, double x) { return orc_traceDiffG((orc *)ctx, x); }
+
+/* green.cpp:3185-3282 */
+int orc_find_mu(orc *o)
+{
+    const double init[3] = {0.0, -10.0, 10.0}, lims[2] = {-20, 20};
+    double root = 0.0;
+    if (orc_find_zero(fz_mu, o, init, &root, lims, &o->nevals_mu)) {
+        o->mu = root;
+        o->mu_ok = 1;
+    } else
+        o->mu_ok = 0;
+    orc_trace_self_G2(o);
+    return o->mu_ok;
+}
+
+/* green.cpp:2929-2997 */
+int orc_trace_self_G2(orc *o)
+{
+    const int nk = o->nbk, nbw = o->nbw;
+    const double nk02 = 4.0 * (nk - 1) * (nk - 1), T = o->p.T, mu = o->mu, U = o->p.U;
+    double sum = 0;
+    for (int l = 0; l < nk; l++)
+        for (int m = 0; m <= l; m++) {
+            const double e = o->ek[m + l * nk];
+            const int w = wedge_w8(l, m, nk);
+            const long i = widx(l, m);
+            for (int j = 0; j < nbw / 2; j++) {
+                const double wn = (2 * j + 1) * PI * T;
+                const double complex s = sigma_at(o, i, j, wn);
+                const double rS = creal(s), iS = cimag(s);
+                const double tmp = w * T * 2 / (nk02 * ((wn - iS) * (wn - iS) + (e - mu + rS) * (e - mu + rS)));
+                sum += tmp * (iS * wn - iS * iS - (e - mu) * rS - rS * rS);
+            }
+        }
+    o->trSG2 = sum;
+    o->dblOcc2 = (sum + U * o->p.n * o->p.n / 4.0) / U;
+    return 0;
+}
+
+/* ---------------------------------------------------------------- Lambda_jj (a10) */
+/* per-k Dyson G on the Matsubara grid ordered n = j - nw, j = 0..nbw-1; fills
+   Gm[j] = G - G_asym and optionally J[j] = G*J - G_asym*Jinf (cond_opt.cpp:3354-3407) */
+typedef struct {
+    double complex r1, r2, a1, a2, b1, b2, den1, den2;
+} pole_pair;
+
+static void asym_poles(const orc *o, double ekv, double CJ1, pole_pair *pp)
+{
+    const double T = o->p.T;
+    double complex coef[3] = {1.0, -ekv + o->mu, -o->Sigma_inf};
+    double complex roots[2] = {0, 0};
+    orc_roots_2pol(coef, roots);
+    pp->r1 = roots[0];
+    pp->r2 = roots[1];
+    pp->a1 = pp->r1 / (pp->r1 - pp->r2);
+    pp->a2 = pp->r2 / (pp->r2 - pp->r1);
+    pp->b1 = CJ1 / (pp->r1 - pp->r2);
+    pp->b2 = -pp->b1;
+    pp->den1 = (creal(pp->r1) > 0 ? cexp(-pp->r1 / T) : cexp(pp->r1 / T)) + 1.0;
+    pp->den2 = (creal(pp->r2) > 0 ? cexp(-pp->r2 / T) : cexp(pp->r2 / T)) + 1.0;
+}
+
+/* cond_opt.cpp:2537-4290 with chi3 = false */
+int orc_chijj(orc *o, int N_theta)
+{
+    const int nbk = o->nbk, nbw = o->nbw, nw = nbw / 2, nk0 = 2 * (nbk - 1);
+    const long nq = (long)nbk * (nbk - 2), nk8 = (long)nbk * (nbk + 1) / 2;
+    const double T = o->p.T, mu0 = o->mu0, mu = o->mu, U = o->p.U;
+    const double normFact = 1.0 / ((double)nk0 * nk0);
+    const double *V = o->V;
+    if (N_theta < 1) N_theta = 1;
+    o->N_theta = N_theta;
+
+    double *dek = dalloc((size_t)nq), *d2ek = dalloc((size_t)nbk * nbk), *ekmu = dalloc((size_t)nbk * nbk);
+    for (int i = 0; i < nbk * nbk; i++) ekmu[i] = o->ek[i] - mu0;
+    for (int l = 0; l < nbk; l++)
+        for (int m = 0; m < nbk; m++) d2ek[m + l * nbk] = orc_hess_ek_xx(o, l * PI / (nbk - 1), m * PI / (nbk - 1));
+    for (int l = 0; l < nbk - 2; l++)
+        for (int m = 0; m < nbk; m++) dek[m + l * nbk] = orc_grad_ek_x(o, (l + 1) * PI / (nbk - 1), m * PI / (nbk - 1));
+
+    const size_t hsz = (size_t)nq * (nbw + 1);
+    double *h = dalloc(hsz), *f0 = dalloc(hsz);
+    /* h(r,tau) (cond_opt.cpp:2697-2735) and f0(r,tau) (:2795-2827) */
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j <= nbw; j++) {
+        const double t = (double)j / (T * nbw);
+        double *fh = h + (size_t)j * nq, *ff = f0 + (size_t)j * nq;
+        for (int l = 0; l < nbk - 2; l++)
+            for (int m = 0; m < nbk; m++) {
+                const double x = ekmu[m + (l + 1) * nbk], v = dek[m + l * nbk];
+                const double e = o->ek[m + (l + 1) * nbk];
+                if (x > 0)
+                    fh[m + l * nbk] = normFact * v * exp(-x * t) / (exp(-x / T) + 1.0);
+                else
+                    fh[m + l * nbk] = normFact * v * exp(x * (1.0 / T - t)) / (exp(x / T) + 1.0);
+                if (e > mu0) {
+                    const double ex = exp((mu0 - e) / T);
+                    ff[m + l * nbk] = normFact * v * exp((mu0 - e) * t) * (t / (ex + 1.0) - ex / (T * (ex + 1.0) * (ex + 1.0)));
+                } else {
+                    const double ex = exp((e - mu0) / T);
+                    ff[m + l * nbk] = normFact * v * exp((e - mu0) * (1.0 / T - t)) * (t / (ex + 1.0) - 1.0 / (T * (ex + 1.0) * (ex + 1.0)));
+                }
+            }
+        orc_r2r_2d(fh, nbk - 2, nbk, 1, 0);
+        orc_r2r_2d(ff, nbk - 2, nbk, 1, 0);
+    }
+    /* boundary derivatives dh, df0 at tau = 0, beta (cond_opt.cpp:2745-2887) */
+    double *dh = dalloc(2 * (size_t)nq), *df0 = dalloc(2 * (size_t)nq);
+    for (int l = 0; l < nbk - 2; l++)
+        for (int m = 0; m < nbk; m++) {
+            const long q = m + l * nbk;
+            const double e = o->ek[m + (l + 1) * nbk], v = dek[q];
+            if (e > mu0) {
+                const double ex = exp((mu0 - e) / T);
+                dh[q] = -normFact * v * (e - mu0) / (ex + 1.0);
+                dh[q + nq] = -normFact * v * (e - mu0) * ex / (ex + 1.0);
+                df0[q] = normFact * v * ((e - mu0) * ex / (T * (ex + 1.0) * (ex + 1.0)) + 1.0 / (ex + 1.0));
+                df0[q + nq] = normFact * v * ex * ((mu0 - e) * (1.0 / (T * (ex + 1.0)) - ex / (T * (ex + 1.0) * (ex + 1.0))) + 1.0 / (ex + 1.0));
+            } else {
+                const double ex = exp((e - mu0) / T);
+                dh[q] = -normFact * v * (e - mu0) * ex / (ex + 1.0);
+                dh[q + nq] = -normFact * v * (e - mu0) / (ex + 1.0);
+                df0[q] = normFact * v * ex * ((e - mu0) / (T * (ex + 1.0) * (ex + 1.0)) + 1.0 / (ex + 1.0));
+                df0[q + nq] = normFact * v * ((mu0 - e) * (1.0 / (T * (ex + 1.0)) - 1.0 / (T * (ex + 1.0) * (ex + 1.0))) + 1.0 / (ex + 1.0));
+            }
+        }
+    orc_r2r_2d(dh, nbk - 2, nbk, 1, 0);
+    orc_r2r_2d(dh + nq, nbk - 2, nbk, 1, 0);
+    orc_r2r_2d(df0, nbk - 2, nbk, 1, 0);
+    orc_r2r_2d(df0 + nq, nbk - 2, nbk, 1, 0);
+
+    /* dhV, df0V (cond_opt.cpp:2892-3014); V_array is 2x vertex_rt_array */
+    const double fact4 = U * (3.0 * o->Usp + o->Uch) / 4;
+    double *dhV = dalloc(2 * (size_t)nq), *df0V = dalloc(2 * (size_t)nq);
+    for (int l = 0; l < nbk - 2; l++)
+        for (int m = 0; m < nbk; m++) {
+            const long q = m + l * nbk, r = fold_idx(l + 1, m, nk0);
+            dhV[q] = 2.0 * dh[q] * V[r] + h[q] * fact4 * o->dtau_chirt0[r];
+            dhV[q + nq] = 2.0 * dh[q + nq] * V[r] - h[q + (size_t)nbw * nq] * fact4 * o->dtau_chirt0[r];
+            df0V[q] = 2.0 * df0[q] * V[r] + f0[q] * fact4 * o->dtau_chirt0[r];
+            df0V[q + nq] = 2.0 * df0[q + nq] * V[r] - f0[q + (size_t)nbw * nq] * fact4 * o->dtau_chirt0[r];
+        }
+    orc_r2r_2d(dhV, nbk - 2, nbk, 1, 0);
+    orc_r2r_2d(dhV + nq, nbk - 2, nbk, 1, 0);
+    orc_r2r_2d(df0V, nbk - 2, nbk, 1, 0);
+    orc_r2r_2d(df0V + nq, nbk - 2, nbk, 1, 0);
+    free(dh);
+    free(df0);
+
+    /* hV(k,tau), f0V(k,tau) overwrite h, f0 (cond_opt.cpp:3029-3114) */
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j <= nbw; j++) {
+        const int jv = (j > nbw / 2) ? nbw - j : j;
+        double *fh = h + (size_t)j * nq, *ff = f0 + (size_t)j * nq;
+        for (int l = 0; l < nbk - 2; l++)
+            for (int m = 0; m < nbk; m++) {
+                const long q = m + l * nbk, r = fold_idx(l + 1, m, nk0) + (long)jv * nk8;
+                fh[q] = 2.0 * fh[q] * V[r];
+                ff[q] = 2.0 * ff[q] * V[r];
+            }
+        orc_r2r_2d(fh, nbk - 2, nbk, 1, 0);
+        orc_r2r_2d(ff, nbk - 2, nbk, 1, 0);
+    }
+
+    /* per-k pencils (cond_opt.cpp:3179-3871), deterministic block partial sums */
+    const int NS0 = nbw / 2 + 1;
+    const int nwb = nw + 1;
+    const double Dtheta = PI / (2.0 * N_theta);
+    double *p1 = dcalloc((size_t)NBLK * nwb), *p2 = dcalloc((size_t)NBLK * nwb);
+    double *p1k = dcalloc((size_t)NBLK * nwb * N_theta), *p2k = dcalloc((size_t)NBLK * nwb * N_theta);
+    double pk0[NBLK], pinf[NBLK][3];
+#pragma omp parallel
+    {
+        double complex *Jt = (double complex *)xmalloc(sizeof(double complex) * (size_t)nbw);
+        double complex *J0 = (double complex *)xmalloc(sizeof(double complex) * (size_t)nbw);
+        double complex *G = (double complex *)xmalloc(sizeof(double complex) * (size_t)nbw);
+        double complex *GG = (double complex *)xmalloc(sizeof(double complex) * (size_t)nbw);
+        double *tau2 = dalloc((size_t)nbw + 1), *v2 = dalloc((size_t)nbw + 1), *cf2 = dalloc(4 * (size_t)nbw);
+        double *tau = dalloc(NS0), *v1 = dalloc(NS0), *cf = dalloc(4 * (size_t)(NS0 - 1));
+        sws ws2 = sws_new(nbw + 1), ws1 = sws_new(NS0);
+#pragma omp for schedule(static)
+        for (int b = 0; b < NBLK; b++) {
+            double *c1 = p1 + (size_t)b * nwb, *c2 = p2 + (size_t)b * nwb;
+            double *c1k = p1k + (size_t)b * nwb * N_theta, *c2k = p2k + (size_t)b * nwb * N_theta;
+            double k0acc = 0, inf1 = 0, inf2 = 0, inf4 = 0;
+            const long qa = nq * b / NBLK, qb = nq * (b + 1) / NBLK;
+            for (long qq = qa; qq < qb; qq++) {
+                const int l = (int)(qq / nbk), m = (int)(qq % nbk);
+                const long q = qq, ki = fold_idx(l + 1, m, nk0);
+                const double C2 = o->Sigma_inf2[ki], C3 = o->Sigma_inf3[ki];
+                const double ekv = o->ek[m + (l + 1) * nbk];
+                const double v = dek[q];
+                double CJ[2][3];
+                /* splines + twiddled DFTs of hV and f0V -> J(ikn), J0(ikn) (:3276-3347) */
+                for (int which = 0; which < 2; which++) {
+                    const double *src = which == 0 ? h : f0;
+                    const double *dsrc = which == 0 ? dhV : df0V;
+                    double complex *dst = which == 0 ? Jt : J0;
+                    for (int j = 0; j <= nbw; j++) {
+                        tau2[j] = j / (nbw * T);
+                        v2[j] = src[q + (size_t)j * nq];
+                    }
+                    cf2[0] = dsrc[q];
+                    cf2[1] = dsrc[q + nq];
+                    orc_spline_coeffs_rel_ws(tau2, v2, nbw + 1, cf2, ws2.B, ws2.r);
+                    for (int j = 0; j < nbw; j++) {
+                        const double ang = j * PI / nbw;
+                        G[j] = (cos(ang) + I * sin(ang)) * (6.0 * cf2[4 * j]);
+                    }
+                    orc_dft(G, nbw, +1);
+                    const double cj1 = -src[q] - src[q + (size_t)nbw * nq];
+                    const double cj2 = dsrc[q] + dsrc[q + nq];
+                    const double d2J0 = 2 * cf2[1];
+                    const double d2Jb = 6.0 * cf2[4 * nbw - 4] / (nbw * T) + 2.0 * cf2[4 * nbw - 3];
+                    const double cj3 = -d2J0 - d2Jb;
+                    CJ[which][0] = cj1;
+                    CJ[which][1] = cj2;
+                    CJ[which][2] = cj3;
+                    for (int j = 0; j < nbw / 2; j++) {
+                        const double kn = (2 * j + 1) * PI * T;
+                        const double ang = (2 * j + 1) * PI / nbw;
+                        dst[j + nbw / 2] = -I * cj1 / kn - cj2 / (kn * kn) + I * cj3 / (kn * kn * kn) +
+                                           (1.0 - (cos(ang) + I * sin(ang))) * G[j] / (kn * kn * kn * kn);
+                    }
+                    for (int j = nbw / 2; j < nbw; j++) {
+                        const double kn = (2 * (j - nbw) + 1) * PI * T;
+                        const double ang = (2 * (j - nbw) + 1) * PI / nbw;
+                        dst[j - nbw / 2] = -I * cj1 / kn - cj2 / (kn * kn) + I * cj3 / (kn * kn * kn) +
+                                           (1.0 - (cos(ang) + I * sin(ang))) * G[j] / (kn * kn * kn * kn);
+                    }
+                }
+                const double CJ1 = CJ[0][0];
+                /* Dyson G, asymptote subtraction (:3349-3407) */
+                double complex k0 = 0, qn0 = 0;
+                double dGp = 0, dJ0 = 0;
+                for (int j = 0; j < nw; j++) {
+                    const double kn = (2.0 * (j - nw) + 1) * PI * T;
+                    const double complex R = o->Self[ki + (size_t)(nw - j - 1) * nk8];
+                    const double complex sg = (-C2 / (kn * kn) + creal(R) / (kn * kn * kn * kn)) +
+                                              I * (-o->Sigma_inf / kn + C3 / (kn * kn * kn) - cimag(R) / (kn * kn * kn * kn));
+                    const double complex Gv = T / ((I * kn - ekv) + mu - sg);
+                    Jt[j] = Gv * Jt[j];
+                    qn0 += Gv * Gv * J0[j] / T;
+                    const double complex G2 = T / ((-ekv + mu) + I * (kn + o->Sigma_inf / kn));
+                    G[j] = Gv - G2;
+                    Jt[j] = Jt[j] - G2 * (-I * CJ1 / kn);
+                    k0 += G[j];
+                    dGp += kn * cimag(G[j]);
+                    dJ0 -= kn * cimag(Jt[j]);
+                }
+                for (int j = nbw - 1; j >= nw; j--) {
+                    const double kn = (2.0 * (j - nw) + 1) * PI * T;
+                    const double complex R = o->Self[ki + (size_t)(j - nw) * nk8];
+                    const double complex sg = (-C2 / (kn * kn) + creal(R) / (kn * kn * kn * kn)) +
+                                              I * (-o->Sigma_inf / kn + C3 / (kn * kn * kn) + cimag(R) / (kn * kn * kn * kn));
+                    const double complex Gv = T / ((I * kn - ekv) + mu - sg);
+                    Jt[j] = Gv * Jt[j];
+                    qn0 += Gv * Gv * J0[j] / T;
+                    const double complex G2 = T / ((-ekv + mu) + I * (kn + o->Sigma_inf / kn));
+                    G[j] = Gv - G2;
+                    Jt[j] = Jt[j] - G2 * (-I * CJ1 / kn);
+                    k0 += G[j];
+                    dGp += kn * cimag(G[j]);
+                    dJ0 -= kn * cimag(Jt[j]);
+                }
+                /* 2-pole asymptotic G and J in tau (:3410-3467) */
+                pole_pair pp;
+                asym_poles(o, ekv, CJ1, &pp);
+                double complex f1p, f1m, fj1, fj1b, f2p, f2m, fj2, fj2b;
+                if (creal(pp.r1) > 0) {
+                    const double complex e = cexp(-pp.r1 / T);
+                    f1p = pp.a1 / pp.den1;
+                    f1m = pp.a1 * e / pp.den1;
+                    fj1 = pp.r1 * pp.b1 * e / pp.den1;
+                    fj1b = pp.r1 * pp.b1 / pp.den1;
+                } else {
+                    const double complex e = cexp(pp.r1 / T);
+                    f1p = pp.a1 * e / pp.den1;
+                    f1m = pp.a1 / pp.den1;
+                    fj1 = pp.r1 * pp.b1 / pp.den1;
+                    fj1b = pp.r1 * pp.b1 * e / pp.den1;
+                }
+                if (creal(pp.r2) > 0) {
+                    const double complex e = cexp(-pp.r2 / T);
+                    f2p = pp.a2 / pp.den2;
+                    f2m = pp.a2 * e / pp.den2;
+                    fj2 = pp.r2 * pp.b2 * e / pp.den2;
+                    fj2b = pp.r2 * pp.b2 / pp.den2;
+                } else {
+                    const double complex e = cexp(pp.r2 / T);
+                    f2p = pp.a2 * e / pp.den2;
+                    f2m = pp.a2 / pp.den2;
+                    fj2 = pp.r2 * pp.b2 / pp.den2;
+                    fj2b = pp.r2 * pp.b2 * e / pp.den2;
+                }
+                k0 += creal(f1m + f2m);
+                double dGm = -dGp;
+                dGp += creal(pp.r1 * f1p + pp.r2 * f2p);
+                dGm += creal(pp.r1 * f1m + pp.r2 * f2m);
+                double dJb = -dJ0;
+                dJ0 += creal(fj1 + fj2);
+                dJb += creal(fj1b + fj2b);
+                /* to tau (:3574-3697) */
+                orc_dft(G, nbw, -1);
+                double complex Gb = G[0];
+                orc_dft(Jt, nbw, +1);
+                double complex Jb = Jt[0];
+                for (int j = 0; j < nbw; j++) {
+                    const double t = j / (nbw * T);
+                    double complex e1, ej1, e2, ej2;
+                    if (creal(pp.r1) > 0) {
+                        e1 = cexp(-t * pp.r1);
+                        ej1 = cexp(pp.r1 * (t - 1.0 / T));
+                    } else {
+                        e1 = cexp((1.0 / T - t) * pp.r1);
+                        ej1 = cexp(pp.r1 * t);
+                    }
+                    if (creal(pp.r2) > 0) {
+                        e2 = cexp(-t * pp.r2);
+                        ej2 = cexp(pp.r2 * (t - 1.0 / T));
+                    } else {
+                        e2 = cexp((1.0 / T - t) * pp.r2);
+                        ej2 = cexp(pp.r2 * t);
+                    }
+                    const double G0t = creal(-pp.a1 * e1 / pp.den1 - pp.a2 * e2 / pp.den2);
+                    const double Jinf = creal(pp.b1 * ej1 / pp.den1 + pp.b2 * ej2 / pp.den2);
+                    const double arg = (j * PI * (nbw - 1)) / nbw;
+                    const double complex ph = cos(arg) + I * sin(arg);
+                    G[j] = ph * G[j] + G0t;
+                    Jt[j] = Jt[j] / ph + Jinf;
+                }
+                {
+                    const double t = 1.0 / T;
+                    double complex e1, ej1, e2, ej2;
+                    if (creal(pp.r1) > 0) {
+                        e1 = cexp(-t * pp.r1);
+                        ej1 = 1.0;
+                    } else {
+                        e1 = 1.0;
+                        ej1 = cexp(pp.r1 * t);
+                    }
+                    if (creal(pp.r2) > 0) {
+                        e2 = cexp(-t * pp.r2);
+                        ej2 = 1.0;
+                    } else {
+                        e2 = 1.0;
+                        ej2 = cexp(pp.r2 * t);
+                    }
+                    const double G0t = creal(-pp.a1 * e1 / pp.den1 - pp.a2 * e2 / pp.den2);
+                    const double Jinf = creal(pp.b1 * ej1 / pp.den1 + pp.b2 * ej2 / pp.den2);
+                    const double arg = PI * (nbw - 1);
+                    const double complex ph = cos(arg) + I * sin(arg);
+                    Gb = ph * Gb + G0t;
+                    Jb = Jb / ph + Jinf;
+                }
+                /* G(tau) J(-tau): spline + DFT -> MT term (:3699-3732) */
+                for (int j = 0; j < nbw; j++) {
+                    tau2[j] = j / (nbw * T);
+                    v2[j] = creal(G[j] * Jt[j]);
+                }
+                tau2[nbw] = 1.0 / T;
+                v2[nbw] = creal((-G[0] - 1.0) * Jb);
+                cf2[0] = creal(dGp * Jt[0] + G[0] * dJ0);
+                cf2[1] = creal(dGm * Jb - (G[0] + 1.0) * dJb);
+                const double MJ1 = v2[0] - v2[nbw];
+                orc_spline_coeffs_rel_ws(tau2, v2, nbw + 1, cf2, ws2.B, ws2.r);
+                const double d2J0 = 2 * cf2[1];
+                const double d2Jb = 6.0 * cf2[4 * nbw - 4] / (nbw * T) + 2.0 * cf2[4 * nbw - 3];
+                const double MJ3 = d2J0 - d2Jb;
+                for (int j = 0; j < nbw; j++) J0[j] = 6.0 * cf2[4 * j];
+                orc_dft(J0, nbw, -1);
+                /* -G(tau)G(-tau): spline + DFT -> bubble (:3734-3778) */
+                GG[0] = G[0] * (G[0] + 1.0);
+                for (int j = 1; j < nbw; j++) GG[j] = -G[j] * G[nbw - j];
+                for (int j = 0; j < NS0; j++) {
+                    tau[j] = j / (nbw * T);
+                    v1[j] = creal(GG[j]);
+                }
+                const double dchi = dGp * (creal(G[0]) + 1.0) + creal(G[0]) * dGm;
+                cf[0] = dchi;
+                cf[1] = 0;
+                orc_spline_coeffs_rel_ws(tau, v1, NS0, cf, ws1.B, ws1.r);
+                for (int j = 0; j < nbw / 2; j++) G[j] = 6.0 * cf[4 * j];
+                for (int j = nbw / 2; j < nbw; j++) G[j] = -G[nbw - j - 1];
+                orc_dft(G, nbw, -1);
+                double integ = 0;
+                for (int j = 0; j < nbw / 2; j++) {
+                    const double a = cf[4 * j], bb = cf[4 * j + 1], c = cf[4 * j + 2], d = cf[4 * j + 3];
+                    const double x1 = tau[j + 1] - tau[j];
+                    integ += a * x1 * x1 * x1 * x1 / 4.0 + bb * x1 * x1 * x1 / 3.0 + c * x1 * x1 / 2.0 + d * x1;
+                }
+                /* accumulate (:3780-3822) */
+                double wt = 4;
+                if (m == 0 || m == nbk - 1) wt = wt / 2;
+                k0acc -= 2.0 * wt * normFact * d2ek[m + (l + 1) * nbk] * creal(k0);
+                const double bub0 = 2 * integ;
+                c1[0] -= 2.0 * wt * normFact * v * v * bub0;
+                inf1 += 4.0 * dchi * normFact * wt * v * v;
+                inf2 += normFact * wt * v * 2.0 * MJ1;
+                inf4 -= normFact * wt * v * 2.0 * MJ3;
+                c2[0] -= normFact * wt * v * creal(qn0);
+                const double th = atan((1.0 * (nk0 / 2 - m)) / (nk0 / 2 - l - 1));
+                int it = (int)floor(th / Dtheta);
+                if (it >= N_theta) it = N_theta - 1;
+                if (it < 0) it = 0;
+                c1k[it] -= 2.0 * wt * normFact * v * v * bub0;
+                c2k[it] -= normFact * wt * v * creal(qn0);
+                for (int j = 1; j <= nw; j++) {
+                    const double wn = 2 * j * PI * T;
+                    const double ang = -2 * j * PI / nbw;
+                    const double complex e = 1.0 - (cos(ang) + I * sin(ang));
+                    const double imJ = -2.0 * MJ1 / wn + 2.0 * MJ3 / (wn * wn * wn) + 2.0 * cimag(e * J0[j] / (wn * wn * wn * wn));
+                    c2[j] -= normFact * wt * v * imJ / wn;
+                    c2k[it + (size_t)j * N_theta] -= normFact * wt * v * imJ / wn;
+                    const double bub = -2.0 * dchi / (wn * wn) + creal(e * G[j] / (wn * wn * wn * wn));
+                    c1[j] -= 2.0 * normFact * wt * v * v * bub;
+                    c1k[it + (size_t)j * N_theta] -= 2.0 * normFact * wt * v * v * bub;
+                }
+            }
+            pk0[b] = k0acc;
+            pinf[b][0] = inf1;
+            pinf[b][1] = inf2;
+            pinf[b][2] = inf4;
+        }
+        free(Jt);
+        free(J0);
+        free(G);
+        free(GG);
+        free(tau2);
+        free(v2);
+        free(cf2);
+        free(tau);
+        free(v1);
+        free(cf);
+        sws_free(&ws2);
+        sws_free(&ws1);
+    }
+    free(o->chijj1);
+    free(o->chijj2);
+    free(o->chijj1_k);
+    free(o->chijj2_k);
+    o->chijj1 = dcalloc(nwb);
+    o->chijj2 = dcalloc(nwb);
+    o->chijj1_k = dcalloc((size_t)nwb * N_theta);
+    o->chijj2_k = dcalloc((size_t)nwb * N_theta);
+    double k0sum = 0;
+    o->chijj1_inf2 = o->chijj2_inf2 = o->chijj2_inf4 = 0;
+    for (int b = 0; b < NBLK; b++) {
+        for (int j = 0; j < nwb; j++) {
+            o->chijj1[j] += p1[(size_t)b * nwb + j];
+            o->chijj2[j] += p2[(size_t)b * nwb + j];
+        }
+        for (size_t j = 0; j < (size_t)nwb * N_theta; j++) {
+            o->chijj1_k[j] += p1k[(size_t)b * nwb * N_theta + j];
+            o->chijj2_k[j] += p2k[(size_t)b * nwb * N_theta + j];
+        }
+        k0sum += pk0[b];
+        o->chijj1_inf2 += pinf[b][0];
+        o->chijj2_inf2 += pinf[b][1];
+        o->chijj2_inf4 += pinf[b][2];
+    }
+    free(p1);
+    free(p2);
+    free(p1k);
+    free(p2k);
+
+    /* kx = 0 and kx = pi rows of the diamagnetic term (cond_opt.cpp:3993-4173) */
+    for (int l = 0; l < nbk; l += nk0 / 2) {
+        for (int m = 0; m < nbk; m++) {
+            const long ki = fold_idx(l, m, nk0);
+            const double C2 = o->Sigma_inf2[ki], C3 = o->Sigma_inf3[ki];
+            const double ekv = o->ek[m + l * nbk];
+            double complex k0 = 0;
+            for (int j = 0; j < nw; j++) {
+                const double kn = (2.0 * (j - nw) + 1) * PI * T;
+                const double complex R = o->Self[ki + (size_t)(nw - j - 1) * nk8];
+                const double complex sg = (-C2 / (kn * kn) + creal(R) / (kn * kn * kn * kn)) +
+                                          I * (-o->Sigma_inf / kn + C3 / (kn * kn * kn) - cimag(R) / (kn * kn * kn * kn));
+                const double complex Gv = T / ((I * kn - ekv) + mu - sg);
+                const double complex G0v = T / ((-ekv + mu) + I * (kn + o->Sigma_inf / kn));
+                k0 += Gv - G0v;
+            }
+            for (int j = nbw - 1; j >= nw; j--) {
+                const double kn = (2.0 * (j - nw) + 1) * PI * T;
+                const double complex R = o->Self[ki + (size_t)(j - nw) * nk8];
+                const double complex sg = (-C2 / (kn * kn) + creal(R) / (kn * kn * kn * kn)) +
+                                          I * (-o->Sigma_inf / kn + C3 / (kn * kn * kn) + cimag(R) / (kn * kn * kn * kn));
+                const double complex Gv = T / ((I * kn - ekv) + mu - sg);
+                const double complex G0v = T / ((-ekv + mu) + I * (kn + o->Sigma_inf / kn));
+                k0 += Gv - G0v;
+            }
+            pole_pair pp;
+            asym_poles(o, ekv, 0.0, &pp);
+            const double complex f1m = creal(pp.r1) > 0 ? pp.a1 * cexp(-pp.r1 / T) / pp.den1 : pp.a1 / pp.den1;
+            const double complex f2m = creal(pp.r2) > 0 ? pp.a2 * cexp(-pp.r2 / T) / pp.den2 : pp.a2 / pp.den2;
+            k0 += creal(f1m + f2m);
+            double wt = 2;
+            if (m == 0 || m == nbk - 1) wt = wt / 2;
+            k0sum -= 2.0 * wt * normFact * d2ek[m + l * nbk] * creal(k0);
+        }
+    }
+    o->kx0 = k0sum;
+    free(h);
+    free(f0);
+    free(dhV);
+    free(df0V);
+    free(dek);
+    free(d2ek);
+    free(ekmu);
+    return 0;
+}
+
+/* ---------------------------------------------------------------- full solve */
+/* tpsc_main.cpp:61-75 (green ctor, calc_Self_FFT_spline, traceSelfG, find_mu) then the
+   Lambda_jj call of tpsc_main.cpp:243 (chi3 = false, N_theta_k = 1) */
+int orc_solve(orc *o)
+{
+    int ok = orc_find_Usp_Uch(o);
+    orc_vertex(o);
+    orc_self_energy(o);
+    orc_trace_self_G(o);
+    ok &= orc_find_mu(o);
+    orc_chijj(o, o->N_theta > 0 ? o->N_theta : 1);
+    return ok;
+}
+
+void orc_self_kw_brute_out(orc *o, int kx, int ky, int n, double out[2])
+{
+    const double complex s = orc_self_kw_brute(o, kx, ky, n);
+    out[0] = creal(s);
+    out[1] = cimag(s);
+}
