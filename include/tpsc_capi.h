@@ -1,0 +1,148 @@
+/*
+ * tpsc_capi.h -- C ABI of the MI355X-native TPSC hot path (libtpsc_hip.so).
+ *
+ * The reference (dbergeron1/TPSC_conductivity) has no plugin API: its boundary is the
+ * public method surface of the class chain cond_opt > green > chi > chi0 > green0 >
+ * hamiltonien.  Each entry point below replaces one of those methods, keeps its order
+ * of use and its semantics, and is what an FFI (ctypes, cgo, JNI) would bind.  The
+ * reference's own callers are tpsc_main.cpp:61-114 and the lazy chain at
+ * cond_opt.cpp:2550-2560.
+ *
+ * Conventions: every call returns 0 on success or a negative TPSC_E_* code, with the
+ * message in tpsc_last_error(ctx).  A context owns its device buffers and one HIP
+ * stream; calls are synchronous for the scalars they return; a context is used from one
+ * host thread at a time.  Arrays copied out by tpsc_get keep the reference layouts:
+ * packed wedge idx(l,m) = m + l(l+1)/2 (m <= l), frequency/tau-major a[idx + j*nk8].
+ */
+#ifndef TPSC_CAPI_H
+#define TPSC_CAPI_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TPSC_OK 0
+#define TPSC_E_INVALID -1     /* bad argument or unsupported size          */
+#define TPSC_E_DEVICE -2      /* HIP runtime error                         */
+#define TPSC_E_NOMEM -3       /* device allocation failed                  */
+#define TPSC_E_ORDER -4       /* stage called before its inputs exist      */
+#define TPSC_E_NOROOT -5      /* a root finder did not converge            */
+
+/* Model and grid, as the constructors take them:
+ * cond_opt(int nk0, int Nw0, double n, double T, double eta, double params[4]={t',t'',t3,U})
+ * (cond_opt.h:25, cond_opt.cpp:10; green.h:26; chi.h:25; hamiltonien.cpp:14-34). */
+typedef struct tpsc_params {
+    int nk0;         /* N, lattice N x N; power of two, 4..256                  */
+    int Nw0;         /* total number of fermionic +-iwn; power of two, 16..4096 */
+    double n;        /* density                                                */
+    double T;        /* temperature                                            */
+    double eta;      /* broadening, kept for interface parity                  */
+    double tp, tpp, t3, U;
+    int coarse_pass; /* 1: find_Usp_Uch() runs the (N/2, Nw/2) pass first, as the
+                        reference does (chi.cpp:285-310); 0: full grid only      */
+    int precision;   /* 0: FP64 (only value accepted by this build)             */
+} tpsc_params;
+
+typedef struct tpsc_ctx tpsc_ctx;
+
+/* Fields for tpsc_get (array copy-out) and tpsc_get_scalar. */
+enum tpsc_field {
+    TPSC_F_EK = 1,        /* eps(k) on the nk x nk quadrant            (hamiltonien.cpp:49-68) */
+    TPSC_F_CHI0 = 2,      /* chi0(q,iqn), nwb*nk8                       (chi0.cpp:669-690)      */
+    TPSC_F_CHIRT = 3,     /* chi0(r,tau), nwb*nk8                       (chi0.cpp:504)          */
+    TPSC_F_CHI0_INF = 4,  /* 1/qn^2 tail coefficient, nk8                (chi0.cpp:673)          */
+    TPSC_F_CHI0_INF2 = 5, /* 1/qn^4 tail coefficient, nk8                (chi0.cpp:682)          */
+    TPSC_F_DTAU_CHIRT0 = 6,
+    TPSC_F_V = 7,         /* V(r,tau), nwb*nk8                          (chi.cpp:166)           */
+    TPSC_F_SIGMA_TAU = 8, /* Sigma(k,tau), (Nw+1)*nk8                   (green.cpp:1872-1885)   */
+    TPSC_F_SELF = 9,      /* R(k,n) complex, (Nw/2)*nk8 interleaved re/im (green.cpp:2006)      */
+    TPSC_F_SIGMA_INF2 = 10,
+    TPSC_F_SIGMA_INF3 = 11,
+    TPSC_F_SIGMA_INF4 = 12,
+    TPSC_F_CHIJJ1 = 13,   /* bubble Lambda_jj1(iqn), nwb                (cond_opt.cpp:3784-3817) */
+    TPSC_F_CHIJJ2 = 14,   /* Maki-Thompson Lambda_jj2(iqn), nwb         (cond_opt.cpp:3790-3807) */
+    TPSC_F_DTAU_SIGMA = 15, /* 2*nk8                                    (green.cpp:1743-1817)   */
+    TPSC_F_CHI0_TAU = 16  /* chi0(q,tau) before the tau->iqn transform (kept only after tpsc_chi0) */
+};
+enum tpsc_scalar {
+    TPSC_S_MU0 = 1, TPSC_S_CHI0MAX, TPSC_S_QMAX_L, TPSC_S_QMAX_M, TPSC_S_USP, TPSC_S_UCH,
+    TPSC_S_DBLOCC, TPSC_S_DIFREL_SP, TPSC_S_DIFREL_CH, TPSC_S_SIGMA_INF, TPSC_S_TRSG,
+    TPSC_S_TRSG_CORR, TPSC_S_MU, TPSC_S_TRSG2, TPSC_S_DBLOCC2, TPSC_S_KX0, TPSC_S_CHIJJ1_INF2,
+    TPSC_S_CHIJJ2_INF2, TPSC_S_CHIJJ2_INF4, TPSC_S_NEVAL_SP, TPSC_S_NEVAL_CH, TPSC_S_NEVAL_MU,
+    TPSC_S_NK, TPSC_S_NK8, TPSC_S_NWB, TPSC_S_NQ
+};
+
+/* constructor / destructor (cond_opt.cpp:10-62 and the base-class constructors). Builds
+ * eps(k) and mu0 (green0.cpp:18-23). */
+int tpsc_create(const tpsc_params* p, tpsc_ctx** out);
+void tpsc_destroy(tpsc_ctx* ctx);
+const char* tpsc_last_error(const tpsc_ctx* ctx);
+
+/* green0::find_mu, green0.cpp:240-262 (finite-lattice mu0, root of ddensity_mu_PBC) */
+int tpsc_find_mu0(tpsc_ctx* ctx, double* mu0);
+
+/* chi0::calc_chiqw_FFT_spline(int nq0, int nw0), chi0.cpp:279-718; also set_grid
+ * (green0.h:44) to (nq0, nw0) */
+int tpsc_chi0(tpsc_ctx* ctx, int nq0, int nw0);
+
+/* chi::sum_rule_sp / sum_rule_ch at one candidate vertex, chi.cpp:833-843
+ * (channel 0: spin, 1: charge; the charge rule uses the current Usp) */
+int tpsc_sum_rule(tpsc_ctx* ctx, int channel, double Ux, double* value);
+
+/* chi::find_Usp_Uch(int nq0, int nw0, double Usc[2]), chi.cpp:846-876 */
+int tpsc_find_Usp_Uch_grid(tpsc_ctx* ctx, int nq0, int nw0, double* Usp, double* Uch);
+
+/* chi::find_Usp_Uch(), chi.cpp:210-417 (coarse pass when params.coarse_pass, then full) */
+int tpsc_find_Usp_Uch(tpsc_ctx* ctx, double* Usp, double* Uch);
+
+/* chi::calc_chirt, chi.cpp:37-206 (V(r,tau) for tau in [0, beta/2]) */
+int tpsc_vertex_rt(tpsc_ctx* ctx);
+
+/* green::calc_Self_FFT_spline, green.cpp:1673-2031, without its leading find_Usp_Uch()
+ * and calc_chirt() calls (use tpsc_calc_self_energy_full for the whole method) */
+int tpsc_self_energy(tpsc_ctx* ctx);
+int tpsc_calc_self_energy_full(tpsc_ctx* ctx);
+
+/* green::traceSelfG, green.cpp:3001-3104: Tr(Sigma G0) and its Sigma_inf-corrected value */
+int tpsc_trace_self_G(tpsc_ctx* ctx, double* trace, double* trace_corr);
+
+/* green::traceDiffG(mu), green.cpp:3108-3182 */
+int tpsc_trace_diff_G(tpsc_ctx* ctx, double mu, double* value);
+
+/* green::find_mu, green.cpp:3185-3282 (then traceSelfG2, green.cpp:2929-2997) */
+int tpsc_find_mu(tpsc_ctx* ctx, double* mu);
+
+/* cond_opt::calc_chijj_vertex_corr_all_optim(chi3, mmin, N_theta_k, k_chijj, nk_chijj),
+ * cond_opt.cpp:2537-4290.  chi3 must be 0 (Aslamazov-Larkin not in this build) and
+ * N_theta_k 1.  Runs the missing upstream stages first, as cond_opt.cpp:2550-2560 does. */
+int tpsc_chijj(tpsc_ctx* ctx, int chi3, int mmin, int N_theta_k);
+
+/* one full solve: constructor state -> calc_Self_FFT_spline -> traceSelfG -> find_mu ->
+ * calc_chijj_vertex_corr_all_optim (tpsc_main.cpp:61-75 and :243) */
+int tpsc_solve(tpsc_ctx* ctx);
+
+/* copy-out in the reference layouts; bytes must be >= the field size */
+int tpsc_get(tpsc_ctx* ctx, int field, void* dst, size_t bytes);
+size_t tpsc_field_size(tpsc_ctx* ctx, int field);
+int tpsc_get_scalar(tpsc_ctx* ctx, int which, double* value);
+
+/* Per-kernel timing of the last calls when enabled: names are the kernel families
+ * ("chi0_slab", "sigma_pencil", ...).  tpsc_prof_get fills up to cap entries. */
+int tpsc_prof_enable(tpsc_ctx* ctx, int on);
+int tpsc_prof_reset(tpsc_ctx* ctx);
+int tpsc_prof_count(tpsc_ctx* ctx);
+int tpsc_prof_get(tpsc_ctx* ctx, int i, const char** name, double* total_ms, int* calls);
+
+/* keep a device copy of chi0(q,tau) at the next tpsc_chi0 call (test hook; allocates
+ * nwb*nk8 doubles once) */
+int tpsc_keep_chi_tau(tpsc_ctx* ctx, int on);
+
+/* device properties useful to callers (HBM bytes of the context's buffers) */
+int tpsc_device_bytes(tpsc_ctx* ctx, double* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

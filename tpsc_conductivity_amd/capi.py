@@ -1,0 +1,157 @@
+"""ctypes binding of libtpsc_hip.so (the C ABI declared in include/tpsc_capi.h).
+
+The product path: every computation runs in the HIP library.  If the shared library is
+missing this module raises at import time instead of falling back to anything else.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "libtpsc_hip.so")
+
+TPSC_OK = 0
+ERRORS = {-1: "TPSC_E_INVALID", -2: "TPSC_E_DEVICE", -3: "TPSC_E_NOMEM", -4: "TPSC_E_ORDER", -5: "TPSC_E_NOROOT"}
+
+FIELDS = {
+    "ek": 1, "chi0": 2, "chirt": 3, "chi0_inf": 4, "chi0_inf2": 5, "dtau_chirt0": 6, "V": 7,
+    "Sigma_tau": 8, "Self": 9, "Sigma_inf2": 10, "Sigma_inf3": 11, "Sigma_inf4": 12,
+    "chijj1": 13, "chijj2": 14, "dtau_Sigma": 15, "chi0_tau": 16,
+}
+SCALARS = {
+    "mu0": 1, "chi0max": 2, "qmax_l": 3, "qmax_m": 4, "Usp": 5, "Uch": 6, "dblOcc": 7,
+    "difRelsp": 8, "difRelch": 9, "Sigma_inf": 10, "trSG": 11, "trSG_corr": 12, "mu": 13,
+    "trSG2": 14, "dblOcc2": 15, "kx0": 16, "chijj1_inf2": 17, "chijj2_inf2": 18,
+    "chijj2_inf4": 19, "nevals_sp": 20, "nevals_ch": 21, "nevals_mu": 22, "nk": 23, "nk8": 24,
+    "nwb": 25, "nq": 26,
+}
+
+
+class TPSCError(RuntimeError):
+    pass
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("nk0", C.c_int), ("Nw0", C.c_int), ("n", C.c_double), ("T", C.c_double),
+        ("eta", C.c_double), ("tp", C.c_double), ("tpp", C.c_double), ("t3", C.c_double),
+        ("U", C.c_double), ("coarse_pass", C.c_int), ("precision", C.c_int),
+    ]
+
+
+_lib = None
+
+EXPORTS = [
+    "tpsc_create", "tpsc_destroy", "tpsc_last_error", "tpsc_find_mu0", "tpsc_chi0", "tpsc_sum_rule",
+    "tpsc_find_Usp_Uch_grid", "tpsc_find_Usp_Uch", "tpsc_vertex_rt", "tpsc_self_energy",
+    "tpsc_calc_self_energy_full", "tpsc_trace_self_G", "tpsc_trace_diff_G", "tpsc_find_mu",
+    "tpsc_chijj", "tpsc_solve", "tpsc_get", "tpsc_field_size", "tpsc_get_scalar",
+    "tpsc_prof_enable", "tpsc_prof_reset", "tpsc_prof_count", "tpsc_prof_get",
+    "tpsc_keep_chi_tau", "tpsc_device_bytes",
+]
+
+
+def lib():
+    """Load libtpsc_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise TPSCError(f"{LIB_PATH} not built: run __graft_entry__.build() (make -C tpsc_conductivity_amd/csrc)")
+    L = C.CDLL(LIB_PATH)
+    vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)
+    L.tpsc_create.argtypes = [C.POINTER(Params), C.POINTER(vp)]
+    L.tpsc_destroy.argtypes = [vp]
+    L.tpsc_destroy.restype = None
+    L.tpsc_last_error.argtypes = [vp]
+    L.tpsc_last_error.restype = C.c_char_p
+    L.tpsc_find_mu0.argtypes = [vp, dp]
+    L.tpsc_chi0.argtypes = [vp, C.c_int, C.c_int]
+    L.tpsc_sum_rule.argtypes = [vp, C.c_int, C.c_double, dp]
+    L.tpsc_find_Usp_Uch_grid.argtypes = [vp, C.c_int, C.c_int, dp, dp]
+    L.tpsc_find_Usp_Uch.argtypes = [vp, dp, dp]
+    L.tpsc_vertex_rt.argtypes = [vp]
+    L.tpsc_self_energy.argtypes = [vp]
+    L.tpsc_calc_self_energy_full.argtypes = [vp]
+    L.tpsc_trace_self_G.argtypes = [vp, dp, dp]
+    L.tpsc_trace_diff_G.argtypes = [vp, C.c_double, dp]
+    L.tpsc_find_mu.argtypes = [vp, dp]
+    L.tpsc_chijj.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+    L.tpsc_solve.argtypes = [vp]
+    L.tpsc_get.argtypes = [vp, C.c_int, vp, C.c_size_t]
+    L.tpsc_field_size.argtypes = [vp, C.c_int]
+    L.tpsc_field_size.restype = C.c_size_t
+    L.tpsc_get_scalar.argtypes = [vp, C.c_int, dp]
+    L.tpsc_prof_enable.argtypes = [vp, C.c_int]
+    L.tpsc_prof_reset.argtypes = [vp]
+    L.tpsc_prof_count.argtypes = [vp]
+    L.tpsc_prof_get.argtypes = [vp, C.c_int, C.POINTER(C.c_char_p), dp, ip]
+    L.tpsc_keep_chi_tau.argtypes = [vp, C.c_int]
+    L.tpsc_device_bytes.argtypes = [vp, dp]
+    _lib = L
+    return L
+
+
+def exported_symbols():
+    """Names of include/tpsc_capi.h entry points present in the loaded library."""
+    L = lib()
+    return [n for n in EXPORTS if hasattr(L, n)]
+
+
+class Context:
+    """Owns one tpsc_ctx (device buffers + stream for one parameter point)."""
+
+    def __init__(self, nk0, Nw0, n, T, U, tp=0.0, tpp=0.0, t3=0.0, eta=0.1, coarse_pass=True, precision=0):
+        self._h = C.c_void_p(None)
+        self.params = Params(nk0, Nw0, n, T, eta, tp, tpp, t3, U, int(bool(coarse_pass)), precision)
+        rc = lib().tpsc_create(C.byref(self.params), C.byref(self._h))
+        if rc != TPSC_OK:
+            msg = self.last_error()
+            self.close()
+            raise TPSCError(f"tpsc_create failed ({ERRORS.get(rc, rc)}): {msg}")
+
+    def last_error(self):
+        if not self._h:
+            return "no context"
+        m = lib().tpsc_last_error(self._h)
+        return m.decode() if m else ""
+
+    def check(self, rc, what):
+        if rc != TPSC_OK:
+            raise TPSCError(f"{what} failed ({ERRORS.get(rc, rc)}): {self.last_error()}")
+
+    def close(self):
+        if self._h:
+            lib().tpsc_destroy(self._h)
+            self._h = C.c_void_p(None)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def scalar(self, name):
+        v = C.c_double(0.0)
+        self.check(lib().tpsc_get_scalar(self._h, SCALARS[name], C.byref(v)), f"tpsc_get_scalar({name})")
+        return v.value
+
+    def get(self, name):
+        f = FIELDS[name]
+        nbytes = lib().tpsc_field_size(self._h, f)
+        out = np.empty(nbytes // 8, dtype=np.float64)
+        self.check(lib().tpsc_get(self._h, f, out.ctypes.data_as(C.c_void_p), nbytes), f"tpsc_get({name})")
+        if name == "Self":
+            return out.view(np.complex128)
+        return out

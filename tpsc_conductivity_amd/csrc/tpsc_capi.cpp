@@ -1,0 +1,995 @@
+// tpsc_capi.cpp -- host side of libtpsc_hip.so: the C ABI of include/tpsc_capi.h over the
+// HIP kernels of tpsc_kernels.hip.  Stage order and scalar logic follow the reference
+// object methods (cond_opt > green > chi > chi0 > green0 > hamiltonien); the heavy
+// arrays stay in HBM from chi0 to Lambda_jj, only scalars cross to the host.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/tpsc_capi.h"
+#include "tpsc_launch.hpp"
+
+using namespace tpsc;
+
+namespace {
+
+constexpr double PI = 3.14159265358979323846;
+
+struct ProfPair {
+    std::string name;
+    hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct tpsc_ctx {
+    tpsc_params p{};
+    // current grid (green0::set_grid)
+    int nbk = 0, nbw = 0, nwmax = 0;
+    Grid g{};
+    std::vector<double> ek, er;
+    double mu0 = 0.0;
+    double nsr = 0.0;
+    // results
+    double chi0max = 0.0;
+    int qmax[2] = {0, 0};
+    double Usp = 0.0, Uch = 0.0, dblOcc = 0.0, difRelsp = 0.0, difRelch = 0.0;
+    double Sigma_inf = 0.0, trSG = 0.0, trSG_corr = 0.0, trSG2 = 0.0, dblOcc2 = 0.0;
+    double mu = 0.0;
+    bool mu_ok = false;
+    int nev_sp = 0, nev_ch = 0, nev_mu = 0;
+    std::vector<double> chijj1, chijj2;
+    double kx0 = 0.0, jj_inf1 = 0.0, jj_inf2 = 0.0, jj_inf4 = 0.0;
+    // stage state
+    bool have_chi0 = false, have_U = false, have_V = false, have_sigma = false, have_jj = false;
+    bool keep_chi_tau = false;
+    // device
+    hipStream_t stream = nullptr;
+    DevArrays d{};
+    double* chi_tau = nullptr;
+    double* host_scalar = nullptr;  // pinned
+    size_t dev_bytes = 0;
+    std::vector<void*> allocs;
+    std::string err;
+    // profiling
+    bool prof = false;
+    std::vector<ProfPair> pending;
+    std::map<std::string, std::pair<double, int>> prof_acc;
+    std::vector<std::string> prof_names;
+};
+
+namespace {
+
+int fail(tpsc_ctx* c, int code, const std::string& msg)
+{
+    c->err = msg;
+    return code;
+}
+
+#define HIPCHK(c, expr)                                                                          \
+    do {                                                                                         \
+        hipError_t _e = (expr);                                                                  \
+        if (_e != hipSuccess)                                                                    \
+            return fail((c), TPSC_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+void prof_begin(tpsc_ctx* c, const char* name)
+{
+    if (!c->prof) return;
+    ProfPair pp;
+    pp.name = name;
+    hipEventCreate(&pp.a);
+    hipEventCreate(&pp.b);
+    hipEventRecord(pp.a, c->stream);
+    c->pending.push_back(pp);
+}
+void prof_end(tpsc_ctx* c)
+{
+    if (!c->prof || c->pending.empty()) return;
+    hipEventRecord(c->pending.back().b, c->stream);
+}
+void prof_flush(tpsc_ctx* c)
+{
+    if (c->pending.empty()) return;
+    hipStreamSynchronize(c->stream);
+    for (auto& pp : c->pending) {
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, pp.a, pp.b);
+        auto it = c->prof_acc.find(pp.name);
+        if (it == c->prof_acc.end()) {
+            c->prof_acc[pp.name] = {ms, 1};
+            c->prof_names.push_back(pp.name);
+        } else {
+            it->second.first += ms;
+            it->second.second += 1;
+        }
+        hipEventDestroy(pp.a);
+        hipEventDestroy(pp.b);
+    }
+    c->pending.clear();
+}
+
+// ------------------------------------------------------------------ model (host)
+double dispk(const tpsc_params& p, double kx, double ky)
+{
+    return -2.0 * (cos(kx) + cos(ky)) - 4.0 * p.tp * cos(kx) * cos(ky) - 2.0 * p.tpp * (cos(2 * kx) + cos(2 * ky)) -
+           4 * p.t3 * (cos(2 * kx) * cos(ky) + cos(2 * ky) * cos(kx));
+}
+double grad_x(const tpsc_params& p, double kx, double ky)
+{
+    return 2.0 * sin(kx) + 4.0 * p.tp * sin(kx) * cos(ky) + 4.0 * p.tpp * sin(2 * kx) +
+           4 * p.t3 * (2.0 * sin(2 * kx) * cos(ky) + cos(2 * ky) * sin(kx));
+}
+double hess_xx(const tpsc_params& p, double kx, double ky)
+{
+    return 2.0 * cos(kx) + 4.0 * p.tp * cos(kx) * cos(ky) + 8.0 * p.tpp * cos(2 * kx) +
+           4 * p.t3 * (4.0 * cos(2 * kx) * cos(ky) + cos(2 * ky) * cos(kx));
+}
+
+// ------------------------------------------------------------------ root finder
+// generique.cpp:604-913 (lims == nullptr) and :917-1234 (lims given): bracketed inverse
+// quadratic steps with secant fallback and bisection on stalls; secant search first when
+// the initial interval has no sign change.  Returns the abscissa of the smallest |f| kept.
+// The reference leaves the secant start unset when f(init[0]) is the smallest; it starts
+// from init[0] here.
+bool find_zero(const std::function<double(double)>& f, const double init[3], double* root, const double* lims,
+               int* nevals)
+{
+    const double tol = lims ? 1.0e-15 : 1.0e-16;
+    const double eps = (lims ? 2.0 : 10.0) * DBL_EPSILON;
+    const int itmax = lims ? 12 : 20;
+    const double dytol = 0.01;
+    int it = 0, ne = 0;
+    double a = 0.0, b = 0.0, c = 0.0;
+    const double xmin = init[1], xmax = init[2];
+    double x[3] = {init[0], xmin, xmax}, y[3] = {0.0, 0.0, 0.0};
+    auto F = [&](double v) {
+        ne++;
+        return f(v);
+    };
+    auto done = [&](double v, bool ok) {
+        *root = v;
+        if (nevals) *nevals = ne;
+        return ok;
+    };
+    y[1] = F(xmin);
+    y[2] = F(xmax);
+    if (fabs(y[1]) < tol) return done(xmin, true);
+    if (fabs(y[2]) < tol) return done(xmax, true);
+    if (y[1] * y[2] > 0) {
+        y[0] = F(x[0]);
+        if (fabs(y[0]) < tol) return done(x[0], true);
+        if (y[0] * y[2] < 0) {
+            const double xt = x[0];
+            x[0] = (x[0] + x[2]) / 2;
+            x[1] = xt;
+            y[1] = y[0];
+        } else {
+            double yt = y[0], xt = x[0];
+            if (fabs(y[1]) < fabs(yt)) yt = y[1], xt = x[1];
+            if (fabs(y[2]) < fabs(yt)) yt = y[2], xt = x[2];
+            double xt1 = xt + 10 * eps, yt1 = F(xt1);
+            xt1 = xt - yt * ((xt1 - xt) / (yt1 - yt));
+            yt1 = F(xt1);
+            it++;
+            while (fabs(yt1) > tol && it < itmax && yt1 * yt > 0 && std::isfinite(xt1)) {
+                xt = xt1;
+                yt = yt1;
+                xt1 = xt + 10 * eps;
+                yt1 = F(xt1);
+                xt1 = xt - yt * ((xt1 - xt) / (yt1 - yt));
+                yt1 = F(xt1);
+                it++;
+            }
+            if (fabs(yt1) < tol) return done(xt1, true);
+            if (yt1 * yt < 0) {
+                x[0] = (xt + xt1) / 2;
+                x[1] = xt;
+                y[1] = yt;
+                x[2] = xt1;
+                y[2] = yt1;
+            }
+            if (it >= itmax || !std::isfinite(xt1)) return done(xt1, false);
+        }
+    }
+    if (lims && (x[1] < lims[0] || x[1] > lims[1] || x[2] < lims[0] || x[2] > lims[1])) return done(x[0], false);
+    if (fabs(xmax - xmin) < eps) return done(xmin, true);
+    y[0] = F(x[0]);
+    if (fabs(y[0]) < tol) return done(x[0], true);
+
+    int i3, imid, ifar, im, ip;
+    auto sides = [&]() {
+        int i1, i2;
+        if (y[0] * y[1] > 0)
+            i1 = 0, i2 = 1;
+        else if (y[0] * y[2] > 0)
+            i1 = 0, i2 = 2;
+        else
+            i1 = 1, i2 = 2;
+        i3 = 3 - i1 - i2;
+        if (x[i3] > x[i1]) {
+            if (x[i1] > x[i2])
+                ifar = i2, imid = i1;
+            else
+                ifar = i1, imid = i2;
+            im = imid, ip = i3;
+        } else {
+            if (x[i1] < x[i2])
+                ifar = i2, imid = i1;
+            else
+                ifar = i1, imid = i2;
+            im = i3, ip = imid;
+        }
+    };
+    sides();
+    x[ifar] = -y[imid] * (x[i3] - x[imid]) / (y[i3] - y[imid]) + x[imid];
+    y[ifar] = F(x[ifar]);
+    double dy = y[ifar];
+    while (fabs(dy) > tol && it < itmax) {
+        sides();
+        const double den = -x[0] * x[0] * x[2] + x[0] * x[0] * x[1] - x[0] * x[1] * x[1] + x[0] * x[2] * x[2] -
+                           x[2] * x[2] * x[1] + x[2] * x[1] * x[1];
+        if (fabs(den) > 0.0) {
+            a = (x[0] * y[2] - x[0] * y[1] + x[2] * y[1] - y[2] * x[1] - y[0] * x[2] + y[0] * x[1]) / den;
+            b = -(-y[2] * x[1] * x[1] + x[1] * x[1] * y[0] - x[2] * x[2] * y[0] - y[1] * x[0] * x[0] +
+                  y[2] * x[0] * x[0] + y[1] * x[2] * x[2]) / den;
+            c = (y[2] * x[0] * x[0] * x[1] - y[1] * x[0] * x[0] * x[2] - x[1] * x[1] * x[0] * y[2] +
+                 x[2] * x[2] * x[0] * y[1] + x[1] * x[1] * y[0] * x[2] - x[2] * x[2] * y[0] * x[1]) / den;
+        }
+        auto secant_or_stop = [&](bool& stop) {
+            if (fabs(y[i3] - y[imid]) > tol)
+                x[ifar] = -y[imid] * (x[i3] - x[imid]) / (y[i3] - y[imid]) + x[imid];
+            else
+                stop = true;
+        };
+        bool stop = false;
+        if ((b * b - 4.0 * a * c) > 0.0 && fabs(den) > 0.0) {
+            const double xp = (-b + sqrt(b * b - 4.0 * a * c)) / (2 * a);
+            const double xm = (-b - sqrt(b * b - 4.0 * a * c)) / (2 * a);
+            if (xp > x[im] && xp < x[ip])
+                x[ifar] = xp;
+            else if (xm > x[im] && xm < x[ip])
+                x[ifar] = xm;
+            else
+                secant_or_stop(stop);
+        } else
+            secant_or_stop(stop);
+        if (stop) return done(x[imid], true);
+        y[ifar] = F(x[ifar]);
+        double dyprec = fabs(dy);
+        dy = y[ifar];
+        if (fabs(dyprec - fabs(dy)) < dytol * dyprec || dyprec < fabs(dy)) {
+            x[ifar] = (x[i3] + x[imid]) / 2.0;
+            y[ifar] = F(x[ifar]);
+            dy = y[ifar];
+        }
+        it++;
+    }
+    int best = 0;
+    if (fabs(y[1]) < fabs(y[best])) best = 1;
+    if (fabs(y[2]) < fabs(y[best])) best = 2;
+    return done(x[best], true);
+}
+
+// ------------------------------------------------------------------ device helpers
+template <typename T>
+int dalloc(tpsc_ctx* c, T** ptr, size_t count)
+{
+    void* p = nullptr;
+    const size_t bytes = sizeof(T) * (count ? count : 1);
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) return fail(c, TPSC_E_NOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    c->allocs.push_back(p);
+    c->dev_bytes += bytes;
+    *ptr = (T*)p;
+    return 0;
+}
+
+int read_scalar(tpsc_ctx* c, const double* dptr, double* out)
+{
+    HIPCHK(c, hipMemcpyAsync(c->host_scalar, dptr, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *out = c->host_scalar[0];
+    return 0;
+}
+
+// green0::find_mu (green0.cpp:240-262) with the density evaluated on the device
+int find_mu0(tpsc_ctx* c)
+{
+    if (c->p.n == 1.0 && c->p.tp == 0 && c->p.tpp == 0) {
+        c->mu0 = 0.0;
+        return 0;
+    }
+    int rc = 0;
+    auto f = [&](double m) {
+        double v = 0.0;
+        if (rc == 0) {
+            hipError_t e = launch_density(c->g, c->d, m, c->p.n, c->stream);
+            if (e != hipSuccess) rc = fail(c, TPSC_E_DEVICE, hipGetErrorString(e));
+            if (rc == 0) rc = read_scalar(c, c->d.red, &v);
+        }
+        return v;
+    };
+    const double init[3] = {0.0, -8.0, 8.0};
+    double root = 0.0;
+    const bool ok = find_zero(f, init, &root, nullptr, nullptr);
+    if (rc) return rc;
+    if (ok) c->mu0 = root;  // on failure the previous mu0 is kept, as in the reference
+    return 0;
+}
+
+// green0::set_grid (green0.h:44): quadrant eps(k), wedge tables, mu0
+int set_grid(tpsc_ctx* c, int N, int Nw)
+{
+    if (!supported_N(N) || !supported_Nw(Nw))
+        return fail(c, TPSC_E_INVALID, "unsupported grid N=" + std::to_string(N) + " Nw=" + std::to_string(Nw));
+    const int nk = N / 2 + 1;
+    c->nbk = nk;
+    c->nbw = Nw;
+    c->nwmax = Nw / 2;
+    Grid& g = c->g;
+    g.N = N;
+    g.nk = nk;
+    g.Nw = Nw;
+    g.nwb = Nw / 2 + 1;
+    g.nk8 = (long)nk * (nk + 1) / 2;
+    g.nq = (long)nk * (nk - 2);
+    g.T = c->p.T;
+    c->ek.assign((size_t)nk * nk, 0.0);
+    for (int l = 0; l < nk; l++)
+        for (int m = 0; m < nk; m++) c->ek[m + l * nk] = dispk(c->p, l * PI / (nk - 1), m * PI / (nk - 1));
+    c->er.assign(std::max<long>(g.nk8, 5), 0.0);
+    c->er[1] = -1;
+    c->er[2] = -c->p.tp;
+    c->er[3] = -c->p.tpp;
+    c->er[4] = -c->p.t3;
+    std::vector<double> ekw(g.nk8), wq(g.nk8), d2(nk * nk), dek(std::max<long>(g.nq, 1));
+    for (int l = 0; l < nk; l++)
+        for (int m = 0; m <= l; m++) {
+            const long i = (long)m + (long)l * (l + 1) / 2;
+            ekw[i] = c->ek[m + l * nk];
+            double w = 8;
+            if (l == 0 || l == nk - 1) w /= 2;
+            if (m == 0 || m == nk - 1) w /= 2;
+            if (m == l) w /= 2;
+            wq[i] = w;
+        }
+    for (int l = 0; l < nk; l++)
+        for (int m = 0; m < nk; m++) d2[m + l * nk] = hess_xx(c->p, l * PI / (nk - 1), m * PI / (nk - 1));
+    for (int l = 0; l < nk - 2; l++)
+        for (int m = 0; m < nk; m++) dek[m + l * nk] = grad_x(c->p, (l + 1) * PI / (nk - 1), m * PI / (nk - 1));
+    hipStream_t s = c->stream;
+    HIPCHK(c, hipMemcpyAsync(c->d.ek, c->ek.data(), sizeof(double) * nk * nk, hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(c->d.er, c->er.data(), sizeof(double) * g.nk8, hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(c->d.ekw, ekw.data(), sizeof(double) * g.nk8, hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(c->d.wq, wq.data(), sizeof(double) * g.nk8, hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(c->d.d2ek, d2.data(), sizeof(double) * nk * nk, hipMemcpyHostToDevice, s));
+    if (g.nq > 0) HIPCHK(c, hipMemcpyAsync(c->d.dek, dek.data(), sizeof(double) * g.nq, hipMemcpyHostToDevice, s));
+    HIPCHK(c, launch_fill_tw(c->d.twN, Nw, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    c->have_chi0 = false;
+    return find_mu0(c);
+}
+
+// ------------------------------------------------------------------ stages
+// chi0::calc_chiqw_FFT_spline, chi0.cpp:279-718
+int stage_chi0(tpsc_ctx* c, int nq0, int nw0)
+{
+    if (nq0 > c->p.nk0 || nw0 > c->p.Nw0) return fail(c, TPSC_E_INVALID, "chi0 grid larger than the context grid");
+    int rc = set_grid(c, nq0, nw0);
+    if (rc) return rc;
+    hipStream_t s = c->stream;
+    prof_begin(c, "chi0_tau0");
+    HIPCHK(c, launch_chi0_tau0(c->g, c->d, c->mu0, s));
+    prof_end(c);
+    prof_begin(c, "chi0_slab");
+    HIPCHK(c, launch_chi0_slab(c->g, c->d, c->mu0, s));
+    prof_end(c);
+    if (c->keep_chi_tau && c->chi_tau)
+        HIPCHK(c, hipMemcpyAsync(c->chi_tau, c->d.chi, sizeof(double) * c->g.nwb * c->g.nk8, hipMemcpyDeviceToDevice, s));
+    prof_begin(c, "chi0_pencil");
+    HIPCHK(c, launch_chi0_pencil(c->g, c->d, s));
+    prof_end(c);
+    HIPCHK(c, launch_chi0max(c->g, c->d, s));
+    int imax = 0;
+    HIPCHK(c, hipMemcpyAsync(c->host_scalar, c->d.red, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipMemcpyAsync(&imax, c->d.imax, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    prof_flush(c);
+    c->chi0max = c->host_scalar[0];
+    int l = 0;
+    while ((long)(l + 1) * (l + 2) / 2 <= imax) l++;
+    c->qmax[0] = l;
+    c->qmax[1] = imax - l * (l + 1) / 2;
+    c->have_chi0 = true;
+    return 0;
+}
+
+int eval_sum_rule(tpsc_ctx* c, int channel, double Ux, double* out)
+{
+    if (!c->have_chi0) return fail(c, TPSC_E_ORDER, "sum rule before chi0");
+    double tr = 0.0;
+    HIPCHK(c, launch_sum_rule(c->g, c->d, channel == 0 ? -Ux : Ux, c->stream));
+    int rc = read_scalar(c, c->d.red, &tr);
+    if (rc) return rc;
+    const double n = c->p.n, nsr = c->nsr, U = c->p.U;
+    if (channel == 0)
+        *out = tr + n - n * n / 2.0 - nsr + Ux * nsr * nsr / (2.0 * U);
+    else
+        *out = tr + n - n * n / 2.0 - (nsr + c->Usp * nsr * nsr / (2.0 * U) - nsr * nsr);
+    return 0;
+}
+
+// chi::find_Usp_Uch(nq0, nw0, Usc), chi.cpp:846-876
+int stage_UspUch_grid(tpsc_ctx* c, int nq0, int nw0, double Usc[2], bool* ok_out)
+{
+    int rc = stage_chi0(c, nq0, nw0);
+    if (rc) return rc;
+    prof_begin(c, "sum_rules");
+    const double U = c->p.U;
+    auto fsp = [&](double x) {
+        double v = 0.0;
+        if (rc == 0) rc = eval_sum_rule(c, 0, x, &v);
+        return v;
+    };
+    auto fch = [&](double x) {
+        double v = 0.0;
+        if (rc == 0) rc = eval_sum_rule(c, 1, x, &v);
+        return v;
+    };
+    double root = 0.0;
+    double init[3] = {U / 2.0, 0.0, 2.0 / c->chi0max - 2 * DBL_EPSILON};
+    bool ok = true;
+    if (find_zero(fsp, init, &root, nullptr, &c->nev_sp)) {
+        Usc[0] = root;
+        c->Usp = root;
+    } else
+        ok = false;
+    if (rc) return rc;
+    root = 0.0;
+    init[0] = 0;
+    init[2] = 1000.0 * U;
+    if (find_zero(fch, init, &root, nullptr, &c->nev_ch))
+        Usc[1] = root;
+    else
+        ok = false;
+    prof_end(c);
+    prof_flush(c);
+    if (rc) return rc;
+    *ok_out = ok;
+    return 0;
+}
+
+// chi::find_Usp_Uch(), chi.cpp:210-417
+int stage_UspUch(tpsc_ctx* c)
+{
+    double Usc[2] = {0.0, 0.0}, Us0 = 0.0, Uc0 = 0.0;
+    bool ok = true, ok0 = true;
+    int rc;
+    if (c->p.coarse_pass) {
+        rc = stage_UspUch_grid(c, c->p.nk0 / 2, c->p.Nw0 / 2, Usc, &ok0);
+        if (rc) return rc;
+        Us0 = Usc[0];
+        Uc0 = Usc[1];
+    }
+    rc = stage_UspUch_grid(c, c->p.nk0, c->p.Nw0, Usc, &ok);
+    if (rc) return rc;
+    c->difRelsp = fabs((Usc[0] - Us0) / Usc[0]);
+    c->difRelch = fabs((Usc[1] - Uc0) / Usc[1]);
+    c->Usp = Usc[0];
+    c->Uch = Usc[1];
+    const double U = c->p.U, nsr = c->nsr;
+    c->dblOcc = (c->p.n > 1.0) ? c->Usp / U * nsr * nsr / 4.0 + 1.0 - nsr : c->Usp / U * nsr * nsr / 4.0;
+    c->have_U = true;
+    if (!ok) return fail(c, TPSC_E_NOROOT, "Usp or Uch root not found");
+    return 0;
+}
+
+bool full_grid(const tpsc_ctx* c) { return c->g.N == c->p.nk0 && c->g.Nw == c->p.Nw0; }
+
+// chi::calc_chirt, chi.cpp:37-206
+int stage_vertex(tpsc_ctx* c)
+{
+    if (!c->have_U || !c->have_chi0 || !full_grid(c)) {
+        int rc = stage_UspUch(c);
+        if (rc) return rc;
+    }
+    prof_begin(c, "vertex_slab");
+    HIPCHK(c, launch_vertex_slab(c->g, c->d, c->p.U, c->Usp, c->Uch, c->stream));
+    prof_end(c);
+    prof_begin(c, "vertex_pencil");
+    HIPCHK(c, launch_vertex_pencil(c->g, c->d, c->p.U, c->Usp, c->Uch, c->stream));
+    prof_end(c);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    prof_flush(c);
+    c->have_V = true;
+    return 0;
+}
+
+// green::calc_Self_FFT_spline slab + pencil parts, green.cpp:1737-2031
+int stage_sigma(tpsc_ctx* c)
+{
+    if (!c->have_V) return fail(c, TPSC_E_ORDER, "self-energy before the vertex");
+    const double fact = c->p.U * (3.0 * c->Usp + c->Uch) / 8.0;
+    prof_begin(c, "sigma_bnd");
+    HIPCHK(c, launch_sigma_bnd(c->g, c->d, c->mu0, fact, c->stream));
+    prof_end(c);
+    prof_begin(c, "sigma_slab");
+    HIPCHK(c, launch_sigma_slab(c->g, c->d, c->mu0, c->stream));
+    prof_end(c);
+    prof_begin(c, "sigma_pencil");
+    HIPCHK(c, launch_sigma_pencil(c->g, c->d, c->stream));
+    prof_end(c);
+    int rc = read_scalar(c, c->d.Sinf, &c->Sigma_inf);
+    if (rc) return rc;
+    prof_flush(c);
+    c->have_sigma = true;
+    c->mu_ok = false;
+    return 0;
+}
+
+int reduce_trace(tpsc_ctx* c, int which, double* out)
+{
+    HIPCHK(c, launch_trace_selfG(c->g, c->d, c->mu0, c->mu, which, c->stream));
+    return read_scalar(c, c->d.red, out);
+}
+
+// green::traceSelfG, green.cpp:3001-3104
+int stage_trace_selfG(tpsc_ctx* c)
+{
+    if (!c->have_sigma) return fail(c, TPSC_E_ORDER, "traceSelfG before the self-energy");
+    double s0 = 0.0, s1 = 0.0;
+    int rc = reduce_trace(c, 0, &s0);
+    if (!rc) rc = reduce_trace(c, 1, &s1);
+    if (rc) return rc;
+    c->trSG = s0;
+    c->trSG_corr = s0 + s1;
+    return 0;
+}
+
+// green::traceSelfG2, green.cpp:2929-2997
+int stage_trace_selfG2(tpsc_ctx* c)
+{
+    double s = 0.0;
+    int rc = reduce_trace(c, 2, &s);
+    if (rc) return rc;
+    c->trSG2 = s;
+    c->dblOcc2 = (s + c->p.U * c->p.n * c->p.n / 4.0) / c->p.U;
+    return 0;
+}
+
+int eval_diffG(tpsc_ctx* c, double pmu, double* out)
+{
+    HIPCHK(c, launch_trace_diffG(c->g, c->d, pmu, c->mu0, c->stream));
+    return read_scalar(c, c->d.red, out);
+}
+
+// green::find_mu, green.cpp:3185-3282
+int stage_mu(tpsc_ctx* c)
+{
+    if (!c->have_sigma) return fail(c, TPSC_E_ORDER, "find_mu before the self-energy");
+    int rc = 0;
+    prof_begin(c, "mu_root");
+    auto f = [&](double m) {
+        double v = 0.0;
+        if (rc == 0) rc = eval_diffG(c, m, &v);
+        return v;
+    };
+    const double init[3] = {0.0, -10.0, 10.0}, lims[2] = {-20, 20};
+    double root = 0.0;
+    const bool ok = find_zero(f, init, &root, lims, &c->nev_mu);
+    prof_end(c);
+    prof_flush(c);
+    if (rc) return rc;
+    if (ok) {
+        c->mu = root;
+        c->mu_ok = true;
+    }
+    rc = stage_trace_selfG2(c);
+    if (rc) return rc;
+    if (!ok) return fail(c, TPSC_E_NOROOT, "mu not found");
+    return 0;
+}
+
+void jj_mark(void* ctx, int k)
+{
+    tpsc_ctx* c = (tpsc_ctx*)ctx;
+    static const char* names[] = {"jj_hf0_slab", "jj_hf0_bnd", "jj_hVf0V_slab", "jj_pencil", "jj_kx0_rows"};
+    if (k > 0) prof_end(c);
+    if (k < 5) prof_begin(c, names[k]);
+}
+
+// cond_opt::calc_chijj_vertex_corr_all_optim (chi3 = false), cond_opt.cpp:2537-4290
+int stage_chijj(tpsc_ctx* c)
+{
+    if (!c->have_sigma) {  // cond_opt.cpp:2550-2560
+        int rc = tpsc_calc_self_energy_full(c);
+        if (!rc) rc = stage_trace_selfG(c);
+        if (!rc) rc = stage_mu(c);
+        if (rc) return rc;
+    }
+    if (!c->mu_ok) return fail(c, TPSC_E_ORDER, "calc_chijj_vertex_corr_all_optim(): mu not computed");
+    const Grid& g = c->g;
+    const double fact4 = c->p.U * (3.0 * c->Usp + c->Uch) / 4;
+    HIPCHK(c, launch_chijj(g, c->d, c->mu0, c->mu, fact4, c->stream, c->prof ? jj_mark : nullptr, c));
+    const int nblk = (int)((g.nq + kJJK - 1) / kJJK);
+    const int nwb = g.nwb;
+    std::vector<double> part((size_t)nblk * 2 * nwb), scal((size_t)nblk * 4), rows(2 * g.nk);
+    HIPCHK(c, hipMemcpyAsync(part.data(), c->d.jjpart, sizeof(double) * part.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(scal.data(), c->d.jjscal, sizeof(double) * scal.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(rows.data(), c->d.kx0part, sizeof(double) * rows.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    prof_flush(c);
+    c->chijj1.assign(nwb, 0.0);
+    c->chijj2.assign(nwb, 0.0);
+    double k0 = 0.0, i1 = 0.0, i2 = 0.0, i4 = 0.0;
+    for (int b = 0; b < nblk; b++) {
+        const double* r = &part[(size_t)b * 2 * nwb];
+        for (int n = 0; n < nwb; n++) {
+            c->chijj1[n] += r[n];
+            c->chijj2[n] += r[nwb + n];
+        }
+        k0 += scal[b * 4 + 0];
+        i1 += scal[b * 4 + 1];
+        i2 += scal[b * 4 + 2];
+        i4 += scal[b * 4 + 3];
+    }
+    for (double v : rows) k0 += v;
+    c->kx0 = k0;
+    c->jj_inf1 = i1;
+    c->jj_inf2 = i2;
+    c->jj_inf4 = i4;
+    c->have_jj = true;
+    return 0;
+}
+
+size_t field_count(tpsc_ctx* c, int field)
+{
+    const Grid& g = c->g;
+    switch (field) {
+        case TPSC_F_EK: return (size_t)g.nk * g.nk;
+        case TPSC_F_CHI0:
+        case TPSC_F_CHIRT:
+        case TPSC_F_V:
+        case TPSC_F_CHI0_TAU: return (size_t)g.nwb * g.nk8;
+        case TPSC_F_CHI0_INF:
+        case TPSC_F_CHI0_INF2:
+        case TPSC_F_DTAU_CHIRT0:
+        case TPSC_F_SIGMA_INF2:
+        case TPSC_F_SIGMA_INF3:
+        case TPSC_F_SIGMA_INF4: return (size_t)g.nk8;
+        case TPSC_F_DTAU_SIGMA: return 2 * (size_t)g.nk8;
+        case TPSC_F_SIGMA_TAU: return (size_t)(g.Nw + 1) * g.nk8;
+        case TPSC_F_SELF: return 2 * (size_t)(g.Nw / 2) * g.nk8;
+        case TPSC_F_CHIJJ1:
+        case TPSC_F_CHIJJ2: return (size_t)g.nwb;
+        default: return 0;
+    }
+}
+
+}  // namespace
+
+// ================================================================== C ABI
+extern "C" {
+
+int tpsc_create(const tpsc_params* p, tpsc_ctx** out)
+{
+    if (!p || !out) return TPSC_E_INVALID;
+    *out = nullptr;
+    tpsc_ctx* c = new tpsc_ctx();
+    c->p = *p;
+    if (p->precision != 0) {
+        *out = c;
+        return fail(c, TPSC_E_INVALID, "only precision 0 (FP64) is available");
+    }
+    if (!supported_N(p->nk0) || !supported_Nw(p->Nw0) || (p->coarse_pass && (p->nk0 < 8 || p->Nw0 < 32))) {
+        *out = c;
+        return fail(c, TPSC_E_INVALID, "unsupported grid");
+    }
+    if (!(p->T > 0.0)) {
+        *out = c;
+        return fail(c, TPSC_E_INVALID, "temperature must be positive");
+    }
+    *out = c;
+    c->nsr = (p->n > 1.0) ? 2.0 - p->n : p->n;  // chi.cpp:18-19
+    int dev_count = 0;
+    if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count < 1)
+        return fail(c, TPSC_E_DEVICE, "no HIP device available");
+    HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIPCHK(c, hipHostMalloc((void**)&c->host_scalar, 16 * sizeof(double), hipHostMallocDefault));
+    const int N = p->nk0, Nw = p->Nw0, nk = N / 2 + 1;
+    const long nk8 = (long)nk * (nk + 1) / 2, nq = (long)nk * (nk - 2);
+    const long nwb = Nw / 2 + 1;
+    DevArrays& d = c->d;
+    int rc = 0;
+    rc |= dalloc(c, &d.ek, (size_t)nk * nk);
+    rc |= dalloc(c, &d.ekw, nk8);
+    rc |= dalloc(c, &d.wq, nk8);
+    rc |= dalloc(c, &d.er, std::max<long>(nk8, 5));
+    rc |= dalloc(c, &d.chi, (size_t)nwb * nk8);
+    rc |= dalloc(c, &d.chirt, (size_t)nwb * nk8);
+    rc |= dalloc(c, &d.dchirt0, nk8);
+    rc |= dalloc(c, &d.dGrt0p, nk8);
+    rc |= dalloc(c, &d.dchiqt0, nk8);
+    rc |= dalloc(c, &d.chi0inf, nk8);
+    rc |= dalloc(c, &d.chi0inf2, nk8);
+    rc |= dalloc(c, &d.V, (size_t)nwb * nk8);
+    rc |= dalloc(c, &d.Stau, (size_t)(Nw + 1) * nk8);
+    rc |= dalloc(c, &d.dSig, 2 * nk8);
+    rc |= dalloc(c, &d.Self, (size_t)(Nw / 2) * nk8);
+    rc |= dalloc(c, &d.Sinf2, nk8);
+    rc |= dalloc(c, &d.Sinf3, nk8);
+    rc |= dalloc(c, &d.Sinf4, nk8);
+    rc |= dalloc(c, &d.Sinf, 1);
+    rc |= dalloc(c, &d.dek, std::max<long>(nq, 1));
+    rc |= dalloc(c, &d.d2ek, (size_t)nk * nk);
+    rc |= dalloc(c, &d.h, (size_t)(Nw + 1) * std::max<long>(nq, 1));
+    rc |= dalloc(c, &d.f0, (size_t)(Nw + 1) * std::max<long>(nq, 1));
+    rc |= dalloc(c, &d.dhV, 2 * std::max<long>(nq, 1));
+    rc |= dalloc(c, &d.df0V, 2 * std::max<long>(nq, 1));
+    const long nblk = (nq + kJJK - 1) / kJJK;
+    rc |= dalloc(c, &d.jjpart, (size_t)std::max<long>(nblk, 1) * 2 * nwb);
+    rc |= dalloc(c, &d.jjscal, (size_t)std::max<long>(nblk, 1) * 4);
+    rc |= dalloc(c, &d.kx0part, 2 * (size_t)nk);
+    rc |= dalloc(c, &d.tmp, 4 * (size_t)nk8);
+    rc |= dalloc(c, &d.part, kRedBlocks);
+    rc |= dalloc(c, &d.red, 16);
+    rc |= dalloc(c, &d.twN, (size_t)Nw);
+    rc |= dalloc(c, &d.imax, 4);
+    if (rc) return TPSC_E_NOMEM;
+    return set_grid(c, N, Nw);  // hamiltonien + green0 constructors
+}
+
+void tpsc_destroy(tpsc_ctx* c)
+{
+    if (!c) return;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (auto& pp : c->pending) {
+        hipEventDestroy(pp.a);
+        hipEventDestroy(pp.b);
+    }
+    for (void* p : c->allocs) hipFree(p);
+    if (c->chi_tau) hipFree(c->chi_tau);
+    if (c->host_scalar) hipHostFree(c->host_scalar);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* tpsc_last_error(const tpsc_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int tpsc_find_mu0(tpsc_ctx* c, double* mu0)
+{
+    if (!c) return TPSC_E_INVALID;
+    int rc = find_mu0(c);
+    if (mu0) *mu0 = c->mu0;
+    return rc;
+}
+
+int tpsc_chi0(tpsc_ctx* c, int nq0, int nw0)
+{
+    if (!c) return TPSC_E_INVALID;
+    return stage_chi0(c, nq0, nw0);
+}
+
+int tpsc_sum_rule(tpsc_ctx* c, int channel, double Ux, double* value)
+{
+    if (!c || !value || (channel != 0 && channel != 1)) return TPSC_E_INVALID;
+    return eval_sum_rule(c, channel, Ux, value);
+}
+
+int tpsc_find_Usp_Uch_grid(tpsc_ctx* c, int nq0, int nw0, double* Usp, double* Uch)
+{
+    if (!c) return TPSC_E_INVALID;
+    double Usc[2] = {0.0, 0.0};
+    bool ok = true;
+    int rc = stage_UspUch_grid(c, nq0, nw0, Usc, &ok);
+    if (Usp) *Usp = Usc[0];
+    if (Uch) *Uch = Usc[1];
+    if (rc) return rc;
+    return ok ? 0 : fail(c, TPSC_E_NOROOT, "Usp or Uch root not found");
+}
+
+int tpsc_find_Usp_Uch(tpsc_ctx* c, double* Usp, double* Uch)
+{
+    if (!c) return TPSC_E_INVALID;
+    int rc = stage_UspUch(c);
+    if (Usp) *Usp = c->Usp;
+    if (Uch) *Uch = c->Uch;
+    return rc;
+}
+
+int tpsc_vertex_rt(tpsc_ctx* c)
+{
+    if (!c) return TPSC_E_INVALID;
+    return stage_vertex(c);
+}
+
+int tpsc_self_energy(tpsc_ctx* c)
+{
+    if (!c) return TPSC_E_INVALID;
+    return stage_sigma(c);
+}
+
+int tpsc_calc_self_energy_full(tpsc_ctx* c)
+{
+    if (!c) return TPSC_E_INVALID;
+    int rc = stage_UspUch(c);  // green.cpp:1677-1678
+    if (rc) return rc;
+    rc = stage_vertex(c);      // green.cpp:1682
+    if (rc) return rc;
+    return stage_sigma(c);
+}
+
+int tpsc_trace_self_G(tpsc_ctx* c, double* trace, double* trace_corr)
+{
+    if (!c) return TPSC_E_INVALID;
+    int rc = stage_trace_selfG(c);
+    if (trace) *trace = c->trSG;
+    if (trace_corr) *trace_corr = c->trSG_corr;
+    return rc;
+}
+
+int tpsc_trace_diff_G(tpsc_ctx* c, double mu, double* value)
+{
+    if (!c || !value) return TPSC_E_INVALID;
+    if (!c->have_sigma) return fail(c, TPSC_E_ORDER, "traceDiffG before the self-energy");
+    return eval_diffG(c, mu, value);
+}
+
+int tpsc_find_mu(tpsc_ctx* c, double* mu)
+{
+    if (!c) return TPSC_E_INVALID;
+    int rc = stage_mu(c);
+    if (mu) *mu = c->mu;
+    return rc;
+}
+
+int tpsc_chijj(tpsc_ctx* c, int chi3, int mmin, int N_theta_k)
+{
+    (void)mmin;
+    if (!c) return TPSC_E_INVALID;
+    if (chi3) return fail(c, TPSC_E_INVALID, "Aslamazov-Larkin term (chi3) is not available in this build");
+    if (N_theta_k != 1) return fail(c, TPSC_E_INVALID, "N_theta_k must be 1");
+    if (c->g.Nw > 2048) return fail(c, TPSC_E_INVALID, "Lambda_jj needs Nw <= 2048");
+    return stage_chijj(c);
+}
+
+int tpsc_solve(tpsc_ctx* c)
+{
+    if (!c) return TPSC_E_INVALID;
+    int rc = tpsc_calc_self_energy_full(c);
+    if (!rc) rc = stage_trace_selfG(c);
+    if (!rc) rc = stage_mu(c);
+    if (!rc) rc = tpsc_chijj(c, 0, 0, 1);
+    return rc;
+}
+
+size_t tpsc_field_size(tpsc_ctx* c, int field) { return c ? field_count(c, field) * sizeof(double) : 0; }
+
+int tpsc_get(tpsc_ctx* c, int field, void* dst, size_t bytes)
+{
+    if (!c || !dst) return TPSC_E_INVALID;
+    const size_t n = field_count(c, field);
+    if (n == 0) return fail(c, TPSC_E_INVALID, "unknown field");
+    if (bytes < n * sizeof(double)) return fail(c, TPSC_E_INVALID, "destination too small");
+    const void* src = nullptr;
+    switch (field) {
+        case TPSC_F_EK: memcpy(dst, c->ek.data(), n * sizeof(double)); return 0;
+        case TPSC_F_CHIJJ1:
+        case TPSC_F_CHIJJ2: {
+            const std::vector<double>& v = (field == TPSC_F_CHIJJ1) ? c->chijj1 : c->chijj2;
+            if (v.size() != n) return fail(c, TPSC_E_ORDER, "Lambda_jj not computed");
+            memcpy(dst, v.data(), n * sizeof(double));
+            return 0;
+        }
+        case TPSC_F_CHI0: src = c->d.chi; break;
+        case TPSC_F_CHIRT: src = c->d.chirt; break;
+        case TPSC_F_CHI0_INF: src = c->d.chi0inf; break;
+        case TPSC_F_CHI0_INF2: src = c->d.chi0inf2; break;
+        case TPSC_F_DTAU_CHIRT0: src = c->d.dchirt0; break;
+        case TPSC_F_V: src = c->d.V; break;
+        case TPSC_F_SIGMA_TAU: src = c->d.Stau; break;
+        case TPSC_F_SELF: src = c->d.Self; break;
+        case TPSC_F_SIGMA_INF2: src = c->d.Sinf2; break;
+        case TPSC_F_SIGMA_INF3: src = c->d.Sinf3; break;
+        case TPSC_F_SIGMA_INF4: src = c->d.Sinf4; break;
+        case TPSC_F_DTAU_SIGMA: src = c->d.dSig; break;
+        case TPSC_F_CHI0_TAU:
+            if (!c->chi_tau) return fail(c, TPSC_E_ORDER, "chi0(q,tau) not kept");
+            src = c->chi_tau;
+            break;
+        default: return fail(c, TPSC_E_INVALID, "unknown field");
+    }
+    HIPCHK(c, hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int tpsc_get_scalar(tpsc_ctx* c, int which, double* v)
+{
+    if (!c || !v) return TPSC_E_INVALID;
+    switch (which) {
+        case TPSC_S_MU0: *v = c->mu0; break;
+        case TPSC_S_CHI0MAX: *v = c->chi0max; break;
+        case TPSC_S_QMAX_L: *v = c->qmax[0]; break;
+        case TPSC_S_QMAX_M: *v = c->qmax[1]; break;
+        case TPSC_S_USP: *v = c->Usp; break;
+        case TPSC_S_UCH: *v = c->Uch; break;
+        case TPSC_S_DBLOCC: *v = c->dblOcc; break;
+        case TPSC_S_DIFREL_SP: *v = c->difRelsp; break;
+        case TPSC_S_DIFREL_CH: *v = c->difRelch; break;
+        case TPSC_S_SIGMA_INF: *v = c->Sigma_inf; break;
+        case TPSC_S_TRSG: *v = c->trSG; break;
+        case TPSC_S_TRSG_CORR: *v = c->trSG_corr; break;
+        case TPSC_S_MU: *v = c->mu; break;
+        case TPSC_S_TRSG2: *v = c->trSG2; break;
+        case TPSC_S_DBLOCC2: *v = c->dblOcc2; break;
+        case TPSC_S_KX0: *v = c->kx0; break;
+        case TPSC_S_CHIJJ1_INF2: *v = c->jj_inf1; break;
+        case TPSC_S_CHIJJ2_INF2: *v = c->jj_inf2; break;
+        case TPSC_S_CHIJJ2_INF4: *v = c->jj_inf4; break;
+        case TPSC_S_NEVAL_SP: *v = c->nev_sp; break;
+        case TPSC_S_NEVAL_CH: *v = c->nev_ch; break;
+        case TPSC_S_NEVAL_MU: *v = c->nev_mu; break;
+        case TPSC_S_NK: *v = c->g.nk; break;
+        case TPSC_S_NK8: *v = (double)c->g.nk8; break;
+        case TPSC_S_NWB: *v = c->g.nwb; break;
+        case TPSC_S_NQ: *v = (double)c->g.nq; break;
+        default: return fail(c, TPSC_E_INVALID, "unknown scalar");
+    }
+    return 0;
+}
+
+int tpsc_prof_enable(tpsc_ctx* c, int on)
+{
+    if (!c) return TPSC_E_INVALID;
+    c->prof = on != 0;
+    return 0;
+}
+int tpsc_prof_reset(tpsc_ctx* c)
+{
+    if (!c) return TPSC_E_INVALID;
+    prof_flush(c);
+    c->prof_acc.clear();
+    c->prof_names.clear();
+    return 0;
+}
+int tpsc_prof_count(tpsc_ctx* c) { return c ? (int)c->prof_names.size() : 0; }
+int tpsc_prof_get(tpsc_ctx* c, int i, const char** name, double* total_ms, int* calls)
+{
+    if (!c || i < 0 || i >= (int)c->prof_names.size()) return TPSC_E_INVALID;
+    const std::string& n = c->prof_names[i];
+    const auto& v = c->prof_acc[n];
+    if (name) *name = n.c_str();
+    if (total_ms) *total_ms = v.first;
+    if (calls) *calls = v.second;
+    return 0;
+}
+
+int tpsc_device_bytes(tpsc_ctx* c, double* bytes)
+{
+    if (!c || !bytes) return TPSC_E_INVALID;
+    *bytes = (double)c->dev_bytes;
+    return 0;
+}
+
+int tpsc_keep_chi_tau(tpsc_ctx* c, int on)
+{
+    if (!c) return TPSC_E_INVALID;
+    c->keep_chi_tau = on != 0;
+    if (on && !c->chi_tau) {
+        const size_t n = (size_t)(c->p.Nw0 / 2 + 1) * ((long)(c->p.nk0 / 2 + 1) * (c->p.nk0 / 2 + 2) / 2);
+        HIPCHK(c, hipMalloc((void**)&c->chi_tau, n * sizeof(double)));
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,442 @@
+// tpsc_device.hpp -- device building blocks of the TPSC hot path on gfx950 (CDNA4).
+//
+//  * cd            : complex double with the plain (c^2+d^2) division formula
+//  * wave_r2r_*    : FFTW REDFT00 / RODFT00 of LDS-resident rows, two real rows packed
+//                    into one complex sequence, one length-N FFT per 64-lane wavefront
+//                    (R values per lane, cross-lane radix-2 stages with xor shuffles)
+//  * block_fft     : workgroup Stockham radix-4/2 complex FFT of a tau-pencil in LDS
+//  * block_spline  : clamped cubic spline moments on a uniform grid, solved as a
+//                    causal + anti-causal first-order recursion (pole sqrt(3)-2) over
+//                    chunks with a 32-sample warm-up, plus the two boundary modes
+//  * block reductions in FP64
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tpsc {
+
+constexpr double kPi = 3.14159265358979323846;
+
+struct cd {
+    double x, y;
+};
+__host__ __device__ inline cd mk(double a, double b) { return cd{a, b}; }
+__host__ __device__ inline cd operator+(cd a, cd b) { return cd{a.x + b.x, a.y + b.y}; }
+__host__ __device__ inline cd operator-(cd a, cd b) { return cd{a.x - b.x, a.y - b.y}; }
+__host__ __device__ inline cd operator-(cd a) { return cd{-a.x, -a.y}; }
+__host__ __device__ inline cd operator*(cd a, cd b) { return cd{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+__host__ __device__ inline cd operator*(double s, cd a) { return cd{s * a.x, s * a.y}; }
+__host__ __device__ inline cd operator*(cd a, double s) { return cd{a.x * s, a.y * s}; }
+__host__ __device__ inline cd operator/(cd a, double s) { return cd{a.x / s, a.y / s}; }
+__host__ __device__ inline cd operator/(cd a, cd b)
+{
+    const double den = b.x * b.x + b.y * b.y;
+    return cd{(a.x * b.x + a.y * b.y) / den, (a.y * b.x - a.x * b.y) / den};
+}
+__host__ __device__ inline cd operator/(double s, cd b) { return cd{s, 0.0} / b; }
+__host__ __device__ inline cd conjg(cd a) { return cd{a.x, -a.y}; }
+__host__ __device__ inline double cabs2(cd a) { return sqrt(a.x * a.x + a.y * a.y); }
+__device__ inline cd cexpd(cd z)
+{
+    double s, c;
+    sincos(z.y, &s, &c);
+    const double e = exp(z.x);
+    return cd{e * c, e * s};
+}
+__device__ inline cd cis(double ang)
+{
+    double s, c;
+    sincos(ang, &s, &c);
+    return cd{c, s};
+}
+__device__ inline cd csqrtd(cd z)
+{
+    // principal branch, cut along the negative real axis
+    const double r = hypot(z.x, z.y);
+    if (r == 0.0) return cd{0.0, z.y};
+    if (z.x >= 0.0) {
+        const double t = sqrt(0.5 * (r + z.x));
+        return cd{t, z.y / (2.0 * t)};
+    }
+    const double t = sqrt(0.5 * (r - z.x));
+    return cd{fabs(z.y) / (2.0 * t), copysign(t, z.y)};
+}
+
+__device__ inline long widx(long l, long m) { return m + (l * (l + 1)) / 2; }
+__device__ inline long fold_idx(long x, long y, long N)
+{
+    if (x > N / 2) x = N - x;
+    if (y > N / 2) y = N - y;
+    if (y > x) {
+        const long t = x;
+        x = y;
+        y = t;
+    }
+    return widx(x, y);
+}
+
+// blockIdx -> logical block so that consecutive logical blocks run on the same XCD
+// (the hardware deals workgroups round-robin over 8 XCDs); neighbouring pencils then
+// share L2 lines of the tau-major arrays.
+__device__ inline int xcd_remap(int b, int nb)
+{
+    const int nx = 8;
+    const int per = (nb + nx - 1) / nx;
+    const int x = b % nx, i = b / nx;
+    const int full = nb - (per - 1) * nx;  // XCDs that get `per` blocks
+    int base = (x < full) ? x * per : full * per + (x - full) * (per - 1);
+    int lb = base + i;
+    return lb;
+}
+
+__device__ inline cd shfl_xor_cd(cd v, int mask)
+{
+    return cd{__shfl_xor(v.x, mask, 64), __shfl_xor(v.y, mask, 64)};
+}
+
+// ---------------------------------------------------------------------------------
+// in-register DFT of length R (R = 1, 2, 4, 8), forward sign, natural order output
+// ---------------------------------------------------------------------------------
+template <int R>
+__device__ inline void reg_dft(cd* v);
+template <>
+__device__ inline void reg_dft<1>(cd*) {}
+template <>
+__device__ inline void reg_dft<2>(cd* v)
+{
+    const cd a = v[0], b = v[1];
+    v[0] = a + b;
+    v[1] = a - b;
+}
+__device__ inline cd mul_mi(cd a) { return cd{a.y, -a.x}; }  // a * (-i)
+__device__ inline cd mul_pi(cd a) { return cd{-a.y, a.x}; }  // a * (+i)
+template <>
+__device__ inline void reg_dft<4>(cd* v)
+{
+    const cd a0 = v[0], a1 = v[1], a2 = v[2], a3 = v[3];
+    const cd s02 = a0 + a2, d02 = a0 - a2, s13 = a1 + a3, d13 = a1 - a3;
+    v[0] = s02 + s13;
+    v[2] = s02 - s13;
+    v[1] = d02 + mul_mi(d13);
+    v[3] = d02 - mul_mi(d13);
+}
+template <>
+__device__ inline void reg_dft<8>(cd* v)
+{
+    const double h = 0.70710678118654752440;
+    cd e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
+    reg_dft<4>(e);
+    reg_dft<4>(o);
+    const cd w1 = cd{h, -h}, w3 = cd{-h, -h};
+    o[1] = o[1] * w1;
+    o[2] = mul_mi(o[2]);
+    o[3] = o[3] * w3;
+    for (int k = 0; k < 4; k++) {
+        v[k] = e[k] + o[k];
+        v[k + 4] = e[k] - o[k];
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// REDFT00 / RODFT00 of LDS rows by one length-L complex FFT per row pair.
+//   kind 0 (REDFT00): rows of n = L/2+1 values, even extension z_j = x_{min(j,L-j)}
+//   kind 1 (RODFT00): rows of n = L/2-1 values, odd extension z_0 = z_{L/2} = 0,
+//                     z_j = x_{j-1}, z_{L-j} = -x_{j-1}
+// Row r element e lives at base[r*rs + e*es].  The DFT of a real even (odd) sequence is
+// real (imaginary), so packing rows r and r+1 as real and imaginary parts yields both
+// transforms from one FFT.  L = R*G, G = min(L,64) lanes per FFT.
+// Must be called by every thread of the block (uniform control flow); caller syncs.
+// tw: LDS table tw[t] = exp(-2 pi i t / L), t < L.
+// ---------------------------------------------------------------------------------
+template <typename Real, int R, int G>
+__device__ void wave_r2r_rows(Real* base, int nrows, int rs, int es, int kind, const cd* tw)
+{
+    constexpr int L = R * G;
+    constexpr int P = 64 / G;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int nwaves = blockDim.x >> 6;
+    const int sub = lane / G, p = lane % G;
+    const int npairs = (nrows + 1) >> 1;
+    const int half = L / 2;
+    const int rounds = (npairs + nwaves * P - 1) / (nwaves * P);
+    for (int rd = 0; rd < rounds; rd++) {
+        const int task = (rd * nwaves + wave) * P + sub;
+        const bool valid = task < npairs;
+        const int r1 = 2 * task, r2 = 2 * task + 1;
+        const bool has2 = valid && (r2 < nrows);
+        cd v[R];
+#pragma unroll
+        for (int k = 0; k < R; k++) {
+            const int j = p + G * k;
+            double a = 0.0, b = 0.0;
+            if (valid) {
+                if (kind == 0) {
+                    const int e = (j <= half) ? j : L - j;
+                    a = (double)base[(long)r1 * rs + (long)e * es];
+                    if (has2) b = (double)base[(long)r2 * rs + (long)e * es];
+                } else if (j != 0 && j != half) {
+                    const int e = (j < half) ? j - 1 : L - j - 1;
+                    const double sg = (j < half) ? 1.0 : -1.0;
+                    a = sg * (double)base[(long)r1 * rs + (long)e * es];
+                    if (has2) b = sg * (double)base[(long)r2 * rs + (long)e * es];
+                }
+            }
+            v[k] = cd{a, b};
+        }
+        reg_dft<R>(v);
+        // twiddle by exp(-2 pi i p k1 / L)
+#pragma unroll
+        for (int k1 = 1; k1 < R; k1++) v[k1] = v[k1] * tw[(p * k1) % L];
+        // G-point DIF across lanes; lane p ends with output index bitrev(p)
+#pragma unroll
+        for (int hh = G / 2; hh >= 1; hh >>= 1) {
+            const bool upper = (p & hh) != 0;
+            const cd w = tw[(p & (hh - 1)) * (L / (2 * hh))];
+#pragma unroll
+            for (int k1 = 0; k1 < R; k1++) {
+                const cd o = shfl_xor_cd(v[k1], hh);
+                v[k1] = upper ? (o - v[k1]) * w : (v[k1] + o);
+            }
+        }
+        int br = 0;
+#pragma unroll
+        for (int b = 1, q = G >> 1; b < G; b <<= 1, q >>= 1)
+            if (p & b) br |= q;
+        if (valid) {
+#pragma unroll
+            for (int k1 = 0; k1 < R; k1++) {
+                const int k = k1 + R * br;
+                if (kind == 0) {
+                    if (k <= half) {
+                        base[(long)r1 * rs + (long)k * es] = (Real)v[k1].x;
+                        if (has2) base[(long)r2 * rs + (long)k * es] = (Real)v[k1].y;
+                    }
+                } else if (k >= 1 && k <= half - 1) {
+                    base[(long)r1 * rs + (long)(k - 1) * es] = (Real)(-v[k1].y);
+                    if (has2) base[(long)r2 * rs + (long)(k - 1) * es] = (Real)v[k1].x;
+                }
+            }
+        }
+    }
+}
+
+// fill tw[t] = exp(-2 pi i t / L) for t < L (block-cooperative; caller syncs)
+__device__ inline void fill_twiddles(cd* tw, int L)
+{
+    for (int t = threadIdx.x; t < L; t += blockDim.x) {
+        double s, c;
+        sincospi(2.0 * (double)t / (double)L, &s, &c);
+        tw[t] = cd{c, -s};
+    }
+}
+
+// 2D transform of an n0 x n1 LDS slab (row-major, row stride ld): axis 1 then axis 0.
+// kind0 along axis 0 (slow index), kind1 along axis 1.  Both use FFT length L = N.
+template <typename Real, int N>
+__device__ void slab_r2r_2d(Real* slab, int n0, int n1, int ld, int kind0, int kind1, const cd* tw)
+{
+    constexpr int G = (N < 64) ? N : 64;
+    constexpr int R = N / G;
+    wave_r2r_rows<Real, R, G>(slab, n0, ld, 1, kind1, tw);  // rows (axis 1)
+    __syncthreads();
+    wave_r2r_rows<Real, R, G>(slab, n1, 1, ld, kind0, tw);  // columns (axis 0)
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------------
+// Workgroup Stockham FFT, radix 4 (+ one radix-2 pass for odd log2 L), in place in LDS.
+// sign = -1: exp(-2 pi i jk/L) (FFTW_FORWARD); sign = +1: FFTW_BACKWARD.
+// twg: global table exp(-2 pi i t / L), t < L.
+// ---------------------------------------------------------------------------------
+template <int L, int NT>
+__device__ void block_fft(cd* buf, int sign, const cd* __restrict__ twg)
+{
+    constexpr int LOG2 = __builtin_ctz(L);
+    constexpr int P4 = LOG2 / 2;
+    constexpr int NB4 = L / 4;
+    constexpr int PER4 = (NB4 + NT - 1) / NT;
+    const int tid = threadIdx.x;
+    int Ns = 1;
+    for (int pass = 0; pass < P4; pass++) {
+        cd v[PER4][4];
+#pragma unroll
+        for (int i = 0; i < PER4; i++) {
+            const int b = tid + i * NT;
+            if (b < NB4) {
+#pragma unroll
+                for (int r = 0; r < 4; r++) v[i][r] = buf[b + r * NB4];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < PER4; i++) {
+            const int b = tid + i * NT;
+            if (b < NB4) {
+                const int k = b % Ns;
+                const int stride = L / (Ns * 4);
+#pragma unroll
+                for (int r = 1; r < 4; r++) {
+                    cd w = twg[(r * k * stride) & (L - 1)];
+                    if (sign > 0) w.y = -w.y;
+                    v[i][r] = v[i][r] * w;
+                }
+                const cd a0 = v[i][0], a1 = v[i][1], a2 = v[i][2], a3 = v[i][3];
+                const cd s02 = a0 + a2, d02 = a0 - a2, s13 = a1 + a3, d13 = a1 - a3;
+                const cd rot = (sign < 0) ? mul_mi(d13) : mul_pi(d13);
+                const int dst = (b / Ns) * Ns * 4 + k;
+                buf[dst] = s02 + s13;
+                buf[dst + Ns] = d02 + rot;
+                buf[dst + 2 * Ns] = s02 - s13;
+                buf[dst + 3 * Ns] = d02 - rot;
+            }
+        }
+        __syncthreads();
+        Ns *= 4;
+    }
+    if (LOG2 & 1) {
+        constexpr int NB2 = L / 2;
+        constexpr int PER2 = (NB2 + NT - 1) / NT;
+        cd v[PER2][2];
+#pragma unroll
+        for (int i = 0; i < PER2; i++) {
+            const int b = tid + i * NT;
+            if (b < NB2) {
+                v[i][0] = buf[b];
+                v[i][1] = buf[b + NB2];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < PER2; i++) {
+            const int b = tid + i * NT;
+            if (b < NB2) {
+                const int k = b % Ns;
+                cd w = twg[(k * (L / (Ns * 2))) & (L - 1)];
+                if (sign > 0) w.y = -w.y;
+                const cd a = v[i][0], c = v[i][1] * w;
+                const int dst = (b / Ns) * Ns * 2 + k;
+                buf[dst] = a + c;
+                buf[dst + Ns] = a - c;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// Clamped cubic spline on a uniform grid x_j = j h, j = 0..N (N intervals).
+// Moments M_j = S''(x_j) solve
+//   2 M_0 + M_1 = r_0,  M_{j-1} + 4 M_j + M_{j+1} = r_j,  M_{N-1} + 2 M_N = r_N
+// with r_0 = 6/h ((y_1-y_0)/h - d0), r_j = 6/h^2 (y_{j+1} - 2 y_j + y_{j-1}),
+// r_N = 6/h (dN - (y_N-y_{N-1})/h).  The interior operator has Green's function
+// A z^{|j|}, z = sqrt(3)-2, A = 1/(2 sqrt 3); P_j = A sum_{i=1}^{N-1} z^{|j-i|} r_i
+// = A (u_j + w_j - r_j) with u (w) the causal (anti-causal) recursion s <- r + z s.
+// M_j = P_j + alpha z^j + beta z^{N-j}; alpha, beta from the two boundary rows.
+// Each thread runs the recursions over a chunk with a KW-sample warm-up
+// (|z|^33 < 2e-19, so the truncation is below FP64 rounding).
+// y: LDS input (N+1); M: LDS output (N+1); rr: LDS scratch (N+1).
+// ---------------------------------------------------------------------------------
+constexpr int KW = 33;
+
+__device__ inline void block_spline_moments(const double* y, double* M, double* rr, int N, double h, double d0,
+                                            double dN)
+{
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const double z = 1.7320508075688772935 - 2.0;   // sqrt(3) - 2
+    const double A = 0.28867513459481288225;         // 1 / (2 sqrt 3)
+    const double ih = 1.0 / h;
+    for (int j = tid; j <= N; j += nt) {
+        double r;
+        if (j == 0)
+            r = 6.0 * ih * ((y[1] - y[0]) * ih - d0);
+        else if (j == N)
+            r = 6.0 * ih * (dN - (y[N] - y[N - 1]) * ih);
+        else
+            r = 6.0 * ih * ih * (y[j + 1] - 2.0 * y[j] + y[j - 1]);
+        rr[j] = r;
+    }
+    __syncthreads();
+    // chunked causal + anti-causal recursions over interior r (r_0 = r_N = 0 here)
+    const int C = (N + 1 + nt - 1) / nt;
+    const int j0 = tid * C, j1 = min(N + 1, j0 + C);
+    double pv[20];  // C <= 20: N+1 <= 5120 with 256 threads
+    if (j0 < j1) {
+        double s = 0.0;
+        for (int i = max(1, j0 - KW); i < j0; i++) s = rr[i] + z * s;
+        for (int i = j0; i < j1; i++) {
+            const double ri = (i >= 1 && i <= N - 1) ? rr[i] : 0.0;
+            s = ri + z * s;
+            pv[i - j0] = s;
+        }
+        s = 0.0;
+        for (int i = min(N - 1, j1 - 1 + KW); i >= j1; i--) s = rr[i] + z * s;
+        for (int i = j1 - 1; i >= j0; i--) {
+            const double ri = (i >= 1 && i <= N - 1) ? rr[i] : 0.0;
+            s = ri + z * s;
+            pv[i - j0] = A * (pv[i - j0] + s - ri);
+        }
+    }
+    __syncthreads();  // all reads of rr done before M may alias nothing; write P into M
+    if (j0 < j1)
+        for (int i = j0; i < j1; i++) M[i] = pv[i - j0];
+    __syncthreads();
+    // boundary modes
+    const double R0 = rr[0] - 2.0 * M[0] - M[1];
+    const double RN = rr[N] - M[N - 1] - 2.0 * M[N];
+    double zN1 = 1.0;  // z^(N-1)
+    {
+        double zz = z;
+        int e = N - 1;
+        while (e > 0) {
+            if (e & 1) zN1 *= zz;
+            zz *= zz;
+            e >>= 1;
+        }
+    }
+    const double dg = 2.0 + z, off = zN1 * (1.0 + 2.0 * z);
+    const double det = dg * dg - off * off;
+    const double alpha = (dg * R0 - off * RN) / det;
+    const double beta = (dg * RN - off * R0) / det;
+    __syncthreads();
+    for (int j = tid; j <= N; j += nt) {
+        double add = 0.0;
+        if (j <= 2 * KW) {
+            double zp = 1.0;
+            for (int e = 0; e < j; e++) zp *= z;
+            add += alpha * zp;
+        }
+        if (N - j <= 2 * KW) {
+            double zp = 1.0;
+            for (int e = 0; e < N - j; e++) zp *= z;
+            add += beta * zp;
+        }
+        M[j] += add;
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------------
+// reductions
+// ---------------------------------------------------------------------------------
+__device__ inline double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+// block-wide sum; every thread gets the result. scratch: LDS double[>= nwaves]
+__device__ inline double block_sum(double v, double* scratch)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    v = wave_sum(v);
+    __syncthreads();
+    if (lane == 0) scratch[wave] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (int i = 0; i < nw; i++) s += scratch[i];
+    __syncthreads();
+    return s;
+}
+
+}  // namespace tpsc

@@ -1,0 +1,1398 @@
+// tpsc_kernels.hip -- FP64 HIP kernels of the TPSC hot path for gfx950.
+//
+// Stage map (reference = dbergeron1/TPSC_conductivity):
+//   k_chi0_tau0, k_chi0_slab     chi0.cpp:310-559   G0(k,+-tau) -> G(r,+-tau) -> chi0(r,tau) -> chi0(q,tau)
+//   k_chi0_pencil                chi0.cpp:566-698   clamped spline in tau + DFT -> chi0(q,iqn), tails
+//   k_sum_rule                   chi.cpp:738-830    sum-rule traces (one candidate U per launch)
+//   k_density                    green0.cpp:182-203 finite-lattice density for mu0
+//   k_vertex_slab/_pencil        chi.cpp:85-170     V(r,tau)
+//   k_sigma_bnd/_slab/_pencil    green.cpp:1743-2018 Sigma(k,iwn) moments + remainder
+//   k_trace_diffG, k_trace_selfG green.cpp:2929-3182 traces for mu and consistency checks
+//   k_hf0_slab, k_hf0_bnd,
+//   k_hVf0V_slab, k_jj_pencil,
+//   k_kx0_rows                   cond_opt.cpp:2584-4173 Lambda_jj bubble + Maki-Thompson + kx0
+// Slab kernels hold one (nk x nk) FP64 slab in LDS (133 KB at N=256) and transform it with
+// wavefront FFTs; pencil kernels hold one tau-pencil per workgroup.
+#include "tpsc_launch.hpp"
+
+namespace tpsc {
+
+#define SLAB_THREADS 1024
+#define PEN_THREADS 256
+
+// ================================================================== small helpers
+__global__ void k_fill_tw(cd* tw, int L)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < L) {
+        double s, c;
+        sincospi(2.0 * (double)t / (double)L, &s, &c);
+        tw[t] = cd{c, -s};
+    }
+}
+
+// sum n partials in a fixed order (single block)
+__global__ void k_finalize(const double* __restrict__ part, int n, double* out)
+{
+    __shared__ double sc[16];
+    double a = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) a += part[i];
+    a = block_sum(a, sc);
+    if (threadIdx.x == 0) out[0] = a;
+}
+
+// wedge (l,m) of a packed index, m <= l
+__device__ inline void wedge_lm(long idx, int& l, int& m)
+{
+    int L = (int)((sqrt(8.0 * (double)idx + 1.0) - 1.0) * 0.5);
+    while (widx(L + 1, 0) <= idx) L++;
+    while (widx(L, 0) > idx) L--;
+    l = L;
+    m = (int)(idx - widx(L, 0));
+}
+
+// write a wedge value to both symmetric positions of an nk x nk LDS slab
+__device__ inline void put_sym(double* slab, int nk, int l, int m, double v)
+{
+    slab[m + l * nk] = v;
+    slab[l + m * nk] = v;
+}
+
+// ================================================================== chi0 slabs
+// G(k,t)/N^2, chi0.cpp:468-471 (form==0) and :535-538 (form==1, tau = beta/2)
+__device__ inline double g0_plus(double x, double t, double T, double nk02, int form)
+{
+    if (x >= 0.0) return exp(-t * x) * (-1.0 / (exp(-x / T) + 1.0)) / nk02;
+    if (form == 0) return -exp((1.0 / T - t) * x) / (exp(x / T) + 1.0) / nk02;
+    return -exp((1.0 / T - t) * x) / ((exp(x / T) + 1.0) * nk02);
+}
+// G(k,-t)/N^2, chi0.cpp:486-489
+__device__ inline double g0_minus(double x, double t, double T, double nk02)
+{
+    if (x >= 0.0) return -exp((t - 1.0 / T) * x) * (-1.0 / (exp(-x / T) + 1.0)) / nk02;
+    return exp(t * x) / ((exp(x / T) + 1.0) * nk02);
+}
+
+// tau = 0 slice plus the tau-derivative data (chi0.cpp:323-447); one block.
+// tmp[0:nk8] <- G(r,0), tmp[nk8:2nk8] <- dG(r,0-), tmp[2nk8:3nk8] <- staging
+template <int N>
+__global__ void __launch_bounds__(SLAB_THREADS)
+    k_chi0_tau0(const double* __restrict__ ekw, const double* __restrict__ er, double mu0, double T, double* chi,
+                double* chirt, double* dchirt0, double* dGrt0p, double* dchiqt0, double* tmp)
+{
+    constexpr int nk = N / 2 + 1;
+    constexpr long nk8 = (long)nk * (nk + 1) / 2;
+    const double nk02 = (double)N * N;
+    extern __shared__ double smem[];
+    double* slab = smem;
+    cd* tw = (cd*)(smem + nk * nk + (nk * nk & 1));
+    fill_twiddles(tw, N);
+    double* G0r = tmp;
+    double* dGm = tmp + nk8;
+    double* stg = tmp + 2 * nk8;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    // d/dtau G(r,0+) = DCT[-ekmu Gkt / N^2]
+    for (long i = tid; i < nk8; i += nt) {
+        int l, m;
+        wedge_lm(i, l, m);
+        const double x = ekw[i] - mu0;
+        const double Gkt = -1.0 / (exp(-x / T) + 1.0);
+        put_sym(slab, nk, l, m, -x * Gkt / nk02);
+    }
+    __syncthreads();
+    slab_r2r_2d<double, N>(slab, nk, nk, nk, 0, 0, tw);
+    for (long i = tid; i < nk8; i += nt) {
+        int l, m;
+        wedge_lm(i, l, m);
+        const double v = slab[m + l * nk];
+        dGrt0p[i] = v;
+        dGm[i] = ((i == 0) ? -mu0 : er[i]) - v;
+    }
+    __syncthreads();
+    // G(r,0)
+    for (long i = tid; i < nk8; i += nt) {
+        int l, m;
+        wedge_lm(i, l, m);
+        const double x = ekw[i] - mu0;
+        put_sym(slab, nk, l, m, (-1.0 / (exp(-x / T) + 1.0)) / nk02);
+    }
+    __syncthreads();
+    slab_r2r_2d<double, N>(slab, nk, nk, nk, 0, 0, tw);
+    for (long i = tid; i < nk8; i += nt) {
+        int l, m;
+        wedge_lm(i, l, m);
+        const double g = slab[m + l * nk];
+        G0r[i] = g;
+        const double c = (i == 0) ? -2.0 * g * (g + 1.0) : -2.0 * g * g;
+        chirt[i] = c;
+    }
+    __syncthreads();
+    for (long i = tid; i < nk8; i += nt) {
+        int l, m;
+        wedge_lm(i, l, m);
+        put_sym(slab, nk, l, m, chirt[i]);
+    }
+    __syncthreads();
+    slab_r2r_2d<double, N>(slab, nk, nk, nk, 0, 0, tw);
+    for (long i = tid; i < nk8; i += nt) {
+        int l, m;
+        wedge_lm(i, l, m);
+        chi[i] = slab[m + l * nk];
+    }
+    __syncthreads();
+    // d/dtau chi0(r,0) and its transform
+    for (long i = tid; i < nk8; i += nt) {
+        const double g = G0r[i];
+        const double v = (i == 0) ? -2.0 * (dGrt0p[i] * (1.0 + g) + dGm[i] * g)
+                                  : -2.0 * (dGrt0p[i] * g + dGm[i] * g);
+        dchirt0[i] = v;
+        stg[i] = v;
+    }
+    __syncthreads();
+    for (long i = tid; i < nk8; i += nt) {
+        int l, m;
+        wedge_lm(i, l, m);
+        put_sym(slab, nk, l, m, stg[i]);
+    }
+    __syncthreads();
+    slab_r2r_2d<double, N>(slab, nk, nk, nk, 0, 0, tw);
+    for (long i = tid; i < nk8; i += nt) {
+        int l, m;
+        wedge_lm(i, l, m);
+        dchiqt0[i] = slab[m + l * nk];
+    }
+}
+
+// slices j = 1 .. nwb-1 (chi0.cpp:454-559); block b handles j = b + 1
+template <int N>
+__global__ void __launch_bounds__(SLAB_THREADS)
+    k_chi0_slab(const double* __restrict__ ekw, double mu0, double T, int Nw, double* chi, double* chirt)
+{
+    constexpr int nk = N / 2 + 1;
+    constexpr long nk8 = (long)nk * (nk + 1) / 2;
+    const double nk02 = (double)N * N;
+    extern __shared__ double smem[];
+    double* slab = smem;
+    cd* tw = (cd*)(smem + nk * nk + (nk * nk & 1));
+    fill_twiddles(tw, N);
+    const int nwb = Nw / 2 + 1;
+    const int j = blockIdx.x + 1;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    double* cr = chirt + (long)j * nk8;
+    double* cq = chi + (long)j * nk8;
+    if (j < nwb - 1) {
+        const double t = j / (Nw * T);
+        for (long i = tid; i < nk8; i += nt) {
+            int l, m;
+            wedge_lm(i, l, m);
+            put_sym(slab, nk, l, m, g0_plus(ekw[i] - mu0, t, T, nk02, 0));
+        }
+        __syncthreads();
+        slab_r2r_2d<double, N>(slab, nk, nk, nk, 0, 0, tw);
+        for (long i = tid; i < nk8; i += nt) {  // stage G(r,t) in chirt[j]
+            int l, m;
+            wedge_lm(i, l, m);
+            cr[i] = slab[m + l * nk];
+        }
+        __syncthreads();
+        for (long i = tid; i < nk8; i += nt) {
+            int l, m;
+            wedge_lm(i, l, m);
+            put_sym(slab, nk, l, m, g0_minus(ekw[i] - mu0, t, T, nk02));
+        }
+        __syncthreads();
+        slab_r2r_2d<double, N>(slab, nk, nk, nk, 0, 0, tw);
+        for (long i = tid; i < nk8; i += nt) {  // chi0(r,t) = -2 G(r,-t) G(r,t)
+            int l, m;
+            wedge_lm(i, l, m);
+            cr[i] = -2.0 * slab[m + l * nk] * cr[i];
+        }
+        __syncthreads();
+        for (long i = tid; i < nk8; i += nt) {
+            int l, m;
+            wedge_lm(i, l, m);
+            put_sym(slab, nk, l, m, cr[i]);
+        }
+        __syncthreads();
+    } else {
+        const double t = 1.0 / (2.0 * T);
+        for (long i = tid; i < nk8; i += nt) {
+            int l, m;
+            wedge_lm(i, l, m);
+            put_sym(slab, nk, l, m, g0_plus(ekw[i] - mu0, t, T, nk02, 1));
+        }
+        __syncthreads();
+        slab_r2r_2d<double, N>(slab, nk, nk, nk, 0, 0, tw);
+        for (int s = tid; s < nk * nk; s += nt) slab[s] = 2.0 * slab[s] * slab[s];
+        __syncthreads();
+        for (long i = tid; i < nk8; i += nt) {
+            int l, m;
+            wedge_lm(i, l, m);
+            cr[i] = slab[l + m * nk];
+        }
+        __syncthreads();
+    }
+    slab_r2r_2d<double, N>(slab, nk, nk, nk, 0, 0, tw);
+    for (long i = tid; i < nk8; i += nt) {
+        int l, m;
+        wedge_lm(i, l, m);
+        cq[i] = slab[m + l * nk];
+    }
+}
+
+// ================================================================== chi0 pencils
+// chi0.cpp:586-698: per wedge q, clamped spline of chi0(q,tau_j), j=0..Nw/2, slopes
+// (dchiqt0[q], 0); F = BACKWARD DFT_Nw of the antisymmetrised 6a_j; tails.
+template <int L>
+__global__ void __launch_bounds__(PEN_THREADS)
+    k_chi0_pencil(double* chi, const double* __restrict__ dchiqt0, double* chi0inf, double* chi0inf2, long nk8,
+                  double T, const cd* __restrict__ tw)
+{
+    constexpr int NS = L / 2;  // intervals
+    __shared__ double y[NS + 1], M[NS + 1];
+    __shared__ cd X[L];
+    __shared__ double sc[16];
+    const long q = xcd_remap(blockIdx.x, gridDim.x);
+    if (q >= nk8) return;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    double* rr = (double*)X;  // spline scratch aliases the FFT buffer
+    for (int j = tid; j <= NS; j += nt) y[j] = chi[q + (long)j * nk8];
+    const double d0 = dchiqt0[q];
+    const double h = 1.0 / (L * T);
+    __syncthreads();
+    block_spline_moments(y, M, rr, NS, h, d0, 0.0);
+    double integ = 0.0;
+    for (int j = tid; j < NS; j += nt) {
+        const double a6 = (M[j + 1] - M[j]) / h;
+        X[j] = cd{a6, 0.0};
+        X[L - 1 - j] = cd{-a6, 0.0};
+        integ += h * (y[j] + y[j + 1]) * 0.5 - h * h * h * (M[j] + M[j + 1]) / 24.0;
+    }
+    integ = block_sum(integ, sc);
+    block_fft<L, PEN_THREADS>(X, +1, tw);
+    if (tid == 0) chi[q] = 2 * integ;
+    if (q != 0) {
+        const double inf = -2.0 * d0;
+        for (int j = 1 + tid; j <= NS; j += nt) {
+            const double wn = 2 * j * kPi * T;
+            const double ang = 2 * j * kPi / L;
+            chi[q + (long)j * nk8] =
+                inf / (wn * wn) + ((1.0 - cos(ang)) * X[j].x + sin(ang) * X[j].y) / (wn * wn * wn * wn);
+        }
+        if (tid == 0) {
+            chi0inf[q] = inf;
+            chi0inf2[q] = 2.0 * X[NS].x;
+        }
+    } else {
+        for (int j = 1 + tid; j <= NS; j += nt) chi[(long)j * nk8] = 0.0;
+        if (tid == 0) {
+            chi0inf[0] = 0.0;
+            chi0inf2[0] = 0.0;
+        }
+    }
+}
+
+// chi0max and its wedge position (chi0.cpp:703-717): largest value, ties to the
+// larger packed index (the reference scans l, m downwards with a strict >).
+__global__ void k_chi0max(const double* __restrict__ chi, long nk8, double* out, int* imax)
+{
+    __shared__ double sv[1024];
+    __shared__ long si[1024];
+    double best = 0.0;
+    long bi = -1;
+    for (long i = threadIdx.x; i < nk8; i += blockDim.x) {
+        const double v = chi[i];
+        if (v > best || (v == best && bi >= 0 && i > bi) ) {
+            if (v > 0.0 || bi >= 0) {
+                best = v;
+                bi = i;
+            }
+        }
+    }
+    sv[threadIdx.x] = best;
+    si[threadIdx.x] = bi;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            const double v2 = sv[threadIdx.x + s];
+            const long i2 = si[threadIdx.x + s];
+            if (i2 >= 0 && (v2 > sv[threadIdx.x] || (v2 == sv[threadIdx.x] && i2 > si[threadIdx.x]) ||
+                            si[threadIdx.x] < 0)) {
+                sv[threadIdx.x] = v2;
+                si[threadIdx.x] = i2;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = (si[0] >= 0) ? sv[0] : 0.0;
+        imax[0] = (int)((si[0] >= 0) ? si[0] : 0);
+    }
+}
+
+// ================================================================== reductions
+// sum-rule trace sum_{n=0}^{Nw} sum_q (w T/Nq) [chi/(1 + sU chi/2) - chi]  (chi.cpp:738-830)
+// sU = -Usp for the spin channel, +Uch for the charge channel.
+__global__ void k_sum_rule(const double* __restrict__ chi, const double* __restrict__ inf1,
+                           const double* __restrict__ inf2, const double* __restrict__ wq, long nk8, int nwmax,
+                           double T, double Nq, double sU, double* part)
+{
+    __shared__ double sc[16];
+    const long nterm = (long)(2 * nwmax + 1) * nk8;
+    double acc = 0.0;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < nterm; i += (long)gridDim.x * blockDim.x) {
+        const long j = i / nk8, q = i - j * nk8;
+        double w = 2.0 * wq[q];
+        if (j == 0) w *= 0.5;
+        double c;
+        if (j <= nwmax)
+            c = chi[i];
+        else {
+            const double wn = 2 * j * kPi * T;
+            c = inf1[q] / (wn * wn) + inf2[q] / (wn * wn * wn * wn);
+        }
+        acc += w * T / Nq * (c / (1 + sU * c / 2.0) - c);
+    }
+    acc = block_sum(acc, sc);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+// density for the mu0 root (green0.cpp:182-203), one block
+__global__ void k_density(const double* __restrict__ ekw, const double* __restrict__ wq, long nk8, double Nk2,
+                          double T, double mup, double dens, double* out)
+{
+    __shared__ double sc[16];
+    double acc = 0.0;
+    for (long i = threadIdx.x; i < nk8; i += blockDim.x) acc += 2 * wq[i] / ((exp((ekw[i] - mup) / T) + 1.0) * Nk2);
+    acc = block_sum(acc, sc);
+    if (threadIdx.x == 0) out[0] = acc - dens;
+}
+
+// ================================================================== vertex V(r,tau)
+// chi.cpp:85-119, block j = iqn index
+template <int N>
+__global__ void __launch_bounds__(SLAB_THREADS)
+    k_vertex_slab(const double* __restrict__ chi, double* V, double normFact, double U, double Usp, double Uch)
+{
+    constexpr int nk = N / 2 + 1;
+    constexpr long nk8 = (long)nk * (nk + 1) / 2;
+    extern __shared__ double smem[];
+    double* slab = smem;
+    cd* tw = (cd*)(smem + nk * nk + (nk * nk & 1));
+    fill_twiddles(tw, N);
+    const long j = blockIdx.x;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (long i = tid; i < nk8; i += nt) {
+        int l, m;
+        wedge_lm(i, l, m);
+        const double c = chi[i + j * nk8];
+        const double csp = c / (1.0 - Usp * c / 2.0), cch = c / (1.0 + Uch * c / 2.0);
+        put_sym(slab, nk, l, m, normFact * U * (3.0 * Usp * (csp - c) + Uch * (cch - c)) / 8.0);
+    }
+    __syncthreads();
+    slab_r2r_2d<double, N>(slab, nk, nk, nk, 0, 0, tw);
+    for (long i = tid; i < nk8; i += nt) {
+        int l, m;
+        wedge_lm(i, l, m);
+        V[i + j * nk8] = slab[m + l * nk];
+    }
+}
+
+// chi.cpp:123-170: REDFT00 over n (length Nw/2+1) for two wedge points per block, as one
+// complex FFT of length Nw of the even extensions; then V += fact * chi0(r,tau)
+template <int L>
+__global__ void __launch_bounds__(PEN_THREADS)
+    k_vertex_pencil(double* V, const double* __restrict__ chirt, long nk8, double fact, const cd* __restrict__ tw)
+{
+    constexpr int H = L / 2;
+    __shared__ cd X[L];
+    const long q = 2 * (long)xcd_remap(blockIdx.x, gridDim.x);
+    if (q >= nk8) return;
+    const bool two = (q + 1) < nk8;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int j = tid; j <= H; j += nt) {
+        const double a = V[q + (long)j * nk8];
+        const double b = two ? V[q + 1 + (long)j * nk8] : 0.0;
+        X[j] = cd{a, b};
+        if (j > 0 && j < H) X[L - j] = cd{a, b};
+    }
+    __syncthreads();
+    block_fft<L, PEN_THREADS>(X, -1, tw);
+    for (int j = tid; j <= H; j += nt) {
+        const long o = q + (long)j * nk8;
+        V[o] = X[j].x + fact * chirt[o];
+        if (two) V[o + 1] = X[j].y + fact * chirt[o + 1];
+    }
+}
+
+// ================================================================== self-energy
+// boundary slopes of Sigma(k,tau) at 0 and beta (green.cpp:1743-1817); one block.
+// tmp used as staging [0:nk8).
+template <int N>
+__global__ void __launch_bounds__(SLAB_THREADS)
+    k_sigma_bnd(const double* __restrict__ ek, const double* __restrict__ er, const double* __restrict__ V,
+                const double* __restrict__ dchirt0, const double* __restrict__ dGrt0p, double mu0, double T,
+                double fact, double* dSig, double* tmp)
+{
+    constexpr int nk = N / 2 + 1;
+    constexpr long nk8 = (long)nk * (nk + 1) / 2;
+    const double nk02 = (double)N * N;
+    extern __shared__ double smem[];
+    double* slab = smem;
+    cd* tw = (cd*)(smem + nk * nk + (nk * nk & 1));
+    fill_twiddles(tw, N);
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int side = 0; side < 2; side++) {
+        for (int s = tid; s < nk * nk; s += nt) {
+            const double x = ek[s] - mu0;
+            slab[s] = (side == 0) ? -1.0 / (exp(-x / T) + 1.0) / nk02 : -1.0 / (exp(x / T) + 1.0) / nk02;
+        }
+        __syncthreads();
+        slab_r2r_2d<double, N>(slab, nk, nk, nk, 0, 0, tw);
+        for (long i = tid; i < nk8; i += nt) {
+            int l, m;
+            wedge_lm(i, l, m);
+            const double g = slab[m + l * nk];
+            if (side == 0)
+                tmp[i] = fact * dchirt0[i] * g + V[i] * dGrt0p[i];
+            else {
+                const double xi = (i == 0) ? -mu0 : er[i];
+                tmp[i] = -fact * dchirt0[i] * g + V[i] * (-dGrt0p[i] + xi);
+            }
+        }
+        __syncthreads();
+        for (long i = tid; i < nk8; i += nt) {
+            int l, m;
+            wedge_lm(i, l, m);
+            put_sym(slab, nk, l, m, tmp[i]);
+        }
+        __syncthreads();
+        slab_r2r_2d<double, N>(slab, nk, nk, nk, 0, 0, tw);
+        for (long i = tid; i < nk8; i += nt) {
+            int l, m;
+            wedge_lm(i, l, m);
+            dSig[i + side * nk8] = slab[m + l * nk];
+        }
+        __syncthreads();
+    }
+}
+
+// Sigma(k,tau_j), j = 0..Nw (green.cpp:1822-1892): G0(k,tau) -> r, x V(r,|tau|), -> k
+template <int N>
+__global__ void __launch_bounds__(SLAB_THREADS)
+    k_sigma_slab(const double* __restrict__ ek, const double* __restrict__ V, double mu0, double T, int Nw,
+                 double* Stau)
+{
+    constexpr int nk = N / 2 + 1;
+    constexpr long nk8 = (long)nk * (nk + 1) / 2;
+    const double nk02 = (double)N * N;
+    extern __shared__ double smem[];
+    double* slab = smem;
+    cd* tw = (cd*)(smem + nk * nk + (nk * nk & 1));
+    fill_twiddles(tw, N);
+    const int j = blockIdx.x;
+    const int nwb = Nw / 2 + 1;
+    const int jv = (j < nwb) ? j : Nw - j;
+    const double t = j / (Nw * T);
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int s = tid; s < nk * nk; s += nt) {
+        const double x = ek[s] - mu0;
+        slab[s] = (x > 0) ? -exp(-t * x) / (exp(-x / T) + 1.0) / nk02
+                          : -exp((1.0 / T - t) * x) / (exp(x / T) + 1.0) / nk02;
+    }
+    __syncthreads();
+    slab_r2r_2d<double, N>(slab, nk, nk, nk, 0, 0, tw);
+    const double* Vj = V + (long)jv * nk8;
+    for (long i = tid; i < nk8; i += nt) {
+        int l, m;
+        wedge_lm(i, l, m);
+        const double v = slab[m + l * nk] * Vj[i];
+        put_sym(slab, nk, l, m, v);
+    }
+    __syncthreads();
+    slab_r2r_2d<double, N>(slab, nk, nk, nk, 0, 0, tw);
+    double* o = Stau + (long)j * nk8;
+    for (long i = tid; i < nk8; i += nt) {
+        int l, m;
+        wedge_lm(i, l, m);
+        o[i] = slab[m + l * nk];
+    }
+}
+
+// green.cpp:1925-2018: per wedge k, spline of Sigma(k,tau_j) (j=0..Nw) with slopes dSig,
+// F = BACKWARD DFT[exp(i pi j/Nw) 6a_j], R(k,n) = (1 - exp(i pi (2n+1)/Nw)) F_n
+template <int L>
+__global__ void __launch_bounds__(PEN_THREADS)
+    k_sigma_pencil(const double* __restrict__ Stau, const double* __restrict__ dSig, long nk8, double T, cd* Self,
+                   double* Sinf2, double* Sinf3, double* Sinf4, double* Sinf, const cd* __restrict__ tw)
+{
+    __shared__ double y[L + 1], M[L + 1];
+    __shared__ cd X[L];
+    const long k = xcd_remap(blockIdx.x, gridDim.x);
+    if (k >= nk8) return;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    double* rr = (double*)X;
+    for (int j = tid; j <= L; j += nt) y[j] = Stau[k + (long)j * nk8];
+    const double d0 = dSig[k], dN = dSig[k + nk8];
+    const double h = 1.0 / (L * T);
+    __syncthreads();
+    block_spline_moments(y, M, rr, L, h, d0, dN);
+    for (int j = tid; j < L; j += nt) {
+        const double a6 = (M[j + 1] - M[j]) / h;
+        const cd ph = cis(j * kPi / L);
+        X[j] = ph * a6;
+    }
+    __syncthreads();
+    block_fft<L, PEN_THREADS>(X, +1, tw);
+    for (int n = tid; n < L / 2; n += nt) {
+        const cd e = cis((2 * n + 1) * kPi / L);
+        const cd R = cd{1.0 - e.x, -e.y} * X[n];
+        Self[k + (long)n * nk8] = R;
+        if (n == L / 2 - 1) Sinf4[k] = R.x;
+    }
+    if (tid == 0) {
+        Sinf2[k] = d0 + dN;
+        Sinf3[k] = -(M[0] + M[L]);
+        if (k == 0) Sinf[0] = -(y[0] + y[L]);
+    }
+}
+
+// ================================================================== traces
+__device__ inline cd sigma_mom(double Sinf, double s2, double s3, cd R, double wn)
+{
+    // Sinf/z + s2/z^2 + s3/z^3 + R/z^4, z = i wn
+    const double w2 = wn * wn, w3 = w2 * wn, w4 = w2 * w2;
+    return cd{-s2 / w2 + R.x / w4, -Sinf / wn + s3 / w3 + R.y / w4};
+}
+
+// traceDiffG(pmu) partials (green.cpp:3108-3182)
+__global__ void k_trace_diffG(const double* __restrict__ ekw, const double* __restrict__ wq,
+                              const cd* __restrict__ Self, const double* __restrict__ Sinf2,
+                              const double* __restrict__ Sinf3, const double* __restrict__ Sinf4,
+                              const double* __restrict__ Sinf, long nk8, int Nw, double T, double mu0, double pmu,
+                              double* part)
+{
+    __shared__ double sc[16];
+    const double Si = Sinf[0];
+    const long nterm = (long)(Nw + 1) * nk8;
+    double acc = 0.0;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < nterm; i += (long)gridDim.x * blockDim.x) {
+        const long j = i / nk8, q = i - j * nk8;
+        const double e = ekw[q], w = 2.0 * wq[q];
+        const double wn = (2 * j + 1) * kPi * T;
+        double rS, iS;
+        if (j >= Nw / 2) {
+            rS = -Sinf2[q] / (wn * wn) + Sinf4[q] / (wn * wn * wn * wn);
+            iS = -Si / wn + Sinf3[q] / (wn * wn * wn);
+        } else {
+            const cd s = sigma_mom(Si, Sinf2[q], Sinf3[q], Self[i], wn);
+            rS = s.x;
+            iS = s.y;
+        }
+        const double tmp = w / (((wn - iS) * (wn - iS) + (e - pmu + rS) * (e - pmu + rS)) * (wn * wn + (e - mu0) * (e - mu0)));
+        acc += tmp * ((mu0 - pmu + rS) * (e - pmu + rS) * (e - mu0) + iS * (iS - 2 * wn) * (e - mu0) -
+                      wn * wn * (mu0 - pmu + rS));
+    }
+    acc = block_sum(acc, sc);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+// which = 0: Tr Sigma G0 main sum (green.cpp:3018-3051); 1: its Sigma_inf correction
+// (:3053-3084, uses j < Nw/2 terms plus one per k); 2: Tr Sigma G (green.cpp:2944-2976)
+__global__ void k_trace_selfG(const double* __restrict__ ekw, const double* __restrict__ wq,
+                              const cd* __restrict__ Self, const double* __restrict__ Sinf2,
+                              const double* __restrict__ Sinf3, const double* __restrict__ Sinf, long nk8, int Nw,
+                              double T, double mu0, double mu, double nk02, int which, double* part)
+{
+    __shared__ double sc[16];
+    const double Si = Sinf[0];
+    const long nterm = (long)(Nw / 2) * nk8;
+    double acc = 0.0;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < nterm; i += (long)gridDim.x * blockDim.x) {
+        const long j = i / nk8, q = i - j * nk8;
+        const double e = ekw[q], w = wq[q];
+        const double wn = (2 * j + 1) * kPi * T;
+        if (which == 1) {
+            acc += w * Si * T * 2 / (nk02 * (wn * wn + (e - mu0) * (e - mu0)));
+            if (j == 0) {
+                if (fabs(e - mu0) > 1.0e-12)
+                    acc += w * Si * (1.0 / (exp((e - mu0) / T) + 1.0) - 0.5) / (e - mu0) / nk02;
+                else {
+                    const double ex = (e - mu0) < 0 ? exp((e - mu0) / T) : exp(-(e - mu0) / T);
+                    acc -= w * Si * ex / ((ex + 1.0) * (ex + 1.0) * nk02 * T);
+                }
+            }
+            continue;
+        }
+        const cd s = sigma_mom(Si, Sinf2[q], Sinf3[q], Self[i], wn);
+        if (which == 0) {
+            const double tmp = w * T * 2 / (nk02 * (wn * wn + (e - mu0) * (e - mu0)));
+            acc += tmp * (s.x * (mu0 - e) + s.y * wn);
+        } else {
+            const double tmp = w * T * 2 / (nk02 * ((wn - s.y) * (wn - s.y) + (e - mu + s.x) * (e - mu + s.x)));
+            acc += tmp * (s.y * wn - s.y * s.y - (e - mu) * s.x - s.x * s.x);
+        }
+    }
+    acc = block_sum(acc, sc);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+// ================================================================== Lambda_jj slabs
+// h(r,tau_j) and f0(r,tau_j), j = 0..Nw (cond_opt.cpp:2697-2735, 2795-2827):
+// RODFT00 along kx (index l+1), REDFT00 along ky.
+template <int N>
+__global__ void __launch_bounds__(SLAB_THREADS)
+    k_hf0_slab(const double* __restrict__ ek, const double* __restrict__ dek, double mu0, double T, int Nw,
+               double* h, double* f0)
+{
+    constexpr int nk = N / 2 + 1;
+    constexpr long nq = (long)nk * (nk - 2);
+    const double nf = 1.0 / ((double)N * N);
+    extern __shared__ double smem[];
+    double* slab = smem;
+    cd* tw = (cd*)(smem + nq + (nq & 1));
+    fill_twiddles(tw, N);
+    const int j = blockIdx.x;
+    const double t = (double)j / (T * Nw);
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int which = 0; which < 2; which++) {
+        for (long s = tid; s < nq; s += nt) {
+            const int l = (int)(s / nk), m = (int)(s % nk);
+            const double e = ek[m + (l + 1) * nk], v = dek[s];
+            double val;
+            if (which == 0) {
+                const double x = e - mu0;
+                val = (x > 0) ? nf * v * exp(-x * t) / (exp(-x / T) + 1.0)
+                              : nf * v * exp(x * (1.0 / T - t)) / (exp(x / T) + 1.0);
+            } else if (e > mu0) {
+                const double ex = exp((mu0 - e) / T);
+                val = nf * v * exp((mu0 - e) * t) * (t / (ex + 1.0) - ex / (T * (ex + 1.0) * (ex + 1.0)));
+            } else {
+                const double ex = exp((e - mu0) / T);
+                val = nf * v * exp((e - mu0) * (1.0 / T - t)) * (t / (ex + 1.0) - 1.0 / (T * (ex + 1.0) * (ex + 1.0)));
+            }
+            slab[s] = val;
+        }
+        __syncthreads();
+        slab_r2r_2d<double, N>(slab, nk - 2, nk, nk, 1, 0, tw);
+        double* o = (which == 0 ? h : f0) + (long)j * nq;
+        for (long s = tid; s < nq; s += nt) o[s] = slab[s];
+        __syncthreads();
+    }
+}
+
+// dh, df0 at tau = 0, beta and the boundary data dhV, df0V (cond_opt.cpp:2745-3014).
+// Block b: which = b/2 (0: h, 1: f0), side = b%2 (0: tau=0, 1: tau=beta). Must run
+// before k_hVf0V_slab overwrites h and f0.
+template <int N>
+__global__ void __launch_bounds__(SLAB_THREADS)
+    k_hf0_bnd(const double* __restrict__ ek, const double* __restrict__ dek, const double* __restrict__ V,
+              const double* __restrict__ dchirt0, const double* __restrict__ h, const double* __restrict__ f0,
+              double mu0, double T, int Nw, double fact4, double* dhV, double* df0V)
+{
+    constexpr int nk = N / 2 + 1;
+    constexpr long nq = (long)nk * (nk - 2);
+    const double nf = 1.0 / ((double)N * N);
+    extern __shared__ double smem[];
+    double* slab = smem;
+    cd* tw = (cd*)(smem + nq + (nq & 1));
+    fill_twiddles(tw, N);
+    const int which = blockIdx.x >> 1, side = blockIdx.x & 1;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (long s = tid; s < nq; s += nt) {
+        const int l = (int)(s / nk), m = (int)(s % nk);
+        const double e = ek[m + (l + 1) * nk], v = dek[s];
+        double val;
+        if (which == 0) {
+            if (e > mu0) {
+                const double ex = exp((mu0 - e) / T);
+                val = (side == 0) ? -nf * v * (e - mu0) / (ex + 1.0) : -nf * v * (e - mu0) * ex / (ex + 1.0);
+            } else {
+                const double ex = exp((e - mu0) / T);
+                val = (side == 0) ? -nf * v * (e - mu0) * ex / (ex + 1.0) : -nf * v * (e - mu0) / (ex + 1.0);
+            }
+        } else {
+            if (e > mu0) {
+                const double ex = exp((mu0 - e) / T);
+                val = (side == 0)
+                          ? nf * v * ((e - mu0) * ex / (T * (ex + 1.0) * (ex + 1.0)) + 1.0 / (ex + 1.0))
+                          : nf * v * ex * ((mu0 - e) * (1.0 / (T * (ex + 1.0)) - ex / (T * (ex + 1.0) * (ex + 1.0))) + 1.0 / (ex + 1.0));
+            } else {
+                const double ex = exp((e - mu0) / T);
+                val = (side == 0)
+                          ? nf * v * ex * ((e - mu0) / (T * (ex + 1.0) * (ex + 1.0)) + 1.0 / (ex + 1.0))
+                          : nf * v * ((mu0 - e) * (1.0 / (T * (ex + 1.0)) - 1.0 / (T * (ex + 1.0) * (ex + 1.0))) + 1.0 / (ex + 1.0));
+            }
+        }
+        slab[s] = val;
+    }
+    __syncthreads();
+    slab_r2r_2d<double, N>(slab, nk - 2, nk, nk, 1, 0, tw);
+    const double* src = (which == 0) ? h : f0;
+    const double* srcj = src + (side == 0 ? 0 : (long)Nw * nq);
+    for (long s = tid; s < nq; s += nt) {
+        const int l = (int)(s / nk), m = (int)(s % nk);
+        const long r = fold_idx(l + 1, m, N);
+        const double d = slab[s];
+        slab[s] = (side == 0) ? 2.0 * d * V[r] + srcj[s] * fact4 * dchirt0[r]
+                              : 2.0 * d * V[r] - srcj[s] * fact4 * dchirt0[r];
+    }
+    __syncthreads();
+    slab_r2r_2d<double, N>(slab, nk - 2, nk, nk, 1, 0, tw);
+    double* o = ((which == 0) ? dhV : df0V) + side * nq;
+    for (long s = tid; s < nq; s += nt) o[s] = slab[s];
+}
+
+// hV(k,tau) = DST x DCT [2 h(r,tau) V(r,|tau|)], same for f0, in place (cond_opt.cpp:3029-3114)
+template <int N>
+__global__ void __launch_bounds__(SLAB_THREADS)
+    k_hVf0V_slab(const double* __restrict__ V, int Nw, double* h, double* f0)
+{
+    constexpr int nk = N / 2 + 1;
+    constexpr long nq = (long)nk * (nk - 2);
+    constexpr long nk8 = (long)nk * (nk + 1) / 2;
+    extern __shared__ double smem[];
+    double* slab = smem;
+    cd* tw = (cd*)(smem + nq + (nq & 1));
+    fill_twiddles(tw, N);
+    const int j = blockIdx.x;
+    const int jv = (j > Nw / 2) ? Nw - j : j;
+    const double* Vj = V + (long)jv * nk8;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int which = 0; which < 2; which++) {
+        double* a = ((which == 0) ? h : f0) + (long)j * nq;
+        for (long s = tid; s < nq; s += nt) {
+            const int l = (int)(s / nk), m = (int)(s % nk);
+            slab[s] = 2.0 * a[s] * Vj[fold_idx(l + 1, m, N)];
+        }
+        __syncthreads();
+        slab_r2r_2d<double, N>(slab, nk - 2, nk, nk, 1, 0, tw);
+        for (long s = tid; s < nq; s += nt) a[s] = slab[s];
+        __syncthreads();
+    }
+}
+
+// ================================================================== Lambda_jj pencils
+struct Poles {
+    cd r1, r2, a1, a2, b1, b2, den1, den2;
+};
+
+// roots of z^2 + (mu - e) z - Sigma_inf with Newton polish (generique.cpp:1666-1726)
+__device__ inline void roots2(cd A, cd B, cd Cc, cd* roots)
+{
+    if (!(cabs2(A) > 1.0e-15)) return;
+    const cd Rr = csqrtd(B * B - 4.0 * A * Cc);
+    roots[0] = (-B - Rr) / (2.0 * A);
+    roots[1] = (-B + Rr) / (2.0 * A);
+    for (int j = 0; j < 2; j++) {
+        cd x = roots[j];
+        cd yv = A * x * x + B * x + Cc;
+        cd p = 2.0 * A * x + B;
+        const cd y0 = yv;
+        int it = 0;
+        while (cabs2(p) != 0.0 && it < 10 && cabs2(yv) > 1.0e-16) {
+            const cd x2 = x, y2 = yv;
+            const cd cc = yv - p * x;
+            x = (-cc) / p;
+            yv = A * x * x + B * x + Cc;
+            p = 2.0 * A * x + B;
+            if (cabs2(y2) < cabs2(yv)) {
+                x = 0.5 * (x + x2);
+                yv = A * x * x + B * x + Cc;
+                p = 2.0 * A * x + B;
+            }
+            it++;
+        }
+        if (cabs2(yv) < cabs2(y0)) roots[j] = x;
+    }
+}
+
+__device__ inline Poles make_poles(double e, double mu, double Sinf, double CJ1, double T)
+{
+    Poles P;
+    cd roots[2] = {cd{0, 0}, cd{0, 0}};
+    roots2(cd{1.0, 0.0}, cd{-e + mu, 0.0}, cd{-Sinf, 0.0}, roots);
+    P.r1 = roots[0];
+    P.r2 = roots[1];
+    P.a1 = P.r1 / (P.r1 - P.r2);
+    P.a2 = P.r2 / (P.r2 - P.r1);
+    P.b1 = CJ1 / (P.r1 - P.r2);
+    P.b2 = -P.b1;
+    P.den1 = (P.r1.x > 0 ? cexpd(-(P.r1 / T)) : cexpd(P.r1 / T)) + cd{1.0, 0.0};
+    P.den2 = (P.r2.x > 0 ? cexpd(-(P.r2 / T)) : cexpd(P.r2 / T)) + cd{1.0, 0.0};
+    return P;
+}
+
+// Dyson G at Matsubara slot s (n = s - Nw/2), with the conjugate Sigma for n < 0
+// (cond_opt.cpp:3354-3407)
+__device__ inline cd dyson_G(const cd* __restrict__ Self, long ki, long nk8, int s, int H, double C2, double C3,
+                             double Sinf, double e, double mu, double T, double& kn)
+{
+    kn = (2.0 * (s - H) + 1) * kPi * T;
+    cd R;
+    double sgn;
+    if (s < H) {
+        R = Self[ki + (long)(H - s - 1) * nk8];
+        sgn = -1.0;
+    } else {
+        R = Self[ki + (long)(s - H) * nk8];
+        sgn = 1.0;
+    }
+    const double k2 = kn * kn, k3 = k2 * kn, k4 = k2 * k2;
+    const cd sg = cd{-C2 / k2 + R.x / k4, -Sinf / kn + C3 / k3 + sgn * R.y / k4};
+    const cd den = cd{0.0 - e, kn} + cd{mu, 0.0} - sg;
+    return T / den;
+}
+
+// J(ikn) at slot s from the spline transform F of hV or f0V (cond_opt.cpp:3302-3311)
+__device__ inline cd j_slot(const cd* X, int s, int L, double T, double cj1, double cj2, double cj3)
+{
+    const int H = L / 2;
+    int j;
+    double kn, ang;
+    if (s >= H) {
+        j = s - H;
+        kn = (2 * j + 1) * kPi * T;
+        ang = (2 * j + 1) * kPi / L;
+    } else {
+        j = s + H;
+        kn = (2 * (j - L) + 1) * kPi * T;
+        ang = (2 * (j - L) + 1) * kPi / L;
+    }
+    const cd e = cis(ang);
+    const cd a = cd{1.0 - e.x, -e.y} * X[j];
+    const double k2 = kn * kn, k3 = k2 * kn, k4 = k2 * k2;
+    // -i cj1/kn - cj2/kn^2 + i cj3/kn^3 + a/kn^4
+    return cd{-cj2 / k2 + a.x / k4, -cj1 / kn + cj3 / k3 + a.y / k4};
+}
+
+// spline of a tau-pencil held in y (N intervals, step h) and twiddled DFT of 6a_j into X
+template <int L>
+__device__ inline void spline_to_X(double* y, double* M, cd* X, int NSint, double h, double d0, double dN,
+                                   int mode)
+{
+    // mode 0: X_j = exp(i pi j/L) 6a_j (j < L); mode 1: X_j = 6a_j (j < L);
+    // mode 2: X_j = 6a_j for j < L/2, X_{L-1-j} = -X_j
+    double* rr = (double*)X;
+    block_spline_moments(y, M, rr, NSint, h, d0, dN);
+    const int tid = threadIdx.x, nt = blockDim.x;
+    if (mode == 2) {
+        for (int j = tid; j < L / 2; j += nt) {
+            const double a6 = (M[j + 1] - M[j]) / h;
+            X[j] = cd{a6, 0.0};
+            X[L - 1 - j] = cd{-a6, 0.0};
+        }
+    } else {
+        for (int j = tid; j < L; j += nt) {
+            const double a6 = (M[j + 1] - M[j]) / h;
+            X[j] = (mode == 0) ? cis(j * kPi / L) * a6 : cd{a6, 0.0};
+        }
+    }
+    __syncthreads();
+}
+
+__device__ inline cd block_sum_cd(cd v, double* sc)
+{
+    const double a = block_sum(v.x, sc);
+    const double b = block_sum(v.y, sc);
+    return cd{a, b};
+}
+
+// per-k Lambda_jj contributions (cond_opt.cpp:3179-3822).  Each workgroup owns kJJK
+// consecutive Lambda-grid points and accumulates into its own partial rows
+// part[blk*2*nwb + n] (bubble) and part[blk*2*nwb + nwb + n] (MT).
+template <int L>
+__global__ void __launch_bounds__(PEN_THREADS)
+    k_jj_pencil(const double* __restrict__ ek, const double* __restrict__ dek, const double* __restrict__ d2ek,
+                const double* __restrict__ h, const double* __restrict__ f0, const double* __restrict__ dhV,
+                const double* __restrict__ df0V, const cd* __restrict__ Self, const double* __restrict__ Sinf2,
+                const double* __restrict__ Sinf3, const double* __restrict__ Sinfp, int N, double T, double mu,
+                const cd* __restrict__ tw, double* part, double* scal)
+{
+    constexpr int H = L / 2;
+    constexpr int NWB = H + 1;
+    __shared__ double y[L + 1], M[L + 1];
+    __shared__ cd X[L], Gb[L], Jb[L];
+    __shared__ double sc[16];
+    const int nk = N / 2 + 1;
+    const long nq = (long)nk * (nk - 2), nk8 = (long)nk * (nk + 1) / 2;
+    const double nf = 1.0 / ((double)N * N);
+    const double Sinf = Sinfp[0];
+    const double hstep = 1.0 / (L * T);
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int blk = xcd_remap(blockIdx.x, gridDim.x);
+    double* c1 = part + (long)blk * 2 * NWB;
+    double* c2 = c1 + NWB;
+    for (int n = tid; n < NWB; n += nt) {
+        c1[n] = 0.0;
+        c2[n] = 0.0;
+    }
+    double k0acc = 0.0, inf1 = 0.0, inf2 = 0.0, inf4 = 0.0;
+    for (int kk = 0; kk < kJJK; kk++) {
+        const long q = (long)blk * kJJK + kk;
+        if (q >= nq) break;
+        const int l = (int)(q / nk), m = (int)(q % nk);
+        const long ki = fold_idx(l + 1, m, N);
+        const double C2 = Sinf2[ki], C3 = Sinf3[ki];
+        const double e = ek[m + (l + 1) * nk];
+        const double v = dek[q];
+        double wt = 4;
+        if (m == 0 || m == nk - 1) wt = wt / 2;
+        // (1) Dyson G for all slots
+        for (int s = tid; s < L; s += nt) {
+            double kn;
+            Gb[s] = dyson_G(Self, ki, nk8, s, H, C2, C3, Sinf, e, mu, T, kn);
+        }
+        // (2) f0V spline -> J0, accumulate chijj2(0) = sum G^2 J0 / T
+        for (int j = tid; j <= L; j += nt) y[j] = f0[q + (long)j * nq];
+        const double fd0 = df0V[q], fdN = df0V[q + nq];
+        __syncthreads();
+        const double cj01 = -y[0] - y[L], cj02 = fd0 + fdN;
+        spline_to_X<L>(y, M, X, L, hstep, fd0, fdN, 0);
+        const double cj03 = -(M[0] + M[L]);
+        block_fft<L, PEN_THREADS>(X, +1, tw);
+        cd qn0 = cd{0.0, 0.0};
+        for (int s = tid; s < L; s += nt) {
+            const cd J0 = j_slot(X, s, L, T, cj01, cj02, cj03);
+            qn0 = qn0 + Gb[s] * Gb[s] * J0 / T;
+        }
+        qn0 = block_sum_cd(qn0, sc);
+        // (3) hV spline -> J, products and asymptote subtraction
+        for (int j = tid; j <= L; j += nt) y[j] = h[q + (long)j * nq];
+        const double hd0 = dhV[q], hdN = dhV[q + nq];
+        __syncthreads();
+        const double CJ1 = -y[0] - y[L], cj2 = hd0 + hdN;
+        spline_to_X<L>(y, M, X, L, hstep, hd0, hdN, 0);
+        const double cj3 = -(M[0] + M[L]);
+        block_fft<L, PEN_THREADS>(X, +1, tw);
+        cd k0 = cd{0.0, 0.0};
+        double dGp = 0.0, dJ0 = 0.0;
+        for (int s = tid; s < L; s += nt) {
+            const cd Jv = j_slot(X, s, L, T, CJ1, cj2, cj3);
+            const double kn = (2.0 * (s - H) + 1) * kPi * T;
+            const cd Gv = Gb[s];
+            const cd G2 = T / cd{-e + mu, kn + Sinf / kn};
+            const cd Jinf = cd{0.0, -CJ1 / kn};
+            const cd Jn = Gv * Jv - G2 * Jinf;
+            const cd Gn = Gv - G2;
+            Jb[s] = Jn;
+            Gb[s] = Gn;
+            k0 = k0 + Gn;
+            dGp += kn * Gn.y;
+            dJ0 -= kn * Jn.y;
+        }
+        k0 = block_sum_cd(k0, sc);
+        dGp = block_sum(dGp, sc);
+        dJ0 = block_sum(dJ0, sc);
+        // (4) two-pole asymptote
+        const Poles P = make_poles(e, mu, Sinf, CJ1, T);
+        cd f1p, f1m, fj1, fj1b, f2p, f2m, fj2, fj2b;
+        if (P.r1.x > 0) {
+            const cd ex = cexpd(-(P.r1 / T));
+            f1p = P.a1 / P.den1;
+            f1m = P.a1 * ex / P.den1;
+            fj1 = P.r1 * P.b1 * ex / P.den1;
+            fj1b = P.r1 * P.b1 / P.den1;
+        } else {
+            const cd ex = cexpd(P.r1 / T);
+            f1p = P.a1 * ex / P.den1;
+            f1m = P.a1 / P.den1;
+            fj1 = P.r1 * P.b1 / P.den1;
+            fj1b = P.r1 * P.b1 * ex / P.den1;
+        }
+        if (P.r2.x > 0) {
+            const cd ex = cexpd(-(P.r2 / T));
+            f2p = P.a2 / P.den2;
+            f2m = P.a2 * ex / P.den2;
+            fj2 = P.r2 * P.b2 * ex / P.den2;
+            fj2b = P.r2 * P.b2 / P.den2;
+        } else {
+            const cd ex = cexpd(P.r2 / T);
+            f2p = P.a2 * ex / P.den2;
+            f2m = P.a2 / P.den2;
+            fj2 = P.r2 * P.b2 / P.den2;
+            fj2b = P.r2 * P.b2 * ex / P.den2;
+        }
+        const double k0r = k0.x + (f1m + f2m).x;
+        double dGm = -dGp;
+        dGp += (P.r1 * f1p + P.r2 * f2p).x;
+        dGm += (P.r1 * f1m + P.r2 * f2m).x;
+        double dJb = -dJ0;
+        dJ0 += (fj1 + fj2).x;
+        dJb += (fj1b + fj2b).x;
+        // (5) to imaginary time
+        block_fft<L, PEN_THREADS>(Gb, -1, tw);
+        block_fft<L, PEN_THREADS>(Jb, +1, tw);
+        const cd Gbeta0 = Gb[0], Jbeta0 = Jb[0];
+        __syncthreads();
+        for (int j = tid; j < L; j += nt) {
+            const double t = j / (L * T);
+            cd e1, ej1, e2, ej2;
+            if (P.r1.x > 0) {
+                e1 = cexpd(-t * P.r1);
+                ej1 = cexpd(P.r1 * (t - 1.0 / T));
+            } else {
+                e1 = cexpd((1.0 / T - t) * P.r1);
+                ej1 = cexpd(P.r1 * t);
+            }
+            if (P.r2.x > 0) {
+                e2 = cexpd(-t * P.r2);
+                ej2 = cexpd(P.r2 * (t - 1.0 / T));
+            } else {
+                e2 = cexpd((1.0 / T - t) * P.r2);
+                ej2 = cexpd(P.r2 * t);
+            }
+            const double G0t = (-(P.a1 * e1 / P.den1) - P.a2 * e2 / P.den2).x;
+            const double Jinf = (P.b1 * ej1 / P.den1 + P.b2 * ej2 / P.den2).x;
+            const cd ph = cis((j * kPi * (L - 1)) / L);
+            Gb[j] = ph * Gb[j] + cd{G0t, 0.0};
+            Jb[j] = Jb[j] / ph + cd{Jinf, 0.0};
+        }
+        cd Gbeta, Jbeta;
+        {
+            const double t = 1.0 / T;
+            cd e1, ej1, e2, ej2;
+            if (P.r1.x > 0) {
+                e1 = cexpd(-t * P.r1);
+                ej1 = cd{1.0, 0.0};
+            } else {
+                e1 = cd{1.0, 0.0};
+                ej1 = cexpd(P.r1 * t);
+            }
+            if (P.r2.x > 0) {
+                e2 = cexpd(-t * P.r2);
+                ej2 = cd{1.0, 0.0};
+            } else {
+                e2 = cd{1.0, 0.0};
+                ej2 = cexpd(P.r2 * t);
+            }
+            const double G0t = (-(P.a1 * e1 / P.den1) - P.a2 * e2 / P.den2).x;
+            const double Jinf = (P.b1 * ej1 / P.den1 + P.b2 * ej2 / P.den2).x;
+            const cd ph = cis(kPi * (L - 1));
+            Gbeta = ph * Gbeta0 + cd{G0t, 0.0};
+            Jbeta = Jbeta0 / ph + cd{Jinf, 0.0};
+        }
+        (void)Gbeta;
+        __syncthreads();
+        const cd G0 = Gb[0], J0t = Jb[0];
+        // (6) G(tau) J(-tau) -> Maki-Thompson
+        for (int j = tid; j < L; j += nt) y[j] = (Gb[j] * Jb[j]).x;
+        if (tid == 0) y[L] = ((-G0 - cd{1.0, 0.0}) * Jbeta).x;
+        __syncthreads();
+        const double gd0 = (dGp * J0t + G0 * dJ0).x;
+        const double gdN = (dGm * Jbeta - (G0 + cd{1.0, 0.0}) * dJb).x;
+        const double MJ1 = y[0] - y[L];
+        spline_to_X<L>(y, M, X, L, hstep, gd0, gdN, 1);
+        const double MJ3 = M[0] - M[L];
+        block_fft<L, PEN_THREADS>(X, -1, tw);
+        for (int n = 1 + tid; n <= H; n += nt) {
+            const double wn = 2 * n * kPi * T;
+            const cd ee = cis(-2 * n * kPi / L);
+            const cd a = cd{1.0 - ee.x, -ee.y} * X[n];
+            const double w4 = wn * wn * wn * wn;
+            const double imJ = -2.0 * MJ1 / wn + 2.0 * MJ3 / (wn * wn * wn) + 2.0 * (a.y / w4);
+            c2[n] -= nf * wt * v * imJ / wn;
+        }
+        if (tid == 0) c2[0] -= nf * wt * v * qn0.x;
+        __syncthreads();
+        // (7) -G(tau) G(-tau) -> bubble
+        const double dchi = dGp * (G0.x + 1.0) + G0.x * dGm;
+        for (int j = tid; j <= H; j += nt) {
+            cd gg;
+            if (j == 0)
+                gg = G0 * (G0 + cd{1.0, 0.0});
+            else
+                gg = -(Gb[j] * Gb[L - j]);
+            y[j] = gg.x;
+        }
+        __syncthreads();
+        spline_to_X<L>(y, M, X, H, hstep, dchi, 0.0, 2);
+        double integ = 0.0;
+        for (int j = tid; j < H; j += nt)
+            integ += hstep * (y[j] + y[j + 1]) * 0.5 - hstep * hstep * hstep * (M[j] + M[j + 1]) / 24.0;
+        integ = block_sum(integ, sc);
+        block_fft<L, PEN_THREADS>(X, -1, tw);
+        for (int n = tid; n <= H; n += nt) {
+            double bub;
+            if (n == 0)
+                bub = 2 * integ;
+            else {
+                const double wn = 2 * n * kPi * T;
+                const cd ee = cis(-2 * n * kPi / L);
+                const cd a = cd{1.0 - ee.x, -ee.y} * X[n];
+                bub = -2.0 * dchi / (wn * wn) + a.x / (wn * wn * wn * wn);
+            }
+            c1[n] -= 2.0 * nf * wt * v * v * bub;
+        }
+        k0acc -= 2.0 * wt * nf * d2ek[m + (l + 1) * nk] * k0r;
+        inf1 += 4.0 * dchi * nf * wt * v * v;
+        inf2 += nf * wt * v * 2.0 * MJ1;
+        inf4 -= nf * wt * v * 2.0 * MJ3;
+        __syncthreads();
+    }
+    if (tid == 0) {
+        scal[blk * 4 + 0] = k0acc;
+        scal[blk * 4 + 1] = inf1;
+        scal[blk * 4 + 2] = inf2;
+        scal[blk * 4 + 3] = inf4;
+    }
+}
+
+// kx = 0 and kx = pi rows of the diamagnetic term (cond_opt.cpp:3993-4173); block b:
+// row l = (b < nk ? 0 : nk-1), m = b % nk
+__global__ void k_kx0_rows(const double* __restrict__ ek, const double* __restrict__ d2ek, const cd* __restrict__ Self,
+                           const double* __restrict__ Sinf2, const double* __restrict__ Sinf3,
+                           const double* __restrict__ Sinfp, int N, int L, double T, double mu, double* out)
+{
+    __shared__ double sc[16];
+    const int nk = N / 2 + 1;
+    const long nk8 = (long)nk * (nk + 1) / 2;
+    const int l = (blockIdx.x < nk) ? 0 : nk - 1, m = blockIdx.x % nk;
+    const long ki = fold_idx(l, m, N);
+    const double C2 = Sinf2[ki], C3 = Sinf3[ki], Sinf = Sinfp[0];
+    const double e = ek[m + l * nk];
+    const int H = L / 2;
+    cd k0 = cd{0, 0};
+    for (int s = threadIdx.x; s < L; s += blockDim.x) {
+        double kn;
+        const cd Gv = dyson_G(Self, ki, nk8, s, H, C2, C3, Sinf, e, mu, T, kn);
+        const cd G0v = T / cd{-e + mu, kn + Sinf / kn};
+        k0 = k0 + (Gv - G0v);
+    }
+    k0 = block_sum_cd(k0, sc);
+    if (threadIdx.x == 0) {
+        const Poles P = make_poles(e, mu, Sinf, 0.0, T);
+        const cd f1m = (P.r1.x > 0) ? P.a1 * cexpd(-(P.r1 / T)) / P.den1 : P.a1 / P.den1;
+        const cd f2m = (P.r2.x > 0) ? P.a2 * cexpd(-(P.r2 / T)) / P.den2 : P.a2 / P.den2;
+        const double k0r = k0.x + (f1m + f2m).x;
+        double wt = 2;
+        if (m == 0 || m == nk - 1) wt = wt / 2;
+        out[blockIdx.x] = -2.0 * wt / ((double)N * N) * d2ek[m + l * nk] * k0r;
+    }
+}
+
+// ================================================================== launchers
+static size_t slab_lds(int N, bool lambda)
+{
+    const int nk = N / 2 + 1;
+    const long n = lambda ? (long)nk * (nk - 2) : (long)nk * nk;
+    return sizeof(double) * (size_t)(n + (n & 1)) + sizeof(cd) * (size_t)N;
+}
+
+bool supported_N(int N) { return N == 4 || N == 8 || N == 16 || N == 32 || N == 64 || N == 128 || N == 256; }
+bool supported_Nw(int Nw)
+{
+    return Nw == 16 || Nw == 32 || Nw == 64 || Nw == 128 || Nw == 256 || Nw == 512 || Nw == 1024 || Nw == 2048 ||
+           Nw == 4096;
+}
+
+#define DISPATCH_N(N, ...)                          \
+    switch (N) {                                    \
+        case 4: { constexpr int NN = 4; __VA_ARGS__; } break;     \
+        case 8: { constexpr int NN = 8; __VA_ARGS__; } break;     \
+        case 16: { constexpr int NN = 16; __VA_ARGS__; } break;   \
+        case 32: { constexpr int NN = 32; __VA_ARGS__; } break;   \
+        case 64: { constexpr int NN = 64; __VA_ARGS__; } break;   \
+        case 128: { constexpr int NN = 128; __VA_ARGS__; } break; \
+        case 256: { constexpr int NN = 256; __VA_ARGS__; } break; \
+        default: return hipErrorInvalidValue;       \
+    }
+#define DISPATCH_L(Lv, ...)                           \
+    switch (Lv) {                                     \
+        case 16: { constexpr int LL = 16; __VA_ARGS__; } break;     \
+        case 32: { constexpr int LL = 32; __VA_ARGS__; } break;     \
+        case 64: { constexpr int LL = 64; __VA_ARGS__; } break;     \
+        case 128: { constexpr int LL = 128; __VA_ARGS__; } break;   \
+        case 256: { constexpr int LL = 256; __VA_ARGS__; } break;   \
+        case 512: { constexpr int LL = 512; __VA_ARGS__; } break;   \
+        case 1024: { constexpr int LL = 1024; __VA_ARGS__; } break; \
+        case 2048: { constexpr int LL = 2048; __VA_ARGS__; } break; \
+        case 4096: { constexpr int LL = 4096; __VA_ARGS__; } break; \
+        default: return hipErrorInvalidValue;         \
+    }
+
+#define DISPATCH_LJJ(Lv, ...)                         \
+    switch (Lv) {                                     \
+        case 16: { constexpr int LL = 16; __VA_ARGS__; } break;     \
+        case 32: { constexpr int LL = 32; __VA_ARGS__; } break;     \
+        case 64: { constexpr int LL = 64; __VA_ARGS__; } break;     \
+        case 128: { constexpr int LL = 128; __VA_ARGS__; } break;   \
+        case 256: { constexpr int LL = 256; __VA_ARGS__; } break;   \
+        case 512: { constexpr int LL = 512; __VA_ARGS__; } break;   \
+        case 1024: { constexpr int LL = 1024; __VA_ARGS__; } break; \
+        case 2048: { constexpr int LL = 2048; __VA_ARGS__; } break; \
+        default: return hipErrorInvalidValue;         \
+    }
+
+template <typename K>
+static void set_lds(K kern, size_t bytes)
+{
+    hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+hipError_t launch_fill_tw(cd* tw, int L, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fill_tw, dim3((L + 255) / 256), dim3(256), 0, s, tw, L);
+    return hipGetLastError();
+}
+
+hipError_t launch_finalize(const double* part, int n, double* out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, part, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_chi0_tau0(const Grid& g, const DevArrays& d, double mu0, hipStream_t s)
+{
+    const size_t lds = slab_lds(g.N, false);
+    DISPATCH_N(g.N, {
+        set_lds(k_chi0_tau0<NN>, lds);
+        hipLaunchKernelGGL(k_chi0_tau0<NN>, dim3(1), dim3(SLAB_THREADS), lds, s, d.ekw, d.er, mu0, g.T, d.chi,
+                           d.chirt, d.dchirt0, d.dGrt0p, d.dchiqt0, d.tmp);
+    });
+    return hipGetLastError();
+}
+
+hipError_t launch_chi0_slab(const Grid& g, const DevArrays& d, double mu0, hipStream_t s)
+{
+    const size_t lds = slab_lds(g.N, false);
+    DISPATCH_N(g.N, {
+        set_lds(k_chi0_slab<NN>, lds);
+        hipLaunchKernelGGL(k_chi0_slab<NN>, dim3(g.nwb - 1), dim3(SLAB_THREADS), lds, s, d.ekw, mu0, g.T, g.Nw, d.chi,
+                           d.chirt);
+    });
+    return hipGetLastError();
+}
+
+hipError_t launch_chi0_pencil(const Grid& g, const DevArrays& d, hipStream_t s)
+{
+    DISPATCH_L(g.Nw, {
+        hipLaunchKernelGGL(k_chi0_pencil<LL>, dim3((unsigned)g.nk8), dim3(PEN_THREADS), 0, s, d.chi, d.dchiqt0,
+                           d.chi0inf, d.chi0inf2, g.nk8, g.T, d.twN);
+    });
+    return hipGetLastError();
+}
+
+hipError_t launch_chi0max(const Grid& g, const DevArrays& d, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_chi0max, dim3(1), dim3(1024), 0, s, d.chi, g.nk8, d.red, d.imax);
+    return hipGetLastError();
+}
+
+hipError_t launch_sum_rule(const Grid& g, const DevArrays& d, double sU, hipStream_t s)
+{
+    const double Nq = 4.0 * (g.nk - 1) * (g.nk - 1);
+    hipLaunchKernelGGL(k_sum_rule, dim3(kRedBlocks), dim3(256), 0, s, d.chi, d.chi0inf, d.chi0inf2, d.wq, g.nk8,
+                       g.Nw / 2, g.T, Nq, sU, d.part);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, d.part, kRedBlocks, d.red);
+    return hipGetLastError();
+}
+
+hipError_t launch_density(const Grid& g, const DevArrays& d, double mu, double n, hipStream_t s)
+{
+    const double Nk2 = 4.0 * (g.nk - 1) * (g.nk - 1);
+    hipLaunchKernelGGL(k_density, dim3(1), dim3(1024), 0, s, d.ekw, d.wq, g.nk8, Nk2, g.T, mu, n, d.red);
+    return hipGetLastError();
+}
+
+hipError_t launch_vertex_slab(const Grid& g, const DevArrays& d, double U, double Usp, double Uch, hipStream_t s)
+{
+    const size_t lds = slab_lds(g.N, false);
+    const double normFact = g.T / ((double)g.N * g.N);
+    DISPATCH_N(g.N, {
+        set_lds(k_vertex_slab<NN>, lds);
+        hipLaunchKernelGGL(k_vertex_slab<NN>, dim3(g.nwb), dim3(SLAB_THREADS), lds, s, d.chi, d.V, normFact, U, Usp,
+                           Uch);
+    });
+    return hipGetLastError();
+}
+
+hipError_t launch_vertex_pencil(const Grid& g, const DevArrays& d, double U, double Usp, double Uch, hipStream_t s)
+{
+    const double fact = U * (3.0 * Usp + Uch) / 8.0;
+    DISPATCH_L(g.Nw, {
+        hipLaunchKernelGGL(k_vertex_pencil<LL>, dim3((unsigned)((g.nk8 + 1) / 2)), dim3(PEN_THREADS), 0, s, d.V,
+                           d.chirt, g.nk8, fact, d.twN);
+    });
+    return hipGetLastError();
+}
+
+hipError_t launch_sigma_bnd(const Grid& g, const DevArrays& d, double mu0, double fact, hipStream_t s)
+{
+    const size_t lds = slab_lds(g.N, false);
+    DISPATCH_N(g.N, {
+        set_lds(k_sigma_bnd<NN>, lds);
+        hipLaunchKernelGGL(k_sigma_bnd<NN>, dim3(1), dim3(SLAB_THREADS), lds, s, d.ek, d.er, d.V, d.dchirt0, d.dGrt0p,
+                           mu0, g.T, fact, d.dSig, d.tmp);
+    });
+    return hipGetLastError();
+}
+
+hipError_t launch_sigma_slab(const Grid& g, const DevArrays& d, double mu0, hipStream_t s)
+{
+    const size_t lds = slab_lds(g.N, false);
+    DISPATCH_N(g.N, {
+        set_lds(k_sigma_slab<NN>, lds);
+        hipLaunchKernelGGL(k_sigma_slab<NN>, dim3(g.Nw + 1), dim3(SLAB_THREADS), lds, s, d.ek, d.V, mu0, g.T, g.Nw,
+                           d.Stau);
+    });
+    return hipGetLastError();
+}
+
+hipError_t launch_sigma_pencil(const Grid& g, const DevArrays& d, hipStream_t s)
+{
+    DISPATCH_L(g.Nw, {
+        hipLaunchKernelGGL(k_sigma_pencil<LL>, dim3((unsigned)g.nk8), dim3(PEN_THREADS), 0, s, d.Stau, d.dSig, g.nk8,
+                           g.T, d.Self, d.Sinf2, d.Sinf3, d.Sinf4, d.Sinf, d.twN);
+    });
+    return hipGetLastError();
+}
+
+hipError_t launch_trace_diffG(const Grid& g, const DevArrays& d, double pmu, double mu0, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_trace_diffG, dim3(kRedBlocks), dim3(256), 0, s, d.ekw, d.wq, d.Self, d.Sinf2, d.Sinf3,
+                       d.Sinf4, d.Sinf, g.nk8, g.Nw, g.T, mu0, pmu, d.part);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, d.part, kRedBlocks, d.red);
+    return hipGetLastError();
+}
+
+hipError_t launch_trace_selfG(const Grid& g, const DevArrays& d, double mu0, double mu, int which, hipStream_t s)
+{
+    const double nk02 = 4.0 * (g.nk - 1) * (g.nk - 1);
+    hipLaunchKernelGGL(k_trace_selfG, dim3(kRedBlocks), dim3(256), 0, s, d.ekw, d.wq, d.Self, d.Sinf2, d.Sinf3,
+                       d.Sinf, g.nk8, g.Nw, g.T, mu0, mu, nk02, which, d.part);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, d.part, kRedBlocks, d.red);
+    return hipGetLastError();
+}
+
+hipError_t launch_chijj(const Grid& g, const DevArrays& d, double mu0, double mu, double fact4, hipStream_t s,
+                        void (*mark)(void*, int), void* mark_ctx)
+{
+    const size_t lds = slab_lds(g.N, true);
+    if (mark) mark(mark_ctx, 0);
+    DISPATCH_N(g.N, {
+        set_lds(k_hf0_slab<NN>, lds);
+        set_lds(k_hf0_bnd<NN>, lds);
+        set_lds(k_hVf0V_slab<NN>, lds);
+        hipLaunchKernelGGL(k_hf0_slab<NN>, dim3(g.Nw + 1), dim3(SLAB_THREADS), lds, s, d.ek, d.dek, mu0, g.T, g.Nw,
+                           d.h, d.f0);
+        if (mark) mark(mark_ctx, 1);
+        hipLaunchKernelGGL(k_hf0_bnd<NN>, dim3(4), dim3(SLAB_THREADS), lds, s, d.ek, d.dek, d.V, d.dchirt0, d.h, d.f0,
+                           mu0, g.T, g.Nw, fact4, d.dhV, d.df0V);
+        if (mark) mark(mark_ctx, 2);
+        hipLaunchKernelGGL(k_hVf0V_slab<NN>, dim3(g.Nw + 1), dim3(SLAB_THREADS), lds, s, d.V, g.Nw, d.h, d.f0);
+    });
+    if (mark) mark(mark_ctx, 3);
+    const unsigned nblk = (unsigned)((g.nq + kJJK - 1) / kJJK);
+    DISPATCH_LJJ(g.Nw, {
+        hipLaunchKernelGGL(k_jj_pencil<LL>, dim3(nblk), dim3(PEN_THREADS), 0, s, d.ek, d.dek, d.d2ek, d.h, d.f0,
+                           d.dhV, d.df0V, d.Self, d.Sinf2, d.Sinf3, d.Sinf, g.N, g.T, mu, d.twN, d.jjpart, d.jjscal);
+    });
+    if (mark) mark(mark_ctx, 4);
+    hipLaunchKernelGGL(k_kx0_rows, dim3(2 * g.nk), dim3(256), 0, s, d.ek, d.d2ek, d.Self, d.Sinf2, d.Sinf3, d.Sinf,
+                       g.N, g.Nw, g.T, mu, d.kx0part);
+    if (mark) mark(mark_ctx, 5);
+    return hipGetLastError();
+}
+
+}  // namespace tpsc

@@ -1,0 +1,80 @@
+// tpsc_launch.hpp -- host-side launchers of the TPSC kernels (FP64 path).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "tpsc_device.hpp"
+
+namespace tpsc {
+
+// Device-resident arrays of one parameter point, sized for the full grid.
+// Layouts follow the reference: wedge idx(l,m) = m + l(l+1)/2, tau/frequency-major
+// a[idx + j*nk8]; Lambda grid a[m + l*nk + j*nq] with kx index l+1.
+struct DevArrays {
+    double* ek = nullptr;        // nk*nk quadrant
+    double* ekw = nullptr;       // nk8  wedge-packed eps(k)
+    double* wq = nullptr;        // nk8  wedge weight 8/2^(edges+diag)
+    double* er = nullptr;        // nk8
+    double* chi = nullptr;       // nwb*nk8 : chi0(q,tau) then chi0(q,iqn)
+    double* chirt = nullptr;     // nwb*nk8 : chi0(r,tau)
+    double* dchirt0 = nullptr;   // nk8
+    double* dGrt0p = nullptr;    // nk8
+    double* dchiqt0 = nullptr;   // nk8
+    double* chi0inf = nullptr;   // nk8
+    double* chi0inf2 = nullptr;  // nk8
+    double* V = nullptr;         // nwb*nk8
+    double* Stau = nullptr;      // (Nw+1)*nk8
+    double* dSig = nullptr;      // 2*nk8
+    cd* Self = nullptr;          // (Nw/2)*nk8
+    double* Sinf2 = nullptr;     // nk8
+    double* Sinf3 = nullptr;
+    double* Sinf4 = nullptr;
+    double* Sinf = nullptr;      // 1
+    double* dek = nullptr;       // nq   (v_x on the Lambda grid)
+    double* d2ek = nullptr;      // nk*nk
+    double* h = nullptr;         // (Nw+1)*nq
+    double* f0 = nullptr;        // (Nw+1)*nq
+    double* dhV = nullptr;       // 2*nq
+    double* df0V = nullptr;      // 2*nq
+    double* jjpart = nullptr;    // nblk * 2*nwb  per-workgroup Lambda partials
+    double* jjscal = nullptr;    // nblk * 4      per-workgroup k0, inf sums
+    double* kx0part = nullptr;   // 2*nk
+    double* tmp = nullptr;       // scratch, >= 4*nk8
+    double* part = nullptr;      // reduction partials
+    double* red = nullptr;       // reduction result(s)
+    cd* twN = nullptr;           // Nw   exp(-2 pi i t/Nw)
+    int* imax = nullptr;         // argmax result
+};
+
+struct Grid {
+    int N, nk, Nw, nwb;
+    long nk8, nq;
+    double T;
+};
+
+constexpr int kRedBlocks = 1024;
+constexpr int kJJK = 16;  // k-points per Lambda workgroup
+
+// ---- launchers; each returns hipGetLastError() of its launch
+hipError_t launch_fill_tw(cd* tw, int L, hipStream_t s);
+hipError_t launch_chi0_tau0(const Grid& g, const DevArrays& d, double mu0, hipStream_t s);
+hipError_t launch_chi0_slab(const Grid& g, const DevArrays& d, double mu0, hipStream_t s);
+hipError_t launch_chi0_pencil(const Grid& g, const DevArrays& d, hipStream_t s);
+hipError_t launch_chi0max(const Grid& g, const DevArrays& d, hipStream_t s);
+hipError_t launch_sum_rule(const Grid& g, const DevArrays& d, double sU, hipStream_t s);
+hipError_t launch_density(const Grid& g, const DevArrays& d, double mu, double n, hipStream_t s);
+hipError_t launch_vertex_slab(const Grid& g, const DevArrays& d, double U, double Usp, double Uch, hipStream_t s);
+hipError_t launch_vertex_pencil(const Grid& g, const DevArrays& d, double U, double Usp, double Uch, hipStream_t s);
+hipError_t launch_sigma_bnd(const Grid& g, const DevArrays& d, double mu0, double fact, hipStream_t s);
+hipError_t launch_sigma_slab(const Grid& g, const DevArrays& d, double mu0, hipStream_t s);
+hipError_t launch_sigma_pencil(const Grid& g, const DevArrays& d, hipStream_t s);
+hipError_t launch_trace_diffG(const Grid& g, const DevArrays& d, double pmu, double mu0, hipStream_t s);
+hipError_t launch_trace_selfG(const Grid& g, const DevArrays& d, double mu0, double mu, int which,
+                              hipStream_t s);
+hipError_t launch_chijj(const Grid& g, const DevArrays& d, double mu0, double mu, double fact4, hipStream_t s,
+                        void (*mark)(void*, int), void* mark_ctx);
+hipError_t launch_finalize(const double* part, int n, double* out, hipStream_t s);
+
+bool supported_N(int N);
+bool supported_Nw(int Nw);
+
+}  // namespace tpsc
