@@ -207,7 +207,7 @@ def test_headline_config_parity_and_invariants(oracle_mod):
     assert abs(g.sum_rule_ch(g.scalar("Uch"))) < 1e-13
     chi = g.get("chi0").reshape(-1, nk8)
     assert np.all(chi[1:, 0] == 0.0)
-    assert abs(g.traceDiffG(g.scalar("mu"))) < 1e-12
+    assert abs(g.traceDiffG(g.scalar("mu"))) < 1e-10  # 17M-term sum at its root
     target = 5.0 * g.scalar("dblOcc") - 5.0 * 0.81 / 4
     assert abs(g.scalar("trSG_corr") - target) < 1e-3 * abs(target)
     c1, c2 = g.get("chijj1"), g.get("chijj2")

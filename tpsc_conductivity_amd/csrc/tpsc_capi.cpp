@@ -21,8 +21,6 @@ using namespace tpsc;
 
 namespace {
 
-constexpr double PI = 3.14159265358979323846;
-
 struct ProfPair {
     std::string name;
     hipEvent_t a, b;
@@ -35,7 +33,6 @@ struct tpsc_ctx {
     // current grid (green0::set_grid)
     int nbk = 0, nbw = 0, nwmax = 0;
     Grid g{};
-    std::vector<double> ek, er;
     double mu0 = 0.0;
     double nsr = 0.0;
     // results
@@ -55,7 +52,8 @@ struct tpsc_ctx {
     hipStream_t stream = nullptr;
     DevArrays d{};
     double* chi_tau = nullptr;
-    double* host_scalar = nullptr;  // pinned
+    double* host_scalar = nullptr;  // pinned, mapped: kernels write root-finder values here
+    double* dev_scalar = nullptr;   // device alias of host_scalar
     size_t dev_bytes = 0;
     std::vector<void*> allocs;
     std::string err;
@@ -115,23 +113,6 @@ void prof_flush(tpsc_ctx* c)
         hipEventDestroy(pp.b);
     }
     c->pending.clear();
-}
-
-// ------------------------------------------------------------------ model (host)
-double dispk(const tpsc_params& p, double kx, double ky)
-{
-    return -2.0 * (cos(kx) + cos(ky)) - 4.0 * p.tp * cos(kx) * cos(ky) - 2.0 * p.tpp * (cos(2 * kx) + cos(2 * ky)) -
-           4 * p.t3 * (cos(2 * kx) * cos(ky) + cos(2 * ky) * cos(kx));
-}
-double grad_x(const tpsc_params& p, double kx, double ky)
-{
-    return 2.0 * sin(kx) + 4.0 * p.tp * sin(kx) * cos(ky) + 4.0 * p.tpp * sin(2 * kx) +
-           4 * p.t3 * (2.0 * sin(2 * kx) * cos(ky) + cos(2 * ky) * sin(kx));
-}
-double hess_xx(const tpsc_params& p, double kx, double ky)
-{
-    return 2.0 * cos(kx) + 4.0 * p.tp * cos(kx) * cos(ky) + 8.0 * p.tpp * cos(2 * kx) +
-           4 * p.t3 * (4.0 * cos(2 * kx) * cos(ky) + cos(2 * ky) * cos(kx));
 }
 
 // ------------------------------------------------------------------ root finder
@@ -295,9 +276,17 @@ int dalloc(tpsc_ctx* c, T** ptr, size_t count)
 
 int read_scalar(tpsc_ctx* c, const double* dptr, double* out)
 {
-    HIPCHK(c, hipMemcpyAsync(c->host_scalar, dptr, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->host_scalar + 1, dptr, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    *out = c->host_scalar[0];
+    *out = c->host_scalar[1];
+    return 0;
+}
+
+// a reduction launched with out = dev_scalar has written its value to mapped host memory
+int wait_mapped(tpsc_ctx* c, double* out)
+{
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *out = ((volatile double*)c->host_scalar)[0];
     return 0;
 }
 
@@ -312,9 +301,9 @@ int find_mu0(tpsc_ctx* c)
     auto f = [&](double m) {
         double v = 0.0;
         if (rc == 0) {
-            hipError_t e = launch_density(c->g, c->d, m, c->p.n, c->stream);
+            hipError_t e = launch_density(c->g, c->d, m, c->p.n, c->dev_scalar, c->stream);
             if (e != hipSuccess) rc = fail(c, TPSC_E_DEVICE, hipGetErrorString(e));
-            if (rc == 0) rc = read_scalar(c, c->d.red, &v);
+            if (rc == 0) rc = wait_mapped(c, &v);
         }
         return v;
     };
@@ -343,36 +332,8 @@ int set_grid(tpsc_ctx* c, int N, int Nw)
     g.nk8 = (long)nk * (nk + 1) / 2;
     g.nq = (long)nk * (nk - 2);
     g.T = c->p.T;
-    c->ek.assign((size_t)nk * nk, 0.0);
-    for (int l = 0; l < nk; l++)
-        for (int m = 0; m < nk; m++) c->ek[m + l * nk] = dispk(c->p, l * PI / (nk - 1), m * PI / (nk - 1));
-    c->er.assign(std::max<long>(g.nk8, 5), 0.0);
-    c->er[1] = -1;
-    c->er[2] = -c->p.tp;
-    c->er[3] = -c->p.tpp;
-    c->er[4] = -c->p.t3;
-    std::vector<double> ekw(g.nk8), wq(g.nk8), d2(nk * nk), dek(std::max<long>(g.nq, 1));
-    for (int l = 0; l < nk; l++)
-        for (int m = 0; m <= l; m++) {
-            const long i = (long)m + (long)l * (l + 1) / 2;
-            ekw[i] = c->ek[m + l * nk];
-            double w = 8;
-            if (l == 0 || l == nk - 1) w /= 2;
-            if (m == 0 || m == nk - 1) w /= 2;
-            if (m == l) w /= 2;
-            wq[i] = w;
-        }
-    for (int l = 0; l < nk; l++)
-        for (int m = 0; m < nk; m++) d2[m + l * nk] = hess_xx(c->p, l * PI / (nk - 1), m * PI / (nk - 1));
-    for (int l = 0; l < nk - 2; l++)
-        for (int m = 0; m < nk; m++) dek[m + l * nk] = grad_x(c->p, (l + 1) * PI / (nk - 1), m * PI / (nk - 1));
     hipStream_t s = c->stream;
-    HIPCHK(c, hipMemcpyAsync(c->d.ek, c->ek.data(), sizeof(double) * nk * nk, hipMemcpyHostToDevice, s));
-    HIPCHK(c, hipMemcpyAsync(c->d.er, c->er.data(), sizeof(double) * g.nk8, hipMemcpyHostToDevice, s));
-    HIPCHK(c, hipMemcpyAsync(c->d.ekw, ekw.data(), sizeof(double) * g.nk8, hipMemcpyHostToDevice, s));
-    HIPCHK(c, hipMemcpyAsync(c->d.wq, wq.data(), sizeof(double) * g.nk8, hipMemcpyHostToDevice, s));
-    HIPCHK(c, hipMemcpyAsync(c->d.d2ek, d2.data(), sizeof(double) * nk * nk, hipMemcpyHostToDevice, s));
-    if (g.nq > 0) HIPCHK(c, hipMemcpyAsync(c->d.dek, dek.data(), sizeof(double) * g.nq, hipMemcpyHostToDevice, s));
+    HIPCHK(c, launch_model(g, c->d, c->p.tp, c->p.tpp, c->p.t3, s));
     HIPCHK(c, launch_fill_tw(c->d.twN, Nw, s));
     HIPCHK(c, hipStreamSynchronize(s));
     c->have_chi0 = false;
@@ -417,8 +378,8 @@ int eval_sum_rule(tpsc_ctx* c, int channel, double Ux, double* out)
 {
     if (!c->have_chi0) return fail(c, TPSC_E_ORDER, "sum rule before chi0");
     double tr = 0.0;
-    HIPCHK(c, launch_sum_rule(c->g, c->d, channel == 0 ? -Ux : Ux, c->stream));
-    int rc = read_scalar(c, c->d.red, &tr);
+    HIPCHK(c, launch_sum_rule(c->g, c->d, channel == 0 ? -Ux : Ux, c->dev_scalar, c->stream));
+    int rc = wait_mapped(c, &tr);
     if (rc) return rc;
     const double n = c->p.n, nsr = c->nsr, U = c->p.U;
     if (channel == 0)
@@ -538,8 +499,8 @@ int stage_sigma(tpsc_ctx* c)
 
 int reduce_trace(tpsc_ctx* c, int which, double* out)
 {
-    HIPCHK(c, launch_trace_selfG(c->g, c->d, c->mu0, c->mu, which, c->stream));
-    return read_scalar(c, c->d.red, out);
+    HIPCHK(c, launch_trace_selfG(c->g, c->d, c->mu0, c->mu, which, c->dev_scalar, c->stream));
+    return wait_mapped(c, out);
 }
 
 // green::traceSelfG, green.cpp:3001-3104
@@ -568,8 +529,8 @@ int stage_trace_selfG2(tpsc_ctx* c)
 
 int eval_diffG(tpsc_ctx* c, double pmu, double* out)
 {
-    HIPCHK(c, launch_trace_diffG(c->g, c->d, pmu, c->mu0, c->stream));
-    return read_scalar(c, c->d.red, out);
+    HIPCHK(c, launch_trace_diffG(c->g, c->d, pmu, c->mu0, c->dev_scalar, c->stream));
+    return wait_mapped(c, out);
 }
 
 // green::find_mu, green.cpp:3185-3282
@@ -622,26 +583,17 @@ int stage_chijj(tpsc_ctx* c)
     HIPCHK(c, launch_chijj(g, c->d, c->mu0, c->mu, fact4, c->stream, c->prof ? jj_mark : nullptr, c));
     const int nblk = (int)((g.nq + kJJK - 1) / kJJK);
     const int nwb = g.nwb;
-    std::vector<double> part((size_t)nblk * 2 * nwb), scal((size_t)nblk * 4), rows(2 * g.nk);
-    HIPCHK(c, hipMemcpyAsync(part.data(), c->d.jjpart, sizeof(double) * part.size(), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(scal.data(), c->d.jjscal, sizeof(double) * scal.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, launch_reduce_cols(c->d.jjpart, nblk, 2 * nwb, c->d.jjred, c->stream));
+    HIPCHK(c, launch_reduce_cols(c->d.jjscal, nblk, 4, c->d.jjred + 2 * nwb, c->stream));
+    std::vector<double> red(2 * nwb + 4), rows(2 * g.nk);
+    HIPCHK(c, hipMemcpyAsync(red.data(), c->d.jjred, sizeof(double) * red.size(), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipMemcpyAsync(rows.data(), c->d.kx0part, sizeof(double) * rows.size(), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     prof_flush(c);
-    c->chijj1.assign(nwb, 0.0);
-    c->chijj2.assign(nwb, 0.0);
-    double k0 = 0.0, i1 = 0.0, i2 = 0.0, i4 = 0.0;
-    for (int b = 0; b < nblk; b++) {
-        const double* r = &part[(size_t)b * 2 * nwb];
-        for (int n = 0; n < nwb; n++) {
-            c->chijj1[n] += r[n];
-            c->chijj2[n] += r[nwb + n];
-        }
-        k0 += scal[b * 4 + 0];
-        i1 += scal[b * 4 + 1];
-        i2 += scal[b * 4 + 2];
-        i4 += scal[b * 4 + 3];
-    }
+    c->chijj1.assign(red.begin(), red.begin() + nwb);
+    c->chijj2.assign(red.begin() + nwb, red.begin() + 2 * nwb);
+    double k0 = red[2 * nwb + 0];
+    const double i1 = red[2 * nwb + 1], i2 = red[2 * nwb + 2], i4 = red[2 * nwb + 3];
     for (double v : rows) k0 += v;
     c->kx0 = k0;
     c->jj_inf1 = i1;
@@ -704,7 +656,8 @@ int tpsc_create(const tpsc_params* p, tpsc_ctx** out)
     if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count < 1)
         return fail(c, TPSC_E_DEVICE, "no HIP device available");
     HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-    HIPCHK(c, hipHostMalloc((void**)&c->host_scalar, 16 * sizeof(double), hipHostMallocDefault));
+    HIPCHK(c, hipHostMalloc((void**)&c->host_scalar, 16 * sizeof(double), hipHostMallocMapped));
+    HIPCHK(c, hipHostGetDevicePointer((void**)&c->dev_scalar, c->host_scalar, 0));
     const int N = p->nk0, Nw = p->Nw0, nk = N / 2 + 1;
     const long nk8 = (long)nk * (nk + 1) / 2, nq = (long)nk * (nk - 2);
     const long nwb = Nw / 2 + 1;
@@ -740,7 +693,8 @@ int tpsc_create(const tpsc_params* p, tpsc_ctx** out)
     rc |= dalloc(c, &d.jjscal, (size_t)std::max<long>(nblk, 1) * 4);
     rc |= dalloc(c, &d.kx0part, 2 * (size_t)nk);
     rc |= dalloc(c, &d.tmp, 4 * (size_t)nk8);
-    rc |= dalloc(c, &d.part, kRedBlocks);
+    rc |= dalloc(c, &d.part, kPartCap);
+    rc |= dalloc(c, &d.jjred, 2 * (size_t)nwb + 4);
     rc |= dalloc(c, &d.red, 16);
     rc |= dalloc(c, &d.twN, (size_t)Nw);
     rc |= dalloc(c, &d.imax, 4);
@@ -882,7 +836,7 @@ int tpsc_get(tpsc_ctx* c, int field, void* dst, size_t bytes)
     if (bytes < n * sizeof(double)) return fail(c, TPSC_E_INVALID, "destination too small");
     const void* src = nullptr;
     switch (field) {
-        case TPSC_F_EK: memcpy(dst, c->ek.data(), n * sizeof(double)); return 0;
+        case TPSC_F_EK: src = c->d.ek; break;
         case TPSC_F_CHIJJ1:
         case TPSC_F_CHIJJ2: {
             const std::vector<double>& v = (field == TPSC_F_CHIJJ1) ? c->chijj1 : c->chijj2;

@@ -13,6 +13,8 @@
 //   k_kx0_rows                   cond_opt.cpp:2584-4173 Lambda_jj bubble + Maki-Thompson + kx0
 // Slab kernels hold one (nk x nk) FP64 slab in LDS (133 KB at N=256) and transform it with
 // wavefront FFTs; pencil kernels hold one tau-pencil per workgroup.
+#include <algorithm>
+
 #include "tpsc_launch.hpp"
 
 namespace tpsc {
@@ -332,29 +334,32 @@ __global__ void k_chi0max(const double* __restrict__ chi, long nk8, double* out,
 
 // ================================================================== reductions
 // sum-rule trace sum_{n=0}^{Nw} sum_q (w T/Nq) [chi/(1 + sU chi/2) - chi]  (chi.cpp:738-830)
-// sU = -Usp for the spin channel, +Uch for the charge channel.
-__global__ void k_sum_rule(const double* __restrict__ chi, const double* __restrict__ inf1,
-                           const double* __restrict__ inf2, const double* __restrict__ wq, long nk8, int nwmax,
-                           double T, double Nq, double sU, double* part)
+// sU = -Usp for the spin channel, +Uch for the charge channel.  Grid: x over wedge points
+// (256 per block), y over RG row groups; rows j = y, y+RG, ...  One partial per block.
+__global__ void __launch_bounds__(256)
+    k_sum_rule(const double* __restrict__ chi, const double* __restrict__ inf1, const double* __restrict__ inf2,
+               const double* __restrict__ wq, int nk8, int nwmax, double T, double Nq, double sU, double* part)
 {
     __shared__ double sc[16];
-    const long nterm = (long)(2 * nwmax + 1) * nk8;
+    const int q = blockIdx.x * 256 + threadIdx.x;
     double acc = 0.0;
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < nterm; i += (long)gridDim.x * blockDim.x) {
-        const long j = i / nk8, q = i - j * nk8;
-        double w = 2.0 * wq[q];
-        if (j == 0) w *= 0.5;
-        double c;
-        if (j <= nwmax)
-            c = chi[i];
-        else {
-            const double wn = 2 * j * kPi * T;
-            c = inf1[q] / (wn * wn) + inf2[q] / (wn * wn * wn * wn);
+    if (q < nk8) {
+        const double w16 = 2.0 * wq[q];
+        const double a1 = inf1[q], a2 = inf2[q];
+        for (int j = blockIdx.y; j <= 2 * nwmax; j += gridDim.y) {
+            const double w = (j == 0) ? 0.5 * w16 : w16;
+            double c;
+            if (j <= nwmax)
+                c = chi[(long)j * nk8 + q];
+            else {
+                const double wn = 2 * j * kPi * T;
+                c = a1 / (wn * wn) + a2 / (wn * wn * wn * wn);
+            }
+            acc += w * T / Nq * (c / (1 + sU * c / 2.0) - c);
         }
-        acc += w * T / Nq * (c / (1 + sU * c / 2.0) - c);
     }
     acc = block_sum(acc, sc);
-    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+    if (threadIdx.x == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = acc;
 }
 
 // density for the mu0 root (green0.cpp:182-203), one block
@@ -565,76 +570,124 @@ __device__ inline cd sigma_mom(double Sinf, double s2, double s3, cd R, double w
     return cd{-s2 / w2 + R.x / w4, -Sinf / wn + s3 / w3 + R.y / w4};
 }
 
-// traceDiffG(pmu) partials (green.cpp:3108-3182)
-__global__ void k_trace_diffG(const double* __restrict__ ekw, const double* __restrict__ wq,
-                              const cd* __restrict__ Self, const double* __restrict__ Sinf2,
-                              const double* __restrict__ Sinf3, const double* __restrict__ Sinf4,
-                              const double* __restrict__ Sinf, long nk8, int Nw, double T, double mu0, double pmu,
-                              double* part)
+// traceDiffG(pmu) partials (green.cpp:3108-3182); grid as k_sum_rule, rows j = 0..Nw
+__global__ void __launch_bounds__(256)
+    k_trace_diffG(const double* __restrict__ ekw, const double* __restrict__ wq, const cd* __restrict__ Self,
+                  const double* __restrict__ Sinf2, const double* __restrict__ Sinf3,
+                  const double* __restrict__ Sinf4, const double* __restrict__ Sinf, int nk8, int Nw, double T,
+                  double mu0, double pmu, double* part)
 {
     __shared__ double sc[16];
-    const double Si = Sinf[0];
-    const long nterm = (long)(Nw + 1) * nk8;
+    const int q = blockIdx.x * 256 + threadIdx.x;
     double acc = 0.0;
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < nterm; i += (long)gridDim.x * blockDim.x) {
-        const long j = i / nk8, q = i - j * nk8;
+    if (q < nk8) {
+        const double Si = Sinf[0];
         const double e = ekw[q], w = 2.0 * wq[q];
-        const double wn = (2 * j + 1) * kPi * T;
-        double rS, iS;
-        if (j >= Nw / 2) {
-            rS = -Sinf2[q] / (wn * wn) + Sinf4[q] / (wn * wn * wn * wn);
-            iS = -Si / wn + Sinf3[q] / (wn * wn * wn);
-        } else {
-            const cd s = sigma_mom(Si, Sinf2[q], Sinf3[q], Self[i], wn);
-            rS = s.x;
-            iS = s.y;
+        const double s2 = Sinf2[q], s3 = Sinf3[q], s4 = Sinf4[q];
+        for (int j = blockIdx.y; j <= Nw; j += gridDim.y) {
+            const double wn = (2 * j + 1) * kPi * T;
+            double rS, iS;
+            if (j >= Nw / 2) {
+                rS = -s2 / (wn * wn) + s4 / (wn * wn * wn * wn);
+                iS = -Si / wn + s3 / (wn * wn * wn);
+            } else {
+                const cd s = sigma_mom(Si, s2, s3, Self[(long)j * nk8 + q], wn);
+                rS = s.x;
+                iS = s.y;
+            }
+            const double tmp = w / (((wn - iS) * (wn - iS) + (e - pmu + rS) * (e - pmu + rS)) *
+                                    (wn * wn + (e - mu0) * (e - mu0)));
+            acc += tmp * ((mu0 - pmu + rS) * (e - pmu + rS) * (e - mu0) + iS * (iS - 2 * wn) * (e - mu0) -
+                          wn * wn * (mu0 - pmu + rS));
         }
-        const double tmp = w / (((wn - iS) * (wn - iS) + (e - pmu + rS) * (e - pmu + rS)) * (wn * wn + (e - mu0) * (e - mu0)));
-        acc += tmp * ((mu0 - pmu + rS) * (e - pmu + rS) * (e - mu0) + iS * (iS - 2 * wn) * (e - mu0) -
-                      wn * wn * (mu0 - pmu + rS));
     }
     acc = block_sum(acc, sc);
-    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+    if (threadIdx.x == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = acc;
 }
 
 // which = 0: Tr Sigma G0 main sum (green.cpp:3018-3051); 1: its Sigma_inf correction
-// (:3053-3084, uses j < Nw/2 terms plus one per k); 2: Tr Sigma G (green.cpp:2944-2976)
-__global__ void k_trace_selfG(const double* __restrict__ ekw, const double* __restrict__ wq,
-                              const cd* __restrict__ Self, const double* __restrict__ Sinf2,
-                              const double* __restrict__ Sinf3, const double* __restrict__ Sinf, long nk8, int Nw,
-                              double T, double mu0, double mu, double nk02, int which, double* part)
+// (:3053-3084, the j < Nw/2 terms plus one per k); 2: Tr Sigma G (green.cpp:2944-2976)
+__global__ void __launch_bounds__(256)
+    k_trace_selfG(const double* __restrict__ ekw, const double* __restrict__ wq, const cd* __restrict__ Self,
+                  const double* __restrict__ Sinf2, const double* __restrict__ Sinf3, const double* __restrict__ Sinf,
+                  int nk8, int Nw, double T, double mu0, double mu, double nk02, int which, double* part)
 {
     __shared__ double sc[16];
-    const double Si = Sinf[0];
-    const long nterm = (long)(Nw / 2) * nk8;
+    const int q = blockIdx.x * 256 + threadIdx.x;
     double acc = 0.0;
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < nterm; i += (long)gridDim.x * blockDim.x) {
-        const long j = i / nk8, q = i - j * nk8;
+    if (q < nk8) {
+        const double Si = Sinf[0];
         const double e = ekw[q], w = wq[q];
-        const double wn = (2 * j + 1) * kPi * T;
-        if (which == 1) {
-            acc += w * Si * T * 2 / (nk02 * (wn * wn + (e - mu0) * (e - mu0)));
-            if (j == 0) {
-                if (fabs(e - mu0) > 1.0e-12)
-                    acc += w * Si * (1.0 / (exp((e - mu0) / T) + 1.0) - 0.5) / (e - mu0) / nk02;
-                else {
-                    const double ex = (e - mu0) < 0 ? exp((e - mu0) / T) : exp(-(e - mu0) / T);
-                    acc -= w * Si * ex / ((ex + 1.0) * (ex + 1.0) * nk02 * T);
+        for (int j = blockIdx.y; j < Nw / 2; j += gridDim.y) {
+            const double wn = (2 * j + 1) * kPi * T;
+            if (which == 1) {
+                acc += w * Si * T * 2 / (nk02 * (wn * wn + (e - mu0) * (e - mu0)));
+                if (j == 0) {
+                    if (fabs(e - mu0) > 1.0e-12)
+                        acc += w * Si * (1.0 / (exp((e - mu0) / T) + 1.0) - 0.5) / (e - mu0) / nk02;
+                    else {
+                        const double ex = (e - mu0) < 0 ? exp((e - mu0) / T) : exp(-(e - mu0) / T);
+                        acc -= w * Si * ex / ((ex + 1.0) * (ex + 1.0) * nk02 * T);
+                    }
                 }
+                continue;
             }
-            continue;
-        }
-        const cd s = sigma_mom(Si, Sinf2[q], Sinf3[q], Self[i], wn);
-        if (which == 0) {
-            const double tmp = w * T * 2 / (nk02 * (wn * wn + (e - mu0) * (e - mu0)));
-            acc += tmp * (s.x * (mu0 - e) + s.y * wn);
-        } else {
-            const double tmp = w * T * 2 / (nk02 * ((wn - s.y) * (wn - s.y) + (e - mu + s.x) * (e - mu + s.x)));
-            acc += tmp * (s.y * wn - s.y * s.y - (e - mu) * s.x - s.x * s.x);
+            const cd s = sigma_mom(Si, Sinf2[q], Sinf3[q], Self[(long)j * nk8 + q], wn);
+            if (which == 0) {
+                const double tmp = w * T * 2 / (nk02 * (wn * wn + (e - mu0) * (e - mu0)));
+                acc += tmp * (s.x * (mu0 - e) + s.y * wn);
+            } else {
+                const double tmp = w * T * 2 / (nk02 * ((wn - s.y) * (wn - s.y) + (e - mu + s.x) * (e - mu + s.x)));
+                acc += tmp * (s.y * wn - s.y * s.y - (e - mu) * s.x - s.x * s.x);
+            }
         }
     }
     acc = block_sum(acc, sc);
-    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+    if (threadIdx.x == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = acc;
+}
+
+// column sums of a row-major (nrows x ncols) partial table in fixed row order
+__global__ void k_reduce_cols(const double* __restrict__ part, int nrows, int ncols, double* out)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncols) return;
+    double s = 0.0;
+    for (int r = 0; r < nrows; r++) s += part[(long)r * ncols + c];
+    out[c] = s;
+}
+
+// dispersion tables on the device (hamiltonien.cpp:49-68,125-155; green0 wedge weights)
+__global__ void k_model(int N, double tp, double tpp, double t3, double* ek, double* ekw, double* wq, double* er,
+                        double* d2ek, double* dek)
+{
+    const int nk = N / 2 + 1;
+    const long nk8 = (long)nk * (nk + 1) / 2, nq = (long)nk * (nk - 2);
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const double dk = kPi / (nk - 1);
+    if (i < (long)nk * nk) {
+        const int l = (int)(i / nk), m = (int)(i % nk);
+        const double kx = l * dk, ky = m * dk;
+        const double cx = cos(kx), cy = cos(ky), c2x = cos(2 * kx), c2y = cos(2 * ky);
+        ek[i] = -2.0 * (cx + cy) - 4.0 * tp * cx * cy - 2.0 * tpp * (c2x + c2y) - 4 * t3 * (c2x * cy + c2y * cx);
+        d2ek[i] = 2.0 * cx + 4.0 * tp * cx * cy + 8.0 * tpp * c2x + 4 * t3 * (4.0 * c2x * cy + c2y * cx);
+        if (m <= l) {
+            const long w = widx(l, m);
+            ekw[w] = ek[i];
+            double ww = 8;
+            if (l == 0 || l == nk - 1) ww /= 2;
+            if (m == 0 || m == nk - 1) ww /= 2;
+            if (m == l) ww /= 2;
+            wq[w] = ww;
+            er[w] = (w == 1) ? -1.0 : (w == 2) ? -tp : (w == 3) ? -tpp : (w == 4) ? -t3 : 0.0;
+        }
+    }
+    if (i < nq) {
+        const int l = (int)(i / nk), m = (int)(i % nk);
+        const double kx = (l + 1) * dk, ky = m * dk;
+        dek[i] = 2.0 * sin(kx) + 4.0 * tp * sin(kx) * cos(ky) + 4.0 * tpp * sin(2 * kx) +
+                 4 * t3 * (2.0 * sin(2 * kx) * cos(ky) + cos(2 * ky) * sin(kx));
+    }
+    (void)nk8;
 }
 
 // ================================================================== Lambda_jj slabs
@@ -1279,19 +1332,36 @@ hipError_t launch_chi0max(const Grid& g, const DevArrays& d, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_sum_rule(const Grid& g, const DevArrays& d, double sU, hipStream_t s)
+static dim3 red_grid(const Grid& g) { return dim3((unsigned)((g.nk8 + 255) / 256), kRowGroups); }
+
+hipError_t launch_sum_rule(const Grid& g, const DevArrays& d, double sU, double* out, hipStream_t s)
 {
     const double Nq = 4.0 * (g.nk - 1) * (g.nk - 1);
-    hipLaunchKernelGGL(k_sum_rule, dim3(kRedBlocks), dim3(256), 0, s, d.chi, d.chi0inf, d.chi0inf2, d.wq, g.nk8,
-                       g.Nw / 2, g.T, Nq, sU, d.part);
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, d.part, kRedBlocks, d.red);
+    const dim3 gr = red_grid(g);
+    hipLaunchKernelGGL(k_sum_rule, gr, dim3(256), 0, s, d.chi, d.chi0inf, d.chi0inf2, d.wq, (int)g.nk8, g.Nw / 2,
+                       g.T, Nq, sU, d.part);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, d.part, (int)(gr.x * gr.y), out);
     return hipGetLastError();
 }
 
-hipError_t launch_density(const Grid& g, const DevArrays& d, double mu, double n, hipStream_t s)
+hipError_t launch_model(const Grid& g, const DevArrays& d, double tp, double tpp, double t3, hipStream_t s)
+{
+    const long n = std::max<long>((long)g.nk * g.nk, g.nq);
+    hipLaunchKernelGGL(k_model, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g.N, tp, tpp, t3, d.ek, d.ekw,
+                       d.wq, d.er, d.d2ek, d.dek);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_cols(const double* part, int nrows, int ncols, double* out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_reduce_cols, dim3((ncols + 255) / 256), dim3(256), 0, s, part, nrows, ncols, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_density(const Grid& g, const DevArrays& d, double mu, double n, double* out, hipStream_t s)
 {
     const double Nk2 = 4.0 * (g.nk - 1) * (g.nk - 1);
-    hipLaunchKernelGGL(k_density, dim3(1), dim3(1024), 0, s, d.ekw, d.wq, g.nk8, Nk2, g.T, mu, n, d.red);
+    hipLaunchKernelGGL(k_density, dim3(1), dim3(1024), 0, s, d.ekw, d.wq, g.nk8, Nk2, g.T, mu, n, out);
     return hipGetLastError();
 }
 
@@ -1348,20 +1418,24 @@ hipError_t launch_sigma_pencil(const Grid& g, const DevArrays& d, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_trace_diffG(const Grid& g, const DevArrays& d, double pmu, double mu0, hipStream_t s)
+hipError_t launch_trace_diffG(const Grid& g, const DevArrays& d, double pmu, double mu0, double* out,
+                              hipStream_t s)
 {
-    hipLaunchKernelGGL(k_trace_diffG, dim3(kRedBlocks), dim3(256), 0, s, d.ekw, d.wq, d.Self, d.Sinf2, d.Sinf3,
-                       d.Sinf4, d.Sinf, g.nk8, g.Nw, g.T, mu0, pmu, d.part);
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, d.part, kRedBlocks, d.red);
+    const dim3 gr = red_grid(g);
+    hipLaunchKernelGGL(k_trace_diffG, gr, dim3(256), 0, s, d.ekw, d.wq, d.Self, d.Sinf2, d.Sinf3, d.Sinf4, d.Sinf,
+                       (int)g.nk8, g.Nw, g.T, mu0, pmu, d.part);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, d.part, (int)(gr.x * gr.y), out);
     return hipGetLastError();
 }
 
-hipError_t launch_trace_selfG(const Grid& g, const DevArrays& d, double mu0, double mu, int which, hipStream_t s)
+hipError_t launch_trace_selfG(const Grid& g, const DevArrays& d, double mu0, double mu, int which, double* out,
+                              hipStream_t s)
 {
     const double nk02 = 4.0 * (g.nk - 1) * (g.nk - 1);
-    hipLaunchKernelGGL(k_trace_selfG, dim3(kRedBlocks), dim3(256), 0, s, d.ekw, d.wq, d.Self, d.Sinf2, d.Sinf3,
-                       d.Sinf, g.nk8, g.Nw, g.T, mu0, mu, nk02, which, d.part);
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, d.part, kRedBlocks, d.red);
+    const dim3 gr = red_grid(g);
+    hipLaunchKernelGGL(k_trace_selfG, gr, dim3(256), 0, s, d.ekw, d.wq, d.Self, d.Sinf2, d.Sinf3, d.Sinf, (int)g.nk8,
+                       g.Nw, g.T, mu0, mu, nk02, which, d.part);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, d.part, (int)(gr.x * gr.y), out);
     return hipGetLastError();
 }
 

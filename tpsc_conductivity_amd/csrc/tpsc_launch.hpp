@@ -38,6 +38,7 @@ struct DevArrays {
     double* jjpart = nullptr;    // nblk * 2*nwb  per-workgroup Lambda partials
     double* jjscal = nullptr;    // nblk * 4      per-workgroup k0, inf sums
     double* kx0part = nullptr;   // 2*nk
+    double* jjred = nullptr;     // 2*nwb + 4 reduced Lambda rows and scalars
     double* tmp = nullptr;       // scratch, >= 4*nk8
     double* part = nullptr;      // reduction partials
     double* red = nullptr;       // reduction result(s)
@@ -52,6 +53,8 @@ struct Grid {
 };
 
 constexpr int kRedBlocks = 1024;
+constexpr int kRowGroups = 32;       // frequency-row groups of the reduction grids
+constexpr int kPartCap = 8192;       // capacity of DevArrays::part
 constexpr int kJJK = 16;  // k-points per Lambda workgroup
 
 // ---- launchers; each returns hipGetLastError() of its launch
@@ -60,15 +63,18 @@ hipError_t launch_chi0_tau0(const Grid& g, const DevArrays& d, double mu0, hipSt
 hipError_t launch_chi0_slab(const Grid& g, const DevArrays& d, double mu0, hipStream_t s);
 hipError_t launch_chi0_pencil(const Grid& g, const DevArrays& d, hipStream_t s);
 hipError_t launch_chi0max(const Grid& g, const DevArrays& d, hipStream_t s);
-hipError_t launch_sum_rule(const Grid& g, const DevArrays& d, double sU, hipStream_t s);
-hipError_t launch_density(const Grid& g, const DevArrays& d, double mu, double n, hipStream_t s);
+hipError_t launch_sum_rule(const Grid& g, const DevArrays& d, double sU, double* out, hipStream_t s);
+hipError_t launch_density(const Grid& g, const DevArrays& d, double mu, double n, double* out, hipStream_t s);
+hipError_t launch_model(const Grid& g, const DevArrays& d, double tp, double tpp, double t3, hipStream_t s);
+hipError_t launch_reduce_cols(const double* part, int nrows, int ncols, double* out, hipStream_t s);
 hipError_t launch_vertex_slab(const Grid& g, const DevArrays& d, double U, double Usp, double Uch, hipStream_t s);
 hipError_t launch_vertex_pencil(const Grid& g, const DevArrays& d, double U, double Usp, double Uch, hipStream_t s);
 hipError_t launch_sigma_bnd(const Grid& g, const DevArrays& d, double mu0, double fact, hipStream_t s);
 hipError_t launch_sigma_slab(const Grid& g, const DevArrays& d, double mu0, hipStream_t s);
 hipError_t launch_sigma_pencil(const Grid& g, const DevArrays& d, hipStream_t s);
-hipError_t launch_trace_diffG(const Grid& g, const DevArrays& d, double pmu, double mu0, hipStream_t s);
-hipError_t launch_trace_selfG(const Grid& g, const DevArrays& d, double mu0, double mu, int which,
+hipError_t launch_trace_diffG(const Grid& g, const DevArrays& d, double pmu, double mu0, double* out,
+                              hipStream_t s);
+hipError_t launch_trace_selfG(const Grid& g, const DevArrays& d, double mu0, double mu, int which, double* out,
                               hipStream_t s);
 hipError_t launch_chijj(const Grid& g, const DevArrays& d, double mu0, double mu, double fact4, hipStream_t s,
                         void (*mark)(void*, int), void* mark_ctx);
