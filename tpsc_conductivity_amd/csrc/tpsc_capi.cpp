@@ -334,7 +334,7 @@ int set_grid(tpsc_ctx* c, int N, int Nw)
     g.T = c->p.T;
     hipStream_t s = c->stream;
     HIPCHK(c, launch_model(g, c->d, c->p.tp, c->p.tpp, c->p.t3, s));
-    HIPCHK(c, launch_fill_tw(c->d.twN, Nw, s));
+    HIPCHK(c, launch_fill_tw(c->d, Nw, s));
     HIPCHK(c, hipStreamSynchronize(s));
     c->have_chi0 = false;
     return find_mu0(c);
@@ -697,6 +697,10 @@ int tpsc_create(const tpsc_params* p, tpsc_ctx** out)
     rc |= dalloc(c, &d.jjred, 2 * (size_t)nwb + 4);
     rc |= dalloc(c, &d.red, 16);
     rc |= dalloc(c, &d.twN, (size_t)Nw);
+    rc |= dalloc(c, &d.twX, (size_t)Nw);
+    rc |= dalloc(c, &d.twS, (size_t)Nw);
+    rc |= dalloc(c, &d.twM, (size_t)Nw);
+    rc |= dalloc(c, &d.ph, (size_t)Nw + 1);
     rc |= dalloc(c, &d.imax, 4);
     if (rc) return TPSC_E_NOMEM;
     return set_grid(c, N, Nw);  // hamiltonien + green0 constructors

@@ -149,7 +149,7 @@ __device__ inline void reg_dft<8>(cd* v)
 // tw: LDS table tw[t] = exp(-2 pi i t / L), t < L.
 // ---------------------------------------------------------------------------------
 template <typename Real, int R, int G>
-__device__ void wave_r2r_rows(Real* base, int nrows, int rs, int es, int kind, const cd* tw)
+__device__ __forceinline__ void wave_r2r_rows(Real* base, int nrows, int rs, int es, int kind, const cd* tw)
 {
     constexpr int L = R * G;
     constexpr int P = 64 / G;
@@ -222,7 +222,7 @@ __device__ void wave_r2r_rows(Real* base, int nrows, int rs, int es, int kind, c
 }
 
 // fill tw[t] = exp(-2 pi i t / L) for t < L (block-cooperative; caller syncs)
-__device__ inline void fill_twiddles(cd* tw, int L)
+__device__ __forceinline__ void fill_twiddles(cd* tw, int L)
 {
     for (int t = threadIdx.x; t < L; t += blockDim.x) {
         double s, c;
@@ -234,7 +234,7 @@ __device__ inline void fill_twiddles(cd* tw, int L)
 // 2D transform of an n0 x n1 LDS slab (row-major, row stride ld): axis 1 then axis 0.
 // kind0 along axis 0 (slow index), kind1 along axis 1.  Both use FFT length L = N.
 template <typename Real, int N>
-__device__ void slab_r2r_2d(Real* slab, int n0, int n1, int ld, int kind0, int kind1, const cd* tw)
+__device__ __forceinline__ void slab_r2r_2d(Real* slab, int n0, int n1, int ld, int kind0, int kind1, const cd* tw)
 {
     constexpr int G = (N < 64) ? N : 64;
     constexpr int R = N / G;
@@ -250,7 +250,7 @@ __device__ void slab_r2r_2d(Real* slab, int n0, int n1, int ld, int kind0, int k
 // twg: global table exp(-2 pi i t / L), t < L.
 // ---------------------------------------------------------------------------------
 template <int L, int NT>
-__device__ void block_fft(cd* buf, int sign, const cd* __restrict__ twg)
+__device__ __forceinline__ void block_fft(cd* buf, int sign, const cd* __restrict__ twg)
 {
     constexpr int LOG2 = __builtin_ctz(L);
     constexpr int P4 = LOG2 / 2;
@@ -339,14 +339,22 @@ __device__ void block_fft(cd* buf, int sign, const cd* __restrict__ twg)
 // ---------------------------------------------------------------------------------
 constexpr int KW = 33;
 
-__device__ inline void block_spline_moments(const double* y, double* M, double* rr, int N, double h, double d0,
-                                            double dN)
+// NPMAX: largest N+1 the caller passes; NT: threads of the block.  Each thread owns a
+// chunk of Cr consecutive samples (Cr <= C, a compile-time bound, so the chunk stays in
+// registers).  The recursions are a chunked scan: local recursion from a zero state,
+// chunk-end (start) states exchanged through LDS, then the carry S from the preceding
+// chunks, S_{t-1} = sum_{d>=1} z^{Cr(d-1)} E_{t-d}, truncated once z^{Cr(d-1)} < 1e-19.
+// ES: LDS scratch of 2*NT doubles.
+template <int NPMAX, int NT>
+__device__ __forceinline__ void block_spline_moments(const double* y, double* M, double* rr, int N, double h,
+                                                     double d0, double dN, double* ES)
 {
-    const int tid = threadIdx.x, nt = blockDim.x;
+    constexpr int C = (NPMAX + NT - 1) / NT;
+    const int tid = threadIdx.x;
     const double z = 1.7320508075688772935 - 2.0;   // sqrt(3) - 2
     const double A = 0.28867513459481288225;         // 1 / (2 sqrt 3)
     const double ih = 1.0 / h;
-    for (int j = tid; j <= N; j += nt) {
+    for (int j = tid; j <= N; j += NT) {
         double r;
         if (j == 0)
             r = 6.0 * ih * ((y[1] - y[0]) * ih - d0);
@@ -357,29 +365,65 @@ __device__ inline void block_spline_moments(const double* y, double* M, double* 
         rr[j] = r;
     }
     __syncthreads();
-    // chunked causal + anti-causal recursions over interior r (r_0 = r_N = 0 here)
-    const int C = (N + 1 + nt - 1) / nt;
-    const int j0 = tid * C, j1 = min(N + 1, j0 + C);
-    double pv[20];  // C <= 20: N+1 <= 5120 with 256 threads
+    const int Cr = (N + 1 + NT - 1) / NT;
+    const int nchunk = (N + 1 + Cr - 1) / Cr;
+    const int j0 = tid * Cr, j1 = min(N + 1, j0 + Cr);
+    double ri[C], u[C], w[C];
+    double eu = 0.0, ew = 0.0;
     if (j0 < j1) {
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+            const int i = j0 + c;
+            ri[c] = (i < j1 && i >= 1 && i <= N - 1) ? rr[i] : 0.0;
+        }
         double s = 0.0;
-        for (int i = max(1, j0 - KW); i < j0; i++) s = rr[i] + z * s;
-        for (int i = j0; i < j1; i++) {
-            const double ri = (i >= 1 && i <= N - 1) ? rr[i] : 0.0;
-            s = ri + z * s;
-            pv[i - j0] = s;
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+            s = ri[c] + z * s;
+            u[c] = s;
         }
+#pragma unroll
+        for (int c = 0; c < C; c++)
+            if (c == Cr - 1) eu = u[c];
         s = 0.0;
-        for (int i = min(N - 1, j1 - 1 + KW); i >= j1; i--) s = rr[i] + z * s;
-        for (int i = j1 - 1; i >= j0; i--) {
-            const double ri = (i >= 1 && i <= N - 1) ? rr[i] : 0.0;
-            s = ri + z * s;
-            pv[i - j0] = A * (pv[i - j0] + s - ri);
+#pragma unroll
+        for (int c = C - 1; c >= 0; c--) {
+            if (j0 + c < j1) s = ri[c] + z * s;
+            w[c] = s;
         }
+        ew = s;
     }
-    __syncthreads();  // all reads of rr done before M may alias nothing; write P into M
-    if (j0 < j1)
-        for (int i = j0; i < j1; i++) M[i] = pv[i - j0];
+    ES[tid] = eu;       // causal state at the chunk end
+    ES[NT + tid] = ew;  // anti-causal state at the chunk start
+    __syncthreads();
+    if (j0 < j1) {
+        double zc = 1.0;  // z^Cr
+        for (int c = 0; c < Cr; c++) zc *= z;
+        double Su = 0.0, Sw = 0.0, zd = 1.0;
+        for (int d = 1; d <= nchunk && fabs(zd) > 1.0e-19; d++) {
+            if (tid - d >= 0) Su += zd * ES[tid - d];
+            if (tid + d < nchunk) Sw += zd * ES[NT + tid + d];
+            zd *= zc;
+        }
+        double zp = z;
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+            u[c] += zp * Su;
+            zp *= z;
+        }
+        // anti-causal carry enters from the right end of the chunk
+        zp = z;
+#pragma unroll
+        for (int c = C - 1; c >= 0; c--) {
+            if (j0 + c < j1) {
+                w[c] += zp * Sw;
+                zp *= z;
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < C; c++)
+            if (j0 + c < j1) M[j0 + c] = A * (u[c] + w[c] - ri[c]);
+    }
     __syncthreads();
     // boundary modes
     const double R0 = rr[0] - 2.0 * M[0] - M[1];
@@ -399,7 +443,7 @@ __device__ inline void block_spline_moments(const double* y, double* M, double* 
     const double alpha = (dg * R0 - off * RN) / det;
     const double beta = (dg * RN - off * R0) / det;
     __syncthreads();
-    for (int j = tid; j <= N; j += nt) {
+    for (int j = tid; j <= N; j += NT) {
         double add = 0.0;
         if (j <= 2 * KW) {
             double zp = 1.0;
@@ -425,6 +469,28 @@ __device__ inline double wave_sum(double v)
     for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
 }
+// block-wide sums of K values at once; every thread gets the results.
+// scratch: LDS double[>= K * nwaves]
+template <int K>
+__device__ __forceinline__ void block_sum_k(double* v, double* scratch)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+    for (int k = 0; k < K; k++) v[k] = wave_sum(v[k]);
+    __syncthreads();
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < K; k++) scratch[k * nw + wave] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        double s = 0.0;
+        for (int i = 0; i < nw; i++) s += scratch[k * nw + i];
+        v[k] = s;
+    }
+    __syncthreads();
+}
+
 // block-wide sum; every thread gets the result. scratch: LDS double[>= nwaves]
 __device__ inline double block_sum(double v, double* scratch)
 {

@@ -21,6 +21,7 @@ namespace tpsc {
 
 #define SLAB_THREADS 1024
 #define PEN_THREADS 256
+#define JJ_THREADS 512
 
 // ================================================================== small helpers
 __global__ void k_fill_tw(cd* tw, int L)
@@ -44,7 +45,7 @@ __global__ void k_finalize(const double* __restrict__ part, int n, double* out)
 }
 
 // wedge (l,m) of a packed index, m <= l
-__device__ inline void wedge_lm(long idx, int& l, int& m)
+__device__ __forceinline__ void wedge_lm(long idx, int& l, int& m)
 {
     int L = (int)((sqrt(8.0 * (double)idx + 1.0) - 1.0) * 0.5);
     while (widx(L + 1, 0) <= idx) L++;
@@ -54,7 +55,7 @@ __device__ inline void wedge_lm(long idx, int& l, int& m)
 }
 
 // write a wedge value to both symmetric positions of an nk x nk LDS slab
-__device__ inline void put_sym(double* slab, int nk, int l, int m, double v)
+__device__ __forceinline__ void put_sym(double* slab, int nk, int l, int m, double v)
 {
     slab[m + l * nk] = v;
     slab[l + m * nk] = v;
@@ -62,14 +63,14 @@ __device__ inline void put_sym(double* slab, int nk, int l, int m, double v)
 
 // ================================================================== chi0 slabs
 // G(k,t)/N^2, chi0.cpp:468-471 (form==0) and :535-538 (form==1, tau = beta/2)
-__device__ inline double g0_plus(double x, double t, double T, double nk02, int form)
+__device__ __forceinline__ double g0_plus(double x, double t, double T, double nk02, int form)
 {
     if (x >= 0.0) return exp(-t * x) * (-1.0 / (exp(-x / T) + 1.0)) / nk02;
     if (form == 0) return -exp((1.0 / T - t) * x) / (exp(x / T) + 1.0) / nk02;
     return -exp((1.0 / T - t) * x) / ((exp(x / T) + 1.0) * nk02);
 }
 // G(k,-t)/N^2, chi0.cpp:486-489
-__device__ inline double g0_minus(double x, double t, double T, double nk02)
+__device__ __forceinline__ double g0_minus(double x, double t, double T, double nk02)
 {
     if (x >= 0.0) return -exp((t - 1.0 / T) * x) * (-1.0 / (exp(-x / T) + 1.0)) / nk02;
     return exp(t * x) / ((exp(x / T) + 1.0) * nk02);
@@ -253,6 +254,7 @@ __global__ void __launch_bounds__(PEN_THREADS)
     constexpr int NS = L / 2;  // intervals
     __shared__ double y[NS + 1], M[NS + 1];
     __shared__ cd X[L];
+    __shared__ double ES[2 * PEN_THREADS];
     __shared__ double sc[16];
     const long q = xcd_remap(blockIdx.x, gridDim.x);
     if (q >= nk8) return;
@@ -262,7 +264,7 @@ __global__ void __launch_bounds__(PEN_THREADS)
     const double d0 = dchiqt0[q];
     const double h = 1.0 / (L * T);
     __syncthreads();
-    block_spline_moments(y, M, rr, NS, h, d0, 0.0);
+    block_spline_moments<NS + 1, PEN_THREADS>(y, M, rr, NS, h, d0, 0.0, ES);
     double integ = 0.0;
     for (int j = tid; j < NS; j += nt) {
         const double a6 = (M[j + 1] - M[j]) / h;
@@ -529,10 +531,12 @@ __global__ void __launch_bounds__(SLAB_THREADS)
 template <int L>
 __global__ void __launch_bounds__(PEN_THREADS)
     k_sigma_pencil(const double* __restrict__ Stau, const double* __restrict__ dSig, long nk8, double T, cd* Self,
-                   double* Sinf2, double* Sinf3, double* Sinf4, double* Sinf, const cd* __restrict__ tw)
+                   double* Sinf2, double* Sinf3, double* Sinf4, double* Sinf, const cd* __restrict__ tw,
+                   const cd* __restrict__ twX, const cd* __restrict__ twS)
 {
     __shared__ double y[L + 1], M[L + 1];
     __shared__ cd X[L];
+    __shared__ double ES[2 * PEN_THREADS];
     const long k = xcd_remap(blockIdx.x, gridDim.x);
     if (k >= nk8) return;
     const int tid = threadIdx.x, nt = blockDim.x;
@@ -541,17 +545,15 @@ __global__ void __launch_bounds__(PEN_THREADS)
     const double d0 = dSig[k], dN = dSig[k + nk8];
     const double h = 1.0 / (L * T);
     __syncthreads();
-    block_spline_moments(y, M, rr, L, h, d0, dN);
+    block_spline_moments<L + 1, PEN_THREADS>(y, M, rr, L, h, d0, dN, ES);
     for (int j = tid; j < L; j += nt) {
         const double a6 = (M[j + 1] - M[j]) / h;
-        const cd ph = cis(j * kPi / L);
-        X[j] = ph * a6;
+        X[j] = twX[j] * a6;
     }
     __syncthreads();
     block_fft<L, PEN_THREADS>(X, +1, tw);
     for (int n = tid; n < L / 2; n += nt) {
-        const cd e = cis((2 * n + 1) * kPi / L);
-        const cd R = cd{1.0 - e.x, -e.y} * X[n];
+        const cd R = twS[n] * X[n];
         Self[k + (long)n * nk8] = R;
         if (n == L / 2 - 1) Sinf4[k] = R.x;
     }
@@ -562,8 +564,28 @@ __global__ void __launch_bounds__(PEN_THREADS)
     }
 }
 
+// phase tables of the pencil transforms, evaluated from the same double arguments as the
+// reference: twX[j] = exp(i j pi/L) (green.cpp:1957, cond_opt.cpp:3290); twS[j] = 1 -
+// exp(i a_j pi/L) with a_j = 2j+1 (j < L/2) or 2(j-L)+1 (green.cpp:2006,
+// cond_opt.cpp:3305-3310); twM[n] = 1 - exp(-2 n i pi/L) (cond_opt.cpp:3731,3805);
+// ph[j] = exp(i (j pi (L-1))/L) (cond_opt.cpp:3635), ph[L] = exp(i pi (L-1)) (:3693)
+__global__ void k_fill_tables(int L, cd* twX, cd* twS, cd* twM, cd* ph)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < L) {
+        twX[j] = cis(j * kPi / L);
+        const double a = (j < L / 2) ? (2 * j + 1) * kPi / L : (2 * (j - L) + 1) * kPi / L;
+        const cd e = cis(a);
+        twS[j] = cd{1.0 - e.x, -e.y};
+        const cd em = cis(-2 * j * kPi / L);
+        twM[j] = cd{1.0 - em.x, -em.y};
+        ph[j] = cis((j * kPi * (L - 1)) / L);
+    }
+    if (j == L) ph[L] = cis(kPi * (L - 1));
+}
+
 // ================================================================== traces
-__device__ inline cd sigma_mom(double Sinf, double s2, double s3, cd R, double wn)
+__device__ __forceinline__ cd sigma_mom(double Sinf, double s2, double s3, cd R, double wn)
 {
     // Sinf/z + s2/z^2 + s3/z^3 + R/z^4, z = i wn
     const double w2 = wn * wn, w3 = w2 * wn, w4 = w2 * w2;
@@ -831,7 +853,7 @@ struct Poles {
 };
 
 // roots of z^2 + (mu - e) z - Sigma_inf with Newton polish (generique.cpp:1666-1726)
-__device__ inline void roots2(cd A, cd B, cd Cc, cd* roots)
+__device__ __forceinline__ void roots2(cd A, cd B, cd Cc, cd* roots)
 {
     if (!(cabs2(A) > 1.0e-15)) return;
     const cd Rr = csqrtd(B * B - 4.0 * A * Cc);
@@ -860,7 +882,7 @@ __device__ inline void roots2(cd A, cd B, cd Cc, cd* roots)
     }
 }
 
-__device__ inline Poles make_poles(double e, double mu, double Sinf, double CJ1, double T)
+__device__ __forceinline__ Poles make_poles(double e, double mu, double Sinf, double CJ1, double T)
 {
     Poles P;
     cd roots[2] = {cd{0, 0}, cd{0, 0}};
@@ -878,7 +900,7 @@ __device__ inline Poles make_poles(double e, double mu, double Sinf, double CJ1,
 
 // Dyson G at Matsubara slot s (n = s - Nw/2), with the conjugate Sigma for n < 0
 // (cond_opt.cpp:3354-3407)
-__device__ inline cd dyson_G(const cd* __restrict__ Self, long ki, long nk8, int s, int H, double C2, double C3,
+__device__ __forceinline__ cd dyson_G(const cd* __restrict__ Self, long ki, long nk8, int s, int H, double C2, double C3,
                              double Sinf, double e, double mu, double T, double& kn)
 {
     kn = (2.0 * (s - H) + 1) * kPi * T;
@@ -897,86 +919,72 @@ __device__ inline cd dyson_G(const cd* __restrict__ Self, long ki, long nk8, int
     return T / den;
 }
 
-// J(ikn) at slot s from the spline transform F of hV or f0V (cond_opt.cpp:3302-3311)
-__device__ inline cd j_slot(const cd* X, int s, int L, double T, double cj1, double cj2, double cj3)
+// J(ikn) at slot s (n = s - L/2) from the spline transform F of hV or f0V
+// (cond_opt.cpp:3302-3311): -i cj1/kn - cj2/kn^2 + i cj3/kn^3 + twS F/kn^4
+__device__ __forceinline__ cd j_slot(const cd* X, const cd* __restrict__ twS, int s, int L, double T, double cj1,
+                                     double cj2, double cj3)
 {
     const int H = L / 2;
-    int j;
-    double kn, ang;
-    if (s >= H) {
-        j = s - H;
-        kn = (2 * j + 1) * kPi * T;
-        ang = (2 * j + 1) * kPi / L;
-    } else {
-        j = s + H;
-        kn = (2 * (j - L) + 1) * kPi * T;
-        ang = (2 * (j - L) + 1) * kPi / L;
-    }
-    const cd e = cis(ang);
-    const cd a = cd{1.0 - e.x, -e.y} * X[j];
+    const int j = (s >= H) ? s - H : s + H;
+    const double kn = (j < H) ? (2 * j + 1) * kPi * T : (2 * (j - L) + 1) * kPi * T;
+    const cd a = twS[j] * X[j];
     const double k2 = kn * kn, k3 = k2 * kn, k4 = k2 * k2;
-    // -i cj1/kn - cj2/kn^2 + i cj3/kn^3 + a/kn^4
     return cd{-cj2 / k2 + a.x / k4, -cj1 / kn + cj3 / k3 + a.y / k4};
 }
 
-// spline of a tau-pencil held in y (N intervals, step h) and twiddled DFT of 6a_j into X
-template <int L>
-__device__ inline void spline_to_X(double* y, double* M, cd* X, int NSint, double h, double d0, double dN,
-                                   int mode)
+// spline of a tau-pencil held in y (NSint intervals, step h) and the DFT input into X:
+// mode 0: X_j = exp(i pi j/L) 6a_j (j < L); mode 1: X_j = 6a_j (j < L);
+// mode 2: X_j = 6a_j for j < L/2, X_{L-1-j} = -X_j
+template <int L, int NT>
+__device__ __forceinline__ void spline_to_X(double* y, double* M, cd* X, int NSint, double h, double d0, double dN,
+                                            int mode, const cd* __restrict__ twX, double* ES)
 {
-    // mode 0: X_j = exp(i pi j/L) 6a_j (j < L); mode 1: X_j = 6a_j (j < L);
-    // mode 2: X_j = 6a_j for j < L/2, X_{L-1-j} = -X_j
     double* rr = (double*)X;
-    block_spline_moments(y, M, rr, NSint, h, d0, dN);
-    const int tid = threadIdx.x, nt = blockDim.x;
+    block_spline_moments<L + 1, NT>(y, M, rr, NSint, h, d0, dN, ES);
+    const int tid = threadIdx.x;
     if (mode == 2) {
-        for (int j = tid; j < L / 2; j += nt) {
+        for (int j = tid; j < L / 2; j += NT) {
             const double a6 = (M[j + 1] - M[j]) / h;
             X[j] = cd{a6, 0.0};
             X[L - 1 - j] = cd{-a6, 0.0};
         }
     } else {
-        for (int j = tid; j < L; j += nt) {
+        for (int j = tid; j < L; j += NT) {
             const double a6 = (M[j + 1] - M[j]) / h;
-            X[j] = (mode == 0) ? cis(j * kPi / L) * a6 : cd{a6, 0.0};
+            X[j] = (mode == 0) ? twX[j] * a6 : cd{a6, 0.0};
         }
     }
     __syncthreads();
 }
 
-__device__ inline cd block_sum_cd(cd v, double* sc)
-{
-    const double a = block_sum(v.x, sc);
-    const double b = block_sum(v.y, sc);
-    return cd{a, b};
-}
-
-// per-k Lambda_jj contributions (cond_opt.cpp:3179-3822).  Each workgroup owns kJJK
-// consecutive Lambda-grid points and accumulates into its own partial rows
-// part[blk*2*nwb + n] (bubble) and part[blk*2*nwb + nwb + n] (MT).
-template <int L>
-__global__ void __launch_bounds__(PEN_THREADS)
+// per-k Lambda_jj contributions (cond_opt.cpp:3179-3822).  One workgroup of NT threads per
+// Lambda-grid point at a time; each workgroup owns kJJK consecutive points and accumulates
+// into its own partial rows part[blk*2*nwb + n] (bubble) and part[blk*2*nwb + nwb + n] (MT).
+template <int L, int NT>
+__global__ void __launch_bounds__(NT)
     k_jj_pencil(const double* __restrict__ ek, const double* __restrict__ dek, const double* __restrict__ d2ek,
                 const double* __restrict__ h, const double* __restrict__ f0, const double* __restrict__ dhV,
                 const double* __restrict__ df0V, const cd* __restrict__ Self, const double* __restrict__ Sinf2,
                 const double* __restrict__ Sinf3, const double* __restrict__ Sinfp, int N, double T, double mu,
-                const cd* __restrict__ tw, double* part, double* scal)
+                const cd* __restrict__ tw, const cd* __restrict__ twX, const cd* __restrict__ twS,
+                const cd* __restrict__ twM, const cd* __restrict__ phT, double* part, double* scal)
 {
     constexpr int H = L / 2;
     constexpr int NWB = H + 1;
     __shared__ double y[L + 1], M[L + 1];
     __shared__ cd X[L], Gb[L], Jb[L];
-    __shared__ double sc[16];
+    __shared__ double sc[8 * (NT / 64)];
+    __shared__ double ES[2 * NT];
     const int nk = N / 2 + 1;
     const long nq = (long)nk * (nk - 2), nk8 = (long)nk * (nk + 1) / 2;
     const double nf = 1.0 / ((double)N * N);
     const double Sinf = Sinfp[0];
     const double hstep = 1.0 / (L * T);
-    const int tid = threadIdx.x, nt = blockDim.x;
+    const int tid = threadIdx.x;
     const int blk = xcd_remap(blockIdx.x, gridDim.x);
     double* c1 = part + (long)blk * 2 * NWB;
     double* c2 = c1 + NWB;
-    for (int n = tid; n < NWB; n += nt) {
+    for (int n = tid; n < NWB; n += NT) {
         c1[n] = 0.0;
         c2[n] = 0.0;
     }
@@ -991,37 +999,39 @@ __global__ void __launch_bounds__(PEN_THREADS)
         const double v = dek[q];
         double wt = 4;
         if (m == 0 || m == nk - 1) wt = wt / 2;
-        // (1) Dyson G for all slots
-        for (int s = tid; s < L; s += nt) {
+        // (1) Dyson G for all slots, and the f0V pencil
+        for (int s = tid; s < L; s += NT) {
             double kn;
             Gb[s] = dyson_G(Self, ki, nk8, s, H, C2, C3, Sinf, e, mu, T, kn);
         }
-        // (2) f0V spline -> J0, accumulate chijj2(0) = sum G^2 J0 / T
-        for (int j = tid; j <= L; j += nt) y[j] = f0[q + (long)j * nq];
+        for (int j = tid; j <= L; j += NT) y[j] = f0[q + (long)j * nq];
         const double fd0 = df0V[q], fdN = df0V[q + nq];
         __syncthreads();
         const double cj01 = -y[0] - y[L], cj02 = fd0 + fdN;
-        spline_to_X<L>(y, M, X, L, hstep, fd0, fdN, 0);
+        spline_to_X<L, NT>(y, M, X, L, hstep, fd0, fdN, 0, twX, ES);
         const double cj03 = -(M[0] + M[L]);
-        block_fft<L, PEN_THREADS>(X, +1, tw);
-        cd qn0 = cd{0.0, 0.0};
-        for (int s = tid; s < L; s += nt) {
-            const cd J0 = j_slot(X, s, L, T, cj01, cj02, cj03);
-            qn0 = qn0 + Gb[s] * Gb[s] * J0 / T;
+        block_fft<L, NT>(X, +1, tw);
+        // (2) chijj2(iqn=0) = sum G^2 J0 / T
+        double r2[2] = {0.0, 0.0};
+        for (int s = tid; s < L; s += NT) {
+            const cd J0 = j_slot(X, twS, s, L, T, cj01, cj02, cj03);
+            const cd t = Gb[s] * Gb[s] * J0 / T;
+            r2[0] += t.x;
+            r2[1] += t.y;
         }
-        qn0 = block_sum_cd(qn0, sc);
-        // (3) hV spline -> J, products and asymptote subtraction
-        for (int j = tid; j <= L; j += nt) y[j] = h[q + (long)j * nq];
+        block_sum_k<2>(r2, sc);
+        const double qn0 = r2[0];
+        // (3) hV pencil -> J, products and asymptote subtraction
+        for (int j = tid; j <= L; j += NT) y[j] = h[q + (long)j * nq];
         const double hd0 = dhV[q], hdN = dhV[q + nq];
         __syncthreads();
         const double CJ1 = -y[0] - y[L], cj2 = hd0 + hdN;
-        spline_to_X<L>(y, M, X, L, hstep, hd0, hdN, 0);
+        spline_to_X<L, NT>(y, M, X, L, hstep, hd0, hdN, 0, twX, ES);
         const double cj3 = -(M[0] + M[L]);
-        block_fft<L, PEN_THREADS>(X, +1, tw);
-        cd k0 = cd{0.0, 0.0};
-        double dGp = 0.0, dJ0 = 0.0;
-        for (int s = tid; s < L; s += nt) {
-            const cd Jv = j_slot(X, s, L, T, CJ1, cj2, cj3);
+        block_fft<L, NT>(X, +1, tw);
+        double r4[4] = {0.0, 0.0, 0.0, 0.0};  // Re k0, Im k0, dGp, dJ0
+        for (int s = tid; s < L; s += NT) {
+            const cd Jv = j_slot(X, twS, s, L, T, CJ1, cj2, cj3);
             const double kn = (2.0 * (s - H) + 1) * kPi * T;
             const cd Gv = Gb[s];
             const cd G2 = T / cd{-e + mu, kn + Sinf / kn};
@@ -1030,13 +1040,13 @@ __global__ void __launch_bounds__(PEN_THREADS)
             const cd Gn = Gv - G2;
             Jb[s] = Jn;
             Gb[s] = Gn;
-            k0 = k0 + Gn;
-            dGp += kn * Gn.y;
-            dJ0 -= kn * Jn.y;
+            r4[0] += Gn.x;
+            r4[1] += Gn.y;
+            r4[2] += kn * Gn.y;
+            r4[3] -= kn * Jn.y;
         }
-        k0 = block_sum_cd(k0, sc);
-        dGp = block_sum(dGp, sc);
-        dJ0 = block_sum(dJ0, sc);
+        block_sum_k<4>(r4, sc);
+        double dGp = r4[2], dJ0 = r4[3];
         // (4) two-pole asymptote
         const Poles P = make_poles(e, mu, Sinf, CJ1, T);
         cd f1p, f1m, fj1, fj1b, f2p, f2m, fj2, fj2b;
@@ -1066,7 +1076,7 @@ __global__ void __launch_bounds__(PEN_THREADS)
             fj2 = P.r2 * P.b2 / P.den2;
             fj2b = P.r2 * P.b2 * ex / P.den2;
         }
-        const double k0r = k0.x + (f1m + f2m).x;
+        const double k0r = r4[0] + (f1m + f2m).x;
         double dGm = -dGp;
         dGp += (P.r1 * f1p + P.r2 * f2p).x;
         dGm += (P.r1 * f1m + P.r2 * f2m).x;
@@ -1074,11 +1084,11 @@ __global__ void __launch_bounds__(PEN_THREADS)
         dJ0 += (fj1 + fj2).x;
         dJb += (fj1b + fj2b).x;
         // (5) to imaginary time
-        block_fft<L, PEN_THREADS>(Gb, -1, tw);
-        block_fft<L, PEN_THREADS>(Jb, +1, tw);
+        block_fft<L, NT>(Gb, -1, tw);
+        block_fft<L, NT>(Jb, +1, tw);
         const cd Gbeta0 = Gb[0], Jbeta0 = Jb[0];
         __syncthreads();
-        for (int j = tid; j < L; j += nt) {
+        for (int j = tid; j < L; j += NT) {
             const double t = j / (L * T);
             cd e1, ej1, e2, ej2;
             if (P.r1.x > 0) {
@@ -1097,60 +1107,44 @@ __global__ void __launch_bounds__(PEN_THREADS)
             }
             const double G0t = (-(P.a1 * e1 / P.den1) - P.a2 * e2 / P.den2).x;
             const double Jinf = (P.b1 * ej1 / P.den1 + P.b2 * ej2 / P.den2).x;
-            const cd ph = cis((j * kPi * (L - 1)) / L);
+            const cd ph = phT[j];
             Gb[j] = ph * Gb[j] + cd{G0t, 0.0};
             Jb[j] = Jb[j] / ph + cd{Jinf, 0.0};
         }
-        cd Gbeta, Jbeta;
+        cd Jbeta;
         {
             const double t = 1.0 / T;
-            cd e1, ej1, e2, ej2;
-            if (P.r1.x > 0) {
-                e1 = cexpd(-t * P.r1);
-                ej1 = cd{1.0, 0.0};
-            } else {
-                e1 = cd{1.0, 0.0};
-                ej1 = cexpd(P.r1 * t);
-            }
-            if (P.r2.x > 0) {
-                e2 = cexpd(-t * P.r2);
-                ej2 = cd{1.0, 0.0};
-            } else {
-                e2 = cd{1.0, 0.0};
-                ej2 = cexpd(P.r2 * t);
-            }
-            const double G0t = (-(P.a1 * e1 / P.den1) - P.a2 * e2 / P.den2).x;
+            cd ej1, ej2;
+            ej1 = (P.r1.x > 0) ? cd{1.0, 0.0} : cexpd(P.r1 * t);
+            ej2 = (P.r2.x > 0) ? cd{1.0, 0.0} : cexpd(P.r2 * t);
             const double Jinf = (P.b1 * ej1 / P.den1 + P.b2 * ej2 / P.den2).x;
-            const cd ph = cis(kPi * (L - 1));
-            Gbeta = ph * Gbeta0 + cd{G0t, 0.0};
-            Jbeta = Jbeta0 / ph + cd{Jinf, 0.0};
+            Jbeta = Jbeta0 / phT[L] + cd{Jinf, 0.0};
         }
-        (void)Gbeta;
+        (void)Gbeta0;
         __syncthreads();
         const cd G0 = Gb[0], J0t = Jb[0];
         // (6) G(tau) J(-tau) -> Maki-Thompson
-        for (int j = tid; j < L; j += nt) y[j] = (Gb[j] * Jb[j]).x;
+        for (int j = tid; j < L; j += NT) y[j] = (Gb[j] * Jb[j]).x;
         if (tid == 0) y[L] = ((-G0 - cd{1.0, 0.0}) * Jbeta).x;
         __syncthreads();
         const double gd0 = (dGp * J0t + G0 * dJ0).x;
         const double gdN = (dGm * Jbeta - (G0 + cd{1.0, 0.0}) * dJb).x;
         const double MJ1 = y[0] - y[L];
-        spline_to_X<L>(y, M, X, L, hstep, gd0, gdN, 1);
+        spline_to_X<L, NT>(y, M, X, L, hstep, gd0, gdN, 1, twX, ES);
         const double MJ3 = M[0] - M[L];
-        block_fft<L, PEN_THREADS>(X, -1, tw);
-        for (int n = 1 + tid; n <= H; n += nt) {
+        block_fft<L, NT>(X, -1, tw);
+        for (int n = 1 + tid; n <= H; n += NT) {
             const double wn = 2 * n * kPi * T;
-            const cd ee = cis(-2 * n * kPi / L);
-            const cd a = cd{1.0 - ee.x, -ee.y} * X[n];
+            const cd a = twM[n] * X[n];
             const double w4 = wn * wn * wn * wn;
             const double imJ = -2.0 * MJ1 / wn + 2.0 * MJ3 / (wn * wn * wn) + 2.0 * (a.y / w4);
             c2[n] -= nf * wt * v * imJ / wn;
         }
-        if (tid == 0) c2[0] -= nf * wt * v * qn0.x;
+        if (tid == 0) c2[0] -= nf * wt * v * qn0;
         __syncthreads();
         // (7) -G(tau) G(-tau) -> bubble
         const double dchi = dGp * (G0.x + 1.0) + G0.x * dGm;
-        for (int j = tid; j <= H; j += nt) {
+        for (int j = tid; j <= H; j += NT) {
             cd gg;
             if (j == 0)
                 gg = G0 * (G0 + cd{1.0, 0.0});
@@ -1159,20 +1153,19 @@ __global__ void __launch_bounds__(PEN_THREADS)
             y[j] = gg.x;
         }
         __syncthreads();
-        spline_to_X<L>(y, M, X, H, hstep, dchi, 0.0, 2);
-        double integ = 0.0;
-        for (int j = tid; j < H; j += nt)
-            integ += hstep * (y[j] + y[j + 1]) * 0.5 - hstep * hstep * hstep * (M[j] + M[j + 1]) / 24.0;
-        integ = block_sum(integ, sc);
-        block_fft<L, PEN_THREADS>(X, -1, tw);
-        for (int n = tid; n <= H; n += nt) {
+        spline_to_X<L, NT>(y, M, X, H, hstep, dchi, 0.0, 2, twX, ES);
+        double integ[1] = {0.0};
+        for (int j = tid; j < H; j += NT)
+            integ[0] += hstep * (y[j] + y[j + 1]) * 0.5 - hstep * hstep * hstep * (M[j] + M[j + 1]) / 24.0;
+        block_sum_k<1>(integ, sc);
+        block_fft<L, NT>(X, -1, tw);
+        for (int n = tid; n <= H; n += NT) {
             double bub;
             if (n == 0)
-                bub = 2 * integ;
+                bub = 2 * integ[0];
             else {
                 const double wn = 2 * n * kPi * T;
-                const cd ee = cis(-2 * n * kPi / L);
-                const cd a = cd{1.0 - ee.x, -ee.y} * X[n];
+                const cd a = twM[n] * X[n];
                 bub = -2.0 * dchi / (wn * wn) + a.x / (wn * wn * wn * wn);
             }
             c1[n] -= 2.0 * nf * wt * v * v * bub;
@@ -1212,7 +1205,9 @@ __global__ void k_kx0_rows(const double* __restrict__ ek, const double* __restri
         const cd G0v = T / cd{-e + mu, kn + Sinf / kn};
         k0 = k0 + (Gv - G0v);
     }
-    k0 = block_sum_cd(k0, sc);
+    double kv[2] = {k0.x, k0.y};
+    block_sum_k<2>(kv, sc);
+    k0 = cd{kv[0], kv[1]};
     if (threadIdx.x == 0) {
         const Poles P = make_poles(e, mu, Sinf, 0.0, T);
         const cd f1m = (P.r1.x > 0) ? P.a1 * cexpd(-(P.r1 / T)) / P.den1 : P.a1 / P.den1;
@@ -1283,9 +1278,10 @@ static void set_lds(K kern, size_t bytes)
     hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
-hipError_t launch_fill_tw(cd* tw, int L, hipStream_t s)
+hipError_t launch_fill_tw(const DevArrays& d, int L, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_fill_tw, dim3((L + 255) / 256), dim3(256), 0, s, tw, L);
+    hipLaunchKernelGGL(k_fill_tw, dim3((L + 255) / 256), dim3(256), 0, s, d.twN, L);
+    hipLaunchKernelGGL(k_fill_tables, dim3((L + 256) / 256), dim3(256), 0, s, L, d.twX, d.twS, d.twM, d.ph);
     return hipGetLastError();
 }
 
@@ -1413,7 +1409,7 @@ hipError_t launch_sigma_pencil(const Grid& g, const DevArrays& d, hipStream_t s)
 {
     DISPATCH_L(g.Nw, {
         hipLaunchKernelGGL(k_sigma_pencil<LL>, dim3((unsigned)g.nk8), dim3(PEN_THREADS), 0, s, d.Stau, d.dSig, g.nk8,
-                           g.T, d.Self, d.Sinf2, d.Sinf3, d.Sinf4, d.Sinf, d.twN);
+                           g.T, d.Self, d.Sinf2, d.Sinf3, d.Sinf4, d.Sinf, d.twN, d.twX, d.twS);
     });
     return hipGetLastError();
 }
@@ -1459,8 +1455,9 @@ hipError_t launch_chijj(const Grid& g, const DevArrays& d, double mu0, double mu
     if (mark) mark(mark_ctx, 3);
     const unsigned nblk = (unsigned)((g.nq + kJJK - 1) / kJJK);
     DISPATCH_LJJ(g.Nw, {
-        hipLaunchKernelGGL(k_jj_pencil<LL>, dim3(nblk), dim3(PEN_THREADS), 0, s, d.ek, d.dek, d.d2ek, d.h, d.f0,
-                           d.dhV, d.df0V, d.Self, d.Sinf2, d.Sinf3, d.Sinf, g.N, g.T, mu, d.twN, d.jjpart, d.jjscal);
+        hipLaunchKernelGGL((k_jj_pencil<LL, JJ_THREADS>), dim3(nblk), dim3(JJ_THREADS), 0, s, d.ek, d.dek, d.d2ek,
+                           d.h, d.f0, d.dhV, d.df0V, d.Self, d.Sinf2, d.Sinf3, d.Sinf, g.N, g.T, mu, d.twN, d.twX,
+                           d.twS, d.twM, d.ph, d.jjpart, d.jjscal);
     });
     if (mark) mark(mark_ctx, 4);
     hipLaunchKernelGGL(k_kx0_rows, dim3(2 * g.nk), dim3(256), 0, s, d.ek, d.d2ek, d.Self, d.Sinf2, d.Sinf3, d.Sinf,

@@ -43,6 +43,10 @@ struct DevArrays {
     double* part = nullptr;      // reduction partials
     double* red = nullptr;       // reduction result(s)
     cd* twN = nullptr;           // Nw   exp(-2 pi i t/Nw)
+    cd* twX = nullptr;           // Nw   phase tables of the pencil transforms (k_fill_tables)
+    cd* twS = nullptr;
+    cd* twM = nullptr;
+    cd* ph = nullptr;            // Nw+1
     int* imax = nullptr;         // argmax result
 };
 
@@ -58,7 +62,7 @@ constexpr int kPartCap = 8192;       // capacity of DevArrays::part
 constexpr int kJJK = 16;  // k-points per Lambda workgroup
 
 // ---- launchers; each returns hipGetLastError() of its launch
-hipError_t launch_fill_tw(cd* tw, int L, hipStream_t s);
+hipError_t launch_fill_tw(const DevArrays& d, int L, hipStream_t s);
 hipError_t launch_chi0_tau0(const Grid& g, const DevArrays& d, double mu0, hipStream_t s);
 hipError_t launch_chi0_slab(const Grid& g, const DevArrays& d, double mu0, hipStream_t s);
 hipError_t launch_chi0_pencil(const Grid& g, const DevArrays& d, hipStream_t s);
