@@ -42,8 +42,17 @@ typedef struct tpsc_params {
     double tp, tpp, t3, U;
     int coarse_pass; /* 1: find_Usp_Uch() runs the (N/2, Nw/2) pass first, as the
                         reference does (chi.cpp:285-310); 0: full grid only      */
-    int precision;   /* 0: FP64 (only value accepted by this build)             */
+    int precision;   /* TPSC_PREC_FP64 or TPSC_PREC_MIXED (no reference counterpart:
+                        the reference computes in FP64 only)                     */
 } tpsc_params;
+
+/* precision values.  MIXED: the slab transforms (2D DCT/DST per tau or iqn slice) and their
+ * pointwise products run in FP32 and the tau/iqn arrays (chi0, chi0(r,tau), V, Sigma(k,tau),
+ * h, f0) are stored in FP32; the tau = 0 / beta boundary data, spline moments, tail moments,
+ * pencil Fourier sums, Usp/Uch/mu root reductions and the Lambda_jj Matsubara sums stay in
+ * FP64, and Sigma(k,iwn) is stored assembled in FP32 complex.  tpsc_get widens to FP64. */
+#define TPSC_PREC_FP64 0
+#define TPSC_PREC_MIXED 1
 
 typedef struct tpsc_ctx tpsc_ctx;
 
@@ -64,7 +73,10 @@ enum tpsc_field {
     TPSC_F_CHIJJ1 = 13,   /* bubble Lambda_jj1(iqn), nwb                (cond_opt.cpp:3784-3817) */
     TPSC_F_CHIJJ2 = 14,   /* Maki-Thompson Lambda_jj2(iqn), nwb         (cond_opt.cpp:3790-3807) */
     TPSC_F_DTAU_SIGMA = 15, /* 2*nk8                                    (green.cpp:1743-1817)   */
-    TPSC_F_CHI0_TAU = 16  /* chi0(q,tau) before the tau->iqn transform (kept only after tpsc_chi0) */
+    TPSC_F_CHI0_TAU = 16, /* chi0(q,tau) before the tau->iqn transform (kept only after tpsc_chi0) */
+    TPSC_F_SIGMA_KW = 17  /* Sigma(k,iwn), n = 0..Nw/2-1, (Nw/2)*nk8 complex interleaved re/im:
+                             Sigma_inf/z + C2/z^2 + C3/z^3 + R/z^4, z = iwn (green.cpp:3039);
+                             the only Sigma(k,iwn) field of TPSC_PREC_MIXED                  */
 };
 enum tpsc_scalar {
     TPSC_S_MU0 = 1, TPSC_S_CHI0MAX, TPSC_S_QMAX_L, TPSC_S_QMAX_M, TPSC_S_USP, TPSC_S_UCH,

@@ -17,8 +17,11 @@ ERRORS = {-1: "TPSC_E_INVALID", -2: "TPSC_E_DEVICE", -3: "TPSC_E_NOMEM", -4: "TP
 FIELDS = {
     "ek": 1, "chi0": 2, "chirt": 3, "chi0_inf": 4, "chi0_inf2": 5, "dtau_chirt0": 6, "V": 7,
     "Sigma_tau": 8, "Self": 9, "Sigma_inf2": 10, "Sigma_inf3": 11, "Sigma_inf4": 12,
-    "chijj1": 13, "chijj2": 14, "dtau_Sigma": 15, "chi0_tau": 16,
+    "chijj1": 13, "chijj2": 14, "dtau_Sigma": 15, "chi0_tau": 16, "Sigma_kw": 17,
 }
+PREC_FP64 = 0
+PREC_MIXED = 1
+PRECISIONS = {"fp64": PREC_FP64, "mixed": PREC_MIXED}
 SCALARS = {
     "mu0": 1, "chi0max": 2, "qmax_l": 3, "qmax_m": 4, "Usp": 5, "Uch": 6, "dblOcc": 7,
     "difRelsp": 8, "difRelch": 9, "Sigma_inf": 10, "trSG": 11, "trSG_corr": 12, "mu": 13,
@@ -152,6 +155,6 @@ class Context:
         nbytes = lib().tpsc_field_size(self._h, f)
         out = np.empty(nbytes // 8, dtype=np.float64)
         self.check(lib().tpsc_get(self._h, f, out.ctypes.data_as(C.c_void_p), nbytes), f"tpsc_get({name})")
-        if name == "Self":
+        if name in ("Self", "Sigma_kw"):
             return out.view(np.complex128)
         return out

@@ -8,6 +8,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <map>
@@ -24,6 +25,7 @@ namespace {
 struct ProfPair {
     std::string name;
     hipEvent_t a, b;
+    hipStream_t s;
 };
 
 }  // namespace
@@ -50,6 +52,8 @@ struct tpsc_ctx {
     bool keep_chi_tau = false;
     // device
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;  // boundary kernels, overlapped with the slab kernels
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     DevArrays d{};
     double* chi_tau = nullptr;
     double* host_scalar = nullptr;  // pinned, mapped: kernels write root-finder values here
@@ -79,28 +83,30 @@ int fail(tpsc_ctx* c, int code, const std::string& msg)
             return fail((c), TPSC_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(_e)); \
     } while (0)
 
-void prof_begin(tpsc_ctx* c, const char* name)
+void prof_begin(tpsc_ctx* c, const char* name, hipStream_t st = nullptr)
 {
     if (!c->prof) return;
     ProfPair pp;
     pp.name = name;
-    hipEventCreate(&pp.a);
-    hipEventCreate(&pp.b);
-    hipEventRecord(pp.a, c->stream);
+    pp.s = st ? st : c->stream;
+    (void)hipEventCreate(&pp.a);
+    (void)hipEventCreate(&pp.b);
+    (void)hipEventRecord(pp.a, pp.s);
     c->pending.push_back(pp);
 }
 void prof_end(tpsc_ctx* c)
 {
     if (!c->prof || c->pending.empty()) return;
-    hipEventRecord(c->pending.back().b, c->stream);
+    (void)hipEventRecord(c->pending.back().b, c->pending.back().s);
 }
 void prof_flush(tpsc_ctx* c)
 {
     if (c->pending.empty()) return;
-    hipStreamSynchronize(c->stream);
+    (void)hipStreamSynchronize(c->side);
+    (void)hipStreamSynchronize(c->stream);
     for (auto& pp : c->pending) {
         float ms = 0.f;
-        hipEventElapsedTime(&ms, pp.a, pp.b);
+        (void)hipEventElapsedTime(&ms, pp.a, pp.b);
         auto it = c->prof_acc.find(pp.name);
         if (it == c->prof_acc.end()) {
             c->prof_acc[pp.name] = {ms, 1};
@@ -109,8 +115,8 @@ void prof_flush(tpsc_ctx* c)
             it->second.first += ms;
             it->second.second += 1;
         }
-        hipEventDestroy(pp.a);
-        hipEventDestroy(pp.b);
+        (void)hipEventDestroy(pp.a);
+        (void)hipEventDestroy(pp.b);
     }
     c->pending.clear();
 }
@@ -260,12 +266,69 @@ bool find_zero(const std::function<double(double)>& f, const double init[3], dou
     return done(x[best], true);
 }
 
+// TPSC_TRACE_ROOTS=1 prints every root-finder pass to stderr
+bool trace_roots()
+{
+    static const bool on = getenv("TPSC_TRACE_ROOTS") && atoi(getenv("TPSC_TRACE_ROOTS")) != 0;
+    return on;
+}
+
+// Root of a monotone f inside (lo, hi) from x by Halley steps, with the bracket narrowed by the
+// sign of every evaluation and a bisection whenever a step leaves it.  fgh(x, v) evaluates
+// v = {f, f', f''} (one pass over the device array) and returns a TPSC status.  Converged when
+// a step is below 1e-13 max(|x|, 1): the cubic order leaves the last iterate at the rounding
+// level of f.  Returns false (and the caller falls back to find_zero) if f' changes sign, the
+// root lands on an end of the interval, or itmax passes are spent.  A step that leaves the
+// bracket is replaced by the Newton step, and that by a bisection.
+bool halley_root(const std::function<int(double, double*)>& fgh, double x, double lo, double hi, int itmax,
+                 double* root, int* nevals, int* rc)
+{
+    int ne = 0, dir = 0;
+    const double lo0 = lo, hi0 = hi;
+    for (int it = 0; it < itmax; it++) {
+        double v[3] = {0.0, 0.0, 0.0};
+        *rc = fgh(x, v);
+        ne++;
+        if (nevals) *nevals = ne;
+        if (*rc) return false;
+        const double f = v[0], f1 = v[1], f2 = v[2];
+        if (trace_roots()) fprintf(stderr, "[tpsc root] pass %d x=%.17g f=%.6e f1=%.6e f2=%.6e [%.17g, %.17g]\n", ne, x, f, f1, f2, lo, hi);
+        if (!std::isfinite(f) || !std::isfinite(f1) || f1 == 0.0) return false;
+        if (dir == 0) dir = (f1 > 0) ? 1 : -1;
+        if ((f1 > 0 ? 1 : -1) != dir) return false;
+        if (f == 0.0) {
+            *root = x;
+            return true;
+        }
+        if ((f > 0) == (dir > 0))
+            hi = x;
+        else
+            lo = x;
+        const double den = 2.0 * f1 * f1 - f * f2;
+        const double dn = -f / f1;  // Newton step
+        const double dh = (den != 0.0 && std::isfinite(den)) ? -2.0 * f * f1 / den : dn;
+        if (fabs(dh) <= 1.0e-13 * std::max(fabs(x), 1.0)) {  // converged: the step is below the tolerance
+            double xn = x + dh;
+            if (!(xn >= lo && xn <= hi)) xn = x;
+            if (xn <= lo0 || xn >= hi0) return false;
+            *root = xn;
+            return true;
+        }
+        double xn = x + dh;
+        if (!(xn > lo && xn < hi)) xn = x + dn;
+        if (!(xn > lo && xn < hi)) xn = 0.5 * (lo + hi);
+        x = xn;
+    }
+    return false;
+}
+
 // ------------------------------------------------------------------ device helpers
+// count elements of elem bytes (elem = 0: sizeof(T))
 template <typename T>
-int dalloc(tpsc_ctx* c, T** ptr, size_t count)
+int dalloc(tpsc_ctx* c, T** ptr, size_t count, size_t elem = 0)
 {
     void* p = nullptr;
-    const size_t bytes = sizeof(T) * (count ? count : 1);
+    const size_t bytes = (elem ? elem : sizeof(T)) * (count ? count : 1);
     hipError_t e = hipMalloc(&p, bytes);
     if (e != hipSuccess) return fail(c, TPSC_E_NOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
     c->allocs.push_back(p);
@@ -273,6 +336,9 @@ int dalloc(tpsc_ctx* c, T** ptr, size_t count)
     *ptr = (T*)p;
     return 0;
 }
+
+// bytes of one element of the tau / iqn arrays (FP64 path: 8, mixed path: 4)
+size_t elem_bytes(const tpsc_ctx* c) { return c->p.precision ? sizeof(float) : sizeof(double); }
 
 int read_scalar(tpsc_ctx* c, const double* dptr, double* out)
 {
@@ -282,11 +348,26 @@ int read_scalar(tpsc_ctx* c, const double* dptr, double* out)
     return 0;
 }
 
-// a reduction launched with out = dev_scalar has written its value to mapped host memory
-int wait_mapped(tpsc_ctx* c, double* out)
+// a reduction launched with out = dev_scalar has written its value(s) to mapped host memory
+int wait_mapped(tpsc_ctx* c, double* out, int n = 1)
 {
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    *out = ((volatile double*)c->host_scalar)[0];
+    for (int i = 0; i < n; i++) out[i] = ((volatile double*)c->host_scalar)[i];
+    return 0;
+}
+
+// the side stream starts after the work queued so far on the main stream
+int fork_side(tpsc_ctx* c)
+{
+    HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
+    HIPCHK(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));
+    return 0;
+}
+// the main stream continues after the work queued so far on the side stream
+int join_side(tpsc_ctx* c)
+{
+    HIPCHK(c, hipEventRecord(c->ev_join, c->side));
+    HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
     return 0;
 }
 
@@ -332,6 +413,7 @@ int set_grid(tpsc_ctx* c, int N, int Nw)
     g.nk8 = (long)nk * (nk + 1) / 2;
     g.nq = (long)nk * (nk - 2);
     g.T = c->p.T;
+    g.prec = c->p.precision;
     hipStream_t s = c->stream;
     HIPCHK(c, launch_model(g, c->d, c->p.tp, c->p.tpp, c->p.t3, s));
     HIPCHK(c, launch_fill_tw(c->d, Nw, s));
@@ -348,14 +430,17 @@ int stage_chi0(tpsc_ctx* c, int nq0, int nw0)
     int rc = set_grid(c, nq0, nw0);
     if (rc) return rc;
     hipStream_t s = c->stream;
-    prof_begin(c, "chi0_tau0");
-    HIPCHK(c, launch_chi0_tau0(c->g, c->d, c->mu0, s));
+    if ((rc = fork_side(c))) return rc;
+    prof_begin(c, "chi0_tau0", c->side);
+    HIPCHK(c, launch_chi0_tau0(c->g, c->d, c->mu0, c->side));
     prof_end(c);
     prof_begin(c, "chi0_slab");
     HIPCHK(c, launch_chi0_slab(c->g, c->d, c->mu0, s));
     prof_end(c);
+    if ((rc = join_side(c))) return rc;
     if (c->keep_chi_tau && c->chi_tau)
-        HIPCHK(c, hipMemcpyAsync(c->chi_tau, c->d.chi, sizeof(double) * c->g.nwb * c->g.nk8, hipMemcpyDeviceToDevice, s));
+        HIPCHK(c, hipMemcpyAsync(c->chi_tau, c->d.chi, elem_bytes(c) * c->g.nwb * c->g.nk8, hipMemcpyDeviceToDevice,
+                                 s));
     prof_begin(c, "chi0_pencil");
     HIPCHK(c, launch_chi0_pencil(c->g, c->d, s));
     prof_end(c);
@@ -406,19 +491,44 @@ int stage_UspUch_grid(tpsc_ctx* c, int nq0, int nw0, double Usc[2], bool* ok_out
         if (rc == 0) rc = eval_sum_rule(c, 1, x, &v);
         return v;
     };
+    // sum rule, d/dUx and d2/dUx2 in one pass; channel 0: sU = -Ux, channel 1: sU = +Ux
+    auto gsr = [&](int channel, double x, double* v) -> int {
+        HIPCHK(c, launch_sum_rule3(c->g, c->d, channel == 0 ? -1.0 : 1.0, x, c->dev_scalar, c->stream));
+        int r = wait_mapped(c, v, 3);
+        if (r) return r;
+        const double n = c->p.n, nsr = c->nsr;
+        if (channel == 0) {
+            v[0] += n - n * n / 2.0 - nsr + x * nsr * nsr / (2.0 * U);
+            v[1] += nsr * nsr / (2.0 * U);
+        } else
+            v[0] += n - n * n / 2.0 - (nsr + c->Usp * nsr * nsr / (2.0 * U) - nsr * nsr);
+        return 0;
+    };
     double root = 0.0;
     double init[3] = {U / 2.0, 0.0, 2.0 / c->chi0max - 2 * DBL_EPSILON};
     bool ok = true;
-    if (find_zero(fsp, init, &root, nullptr, &c->nev_sp)) {
+    int nh = 0;
+    bool hok = halley_root([&](double x, double* v) { return gsr(0, x, v); }, init[0], init[1], init[2], 40, &root,
+                           &nh, &rc);
+    if (rc) return rc;
+    if (!hok) hok = find_zero(fsp, init, &root, nullptr, &c->nev_sp);
+    else c->nev_sp = nh;
+    if (rc) return rc;
+    if (hok) {
         Usc[0] = root;
         c->Usp = root;
     } else
         ok = false;
-    if (rc) return rc;
     root = 0.0;
     init[0] = 0;
     init[2] = 1000.0 * U;
-    if (find_zero(fch, init, &root, nullptr, &c->nev_ch))
+    hok = halley_root([&](double x, double* v) { return gsr(1, x, v); }, init[0], init[1], init[2], 40, &root, &nh,
+                      &rc);
+    if (rc) return rc;
+    if (!hok) hok = find_zero(fch, init, &root, nullptr, &c->nev_ch);
+    else c->nev_ch = nh;
+    if (rc) return rc;
+    if (hok)
         Usc[1] = root;
     else
         ok = false;
@@ -480,16 +590,19 @@ int stage_sigma(tpsc_ctx* c)
 {
     if (!c->have_V) return fail(c, TPSC_E_ORDER, "self-energy before the vertex");
     const double fact = c->p.U * (3.0 * c->Usp + c->Uch) / 8.0;
-    prof_begin(c, "sigma_bnd");
-    HIPCHK(c, launch_sigma_bnd(c->g, c->d, c->mu0, fact, c->stream));
+    int rc = fork_side(c);
+    if (rc) return rc;
+    prof_begin(c, "sigma_bnd", c->side);
+    HIPCHK(c, launch_sigma_bnd(c->g, c->d, c->mu0, fact, c->side));
     prof_end(c);
     prof_begin(c, "sigma_slab");
     HIPCHK(c, launch_sigma_slab(c->g, c->d, c->mu0, c->stream));
     prof_end(c);
+    if ((rc = join_side(c))) return rc;
     prof_begin(c, "sigma_pencil");
     HIPCHK(c, launch_sigma_pencil(c->g, c->d, c->stream));
     prof_end(c);
-    int rc = read_scalar(c, c->d.Sinf, &c->Sigma_inf);
+    rc = read_scalar(c, c->d.Sinf, &c->Sigma_inf);
     if (rc) return rc;
     prof_flush(c);
     c->have_sigma = true;
@@ -546,7 +659,17 @@ int stage_mu(tpsc_ctx* c)
     };
     const double init[3] = {0.0, -10.0, 10.0}, lims[2] = {-20, 20};
     double root = 0.0;
-    const bool ok = find_zero(f, init, &root, lims, &c->nev_mu);
+    auto g3 = [&](double m, double* v) -> int {
+        HIPCHK(c, launch_trace_diffG3(c->g, c->d, m, c->mu0, c->dev_scalar, c->stream));
+        return wait_mapped(c, v, 3);
+    };
+    int nh = 0;
+    bool ok = halley_root(g3, init[0], lims[0], lims[1], 40, &root, &nh, &rc);
+    if (rc) return rc;
+    if (ok)
+        c->nev_mu = nh;
+    else
+        ok = find_zero(f, init, &root, lims, &c->nev_mu);
     prof_end(c);
     prof_flush(c);
     if (rc) return rc;
@@ -558,14 +681,6 @@ int stage_mu(tpsc_ctx* c)
     if (rc) return rc;
     if (!ok) return fail(c, TPSC_E_NOROOT, "mu not found");
     return 0;
-}
-
-void jj_mark(void* ctx, int k)
-{
-    tpsc_ctx* c = (tpsc_ctx*)ctx;
-    static const char* names[] = {"jj_hf0_slab", "jj_hf0_bnd", "jj_hVf0V_slab", "jj_pencil", "jj_kx0_rows"};
-    if (k > 0) prof_end(c);
-    if (k < 5) prof_begin(c, names[k]);
 }
 
 // cond_opt::calc_chijj_vertex_corr_all_optim (chi3 = false), cond_opt.cpp:2537-4290
@@ -580,7 +695,25 @@ int stage_chijj(tpsc_ctx* c)
     if (!c->mu_ok) return fail(c, TPSC_E_ORDER, "calc_chijj_vertex_corr_all_optim(): mu not computed");
     const Grid& g = c->g;
     const double fact4 = c->p.U * (3.0 * c->Usp + c->Uch) / 4;
-    HIPCHK(c, launch_chijj(g, c->d, c->mu0, c->mu, fact4, c->stream, c->prof ? jj_mark : nullptr, c));
+    hipStream_t s = c->stream;
+    int rc = fork_side(c);
+    if (rc) return rc;
+    prof_begin(c, "jj_hf0_bnd", c->side);
+    HIPCHK(c, launch_hf0_bnd(g, c->d, c->mu0, fact4, c->side));
+    prof_end(c);
+    prof_begin(c, "jj_kx0_rows", c->side);
+    HIPCHK(c, launch_kx0_rows(g, c->d, c->mu, c->side));
+    prof_end(c);
+    prof_begin(c, "jj_hf0_slab");
+    HIPCHK(c, launch_hf0_slab(g, c->d, c->mu0, s));
+    prof_end(c);
+    prof_begin(c, "jj_hVf0V_slab");
+    HIPCHK(c, launch_hVf0V_slab(g, c->d, s));
+    prof_end(c);
+    if ((rc = join_side(c))) return rc;
+    prof_begin(c, "jj_pencil");
+    HIPCHK(c, launch_jj_pencil(g, c->d, c->mu, s));
+    prof_end(c);
     const int nblk = (int)((g.nq + kJJK - 1) / kJJK);
     const int nwb = g.nwb;
     HIPCHK(c, launch_reduce_cols(c->d.jjpart, nblk, 2 * nwb, c->d.jjred, c->stream));
@@ -620,11 +753,45 @@ size_t field_count(tpsc_ctx* c, int field)
         case TPSC_F_SIGMA_INF4: return (size_t)g.nk8;
         case TPSC_F_DTAU_SIGMA: return 2 * (size_t)g.nk8;
         case TPSC_F_SIGMA_TAU: return (size_t)(g.Nw + 1) * g.nk8;
-        case TPSC_F_SELF: return 2 * (size_t)(g.Nw / 2) * g.nk8;
+        case TPSC_F_SELF:
+        case TPSC_F_SIGMA_KW: return 2 * (size_t)(g.Nw / 2) * g.nk8;
         case TPSC_F_CHIJJ1:
         case TPSC_F_CHIJJ2: return (size_t)g.nwb;
         default: return 0;
     }
+}
+
+// Sigma(k,iw_n), n = 0..Nw/2-1, [n*nk8 + k] complex128: assembled from the remainder and the
+// moments as in green.cpp:3039 (FP64 path) or widened from the stored FP32 values (mixed path)
+int get_sigma_kw(tpsc_ctx* c, double* dst)
+{
+    if (!c->have_sigma) return fail(c, TPSC_E_ORDER, "self-energy not computed");
+    const Grid& g = c->g;
+    const size_t n = (size_t)(g.Nw / 2) * g.nk8;
+    hipStream_t s = c->stream;
+    if (c->p.precision) {
+        std::vector<float> tmp(2 * n);
+        HIPCHK(c, hipMemcpyAsync(tmp.data(), c->d.Self, n * sizeof(cf), hipMemcpyDeviceToHost, s));
+        HIPCHK(c, hipStreamSynchronize(s));
+        for (size_t i = 0; i < 2 * n; i++) dst[i] = (double)tmp[i];
+        return 0;
+    }
+    std::vector<double> s2(g.nk8), s3(g.nk8);
+    HIPCHK(c, hipMemcpyAsync(dst, c->d.Self, n * sizeof(cd), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipMemcpyAsync(s2.data(), c->d.Sinf2, g.nk8 * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipMemcpyAsync(s3.data(), c->d.Sinf3, g.nk8 * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    for (int j = 0; j < g.Nw / 2; j++) {
+        const double wn = (2 * j + 1) * kPi * g.T, w2 = wn * wn, w3 = w2 * wn, w4 = w2 * w2;
+        for (long k = 0; k < g.nk8; k++) {
+            double* z = dst + 2 * ((size_t)j * g.nk8 + k);
+            const double re = -s2[k] / w2 + z[0] / w4;
+            const double im = -c->Sigma_inf / wn + s3[k] / w3 + z[1] / w4;
+            z[0] = re;
+            z[1] = im;
+        }
+    }
+    return 0;
 }
 
 }  // namespace
@@ -638,9 +805,9 @@ int tpsc_create(const tpsc_params* p, tpsc_ctx** out)
     *out = nullptr;
     tpsc_ctx* c = new tpsc_ctx();
     c->p = *p;
-    if (p->precision != 0) {
+    if (p->precision != TPSC_PREC_FP64 && p->precision != TPSC_PREC_MIXED) {
         *out = c;
-        return fail(c, TPSC_E_INVALID, "only precision 0 (FP64) is available");
+        return fail(c, TPSC_E_INVALID, "precision must be TPSC_PREC_FP64 (0) or TPSC_PREC_MIXED (1)");
     }
     if (!supported_N(p->nk0) || !supported_Nw(p->Nw0) || (p->coarse_pass && (p->nk0 < 8 || p->Nw0 < 32))) {
         *out = c;
@@ -656,38 +823,53 @@ int tpsc_create(const tpsc_params* p, tpsc_ctx** out)
     if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count < 1)
         return fail(c, TPSC_E_DEVICE, "no HIP device available");
     HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIPCHK(c, hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+    HIPCHK(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+    HIPCHK(c, hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
     HIPCHK(c, hipHostMalloc((void**)&c->host_scalar, 16 * sizeof(double), hipHostMallocMapped));
     HIPCHK(c, hipHostGetDevicePointer((void**)&c->dev_scalar, c->host_scalar, 0));
     const int N = p->nk0, Nw = p->Nw0, nk = N / 2 + 1;
     const long nk8 = (long)nk * (nk + 1) / 2, nq = (long)nk * (nk - 2);
     const long nwb = Nw / 2 + 1;
     DevArrays& d = c->d;
+    const size_t es = elem_bytes(c);  // tau / iqn arrays
+    const size_t ss = p->precision ? sizeof(cf) : sizeof(cd);  // stored Sigma(k,iwn)
     int rc = 0;
     rc |= dalloc(c, &d.ek, (size_t)nk * nk);
     rc |= dalloc(c, &d.ekw, nk8);
     rc |= dalloc(c, &d.wq, nk8);
     rc |= dalloc(c, &d.er, std::max<long>(nk8, 5));
-    rc |= dalloc(c, &d.chi, (size_t)nwb * nk8);
-    rc |= dalloc(c, &d.chirt, (size_t)nwb * nk8);
+    rc |= dalloc(c, &d.chi, (size_t)nwb * nk8, es);
+    if (p->precision)
+        rc |= dalloc(c, &d.chiw, (size_t)nwb * nk8);
+    else
+        d.chiw = d.chi;
+    rc |= dalloc(c, &d.chirt, (size_t)nwb * nk8, es);
     rc |= dalloc(c, &d.dchirt0, nk8);
     rc |= dalloc(c, &d.dGrt0p, nk8);
     rc |= dalloc(c, &d.dchiqt0, nk8);
     rc |= dalloc(c, &d.chi0inf, nk8);
     rc |= dalloc(c, &d.chi0inf2, nk8);
-    rc |= dalloc(c, &d.V, (size_t)nwb * nk8);
-    rc |= dalloc(c, &d.Stau, (size_t)(Nw + 1) * nk8);
+    rc |= dalloc(c, &d.V, (size_t)nwb * nk8, es);
+    rc |= dalloc(c, &d.Stau, (size_t)(Nw + 1) * nk8, es);
     rc |= dalloc(c, &d.dSig, 2 * nk8);
-    rc |= dalloc(c, &d.Self, (size_t)(Nw / 2) * nk8);
+    rc |= dalloc(c, &d.Sb, 2 * nk8);
+    rc |= dalloc(c, &d.chirt0, nk8);
+    rc |= dalloc(c, &d.V0, nk8);
+    rc |= dalloc(c, &d.Self, (size_t)(Nw / 2) * nk8, ss);
     rc |= dalloc(c, &d.Sinf2, nk8);
     rc |= dalloc(c, &d.Sinf3, nk8);
     rc |= dalloc(c, &d.Sinf4, nk8);
     rc |= dalloc(c, &d.Sinf, 1);
     rc |= dalloc(c, &d.dek, std::max<long>(nq, 1));
     rc |= dalloc(c, &d.d2ek, (size_t)nk * nk);
-    rc |= dalloc(c, &d.h, (size_t)(Nw + 1) * std::max<long>(nq, 1));
-    rc |= dalloc(c, &d.f0, (size_t)(Nw + 1) * std::max<long>(nq, 1));
+    rc |= dalloc(c, &d.h, (size_t)(Nw + 1) * std::max<long>(nq, 1), es);
+    rc |= dalloc(c, &d.f0, (size_t)(Nw + 1) * std::max<long>(nq, 1), es);
     rc |= dalloc(c, &d.dhV, 2 * std::max<long>(nq, 1));
     rc |= dalloc(c, &d.df0V, 2 * std::max<long>(nq, 1));
+    rc |= dalloc(c, &d.hVb, 2 * std::max<long>(nq, 1));
+    rc |= dalloc(c, &d.f0Vb, 2 * std::max<long>(nq, 1));
+    rc |= dalloc(c, &d.bscr, 4 * std::max<long>(nq, 1));
     const long nblk = (nq + kJJK - 1) / kJJK;
     rc |= dalloc(c, &d.jjpart, (size_t)std::max<long>(nblk, 1) * 2 * nwb);
     rc |= dalloc(c, &d.jjscal, (size_t)std::max<long>(nblk, 1) * 4);
@@ -709,15 +891,19 @@ int tpsc_create(const tpsc_params* p, tpsc_ctx** out)
 void tpsc_destroy(tpsc_ctx* c)
 {
     if (!c) return;
-    if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->side) (void)hipStreamSynchronize(c->side);
     for (auto& pp : c->pending) {
-        hipEventDestroy(pp.a);
-        hipEventDestroy(pp.b);
+        (void)hipEventDestroy(pp.a);
+        (void)hipEventDestroy(pp.b);
     }
-    for (void* p : c->allocs) hipFree(p);
-    if (c->chi_tau) hipFree(c->chi_tau);
-    if (c->host_scalar) hipHostFree(c->host_scalar);
-    if (c->stream) hipStreamDestroy(c->stream);
+    for (void* p : c->allocs) (void)hipFree(p);
+    if (c->chi_tau) (void)hipFree(c->chi_tau);
+    if (c->host_scalar) (void)hipHostFree(c->host_scalar);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->side) (void)hipStreamDestroy(c->side);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
 
@@ -839,6 +1025,7 @@ int tpsc_get(tpsc_ctx* c, int field, void* dst, size_t bytes)
     if (n == 0) return fail(c, TPSC_E_INVALID, "unknown field");
     if (bytes < n * sizeof(double)) return fail(c, TPSC_E_INVALID, "destination too small");
     const void* src = nullptr;
+    bool stored = false;  // array held in the tau / iqn storage precision
     switch (field) {
         case TPSC_F_EK: src = c->d.ek; break;
         case TPSC_F_CHIJJ1:
@@ -848,14 +1035,20 @@ int tpsc_get(tpsc_ctx* c, int field, void* dst, size_t bytes)
             memcpy(dst, v.data(), n * sizeof(double));
             return 0;
         }
-        case TPSC_F_CHI0: src = c->d.chi; break;
-        case TPSC_F_CHIRT: src = c->d.chirt; break;
+        case TPSC_F_CHI0: src = c->d.chiw; break;
+        case TPSC_F_CHIRT: src = c->d.chirt; stored = true; break;
         case TPSC_F_CHI0_INF: src = c->d.chi0inf; break;
         case TPSC_F_CHI0_INF2: src = c->d.chi0inf2; break;
         case TPSC_F_DTAU_CHIRT0: src = c->d.dchirt0; break;
-        case TPSC_F_V: src = c->d.V; break;
-        case TPSC_F_SIGMA_TAU: src = c->d.Stau; break;
-        case TPSC_F_SELF: src = c->d.Self; break;
+        case TPSC_F_V: src = c->d.V; stored = true; break;
+        case TPSC_F_SIGMA_TAU: src = c->d.Stau; stored = true; break;
+        case TPSC_F_SELF:
+            if (c->p.precision)
+                return fail(c, TPSC_E_INVALID,
+                            "the mixed-precision path stores the assembled Sigma(k,iwn): read TPSC_F_SIGMA_KW");
+            src = c->d.Self;
+            break;
+        case TPSC_F_SIGMA_KW: return get_sigma_kw(c, (double*)dst);
         case TPSC_F_SIGMA_INF2: src = c->d.Sinf2; break;
         case TPSC_F_SIGMA_INF3: src = c->d.Sinf3; break;
         case TPSC_F_SIGMA_INF4: src = c->d.Sinf4; break;
@@ -863,8 +1056,17 @@ int tpsc_get(tpsc_ctx* c, int field, void* dst, size_t bytes)
         case TPSC_F_CHI0_TAU:
             if (!c->chi_tau) return fail(c, TPSC_E_ORDER, "chi0(q,tau) not kept");
             src = c->chi_tau;
+            stored = true;
             break;
         default: return fail(c, TPSC_E_INVALID, "unknown field");
+    }
+    if (stored && c->p.precision) {
+        std::vector<float> tmp(n);
+        HIPCHK(c, hipMemcpyAsync(tmp.data(), src, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        double* o = (double*)dst;
+        for (size_t i = 0; i < n; i++) o[i] = (double)tmp[i];
+        return 0;
     }
     HIPCHK(c, hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -945,7 +1147,7 @@ int tpsc_keep_chi_tau(tpsc_ctx* c, int on)
     c->keep_chi_tau = on != 0;
     if (on && !c->chi_tau) {
         const size_t n = (size_t)(c->p.Nw0 / 2 + 1) * ((long)(c->p.nk0 / 2 + 1) * (c->p.nk0 / 2 + 2) / 2);
-        HIPCHK(c, hipMalloc((void**)&c->chi_tau, n * sizeof(double)));
+        HIPCHK(c, hipMalloc((void**)&c->chi_tau, n * elem_bytes(c)));
     }
     return 0;
 }

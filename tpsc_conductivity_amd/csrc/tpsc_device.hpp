@@ -89,51 +89,71 @@ __device__ inline int xcd_remap(int b, int nb)
     return lb;
 }
 
-__device__ inline cd shfl_xor_cd(cd v, int mask)
+// single-precision complex for the FP32 slab transforms of the mixed-precision path
+struct cf {
+    float x, y;
+};
+__host__ __device__ inline cf operator+(cf a, cf b) { return cf{a.x + b.x, a.y + b.y}; }
+__host__ __device__ inline cf operator-(cf a, cf b) { return cf{a.x - b.x, a.y - b.y}; }
+__host__ __device__ inline cf operator*(cf a, cf b) { return cf{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+
+template <typename Real>
+struct cplx_of;
+template <>
+struct cplx_of<double> {
+    using type = cd;
+};
+template <>
+struct cplx_of<float> {
+    using type = cf;
+};
+
+__device__ inline cd shfl_xor_c(cd v, int mask)
 {
     return cd{__shfl_xor(v.x, mask, 64), __shfl_xor(v.y, mask, 64)};
+}
+__device__ inline cf shfl_xor_c(cf v, int mask)
+{
+    return cf{__shfl_xor(v.x, mask, 64), __shfl_xor(v.y, mask, 64)};
 }
 
 // ---------------------------------------------------------------------------------
 // in-register DFT of length R (R = 1, 2, 4, 8), forward sign, natural order output
 // ---------------------------------------------------------------------------------
-template <int R>
-__device__ inline void reg_dft(cd* v);
-template <>
-__device__ inline void reg_dft<1>(cd*) {}
-template <>
-__device__ inline void reg_dft<2>(cd* v)
+template <typename C>
+__device__ inline C mul_mi(C a) { return C{a.y, -a.x}; }  // a * (-i)
+template <typename C>
+__device__ inline C mul_pi(C a) { return C{-a.y, a.x}; }  // a * (+i)
+
+template <int R, typename C>
+__device__ __forceinline__ void reg_dft(C* v)
 {
-    const cd a = v[0], b = v[1];
-    v[0] = a + b;
-    v[1] = a - b;
-}
-__device__ inline cd mul_mi(cd a) { return cd{a.y, -a.x}; }  // a * (-i)
-__device__ inline cd mul_pi(cd a) { return cd{-a.y, a.x}; }  // a * (+i)
-template <>
-__device__ inline void reg_dft<4>(cd* v)
-{
-    const cd a0 = v[0], a1 = v[1], a2 = v[2], a3 = v[3];
-    const cd s02 = a0 + a2, d02 = a0 - a2, s13 = a1 + a3, d13 = a1 - a3;
-    v[0] = s02 + s13;
-    v[2] = s02 - s13;
-    v[1] = d02 + mul_mi(d13);
-    v[3] = d02 - mul_mi(d13);
-}
-template <>
-__device__ inline void reg_dft<8>(cd* v)
-{
-    const double h = 0.70710678118654752440;
-    cd e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
-    reg_dft<4>(e);
-    reg_dft<4>(o);
-    const cd w1 = cd{h, -h}, w3 = cd{-h, -h};
-    o[1] = o[1] * w1;
-    o[2] = mul_mi(o[2]);
-    o[3] = o[3] * w3;
-    for (int k = 0; k < 4; k++) {
-        v[k] = e[k] + o[k];
-        v[k + 4] = e[k] - o[k];
+    if constexpr (R == 2) {
+        const C a = v[0], b = v[1];
+        v[0] = a + b;
+        v[1] = a - b;
+    } else if constexpr (R == 4) {
+        const C a0 = v[0], a1 = v[1], a2 = v[2], a3 = v[3];
+        const C s02 = a0 + a2, d02 = a0 - a2, s13 = a1 + a3, d13 = a1 - a3;
+        v[0] = s02 + s13;
+        v[2] = s02 - s13;
+        v[1] = d02 + mul_mi(d13);
+        v[3] = d02 - mul_mi(d13);
+    } else if constexpr (R == 8) {
+        using F = decltype(C{}.x);
+        const F h = (F)0.70710678118654752440;
+        C e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
+        reg_dft<4>(e);
+        reg_dft<4>(o);
+        const C w1 = C{h, -h}, w3 = C{-h, -h};
+        o[1] = o[1] * w1;
+        o[2] = mul_mi(o[2]);
+        o[3] = o[3] * w3;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            v[k] = e[k] + o[k];
+            v[k + 4] = e[k] - o[k];
+        }
     }
 }
 
@@ -144,13 +164,16 @@ __device__ inline void reg_dft<8>(cd* v)
 //                     z_j = x_{j-1}, z_{L-j} = -x_{j-1}
 // Row r element e lives at base[r*rs + e*es].  The DFT of a real even (odd) sequence is
 // real (imaginary), so packing rows r and r+1 as real and imaginary parts yields both
-// transforms from one FFT.  L = R*G, G = min(L,64) lanes per FFT.
+// transforms from one FFT.  L = R*G, G = min(L,64) lanes per FFT.  The arithmetic runs
+// in the precision of Real (FP64 or FP32).
 // Must be called by every thread of the block (uniform control flow); caller syncs.
-// tw: LDS table tw[t] = exp(-2 pi i t / L), t < L.
+// tw: LDS table tw[t] = exp(-2 pi i t / L), t < L, in the precision of Real.
 // ---------------------------------------------------------------------------------
 template <typename Real, int R, int G>
-__device__ __forceinline__ void wave_r2r_rows(Real* base, int nrows, int rs, int es, int kind, const cd* tw)
+__device__ __forceinline__ void wave_r2r_rows(Real* base, int nrows, int rs, int es, int kind,
+                                              const typename cplx_of<Real>::type* tw)
 {
+    using C = typename cplx_of<Real>::type;
     constexpr int L = R * G;
     constexpr int P = 64 / G;
     const int lane = threadIdx.x & 63;
@@ -165,24 +188,27 @@ __device__ __forceinline__ void wave_r2r_rows(Real* base, int nrows, int rs, int
         const bool valid = task < npairs;
         const int r1 = 2 * task, r2 = 2 * task + 1;
         const bool has2 = valid && (r2 < nrows);
-        cd v[R];
+        C v[R];
 #pragma unroll
         for (int k = 0; k < R; k++) {
             const int j = p + G * k;
-            double a = 0.0, b = 0.0;
+            Real a = 0, b = 0;
             if (valid) {
                 if (kind == 0) {
                     const int e = (j <= half) ? j : L - j;
-                    a = (double)base[(long)r1 * rs + (long)e * es];
-                    if (has2) b = (double)base[(long)r2 * rs + (long)e * es];
+                    a = base[(long)r1 * rs + (long)e * es];
+                    if (has2) b = base[(long)r2 * rs + (long)e * es];
                 } else if (j != 0 && j != half) {
                     const int e = (j < half) ? j - 1 : L - j - 1;
-                    const double sg = (j < half) ? 1.0 : -1.0;
-                    a = sg * (double)base[(long)r1 * rs + (long)e * es];
-                    if (has2) b = sg * (double)base[(long)r2 * rs + (long)e * es];
+                    a = base[(long)r1 * rs + (long)e * es];
+                    if (has2) b = base[(long)r2 * rs + (long)e * es];
+                    if (j > half) {
+                        a = -a;
+                        b = -b;
+                    }
                 }
             }
-            v[k] = cd{a, b};
+            v[k] = C{a, b};
         }
         reg_dft<R>(v);
         // twiddle by exp(-2 pi i p k1 / L)
@@ -192,10 +218,10 @@ __device__ __forceinline__ void wave_r2r_rows(Real* base, int nrows, int rs, int
 #pragma unroll
         for (int hh = G / 2; hh >= 1; hh >>= 1) {
             const bool upper = (p & hh) != 0;
-            const cd w = tw[(p & (hh - 1)) * (L / (2 * hh))];
+            const C w = tw[(p & (hh - 1)) * (L / (2 * hh))];
 #pragma unroll
             for (int k1 = 0; k1 < R; k1++) {
-                const cd o = shfl_xor_cd(v[k1], hh);
+                const C o = shfl_xor_c(v[k1], hh);
                 v[k1] = upper ? (o - v[k1]) * w : (v[k1] + o);
             }
         }
@@ -209,32 +235,36 @@ __device__ __forceinline__ void wave_r2r_rows(Real* base, int nrows, int rs, int
                 const int k = k1 + R * br;
                 if (kind == 0) {
                     if (k <= half) {
-                        base[(long)r1 * rs + (long)k * es] = (Real)v[k1].x;
-                        if (has2) base[(long)r2 * rs + (long)k * es] = (Real)v[k1].y;
+                        base[(long)r1 * rs + (long)k * es] = v[k1].x;
+                        if (has2) base[(long)r2 * rs + (long)k * es] = v[k1].y;
                     }
                 } else if (k >= 1 && k <= half - 1) {
-                    base[(long)r1 * rs + (long)(k - 1) * es] = (Real)(-v[k1].y);
-                    if (has2) base[(long)r2 * rs + (long)(k - 1) * es] = (Real)v[k1].x;
+                    base[(long)r1 * rs + (long)(k - 1) * es] = -v[k1].y;
+                    if (has2) base[(long)r2 * rs + (long)(k - 1) * es] = v[k1].x;
                 }
             }
         }
     }
 }
 
-// fill tw[t] = exp(-2 pi i t / L) for t < L (block-cooperative; caller syncs)
-__device__ __forceinline__ void fill_twiddles(cd* tw, int L)
+// fill tw[t] = exp(-2 pi i t / L) for t < L (block-cooperative; caller syncs); the angle
+// is evaluated in FP64 and rounded to the table precision
+template <typename C>
+__device__ __forceinline__ void fill_twiddles(C* tw, int L)
 {
+    using F = decltype(C{}.x);
     for (int t = threadIdx.x; t < L; t += blockDim.x) {
         double s, c;
         sincospi(2.0 * (double)t / (double)L, &s, &c);
-        tw[t] = cd{c, -s};
+        tw[t] = C{(F)c, (F)(-s)};
     }
 }
 
 // 2D transform of an n0 x n1 LDS slab (row-major, row stride ld): axis 1 then axis 0.
 // kind0 along axis 0 (slow index), kind1 along axis 1.  Both use FFT length L = N.
 template <typename Real, int N>
-__device__ __forceinline__ void slab_r2r_2d(Real* slab, int n0, int n1, int ld, int kind0, int kind1, const cd* tw)
+__device__ __forceinline__ void slab_r2r_2d(Real* slab, int n0, int n1, int ld, int kind0, int kind1,
+                                            const typename cplx_of<Real>::type* tw)
 {
     constexpr int G = (N < 64) ? N : 64;
     constexpr int R = N / G;

@@ -1,4 +1,4 @@
-// tpsc_kernels.hip -- FP64 HIP kernels of the TPSC hot path for gfx950.
+// tpsc_kernels.hip -- HIP kernels of the TPSC hot path for gfx950.
 //
 // Stage map (reference = dbergeron1/TPSC_conductivity):
 //   k_chi0_tau0, k_chi0_slab     chi0.cpp:310-559   G0(k,+-tau) -> G(r,+-tau) -> chi0(r,tau) -> chi0(q,tau)
@@ -11,8 +11,14 @@
 //   k_hf0_slab, k_hf0_bnd,
 //   k_hVf0V_slab, k_jj_pencil,
 //   k_kx0_rows                   cond_opt.cpp:2584-4173 Lambda_jj bubble + Maki-Thompson + kx0
-// Slab kernels hold one (nk x nk) FP64 slab in LDS (133 KB at N=256) and transform it with
+// Slab kernels hold one (nk x nk) slab in LDS (133 KB at N=256 in FP64) and transform it with
 // wavefront FFTs; pencil kernels hold one tau-pencil per workgroup.
+// Precision: S is the storage and slab-arithmetic type of the tau arrays (chi0(q,tau),
+// chi0(r,tau), V, Sigma(k,tau), h, f0); chi0(q,iqn) is stored in FP64 in both paths.  S = double is the FP64 path.  S = float is the mixed path: the
+// slab transforms and pointwise products run in FP32, while the tau = 0 / beta boundary data,
+// the spline moments, the tail moments, the pencil Fourier sums, the root-finder reductions
+// and the Lambda_jj Matsubara sums stay in FP64.  Self holds the remainder R(k,n) as FP64
+// complex (S = double) or the assembled Sigma(k,iwn) as FP32 complex (S = float).
 #include <algorithm>
 
 #include "tpsc_launch.hpp"
@@ -44,6 +50,17 @@ __global__ void k_finalize(const double* __restrict__ part, int n, double* out)
     if (threadIdx.x == 0) out[0] = a;
 }
 
+// out[b] = sum of the n partials part[b*n .. b*n+n) in a fixed order; one block per value
+__global__ void k_finalize_k(const double* __restrict__ part, int n, double* out)
+{
+    __shared__ double sc[16];
+    const double* p = part + (long)blockIdx.x * n;
+    double a = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) a += p[i];
+    a = block_sum(a, sc);
+    if (threadIdx.x == 0) out[blockIdx.x] = a;
+}
+
 // wedge (l,m) of a packed index, m <= l
 __device__ __forceinline__ void wedge_lm(long idx, int& l, int& m)
 {
@@ -56,6 +73,21 @@ __device__ __forceinline__ void wedge_lm(long idx, int& l, int& m)
 
 // write a wedge value to both symmetric positions of an nk x nk LDS slab
 __device__ __forceinline__ void put_sym(double* slab, int nk, int l, int m, double v)
+{
+    slab[m + l * nk] = v;
+    slab[l + m * nk] = v;
+}
+
+// twiddle table placed after an n-element slab in LDS, 16-byte aligned
+template <typename Real>
+__device__ __forceinline__ typename cplx_of<Real>::type* slab_tw(Real* slab, long n)
+{
+    const long bytes = (n * (long)sizeof(Real) + 15) & ~15L;
+    return (typename cplx_of<Real>::type*)((char*)slab + bytes);
+}
+
+template <typename Real>
+__device__ __forceinline__ void put_sym_t(Real* slab, int nk, int l, int m, Real v)
 {
     slab[m + l * nk] = v;
     slab[l + m * nk] = v;
@@ -78,17 +110,19 @@ __device__ __forceinline__ double g0_minus(double x, double t, double T, double 
 
 // tau = 0 slice plus the tau-derivative data (chi0.cpp:323-447); one block.
 // tmp[0:nk8] <- G(r,0), tmp[nk8:2nk8] <- dG(r,0-), tmp[2nk8:3nk8] <- staging
-template <int N>
+// The boundary data are tail moments: this kernel runs in FP64 for both precisions and
+// only rounds the slice-0 values it stores into the S arrays.
+template <int N, typename S>
 __global__ void __launch_bounds__(SLAB_THREADS)
-    k_chi0_tau0(const double* __restrict__ ekw, const double* __restrict__ er, double mu0, double T, double* chi,
-                double* chirt, double* dchirt0, double* dGrt0p, double* dchiqt0, double* tmp)
+    k_chi0_tau0(const double* __restrict__ ekw, const double* __restrict__ er, double mu0, double T, S* chi,
+                S* chirt, double* dchirt0, double* dGrt0p, double* dchiqt0, double* chirt0, double* tmp)
 {
     constexpr int nk = N / 2 + 1;
     constexpr long nk8 = (long)nk * (nk + 1) / 2;
     const double nk02 = (double)N * N;
     extern __shared__ double smem[];
     double* slab = smem;
-    cd* tw = (cd*)(smem + nk * nk + (nk * nk & 1));
+    cd* tw = slab_tw(slab, (long)nk * nk);
     fill_twiddles(tw, N);
     double* G0r = tmp;
     double* dGm = tmp + nk8;
@@ -127,20 +161,22 @@ __global__ void __launch_bounds__(SLAB_THREADS)
         const double g = slab[m + l * nk];
         G0r[i] = g;
         const double c = (i == 0) ? -2.0 * g * (g + 1.0) : -2.0 * g * g;
-        chirt[i] = c;
+        chirt[i] = (S)c;
+        chirt0[i] = c;
+        stg[i] = c;
     }
     __syncthreads();
     for (long i = tid; i < nk8; i += nt) {
         int l, m;
         wedge_lm(i, l, m);
-        put_sym(slab, nk, l, m, chirt[i]);
+        put_sym(slab, nk, l, m, stg[i]);
     }
     __syncthreads();
     slab_r2r_2d<double, N>(slab, nk, nk, nk, 0, 0, tw);
     for (long i = tid; i < nk8; i += nt) {
         int l, m;
         wedge_lm(i, l, m);
-        chi[i] = slab[m + l * nk];
+        chi[i] = (S)slab[m + l * nk];
     }
     __syncthreads();
     // d/dtau chi0(r,0) and its transform
@@ -167,31 +203,31 @@ __global__ void __launch_bounds__(SLAB_THREADS)
 }
 
 // slices j = 1 .. nwb-1 (chi0.cpp:454-559); block b handles j = b + 1
-template <int N>
+template <int N, typename S>
 __global__ void __launch_bounds__(SLAB_THREADS)
-    k_chi0_slab(const double* __restrict__ ekw, double mu0, double T, int Nw, double* chi, double* chirt)
+    k_chi0_slab(const double* __restrict__ ekw, double mu0, double T, int Nw, S* chi, S* chirt)
 {
     constexpr int nk = N / 2 + 1;
     constexpr long nk8 = (long)nk * (nk + 1) / 2;
     const double nk02 = (double)N * N;
     extern __shared__ double smem[];
-    double* slab = smem;
-    cd* tw = (cd*)(smem + nk * nk + (nk * nk & 1));
+    S* slab = (S*)smem;
+    auto* tw = slab_tw(slab, (long)nk * nk);
     fill_twiddles(tw, N);
     const int nwb = Nw / 2 + 1;
     const int j = blockIdx.x + 1;
     const int tid = threadIdx.x, nt = blockDim.x;
-    double* cr = chirt + (long)j * nk8;
-    double* cq = chi + (long)j * nk8;
+    S* cr = chirt + (long)j * nk8;
+    S* cq = chi + (long)j * nk8;
     if (j < nwb - 1) {
         const double t = j / (Nw * T);
         for (long i = tid; i < nk8; i += nt) {
             int l, m;
             wedge_lm(i, l, m);
-            put_sym(slab, nk, l, m, g0_plus(ekw[i] - mu0, t, T, nk02, 0));
+            put_sym_t(slab, nk, l, m, (S)g0_plus(ekw[i] - mu0, t, T, nk02, 0));
         }
         __syncthreads();
-        slab_r2r_2d<double, N>(slab, nk, nk, nk, 0, 0, tw);
+        slab_r2r_2d<S, N>(slab, nk, nk, nk, 0, 0, tw);
         for (long i = tid; i < nk8; i += nt) {  // stage G(r,t) in chirt[j]
             int l, m;
             wedge_lm(i, l, m);
@@ -201,20 +237,20 @@ __global__ void __launch_bounds__(SLAB_THREADS)
         for (long i = tid; i < nk8; i += nt) {
             int l, m;
             wedge_lm(i, l, m);
-            put_sym(slab, nk, l, m, g0_minus(ekw[i] - mu0, t, T, nk02));
+            put_sym_t(slab, nk, l, m, (S)g0_minus(ekw[i] - mu0, t, T, nk02));
         }
         __syncthreads();
-        slab_r2r_2d<double, N>(slab, nk, nk, nk, 0, 0, tw);
+        slab_r2r_2d<S, N>(slab, nk, nk, nk, 0, 0, tw);
         for (long i = tid; i < nk8; i += nt) {  // chi0(r,t) = -2 G(r,-t) G(r,t)
             int l, m;
             wedge_lm(i, l, m);
-            cr[i] = -2.0 * slab[m + l * nk] * cr[i];
+            cr[i] = (S)(-2) * slab[m + l * nk] * cr[i];
         }
         __syncthreads();
         for (long i = tid; i < nk8; i += nt) {
             int l, m;
             wedge_lm(i, l, m);
-            put_sym(slab, nk, l, m, cr[i]);
+            put_sym_t(slab, nk, l, m, cr[i]);
         }
         __syncthreads();
     } else {
@@ -222,11 +258,11 @@ __global__ void __launch_bounds__(SLAB_THREADS)
         for (long i = tid; i < nk8; i += nt) {
             int l, m;
             wedge_lm(i, l, m);
-            put_sym(slab, nk, l, m, g0_plus(ekw[i] - mu0, t, T, nk02, 1));
+            put_sym_t(slab, nk, l, m, (S)g0_plus(ekw[i] - mu0, t, T, nk02, 1));
         }
         __syncthreads();
-        slab_r2r_2d<double, N>(slab, nk, nk, nk, 0, 0, tw);
-        for (int s = tid; s < nk * nk; s += nt) slab[s] = 2.0 * slab[s] * slab[s];
+        slab_r2r_2d<S, N>(slab, nk, nk, nk, 0, 0, tw);
+        for (int s = tid; s < nk * nk; s += nt) slab[s] = (S)2 * slab[s] * slab[s];
         __syncthreads();
         for (long i = tid; i < nk8; i += nt) {
             int l, m;
@@ -235,7 +271,7 @@ __global__ void __launch_bounds__(SLAB_THREADS)
         }
         __syncthreads();
     }
-    slab_r2r_2d<double, N>(slab, nk, nk, nk, 0, 0, tw);
+    slab_r2r_2d<S, N>(slab, nk, nk, nk, 0, 0, tw);
     for (long i = tid; i < nk8; i += nt) {
         int l, m;
         wedge_lm(i, l, m);
@@ -246,10 +282,12 @@ __global__ void __launch_bounds__(SLAB_THREADS)
 // ================================================================== chi0 pencils
 // chi0.cpp:586-698: per wedge q, clamped spline of chi0(q,tau_j), j=0..Nw/2, slopes
 // (dchiqt0[q], 0); F = BACKWARD DFT_Nw of the antisymmetrised 6a_j; tails.
-template <int L>
+// Reads chi0(q,tau) from chit, writes chi0(q,iqn) to chiw in FP64 (chiw may alias chit in
+// the FP64 path: each block reads its column completely before writing it).
+template <int L, typename S>
 __global__ void __launch_bounds__(PEN_THREADS)
-    k_chi0_pencil(double* chi, const double* __restrict__ dchiqt0, double* chi0inf, double* chi0inf2, long nk8,
-                  double T, const cd* __restrict__ tw)
+    k_chi0_pencil(const S* chit, double* chiw, const double* __restrict__ dchiqt0, double* chi0inf,
+                  double* chi0inf2, long nk8, double T, const cd* __restrict__ tw)
 {
     constexpr int NS = L / 2;  // intervals
     __shared__ double y[NS + 1], M[NS + 1];
@@ -260,7 +298,7 @@ __global__ void __launch_bounds__(PEN_THREADS)
     if (q >= nk8) return;
     const int tid = threadIdx.x, nt = blockDim.x;
     double* rr = (double*)X;  // spline scratch aliases the FFT buffer
-    for (int j = tid; j <= NS; j += nt) y[j] = chi[q + (long)j * nk8];
+    for (int j = tid; j <= NS; j += nt) y[j] = (double)chit[q + (long)j * nk8];
     const double d0 = dchiqt0[q];
     const double h = 1.0 / (L * T);
     __syncthreads();
@@ -274,13 +312,13 @@ __global__ void __launch_bounds__(PEN_THREADS)
     }
     integ = block_sum(integ, sc);
     block_fft<L, PEN_THREADS>(X, +1, tw);
-    if (tid == 0) chi[q] = 2 * integ;
+    if (tid == 0) chiw[q] = 2 * integ;
     if (q != 0) {
         const double inf = -2.0 * d0;
         for (int j = 1 + tid; j <= NS; j += nt) {
             const double wn = 2 * j * kPi * T;
             const double ang = 2 * j * kPi / L;
-            chi[q + (long)j * nk8] =
+            chiw[q + (long)j * nk8] =
                 inf / (wn * wn) + ((1.0 - cos(ang)) * X[j].x + sin(ang) * X[j].y) / (wn * wn * wn * wn);
         }
         if (tid == 0) {
@@ -288,7 +326,7 @@ __global__ void __launch_bounds__(PEN_THREADS)
             chi0inf2[q] = 2.0 * X[NS].x;
         }
     } else {
-        for (int j = 1 + tid; j <= NS; j += nt) chi[(long)j * nk8] = 0.0;
+        for (int j = 1 + tid; j <= NS; j += nt) chiw[(long)j * nk8] = 0.0;
         if (tid == 0) {
             chi0inf[0] = 0.0;
             chi0inf2[0] = 0.0;
@@ -364,6 +402,42 @@ __global__ void __launch_bounds__(256)
     if (threadIdx.x == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = acc;
 }
 
+// The same trace and its first two derivatives with respect to Ux, where sU = sgn * Ux:
+// d/dUx [c/(1 + sU c/2)] = -sgn c^2/2 / (1 + sU c/2)^2, d2/dUx2 = c^3/2 / (1 + sU c/2)^3.
+// Partials: part[k * nblk + blk], k = 0, 1, 2.
+__global__ void __launch_bounds__(256)
+    k_sum_rule3(const double* __restrict__ chi, const double* __restrict__ inf1, const double* __restrict__ inf2,
+                const double* __restrict__ wq, int nk8, int nwmax, double T, double Nq, double sgn, double Ux,
+                double* part)
+{
+    __shared__ double sc[3 * 4];
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    const double sU = sgn * Ux;
+    double acc[3] = {0.0, 0.0, 0.0};
+    if (q < nk8) {
+        const double w16 = 2.0 * wq[q];
+        const double a1 = inf1[q], a2 = inf2[q];
+        for (int j = blockIdx.y; j <= 2 * nwmax; j += gridDim.y) {
+            const double w = ((j == 0) ? 0.5 * w16 : w16) * T / Nq;
+            double c;
+            if (j <= nwmax)
+                c = chi[(long)j * nk8 + q];
+            else {
+                const double wn = 2 * j * kPi * T;
+                c = a1 / (wn * wn) + a2 / (wn * wn * wn * wn);
+            }
+            const double r = 1.0 / (1 + sU * c / 2.0);
+            acc[0] += w * (c * r - c);
+            acc[1] -= w * sgn * 0.5 * c * c * r * r;
+            acc[2] += w * 0.5 * c * c * c * r * r * r;
+        }
+    }
+    block_sum_k<3>(acc, sc);
+    const int nblk = gridDim.x * gridDim.y, b = blockIdx.y * gridDim.x + blockIdx.x;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 3; k++) part[k * nblk + b] = acc[k];
+}
+
 // density for the mu0 root (green0.cpp:182-203), one block
 __global__ void k_density(const double* __restrict__ ekw, const double* __restrict__ wq, long nk8, double Nk2,
                           double T, double mup, double dens, double* out)
@@ -377,15 +451,15 @@ __global__ void k_density(const double* __restrict__ ekw, const double* __restri
 
 // ================================================================== vertex V(r,tau)
 // chi.cpp:85-119, block j = iqn index
-template <int N>
+template <int N, typename S>
 __global__ void __launch_bounds__(SLAB_THREADS)
-    k_vertex_slab(const double* __restrict__ chi, double* V, double normFact, double U, double Usp, double Uch)
+    k_vertex_slab(const double* __restrict__ chi, S* V, double normFact, double U, double Usp, double Uch)
 {
     constexpr int nk = N / 2 + 1;
     constexpr long nk8 = (long)nk * (nk + 1) / 2;
     extern __shared__ double smem[];
-    double* slab = smem;
-    cd* tw = (cd*)(smem + nk * nk + (nk * nk & 1));
+    S* slab = (S*)smem;
+    auto* tw = slab_tw(slab, (long)nk * nk);
     fill_twiddles(tw, N);
     const long j = blockIdx.x;
     const int tid = threadIdx.x, nt = blockDim.x;
@@ -394,10 +468,10 @@ __global__ void __launch_bounds__(SLAB_THREADS)
         wedge_lm(i, l, m);
         const double c = chi[i + j * nk8];
         const double csp = c / (1.0 - Usp * c / 2.0), cch = c / (1.0 + Uch * c / 2.0);
-        put_sym(slab, nk, l, m, normFact * U * (3.0 * Usp * (csp - c) + Uch * (cch - c)) / 8.0);
+        put_sym_t(slab, nk, l, m, (S)(normFact * U * (3.0 * Usp * (csp - c) + Uch * (cch - c)) / 8.0));
     }
     __syncthreads();
-    slab_r2r_2d<double, N>(slab, nk, nk, nk, 0, 0, tw);
+    slab_r2r_2d<S, N>(slab, nk, nk, nk, 0, 0, tw);
     for (long i = tid; i < nk8; i += nt) {
         int l, m;
         wedge_lm(i, l, m);
@@ -407,9 +481,10 @@ __global__ void __launch_bounds__(SLAB_THREADS)
 
 // chi.cpp:123-170: REDFT00 over n (length Nw/2+1) for two wedge points per block, as one
 // complex FFT of length Nw of the even extensions; then V += fact * chi0(r,tau)
-template <int L>
+template <int L, typename S>
 __global__ void __launch_bounds__(PEN_THREADS)
-    k_vertex_pencil(double* V, const double* __restrict__ chirt, long nk8, double fact, const cd* __restrict__ tw)
+    k_vertex_pencil(S* V, const S* __restrict__ chirt, const double* __restrict__ chirt0, double* V0, long nk8,
+                    double fact, const cd* __restrict__ tw)
 {
     constexpr int H = L / 2;
     __shared__ cd X[L];
@@ -418,8 +493,8 @@ __global__ void __launch_bounds__(PEN_THREADS)
     const bool two = (q + 1) < nk8;
     const int tid = threadIdx.x, nt = blockDim.x;
     for (int j = tid; j <= H; j += nt) {
-        const double a = V[q + (long)j * nk8];
-        const double b = two ? V[q + 1 + (long)j * nk8] : 0.0;
+        const double a = (double)V[q + (long)j * nk8];
+        const double b = two ? (double)V[q + 1 + (long)j * nk8] : 0.0;
         X[j] = cd{a, b};
         if (j > 0 && j < H) X[L - j] = cd{a, b};
     }
@@ -427,75 +502,117 @@ __global__ void __launch_bounds__(PEN_THREADS)
     block_fft<L, PEN_THREADS>(X, -1, tw);
     for (int j = tid; j <= H; j += nt) {
         const long o = q + (long)j * nk8;
-        V[o] = X[j].x + fact * chirt[o];
-        if (two) V[o + 1] = X[j].y + fact * chirt[o + 1];
+        V[o] = (S)(X[j].x + fact * (double)chirt[o]);
+        if (two) V[o + 1] = (S)(X[j].y + fact * (double)chirt[o + 1]);
+    }
+    if (tid == 0) {  // tau = 0 in FP64 for the boundary kernels
+        V0[q] = X[0].x + fact * chirt0[q];
+        if (two) V0[q + 1] = X[0].y + fact * chirt0[q + 1];
     }
 }
 
 // ================================================================== self-energy
-// boundary slopes of Sigma(k,tau) at 0 and beta (green.cpp:1743-1817); one block.
-// tmp used as staging [0:nk8).
+__device__ __forceinline__ cd sigma_mom(double Sinf, double s2, double s3, cd R, double wn)
+{
+    // Sinf/z + s2/z^2 + s3/z^3 + R/z^4, z = i wn (green.cpp:3039)
+    const double w2 = wn * wn, w3 = w2 * wn, w4 = w2 * w2;
+    return cd{-s2 / w2 + R.x / w4, -Sinf / wn + s3 / w3 + R.y / w4};
+}
+
+// Sigma(k, i w_n), w_n > 0, at offset o of the stored form: the remainder R (FP64 path,
+// assembled with the moments) or the assembled value itself (mixed path)
+__device__ __forceinline__ cd sigma_kw(const cd* __restrict__ Self, long o, double Sinf, double s2, double s3,
+                                       double wn)
+{
+    return sigma_mom(Sinf, s2, s3, Self[o], wn);
+}
+__device__ __forceinline__ cd sigma_kw(const cf* __restrict__ Self, long o, double, double, double, double)
+{
+    const cf v = Self[o];
+    return cd{(double)v.x, (double)v.y};
+}
+__device__ __forceinline__ void store_self(cd* Self, long o, cd R, double, double, double, double) { Self[o] = R; }
+__device__ __forceinline__ void store_self(cf* Self, long o, cd R, double Sinf, double s2, double s3, double wn)
+{
+    const cd v = sigma_mom(Sinf, s2, s3, R, wn);
+    Self[o] = cf{(float)v.x, (float)v.y};
+}
+
+// Sigma_inf = -(Sigma(k=0,0) + Sigma(k=0,beta)) (green.cpp:1996); one thread
+__global__ void k_sigma_inf(const double* __restrict__ Sb, long nk8, double* Sinf)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) Sinf[0] = -(Sb[0] + Sb[nk8]);
+}
+// Boundary data of Sigma(k,tau) in FP64, block `side` = 0 (tau = 0) or 1 (tau = beta):
+// the slope dSigma/dtau (green.cpp:1743-1817) and the value Sigma(k,tau_b)
+// = DCT[G0(r,tau_b) V(r,0)] (green.cpp:1822-1892 at j = 0, Nw; V(r,beta) = V(r,0)).
+// G0(k,0+) = -1/(e^{-x/T}+1) and G0(k,beta-) = -1/(e^{x/T}+1), both over N^2.
+// tmp: staging, 2*nk8 per side.
 template <int N>
 __global__ void __launch_bounds__(SLAB_THREADS)
-    k_sigma_bnd(const double* __restrict__ ek, const double* __restrict__ er, const double* __restrict__ V,
+    k_sigma_bnd(const double* __restrict__ ek, const double* __restrict__ er, const double* __restrict__ V0,
                 const double* __restrict__ dchirt0, const double* __restrict__ dGrt0p, double mu0, double T,
-                double fact, double* dSig, double* tmp)
+                double fact, double* dSig, double* Sb, double* tmp)
 {
     constexpr int nk = N / 2 + 1;
     constexpr long nk8 = (long)nk * (nk + 1) / 2;
     const double nk02 = (double)N * N;
     extern __shared__ double smem[];
     double* slab = smem;
-    cd* tw = (cd*)(smem + nk * nk + (nk * nk & 1));
+    cd* tw = slab_tw(slab, (long)nk * nk);
     fill_twiddles(tw, N);
     const int tid = threadIdx.x, nt = blockDim.x;
-    for (int side = 0; side < 2; side++) {
-        for (int s = tid; s < nk * nk; s += nt) {
-            const double x = ek[s] - mu0;
-            slab[s] = (side == 0) ? -1.0 / (exp(-x / T) + 1.0) / nk02 : -1.0 / (exp(x / T) + 1.0) / nk02;
+    const int side = blockIdx.x;
+    double* stg = tmp + 2 * side * nk8;
+    for (int s = tid; s < nk * nk; s += nt) {
+        const double x = ek[s] - mu0;
+        slab[s] = (side == 0) ? -1.0 / (exp(-x / T) + 1.0) / nk02 : -1.0 / (exp(x / T) + 1.0) / nk02;
+    }
+    __syncthreads();
+    slab_r2r_2d<double, N>(slab, nk, nk, nk, 0, 0, tw);
+    for (long i = tid; i < nk8; i += nt) {
+        int l, m;
+        wedge_lm(i, l, m);
+        const double g = slab[m + l * nk];
+        const double v = V0[i];
+        if (side == 0)
+            stg[i] = fact * dchirt0[i] * g + v * dGrt0p[i];
+        else {
+            const double xi = (i == 0) ? -mu0 : er[i];
+            stg[i] = -fact * dchirt0[i] * g + v * (-dGrt0p[i] + xi);
+        }
+        stg[nk8 + i] = g * v;
+    }
+    __syncthreads();
+    for (int pass = 0; pass < 2; pass++) {
+        for (long i = tid; i < nk8; i += nt) {
+            int l, m;
+            wedge_lm(i, l, m);
+            put_sym(slab, nk, l, m, stg[pass * nk8 + i]);
         }
         __syncthreads();
         slab_r2r_2d<double, N>(slab, nk, nk, nk, 0, 0, tw);
+        double* o = (pass == 0) ? dSig : Sb;
         for (long i = tid; i < nk8; i += nt) {
             int l, m;
             wedge_lm(i, l, m);
-            const double g = slab[m + l * nk];
-            if (side == 0)
-                tmp[i] = fact * dchirt0[i] * g + V[i] * dGrt0p[i];
-            else {
-                const double xi = (i == 0) ? -mu0 : er[i];
-                tmp[i] = -fact * dchirt0[i] * g + V[i] * (-dGrt0p[i] + xi);
-            }
-        }
-        __syncthreads();
-        for (long i = tid; i < nk8; i += nt) {
-            int l, m;
-            wedge_lm(i, l, m);
-            put_sym(slab, nk, l, m, tmp[i]);
-        }
-        __syncthreads();
-        slab_r2r_2d<double, N>(slab, nk, nk, nk, 0, 0, tw);
-        for (long i = tid; i < nk8; i += nt) {
-            int l, m;
-            wedge_lm(i, l, m);
-            dSig[i + side * nk8] = slab[m + l * nk];
+            o[i + side * nk8] = slab[m + l * nk];
         }
         __syncthreads();
     }
 }
 
 // Sigma(k,tau_j), j = 0..Nw (green.cpp:1822-1892): G0(k,tau) -> r, x V(r,|tau|), -> k
-template <int N>
+template <int N, typename S>
 __global__ void __launch_bounds__(SLAB_THREADS)
-    k_sigma_slab(const double* __restrict__ ek, const double* __restrict__ V, double mu0, double T, int Nw,
-                 double* Stau)
+    k_sigma_slab(const double* __restrict__ ek, const S* __restrict__ V, double mu0, double T, int Nw, S* Stau)
 {
     constexpr int nk = N / 2 + 1;
     constexpr long nk8 = (long)nk * (nk + 1) / 2;
     const double nk02 = (double)N * N;
     extern __shared__ double smem[];
-    double* slab = smem;
-    cd* tw = (cd*)(smem + nk * nk + (nk * nk & 1));
+    S* slab = (S*)smem;
+    auto* tw = slab_tw(slab, (long)nk * nk);
     fill_twiddles(tw, N);
     const int j = blockIdx.x;
     const int nwb = Nw / 2 + 1;
@@ -504,21 +621,21 @@ __global__ void __launch_bounds__(SLAB_THREADS)
     const int tid = threadIdx.x, nt = blockDim.x;
     for (int s = tid; s < nk * nk; s += nt) {
         const double x = ek[s] - mu0;
-        slab[s] = (x > 0) ? -exp(-t * x) / (exp(-x / T) + 1.0) / nk02
-                          : -exp((1.0 / T - t) * x) / (exp(x / T) + 1.0) / nk02;
+        slab[s] = (S)((x > 0) ? -exp(-t * x) / (exp(-x / T) + 1.0) / nk02
+                              : -exp((1.0 / T - t) * x) / (exp(x / T) + 1.0) / nk02);
     }
     __syncthreads();
-    slab_r2r_2d<double, N>(slab, nk, nk, nk, 0, 0, tw);
-    const double* Vj = V + (long)jv * nk8;
+    slab_r2r_2d<S, N>(slab, nk, nk, nk, 0, 0, tw);
+    const S* Vj = V + (long)jv * nk8;
     for (long i = tid; i < nk8; i += nt) {
         int l, m;
         wedge_lm(i, l, m);
-        const double v = slab[m + l * nk] * Vj[i];
-        put_sym(slab, nk, l, m, v);
+        const S v = slab[m + l * nk] * Vj[i];
+        put_sym_t(slab, nk, l, m, v);
     }
     __syncthreads();
-    slab_r2r_2d<double, N>(slab, nk, nk, nk, 0, 0, tw);
-    double* o = Stau + (long)j * nk8;
+    slab_r2r_2d<S, N>(slab, nk, nk, nk, 0, 0, tw);
+    S* o = Stau + (long)j * nk8;
     for (long i = tid; i < nk8; i += nt) {
         int l, m;
         wedge_lm(i, l, m);
@@ -528,11 +645,12 @@ __global__ void __launch_bounds__(SLAB_THREADS)
 
 // green.cpp:1925-2018: per wedge k, spline of Sigma(k,tau_j) (j=0..Nw) with slopes dSig,
 // F = BACKWARD DFT[exp(i pi j/Nw) 6a_j], R(k,n) = (1 - exp(i pi (2n+1)/Nw)) F_n
-template <int L>
+template <int L, typename S, typename SelfT>
 __global__ void __launch_bounds__(PEN_THREADS)
-    k_sigma_pencil(const double* __restrict__ Stau, const double* __restrict__ dSig, long nk8, double T, cd* Self,
-                   double* Sinf2, double* Sinf3, double* Sinf4, double* Sinf, const cd* __restrict__ tw,
-                   const cd* __restrict__ twX, const cd* __restrict__ twS)
+    k_sigma_pencil(const S* __restrict__ Stau, const double* __restrict__ Sb, const double* __restrict__ dSig,
+                   long nk8, double T, SelfT* Self,
+                   double* Sinf2, double* Sinf3, double* Sinf4, const double* __restrict__ Sinfp,
+                   const cd* __restrict__ tw, const cd* __restrict__ twX, const cd* __restrict__ twS)
 {
     __shared__ double y[L + 1], M[L + 1];
     __shared__ cd X[L];
@@ -541,7 +659,8 @@ __global__ void __launch_bounds__(PEN_THREADS)
     if (k >= nk8) return;
     const int tid = threadIdx.x, nt = blockDim.x;
     double* rr = (double*)X;
-    for (int j = tid; j <= L; j += nt) y[j] = Stau[k + (long)j * nk8];
+    for (int j = tid; j <= L; j += nt)
+        y[j] = (j == 0) ? Sb[k] : (j == L) ? Sb[nk8 + k] : (double)Stau[k + (long)j * nk8];
     const double d0 = dSig[k], dN = dSig[k + nk8];
     const double h = 1.0 / (L * T);
     __syncthreads();
@@ -552,15 +671,15 @@ __global__ void __launch_bounds__(PEN_THREADS)
     }
     __syncthreads();
     block_fft<L, PEN_THREADS>(X, +1, tw);
+    const double s2 = d0 + dN, s3 = -(M[0] + M[L]), Si = Sinfp[0];
     for (int n = tid; n < L / 2; n += nt) {
         const cd R = twS[n] * X[n];
-        Self[k + (long)n * nk8] = R;
+        store_self(Self, k + (long)n * nk8, R, Si, s2, s3, (2 * n + 1) * kPi * T);
         if (n == L / 2 - 1) Sinf4[k] = R.x;
     }
     if (tid == 0) {
-        Sinf2[k] = d0 + dN;
-        Sinf3[k] = -(M[0] + M[L]);
-        if (k == 0) Sinf[0] = -(y[0] + y[L]);
+        Sinf2[k] = s2;
+        Sinf3[k] = s3;
     }
 }
 
@@ -585,16 +704,11 @@ __global__ void k_fill_tables(int L, cd* twX, cd* twS, cd* twM, cd* ph)
 }
 
 // ================================================================== traces
-__device__ __forceinline__ cd sigma_mom(double Sinf, double s2, double s3, cd R, double wn)
-{
-    // Sinf/z + s2/z^2 + s3/z^3 + R/z^4, z = i wn
-    const double w2 = wn * wn, w3 = w2 * wn, w4 = w2 * w2;
-    return cd{-s2 / w2 + R.x / w4, -Sinf / wn + s3 / w3 + R.y / w4};
-}
 
 // traceDiffG(pmu) partials (green.cpp:3108-3182); grid as k_sum_rule, rows j = 0..Nw
+template <typename SelfT>
 __global__ void __launch_bounds__(256)
-    k_trace_diffG(const double* __restrict__ ekw, const double* __restrict__ wq, const cd* __restrict__ Self,
+    k_trace_diffG(const double* __restrict__ ekw, const double* __restrict__ wq, const SelfT* __restrict__ Self,
                   const double* __restrict__ Sinf2, const double* __restrict__ Sinf3,
                   const double* __restrict__ Sinf4, const double* __restrict__ Sinf, int nk8, int Nw, double T,
                   double mu0, double pmu, double* part)
@@ -613,7 +727,7 @@ __global__ void __launch_bounds__(256)
                 rS = -s2 / (wn * wn) + s4 / (wn * wn * wn * wn);
                 iS = -Si / wn + s3 / (wn * wn * wn);
             } else {
-                const cd s = sigma_mom(Si, s2, s3, Self[(long)j * nk8 + q], wn);
+                const cd s = sigma_kw(Self, (long)j * nk8 + q, Si, s2, s3, wn);
                 rS = s.x;
                 iS = s.y;
             }
@@ -627,10 +741,55 @@ __global__ void __launch_bounds__(256)
     if (threadIdx.x == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = acc;
 }
 
+// traceDiffG and its first two derivatives with respect to pmu.  With a = e - pmu + Re Sigma,
+// b = wn - Im Sigma: Re G = -a/(a^2+b^2), d/dpmu Re G = (b^2-a^2)/(a^2+b^2)^2,
+// d2/dpmu2 Re G = (6ab^2 - 2a^3)/(a^2+b^2)^3.  Partials as k_sum_rule3.
+template <typename SelfT>
+__global__ void __launch_bounds__(256)
+    k_trace_diffG3(const double* __restrict__ ekw, const double* __restrict__ wq, const SelfT* __restrict__ Self,
+                   const double* __restrict__ Sinf2, const double* __restrict__ Sinf3,
+                   const double* __restrict__ Sinf4, const double* __restrict__ Sinf, int nk8, int Nw, double T,
+                   double mu0, double pmu, double* part)
+{
+    __shared__ double sc[3 * 4];
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    double acc[3] = {0.0, 0.0, 0.0};
+    if (q < nk8) {
+        const double Si = Sinf[0];
+        const double e = ekw[q], w = 2.0 * wq[q];
+        const double s2 = Sinf2[q], s3 = Sinf3[q], s4 = Sinf4[q];
+        for (int j = blockIdx.y; j <= Nw; j += gridDim.y) {
+            const double wn = (2 * j + 1) * kPi * T;
+            double rS, iS;
+            if (j >= Nw / 2) {
+                rS = -s2 / (wn * wn) + s4 / (wn * wn * wn * wn);
+                iS = -Si / wn + s3 / (wn * wn * wn);
+            } else {
+                const cd s = sigma_kw(Self, (long)j * nk8 + q, Si, s2, s3, wn);
+                rS = s.x;
+                iS = s.y;
+            }
+            const double tmp = w / (((wn - iS) * (wn - iS) + (e - pmu + rS) * (e - pmu + rS)) *
+                                    (wn * wn + (e - mu0) * (e - mu0)));
+            acc[0] += tmp * ((mu0 - pmu + rS) * (e - pmu + rS) * (e - mu0) + iS * (iS - 2 * wn) * (e - mu0) -
+                             wn * wn * (mu0 - pmu + rS));
+            const double a = e - pmu + rS, b = wn - iS;
+            const double a2 = a * a, b2 = b * b, r = 1.0 / (a2 + b2);
+            acc[1] += w * (b2 - a2) * r * r;
+            acc[2] += w * (6.0 * a * b2 - 2.0 * a * a2) * r * r * r;
+        }
+    }
+    block_sum_k<3>(acc, sc);
+    const int nblk = gridDim.x * gridDim.y, b = blockIdx.y * gridDim.x + blockIdx.x;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 3; k++) part[k * nblk + b] = acc[k];
+}
+
 // which = 0: Tr Sigma G0 main sum (green.cpp:3018-3051); 1: its Sigma_inf correction
 // (:3053-3084, the j < Nw/2 terms plus one per k); 2: Tr Sigma G (green.cpp:2944-2976)
+template <typename SelfT>
 __global__ void __launch_bounds__(256)
-    k_trace_selfG(const double* __restrict__ ekw, const double* __restrict__ wq, const cd* __restrict__ Self,
+    k_trace_selfG(const double* __restrict__ ekw, const double* __restrict__ wq, const SelfT* __restrict__ Self,
                   const double* __restrict__ Sinf2, const double* __restrict__ Sinf3, const double* __restrict__ Sinf,
                   int nk8, int Nw, double T, double mu0, double mu, double nk02, int which, double* part)
 {
@@ -654,7 +813,7 @@ __global__ void __launch_bounds__(256)
                 }
                 continue;
             }
-            const cd s = sigma_mom(Si, Sinf2[q], Sinf3[q], Self[(long)j * nk8 + q], wn);
+            const cd s = sigma_kw(Self, (long)j * nk8 + q, Si, Sinf2[q], Sinf3[q], wn);
             if (which == 0) {
                 const double tmp = w * T * 2 / (nk02 * (wn * wn + (e - mu0) * (e - mu0)));
                 acc += tmp * (s.x * (mu0 - e) + s.y * wn);
@@ -713,19 +872,36 @@ __global__ void k_model(int N, double tp, double tpp, double t3, double* ek, dou
 }
 
 // ================================================================== Lambda_jj slabs
+// k-space integrand of h (which = 0) or f0 (which = 1) at tau = t (cond_opt.cpp:2697-2735,
+// 2795-2827); nf = 1/N^2, v = v_x(k)
+__device__ __forceinline__ double hf0_val(int which, double e, double v, double mu0, double T, double t, double nf)
+{
+    if (which == 0) {
+        const double x = e - mu0;
+        return (x > 0) ? nf * v * exp(-x * t) / (exp(-x / T) + 1.0)
+                       : nf * v * exp(x * (1.0 / T - t)) / (exp(x / T) + 1.0);
+    }
+    if (e > mu0) {
+        const double ex = exp((mu0 - e) / T);
+        return nf * v * exp((mu0 - e) * t) * (t / (ex + 1.0) - ex / (T * (ex + 1.0) * (ex + 1.0)));
+    }
+    const double ex = exp((e - mu0) / T);
+    return nf * v * exp((e - mu0) * (1.0 / T - t)) * (t / (ex + 1.0) - 1.0 / (T * (ex + 1.0) * (ex + 1.0)));
+}
+
 // h(r,tau_j) and f0(r,tau_j), j = 0..Nw (cond_opt.cpp:2697-2735, 2795-2827):
 // RODFT00 along kx (index l+1), REDFT00 along ky.
-template <int N>
+template <int N, typename S>
 __global__ void __launch_bounds__(SLAB_THREADS)
     k_hf0_slab(const double* __restrict__ ek, const double* __restrict__ dek, double mu0, double T, int Nw,
-               double* h, double* f0)
+               S* h, S* f0)
 {
     constexpr int nk = N / 2 + 1;
     constexpr long nq = (long)nk * (nk - 2);
     const double nf = 1.0 / ((double)N * N);
     extern __shared__ double smem[];
-    double* slab = smem;
-    cd* tw = (cd*)(smem + nq + (nq & 1));
+    S* slab = (S*)smem;
+    auto* tw = slab_tw(slab, nq);
     fill_twiddles(tw, N);
     const int j = blockIdx.x;
     const double t = (double)j / (T * Nw);
@@ -733,47 +909,50 @@ __global__ void __launch_bounds__(SLAB_THREADS)
     for (int which = 0; which < 2; which++) {
         for (long s = tid; s < nq; s += nt) {
             const int l = (int)(s / nk), m = (int)(s % nk);
-            const double e = ek[m + (l + 1) * nk], v = dek[s];
-            double val;
-            if (which == 0) {
-                const double x = e - mu0;
-                val = (x > 0) ? nf * v * exp(-x * t) / (exp(-x / T) + 1.0)
-                              : nf * v * exp(x * (1.0 / T - t)) / (exp(x / T) + 1.0);
-            } else if (e > mu0) {
-                const double ex = exp((mu0 - e) / T);
-                val = nf * v * exp((mu0 - e) * t) * (t / (ex + 1.0) - ex / (T * (ex + 1.0) * (ex + 1.0)));
-            } else {
-                const double ex = exp((e - mu0) / T);
-                val = nf * v * exp((e - mu0) * (1.0 / T - t)) * (t / (ex + 1.0) - 1.0 / (T * (ex + 1.0) * (ex + 1.0)));
-            }
-            slab[s] = val;
+            slab[s] = (S)hf0_val(which, ek[m + (l + 1) * nk], dek[s], mu0, T, t, nf);
         }
         __syncthreads();
-        slab_r2r_2d<double, N>(slab, nk - 2, nk, nk, 1, 0, tw);
-        double* o = (which == 0 ? h : f0) + (long)j * nq;
+        slab_r2r_2d<S, N>(slab, nk - 2, nk, nk, 1, 0, tw);
+        S* o = (which == 0 ? h : f0) + (long)j * nq;
         for (long s = tid; s < nq; s += nt) o[s] = slab[s];
         __syncthreads();
     }
 }
 
-// dh, df0 at tau = 0, beta and the boundary data dhV, df0V (cond_opt.cpp:2745-3014).
-// Block b: which = b/2 (0: h, 1: f0), side = b%2 (0: tau=0, 1: tau=beta). Must run
-// before k_hVf0V_slab overwrites h and f0.
+// Boundary data of hV and f0V in FP64 (cond_opt.cpp:2697-2735, 2745-3114 at tau = 0, beta).
+// Block b: which = b/2 (0: h, 1: f0), side = b%2 (0: tau = 0, 1: tau = beta).
+//   (1) h(r,tau_b) (or f0) by DST x DCT of the k-space integrand at t = 0 or beta
+//   (2) the tau-derivative integrand -> r, combined with V(r,0), dchi0(r,0) -> k: slopes
+//       dhV / df0V (cond_opt.cpp:2745-3014)
+//   (3) 2 h(r,tau_b) V(r,0) -> k: hVb / f0Vb, the tau = 0, beta values of hV / f0V
+// Independent of the tau slabs.  bscr: staging, nq per block.
 template <int N>
 __global__ void __launch_bounds__(SLAB_THREADS)
-    k_hf0_bnd(const double* __restrict__ ek, const double* __restrict__ dek, const double* __restrict__ V,
-              const double* __restrict__ dchirt0, const double* __restrict__ h, const double* __restrict__ f0,
-              double mu0, double T, int Nw, double fact4, double* dhV, double* df0V)
+    k_hf0_bnd(const double* __restrict__ ek, const double* __restrict__ dek, const double* __restrict__ V0,
+              const double* __restrict__ dchirt0, double mu0, double T, int Nw, double fact4, double* dhV,
+              double* df0V, double* hVb, double* f0Vb, double* bscr)
 {
     constexpr int nk = N / 2 + 1;
     constexpr long nq = (long)nk * (nk - 2);
     const double nf = 1.0 / ((double)N * N);
     extern __shared__ double smem[];
     double* slab = smem;
-    cd* tw = (cd*)(smem + nq + (nq & 1));
+    cd* tw = slab_tw(slab, nq);
     fill_twiddles(tw, N);
     const int which = blockIdx.x >> 1, side = blockIdx.x & 1;
     const int tid = threadIdx.x, nt = blockDim.x;
+    double* hr = bscr + blockIdx.x * nq;
+    const double tb = (side == 0) ? 0.0 : (double)Nw / (T * Nw);  // tau_j of k_hf0_slab at j = 0, Nw
+    // (1)
+    for (long s = tid; s < nq; s += nt) {
+        const int l = (int)(s / nk), m = (int)(s % nk);
+        slab[s] = hf0_val(which, ek[m + (l + 1) * nk], dek[s], mu0, T, tb, nf);
+    }
+    __syncthreads();
+    slab_r2r_2d<double, N>(slab, nk - 2, nk, nk, 1, 0, tw);
+    for (long s = tid; s < nq; s += nt) hr[s] = slab[s];
+    __syncthreads();
+    // (2)
     for (long s = tid; s < nq; s += nt) {
         const int l = (int)(s / nk), m = (int)(s % nk);
         const double e = ek[m + (l + 1) * nk], v = dek[s];
@@ -803,45 +982,52 @@ __global__ void __launch_bounds__(SLAB_THREADS)
     }
     __syncthreads();
     slab_r2r_2d<double, N>(slab, nk - 2, nk, nk, 1, 0, tw);
-    const double* src = (which == 0) ? h : f0;
-    const double* srcj = src + (side == 0 ? 0 : (long)Nw * nq);
     for (long s = tid; s < nq; s += nt) {
         const int l = (int)(s / nk), m = (int)(s % nk);
         const long r = fold_idx(l + 1, m, N);
-        const double d = slab[s];
-        slab[s] = (side == 0) ? 2.0 * d * V[r] + srcj[s] * fact4 * dchirt0[r]
-                              : 2.0 * d * V[r] - srcj[s] * fact4 * dchirt0[r];
+        const double d = slab[s], v = V0[r], hs = hr[s];
+        slab[s] = (side == 0) ? 2.0 * d * v + hs * fact4 * dchirt0[r] : 2.0 * d * v - hs * fact4 * dchirt0[r];
     }
     __syncthreads();
     slab_r2r_2d<double, N>(slab, nk - 2, nk, nk, 1, 0, tw);
     double* o = ((which == 0) ? dhV : df0V) + side * nq;
     for (long s = tid; s < nq; s += nt) o[s] = slab[s];
+    __syncthreads();
+    // (3)
+    for (long s = tid; s < nq; s += nt) {
+        const int l = (int)(s / nk), m = (int)(s % nk);
+        slab[s] = 2.0 * hr[s] * V0[fold_idx(l + 1, m, N)];
+    }
+    __syncthreads();
+    slab_r2r_2d<double, N>(slab, nk - 2, nk, nk, 1, 0, tw);
+    double* ob = ((which == 0) ? hVb : f0Vb) + side * nq;
+    for (long s = tid; s < nq; s += nt) ob[s] = slab[s];
 }
 
 // hV(k,tau) = DST x DCT [2 h(r,tau) V(r,|tau|)], same for f0, in place (cond_opt.cpp:3029-3114)
-template <int N>
+template <int N, typename S>
 __global__ void __launch_bounds__(SLAB_THREADS)
-    k_hVf0V_slab(const double* __restrict__ V, int Nw, double* h, double* f0)
+    k_hVf0V_slab(const S* __restrict__ V, int Nw, S* h, S* f0)
 {
     constexpr int nk = N / 2 + 1;
     constexpr long nq = (long)nk * (nk - 2);
     constexpr long nk8 = (long)nk * (nk + 1) / 2;
     extern __shared__ double smem[];
-    double* slab = smem;
-    cd* tw = (cd*)(smem + nq + (nq & 1));
+    S* slab = (S*)smem;
+    auto* tw = slab_tw(slab, nq);
     fill_twiddles(tw, N);
     const int j = blockIdx.x;
     const int jv = (j > Nw / 2) ? Nw - j : j;
-    const double* Vj = V + (long)jv * nk8;
+    const S* Vj = V + (long)jv * nk8;
     const int tid = threadIdx.x, nt = blockDim.x;
     for (int which = 0; which < 2; which++) {
-        double* a = ((which == 0) ? h : f0) + (long)j * nq;
+        S* a = ((which == 0) ? h : f0) + (long)j * nq;
         for (long s = tid; s < nq; s += nt) {
             const int l = (int)(s / nk), m = (int)(s % nk);
-            slab[s] = 2.0 * a[s] * Vj[fold_idx(l + 1, m, N)];
+            slab[s] = (S)2 * a[s] * Vj[fold_idx(l + 1, m, N)];
         }
         __syncthreads();
-        slab_r2r_2d<double, N>(slab, nk - 2, nk, nk, 1, 0, tw);
+        slab_r2r_2d<S, N>(slab, nk - 2, nk, nk, 1, 0, tw);
         for (long s = tid; s < nq; s += nt) a[s] = slab[s];
         __syncthreads();
     }
@@ -900,21 +1086,14 @@ __device__ __forceinline__ Poles make_poles(double e, double mu, double Sinf, do
 
 // Dyson G at Matsubara slot s (n = s - Nw/2), with the conjugate Sigma for n < 0
 // (cond_opt.cpp:3354-3407)
-__device__ __forceinline__ cd dyson_G(const cd* __restrict__ Self, long ki, long nk8, int s, int H, double C2, double C3,
-                             double Sinf, double e, double mu, double T, double& kn)
+template <typename SelfT>
+__device__ __forceinline__ cd dyson_G(const SelfT* __restrict__ Self, long ki, long nk8, int s, int H, double C2,
+                                      double C3, double Sinf, double e, double mu, double T, double& kn)
 {
     kn = (2.0 * (s - H) + 1) * kPi * T;
-    cd R;
-    double sgn;
-    if (s < H) {
-        R = Self[ki + (long)(H - s - 1) * nk8];
-        sgn = -1.0;
-    } else {
-        R = Self[ki + (long)(s - H) * nk8];
-        sgn = 1.0;
-    }
-    const double k2 = kn * kn, k3 = k2 * kn, k4 = k2 * k2;
-    const cd sg = cd{-C2 / k2 + R.x / k4, -Sinf / kn + C3 / k3 + sgn * R.y / k4};
+    const int n = (s < H) ? H - s - 1 : s - H;
+    cd sg = sigma_kw(Self, ki + (long)n * nk8, Sinf, C2, C3, fabs(kn));
+    if (s < H) sg = conjg(sg);
     const cd den = cd{0.0 - e, kn} + cd{mu, 0.0} - sg;
     return T / den;
 }
@@ -960,11 +1139,12 @@ __device__ __forceinline__ void spline_to_X(double* y, double* M, cd* X, int NSi
 // per-k Lambda_jj contributions (cond_opt.cpp:3179-3822).  One workgroup of NT threads per
 // Lambda-grid point at a time; each workgroup owns kJJK consecutive points and accumulates
 // into its own partial rows part[blk*2*nwb + n] (bubble) and part[blk*2*nwb + nwb + n] (MT).
-template <int L, int NT>
+template <int L, int NT, typename S, typename SelfT>
 __global__ void __launch_bounds__(NT)
     k_jj_pencil(const double* __restrict__ ek, const double* __restrict__ dek, const double* __restrict__ d2ek,
-                const double* __restrict__ h, const double* __restrict__ f0, const double* __restrict__ dhV,
-                const double* __restrict__ df0V, const cd* __restrict__ Self, const double* __restrict__ Sinf2,
+                const S* __restrict__ h, const S* __restrict__ f0, const double* __restrict__ dhV,
+                const double* __restrict__ df0V, const double* __restrict__ hVb, const double* __restrict__ f0Vb,
+                const SelfT* __restrict__ Self, const double* __restrict__ Sinf2,
                 const double* __restrict__ Sinf3, const double* __restrict__ Sinfp, int N, double T, double mu,
                 const cd* __restrict__ tw, const cd* __restrict__ twX, const cd* __restrict__ twS,
                 const cd* __restrict__ twM, const cd* __restrict__ phT, double* part, double* scal)
@@ -1004,7 +1184,8 @@ __global__ void __launch_bounds__(NT)
             double kn;
             Gb[s] = dyson_G(Self, ki, nk8, s, H, C2, C3, Sinf, e, mu, T, kn);
         }
-        for (int j = tid; j <= L; j += NT) y[j] = f0[q + (long)j * nq];
+        for (int j = tid; j <= L; j += NT)
+            y[j] = (j == 0) ? f0Vb[q] : (j == L) ? f0Vb[nq + q] : (double)f0[q + (long)j * nq];
         const double fd0 = df0V[q], fdN = df0V[q + nq];
         __syncthreads();
         const double cj01 = -y[0] - y[L], cj02 = fd0 + fdN;
@@ -1022,7 +1203,8 @@ __global__ void __launch_bounds__(NT)
         block_sum_k<2>(r2, sc);
         const double qn0 = r2[0];
         // (3) hV pencil -> J, products and asymptote subtraction
-        for (int j = tid; j <= L; j += NT) y[j] = h[q + (long)j * nq];
+        for (int j = tid; j <= L; j += NT)
+            y[j] = (j == 0) ? hVb[q] : (j == L) ? hVb[nq + q] : (double)h[q + (long)j * nq];
         const double hd0 = dhV[q], hdN = dhV[q + nq];
         __syncthreads();
         const double CJ1 = -y[0] - y[L], cj2 = hd0 + hdN;
@@ -1186,7 +1368,8 @@ __global__ void __launch_bounds__(NT)
 
 // kx = 0 and kx = pi rows of the diamagnetic term (cond_opt.cpp:3993-4173); block b:
 // row l = (b < nk ? 0 : nk-1), m = b % nk
-__global__ void k_kx0_rows(const double* __restrict__ ek, const double* __restrict__ d2ek, const cd* __restrict__ Self,
+template <typename SelfT>
+__global__ void k_kx0_rows(const double* __restrict__ ek, const double* __restrict__ d2ek, const SelfT* __restrict__ Self,
                            const double* __restrict__ Sinf2, const double* __restrict__ Sinf3,
                            const double* __restrict__ Sinfp, int N, int L, double T, double mu, double* out)
 {
@@ -1220,11 +1403,13 @@ __global__ void k_kx0_rows(const double* __restrict__ ek, const double* __restri
 }
 
 // ================================================================== launchers
-static size_t slab_lds(int N, bool lambda)
+// LDS bytes of a slab kernel: the slab (nk x nk, or (nk-2) x nk for Lambda_jj) of elem-byte
+// values, 16-byte aligned, then the N-point twiddle table of the same precision
+static size_t slab_lds(int N, bool lambda, size_t elem)
 {
     const int nk = N / 2 + 1;
-    const long n = lambda ? (long)nk * (nk - 2) : (long)nk * nk;
-    return sizeof(double) * (size_t)(n + (n & 1)) + sizeof(cd) * (size_t)N;
+    const size_t n = lambda ? (size_t)nk * (nk - 2) : (size_t)nk * nk;
+    return ((n * elem + 15) & ~(size_t)15) + 2 * elem * (size_t)N;
 }
 
 bool supported_N(int N) { return N == 4 || N == 8 || N == 16 || N == 32 || N == 64 || N == 128 || N == 256; }
@@ -1272,10 +1457,23 @@ bool supported_Nw(int Nw)
         default: return hipErrorInvalidValue;         \
     }
 
+// SS: storage / slab type of the tau and iqn arrays; ST: stored form of Sigma(k,iwn)
+#define DISPATCH_S(prec, ...)     \
+    if (prec) {                   \
+        using SS = float;         \
+        using ST = cf;            \
+        __VA_ARGS__;              \
+    } else {                      \
+        using SS = double;        \
+        using ST = cd;            \
+        __VA_ARGS__;              \
+    }
+
 template <typename K>
 static void set_lds(K kern, size_t bytes)
 {
-    hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)bytes);
 }
 
 hipError_t launch_fill_tw(const DevArrays& d, int L, hipStream_t s)
@@ -1293,38 +1491,38 @@ hipError_t launch_finalize(const double* part, int n, double* out, hipStream_t s
 
 hipError_t launch_chi0_tau0(const Grid& g, const DevArrays& d, double mu0, hipStream_t s)
 {
-    const size_t lds = slab_lds(g.N, false);
-    DISPATCH_N(g.N, {
-        set_lds(k_chi0_tau0<NN>, lds);
-        hipLaunchKernelGGL(k_chi0_tau0<NN>, dim3(1), dim3(SLAB_THREADS), lds, s, d.ekw, d.er, mu0, g.T, d.chi,
-                           d.chirt, d.dchirt0, d.dGrt0p, d.dchiqt0, d.tmp);
-    });
+    const size_t lds = slab_lds(g.N, false, sizeof(double));
+    DISPATCH_S(g.prec, DISPATCH_N(g.N, {
+        set_lds(k_chi0_tau0<NN, SS>, lds);
+        hipLaunchKernelGGL((k_chi0_tau0<NN, SS>), dim3(1), dim3(SLAB_THREADS), lds, s, d.ekw, d.er, mu0, g.T,
+                           (SS*)d.chi, (SS*)d.chirt, d.dchirt0, d.dGrt0p, d.dchiqt0, d.chirt0, d.tmp);
+    }));
     return hipGetLastError();
 }
 
 hipError_t launch_chi0_slab(const Grid& g, const DevArrays& d, double mu0, hipStream_t s)
 {
-    const size_t lds = slab_lds(g.N, false);
-    DISPATCH_N(g.N, {
-        set_lds(k_chi0_slab<NN>, lds);
-        hipLaunchKernelGGL(k_chi0_slab<NN>, dim3(g.nwb - 1), dim3(SLAB_THREADS), lds, s, d.ekw, mu0, g.T, g.Nw, d.chi,
-                           d.chirt);
-    });
+    DISPATCH_S(g.prec, DISPATCH_N(g.N, {
+        const size_t lds = slab_lds(g.N, false, sizeof(SS));
+        set_lds(k_chi0_slab<NN, SS>, lds);
+        hipLaunchKernelGGL((k_chi0_slab<NN, SS>), dim3(g.nwb - 1), dim3(SLAB_THREADS), lds, s, d.ekw, mu0, g.T, g.Nw,
+                           (SS*)d.chi, (SS*)d.chirt);
+    }));
     return hipGetLastError();
 }
 
 hipError_t launch_chi0_pencil(const Grid& g, const DevArrays& d, hipStream_t s)
 {
-    DISPATCH_L(g.Nw, {
-        hipLaunchKernelGGL(k_chi0_pencil<LL>, dim3((unsigned)g.nk8), dim3(PEN_THREADS), 0, s, d.chi, d.dchiqt0,
-                           d.chi0inf, d.chi0inf2, g.nk8, g.T, d.twN);
-    });
+    DISPATCH_S(g.prec, DISPATCH_L(g.Nw, {
+        hipLaunchKernelGGL((k_chi0_pencil<LL, SS>), dim3((unsigned)g.nk8), dim3(PEN_THREADS), 0, s,
+                           (const SS*)d.chi, d.chiw, d.dchiqt0, d.chi0inf, d.chi0inf2, g.nk8, g.T, d.twN);
+    }));
     return hipGetLastError();
 }
 
 hipError_t launch_chi0max(const Grid& g, const DevArrays& d, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_chi0max, dim3(1), dim3(1024), 0, s, d.chi, g.nk8, d.red, d.imax);
+    hipLaunchKernelGGL(k_chi0max, dim3(1), dim3(1024), 0, s, d.chiw, g.nk8, d.red, d.imax);
     return hipGetLastError();
 }
 
@@ -1334,9 +1532,19 @@ hipError_t launch_sum_rule(const Grid& g, const DevArrays& d, double sU, double*
 {
     const double Nq = 4.0 * (g.nk - 1) * (g.nk - 1);
     const dim3 gr = red_grid(g);
-    hipLaunchKernelGGL(k_sum_rule, gr, dim3(256), 0, s, d.chi, d.chi0inf, d.chi0inf2, d.wq, (int)g.nk8, g.Nw / 2,
+    hipLaunchKernelGGL(k_sum_rule, gr, dim3(256), 0, s, d.chiw, d.chi0inf, d.chi0inf2, d.wq, (int)g.nk8, g.Nw / 2,
                        g.T, Nq, sU, d.part);
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, d.part, (int)(gr.x * gr.y), out);
+    return hipGetLastError();
+}
+
+hipError_t launch_sum_rule3(const Grid& g, const DevArrays& d, double sgn, double Ux, double* out, hipStream_t s)
+{
+    const double Nq = 4.0 * (g.nk - 1) * (g.nk - 1);
+    const dim3 gr = red_grid(g);
+    hipLaunchKernelGGL(k_sum_rule3, gr, dim3(256), 0, s, d.chiw, d.chi0inf, d.chi0inf2, d.wq, (int)g.nk8, g.Nw / 2,
+                       g.T, Nq, sgn, Ux, d.part);
+    hipLaunchKernelGGL(k_finalize_k, dim3(3), dim3(256), 0, s, d.part, (int)(gr.x * gr.y), out);
     return hipGetLastError();
 }
 
@@ -1363,54 +1571,56 @@ hipError_t launch_density(const Grid& g, const DevArrays& d, double mu, double n
 
 hipError_t launch_vertex_slab(const Grid& g, const DevArrays& d, double U, double Usp, double Uch, hipStream_t s)
 {
-    const size_t lds = slab_lds(g.N, false);
     const double normFact = g.T / ((double)g.N * g.N);
-    DISPATCH_N(g.N, {
-        set_lds(k_vertex_slab<NN>, lds);
-        hipLaunchKernelGGL(k_vertex_slab<NN>, dim3(g.nwb), dim3(SLAB_THREADS), lds, s, d.chi, d.V, normFact, U, Usp,
-                           Uch);
-    });
+    DISPATCH_S(g.prec, DISPATCH_N(g.N, {
+        const size_t lds = slab_lds(g.N, false, sizeof(SS));
+        set_lds(k_vertex_slab<NN, SS>, lds);
+        hipLaunchKernelGGL((k_vertex_slab<NN, SS>), dim3(g.nwb), dim3(SLAB_THREADS), lds, s, d.chiw, (SS*)d.V,
+                           normFact, U, Usp, Uch);
+    }));
     return hipGetLastError();
 }
 
 hipError_t launch_vertex_pencil(const Grid& g, const DevArrays& d, double U, double Usp, double Uch, hipStream_t s)
 {
     const double fact = U * (3.0 * Usp + Uch) / 8.0;
-    DISPATCH_L(g.Nw, {
-        hipLaunchKernelGGL(k_vertex_pencil<LL>, dim3((unsigned)((g.nk8 + 1) / 2)), dim3(PEN_THREADS), 0, s, d.V,
-                           d.chirt, g.nk8, fact, d.twN);
-    });
+    DISPATCH_S(g.prec, DISPATCH_L(g.Nw, {
+        hipLaunchKernelGGL((k_vertex_pencil<LL, SS>), dim3((unsigned)((g.nk8 + 1) / 2)), dim3(PEN_THREADS), 0, s,
+                           (SS*)d.V, (const SS*)d.chirt, d.chirt0, d.V0, g.nk8, fact, d.twN);
+    }));
     return hipGetLastError();
 }
 
 hipError_t launch_sigma_bnd(const Grid& g, const DevArrays& d, double mu0, double fact, hipStream_t s)
 {
-    const size_t lds = slab_lds(g.N, false);
+    const size_t lds = slab_lds(g.N, false, sizeof(double));
     DISPATCH_N(g.N, {
         set_lds(k_sigma_bnd<NN>, lds);
-        hipLaunchKernelGGL(k_sigma_bnd<NN>, dim3(1), dim3(SLAB_THREADS), lds, s, d.ek, d.er, d.V, d.dchirt0, d.dGrt0p,
-                           mu0, g.T, fact, d.dSig, d.tmp);
+        hipLaunchKernelGGL(k_sigma_bnd<NN>, dim3(2), dim3(SLAB_THREADS), lds, s, d.ek, d.er, d.V0, d.dchirt0, d.dGrt0p,
+                           mu0, g.T, fact, d.dSig, d.Sb, d.tmp);
     });
     return hipGetLastError();
 }
 
 hipError_t launch_sigma_slab(const Grid& g, const DevArrays& d, double mu0, hipStream_t s)
 {
-    const size_t lds = slab_lds(g.N, false);
-    DISPATCH_N(g.N, {
-        set_lds(k_sigma_slab<NN>, lds);
-        hipLaunchKernelGGL(k_sigma_slab<NN>, dim3(g.Nw + 1), dim3(SLAB_THREADS), lds, s, d.ek, d.V, mu0, g.T, g.Nw,
-                           d.Stau);
-    });
+    DISPATCH_S(g.prec, DISPATCH_N(g.N, {
+        const size_t lds = slab_lds(g.N, false, sizeof(SS));
+        set_lds(k_sigma_slab<NN, SS>, lds);
+        hipLaunchKernelGGL((k_sigma_slab<NN, SS>), dim3(g.Nw + 1), dim3(SLAB_THREADS), lds, s, d.ek, (const SS*)d.V,
+                           mu0, g.T, g.Nw, (SS*)d.Stau);
+    }));
     return hipGetLastError();
 }
 
 hipError_t launch_sigma_pencil(const Grid& g, const DevArrays& d, hipStream_t s)
 {
-    DISPATCH_L(g.Nw, {
-        hipLaunchKernelGGL(k_sigma_pencil<LL>, dim3((unsigned)g.nk8), dim3(PEN_THREADS), 0, s, d.Stau, d.dSig, g.nk8,
-                           g.T, d.Self, d.Sinf2, d.Sinf3, d.Sinf4, d.Sinf, d.twN, d.twX, d.twS);
-    });
+    hipLaunchKernelGGL(k_sigma_inf, dim3(1), dim3(64), 0, s, d.Sb, g.nk8, d.Sinf);
+    DISPATCH_S(g.prec, DISPATCH_L(g.Nw, {
+        hipLaunchKernelGGL((k_sigma_pencil<LL, SS, ST>), dim3((unsigned)g.nk8), dim3(PEN_THREADS), 0, s,
+                           (const SS*)d.Stau, d.Sb, d.dSig, g.nk8, g.T, (ST*)d.Self, d.Sinf2, d.Sinf3, d.Sinf4, d.Sinf,
+                           d.twN, d.twX, d.twS);
+    }));
     return hipGetLastError();
 }
 
@@ -1418,9 +1628,23 @@ hipError_t launch_trace_diffG(const Grid& g, const DevArrays& d, double pmu, dou
                               hipStream_t s)
 {
     const dim3 gr = red_grid(g);
-    hipLaunchKernelGGL(k_trace_diffG, gr, dim3(256), 0, s, d.ekw, d.wq, d.Self, d.Sinf2, d.Sinf3, d.Sinf4, d.Sinf,
-                       (int)g.nk8, g.Nw, g.T, mu0, pmu, d.part);
+    DISPATCH_S(g.prec, {
+        hipLaunchKernelGGL(k_trace_diffG<ST>, gr, dim3(256), 0, s, d.ekw, d.wq, (const ST*)d.Self, d.Sinf2, d.Sinf3,
+                           d.Sinf4, d.Sinf, (int)g.nk8, g.Nw, g.T, mu0, pmu, d.part);
+    });
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, d.part, (int)(gr.x * gr.y), out);
+    return hipGetLastError();
+}
+
+hipError_t launch_trace_diffG3(const Grid& g, const DevArrays& d, double pmu, double mu0, double* out,
+                               hipStream_t s)
+{
+    const dim3 gr = red_grid(g);
+    DISPATCH_S(g.prec, {
+        hipLaunchKernelGGL(k_trace_diffG3<ST>, gr, dim3(256), 0, s, d.ekw, d.wq, (const ST*)d.Self, d.Sinf2, d.Sinf3,
+                           d.Sinf4, d.Sinf, (int)g.nk8, g.Nw, g.T, mu0, pmu, d.part);
+    });
+    hipLaunchKernelGGL(k_finalize_k, dim3(3), dim3(256), 0, s, d.part, (int)(gr.x * gr.y), out);
     return hipGetLastError();
 }
 
@@ -1429,40 +1653,65 @@ hipError_t launch_trace_selfG(const Grid& g, const DevArrays& d, double mu0, dou
 {
     const double nk02 = 4.0 * (g.nk - 1) * (g.nk - 1);
     const dim3 gr = red_grid(g);
-    hipLaunchKernelGGL(k_trace_selfG, gr, dim3(256), 0, s, d.ekw, d.wq, d.Self, d.Sinf2, d.Sinf3, d.Sinf, (int)g.nk8,
-                       g.Nw, g.T, mu0, mu, nk02, which, d.part);
+    DISPATCH_S(g.prec, {
+        hipLaunchKernelGGL(k_trace_selfG<ST>, gr, dim3(256), 0, s, d.ekw, d.wq, (const ST*)d.Self, d.Sinf2, d.Sinf3,
+                           d.Sinf, (int)g.nk8, g.Nw, g.T, mu0, mu, nk02, which, d.part);
+    });
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, d.part, (int)(gr.x * gr.y), out);
     return hipGetLastError();
 }
 
-hipError_t launch_chijj(const Grid& g, const DevArrays& d, double mu0, double mu, double fact4, hipStream_t s,
-                        void (*mark)(void*, int), void* mark_ctx)
+hipError_t launch_hf0_slab(const Grid& g, const DevArrays& d, double mu0, hipStream_t s)
 {
-    const size_t lds = slab_lds(g.N, true);
-    if (mark) mark(mark_ctx, 0);
+    DISPATCH_S(g.prec, DISPATCH_N(g.N, {
+        const size_t lds = slab_lds(g.N, true, sizeof(SS));
+        set_lds(k_hf0_slab<NN, SS>, lds);
+        hipLaunchKernelGGL((k_hf0_slab<NN, SS>), dim3(g.Nw + 1), dim3(SLAB_THREADS), lds, s, d.ek, d.dek, mu0, g.T,
+                           g.Nw, (SS*)d.h, (SS*)d.f0);
+    }));
+    return hipGetLastError();
+}
+
+hipError_t launch_hf0_bnd(const Grid& g, const DevArrays& d, double mu0, double fact4, hipStream_t s)
+{
+    const size_t lds = slab_lds(g.N, true, sizeof(double));
     DISPATCH_N(g.N, {
-        set_lds(k_hf0_slab<NN>, lds);
         set_lds(k_hf0_bnd<NN>, lds);
-        set_lds(k_hVf0V_slab<NN>, lds);
-        hipLaunchKernelGGL(k_hf0_slab<NN>, dim3(g.Nw + 1), dim3(SLAB_THREADS), lds, s, d.ek, d.dek, mu0, g.T, g.Nw,
-                           d.h, d.f0);
-        if (mark) mark(mark_ctx, 1);
-        hipLaunchKernelGGL(k_hf0_bnd<NN>, dim3(4), dim3(SLAB_THREADS), lds, s, d.ek, d.dek, d.V, d.dchirt0, d.h, d.f0,
-                           mu0, g.T, g.Nw, fact4, d.dhV, d.df0V);
-        if (mark) mark(mark_ctx, 2);
-        hipLaunchKernelGGL(k_hVf0V_slab<NN>, dim3(g.Nw + 1), dim3(SLAB_THREADS), lds, s, d.V, g.Nw, d.h, d.f0);
+        hipLaunchKernelGGL(k_hf0_bnd<NN>, dim3(4), dim3(SLAB_THREADS), lds, s, d.ek, d.dek, d.V0, d.dchirt0, mu0, g.T,
+                           g.Nw, fact4, d.dhV, d.df0V, d.hVb, d.f0Vb, d.bscr);
     });
-    if (mark) mark(mark_ctx, 3);
+    return hipGetLastError();
+}
+
+hipError_t launch_hVf0V_slab(const Grid& g, const DevArrays& d, hipStream_t s)
+{
+    DISPATCH_S(g.prec, DISPATCH_N(g.N, {
+        const size_t lds = slab_lds(g.N, true, sizeof(SS));
+        set_lds(k_hVf0V_slab<NN, SS>, lds);
+        hipLaunchKernelGGL((k_hVf0V_slab<NN, SS>), dim3(g.Nw + 1), dim3(SLAB_THREADS), lds, s, (const SS*)d.V, g.Nw,
+                           (SS*)d.h, (SS*)d.f0);
+    }));
+    return hipGetLastError();
+}
+
+hipError_t launch_jj_pencil(const Grid& g, const DevArrays& d, double mu, hipStream_t s)
+{
     const unsigned nblk = (unsigned)((g.nq + kJJK - 1) / kJJK);
-    DISPATCH_LJJ(g.Nw, {
-        hipLaunchKernelGGL((k_jj_pencil<LL, JJ_THREADS>), dim3(nblk), dim3(JJ_THREADS), 0, s, d.ek, d.dek, d.d2ek,
-                           d.h, d.f0, d.dhV, d.df0V, d.Self, d.Sinf2, d.Sinf3, d.Sinf, g.N, g.T, mu, d.twN, d.twX,
-                           d.twS, d.twM, d.ph, d.jjpart, d.jjscal);
+    DISPATCH_S(g.prec, DISPATCH_LJJ(g.Nw, {
+        hipLaunchKernelGGL((k_jj_pencil<LL, JJ_THREADS, SS, ST>), dim3(nblk), dim3(JJ_THREADS), 0, s, d.ek, d.dek,
+                           d.d2ek, (const SS*)d.h, (const SS*)d.f0, d.dhV, d.df0V, d.hVb, d.f0Vb, (const ST*)d.Self,
+                           d.Sinf2, d.Sinf3, d.Sinf, g.N, g.T, mu, d.twN, d.twX, d.twS, d.twM, d.ph, d.jjpart,
+                           d.jjscal);
+    }));
+    return hipGetLastError();
+}
+
+hipError_t launch_kx0_rows(const Grid& g, const DevArrays& d, double mu, hipStream_t s)
+{
+    DISPATCH_S(g.prec, {
+        hipLaunchKernelGGL(k_kx0_rows<ST>, dim3(2 * g.nk), dim3(256), 0, s, d.ek, d.d2ek, (const ST*)d.Self, d.Sinf2,
+                           d.Sinf3, d.Sinf, g.N, g.Nw, g.T, mu, d.kx0part);
     });
-    if (mark) mark(mark_ctx, 4);
-    hipLaunchKernelGGL(k_kx0_rows, dim3(2 * g.nk), dim3(256), 0, s, d.ek, d.d2ek, d.Self, d.Sinf2, d.Sinf3, d.Sinf,
-                       g.N, g.Nw, g.T, mu, d.kx0part);
-    if (mark) mark(mark_ctx, 5);
     return hipGetLastError();
 }
 

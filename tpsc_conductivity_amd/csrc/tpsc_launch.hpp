@@ -1,4 +1,4 @@
-// tpsc_launch.hpp -- host-side launchers of the TPSC kernels (FP64 path).
+// tpsc_launch.hpp -- host-side launchers of the TPSC kernels.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -9,12 +9,16 @@ namespace tpsc {
 // Device-resident arrays of one parameter point, sized for the full grid.
 // Layouts follow the reference: wedge idx(l,m) = m + l(l+1)/2, tau/frequency-major
 // a[idx + j*nk8]; Lambda grid a[m + l*nk + j*nq] with kx index l+1.
+// The tau arrays chi, chirt, V, Stau, h, f0 hold FP32 values in the mixed path (the
+// pointer type is nominal there); Self holds FP32 complex.  Boundary arrays (chirt0, V0, Sb,
+// hVb, f0Vb) are FP64 in both paths.
 struct DevArrays {
     double* ek = nullptr;        // nk*nk quadrant
     double* ekw = nullptr;       // nk8  wedge-packed eps(k)
     double* wq = nullptr;        // nk8  wedge weight 8/2^(edges+diag)
     double* er = nullptr;        // nk8
-    double* chi = nullptr;       // nwb*nk8 : chi0(q,tau) then chi0(q,iqn)
+    double* chi = nullptr;       // nwb*nk8 : chi0(q,tau)
+    double* chiw = nullptr;      // nwb*nk8 : chi0(q,iqn), FP64 (== chi in the FP64 path)
     double* chirt = nullptr;     // nwb*nk8 : chi0(r,tau)
     double* dchirt0 = nullptr;   // nk8
     double* dGrt0p = nullptr;    // nk8
@@ -23,7 +27,10 @@ struct DevArrays {
     double* chi0inf2 = nullptr;  // nk8
     double* V = nullptr;         // nwb*nk8
     double* Stau = nullptr;      // (Nw+1)*nk8
-    double* dSig = nullptr;      // 2*nk8
+    double* dSig = nullptr;      // 2*nk8  dSigma/dtau at tau = 0, beta
+    double* Sb = nullptr;        // 2*nk8  Sigma(k, tau = 0), Sigma(k, tau = beta)
+    double* chirt0 = nullptr;    // nk8    chi0(r, tau = 0)
+    double* V0 = nullptr;        // nk8    V(r, tau = 0)
     cd* Self = nullptr;          // (Nw/2)*nk8
     double* Sinf2 = nullptr;     // nk8
     double* Sinf3 = nullptr;
@@ -35,6 +42,9 @@ struct DevArrays {
     double* f0 = nullptr;        // (Nw+1)*nq
     double* dhV = nullptr;       // 2*nq
     double* df0V = nullptr;      // 2*nq
+    double* hVb = nullptr;       // 2*nq   hV(k, tau = 0), hV(k, tau = beta)
+    double* f0Vb = nullptr;      // 2*nq   f0V(k, tau = 0), f0V(k, tau = beta)
+    double* bscr = nullptr;      // 4*nq   staging of the Lambda boundary kernel
     double* jjpart = nullptr;    // nblk * 2*nwb  per-workgroup Lambda partials
     double* jjscal = nullptr;    // nblk * 4      per-workgroup k0, inf sums
     double* kx0part = nullptr;   // 2*nk
@@ -54,6 +64,7 @@ struct Grid {
     int N, nk, Nw, nwb;
     long nk8, nq;
     double T;
+    int prec;  // 0: FP64; 1: mixed (FP32 slab arrays, FP32-complex Sigma(k,iwn))
 };
 
 constexpr int kRedBlocks = 1024;
@@ -68,6 +79,11 @@ hipError_t launch_chi0_slab(const Grid& g, const DevArrays& d, double mu0, hipSt
 hipError_t launch_chi0_pencil(const Grid& g, const DevArrays& d, hipStream_t s);
 hipError_t launch_chi0max(const Grid& g, const DevArrays& d, hipStream_t s);
 hipError_t launch_sum_rule(const Grid& g, const DevArrays& d, double sU, double* out, hipStream_t s);
+// trace, d/dUx and d2/dUx2 at sU = sgn*Ux into out[0..3)
+hipError_t launch_sum_rule3(const Grid& g, const DevArrays& d, double sgn, double Ux, double* out, hipStream_t s);
+// traceDiffG(pmu) and its first two pmu-derivatives into out[0..3)
+hipError_t launch_trace_diffG3(const Grid& g, const DevArrays& d, double pmu, double mu0, double* out,
+                               hipStream_t s);
 hipError_t launch_density(const Grid& g, const DevArrays& d, double mu, double n, double* out, hipStream_t s);
 hipError_t launch_model(const Grid& g, const DevArrays& d, double tp, double tpp, double t3, hipStream_t s);
 hipError_t launch_reduce_cols(const double* part, int nrows, int ncols, double* out, hipStream_t s);
@@ -80,8 +96,11 @@ hipError_t launch_trace_diffG(const Grid& g, const DevArrays& d, double pmu, dou
                               hipStream_t s);
 hipError_t launch_trace_selfG(const Grid& g, const DevArrays& d, double mu0, double mu, int which, double* out,
                               hipStream_t s);
-hipError_t launch_chijj(const Grid& g, const DevArrays& d, double mu0, double mu, double fact4, hipStream_t s,
-                        void (*mark)(void*, int), void* mark_ctx);
+hipError_t launch_hf0_slab(const Grid& g, const DevArrays& d, double mu0, hipStream_t s);
+hipError_t launch_hf0_bnd(const Grid& g, const DevArrays& d, double mu0, double fact4, hipStream_t s);
+hipError_t launch_hVf0V_slab(const Grid& g, const DevArrays& d, hipStream_t s);
+hipError_t launch_jj_pencil(const Grid& g, const DevArrays& d, double mu, hipStream_t s);
+hipError_t launch_kx0_rows(const Grid& g, const DevArrays& d, double mu, hipStream_t s);
 hipError_t launch_finalize(const double* part, int n, double* out, hipStream_t s);
 
 bool supported_N(int N);

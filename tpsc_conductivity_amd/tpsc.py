@@ -7,17 +7,23 @@ reference reports on stdout (a root not found, a stage called too early) raise T
 """
 import ctypes as C
 
-from .capi import Context, TPSCError, lib
+from .capi import PRECISIONS, Context, TPSCError, lib
 
 __all__ = ["CondOpt", "TPSCError"]
 
 
 class CondOpt:
-    """cond_opt(nk0, Nw0, n, T, eta, params={t', t'', t3, U}) -- cond_opt.h:25."""
+    """cond_opt(nk0, Nw0, n, T, eta, params={t', t'', t3, U}) -- cond_opt.h:25.
 
-    def __init__(self, nk0, Nw0, n, T, eta=0.1, params=(0.0, 0.0, 0.0, 1.0), coarse_pass=True):
+    precision: "fp64" (the reference's arithmetic) or "mixed" (FP32 slab transforms and
+    tau/iqn arrays, FP64 boundary data, moments, roots and Matsubara sums).
+    """
+
+    def __init__(self, nk0, Nw0, n, T, eta=0.1, params=(0.0, 0.0, 0.0, 1.0), coarse_pass=True, precision="fp64"):
         tp, tpp, t3, U = params
-        self.ctx = Context(nk0, Nw0, n, T, U, tp=tp, tpp=tpp, t3=t3, eta=eta, coarse_pass=coarse_pass)
+        self.precision = precision
+        self.ctx = Context(nk0, Nw0, n, T, U, tp=tp, tpp=tpp, t3=t3, eta=eta, coarse_pass=coarse_pass,
+                           precision=PRECISIONS[precision])
         self.nk0, self.Nw0, self.density, self.tem, self.eta = nk0, Nw0, n, T, eta
         self.tp, self.tpp, self.t3, self.U = tp, tpp, t3, U
 
