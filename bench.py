@@ -1,13 +1,18 @@
 #!/usr/bin/env python3
-"""Headline benchmark: TPSC solves/s at 256^2 x 2048 iwn on one MI355X (FP64 path).
+"""Headline benchmark: TPSC solves/s at 256^2 x 2048 iwn on one MI355X, mixed precision
+against the FP64 GPU path.
 
 One step = one full TPSC solve on the GPU: mu0 -> chi0(q,iqn) -> Usp/Uch sum-rule roots ->
 V(r,tau) -> Sigma(k,iwn) -> interacting mu -> Lambda_jj(iqn) bubble + Maki-Thompson + kx0
 (the coarse diagnostic pass of chi.cpp:285-305 is not part of a step).  Inputs are the
 model parameters; every array lives in HBM for the whole solve.
 
+`value` is the mixed-precision path (FP32 slab transforms and tau arrays; FP64 boundary
+data, tail moments, roots and Matsubara sums).  The FP64 path is timed in the same run, and
+every output of the mixed path is compared with it (max |mixed - fp64| / max |fp64|).
+
 Prints one JSON line.  Per-kernel times come from HIP events the library records on its
-own stream during one extra profiled solve after the timed region.
+own streams during one extra profiled solve after the timed region.
 """
 import argparse
 import json
@@ -20,6 +25,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "TPSC Σ/χ/Λjj solves/s at 256²×2048 iωn: mixed vs FP64 GPU at 1e-6; HBM GB/s"
 WORKLOAD = dict(N=256, Nw=2048, n=0.9, T=0.05, U=5.0, tp=-0.2, tpp=0.1, t3=0.0)
+TOL = 1e-6
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 
 
 def cpu_baseline(cfg):
@@ -38,6 +45,60 @@ def cpu_baseline(cfg):
                       f"OpenMP {threads} threads, {dt:.2f} s"}
 
 
+def make(cfg, precision):
+    from tpsc_conductivity_amd.tpsc import CondOpt
+
+    return CondOpt(cfg["N"], cfg["Nw"], cfg["n"], cfg["T"], params=(cfg["tp"], cfg["tpp"], cfg["t3"], cfg["U"]),
+                   coarse_pass=False, precision=precision)
+
+
+def timed(g, steps, warmup, tag):
+    for i in range(warmup):
+        g.solve()
+        print(f"[bench] {tag} warmup {i + 1}/{warmup} done", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.solve()  # returns after the stream is synchronised (scalars copied to the host)
+    ms = 1e3 * (time.perf_counter() - t0) / steps
+    print(f"[bench] {tag}: timed {steps} solves: {ms:.3f} ms/solve", file=sys.stderr, flush=True)
+    return ms
+
+
+def outputs(g):
+    import numpy as np
+
+    chi0 = g.get("chi0")
+    Usp, Uch = g.scalar("Usp"), g.scalar("Uch")
+    return {"chi0": chi0, "chisp": chi0 / (1 - Usp * chi0 / 2), "chich": chi0 / (1 + Uch * chi0 / 2),
+            "Sigma_kw": g.get("Sigma_kw"), "Lambda_jj": g.get("chijj1") + g.get("chijj2"),
+            "Usp": np.array([Usp]), "Uch": np.array([Uch]), "mu": np.array([g.scalar("mu")]),
+            "kx0": np.array([g.scalar("kx0")])}
+
+
+def rel_errors(a, b):
+    import numpy as np
+
+    return {k: float(np.max(np.abs(a[k] - b[k])) / np.max(np.abs(b[k]))) for k in b}
+
+
+def profile(g, N, Nw, precision):
+    from tpsc_conductivity_amd import roofline
+
+    g.profile(True)
+    g.profile_reset()
+    g.solve()
+    table = g.profile_table()
+    g.profile(False)
+    kb = roofline.kernel_bytes(N, Nw, precision)
+    per = {}
+    for name, (tot_ms, calls) in table.items():
+        e = {"avg_ms": tot_ms / calls, "calls": calls}
+        if name in kb:
+            e["GBps"] = kb[name] / (e["avg_ms"] * 1e-3) / 1e9
+        per[name] = e
+    return table, per, kb
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -46,52 +107,56 @@ def main():
     ap.add_argument("--N", type=int, default=WORKLOAD["N"])
     ap.add_argument("--Nw", type=int, default=WORKLOAD["Nw"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--kernel-table", action="store_true", help="print per-kernel times to stderr")
+    ap.add_argument("--precision", choices=("mixed", "fp64"), default="mixed",
+                    help="path timed for `value` (default: mixed)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run an elementwise copy of a 1 GiB device buffer 5 times (reference kernel "
+                         "for the FETCH_SIZE / WRITE_SIZE scale of the traffic figure) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(a parameter point does not span GPUs here)", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        import torch
+
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": src.numel() * 8, "copy_bytes_written": dst.numel() * 8,
+                          "launches": 5}), flush=True)
+        return
 
     from tpsc_conductivity_amd import roofline
-    from tpsc_conductivity_amd.tpsc import CondOpt
 
     cfg = dict(WORKLOAD, N=args.N, Nw=args.Nw)
-    g = CondOpt(cfg["N"], cfg["Nw"], cfg["n"], cfg["T"], params=(cfg["tp"], cfg["tpp"], cfg["t3"], cfg["U"]),
-                coarse_pass=False)
-    for i in range(args.warmup):
-        g.solve()
-        print(f"[bench] warmup {i + 1}/{args.warmup} done", file=sys.stderr, flush=True)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        g.solve()  # returns after the stream is synchronised (scalars copied to the host)
-    dt = time.perf_counter() - t0
-    ms = 1e3 * dt / args.steps
-    print(f"[bench] timed {args.steps} solves: {ms:.3f} ms/solve", file=sys.stderr, flush=True)
-
-    # one profiled solve: HIP events around every kernel family on the library stream
-    g.profile(True)
-    g.profile_reset()
-    g.solve()
-    table = g.profile_table()
-    g.profile(False)
-    kb = roofline.kernel_bytes(cfg["N"], cfg["Nw"])
-    per_kernel = {}
-    for name, (tot_ms, calls) in table.items():
-        if name in kb:
-            avg = tot_ms / calls
-            per_kernel[name] = {"avg_ms": avg, "calls": calls, "GBps": kb[name] / (avg * 1e-3) / 1e9}
-        else:
-            per_kernel[name] = {"avg_ms": tot_ms / calls, "calls": calls}
-    if args.kernel_table:
-        for name, v in sorted(table.items(), key=lambda x: -x[1][0]):
-            extra = f"  {per_kernel[name].get('GBps', 0):8.1f} GB/s" if "GBps" in per_kernel[name] else ""
-            print(f"[bench] {name:16s} {v[0]:9.3f} ms  x{v[1]}{extra}", file=sys.stderr)
-    dominant = max((k for k in per_kernel if k in kb), key=lambda k: table[k][0])
-    d = per_kernel[dominant]
+    prec = args.precision
+    g = make(cfg, prec)
+    ms = timed(g, args.steps, args.warmup, prec)
+    table, per, kb = profile(g, cfg["N"], cfg["Nw"], prec)
     n_sr = int(g.scalar("nevals_sp") + g.scalar("nevals_ch"))
     n_mu = int(g.scalar("nevals_mu"))
-    solve_b = roofline.solve_bytes(cfg["N"], cfg["Nw"], n_sr, n_mu)
+    out_main = outputs(g)
+    g.close()
+    if args.kernel_table:
+        for name, v in sorted(table.items(), key=lambda x: -x[1][0]):
+            extra = f"  {per[name]['GBps']:8.1f} GB/s" if "GBps" in per[name] else ""
+            print(f"[bench] {prec} {name:16s} {v[0]:9.3f} ms  x{v[1]}{extra}", file=sys.stderr)
+    dominant = max((k for k in per if k in kb), key=lambda k: table[k][0])
+    d = per[dominant]
+    slab_ms = sum(table[k][0] for k in roofline.SLAB_KERNELS if k in table)
+    slab_b = sum(kb[k] * table[k][1] for k in roofline.SLAB_KERNELS if k in table)
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            tr = json.load(f)
+        key = f"{prec}:{dominant}:{cfg['N']}x{cfg['Nw']}"
+        if key in tr:
+            traffic = tr[key]["bytes_per_launch"]
     out = {
         "metric": METRIC,
         "value": 1e3 / ms,
@@ -103,17 +168,31 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64" if prec == "mixed" else "f64",
         "data": "synthetic",
         "config": {"workload": f"TPSC solve N={cfg['N']} Nw={cfg['Nw']} n={cfg['n']} T={cfg['T']} U={cfg['U']} "
-                               f"t'={cfg['tp']} t''={cfg['tpp']}, FP64, mu0..Lambda_jj bubble+MT, no AL",
-                   "N": cfg["N"], "Nw": cfg["Nw"], "precision": "fp64"},
+                               f"t'={cfg['tp']} t''={cfg['tpp']}, {prec} precision, mu0..Lambda_jj bubble+MT, "
+                               "no AL",
+                   "N": cfg["N"], "Nw": cfg["Nw"], "precision": prec},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": d["GBps"], "peak": roofline.HBM_PEAK_GBPS,
-                     "unit": "GB/s", "frac": d["GBps"] / roofline.HBM_PEAK_GBPS, "traffic": None,
+                     "unit": "GB/s", "frac": d["GBps"] / roofline.HBM_PEAK_GBPS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": kb[dominant], "avg_launch_ms": d["avg_ms"]},
-        "solve_hbm_GBps": solve_b / (ms * 1e-3) / 1e9,
-        "kernels_ms": {k: round(v["avg_ms"] * v["calls"], 4) for k, v in per_kernel.items()},
+        "slab_passes": {"kernels": list(roofline.SLAB_KERNELS), "ms": slab_ms,
+                        "GBps": slab_b / (slab_ms * 1e-3) / 1e9 if slab_ms > 0 else None,
+                        "frac": slab_b / (slab_ms * 1e-3) / 1e9 / roofline.HBM_PEAK_GBPS if slab_ms > 0 else None},
+        "solve_hbm_GBps": roofline.solve_bytes(cfg["N"], cfg["Nw"], n_sr, n_mu, prec) / (ms * 1e-3) / 1e9,
+        "kernels_ms": {k: round(v["avg_ms"] * v["calls"], 4) for k, v in per.items()},
     }
+    if prec == "mixed" and not args.no_fp64:
+        g = make(cfg, "fp64")
+        ms64 = timed(g, args.steps, args.warmup, "fp64")
+        out_64 = outputs(g)
+        g.close()
+        errs = rel_errors(out_main, out_64)
+        out["fp64"] = {"ms_per_step": ms64, "solves_per_s": 1e3 / ms64, "speedup_mixed_over_fp64": ms64 / ms}
+        out["mixed_rel_err_vs_fp64"] = {k: float(f"{v:.3e}") for k, v in errs.items()}
+        out["mixed_within_tol"] = bool(all(v <= TOL for v in errs.values()))
+        out["tol"] = TOL
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline (oracle) ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(cfg)

@@ -289,6 +289,22 @@ __device__ __forceinline__ void block_fft(cd* buf, int sign, const cd* __restric
     const int tid = threadIdx.x;
     int Ns = 1;
     for (int pass = 0; pass < P4; pass++) {
+        // twiddles first: they do not depend on the data, so their global-memory latency
+        // overlaps the LDS reads and the barrier (pass 0 has k = 0: all twiddles are 1)
+        cd w[PER4][3];
+#pragma unroll
+        for (int i = 0; i < PER4; i++) {
+            const int b = tid + i * NT;
+            const int k = b % Ns;
+            const int stride = L / (Ns * 4);
+#pragma unroll
+            for (int r = 1; r < 4; r++) {
+                cd t = cd{1.0, 0.0};
+                if (pass > 0 && b < NB4) t = twg[(r * k * stride) & (L - 1)];
+                if (sign > 0) t.y = -t.y;
+                w[i][r - 1] = t;
+            }
+        }
         cd v[PER4][4];
 #pragma unroll
         for (int i = 0; i < PER4; i++) {
@@ -304,12 +320,9 @@ __device__ __forceinline__ void block_fft(cd* buf, int sign, const cd* __restric
             const int b = tid + i * NT;
             if (b < NB4) {
                 const int k = b % Ns;
-                const int stride = L / (Ns * 4);
+                if (pass > 0) {
 #pragma unroll
-                for (int r = 1; r < 4; r++) {
-                    cd w = twg[(r * k * stride) & (L - 1)];
-                    if (sign > 0) w.y = -w.y;
-                    v[i][r] = v[i][r] * w;
+                    for (int r = 1; r < 4; r++) v[i][r] = v[i][r] * w[i][r - 1];
                 }
                 const cd a0 = v[i][0], a1 = v[i][1], a2 = v[i][2], a3 = v[i][3];
                 const cd s02 = a0 + a2, d02 = a0 - a2, s13 = a1 + a3, d13 = a1 - a3;
@@ -327,6 +340,15 @@ __device__ __forceinline__ void block_fft(cd* buf, int sign, const cd* __restric
     if (LOG2 & 1) {
         constexpr int NB2 = L / 2;
         constexpr int PER2 = (NB2 + NT - 1) / NT;
+        cd w[PER2];
+#pragma unroll
+        for (int i = 0; i < PER2; i++) {
+            const int b = tid + i * NT;
+            cd t = cd{1.0, 0.0};
+            if (b < NB2) t = twg[((b % Ns) * (L / (Ns * 2))) & (L - 1)];
+            if (sign > 0) t.y = -t.y;
+            w[i] = t;
+        }
         cd v[PER2][2];
 #pragma unroll
         for (int i = 0; i < PER2; i++) {
@@ -342,9 +364,7 @@ __device__ __forceinline__ void block_fft(cd* buf, int sign, const cd* __restric
             const int b = tid + i * NT;
             if (b < NB2) {
                 const int k = b % Ns;
-                cd w = twg[(k * (L / (Ns * 2))) & (L - 1)];
-                if (sign > 0) w.y = -w.y;
-                const cd a = v[i][0], c = v[i][1] * w;
+                const cd a = v[i][0], c = v[i][1] * w[i];
                 const int dst = (b / Ns) * Ns * 2 + k;
                 buf[dst] = a + c;
                 buf[dst + Ns] = a - c;
@@ -487,6 +507,146 @@ __device__ __forceinline__ void block_spline_moments(const double* y, double* M,
         }
         M[j] += add;
     }
+    __syncthreads();
+}
+
+// powers z^e, e = 0..2 KW, of the spline pole z = sqrt(3) - 2, by repeated multiplication
+struct ZPow {
+    double v[2 * KW + 1];
+};
+constexpr ZPow make_zpow()
+{
+    ZPow t{};
+    const double z = 1.7320508075688772935 - 2.0;
+    t.v[0] = 1.0;
+    for (int e = 1; e <= 2 * KW; e++) t.v[e] = t.v[e - 1] * z;
+    return t;
+}
+__constant__ constexpr ZPow kZPow = make_zpow();
+
+// The same clamped spline as block_spline_moments, in place: yM holds y_0..y_N on entry and
+// the moments M_0..M_N on exit.  Each thread keeps its chunk of the right-hand side in
+// registers, so no scratch array is needed.  C: largest chunk, >= ceil((N+1)/NT).
+// trap (optional): each thread receives its part of sum_j h (y_j + y_{j+1})/2
+// - h^3 (M_j + M_{j+1})/24, the integral of the spline over [0, N h]; the caller adds the
+// parts (block_sum).  ES: LDS scratch of 2*NT + 8 doubles.
+template <int C, int NT>
+__device__ __forceinline__ void block_spline_inplace(double* yM, int N, double h, double d0, double dN, double* ES,
+                                                     double* trap)
+{
+    const int tid = threadIdx.x;
+    const double z = 1.7320508075688772935 - 2.0;
+    const double A = 0.28867513459481288225;
+    const double ih = 1.0 / h;
+    const int Cr = (N + 1 + NT - 1) / NT;
+    const int nchunk = (N + 1 + Cr - 1) / Cr;
+    const int j0 = tid * Cr, j1 = min(N + 1, j0 + Cr);
+    double* bnd = ES + 2 * NT;  // r_0, r_N, M_0, M_1, M_{N-1}, M_N
+    double ri[C], u[C], w[C];
+    double eu = 0.0, ew = 0.0, tr = 0.0;
+#pragma unroll
+    for (int c = 0; c < C; c++) {
+        const int j = j0 + c;
+        double r = 0.0;
+        if (j < j1) {
+            if (j == 0) {
+                r = 6.0 * ih * ((yM[1] - yM[0]) * ih - d0);
+                bnd[0] = r;
+                r = 0.0;
+            } else if (j == N) {
+                r = 6.0 * ih * (dN - (yM[N] - yM[N - 1]) * ih);
+                bnd[1] = r;
+                r = 0.0;
+            } else
+                r = 6.0 * ih * ih * (yM[j + 1] - 2.0 * yM[j] + yM[j - 1]);
+            if (trap && j < N) tr += h * (yM[j] + yM[j + 1]) * 0.5;
+        }
+        ri[c] = r;
+    }
+    if (j0 < j1) {
+        double sa = 0.0;
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+            sa = ri[c] + z * sa;
+            u[c] = sa;
+            if (c == Cr - 1) eu = sa;
+        }
+        sa = 0.0;
+#pragma unroll
+        for (int c = C - 1; c >= 0; c--) {
+            if (j0 + c < j1) sa = ri[c] + z * sa;
+            w[c] = sa;
+        }
+        ew = sa;
+    }
+    ES[tid] = eu;
+    ES[NT + tid] = ew;
+    __syncthreads();  // all reads of y are done
+    if (j0 < j1) {
+        double zc = 1.0;
+        for (int c = 0; c < Cr; c++) zc *= z;
+        double Su = 0.0, Sw = 0.0, zd = 1.0;
+        for (int d = 1; d <= nchunk && fabs(zd) > 1.0e-19; d++) {
+            if (tid - d >= 0) Su += zd * ES[tid - d];
+            if (tid + d < nchunk) Sw += zd * ES[NT + tid + d];
+            zd *= zc;
+        }
+        double zp = z;
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+            u[c] += zp * Su;
+            zp *= z;
+        }
+        zp = z;
+#pragma unroll
+        for (int c = C - 1; c >= 0; c--) {
+            if (j0 + c < j1) {
+                w[c] += zp * Sw;
+                zp *= z;
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+            const int j = j0 + c;
+            if (j < j1) {
+                const double m = A * (u[c] + w[c] - ri[c]);
+                yM[j] = m;
+                u[c] = m;
+            }
+        }
+    }
+    __syncthreads();
+    const double R0 = bnd[0] - 2.0 * yM[0] - yM[1];
+    const double RN = bnd[1] - yM[N - 1] - 2.0 * yM[N];
+    double zN1 = 1.0;  // z^(N-1)
+    {
+        double zz = z;
+        int e = N - 1;
+        while (e > 0) {
+            if (e & 1) zN1 *= zz;
+            zz *= zz;
+            e >>= 1;
+        }
+    }
+    const double dg = 2.0 + z, off = zN1 * (1.0 + 2.0 * z);
+    const double det = dg * dg - off * off;
+    const double alpha = (dg * R0 - off * RN) / det;
+    const double beta = (dg * RN - off * R0) / det;
+    __syncthreads();
+    double tm = 0.0;
+#pragma unroll
+    for (int c = 0; c < C; c++) {
+        const int j = j0 + c;
+        if (j < j1) {
+            double add = 0.0;
+            if (j <= 2 * KW) add += alpha * kZPow.v[j];
+            if (N - j <= 2 * KW) add += beta * kZPow.v[N - j];
+            const double m = u[c] + add;
+            if (add != 0.0) yM[j] = m;
+            if (trap) tm += (j == 0 || j == N) ? m : 2.0 * m;
+        }
+    }
+    if (trap) *trap = tr - h * h * h * tm / 24.0;
     __syncthreads();
 }
 

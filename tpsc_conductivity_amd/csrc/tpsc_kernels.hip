@@ -1098,63 +1098,55 @@ __device__ __forceinline__ cd dyson_G(const SelfT* __restrict__ Self, long ki, l
     return T / den;
 }
 
-// J(ikn) at slot s (n = s - L/2) from the spline transform F of hV or f0V
-// (cond_opt.cpp:3302-3311): -i cj1/kn - cj2/kn^2 + i cj3/kn^3 + twS F/kn^4
-__device__ __forceinline__ cd j_slot(const cd* X, const cd* __restrict__ twS, int s, int L, double T, double cj1,
-                                     double cj2, double cj3)
-{
-    const int H = L / 2;
-    const int j = (s >= H) ? s - H : s + H;
-    const double kn = (j < H) ? (2 * j + 1) * kPi * T : (2 * (j - L) + 1) * kPi * T;
-    const cd a = twS[j] * X[j];
-    const double k2 = kn * kn, k3 = k2 * kn, k4 = k2 * k2;
-    return cd{-cj2 / k2 + a.x / k4, -cj1 / kn + cj3 / k3 + a.y / k4};
-}
+// Per-k Lambda_jj contributions (cond_opt.cpp:3179-3822).  One workgroup of NT threads per
+// Lambda-grid point at a time; workgroup blk handles the points q = blk + kk*nblk,
+// kk < kJJK, and accumulates into its own partial rows part[blk*2*nwb + n] (bubble) and
+// part[blk*2*nwb + nwb + n] (Maki-Thompson).
+//
+// The six length-L transforms of the reference run as three complex FFTs of pairs of
+// sequences whose transforms have a known symmetry:
+//   (1) the f0V and hV spline moments 6a_j (real) with the phase exp(i pi j/L): for real a,
+//       A(n) = sum_j a_j e^{2 pi i j (n + 1/2)/L} obeys A(L-1-n) = conj A(n), so the FFT F of
+//       (a + i b) e^{i pi j/L} splits into A(n) = (F_n + conj F_{L-1-n})/2 and
+//       B(n) = (F_n - conj F_{L-1-n})/(2i);
+//   (2) G(ikn) - G2 and conj(J(ikn)) (both Hermitian in n -> -n-1): ph_j * FFT_-(.)_j is real
+//       for each, so ph_j * FFT_-(Gn + i conj Jn)_j = G(tau_j) + i J(tau_j) before the two-pole
+//       terms (FFT_+(Jn)_j / ph_j = conj(ph_j FFT_-(conj Jn)_j));
+//   (3) the Maki-Thompson and bubble moments (real): A_n = (F_n + conj F_{L-n})/2,
+//       B_n = (F_n - conj F_{L-n})/(2i).
+// Imaginary parts of G(tau), J(tau) are zero in exact arithmetic and are not formed.
+// LDS: the spline samples / moments (Y1, Y2, in place), which also hold the packed slot
+// values W of step (2), and one transform buffer X.  Uniform per-point scalars (pole data)
+// are computed once and kept in LDS.
+struct JJPoint {
+    Poles P;
+    double k0r, dGp, dGm, dJ0, dJb, Jbeta;
+};
 
-// spline of a tau-pencil held in y (NSint intervals, step h) and the DFT input into X:
-// mode 0: X_j = exp(i pi j/L) 6a_j (j < L); mode 1: X_j = 6a_j (j < L);
-// mode 2: X_j = 6a_j for j < L/2, X_{L-1-j} = -X_j
-template <int L, int NT>
-__device__ __forceinline__ void spline_to_X(double* y, double* M, cd* X, int NSint, double h, double d0, double dN,
-                                            int mode, const cd* __restrict__ twX, double* ES)
-{
-    double* rr = (double*)X;
-    block_spline_moments<L + 1, NT>(y, M, rr, NSint, h, d0, dN, ES);
-    const int tid = threadIdx.x;
-    if (mode == 2) {
-        for (int j = tid; j < L / 2; j += NT) {
-            const double a6 = (M[j + 1] - M[j]) / h;
-            X[j] = cd{a6, 0.0};
-            X[L - 1 - j] = cd{-a6, 0.0};
-        }
-    } else {
-        for (int j = tid; j < L; j += NT) {
-            const double a6 = (M[j + 1] - M[j]) / h;
-            X[j] = (mode == 0) ? twX[j] * a6 : cd{a6, 0.0};
-        }
-    }
-    __syncthreads();
-}
-
-// per-k Lambda_jj contributions (cond_opt.cpp:3179-3822).  One workgroup of NT threads per
-// Lambda-grid point at a time; each workgroup owns kJJK consecutive points and accumulates
-// into its own partial rows part[blk*2*nwb + n] (bubble) and part[blk*2*nwb + nwb + n] (MT).
+#ifndef JJ_WAVES_PER_SIMD
+#define JJ_WAVES_PER_SIMD 4  // two 512-thread workgroups per CU (74 KB LDS each)
+#endif
 template <int L, int NT, typename S, typename SelfT>
-__global__ void __launch_bounds__(NT)
+__global__ void __launch_bounds__(NT, JJ_WAVES_PER_SIMD)
     k_jj_pencil(const double* __restrict__ ek, const double* __restrict__ dek, const double* __restrict__ d2ek,
                 const S* __restrict__ h, const S* __restrict__ f0, const double* __restrict__ dhV,
                 const double* __restrict__ df0V, const double* __restrict__ hVb, const double* __restrict__ f0Vb,
-                const SelfT* __restrict__ Self, const double* __restrict__ Sinf2,
-                const double* __restrict__ Sinf3, const double* __restrict__ Sinfp, int N, double T, double mu,
-                const cd* __restrict__ tw, const cd* __restrict__ twX, const cd* __restrict__ twS,
-                const cd* __restrict__ twM, const cd* __restrict__ phT, double* part, double* scal)
+                const SelfT* __restrict__ Self, const double* __restrict__ Sinf2, const double* __restrict__ Sinf3,
+                const double* __restrict__ Sinfp, int N, double T, double mu, const cd* __restrict__ tw,
+                const cd* __restrict__ twX, const cd* __restrict__ twS, const cd* __restrict__ twM,
+                const cd* __restrict__ phT, double* part, double* scal)
 {
     constexpr int H = L / 2;
     constexpr int NWB = H + 1;
-    __shared__ double y[L + 1], M[L + 1];
-    __shared__ cd X[L], Gb[L], Jb[L];
+    constexpr int CS = (L + 1 + NT - 1) / NT;  // spline chunk
+    __shared__ __align__(16) double Yb[2 * (L + 1)];
+    __shared__ cd X[L];
     __shared__ double sc[8 * (NT / 64)];
-    __shared__ double ES[2 * NT];
+    __shared__ double ES[2 * NT + 8];
+    __shared__ JJPoint pt;
+    double* Y1 = Yb;
+    double* Y2 = Yb + (L + 1);
+    cd* W = (cd*)Yb;  // L complex values, used while Y1, Y2 are free
     const int nk = N / 2 + 1;
     const long nq = (long)nk * (nk - 2), nk8 = (long)nk * (nk + 1) / 2;
     const double nf = 1.0 / ((double)N * N);
@@ -1162,6 +1154,7 @@ __global__ void __launch_bounds__(NT)
     const double hstep = 1.0 / (L * T);
     const int tid = threadIdx.x;
     const int blk = xcd_remap(blockIdx.x, gridDim.x);
+    const long nblk = gridDim.x;
     double* c1 = part + (long)blk * 2 * NWB;
     double* c2 = c1 + NWB;
     for (int n = tid; n < NWB; n += NT) {
@@ -1169,108 +1162,118 @@ __global__ void __launch_bounds__(NT)
         c2[n] = 0.0;
     }
     double k0acc = 0.0, inf1 = 0.0, inf2 = 0.0, inf4 = 0.0;
+    // point kk of workgroup blk is q = blk + kk * nblk: the workgroups resident on one XCD
+    // (consecutive blk) work on consecutive q at the same time and share the cache lines of
+    // the tau-major hV / f0V rows and of Sigma
     for (int kk = 0; kk < kJJK; kk++) {
-        const long q = (long)blk * kJJK + kk;
+        const long q = (long)blk + kk * nblk;
         if (q >= nq) break;
         const int l = (int)(q / nk), m = (int)(q % nk);
         const long ki = fold_idx(l + 1, m, N);
-        const double C2 = Sinf2[ki], C3 = Sinf3[ki];
         const double e = ek[m + (l + 1) * nk];
         const double v = dek[q];
-        double wt = 4;
-        if (m == 0 || m == nk - 1) wt = wt / 2;
-        // (1) Dyson G for all slots, and the f0V pencil
-        for (int s = tid; s < L; s += NT) {
-            double kn;
-            Gb[s] = dyson_G(Self, ki, nk8, s, H, C2, C3, Sinf, e, mu, T, kn);
+        const double wt = (m == 0 || m == nk - 1) ? 2.0 : 4.0;
+        // (1) f0V and hV pencils, their splines, one packed FFT
+        for (int j = tid; j <= L; j += NT) {
+            Y1[j] = (j == 0) ? f0Vb[q] : (j == L) ? f0Vb[nq + q] : (double)f0[q + (long)j * nq];
+            Y2[j] = (j == 0) ? hVb[q] : (j == L) ? hVb[nq + q] : (double)h[q + (long)j * nq];
         }
-        for (int j = tid; j <= L; j += NT)
-            y[j] = (j == 0) ? f0Vb[q] : (j == L) ? f0Vb[nq + q] : (double)f0[q + (long)j * nq];
-        const double fd0 = df0V[q], fdN = df0V[q + nq];
+        const double CJ1 = -hVb[q] - hVb[nq + q];
         __syncthreads();
-        const double cj01 = -y[0] - y[L], cj02 = fd0 + fdN;
-        spline_to_X<L, NT>(y, M, X, L, hstep, fd0, fdN, 0, twX, ES);
-        const double cj03 = -(M[0] + M[L]);
-        block_fft<L, NT>(X, +1, tw);
-        // (2) chijj2(iqn=0) = sum G^2 J0 / T
-        double r2[2] = {0.0, 0.0};
-        for (int s = tid; s < L; s += NT) {
-            const cd J0 = j_slot(X, twS, s, L, T, cj01, cj02, cj03);
-            const cd t = Gb[s] * Gb[s] * J0 / T;
-            r2[0] += t.x;
-            r2[1] += t.y;
-        }
-        block_sum_k<2>(r2, sc);
-        const double qn0 = r2[0];
-        // (3) hV pencil -> J, products and asymptote subtraction
-        for (int j = tid; j <= L; j += NT)
-            y[j] = (j == 0) ? hVb[q] : (j == L) ? hVb[nq + q] : (double)h[q + (long)j * nq];
-        const double hd0 = dhV[q], hdN = dhV[q + nq];
-        __syncthreads();
-        const double CJ1 = -y[0] - y[L], cj2 = hd0 + hdN;
-        spline_to_X<L, NT>(y, M, X, L, hstep, hd0, hdN, 0, twX, ES);
-        const double cj3 = -(M[0] + M[L]);
-        block_fft<L, NT>(X, +1, tw);
-        double r4[4] = {0.0, 0.0, 0.0, 0.0};  // Re k0, Im k0, dGp, dJ0
-        for (int s = tid; s < L; s += NT) {
-            const cd Jv = j_slot(X, twS, s, L, T, CJ1, cj2, cj3);
-            const double kn = (2.0 * (s - H) + 1) * kPi * T;
-            const cd Gv = Gb[s];
-            const cd G2 = T / cd{-e + mu, kn + Sinf / kn};
-            const cd Jinf = cd{0.0, -CJ1 / kn};
-            const cd Jn = Gv * Jv - G2 * Jinf;
-            const cd Gn = Gv - G2;
-            Jb[s] = Jn;
-            Gb[s] = Gn;
-            r4[0] += Gn.x;
-            r4[1] += Gn.y;
-            r4[2] += kn * Gn.y;
-            r4[3] -= kn * Jn.y;
-        }
-        block_sum_k<4>(r4, sc);
-        double dGp = r4[2], dJ0 = r4[3];
-        // (4) two-pole asymptote
-        const Poles P = make_poles(e, mu, Sinf, CJ1, T);
-        cd f1p, f1m, fj1, fj1b, f2p, f2m, fj2, fj2b;
-        if (P.r1.x > 0) {
-            const cd ex = cexpd(-(P.r1 / T));
-            f1p = P.a1 / P.den1;
-            f1m = P.a1 * ex / P.den1;
-            fj1 = P.r1 * P.b1 * ex / P.den1;
-            fj1b = P.r1 * P.b1 / P.den1;
-        } else {
-            const cd ex = cexpd(P.r1 / T);
-            f1p = P.a1 * ex / P.den1;
-            f1m = P.a1 / P.den1;
-            fj1 = P.r1 * P.b1 / P.den1;
-            fj1b = P.r1 * P.b1 * ex / P.den1;
-        }
-        if (P.r2.x > 0) {
-            const cd ex = cexpd(-(P.r2 / T));
-            f2p = P.a2 / P.den2;
-            f2m = P.a2 * ex / P.den2;
-            fj2 = P.r2 * P.b2 * ex / P.den2;
-            fj2b = P.r2 * P.b2 / P.den2;
-        } else {
-            const cd ex = cexpd(P.r2 / T);
-            f2p = P.a2 * ex / P.den2;
-            f2m = P.a2 / P.den2;
-            fj2 = P.r2 * P.b2 / P.den2;
-            fj2b = P.r2 * P.b2 * ex / P.den2;
-        }
-        const double k0r = r4[0] + (f1m + f2m).x;
-        double dGm = -dGp;
-        dGp += (P.r1 * f1p + P.r2 * f2p).x;
-        dGm += (P.r1 * f1m + P.r2 * f2m).x;
-        double dJb = -dJ0;
-        dJ0 += (fj1 + fj2).x;
-        dJb += (fj1b + fj2b).x;
-        // (5) to imaginary time
-        block_fft<L, NT>(Gb, -1, tw);
-        block_fft<L, NT>(Jb, +1, tw);
-        const cd Gbeta0 = Gb[0], Jbeta0 = Jb[0];
-        __syncthreads();
+        block_spline_inplace<CS, NT>(Y1, L, hstep, df0V[q], df0V[q + nq], ES, nullptr);
+        block_spline_inplace<CS, NT>(Y2, L, hstep, dhV[q], dhV[q + nq], ES, nullptr);
         for (int j = tid; j < L; j += NT) {
+            const double a = (Y1[j + 1] - Y1[j]) / hstep, b = (Y2[j + 1] - Y2[j]) / hstep;
+            X[j] = twX[j] * cd{a, b};
+        }
+        const double cj03 = -(Y1[0] + Y1[L]), cj3 = -(Y2[0] + Y2[L]);
+        __syncthreads();
+        block_fft<L, NT>(X, +1, tw);
+        // (2) per Matsubara slot s (n = s - L/2): Dyson G, J0 (f0V), J (hV) -> W
+        double r6[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // qn0 (re, im), sum Gn (re, im), dGp, dJ0
+        {
+            const double C2 = Sinf2[ki], C3 = Sinf3[ki];
+            const double cj01 = -f0Vb[q] - f0Vb[nq + q], cj02 = df0V[q] + df0V[q + nq];
+            const double cj2 = dhV[q] + dhV[q + nq];
+#pragma unroll 1
+            for (int s = tid; s < L; s += NT) {
+                const int jx = (s >= H) ? s - H : s + H;  // transform index of slot s
+                const cd Fj = X[jx], Fc = conjg(X[L - 1 - jx]);
+                const cd Af = cd{0.5 * (Fj.x + Fc.x), 0.5 * (Fj.y + Fc.y)};
+                const cd Bh = cd{0.5 * (Fj.y - Fc.y), -0.5 * (Fj.x - Fc.x)};
+                const double knj = (jx < H) ? (2 * jx + 1) * kPi * T : (2 * (jx - L) + 1) * kPi * T;
+                const double k2 = knj * knj, k3 = k2 * knj, k4 = k2 * k2;
+                const cd ts = twS[jx];
+                const cd af = ts * Af, ah = ts * Bh;
+                const cd J0 = cd{-cj02 / k2 + af.x / k4, -cj01 / knj + cj03 / k3 + af.y / k4};
+                const cd Jv = cd{-cj2 / k2 + ah.x / k4, -CJ1 / knj + cj3 / k3 + ah.y / k4};
+                double kn;
+                const cd Gv = dyson_G(Self, ki, nk8, s, H, C2, C3, Sinf, e, mu, T, kn);
+                const cd t = Gv * Gv * J0 / T;
+                r6[0] += t.x;
+                r6[1] += t.y;
+                const cd G2 = T / cd{-e + mu, kn + Sinf / kn};
+                const cd Jn = Gv * Jv - G2 * cd{0.0, -CJ1 / kn};
+                const cd Gn = Gv - G2;
+                r6[2] += Gn.x;
+                r6[3] += Gn.y;
+                r6[4] += kn * Gn.y;
+                r6[5] -= kn * Jn.y;
+                W[s] = cd{Gn.x + Jn.y, Gn.y + Jn.x};  // Gn + i conj(Jn)
+            }
+        }
+        block_sum_k<6>(r6, sc);
+        const double qn0 = r6[0];
+        // (3) two-pole asymptote data (cond_opt.cpp:3409-3467), once per point
+        if (tid == 0) {
+            const Poles P = make_poles(e, mu, Sinf, CJ1, T);
+            cd f1p, f1m, fj1, fj1b, f2p, f2m, fj2, fj2b;
+            if (P.r1.x > 0) {
+                const cd ex = cexpd(-(P.r1 / T));
+                f1p = P.a1 / P.den1;
+                f1m = P.a1 * ex / P.den1;
+                fj1 = P.r1 * P.b1 * ex / P.den1;
+                fj1b = P.r1 * P.b1 / P.den1;
+            } else {
+                const cd ex = cexpd(P.r1 / T);
+                f1p = P.a1 * ex / P.den1;
+                f1m = P.a1 / P.den1;
+                fj1 = P.r1 * P.b1 / P.den1;
+                fj1b = P.r1 * P.b1 * ex / P.den1;
+            }
+            if (P.r2.x > 0) {
+                const cd ex = cexpd(-(P.r2 / T));
+                f2p = P.a2 / P.den2;
+                f2m = P.a2 * ex / P.den2;
+                fj2 = P.r2 * P.b2 * ex / P.den2;
+                fj2b = P.r2 * P.b2 / P.den2;
+            } else {
+                const cd ex = cexpd(P.r2 / T);
+                f2p = P.a2 * ex / P.den2;
+                f2m = P.a2 / P.den2;
+                fj2 = P.r2 * P.b2 / P.den2;
+                fj2b = P.r2 * P.b2 * ex / P.den2;
+            }
+            pt.P = P;
+            pt.k0r = r6[2] + (f1m + f2m).x;
+            pt.dGp = r6[4] + (P.r1 * f1p + P.r2 * f2p).x;
+            pt.dGm = -r6[4] + (P.r1 * f1m + P.r2 * f2m).x;
+            pt.dJ0 = r6[5] + (fj1 + fj2).x;
+            pt.dJb = -r6[5] + (fj1b + fj2b).x;
+        }
+        // (4) to imaginary time: X_j <- (G(tau_j), J(tau_j))
+        block_fft<L, NT>(W, -1, tw);
+        if (tid == 0) {
+            const Poles& P = pt.P;
+            const double t = 1.0 / T;
+            const cd ej1 = (P.r1.x > 0) ? cd{1.0, 0.0} : cexpd(P.r1 * t);
+            const cd ej2 = (P.r2.x > 0) ? cd{1.0, 0.0} : cexpd(P.r2 * t);
+            const double Jinf = (P.b1 * ej1 / P.den1 + P.b2 * ej2 / P.den2).x;
+            pt.Jbeta = (cd{W[0].y, 0.0} / phT[L]).x + Jinf;  // W[0].y = FFT_+(Jn)_0 (ph_0 = 1)
+        }
+#pragma unroll 1
+        for (int j = tid; j < L; j += NT) {
+            const Poles& P = pt.P;
             const double t = j / (L * T);
             cd e1, ej1, e2, ej2;
             if (P.r1.x > 0) {
@@ -1289,73 +1292,61 @@ __global__ void __launch_bounds__(NT)
             }
             const double G0t = (-(P.a1 * e1 / P.den1) - P.a2 * e2 / P.den2).x;
             const double Jinf = (P.b1 * ej1 / P.den1 + P.b2 * ej2 / P.den2).x;
-            const cd ph = phT[j];
-            Gb[j] = ph * Gb[j] + cd{G0t, 0.0};
-            Jb[j] = Jb[j] / ph + cd{Jinf, 0.0};
-        }
-        cd Jbeta;
-        {
-            const double t = 1.0 / T;
-            cd ej1, ej2;
-            ej1 = (P.r1.x > 0) ? cd{1.0, 0.0} : cexpd(P.r1 * t);
-            ej2 = (P.r2.x > 0) ? cd{1.0, 0.0} : cexpd(P.r2 * t);
-            const double Jinf = (P.b1 * ej1 / P.den1 + P.b2 * ej2 / P.den2).x;
-            Jbeta = Jbeta0 / phT[L] + cd{Jinf, 0.0};
-        }
-        (void)Gbeta0;
-        __syncthreads();
-        const cd G0 = Gb[0], J0t = Jb[0];
-        // (6) G(tau) J(-tau) -> Maki-Thompson
-        for (int j = tid; j < L; j += NT) y[j] = (Gb[j] * Jb[j]).x;
-        if (tid == 0) y[L] = ((-G0 - cd{1.0, 0.0}) * Jbeta).x;
-        __syncthreads();
-        const double gd0 = (dGp * J0t + G0 * dJ0).x;
-        const double gdN = (dGm * Jbeta - (G0 + cd{1.0, 0.0}) * dJb).x;
-        const double MJ1 = y[0] - y[L];
-        spline_to_X<L, NT>(y, M, X, L, hstep, gd0, gdN, 1, twX, ES);
-        const double MJ3 = M[0] - M[L];
-        block_fft<L, NT>(X, -1, tw);
-        for (int n = 1 + tid; n <= H; n += NT) {
-            const double wn = 2 * n * kPi * T;
-            const cd a = twM[n] * X[n];
-            const double w4 = wn * wn * wn * wn;
-            const double imJ = -2.0 * MJ1 / wn + 2.0 * MJ3 / (wn * wn * wn) + 2.0 * (a.y / w4);
-            c2[n] -= nf * wt * v * imJ / wn;
-        }
-        if (tid == 0) c2[0] -= nf * wt * v * qn0;
-        __syncthreads();
-        // (7) -G(tau) G(-tau) -> bubble
-        const double dchi = dGp * (G0.x + 1.0) + G0.x * dGm;
-        for (int j = tid; j <= H; j += NT) {
-            cd gg;
-            if (j == 0)
-                gg = G0 * (G0 + cd{1.0, 0.0});
-            else
-                gg = -(Gb[j] * Gb[L - j]);
-            y[j] = gg.x;
+            const cd R = phT[j] * W[j];
+            X[j] = cd{R.x + G0t, R.y + Jinf};
         }
         __syncthreads();
-        spline_to_X<L, NT>(y, M, X, H, hstep, dchi, 0.0, 2, twX, ES);
+        const double G0 = X[0].x, J0t = X[0].y, Jbeta = pt.Jbeta;
+        // (5) Maki-Thompson samples G(tau) J(-tau) -> Y1; bubble samples -G(tau) G(-tau) -> Y2
+        for (int j = tid; j < L; j += NT) {
+            const cd gj = X[j];
+            Y1[j] = gj.x * gj.y;
+            if (j <= H) Y2[j] = (j == 0) ? G0 * (G0 + 1.0) : -(gj.x * X[L - j].x);
+        }
+        if (tid == 0) Y1[L] = (-G0 - 1.0) * Jbeta;
+        __syncthreads();
+        const double dGp = pt.dGp, dGm = pt.dGm;
+        const double gd0 = dGp * J0t + G0 * pt.dJ0;
+        const double gdN = dGm * Jbeta - (G0 + 1.0) * pt.dJb;
+        const double MJ1 = Y1[0] - Y1[L];
+        const double dchi = dGp * (G0 + 1.0) + G0 * dGm;
+        block_spline_inplace<CS, NT>(Y1, L, hstep, gd0, gdN, ES, nullptr);
         double integ[1] = {0.0};
-        for (int j = tid; j < H; j += NT)
-            integ[0] += hstep * (y[j] + y[j + 1]) * 0.5 - hstep * hstep * hstep * (M[j] + M[j + 1]) / 24.0;
+        block_spline_inplace<CS, NT>(Y2, H, hstep, dchi, 0.0, ES, integ);
         block_sum_k<1>(integ, sc);
-        block_fft<L, NT>(X, -1, tw);
-        for (int n = tid; n <= H; n += NT) {
-            double bub;
-            if (n == 0)
-                bub = 2 * integ[0];
-            else {
-                const double wn = 2 * n * kPi * T;
-                const cd a = twM[n] * X[n];
-                bub = -2.0 * dchi / (wn * wn) + a.x / (wn * wn * wn * wn);
-            }
-            c1[n] -= 2.0 * nf * wt * v * v * bub;
+        const double MJ3 = Y1[0] - Y1[L];
+        // (6) packed transform of the two moment sequences
+        for (int j = tid; j < L; j += NT) {
+            const double a = (Y1[j + 1] - Y1[j]) / hstep;
+            const double b = (j < H) ? (Y2[j + 1] - Y2[j]) / hstep : -(Y2[L - j] - Y2[L - 1 - j]) / hstep;
+            X[j] = cd{a, b};
         }
-        k0acc -= 2.0 * wt * nf * d2ek[m + (l + 1) * nk] * k0r;
-        inf1 += 4.0 * dchi * nf * wt * v * v;
-        inf2 += nf * wt * v * 2.0 * MJ1;
-        inf4 -= nf * wt * v * 2.0 * MJ3;
+        __syncthreads();
+        block_fft<L, NT>(X, -1, tw);
+        const double wv = nf * wt * v;
+#pragma unroll 1
+        for (int n = tid; n <= H; n += NT) {
+            const cd Fn = X[n], Fc = conjg(X[(L - n) & (L - 1)]);
+            const double wn = 2 * n * kPi * T;
+            double bub;
+            if (n == 0) {
+                bub = 2 * integ[0];
+                c2[0] -= wv * qn0;
+            } else {
+                const double w4 = wn * wn * wn * wn;
+                const cd tm = twM[n];
+                const cd a = tm * cd{0.5 * (Fn.x + Fc.x), 0.5 * (Fn.y + Fc.y)};
+                const double imJ = -2.0 * MJ1 / wn + 2.0 * MJ3 / (wn * wn * wn) + 2.0 * (a.y / w4);
+                c2[n] -= wv * imJ / wn;
+                const cd b = tm * cd{0.5 * (Fn.y - Fc.y), -0.5 * (Fn.x - Fc.x)};
+                bub = -2.0 * dchi / (wn * wn) + b.x / w4;
+            }
+            c1[n] -= 2.0 * wv * v * bub;
+        }
+        k0acc -= 2.0 * wt * nf * d2ek[m + (l + 1) * nk] * pt.k0r;
+        inf1 += 4.0 * dchi * wv * v;
+        inf2 += wv * 2.0 * MJ1;
+        inf4 -= wv * 2.0 * MJ3;
         __syncthreads();
     }
     if (tid == 0) {
