@@ -26,6 +26,16 @@
 namespace tpsc {
 
 #define SLAB_THREADS 1024
+// waves per SIMD a slab kernel is compiled for: two FP32 workgroups (68 KB LDS each at N=256)
+// fit one CU when the kernel stays within 64 VGPRs; an FP64 slab (133 KB) runs alone
+template <typename S>
+struct SlabOcc {
+    static constexpr int waves = 4;
+};
+template <>
+struct SlabOcc<float> {
+    static constexpr int waves = 8;
+};
 #define PEN_THREADS 256
 #define JJ_THREADS 512
 
@@ -204,7 +214,7 @@ __global__ void __launch_bounds__(SLAB_THREADS)
 
 // slices j = 1 .. nwb-1 (chi0.cpp:454-559); block b handles j = b + 1
 template <int N, typename S>
-__global__ void __launch_bounds__(SLAB_THREADS)
+__global__ void __launch_bounds__(SLAB_THREADS, SlabOcc<S>::waves)
     k_chi0_slab(const double* __restrict__ ekw, double mu0, double T, int Nw, S* chi, S* chirt)
 {
     constexpr int nk = N / 2 + 1;
@@ -452,7 +462,7 @@ __global__ void k_density(const double* __restrict__ ekw, const double* __restri
 // ================================================================== vertex V(r,tau)
 // chi.cpp:85-119, block j = iqn index
 template <int N, typename S>
-__global__ void __launch_bounds__(SLAB_THREADS)
+__global__ void __launch_bounds__(SLAB_THREADS, SlabOcc<S>::waves)
     k_vertex_slab(const double* __restrict__ chi, S* V, double normFact, double U, double Usp, double Uch)
 {
     constexpr int nk = N / 2 + 1;
@@ -604,7 +614,7 @@ __global__ void __launch_bounds__(SLAB_THREADS)
 
 // Sigma(k,tau_j), j = 0..Nw (green.cpp:1822-1892): G0(k,tau) -> r, x V(r,|tau|), -> k
 template <int N, typename S>
-__global__ void __launch_bounds__(SLAB_THREADS)
+__global__ void __launch_bounds__(SLAB_THREADS, SlabOcc<S>::waves)
     k_sigma_slab(const double* __restrict__ ek, const S* __restrict__ V, double mu0, double T, int Nw, S* Stau)
 {
     constexpr int nk = N / 2 + 1;
@@ -892,7 +902,7 @@ __device__ __forceinline__ double hf0_val(int which, double e, double v, double 
 // h(r,tau_j) and f0(r,tau_j), j = 0..Nw (cond_opt.cpp:2697-2735, 2795-2827):
 // RODFT00 along kx (index l+1), REDFT00 along ky.
 template <int N, typename S>
-__global__ void __launch_bounds__(SLAB_THREADS)
+__global__ void __launch_bounds__(SLAB_THREADS, SlabOcc<S>::waves)
     k_hf0_slab(const double* __restrict__ ek, const double* __restrict__ dek, double mu0, double T, int Nw,
                S* h, S* f0)
 {
@@ -1006,7 +1016,7 @@ __global__ void __launch_bounds__(SLAB_THREADS)
 
 // hV(k,tau) = DST x DCT [2 h(r,tau) V(r,|tau|)], same for f0, in place (cond_opt.cpp:3029-3114)
 template <int N, typename S>
-__global__ void __launch_bounds__(SLAB_THREADS)
+__global__ void __launch_bounds__(SLAB_THREADS, SlabOcc<S>::waves)
     k_hVf0V_slab(const S* __restrict__ V, int Nw, S* h, S* f0)
 {
     constexpr int nk = N / 2 + 1;
