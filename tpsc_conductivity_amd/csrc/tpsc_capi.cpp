@@ -704,11 +704,17 @@ int stage_chijj(tpsc_ctx* c)
     prof_begin(c, "jj_kx0_rows", c->side);
     HIPCHK(c, launch_kx0_rows(g, c->d, c->mu, c->side));
     prof_end(c);
+    prof_begin(c, "jj_poles", c->side);
+    HIPCHK(c, launch_jj_poles(g, c->d, c->mu, c->side));
+    prof_end(c);
     prof_begin(c, "jj_hf0_slab");
     HIPCHK(c, launch_hf0_slab(g, c->d, c->mu0, s));
     prof_end(c);
     prof_begin(c, "jj_hVf0V_slab");
     HIPCHK(c, launch_hVf0V_slab(g, c->d, s));
+    prof_end(c);
+    prof_begin(c, "jj_transpose");
+    HIPCHK(c, launch_jj_transpose(g, c->d, s));
     prof_end(c);
     if ((rc = join_side(c))) return rc;
     prof_begin(c, "jj_pencil");
@@ -870,6 +876,9 @@ int tpsc_create(const tpsc_params* p, tpsc_ctx** out)
     rc |= dalloc(c, &d.hVb, 2 * std::max<long>(nq, 1));
     rc |= dalloc(c, &d.f0Vb, 2 * std::max<long>(nq, 1));
     rc |= dalloc(c, &d.bscr, 4 * std::max<long>(nq, 1));
+    rc |= dalloc(c, &d.hT, (size_t)(Nw + 1) * std::max<long>(nq, 1), es);
+    rc |= dalloc(c, &d.f0T, (size_t)(Nw + 1) * std::max<long>(nq, 1), es);
+    rc |= dalloc(c, &d.poles, 48 * (size_t)std::max<long>(nq, 1));
     const long nblk = (nq + kJJK - 1) / kJJK;
     rc |= dalloc(c, &d.jjpart, (size_t)std::max<long>(nblk, 1) * 2 * nwb);
     rc |= dalloc(c, &d.jjscal, (size_t)std::max<long>(nblk, 1) * 4);

@@ -1108,6 +1108,81 @@ __device__ __forceinline__ cd dyson_G(const SelfT* __restrict__ Self, long ki, l
     return T / den;
 }
 
+// Two-pole asymptote data of every Lambda-grid point (cond_opt.cpp:3409-3467): the roots of
+// z^2 + (mu - e) z - Sigma_inf, the residues, and the tau = 0+ / beta- pole sums used for the
+// boundary slopes.  One thread per point.
+struct JJPoles {
+    Poles P;
+    cd f1p, f1m, fj1, fj1b, f2p, f2m, fj2, fj2b;
+};
+static_assert(sizeof(JJPoles) <= 48 * sizeof(double), "DevArrays::poles holds 48 doubles per point");
+
+__global__ void k_jj_poles(const double* __restrict__ ek, const double* __restrict__ hVb,
+                           const double* __restrict__ Sinfp, int N, double T, double mu, JJPoles* out)
+{
+    const int nk = N / 2 + 1;
+    const long nq = (long)nk * (nk - 2);
+    const long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    const int l = (int)(q / nk), m = (int)(q % nk);
+    const double e = ek[m + (l + 1) * nk];
+    const double CJ1 = -hVb[q] - hVb[nq + q];
+    JJPoles o;
+    const Poles P = make_poles(e, mu, Sinfp[0], CJ1, T);
+    o.P = P;
+    if (P.r1.x > 0) {
+        const cd ex = cexpd(-(P.r1 / T));
+        o.f1p = P.a1 / P.den1;
+        o.f1m = P.a1 * ex / P.den1;
+        o.fj1 = P.r1 * P.b1 * ex / P.den1;
+        o.fj1b = P.r1 * P.b1 / P.den1;
+    } else {
+        const cd ex = cexpd(P.r1 / T);
+        o.f1p = P.a1 * ex / P.den1;
+        o.f1m = P.a1 / P.den1;
+        o.fj1 = P.r1 * P.b1 / P.den1;
+        o.fj1b = P.r1 * P.b1 * ex / P.den1;
+    }
+    if (P.r2.x > 0) {
+        const cd ex = cexpd(-(P.r2 / T));
+        o.f2p = P.a2 / P.den2;
+        o.f2m = P.a2 * ex / P.den2;
+        o.fj2 = P.r2 * P.b2 * ex / P.den2;
+        o.fj2b = P.r2 * P.b2 / P.den2;
+    } else {
+        const cd ex = cexpd(P.r2 / T);
+        o.f2p = P.a2 * ex / P.den2;
+        o.f2m = P.a2 / P.den2;
+        o.fj2 = P.r2 * P.b2 / P.den2;
+        o.fj2b = P.r2 * P.b2 * ex / P.den2;
+    }
+    out[q] = o;
+}
+
+// [j][q] (j < nj, q < nq) -> [q][j]: 32 x 32 tiles through LDS, both sides coalesced.  The
+// Lambda pencils then read each tau-pencil as one contiguous row.
+template <typename S>
+__global__ void __launch_bounds__(256) k_transpose(const S* __restrict__ in, S* __restrict__ out, long nq, int nj)
+{
+    __shared__ S t[32][33];
+    const long q0 = (long)blockIdx.x * 32;
+    const int j0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+#pragma unroll
+    for (int r = 0; r < 32; r += 8) {
+        const int j = j0 + ty + r;
+        const long q = q0 + tx;
+        if (j < nj && q < nq) t[ty + r][tx] = in[(long)j * nq + q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 32; r += 8) {
+        const long q = q0 + ty + r;
+        const int j = j0 + tx;
+        if (j < nj && q < nq) out[q * nj + j] = t[tx][ty + r];
+    }
+}
+
 // Per-k Lambda_jj contributions (cond_opt.cpp:3179-3822).  One workgroup of NT threads per
 // Lambda-grid point at a time; workgroup blk handles the points q = blk + kk*nblk,
 // kk < kJJK, and accumulates into its own partial rows part[blk*2*nwb + n] (bubble) and
@@ -1126,12 +1201,8 @@ __device__ __forceinline__ cd dyson_G(const SelfT* __restrict__ Self, long ki, l
 //       B_n = (F_n - conj F_{L-n})/(2i).
 // Imaginary parts of G(tau), J(tau) are zero in exact arithmetic and are not formed.
 // LDS: the spline samples / moments (Y1, Y2, in place), which also hold the packed slot
-// values W of step (2), and one transform buffer X.  Uniform per-point scalars (pole data)
-// are computed once and kept in LDS.
-struct JJPoint {
-    Poles P;
-    double k0r, dGp, dGm, dJ0, dJb, Jbeta;
-};
+// values W of step (2), and one transform buffer X.  hT, f0T are the hV, f0V pencils in
+// point-major layout [q][j] (k_transpose); the pole data come from k_jj_poles.
 
 #ifndef JJ_WAVES_PER_SIMD
 #define JJ_WAVES_PER_SIMD 4  // two 512-thread workgroups per CU (74 KB LDS each)
@@ -1139,10 +1210,11 @@ struct JJPoint {
 template <int L, int NT, typename S, typename SelfT>
 __global__ void __launch_bounds__(NT, JJ_WAVES_PER_SIMD)
     k_jj_pencil(const double* __restrict__ ek, const double* __restrict__ dek, const double* __restrict__ d2ek,
-                const S* __restrict__ h, const S* __restrict__ f0, const double* __restrict__ dhV,
+                const S* __restrict__ hT, const S* __restrict__ f0T, const double* __restrict__ dhV,
                 const double* __restrict__ df0V, const double* __restrict__ hVb, const double* __restrict__ f0Vb,
-                const SelfT* __restrict__ Self, const double* __restrict__ Sinf2, const double* __restrict__ Sinf3,
-                const double* __restrict__ Sinfp, int N, double T, double mu, const cd* __restrict__ tw,
+                const JJPoles* __restrict__ poles, const SelfT* __restrict__ Self, const double* __restrict__ Sinf2,
+                const double* __restrict__ Sinf3, const double* __restrict__ Sinfp, int N, double T, double mu,
+                const cd* __restrict__ tw,
                 const cd* __restrict__ twX, const cd* __restrict__ twS, const cd* __restrict__ twM,
                 const cd* __restrict__ phT, double* part, double* scal)
 {
@@ -1153,7 +1225,7 @@ __global__ void __launch_bounds__(NT, JJ_WAVES_PER_SIMD)
     __shared__ cd X[L];
     __shared__ double sc[8 * (NT / 64)];
     __shared__ double ES[2 * NT + 8];
-    __shared__ JJPoint pt;
+    __shared__ double Jbeta_s;
     double* Y1 = Yb;
     double* Y2 = Yb + (L + 1);
     cd* W = (cd*)Yb;  // L complex values, used while Y1, Y2 are free
@@ -1184,9 +1256,13 @@ __global__ void __launch_bounds__(NT, JJ_WAVES_PER_SIMD)
         const double v = dek[q];
         const double wt = (m == 0 || m == nk - 1) ? 2.0 : 4.0;
         // (1) f0V and hV pencils, their splines, one packed FFT
-        for (int j = tid; j <= L; j += NT) {
-            Y1[j] = (j == 0) ? f0Vb[q] : (j == L) ? f0Vb[nq + q] : (double)f0[q + (long)j * nq];
-            Y2[j] = (j == 0) ? hVb[q] : (j == L) ? hVb[nq + q] : (double)h[q + (long)j * nq];
+        {
+            const S* f0q = f0T + q * (L + 1);
+            const S* hq = hT + q * (L + 1);
+            for (int j = tid; j <= L; j += NT) {
+                Y1[j] = (j == 0) ? f0Vb[q] : (j == L) ? f0Vb[nq + q] : (double)f0q[j];
+                Y2[j] = (j == 0) ? hVb[q] : (j == L) ? hVb[nq + q] : (double)hq[j];
+            }
         }
         const double CJ1 = -hVb[q] - hVb[nq + q];
         __syncthreads();
@@ -1234,56 +1310,25 @@ __global__ void __launch_bounds__(NT, JJ_WAVES_PER_SIMD)
         }
         block_sum_k<6>(r6, sc);
         const double qn0 = r6[0];
-        // (3) two-pole asymptote data (cond_opt.cpp:3409-3467), once per point
-        if (tid == 0) {
-            const Poles P = make_poles(e, mu, Sinf, CJ1, T);
-            cd f1p, f1m, fj1, fj1b, f2p, f2m, fj2, fj2b;
-            if (P.r1.x > 0) {
-                const cd ex = cexpd(-(P.r1 / T));
-                f1p = P.a1 / P.den1;
-                f1m = P.a1 * ex / P.den1;
-                fj1 = P.r1 * P.b1 * ex / P.den1;
-                fj1b = P.r1 * P.b1 / P.den1;
-            } else {
-                const cd ex = cexpd(P.r1 / T);
-                f1p = P.a1 * ex / P.den1;
-                f1m = P.a1 / P.den1;
-                fj1 = P.r1 * P.b1 / P.den1;
-                fj1b = P.r1 * P.b1 * ex / P.den1;
-            }
-            if (P.r2.x > 0) {
-                const cd ex = cexpd(-(P.r2 / T));
-                f2p = P.a2 / P.den2;
-                f2m = P.a2 * ex / P.den2;
-                fj2 = P.r2 * P.b2 * ex / P.den2;
-                fj2b = P.r2 * P.b2 / P.den2;
-            } else {
-                const cd ex = cexpd(P.r2 / T);
-                f2p = P.a2 * ex / P.den2;
-                f2m = P.a2 / P.den2;
-                fj2 = P.r2 * P.b2 / P.den2;
-                fj2b = P.r2 * P.b2 * ex / P.den2;
-            }
-            pt.P = P;
-            pt.k0r = r6[2] + (f1m + f2m).x;
-            pt.dGp = r6[4] + (P.r1 * f1p + P.r2 * f2p).x;
-            pt.dGm = -r6[4] + (P.r1 * f1m + P.r2 * f2m).x;
-            pt.dJ0 = r6[5] + (fj1 + fj2).x;
-            pt.dJb = -r6[5] + (fj1b + fj2b).x;
-        }
+        // (3) two-pole asymptote: pole data from k_jj_poles, sums of this point
+        const JJPoles& pp = poles[q];
+        const Poles& P = pp.P;
+        const double k0r = r6[2] + (pp.f1m + pp.f2m).x;
+        const double dGp = r6[4] + (P.r1 * pp.f1p + P.r2 * pp.f2p).x;
+        const double dGm = -r6[4] + (P.r1 * pp.f1m + P.r2 * pp.f2m).x;
+        const double dJ0 = r6[5] + (pp.fj1 + pp.fj2).x;
+        const double dJb = -r6[5] + (pp.fj1b + pp.fj2b).x;
         // (4) to imaginary time: X_j <- (G(tau_j), J(tau_j))
         block_fft<L, NT>(W, -1, tw);
         if (tid == 0) {
-            const Poles& P = pt.P;
             const double t = 1.0 / T;
             const cd ej1 = (P.r1.x > 0) ? cd{1.0, 0.0} : cexpd(P.r1 * t);
             const cd ej2 = (P.r2.x > 0) ? cd{1.0, 0.0} : cexpd(P.r2 * t);
             const double Jinf = (P.b1 * ej1 / P.den1 + P.b2 * ej2 / P.den2).x;
-            pt.Jbeta = (cd{W[0].y, 0.0} / phT[L]).x + Jinf;  // W[0].y = FFT_+(Jn)_0 (ph_0 = 1)
+            Jbeta_s = (cd{W[0].y, 0.0} / phT[L]).x + Jinf;  // W[0].y = FFT_+(Jn)_0 (ph_0 = 1)
         }
 #pragma unroll 1
         for (int j = tid; j < L; j += NT) {
-            const Poles& P = pt.P;
             const double t = j / (L * T);
             cd e1, ej1, e2, ej2;
             if (P.r1.x > 0) {
@@ -1306,7 +1351,7 @@ __global__ void __launch_bounds__(NT, JJ_WAVES_PER_SIMD)
             X[j] = cd{R.x + G0t, R.y + Jinf};
         }
         __syncthreads();
-        const double G0 = X[0].x, J0t = X[0].y, Jbeta = pt.Jbeta;
+        const double G0 = X[0].x, J0t = X[0].y, Jbeta = Jbeta_s;
         // (5) Maki-Thompson samples G(tau) J(-tau) -> Y1; bubble samples -G(tau) G(-tau) -> Y2
         for (int j = tid; j < L; j += NT) {
             const cd gj = X[j];
@@ -1315,9 +1360,8 @@ __global__ void __launch_bounds__(NT, JJ_WAVES_PER_SIMD)
         }
         if (tid == 0) Y1[L] = (-G0 - 1.0) * Jbeta;
         __syncthreads();
-        const double dGp = pt.dGp, dGm = pt.dGm;
-        const double gd0 = dGp * J0t + G0 * pt.dJ0;
-        const double gdN = dGm * Jbeta - (G0 + 1.0) * pt.dJb;
+        const double gd0 = dGp * J0t + G0 * dJ0;
+        const double gdN = dGm * Jbeta - (G0 + 1.0) * dJb;
         const double MJ1 = Y1[0] - Y1[L];
         const double dchi = dGp * (G0 + 1.0) + G0 * dGm;
         block_spline_inplace<CS, NT>(Y1, L, hstep, gd0, gdN, ES, nullptr);
@@ -1353,7 +1397,7 @@ __global__ void __launch_bounds__(NT, JJ_WAVES_PER_SIMD)
             }
             c1[n] -= 2.0 * wv * v * bub;
         }
-        k0acc -= 2.0 * wt * nf * d2ek[m + (l + 1) * nk] * pt.k0r;
+        k0acc -= 2.0 * wt * nf * d2ek[m + (l + 1) * nk] * k0r;
         inf1 += 4.0 * dchi * wv * v;
         inf2 += wv * 2.0 * MJ1;
         inf4 -= wv * 2.0 * MJ3;
@@ -1695,14 +1739,31 @@ hipError_t launch_hVf0V_slab(const Grid& g, const DevArrays& d, hipStream_t s)
     return hipGetLastError();
 }
 
+hipError_t launch_jj_poles(const Grid& g, const DevArrays& d, double mu, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_jj_poles, dim3((unsigned)((g.nq + 127) / 128)), dim3(128), 0, s, d.ek, d.hVb, d.Sinf, g.N,
+                       g.T, mu, (JJPoles*)d.poles);
+    return hipGetLastError();
+}
+
+hipError_t launch_jj_transpose(const Grid& g, const DevArrays& d, hipStream_t s)
+{
+    const dim3 gr((unsigned)((g.nq + 31) / 32), (unsigned)((g.Nw + 1 + 31) / 32));
+    DISPATCH_S(g.prec, {
+        hipLaunchKernelGGL(k_transpose<SS>, gr, dim3(256), 0, s, (const SS*)d.h, (SS*)d.hT, g.nq, g.Nw + 1);
+        hipLaunchKernelGGL(k_transpose<SS>, gr, dim3(256), 0, s, (const SS*)d.f0, (SS*)d.f0T, g.nq, g.Nw + 1);
+    });
+    return hipGetLastError();
+}
+
 hipError_t launch_jj_pencil(const Grid& g, const DevArrays& d, double mu, hipStream_t s)
 {
     const unsigned nblk = (unsigned)((g.nq + kJJK - 1) / kJJK);
     DISPATCH_S(g.prec, DISPATCH_LJJ(g.Nw, {
         hipLaunchKernelGGL((k_jj_pencil<LL, JJ_THREADS, SS, ST>), dim3(nblk), dim3(JJ_THREADS), 0, s, d.ek, d.dek,
-                           d.d2ek, (const SS*)d.h, (const SS*)d.f0, d.dhV, d.df0V, d.hVb, d.f0Vb, (const ST*)d.Self,
-                           d.Sinf2, d.Sinf3, d.Sinf, g.N, g.T, mu, d.twN, d.twX, d.twS, d.twM, d.ph, d.jjpart,
-                           d.jjscal);
+                           d.d2ek, (const SS*)d.hT, (const SS*)d.f0T, d.dhV, d.df0V, d.hVb, d.f0Vb,
+                           (const JJPoles*)d.poles, (const ST*)d.Self, d.Sinf2, d.Sinf3, d.Sinf, g.N, g.T, mu, d.twN,
+                           d.twX, d.twS, d.twM, d.ph, d.jjpart, d.jjscal);
     }));
     return hipGetLastError();
 }

@@ -45,6 +45,9 @@ struct DevArrays {
     double* hVb = nullptr;       // 2*nq   hV(k, tau = 0), hV(k, tau = beta)
     double* f0Vb = nullptr;      // 2*nq   f0V(k, tau = 0), f0V(k, tau = beta)
     double* bscr = nullptr;      // 4*nq   staging of the Lambda boundary kernel
+    double* hT = nullptr;        // (Nw+1)*nq  hV in point-major layout [q][j]
+    double* f0T = nullptr;       // (Nw+1)*nq  f0V in point-major layout
+    double* poles = nullptr;     // nq * 48    two-pole data per Lambda point (k_jj_poles)
     double* jjpart = nullptr;    // nblk * 2*nwb  per-workgroup Lambda partials
     double* jjscal = nullptr;    // nblk * 4      per-workgroup k0, inf sums
     double* kx0part = nullptr;   // 2*nk
@@ -99,6 +102,8 @@ hipError_t launch_trace_selfG(const Grid& g, const DevArrays& d, double mu0, dou
 hipError_t launch_hf0_slab(const Grid& g, const DevArrays& d, double mu0, hipStream_t s);
 hipError_t launch_hf0_bnd(const Grid& g, const DevArrays& d, double mu0, double fact4, hipStream_t s);
 hipError_t launch_hVf0V_slab(const Grid& g, const DevArrays& d, hipStream_t s);
+hipError_t launch_jj_poles(const Grid& g, const DevArrays& d, double mu, hipStream_t s);
+hipError_t launch_jj_transpose(const Grid& g, const DevArrays& d, hipStream_t s);
 hipError_t launch_jj_pencil(const Grid& g, const DevArrays& d, double mu, hipStream_t s);
 hipError_t launch_kx0_rows(const Grid& g, const DevArrays& d, double mu, hipStream_t s);
 hipError_t launch_finalize(const double* part, int n, double* out, hipStream_t s);
