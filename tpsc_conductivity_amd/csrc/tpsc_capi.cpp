@@ -767,27 +767,51 @@ size_t field_count(tpsc_ctx* c, int field)
     }
 }
 
+// The device stores Sigma(k,iw_n) point-major, Self[k*(Nw/2) + n]; copy-outs use the
+// reference's frequency-major layout [n*nk8 + k] (green.cpp:2006).
+// Remainder R(k,n) (FP64 path), complex128 [n*nk8 + k].
+int get_self_remainder(tpsc_ctx* c, double* dst)
+{
+    const Grid& g = c->g;
+    const int H = g.Nw / 2;
+    std::vector<double> tmp(2 * (size_t)H * g.nk8);
+    HIPCHK(c, hipMemcpyAsync(tmp.data(), c->d.Self, tmp.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (long k = 0; k < g.nk8; k++)
+        for (int j = 0; j < H; j++) {
+            dst[2 * ((size_t)j * g.nk8 + k)] = tmp[2 * ((size_t)k * H + j)];
+            dst[2 * ((size_t)j * g.nk8 + k) + 1] = tmp[2 * ((size_t)k * H + j) + 1];
+        }
+    return 0;
+}
+
 // Sigma(k,iw_n), n = 0..Nw/2-1, [n*nk8 + k] complex128: assembled from the remainder and the
 // moments as in green.cpp:3039 (FP64 path) or widened from the stored FP32 values (mixed path)
 int get_sigma_kw(tpsc_ctx* c, double* dst)
 {
     if (!c->have_sigma) return fail(c, TPSC_E_ORDER, "self-energy not computed");
     const Grid& g = c->g;
-    const size_t n = (size_t)(g.Nw / 2) * g.nk8;
+    const int H = g.Nw / 2;
+    const size_t n = (size_t)H * g.nk8;
     hipStream_t s = c->stream;
     if (c->p.precision) {
         std::vector<float> tmp(2 * n);
         HIPCHK(c, hipMemcpyAsync(tmp.data(), c->d.Self, n * sizeof(cf), hipMemcpyDeviceToHost, s));
         HIPCHK(c, hipStreamSynchronize(s));
-        for (size_t i = 0; i < 2 * n; i++) dst[i] = (double)tmp[i];
+        for (long k = 0; k < g.nk8; k++)
+            for (int j = 0; j < H; j++) {
+                dst[2 * ((size_t)j * g.nk8 + k)] = (double)tmp[2 * ((size_t)k * H + j)];
+                dst[2 * ((size_t)j * g.nk8 + k) + 1] = (double)tmp[2 * ((size_t)k * H + j) + 1];
+            }
         return 0;
     }
+    int rc = get_self_remainder(c, dst);
+    if (rc) return rc;
     std::vector<double> s2(g.nk8), s3(g.nk8);
-    HIPCHK(c, hipMemcpyAsync(dst, c->d.Self, n * sizeof(cd), hipMemcpyDeviceToHost, s));
     HIPCHK(c, hipMemcpyAsync(s2.data(), c->d.Sinf2, g.nk8 * sizeof(double), hipMemcpyDeviceToHost, s));
     HIPCHK(c, hipMemcpyAsync(s3.data(), c->d.Sinf3, g.nk8 * sizeof(double), hipMemcpyDeviceToHost, s));
     HIPCHK(c, hipStreamSynchronize(s));
-    for (int j = 0; j < g.Nw / 2; j++) {
+    for (int j = 0; j < H; j++) {
         const double wn = (2 * j + 1) * kPi * g.T, w2 = wn * wn, w3 = w2 * wn, w4 = w2 * w2;
         for (long k = 0; k < g.nk8; k++) {
             double* z = dst + 2 * ((size_t)j * g.nk8 + k);
@@ -1055,8 +1079,7 @@ int tpsc_get(tpsc_ctx* c, int field, void* dst, size_t bytes)
             if (c->p.precision)
                 return fail(c, TPSC_E_INVALID,
                             "the mixed-precision path stores the assembled Sigma(k,iwn): read TPSC_F_SIGMA_KW");
-            src = c->d.Self;
-            break;
+            return get_self_remainder(c, (double*)dst);
         case TPSC_F_SIGMA_KW: return get_sigma_kw(c, (double*)dst);
         case TPSC_F_SIGMA_INF2: src = c->d.Sinf2; break;
         case TPSC_F_SIGMA_INF3: src = c->d.Sinf3; break;

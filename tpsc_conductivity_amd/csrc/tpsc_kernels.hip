@@ -37,7 +37,9 @@ struct SlabOcc<float> {
     static constexpr int waves = 8;
 };
 #define PEN_THREADS 256
-#define JJ_THREADS 512
+#ifndef JJ_THREADS
+#define JJ_THREADS 256
+#endif
 
 // ================================================================== small helpers
 __global__ void k_fill_tw(cd* tw, int L)
@@ -684,7 +686,7 @@ __global__ void __launch_bounds__(PEN_THREADS)
     const double s2 = d0 + dN, s3 = -(M[0] + M[L]), Si = Sinfp[0];
     for (int n = tid; n < L / 2; n += nt) {
         const cd R = twS[n] * X[n];
-        store_self(Self, k + (long)n * nk8, R, Si, s2, s3, (2 * n + 1) * kPi * T);
+        store_self(Self, k * (L / 2) + n, R, Si, s2, s3, (2 * n + 1) * kPi * T);
         if (n == L / 2 - 1) Sinf4[k] = R.x;
     }
     if (tid == 0) {
@@ -715,7 +717,13 @@ __global__ void k_fill_tables(int L, cd* twX, cd* twS, cd* twM, cd* ph)
 
 // ================================================================== traces
 
-// traceDiffG(pmu) partials (green.cpp:3108-3182); grid as k_sum_rule, rows j = 0..Nw
+// Trace reductions over (k, n).  Sigma(k,iwn) is stored point-major, Self[k*(Nw/2) + n], so
+// one wavefront takes one wedge point and its lanes walk the frequencies (coalesced); a block
+// of 256 threads covers 4 wedge points and writes one partial per value, summed afterwards
+// in a fixed order (k_finalize, k_finalize_k).
+constexpr int kTraceK = 4;  // wedge points per trace block
+
+// traceDiffG(pmu) partials (green.cpp:3108-3182), n = 0..Nw
 template <typename SelfT>
 __global__ void __launch_bounds__(256)
     k_trace_diffG(const double* __restrict__ ekw, const double* __restrict__ wq, const SelfT* __restrict__ Self,
@@ -724,20 +732,23 @@ __global__ void __launch_bounds__(256)
                   double mu0, double pmu, double* part)
 {
     __shared__ double sc[16];
-    const int q = blockIdx.x * 256 + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int q = blockIdx.x * kTraceK + (threadIdx.x >> 6);
+    const int H = Nw / 2;
     double acc = 0.0;
     if (q < nk8) {
         const double Si = Sinf[0];
         const double e = ekw[q], w = 2.0 * wq[q];
         const double s2 = Sinf2[q], s3 = Sinf3[q], s4 = Sinf4[q];
-        for (int j = blockIdx.y; j <= Nw; j += gridDim.y) {
+        const SelfT* Sq = Self + (long)q * H;
+        for (int j = lane; j <= Nw; j += 64) {
             const double wn = (2 * j + 1) * kPi * T;
             double rS, iS;
-            if (j >= Nw / 2) {
+            if (j >= H) {
                 rS = -s2 / (wn * wn) + s4 / (wn * wn * wn * wn);
                 iS = -Si / wn + s3 / (wn * wn * wn);
             } else {
-                const cd s = sigma_kw(Self, (long)j * nk8 + q, Si, s2, s3, wn);
+                const cd s = sigma_kw(Sq, j, Si, s2, s3, wn);
                 rS = s.x;
                 iS = s.y;
             }
@@ -748,12 +759,12 @@ __global__ void __launch_bounds__(256)
         }
     }
     acc = block_sum(acc, sc);
-    if (threadIdx.x == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = acc;
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
 }
 
 // traceDiffG and its first two derivatives with respect to pmu.  With a = e - pmu + Re Sigma,
 // b = wn - Im Sigma: Re G = -a/(a^2+b^2), d/dpmu Re G = (b^2-a^2)/(a^2+b^2)^2,
-// d2/dpmu2 Re G = (6ab^2 - 2a^3)/(a^2+b^2)^3.  Partials as k_sum_rule3.
+// d2/dpmu2 Re G = (6ab^2 - 2a^3)/(a^2+b^2)^3.  Partials part[k * nblk + blk], k = 0, 1, 2.
 template <typename SelfT>
 __global__ void __launch_bounds__(256)
     k_trace_diffG3(const double* __restrict__ ekw, const double* __restrict__ wq, const SelfT* __restrict__ Self,
@@ -762,20 +773,23 @@ __global__ void __launch_bounds__(256)
                    double mu0, double pmu, double* part)
 {
     __shared__ double sc[3 * 4];
-    const int q = blockIdx.x * 256 + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int q = blockIdx.x * kTraceK + (threadIdx.x >> 6);
+    const int H = Nw / 2;
     double acc[3] = {0.0, 0.0, 0.0};
     if (q < nk8) {
         const double Si = Sinf[0];
         const double e = ekw[q], w = 2.0 * wq[q];
         const double s2 = Sinf2[q], s3 = Sinf3[q], s4 = Sinf4[q];
-        for (int j = blockIdx.y; j <= Nw; j += gridDim.y) {
+        const SelfT* Sq = Self + (long)q * H;
+        for (int j = lane; j <= Nw; j += 64) {
             const double wn = (2 * j + 1) * kPi * T;
             double rS, iS;
-            if (j >= Nw / 2) {
+            if (j >= H) {
                 rS = -s2 / (wn * wn) + s4 / (wn * wn * wn * wn);
                 iS = -Si / wn + s3 / (wn * wn * wn);
             } else {
-                const cd s = sigma_kw(Self, (long)j * nk8 + q, Si, s2, s3, wn);
+                const cd s = sigma_kw(Sq, j, Si, s2, s3, wn);
                 rS = s.x;
                 iS = s.y;
             }
@@ -790,13 +804,12 @@ __global__ void __launch_bounds__(256)
         }
     }
     block_sum_k<3>(acc, sc);
-    const int nblk = gridDim.x * gridDim.y, b = blockIdx.y * gridDim.x + blockIdx.x;
     if (threadIdx.x == 0)
-        for (int k = 0; k < 3; k++) part[k * nblk + b] = acc[k];
+        for (int k = 0; k < 3; k++) part[k * gridDim.x + blockIdx.x] = acc[k];
 }
 
 // which = 0: Tr Sigma G0 main sum (green.cpp:3018-3051); 1: its Sigma_inf correction
-// (:3053-3084, the j < Nw/2 terms plus one per k); 2: Tr Sigma G (green.cpp:2944-2976)
+// (:3053-3084, the n < Nw/2 terms plus one per k); 2: Tr Sigma G (green.cpp:2944-2976)
 template <typename SelfT>
 __global__ void __launch_bounds__(256)
     k_trace_selfG(const double* __restrict__ ekw, const double* __restrict__ wq, const SelfT* __restrict__ Self,
@@ -804,12 +817,15 @@ __global__ void __launch_bounds__(256)
                   int nk8, int Nw, double T, double mu0, double mu, double nk02, int which, double* part)
 {
     __shared__ double sc[16];
-    const int q = blockIdx.x * 256 + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int q = blockIdx.x * kTraceK + (threadIdx.x >> 6);
+    const int H = Nw / 2;
     double acc = 0.0;
     if (q < nk8) {
         const double Si = Sinf[0];
         const double e = ekw[q], w = wq[q];
-        for (int j = blockIdx.y; j < Nw / 2; j += gridDim.y) {
+        const SelfT* Sq = Self + (long)q * H;
+        for (int j = lane; j < H; j += 64) {
             const double wn = (2 * j + 1) * kPi * T;
             if (which == 1) {
                 acc += w * Si * T * 2 / (nk02 * (wn * wn + (e - mu0) * (e - mu0)));
@@ -823,7 +839,7 @@ __global__ void __launch_bounds__(256)
                 }
                 continue;
             }
-            const cd s = sigma_kw(Self, (long)j * nk8 + q, Si, Sinf2[q], Sinf3[q], wn);
+            const cd s = sigma_kw(Sq, j, Si, Sinf2[q], Sinf3[q], wn);
             if (which == 0) {
                 const double tmp = w * T * 2 / (nk02 * (wn * wn + (e - mu0) * (e - mu0)));
                 acc += tmp * (s.x * (mu0 - e) + s.y * wn);
@@ -834,17 +850,27 @@ __global__ void __launch_bounds__(256)
         }
     }
     acc = block_sum(acc, sc);
-    if (threadIdx.x == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = acc;
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
 }
 
-// column sums of a row-major (nrows x ncols) partial table in fixed row order
-__global__ void k_reduce_cols(const double* __restrict__ part, int nrows, int ncols, double* out)
+// column sums of a row-major (nrows x ncols) partial table in a fixed order: a block of
+// 64 columns x 16 row groups, row group partials added in order
+__global__ void __launch_bounds__(1024) k_reduce_cols(const double* __restrict__ part, int nrows, int ncols,
+                                                      double* out)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncols) return;
-    double s = 0.0;
-    for (int r = 0; r < nrows; r++) s += part[(long)r * ncols + c];
-    out[c] = s;
+    __shared__ double t[16][65];
+    const int cx = threadIdx.x & 63, ry = threadIdx.x >> 6;
+    const int c = blockIdx.x * 64 + cx;
+    double a = 0.0;
+    if (c < ncols)
+        for (int r = ry; r < nrows; r += 16) a += part[(long)r * ncols + c];
+    t[ry][cx] = a;
+    __syncthreads();
+    if (ry == 0 && c < ncols) {
+        double sum = 0.0;
+        for (int i = 0; i < 16; i++) sum += t[i][cx];
+        out[c] = sum;
+    }
 }
 
 // dispersion tables on the device (hamiltonien.cpp:49-68,125-155; green0 wedge weights)
@@ -1102,7 +1128,7 @@ __device__ __forceinline__ cd dyson_G(const SelfT* __restrict__ Self, long ki, l
 {
     kn = (2.0 * (s - H) + 1) * kPi * T;
     const int n = (s < H) ? H - s - 1 : s - H;
-    cd sg = sigma_kw(Self, ki + (long)n * nk8, Sinf, C2, C3, fabs(kn));
+    cd sg = sigma_kw(Self, ki * H + n, Sinf, C2, C3, fabs(kn));
     if (s < H) sg = conjg(sg);
     const cd den = cd{0.0 - e, kn} + cd{mu, 0.0} - sg;
     return T / den;
@@ -1205,7 +1231,7 @@ __global__ void __launch_bounds__(256) k_transpose(const S* __restrict__ in, S* 
 // point-major layout [q][j] (k_transpose); the pole data come from k_jj_poles.
 
 #ifndef JJ_WAVES_PER_SIMD
-#define JJ_WAVES_PER_SIMD 4  // two 512-thread workgroups per CU (74 KB LDS each)
+#define JJ_WAVES_PER_SIMD 2  // two 256-thread workgroups per CU (74 KB LDS each), no register cap
 #endif
 template <int L, int NT, typename S, typename SelfT>
 __global__ void __launch_bounds__(NT, JJ_WAVES_PER_SIMD)
@@ -1572,6 +1598,7 @@ hipError_t launch_chi0max(const Grid& g, const DevArrays& d, hipStream_t s)
 }
 
 static dim3 red_grid(const Grid& g) { return dim3((unsigned)((g.nk8 + 255) / 256), kRowGroups); }
+static dim3 trace_grid(const Grid& g) { return dim3((unsigned)((g.nk8 + kTraceK - 1) / kTraceK)); }
 
 hipError_t launch_sum_rule(const Grid& g, const DevArrays& d, double sU, double* out, hipStream_t s)
 {
@@ -1603,7 +1630,7 @@ hipError_t launch_model(const Grid& g, const DevArrays& d, double tp, double tpp
 
 hipError_t launch_reduce_cols(const double* part, int nrows, int ncols, double* out, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_reduce_cols, dim3((ncols + 255) / 256), dim3(256), 0, s, part, nrows, ncols, out);
+    hipLaunchKernelGGL(k_reduce_cols, dim3((ncols + 63) / 64), dim3(1024), 0, s, part, nrows, ncols, out);
     return hipGetLastError();
 }
 
@@ -1672,7 +1699,7 @@ hipError_t launch_sigma_pencil(const Grid& g, const DevArrays& d, hipStream_t s)
 hipError_t launch_trace_diffG(const Grid& g, const DevArrays& d, double pmu, double mu0, double* out,
                               hipStream_t s)
 {
-    const dim3 gr = red_grid(g);
+    const dim3 gr = trace_grid(g);
     DISPATCH_S(g.prec, {
         hipLaunchKernelGGL(k_trace_diffG<ST>, gr, dim3(256), 0, s, d.ekw, d.wq, (const ST*)d.Self, d.Sinf2, d.Sinf3,
                            d.Sinf4, d.Sinf, (int)g.nk8, g.Nw, g.T, mu0, pmu, d.part);
@@ -1684,7 +1711,7 @@ hipError_t launch_trace_diffG(const Grid& g, const DevArrays& d, double pmu, dou
 hipError_t launch_trace_diffG3(const Grid& g, const DevArrays& d, double pmu, double mu0, double* out,
                                hipStream_t s)
 {
-    const dim3 gr = red_grid(g);
+    const dim3 gr = trace_grid(g);
     DISPATCH_S(g.prec, {
         hipLaunchKernelGGL(k_trace_diffG3<ST>, gr, dim3(256), 0, s, d.ekw, d.wq, (const ST*)d.Self, d.Sinf2, d.Sinf3,
                            d.Sinf4, d.Sinf, (int)g.nk8, g.Nw, g.T, mu0, pmu, d.part);
@@ -1697,7 +1724,7 @@ hipError_t launch_trace_selfG(const Grid& g, const DevArrays& d, double mu0, dou
                               hipStream_t s)
 {
     const double nk02 = 4.0 * (g.nk - 1) * (g.nk - 1);
-    const dim3 gr = red_grid(g);
+    const dim3 gr = trace_grid(g);
     DISPATCH_S(g.prec, {
         hipLaunchKernelGGL(k_trace_selfG<ST>, gr, dim3(256), 0, s, d.ekw, d.wq, (const ST*)d.Self, d.Sinf2, d.Sinf3,
                            d.Sinf, (int)g.nk8, g.Nw, g.T, mu0, mu, nk02, which, d.part);

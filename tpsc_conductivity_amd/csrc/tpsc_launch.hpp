@@ -31,7 +31,7 @@ struct DevArrays {
     double* Sb = nullptr;        // 2*nk8  Sigma(k, tau = 0), Sigma(k, tau = beta)
     double* chirt0 = nullptr;    // nk8    chi0(r, tau = 0)
     double* V0 = nullptr;        // nk8    V(r, tau = 0)
-    cd* Self = nullptr;          // (Nw/2)*nk8
+    cd* Self = nullptr;          // nk8*(Nw/2), point-major: Sigma(k,iw_n) at [k*(Nw/2) + n]
     double* Sinf2 = nullptr;     // nk8
     double* Sinf3 = nullptr;
     double* Sinf4 = nullptr;
