@@ -47,10 +47,11 @@ typedef struct tpsc_params {
 } tpsc_params;
 
 /* precision values.  MIXED: the slab transforms (2D DCT/DST per tau or iqn slice) and their
- * pointwise products run in FP32 and the tau/iqn arrays (chi0, chi0(r,tau), V, Sigma(k,tau),
- * h, f0) are stored in FP32; the tau = 0 / beta boundary data, spline moments, tail moments,
- * pencil Fourier sums, Usp/Uch/mu root reductions and the Lambda_jj Matsubara sums stay in
- * FP64, and Sigma(k,iwn) is stored assembled in FP32 complex.  tpsc_get widens to FP64. */
+ * pointwise products run in FP32 and the tau arrays (chi0(q,tau), chi0(r,tau), V, Sigma(k,tau),
+ * h, f0) are stored in FP32; the tau = 0 / beta boundary slices, spline moments, tail moments,
+ * pencil Fourier sums, chi0(q,iqn), the Usp/Uch/mu root reductions and the Lambda_jj Matsubara
+ * sums stay in FP64, and Sigma(k,iwn) is stored assembled in FP32 complex.  tpsc_get widens
+ * every array to FP64. */
 #define TPSC_PREC_FP64 0
 #define TPSC_PREC_MIXED 1
 
