@@ -275,103 +275,101 @@ __device__ __forceinline__ void slab_r2r_2d(Real* slab, int n0, int n1, int ld, 
 }
 
 // ---------------------------------------------------------------------------------
-// Workgroup Stockham FFT, radix 4 (+ one radix-2 pass for odd log2 L), in place in LDS.
-// sign = -1: exp(-2 pi i jk/L) (FFTW_FORWARD); sign = +1: FFTW_BACKWARD.
-// twg: global table exp(-2 pi i t / L), t < L.
+// Workgroup Stockham FFT in place in LDS: radix-8 passes, then one radix-4 or radix-2 pass
+// for the remaining factor of L.  sign = -1: exp(-2 pi i jk/L) (FFTW_FORWARD); sign = +1:
+// FFTW_BACKWARD.  twg: global table exp(-2 pi i t / L), t < L.
+// Pass with butterfly span Ns: butterfly b (k = b mod Ns) reads x[b + r L/RAD], multiplies
+// by w^(r k L/(Ns RAD)), takes an RAD-point DFT and writes y[(b/Ns) Ns RAD + k + m Ns].
+// The twiddles w^r come from three table reads (w, w^2, w^4) and products, issued before
+// the LDS reads and the barrier so their latency overlaps them.
 // ---------------------------------------------------------------------------------
+template <int RAD, int L, int NT>
+__device__ __forceinline__ void fft_pass(cd* buf, int sign, const cd* __restrict__ twg, int Ns)
+{
+    constexpr int NB = L / RAD;
+    constexpr int PER = (NB + NT - 1) / NT;
+    const int tid = threadIdx.x;
+    const int stride = L / (Ns * RAD);
+    cd w1[PER], w2[PER], w4[PER];
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+        const int b = tid + i * NT;
+        const int k = b % Ns;
+        cd a = cd{1.0, 0.0}, c2 = cd{1.0, 0.0}, c4 = cd{1.0, 0.0};
+        if (Ns > 1 && b < NB) {
+            a = twg[(k * stride) & (L - 1)];
+            if (RAD >= 4) c2 = twg[(2 * k * stride) & (L - 1)];
+            if (RAD >= 8) c4 = twg[(4 * k * stride) & (L - 1)];
+        }
+        if (sign > 0) {
+            a.y = -a.y;
+            c2.y = -c2.y;
+            c4.y = -c4.y;
+        }
+        w1[i] = a;
+        w2[i] = c2;
+        w4[i] = c4;
+    }
+    cd v[PER][RAD];
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+        const int b = tid + i * NT;
+        if (b < NB) {
+#pragma unroll
+            for (int r = 0; r < RAD; r++) v[i][r] = buf[b + r * NB];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+        const int b = tid + i * NT;
+        if (b < NB) {
+            const int k = b % Ns;
+            if (Ns > 1) {
+                v[i][1] = v[i][1] * w1[i];
+                if (RAD >= 4) {
+                    v[i][2] = v[i][2] * w2[i];
+                    v[i][3] = v[i][3] * (w1[i] * w2[i]);
+                }
+                if (RAD >= 8) {
+                    const cd w3 = w1[i] * w2[i];
+                    v[i][4] = v[i][4] * w4[i];
+                    v[i][5] = v[i][5] * (w4[i] * w1[i]);
+                    v[i][6] = v[i][6] * (w4[i] * w2[i]);
+                    v[i][7] = v[i][7] * (w4[i] * w3);
+                }
+            }
+            if (sign > 0) {
+#pragma unroll
+                for (int r = 0; r < RAD; r++) v[i][r].y = -v[i][r].y;
+            }
+            reg_dft<RAD>(v[i]);
+            if (sign > 0) {
+#pragma unroll
+                for (int r = 0; r < RAD; r++) v[i][r].y = -v[i][r].y;
+            }
+            const int dst = (b / Ns) * Ns * RAD + k;
+#pragma unroll
+            for (int r = 0; r < RAD; r++) buf[dst + r * Ns] = v[i][r];
+        }
+    }
+    __syncthreads();
+}
+
 template <int L, int NT>
 __device__ __forceinline__ void block_fft(cd* buf, int sign, const cd* __restrict__ twg)
 {
     constexpr int LOG2 = __builtin_ctz(L);
-    constexpr int P4 = LOG2 / 2;
-    constexpr int NB4 = L / 4;
-    constexpr int PER4 = (NB4 + NT - 1) / NT;
-    const int tid = threadIdx.x;
+    constexpr int P8 = LOG2 / 3;
+    constexpr int REM = LOG2 - 3 * P8;
     int Ns = 1;
-    for (int pass = 0; pass < P4; pass++) {
-        // twiddles first: they do not depend on the data, so their global-memory latency
-        // overlaps the LDS reads and the barrier (pass 0 has k = 0: all twiddles are 1)
-        cd w[PER4][3];
 #pragma unroll
-        for (int i = 0; i < PER4; i++) {
-            const int b = tid + i * NT;
-            const int k = b % Ns;
-            const int stride = L / (Ns * 4);
-#pragma unroll
-            for (int r = 1; r < 4; r++) {
-                cd t = cd{1.0, 0.0};
-                if (pass > 0 && b < NB4) t = twg[(r * k * stride) & (L - 1)];
-                if (sign > 0) t.y = -t.y;
-                w[i][r - 1] = t;
-            }
-        }
-        cd v[PER4][4];
-#pragma unroll
-        for (int i = 0; i < PER4; i++) {
-            const int b = tid + i * NT;
-            if (b < NB4) {
-#pragma unroll
-                for (int r = 0; r < 4; r++) v[i][r] = buf[b + r * NB4];
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int i = 0; i < PER4; i++) {
-            const int b = tid + i * NT;
-            if (b < NB4) {
-                const int k = b % Ns;
-                if (pass > 0) {
-#pragma unroll
-                    for (int r = 1; r < 4; r++) v[i][r] = v[i][r] * w[i][r - 1];
-                }
-                const cd a0 = v[i][0], a1 = v[i][1], a2 = v[i][2], a3 = v[i][3];
-                const cd s02 = a0 + a2, d02 = a0 - a2, s13 = a1 + a3, d13 = a1 - a3;
-                const cd rot = (sign < 0) ? mul_mi(d13) : mul_pi(d13);
-                const int dst = (b / Ns) * Ns * 4 + k;
-                buf[dst] = s02 + s13;
-                buf[dst + Ns] = d02 + rot;
-                buf[dst + 2 * Ns] = s02 - s13;
-                buf[dst + 3 * Ns] = d02 - rot;
-            }
-        }
-        __syncthreads();
-        Ns *= 4;
+    for (int pass = 0; pass < P8; pass++) {
+        fft_pass<8, L, NT>(buf, sign, twg, Ns);
+        Ns *= 8;
     }
-    if (LOG2 & 1) {
-        constexpr int NB2 = L / 2;
-        constexpr int PER2 = (NB2 + NT - 1) / NT;
-        cd w[PER2];
-#pragma unroll
-        for (int i = 0; i < PER2; i++) {
-            const int b = tid + i * NT;
-            cd t = cd{1.0, 0.0};
-            if (b < NB2) t = twg[((b % Ns) * (L / (Ns * 2))) & (L - 1)];
-            if (sign > 0) t.y = -t.y;
-            w[i] = t;
-        }
-        cd v[PER2][2];
-#pragma unroll
-        for (int i = 0; i < PER2; i++) {
-            const int b = tid + i * NT;
-            if (b < NB2) {
-                v[i][0] = buf[b];
-                v[i][1] = buf[b + NB2];
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int i = 0; i < PER2; i++) {
-            const int b = tid + i * NT;
-            if (b < NB2) {
-                const int k = b % Ns;
-                const cd a = v[i][0], c = v[i][1] * w[i];
-                const int dst = (b / Ns) * Ns * 2 + k;
-                buf[dst] = a + c;
-                buf[dst + Ns] = a - c;
-            }
-        }
-        __syncthreads();
-    }
+    if constexpr (REM == 2) fft_pass<4, L, NT>(buf, sign, twg, Ns);
+    if constexpr (REM == 1) fft_pass<2, L, NT>(buf, sign, twg, Ns);
 }
 
 // ---------------------------------------------------------------------------------
@@ -542,7 +540,7 @@ __device__ __forceinline__ void block_spline_inplace(double* yM, int N, double h
     const int nchunk = (N + 1 + Cr - 1) / Cr;
     const int j0 = tid * Cr, j1 = min(N + 1, j0 + Cr);
     double* bnd = ES + 2 * NT;  // r_0, r_N, M_0, M_1, M_{N-1}, M_N
-    double ri[C], u[C], w[C];
+    double ri[C], u[C];
     double eu = 0.0, ew = 0.0, tr = 0.0;
 #pragma unroll
     for (int c = 0; c < C; c++) {
@@ -563,20 +561,17 @@ __device__ __forceinline__ void block_spline_inplace(double* yM, int N, double h
         }
         ri[c] = r;
     }
+    // chunk-end states of the two recursions from a zero start
     if (j0 < j1) {
         double sa = 0.0;
 #pragma unroll
-        for (int c = 0; c < C; c++) {
-            sa = ri[c] + z * sa;
-            u[c] = sa;
-            if (c == Cr - 1) eu = sa;
-        }
+        for (int c = 0; c < C; c++)
+            if (j0 + c < j1) sa = ri[c] + z * sa;
+        eu = sa;
         sa = 0.0;
 #pragma unroll
-        for (int c = C - 1; c >= 0; c--) {
+        for (int c = C - 1; c >= 0; c--)
             if (j0 + c < j1) sa = ri[c] + z * sa;
-            w[c] = sa;
-        }
         ew = sa;
     }
     ES[tid] = eu;
@@ -591,25 +586,20 @@ __device__ __forceinline__ void block_spline_inplace(double* yM, int N, double h
             if (tid + d < nchunk) Sw += zd * ES[NT + tid + d];
             zd *= zc;
         }
-        double zp = z;
+        // the recursions again, seeded with the carries from the neighbouring chunks
+        double sa = Su;
 #pragma unroll
         for (int c = 0; c < C; c++) {
-            u[c] += zp * Su;
-            zp *= z;
+            sa = ri[c] + z * sa;
+            u[c] = sa;
         }
-        zp = z;
+        sa = Sw;
 #pragma unroll
         for (int c = C - 1; c >= 0; c--) {
-            if (j0 + c < j1) {
-                w[c] += zp * Sw;
-                zp *= z;
-            }
-        }
-#pragma unroll
-        for (int c = 0; c < C; c++) {
             const int j = j0 + c;
             if (j < j1) {
-                const double m = A * (u[c] + w[c] - ri[c]);
+                sa = ri[c] + z * sa;
+                const double m = A * (u[c] + sa - ri[c]);
                 yM[j] = m;
                 u[c] = m;
             }
