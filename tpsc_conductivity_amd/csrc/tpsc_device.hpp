@@ -108,13 +108,48 @@ struct cplx_of<float> {
     using type = cf;
 };
 
-__device__ inline cd shfl_xor_c(cd v, int mask)
+// v from lane (lane ^ M), M a power of two < 64.  M = 1, 2: DPP quad permutation; M = 4:
+// quad permutation 3-2-1-0 then the 8-lane mirror (i -> 7 - i = i ^ 7); M = 8: 8-lane mirror
+// then the 16-lane mirror (i ^ 7 ^ 15); these are VALU moves.  M = 16: ds_swizzle in
+// bitmask mode (xor within 32 lanes); M = 32: ds_bpermute.
+template <int M>
+__device__ __forceinline__ int xor_lane_i(int x)
 {
-    return cd{__shfl_xor(v.x, mask, 64), __shfl_xor(v.y, mask, 64)};
+    if constexpr (M == 1) return __builtin_amdgcn_update_dpp(0, x, 0xB1, 0xF, 0xF, false);
+    else if constexpr (M == 2) return __builtin_amdgcn_update_dpp(0, x, 0x4E, 0xF, 0xF, false);
+    else if constexpr (M == 4) {
+        const int t = __builtin_amdgcn_update_dpp(0, x, 0x1B, 0xF, 0xF, false);
+        return __builtin_amdgcn_update_dpp(0, t, 0x141, 0xF, 0xF, false);
+    } else if constexpr (M == 8) {
+        const int t = __builtin_amdgcn_update_dpp(0, x, 0x141, 0xF, 0xF, false);
+        return __builtin_amdgcn_update_dpp(0, t, 0x140, 0xF, 0xF, false);
+    } else if constexpr (M == 16)
+        return __builtin_amdgcn_ds_swizzle(x, 0x401F);
+    else
+        return __shfl_xor(x, M, 64);
 }
-__device__ inline cf shfl_xor_c(cf v, int mask)
+template <int M>
+__device__ __forceinline__ float xor_lane(float v)
 {
-    return cf{__shfl_xor(v.x, mask, 64), __shfl_xor(v.y, mask, 64)};
+    return __int_as_float(xor_lane_i<M>(__float_as_int(v)));
+}
+template <int M>
+__device__ __forceinline__ double xor_lane(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = xor_lane_i<M>((int)(b & 0xffffffffLL));
+    const int hi = xor_lane_i<M>((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+template <int M>
+__device__ __forceinline__ cd shfl_xor_c(cd v)
+{
+    return cd{xor_lane<M>(v.x), xor_lane<M>(v.y)};
+}
+template <int M>
+__device__ __forceinline__ cf shfl_xor_c(cf v)
+{
+    return cf{xor_lane<M>(v.x), xor_lane<M>(v.y)};
 }
 
 // ---------------------------------------------------------------------------------
@@ -169,8 +204,24 @@ __device__ __forceinline__ void reg_dft(C* v)
 // Must be called by every thread of the block (uniform control flow); caller syncs.
 // tw: LDS table tw[t] = exp(-2 pi i t / L), t < L, in the precision of Real.
 // ---------------------------------------------------------------------------------
-template <typename Real, int R, int G>
-__device__ __forceinline__ void wave_r2r_rows(Real* base, int nrows, int rs, int es, int kind,
+// cross-lane DIF stages of span HH, HH/2, ..., 1 (lane p of a G-lane group)
+template <int HH, int R, int L, typename C>
+__device__ __forceinline__ void dif_stages(C* v, int p, const C* tw)
+{
+    if constexpr (HH >= 1) {
+        const bool upper = (p & HH) != 0;
+        const C w = tw[(p & (HH - 1)) * (L / (2 * HH))];
+#pragma unroll
+        for (int k1 = 0; k1 < R; k1++) {
+            const C o = shfl_xor_c<HH>(v[k1]);
+            v[k1] = upper ? (o - v[k1]) * w : (v[k1] + o);
+        }
+        dif_stages<HH / 2, R, L>(v, p, tw);
+    }
+}
+
+template <typename Real, int R, int G, int KIND>
+__device__ __forceinline__ void wave_r2r_rows(Real* base, int nrows, int rs, int es,
                                               const typename cplx_of<Real>::type* tw)
 {
     using C = typename cplx_of<Real>::type;
@@ -181,31 +232,34 @@ __device__ __forceinline__ void wave_r2r_rows(Real* base, int nrows, int rs, int
     const int nwaves = blockDim.x >> 6;
     const int sub = lane / G, p = lane % G;
     const int npairs = (nrows + 1) >> 1;
-    const int half = L / 2;
+    constexpr int half = L / 2;
     const int rounds = (npairs + nwaves * P - 1) / (nwaves * P);
+    int br = 0;  // bit reversal of p over log2(G) bits
+#pragma unroll
+    for (int b = 1, q = G >> 1; b < G; b <<= 1, q >>= 1)
+        if (p & b) br |= q;
     for (int rd = 0; rd < rounds; rd++) {
         const int task = (rd * nwaves + wave) * P + sub;
         const bool valid = task < npairs;
         const int r1 = 2 * task, r2 = 2 * task + 1;
         const bool has2 = valid && (r2 < nrows);
+        const long o1 = (long)r1 * rs, o2 = (long)r2 * rs;
         C v[R];
 #pragma unroll
         for (int k = 0; k < R; k++) {
             const int j = p + G * k;
             Real a = 0, b = 0;
-            if (valid) {
-                if (kind == 0) {
-                    const int e = (j <= half) ? j : L - j;
-                    a = base[(long)r1 * rs + (long)e * es];
-                    if (has2) b = base[(long)r2 * rs + (long)e * es];
-                } else if (j != 0 && j != half) {
-                    const int e = (j < half) ? j - 1 : L - j - 1;
-                    a = base[(long)r1 * rs + (long)e * es];
-                    if (has2) b = base[(long)r2 * rs + (long)e * es];
-                    if (j > half) {
-                        a = -a;
-                        b = -b;
-                    }
+            if (KIND == 0) {
+                const int e = (j <= half) ? j : L - j;
+                if (valid) a = base[o1 + (long)e * es];
+                if (has2) b = base[o2 + (long)e * es];
+            } else if (j != 0 && j != half) {
+                const int e = (j < half) ? j - 1 : L - j - 1;
+                if (valid) a = base[o1 + (long)e * es];
+                if (has2) b = base[o2 + (long)e * es];
+                if (j > half) {
+                    a = -a;
+                    b = -b;
                 }
             }
             v[k] = C{a, b};
@@ -215,32 +269,19 @@ __device__ __forceinline__ void wave_r2r_rows(Real* base, int nrows, int rs, int
 #pragma unroll
         for (int k1 = 1; k1 < R; k1++) v[k1] = v[k1] * tw[(p * k1) % L];
         // G-point DIF across lanes; lane p ends with output index bitrev(p)
-#pragma unroll
-        for (int hh = G / 2; hh >= 1; hh >>= 1) {
-            const bool upper = (p & hh) != 0;
-            const C w = tw[(p & (hh - 1)) * (L / (2 * hh))];
-#pragma unroll
-            for (int k1 = 0; k1 < R; k1++) {
-                const C o = shfl_xor_c(v[k1], hh);
-                v[k1] = upper ? (o - v[k1]) * w : (v[k1] + o);
-            }
-        }
-        int br = 0;
-#pragma unroll
-        for (int b = 1, q = G >> 1; b < G; b <<= 1, q >>= 1)
-            if (p & b) br |= q;
+        dif_stages<G / 2, R, L>(v, p, tw);
         if (valid) {
 #pragma unroll
             for (int k1 = 0; k1 < R; k1++) {
                 const int k = k1 + R * br;
-                if (kind == 0) {
+                if (KIND == 0) {
                     if (k <= half) {
-                        base[(long)r1 * rs + (long)k * es] = v[k1].x;
-                        if (has2) base[(long)r2 * rs + (long)k * es] = v[k1].y;
+                        base[o1 + (long)k * es] = v[k1].x;
+                        if (has2) base[o2 + (long)k * es] = v[k1].y;
                     }
                 } else if (k >= 1 && k <= half - 1) {
-                    base[(long)r1 * rs + (long)(k - 1) * es] = -v[k1].y;
-                    if (has2) base[(long)r2 * rs + (long)(k - 1) * es] = v[k1].x;
+                    base[o1 + (long)(k - 1) * es] = -v[k1].y;
+                    if (has2) base[o2 + (long)(k - 1) * es] = v[k1].x;
                 }
             }
         }
@@ -261,16 +302,16 @@ __device__ __forceinline__ void fill_twiddles(C* tw, int L)
 }
 
 // 2D transform of an n0 x n1 LDS slab (row-major, row stride ld): axis 1 then axis 0.
-// kind0 along axis 0 (slow index), kind1 along axis 1.  Both use FFT length L = N.
-template <typename Real, int N>
-__device__ __forceinline__ void slab_r2r_2d(Real* slab, int n0, int n1, int ld, int kind0, int kind1,
-                                            const typename cplx_of<Real>::type* tw)
+// KIND0 along axis 0 (slow index), KIND1 along axis 1 (0: REDFT00, 1: RODFT00).  Both use
+// FFT length N.
+template <typename Real, int N, int KIND0, int KIND1>
+__device__ __forceinline__ void slab_r2r_2d(Real* slab, int n0, int n1, int ld, const typename cplx_of<Real>::type* tw)
 {
     constexpr int G = (N < 64) ? N : 64;
     constexpr int R = N / G;
-    wave_r2r_rows<Real, R, G>(slab, n0, ld, 1, kind1, tw);  // rows (axis 1)
+    wave_r2r_rows<Real, R, G, KIND1>(slab, n0, ld, 1, tw);  // rows (axis 1)
     __syncthreads();
-    wave_r2r_rows<Real, R, G>(slab, n1, 1, ld, kind0, tw);  // columns (axis 0)
+    wave_r2r_rows<Real, R, G, KIND0>(slab, n1, 1, ld, tw);  // columns (axis 0)
     __syncthreads();
 }
 

@@ -149,7 +149,7 @@ __global__ void __launch_bounds__(SLAB_THREADS)
         put_sym(slab, nk, l, m, -x * Gkt / nk02);
     }
     __syncthreads();
-    slab_r2r_2d<double, N>(slab, nk, nk, nk, 0, 0, tw);
+    slab_r2r_2d<double, N, 0, 0>(slab, nk, nk, nk, tw);
     for (long i = tid; i < nk8; i += nt) {
         int l, m;
         wedge_lm(i, l, m);
@@ -166,7 +166,7 @@ __global__ void __launch_bounds__(SLAB_THREADS)
         put_sym(slab, nk, l, m, (-1.0 / (exp(-x / T) + 1.0)) / nk02);
     }
     __syncthreads();
-    slab_r2r_2d<double, N>(slab, nk, nk, nk, 0, 0, tw);
+    slab_r2r_2d<double, N, 0, 0>(slab, nk, nk, nk, tw);
     for (long i = tid; i < nk8; i += nt) {
         int l, m;
         wedge_lm(i, l, m);
@@ -184,7 +184,7 @@ __global__ void __launch_bounds__(SLAB_THREADS)
         put_sym(slab, nk, l, m, stg[i]);
     }
     __syncthreads();
-    slab_r2r_2d<double, N>(slab, nk, nk, nk, 0, 0, tw);
+    slab_r2r_2d<double, N, 0, 0>(slab, nk, nk, nk, tw);
     for (long i = tid; i < nk8; i += nt) {
         int l, m;
         wedge_lm(i, l, m);
@@ -206,7 +206,7 @@ __global__ void __launch_bounds__(SLAB_THREADS)
         put_sym(slab, nk, l, m, stg[i]);
     }
     __syncthreads();
-    slab_r2r_2d<double, N>(slab, nk, nk, nk, 0, 0, tw);
+    slab_r2r_2d<double, N, 0, 0>(slab, nk, nk, nk, tw);
     for (long i = tid; i < nk8; i += nt) {
         int l, m;
         wedge_lm(i, l, m);
@@ -239,7 +239,7 @@ __global__ void __launch_bounds__(SLAB_THREADS, SlabOcc<S>::waves)
             put_sym_t(slab, nk, l, m, (S)g0_plus(ekw[i] - mu0, t, T, nk02, 0));
         }
         __syncthreads();
-        slab_r2r_2d<S, N>(slab, nk, nk, nk, 0, 0, tw);
+        slab_r2r_2d<S, N, 0, 0>(slab, nk, nk, nk, tw);
         for (long i = tid; i < nk8; i += nt) {  // stage G(r,t) in chirt[j]
             int l, m;
             wedge_lm(i, l, m);
@@ -252,7 +252,7 @@ __global__ void __launch_bounds__(SLAB_THREADS, SlabOcc<S>::waves)
             put_sym_t(slab, nk, l, m, (S)g0_minus(ekw[i] - mu0, t, T, nk02));
         }
         __syncthreads();
-        slab_r2r_2d<S, N>(slab, nk, nk, nk, 0, 0, tw);
+        slab_r2r_2d<S, N, 0, 0>(slab, nk, nk, nk, tw);
         for (long i = tid; i < nk8; i += nt) {  // chi0(r,t) = -2 G(r,-t) G(r,t)
             int l, m;
             wedge_lm(i, l, m);
@@ -273,7 +273,7 @@ __global__ void __launch_bounds__(SLAB_THREADS, SlabOcc<S>::waves)
             put_sym_t(slab, nk, l, m, (S)g0_plus(ekw[i] - mu0, t, T, nk02, 1));
         }
         __syncthreads();
-        slab_r2r_2d<S, N>(slab, nk, nk, nk, 0, 0, tw);
+        slab_r2r_2d<S, N, 0, 0>(slab, nk, nk, nk, tw);
         for (int s = tid; s < nk * nk; s += nt) slab[s] = (S)2 * slab[s] * slab[s];
         __syncthreads();
         for (long i = tid; i < nk8; i += nt) {
@@ -283,7 +283,7 @@ __global__ void __launch_bounds__(SLAB_THREADS, SlabOcc<S>::waves)
         }
         __syncthreads();
     }
-    slab_r2r_2d<S, N>(slab, nk, nk, nk, 0, 0, tw);
+    slab_r2r_2d<S, N, 0, 0>(slab, nk, nk, nk, tw);
     for (long i = tid; i < nk8; i += nt) {
         int l, m;
         wedge_lm(i, l, m);
@@ -436,7 +436,8 @@ __global__ void __launch_bounds__(256)
                 c = chi[(long)j * nk8 + q];
             else {
                 const double wn = 2 * j * kPi * T;
-                c = a1 / (wn * wn) + a2 / (wn * wn * wn * wn);
+                const double i2 = 1.0 / (wn * wn);
+                c = (a1 + a2 * i2) * i2;
             }
             const double r = 1.0 / (1 + sU * c / 2.0);
             acc[0] += w * (c * r - c);
@@ -483,7 +484,7 @@ __global__ void __launch_bounds__(SLAB_THREADS, SlabOcc<S>::waves)
         put_sym_t(slab, nk, l, m, (S)(normFact * U * (3.0 * Usp * (csp - c) + Uch * (cch - c)) / 8.0));
     }
     __syncthreads();
-    slab_r2r_2d<S, N>(slab, nk, nk, nk, 0, 0, tw);
+    slab_r2r_2d<S, N, 0, 0>(slab, nk, nk, nk, tw);
     for (long i = tid; i < nk8; i += nt) {
         int l, m;
         wedge_lm(i, l, m);
@@ -581,7 +582,7 @@ __global__ void __launch_bounds__(SLAB_THREADS)
         slab[s] = (side == 0) ? -1.0 / (exp(-x / T) + 1.0) / nk02 : -1.0 / (exp(x / T) + 1.0) / nk02;
     }
     __syncthreads();
-    slab_r2r_2d<double, N>(slab, nk, nk, nk, 0, 0, tw);
+    slab_r2r_2d<double, N, 0, 0>(slab, nk, nk, nk, tw);
     for (long i = tid; i < nk8; i += nt) {
         int l, m;
         wedge_lm(i, l, m);
@@ -603,7 +604,7 @@ __global__ void __launch_bounds__(SLAB_THREADS)
             put_sym(slab, nk, l, m, stg[pass * nk8 + i]);
         }
         __syncthreads();
-        slab_r2r_2d<double, N>(slab, nk, nk, nk, 0, 0, tw);
+        slab_r2r_2d<double, N, 0, 0>(slab, nk, nk, nk, tw);
         double* o = (pass == 0) ? dSig : Sb;
         for (long i = tid; i < nk8; i += nt) {
             int l, m;
@@ -637,7 +638,7 @@ __global__ void __launch_bounds__(SLAB_THREADS, SlabOcc<S>::waves)
                               : -exp((1.0 / T - t) * x) / (exp(x / T) + 1.0) / nk02);
     }
     __syncthreads();
-    slab_r2r_2d<S, N>(slab, nk, nk, nk, 0, 0, tw);
+    slab_r2r_2d<S, N, 0, 0>(slab, nk, nk, nk, tw);
     const S* Vj = V + (long)jv * nk8;
     for (long i = tid; i < nk8; i += nt) {
         int l, m;
@@ -646,7 +647,7 @@ __global__ void __launch_bounds__(SLAB_THREADS, SlabOcc<S>::waves)
         put_sym_t(slab, nk, l, m, v);
     }
     __syncthreads();
-    slab_r2r_2d<S, N>(slab, nk, nk, nk, 0, 0, tw);
+    slab_r2r_2d<S, N, 0, 0>(slab, nk, nk, nk, tw);
     S* o = Stau + (long)j * nk8;
     for (long i = tid; i < nk8; i += nt) {
         int l, m;
@@ -782,23 +783,24 @@ __global__ void __launch_bounds__(256)
         const double e = ekw[q], w = 2.0 * wq[q];
         const double s2 = Sinf2[q], s3 = Sinf3[q], s4 = Sinf4[q];
         const SelfT* Sq = Self + (long)q * H;
+        const double c = e - mu0;
         for (int j = lane; j <= Nw; j += 64) {
             const double wn = (2 * j + 1) * kPi * T;
             double rS, iS;
             if (j >= H) {
-                rS = -s2 / (wn * wn) + s4 / (wn * wn * wn * wn);
-                iS = -Si / wn + s3 / (wn * wn * wn);
+                const double iw = 1.0 / wn, iw2 = iw * iw;
+                rS = (-s2 + s4 * iw2) * iw2;
+                iS = (-Si + s3 * iw2) * iw;
             } else {
                 const cd s = sigma_kw(Sq, j, Si, s2, s3, wn);
                 rS = s.x;
                 iS = s.y;
             }
-            const double tmp = w / (((wn - iS) * (wn - iS) + (e - pmu + rS) * (e - pmu + rS)) *
-                                    (wn * wn + (e - mu0) * (e - mu0)));
-            acc[0] += tmp * ((mu0 - pmu + rS) * (e - pmu + rS) * (e - mu0) + iS * (iS - 2 * wn) * (e - mu0) -
-                             wn * wn * (mu0 - pmu + rS));
             const double a = e - pmu + rS, b = wn - iS;
             const double a2 = a * a, b2 = b * b, r = 1.0 / (a2 + b2);
+            const double r0 = 1.0 / (wn * wn + c * c);
+            // Re G - Re G0 = [c (a^2+b^2) - a (c^2+wn^2)] r r0, written without cancellation
+            acc[0] += w * r * r0 * ((mu0 - pmu + rS) * a * c + iS * (iS - 2 * wn) * c - wn * wn * (mu0 - pmu + rS));
             acc[1] += w * (b2 - a2) * r * r;
             acc[2] += w * (6.0 * a * b2 - 2.0 * a * a2) * r * r * r;
         }
@@ -948,7 +950,7 @@ __global__ void __launch_bounds__(SLAB_THREADS, SlabOcc<S>::waves)
             slab[s] = (S)hf0_val(which, ek[m + (l + 1) * nk], dek[s], mu0, T, t, nf);
         }
         __syncthreads();
-        slab_r2r_2d<S, N>(slab, nk - 2, nk, nk, 1, 0, tw);
+        slab_r2r_2d<S, N, 1, 0>(slab, nk - 2, nk, nk, tw);
         S* o = (which == 0 ? h : f0) + (long)j * nq;
         for (long s = tid; s < nq; s += nt) o[s] = slab[s];
         __syncthreads();
@@ -985,7 +987,7 @@ __global__ void __launch_bounds__(SLAB_THREADS)
         slab[s] = hf0_val(which, ek[m + (l + 1) * nk], dek[s], mu0, T, tb, nf);
     }
     __syncthreads();
-    slab_r2r_2d<double, N>(slab, nk - 2, nk, nk, 1, 0, tw);
+    slab_r2r_2d<double, N, 1, 0>(slab, nk - 2, nk, nk, tw);
     for (long s = tid; s < nq; s += nt) hr[s] = slab[s];
     __syncthreads();
     // (2)
@@ -1017,7 +1019,7 @@ __global__ void __launch_bounds__(SLAB_THREADS)
         slab[s] = val;
     }
     __syncthreads();
-    slab_r2r_2d<double, N>(slab, nk - 2, nk, nk, 1, 0, tw);
+    slab_r2r_2d<double, N, 1, 0>(slab, nk - 2, nk, nk, tw);
     for (long s = tid; s < nq; s += nt) {
         const int l = (int)(s / nk), m = (int)(s % nk);
         const long r = fold_idx(l + 1, m, N);
@@ -1025,7 +1027,7 @@ __global__ void __launch_bounds__(SLAB_THREADS)
         slab[s] = (side == 0) ? 2.0 * d * v + hs * fact4 * dchirt0[r] : 2.0 * d * v - hs * fact4 * dchirt0[r];
     }
     __syncthreads();
-    slab_r2r_2d<double, N>(slab, nk - 2, nk, nk, 1, 0, tw);
+    slab_r2r_2d<double, N, 1, 0>(slab, nk - 2, nk, nk, tw);
     double* o = ((which == 0) ? dhV : df0V) + side * nq;
     for (long s = tid; s < nq; s += nt) o[s] = slab[s];
     __syncthreads();
@@ -1035,7 +1037,7 @@ __global__ void __launch_bounds__(SLAB_THREADS)
         slab[s] = 2.0 * hr[s] * V0[fold_idx(l + 1, m, N)];
     }
     __syncthreads();
-    slab_r2r_2d<double, N>(slab, nk - 2, nk, nk, 1, 0, tw);
+    slab_r2r_2d<double, N, 1, 0>(slab, nk - 2, nk, nk, tw);
     double* ob = ((which == 0) ? hVb : f0Vb) + side * nq;
     for (long s = tid; s < nq; s += nt) ob[s] = slab[s];
 }
@@ -1063,7 +1065,7 @@ __global__ void __launch_bounds__(SLAB_THREADS, SlabOcc<S>::waves)
             slab[s] = (S)2 * a[s] * Vj[fold_idx(l + 1, m, N)];
         }
         __syncthreads();
-        slab_r2r_2d<S, N>(slab, nk - 2, nk, nk, 1, 0, tw);
+        slab_r2r_2d<S, N, 1, 0>(slab, nk - 2, nk, nk, tw);
         for (long s = tid; s < nq; s += nt) a[s] = slab[s];
         __syncthreads();
     }
