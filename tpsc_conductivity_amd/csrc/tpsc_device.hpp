@@ -1,6 +1,7 @@
 // tpsc_device.hpp -- device building blocks of the TPSC hot path on gfx950 (CDNA4).
 //
-//  * cd            : complex double with the plain (c^2+d^2) division formula
+//  * cd            : complex double; division by the plain formula with one reciprocal of
+//                    (c^2+d^2)
 //  * wave_r2r_*    : FFTW REDFT00 / RODFT00 of LDS-resident rows, two real rows packed
 //                    into one complex sequence, one length-N FFT per 64-lane wavefront
 //                    (R values per lane, cross-lane radix-2 stages with xor shuffles)
@@ -30,8 +31,8 @@ __host__ __device__ inline cd operator*(cd a, double s) { return cd{a.x * s, a.y
 __host__ __device__ inline cd operator/(cd a, double s) { return cd{a.x / s, a.y / s}; }
 __host__ __device__ inline cd operator/(cd a, cd b)
 {
-    const double den = b.x * b.x + b.y * b.y;
-    return cd{(a.x * b.x + a.y * b.y) / den, (a.y * b.x - a.x * b.y) / den};
+    const double r = 1.0 / (b.x * b.x + b.y * b.y);
+    return cd{(a.x * b.x + a.y * b.y) * r, (a.y * b.x - a.x * b.y) * r};
 }
 __host__ __device__ inline cd operator/(double s, cd b) { return cd{s, 0.0} / b; }
 __host__ __device__ inline cd conjg(cd a) { return cd{a.x, -a.y}; }
@@ -643,12 +644,17 @@ __device__ __forceinline__ void block_spline_inplace(double* yM, int N, double h
                 const double m = A * (u[c] + sa - ri[c]);
                 yM[j] = m;
                 u[c] = m;
+                // end moments for the boundary modes (read after the barrier, never rewritten)
+                if (j == 0) bnd[2] = m;
+                if (j == 1) bnd[3] = m;
+                if (j == N - 1) bnd[4] = m;
+                if (j == N) bnd[5] = m;
             }
         }
     }
     __syncthreads();
-    const double R0 = bnd[0] - 2.0 * yM[0] - yM[1];
-    const double RN = bnd[1] - yM[N - 1] - 2.0 * yM[N];
+    const double R0 = bnd[0] - 2.0 * bnd[2] - bnd[3];
+    const double RN = bnd[1] - bnd[4] - 2.0 * bnd[5];
     double zN1 = 1.0;  // z^(N-1)
     {
         double zz = z;
@@ -663,7 +669,6 @@ __device__ __forceinline__ void block_spline_inplace(double* yM, int N, double h
     const double det = dg * dg - off * off;
     const double alpha = (dg * R0 - off * RN) / det;
     const double beta = (dg * RN - off * R0) / det;
-    __syncthreads();
     double tm = 0.0;
 #pragma unroll
     for (int c = 0; c < C; c++) {

@@ -1142,6 +1142,10 @@ __device__ __forceinline__ cd dyson_G(const SelfT* __restrict__ Self, long ki, l
 struct JJPoles {
     Poles P;
     cd f1p, f1m, fj1, fj1b, f2p, f2m, fj2, fj2b;
+    // e1(t) ej1(t) = C1 for all t (C1 = e^{-beta r1} if Re r1 > 0, else e^{beta r1}); used when
+    // beta |Re r1| < 600, so that 1/e1 stays finite.  Same for the second pole.
+    cd C1, C2;
+    int rec1, rec2;
 };
 static_assert(sizeof(JJPoles) <= 48 * sizeof(double), "DevArrays::poles holds 48 doubles per point");
 
@@ -1158,6 +1162,11 @@ __global__ void k_jj_poles(const double* __restrict__ ek, const double* __restri
     JJPoles o;
     const Poles P = make_poles(e, mu, Sinfp[0], CJ1, T);
     o.P = P;
+    const double beta = 1.0 / T;
+    o.C1 = (P.r1.x > 0) ? cexpd(-(beta * P.r1)) : cexpd(beta * P.r1);
+    o.C2 = (P.r2.x > 0) ? cexpd(-(beta * P.r2)) : cexpd(beta * P.r2);
+    o.rec1 = fabs(P.r1.x) * beta < 600.0;
+    o.rec2 = fabs(P.r2.x) * beta < 600.0;
     if (P.r1.x > 0) {
         const cd ex = cexpd(-(P.r1 / T));
         o.f1p = P.a1 / P.den1;
@@ -1361,17 +1370,17 @@ __global__ void __launch_bounds__(NT, JJ_WAVES_PER_SIMD)
             cd e1, ej1, e2, ej2;
             if (P.r1.x > 0) {
                 e1 = cexpd(-t * P.r1);
-                ej1 = cexpd(P.r1 * (t - 1.0 / T));
+                ej1 = pp.rec1 ? pp.C1 / e1 : cexpd(P.r1 * (t - 1.0 / T));
             } else {
                 e1 = cexpd((1.0 / T - t) * P.r1);
-                ej1 = cexpd(P.r1 * t);
+                ej1 = pp.rec1 ? pp.C1 / e1 : cexpd(P.r1 * t);
             }
             if (P.r2.x > 0) {
                 e2 = cexpd(-t * P.r2);
-                ej2 = cexpd(P.r2 * (t - 1.0 / T));
+                ej2 = pp.rec2 ? pp.C2 / e2 : cexpd(P.r2 * (t - 1.0 / T));
             } else {
                 e2 = cexpd((1.0 / T - t) * P.r2);
-                ej2 = cexpd(P.r2 * t);
+                ej2 = pp.rec2 ? pp.C2 / e2 : cexpd(P.r2 * t);
             }
             const double G0t = (-(P.a1 * e1 / P.den1) - P.a2 * e2 / P.den2).x;
             const double Jinf = (P.b1 * ej1 / P.den1 + P.b2 * ej2 / P.den2).x;
