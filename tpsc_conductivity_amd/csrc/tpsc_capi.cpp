@@ -902,7 +902,7 @@ int tpsc_create(const tpsc_params* p, tpsc_ctx** out)
     rc |= dalloc(c, &d.bscr, 4 * std::max<long>(nq, 1));
     rc |= dalloc(c, &d.hT, (size_t)(Nw + 1) * std::max<long>(nq, 1), es);
     rc |= dalloc(c, &d.f0T, (size_t)(Nw + 1) * std::max<long>(nq, 1), es);
-    rc |= dalloc(c, &d.poles, 48 * (size_t)std::max<long>(nq, 1));
+    rc |= dalloc(c, &d.poles, 64 * (size_t)std::max<long>(nq, 1));
     const long nblk = (nq + kJJK - 1) / kJJK;
     rc |= dalloc(c, &d.jjpart, (size_t)std::max<long>(nblk, 1) * 2 * nwb);
     rc |= dalloc(c, &d.jjscal, (size_t)std::max<long>(nblk, 1) * 4);

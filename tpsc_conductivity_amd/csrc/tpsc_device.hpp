@@ -564,15 +564,19 @@ constexpr ZPow make_zpow()
 }
 __constant__ constexpr ZPow kZPow = make_zpow();
 
-// The same clamped spline as block_spline_moments, in place: yM holds y_0..y_N on entry and
-// the moments M_0..M_N on exit.  Each thread keeps its chunk of the right-hand side in
-// registers, so no scratch array is needed.  C: largest chunk, >= ceil((N+1)/NT).
-// trap (optional): each thread receives its part of sum_j h (y_j + y_{j+1})/2
-// - h^3 (M_j + M_{j+1})/24, the integral of the spline over [0, N h]; the caller adds the
-// parts (block_sum).  ES: LDS scratch of 2*NT + 8 doubles.
-template <int C, int NT>
-__device__ __forceinline__ void block_spline_inplace(double* yM, int N, double h, double d0, double dN, double* ES,
-                                                     double* trap)
+// The same clamped spline as block_spline_moments, in place, for K sequences of the same
+// length at once (their latencies overlap; one barrier sequence for all): yM[k] holds
+// y_0..y_N on entry and the moments M_0..M_N on exit.  Each thread keeps its chunk of the
+// right-hand side in registers, so no scratch array is needed.  The two first-order
+// recursions run once from a zero state to get the chunk-end states, and again after the
+// exchange, seeded with the carries of the neighbouring chunks.  C: largest chunk,
+// >= ceil((N+1)/NT).
+// trap (optional): trap[k] receives this thread's part of sum_j h (y_j + y_{j+1})/2
+// - h^3 (M_j + M_{j+1})/24, the integral of spline k over [0, N h]; the caller adds the parts
+// (block_sum).  ES: LDS scratch of K*(2*NT + 8) doubles.
+template <int C, int NT, int K>
+__device__ __forceinline__ void block_spline_inplace_k(double* const* yM, int N, double h, const double* d0,
+                                                       const double* dN, double* ES, double* trap)
 {
     const int tid = threadIdx.x;
     const double z = 1.7320508075688772935 - 2.0;
@@ -581,80 +585,87 @@ __device__ __forceinline__ void block_spline_inplace(double* yM, int N, double h
     const int Cr = (N + 1 + NT - 1) / NT;
     const int nchunk = (N + 1 + Cr - 1) / Cr;
     const int j0 = tid * Cr, j1 = min(N + 1, j0 + Cr);
-    double* bnd = ES + 2 * NT;  // r_0, r_N, M_0, M_1, M_{N-1}, M_N
-    double ri[C], u[C];
-    double eu = 0.0, ew = 0.0, tr = 0.0;
+    double* bnd = ES + 2 * K * NT;  // per sequence: r_0, r_N, M_0, M_1, M_{N-1}, M_N (8 slots)
+    double ri[K][C], u[K][C];
+    double tr[K];
 #pragma unroll
-    for (int c = 0; c < C; c++) {
-        const int j = j0 + c;
-        double r = 0.0;
-        if (j < j1) {
-            if (j == 0) {
-                r = 6.0 * ih * ((yM[1] - yM[0]) * ih - d0);
-                bnd[0] = r;
-                r = 0.0;
-            } else if (j == N) {
-                r = 6.0 * ih * (dN - (yM[N] - yM[N - 1]) * ih);
-                bnd[1] = r;
-                r = 0.0;
-            } else
-                r = 6.0 * ih * ih * (yM[j + 1] - 2.0 * yM[j] + yM[j - 1]);
-            if (trap && j < N) tr += h * (yM[j] + yM[j + 1]) * 0.5;
+    for (int k = 0; k < K; k++) {
+        const double* y = yM[k];
+        tr[k] = 0.0;
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+            const int j = j0 + c;
+            double r = 0.0;
+            if (j < j1) {
+                if (j == 0) {
+                    bnd[8 * k + 0] = 6.0 * ih * ((y[1] - y[0]) * ih - d0[k]);
+                } else if (j == N) {
+                    bnd[8 * k + 1] = 6.0 * ih * (dN[k] - (y[N] - y[N - 1]) * ih);
+                } else
+                    r = 6.0 * ih * ih * (y[j + 1] - 2.0 * y[j] + y[j - 1]);
+                if (trap && j < N) tr[k] += h * (y[j] + y[j + 1]) * 0.5;
+            }
+            ri[k][c] = r;
         }
-        ri[c] = r;
     }
     // chunk-end states of the two recursions from a zero start
-    if (j0 < j1) {
-        double sa = 0.0;
 #pragma unroll
-        for (int c = 0; c < C; c++)
-            if (j0 + c < j1) sa = ri[c] + z * sa;
-        eu = sa;
-        sa = 0.0;
+    for (int k = 0; k < K; k++) {
+        double eu = 0.0, ew = 0.0;
+        if (j0 < j1) {
+            double sa = 0.0;
 #pragma unroll
-        for (int c = C - 1; c >= 0; c--)
-            if (j0 + c < j1) sa = ri[c] + z * sa;
-        ew = sa;
+            for (int c = 0; c < C; c++)
+                if (j0 + c < j1) sa = ri[k][c] + z * sa;
+            eu = sa;
+            sa = 0.0;
+#pragma unroll
+            for (int c = C - 1; c >= 0; c--)
+                if (j0 + c < j1) sa = ri[k][c] + z * sa;
+            ew = sa;
+        }
+        ES[2 * k * NT + tid] = eu;
+        ES[2 * k * NT + NT + tid] = ew;
     }
-    ES[tid] = eu;
-    ES[NT + tid] = ew;
     __syncthreads();  // all reads of y are done
     if (j0 < j1) {
         double zc = 1.0;
         for (int c = 0; c < Cr; c++) zc *= z;
-        double Su = 0.0, Sw = 0.0, zd = 1.0;
-        for (int d = 1; d <= nchunk && fabs(zd) > 1.0e-19; d++) {
-            if (tid - d >= 0) Su += zd * ES[tid - d];
-            if (tid + d < nchunk) Sw += zd * ES[NT + tid + d];
-            zd *= zc;
-        }
-        // the recursions again, seeded with the carries from the neighbouring chunks
-        double sa = Su;
 #pragma unroll
-        for (int c = 0; c < C; c++) {
-            sa = ri[c] + z * sa;
-            u[c] = sa;
-        }
-        sa = Sw;
+        for (int k = 0; k < K; k++) {
+            double Su = 0.0, Sw = 0.0, zd = 1.0;
+            for (int d = 1; d <= nchunk && fabs(zd) > 1.0e-19; d++) {
+                if (tid - d >= 0) Su += zd * ES[2 * k * NT + tid - d];
+                if (tid + d < nchunk) Sw += zd * ES[2 * k * NT + NT + tid + d];
+                zd *= zc;
+            }
+            // the recursions again, seeded with the carries from the neighbouring chunks
+            double sa = Su;
 #pragma unroll
-        for (int c = C - 1; c >= 0; c--) {
-            const int j = j0 + c;
-            if (j < j1) {
-                sa = ri[c] + z * sa;
-                const double m = A * (u[c] + sa - ri[c]);
-                yM[j] = m;
-                u[c] = m;
-                // end moments for the boundary modes (read after the barrier, never rewritten)
-                if (j == 0) bnd[2] = m;
-                if (j == 1) bnd[3] = m;
-                if (j == N - 1) bnd[4] = m;
-                if (j == N) bnd[5] = m;
+            for (int c = 0; c < C; c++) {
+                sa = ri[k][c] + z * sa;
+                u[k][c] = sa;
+            }
+            sa = Sw;
+            double* y = yM[k];
+#pragma unroll
+            for (int c = C - 1; c >= 0; c--) {
+                const int j = j0 + c;
+                if (j < j1) {
+                    sa = ri[k][c] + z * sa;
+                    const double m = A * (u[k][c] + sa - ri[k][c]);
+                    y[j] = m;
+                    u[k][c] = m;
+                    // end moments for the boundary modes (read after the barrier, never rewritten)
+                    if (j == 0) bnd[8 * k + 2] = m;
+                    if (j == 1) bnd[8 * k + 3] = m;
+                    if (j == N - 1) bnd[8 * k + 4] = m;
+                    if (j == N) bnd[8 * k + 5] = m;
+                }
             }
         }
     }
     __syncthreads();
-    const double R0 = bnd[0] - 2.0 * bnd[2] - bnd[3];
-    const double RN = bnd[1] - bnd[4] - 2.0 * bnd[5];
     double zN1 = 1.0;  // z^(N-1)
     {
         double zz = z;
@@ -667,23 +678,38 @@ __device__ __forceinline__ void block_spline_inplace(double* yM, int N, double h
     }
     const double dg = 2.0 + z, off = zN1 * (1.0 + 2.0 * z);
     const double det = dg * dg - off * off;
-    const double alpha = (dg * R0 - off * RN) / det;
-    const double beta = (dg * RN - off * R0) / det;
-    double tm = 0.0;
 #pragma unroll
-    for (int c = 0; c < C; c++) {
-        const int j = j0 + c;
-        if (j < j1) {
-            double add = 0.0;
-            if (j <= 2 * KW) add += alpha * kZPow.v[j];
-            if (N - j <= 2 * KW) add += beta * kZPow.v[N - j];
-            const double m = u[c] + add;
-            if (add != 0.0) yM[j] = m;
-            if (trap) tm += (j == 0 || j == N) ? m : 2.0 * m;
+    for (int k = 0; k < K; k++) {
+        const double R0 = bnd[8 * k + 0] - 2.0 * bnd[8 * k + 2] - bnd[8 * k + 3];
+        const double RN = bnd[8 * k + 1] - bnd[8 * k + 4] - 2.0 * bnd[8 * k + 5];
+        const double alpha = (dg * R0 - off * RN) / det;
+        const double beta = (dg * RN - off * R0) / det;
+        double* y = yM[k];
+        double tm = 0.0;
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+            const int j = j0 + c;
+            if (j < j1) {
+                double add = 0.0;
+                if (j <= 2 * KW) add += alpha * kZPow.v[j];
+                if (N - j <= 2 * KW) add += beta * kZPow.v[N - j];
+                const double m = u[k][c] + add;
+                if (add != 0.0) y[j] = m;
+                if (trap) tm += (j == 0 || j == N) ? m : 2.0 * m;
+            }
         }
+        if (trap) trap[k] = tr[k] - h * h * h * tm / 24.0;
     }
-    if (trap) *trap = tr - h * h * h * tm / 24.0;
     __syncthreads();
+}
+
+// one sequence (see block_spline_inplace_k); ES: 2*NT + 8 doubles
+template <int C, int NT>
+__device__ __forceinline__ void block_spline_inplace(double* yM, int N, double h, double d0, double dN, double* ES,
+                                                     double* trap)
+{
+    double* const ys[1] = {yM};
+    block_spline_inplace_k<C, NT, 1>(ys, N, h, &d0, &dN, ES, trap);
 }
 
 // ---------------------------------------------------------------------------------

@@ -1146,8 +1146,9 @@ struct JJPoles {
     // beta |Re r1| < 600, so that 1/e1 stays finite.  Same for the second pole.
     cd C1, C2;
     int rec1, rec2;
+    cd a1d, a2d, b1d, b2d;  // a1/den1, a2/den2, b1/den1, b2/den2
 };
-static_assert(sizeof(JJPoles) <= 48 * sizeof(double), "DevArrays::poles holds 48 doubles per point");
+static_assert(sizeof(JJPoles) <= 64 * sizeof(double), "DevArrays::poles holds 64 doubles per point");
 
 __global__ void k_jj_poles(const double* __restrict__ ek, const double* __restrict__ hVb,
                            const double* __restrict__ Sinfp, int N, double T, double mu, JJPoles* out)
@@ -1167,6 +1168,10 @@ __global__ void k_jj_poles(const double* __restrict__ ek, const double* __restri
     o.C2 = (P.r2.x > 0) ? cexpd(-(beta * P.r2)) : cexpd(beta * P.r2);
     o.rec1 = fabs(P.r1.x) * beta < 600.0;
     o.rec2 = fabs(P.r2.x) * beta < 600.0;
+    o.a1d = P.a1 / P.den1;
+    o.a2d = P.a2 / P.den2;
+    o.b1d = P.b1 / P.den1;
+    o.b2d = P.b2 / P.den2;
     if (P.r1.x > 0) {
         const cd ex = cexpd(-(P.r1 / T));
         o.f1p = P.a1 / P.den1;
@@ -1261,7 +1266,7 @@ __global__ void __launch_bounds__(NT, JJ_WAVES_PER_SIMD)
     __shared__ __align__(16) double Yb[2 * (L + 1)];
     __shared__ cd X[L];
     __shared__ double sc[8 * (NT / 64)];
-    __shared__ double ES[2 * NT + 8];
+    __shared__ double ES[2 * (2 * NT + 8)];
     __shared__ double Jbeta_s;
     double* Y1 = Yb;
     double* Y2 = Yb + (L + 1);
@@ -1294,17 +1299,32 @@ __global__ void __launch_bounds__(NT, JJ_WAVES_PER_SIMD)
         const double wt = (m == 0 || m == nk - 1) ? 2.0 : 4.0;
         // (1) f0V and hV pencils, their splines, one packed FFT
         {
+            // all loads of the two rows first, then the LDS stores
             const S* f0q = f0T + q * (L + 1);
             const S* hq = hT + q * (L + 1);
-            for (int j = tid; j <= L; j += NT) {
-                Y1[j] = (j == 0) ? f0Vb[q] : (j == L) ? f0Vb[nq + q] : (double)f0q[j];
-                Y2[j] = (j == 0) ? hVb[q] : (j == L) ? hVb[nq + q] : (double)hq[j];
+            S fa[CS], ha[CS];
+#pragma unroll
+            for (int c = 0; c < CS; c++) {
+                const int j = tid + c * NT;
+                fa[c] = (j > 0 && j < L) ? f0q[j] : (S)0;
+                ha[c] = (j > 0 && j < L) ? hq[j] : (S)0;
+            }
+#pragma unroll
+            for (int c = 0; c < CS; c++) {
+                const int j = tid + c * NT;
+                if (j <= L) {
+                    Y1[j] = (j == 0) ? f0Vb[q] : (j == L) ? f0Vb[nq + q] : (double)fa[c];
+                    Y2[j] = (j == 0) ? hVb[q] : (j == L) ? hVb[nq + q] : (double)ha[c];
+                }
             }
         }
         const double CJ1 = -hVb[q] - hVb[nq + q];
         __syncthreads();
-        block_spline_inplace<CS, NT>(Y1, L, hstep, df0V[q], df0V[q + nq], ES, nullptr);
-        block_spline_inplace<CS, NT>(Y2, L, hstep, dhV[q], dhV[q + nq], ES, nullptr);
+        {
+            double* const ys[2] = {Y1, Y2};
+            const double d0s[2] = {df0V[q], dhV[q]}, dNs[2] = {df0V[q + nq], dhV[q + nq]};
+            block_spline_inplace_k<CS, NT, 2>(ys, L, hstep, d0s, dNs, ES, nullptr);
+        }
         for (int j = tid; j < L; j += NT) {
             const double a = (Y1[j + 1] - Y1[j]) / hstep, b = (Y2[j + 1] - Y2[j]) / hstep;
             X[j] = twX[j] * cd{a, b};
@@ -1382,8 +1402,8 @@ __global__ void __launch_bounds__(NT, JJ_WAVES_PER_SIMD)
                 e2 = cexpd((1.0 / T - t) * P.r2);
                 ej2 = pp.rec2 ? pp.C2 / e2 : cexpd(P.r2 * t);
             }
-            const double G0t = (-(P.a1 * e1 / P.den1) - P.a2 * e2 / P.den2).x;
-            const double Jinf = (P.b1 * ej1 / P.den1 + P.b2 * ej2 / P.den2).x;
+            const double G0t = (-(pp.a1d * e1) - pp.a2d * e2).x;
+            const double Jinf = (pp.b1d * ej1 + pp.b2d * ej2).x;
             const cd R = phT[j] * W[j];
             X[j] = cd{R.x + G0t, R.y + Jinf};
         }

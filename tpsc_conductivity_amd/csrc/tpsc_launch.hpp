@@ -47,7 +47,7 @@ struct DevArrays {
     double* bscr = nullptr;      // 4*nq   staging of the Lambda boundary kernel
     double* hT = nullptr;        // (Nw+1)*nq  hV in point-major layout [q][j]
     double* f0T = nullptr;       // (Nw+1)*nq  f0V in point-major layout
-    double* poles = nullptr;     // nq * 48    two-pole data per Lambda point (k_jj_poles)
+    double* poles = nullptr;     // nq * 64    two-pole data per Lambda point (k_jj_poles)
     double* jjpart = nullptr;    // nblk * 2*nwb  per-workgroup Lambda partials
     double* jjscal = nullptr;    // nblk * 4      per-workgroup k0, inf sums
     double* kx0part = nullptr;   // 2*nk
