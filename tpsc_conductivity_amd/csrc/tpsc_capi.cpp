@@ -54,6 +54,8 @@ struct tpsc_ctx {
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;  // boundary kernels, overlapped with the slab kernels
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_tau0 = nullptr, ev_hf0 = nullptr;
+    bool hf0_pending = false;  // h(r,tau), f0(r,tau) queued on the side stream at the chi0 stage
     DevArrays d{};
     double* chi_tau = nullptr;
     double* host_scalar = nullptr;  // pinned, mapped: kernels write root-finder values here
@@ -423,6 +425,8 @@ int set_grid(tpsc_ctx* c, int N, int Nw)
 }
 
 // ------------------------------------------------------------------ stages
+bool full_grid(const tpsc_ctx* c) { return c->g.N == c->p.nk0 && c->g.Nw == c->p.Nw0; }
+
 // chi0::calc_chiqw_FFT_spline, chi0.cpp:279-718
 int stage_chi0(tpsc_ctx* c, int nq0, int nw0)
 {
@@ -434,10 +438,22 @@ int stage_chi0(tpsc_ctx* c, int nq0, int nw0)
     prof_begin(c, "chi0_tau0", c->side);
     HIPCHK(c, launch_chi0_tau0(c->g, c->d, c->mu0, c->side));
     prof_end(c);
+    HIPCHK(c, hipEventRecord(c->ev_tau0, c->side));
+    // h(r,tau) and f0(r,tau) of the Lambda stage need only mu0 and eps(k): on the full grid
+    // they are queued now, behind the tau = 0 slice, and run during the chi0 pencils and the
+    // sum-rule passes; the Lambda stage waits for ev_hf0
+    c->hf0_pending = false;
+    if (full_grid(c) && c->g.Nw <= 2048) {
+        prof_begin(c, "jj_hf0_slab", c->side);
+        HIPCHK(c, launch_hf0_slab(c->g, c->d, c->mu0, c->side));
+        prof_end(c);
+        HIPCHK(c, hipEventRecord(c->ev_hf0, c->side));
+        c->hf0_pending = true;
+    }
     prof_begin(c, "chi0_slab");
     HIPCHK(c, launch_chi0_slab(c->g, c->d, c->mu0, s));
     prof_end(c);
-    if ((rc = join_side(c))) return rc;
+    HIPCHK(c, hipStreamWaitEvent(s, c->ev_tau0, 0));
     if (c->keep_chi_tau && c->chi_tau)
         HIPCHK(c, hipMemcpyAsync(c->chi_tau, c->d.chi, elem_bytes(c) * c->g.nwb * c->g.nk8, hipMemcpyDeviceToDevice,
                                  s));
@@ -564,7 +580,6 @@ int stage_UspUch(tpsc_ctx* c)
     return 0;
 }
 
-bool full_grid(const tpsc_ctx* c) { return c->g.N == c->p.nk0 && c->g.Nw == c->p.Nw0; }
 
 // chi::calc_chirt, chi.cpp:37-206
 int stage_vertex(tpsc_ctx* c)
@@ -707,9 +722,14 @@ int stage_chijj(tpsc_ctx* c)
     prof_begin(c, "jj_poles", c->side);
     HIPCHK(c, launch_jj_poles(g, c->d, c->mu, c->side));
     prof_end(c);
-    prof_begin(c, "jj_hf0_slab");
-    HIPCHK(c, launch_hf0_slab(g, c->d, c->mu0, s));
-    prof_end(c);
+    if (c->hf0_pending) {
+        HIPCHK(c, hipStreamWaitEvent(s, c->ev_hf0, 0));
+        c->hf0_pending = false;
+    } else {
+        prof_begin(c, "jj_hf0_slab");
+        HIPCHK(c, launch_hf0_slab(g, c->d, c->mu0, s));
+        prof_end(c);
+    }
     prof_begin(c, "jj_hVf0V_slab");
     HIPCHK(c, launch_hVf0V_slab(g, c->d, s));
     prof_end(c);
@@ -856,6 +876,8 @@ int tpsc_create(const tpsc_params* p, tpsc_ctx** out)
     HIPCHK(c, hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
     HIPCHK(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
     HIPCHK(c, hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+    HIPCHK(c, hipEventCreateWithFlags(&c->ev_tau0, hipEventDisableTiming));
+    HIPCHK(c, hipEventCreateWithFlags(&c->ev_hf0, hipEventDisableTiming));
     HIPCHK(c, hipHostMalloc((void**)&c->host_scalar, 16 * sizeof(double), hipHostMallocMapped));
     HIPCHK(c, hipHostGetDevicePointer((void**)&c->dev_scalar, c->host_scalar, 0));
     const int N = p->nk0, Nw = p->Nw0, nk = N / 2 + 1;
@@ -935,6 +957,8 @@ void tpsc_destroy(tpsc_ctx* c)
     if (c->host_scalar) (void)hipHostFree(c->host_scalar);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->ev_tau0) (void)hipEventDestroy(c->ev_tau0);
+    if (c->ev_hf0) (void)hipEventDestroy(c->ev_hf0);
     if (c->side) (void)hipStreamDestroy(c->side);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
