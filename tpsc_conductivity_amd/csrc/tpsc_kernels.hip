@@ -527,9 +527,9 @@ __global__ void __launch_bounds__(PEN_THREADS)
 // ================================================================== self-energy
 __device__ __forceinline__ cd sigma_mom(double Sinf, double s2, double s3, cd R, double wn)
 {
-    // Sinf/z + s2/z^2 + s3/z^3 + R/z^4, z = i wn (green.cpp:3039)
-    const double w2 = wn * wn, w3 = w2 * wn, w4 = w2 * w2;
-    return cd{-s2 / w2 + R.x / w4, -Sinf / wn + s3 / w3 + R.y / w4};
+    // Sinf/z + s2/z^2 + s3/z^3 + R/z^4, z = i wn (green.cpp:3039), with one reciprocal of wn
+    const double iw = 1.0 / wn, iw2 = iw * iw;
+    return cd{(-s2 + R.x * iw2) * iw2, (-Sinf + (s3 + R.y * iw) * iw2) * iw};
 }
 
 // Sigma(k, i w_n), w_n > 0, at offset o of the stored form: the remainder R (FP64 path,
@@ -1275,7 +1275,7 @@ __global__ void __launch_bounds__(NT, JJ_WAVES_PER_SIMD)
     const long nq = (long)nk * (nk - 2), nk8 = (long)nk * (nk + 1) / 2;
     const double nf = 1.0 / ((double)N * N);
     const double Sinf = Sinfp[0];
-    const double hstep = 1.0 / (L * T);
+    const double hstep = 1.0 / (L * T), iT = 1.0 / T;
     const int tid = threadIdx.x;
     const int blk = xcd_remap(blockIdx.x, gridDim.x);
     const long nblk = gridDim.x;
@@ -1345,18 +1345,20 @@ __global__ void __launch_bounds__(NT, JJ_WAVES_PER_SIMD)
                 const cd Af = cd{0.5 * (Fj.x + Fc.x), 0.5 * (Fj.y + Fc.y)};
                 const cd Bh = cd{0.5 * (Fj.y - Fc.y), -0.5 * (Fj.x - Fc.x)};
                 const double knj = (jx < H) ? (2 * jx + 1) * kPi * T : (2 * (jx - L) + 1) * kPi * T;
-                const double k2 = knj * knj, k3 = k2 * knj, k4 = k2 * k2;
+                // -i c1/k - c2/k^2 + i c3/k^3 + a/k^4 with one reciprocal of k (cond_opt.cpp:3302-3311)
+                const double ik = 1.0 / knj, ik2 = ik * ik;
                 const cd ts = twS[jx];
                 const cd af = ts * Af, ah = ts * Bh;
-                const cd J0 = cd{-cj02 / k2 + af.x / k4, -cj01 / knj + cj03 / k3 + af.y / k4};
-                const cd Jv = cd{-cj2 / k2 + ah.x / k4, -CJ1 / knj + cj3 / k3 + ah.y / k4};
+                const cd J0 = cd{(-cj02 + af.x * ik2) * ik2, (-cj01 + (cj03 + af.y * ik) * ik2) * ik};
+                const cd Jv = cd{(-cj2 + ah.x * ik2) * ik2, (-CJ1 + (cj3 + ah.y * ik) * ik2) * ik};
                 double kn;
                 const cd Gv = dyson_G(Self, ki, nk8, s, H, C2, C3, Sinf, e, mu, T, kn);
-                const cd t = Gv * Gv * J0 / T;
+                const cd t = (Gv * Gv * J0) * iT;
                 r6[0] += t.x;
                 r6[1] += t.y;
-                const cd G2 = T / cd{-e + mu, kn + Sinf / kn};
-                const cd Jn = Gv * Jv - G2 * cd{0.0, -CJ1 / kn};
+                const double ikn = 1.0 / kn;
+                const cd G2 = T / cd{-e + mu, kn + Sinf * ikn};
+                const cd Jn = Gv * Jv - G2 * cd{0.0, -CJ1 * ikn};
                 const cd Gn = Gv - G2;
                 r6[2] += Gn.x;
                 r6[3] += Gn.y;
@@ -1438,19 +1440,19 @@ __global__ void __launch_bounds__(NT, JJ_WAVES_PER_SIMD)
 #pragma unroll 1
         for (int n = tid; n <= H; n += NT) {
             const cd Fn = X[n], Fc = conjg(X[(L - n) & (L - 1)]);
-            const double wn = 2 * n * kPi * T;
             double bub;
             if (n == 0) {
                 bub = 2 * integ[0];
                 c2[0] -= wv * qn0;
             } else {
-                const double w4 = wn * wn * wn * wn;
+                const double iw = 1.0 / (2 * n * kPi * T), iw2 = iw * iw;
                 const cd tm = twM[n];
                 const cd a = tm * cd{0.5 * (Fn.x + Fc.x), 0.5 * (Fn.y + Fc.y)};
-                const double imJ = -2.0 * MJ1 / wn + 2.0 * MJ3 / (wn * wn * wn) + 2.0 * (a.y / w4);
-                c2[n] -= wv * imJ / wn;
+                // -2 MJ1/wn + 2 MJ3/wn^3 + 2 a/wn^4, then / wn
+                const double imJ = (-2.0 * MJ1 + (2.0 * MJ3 + 2.0 * a.y * iw) * iw2) * iw;
+                c2[n] -= wv * imJ * iw;
                 const cd b = tm * cd{0.5 * (Fn.y - Fc.y), -0.5 * (Fn.x - Fc.x)};
-                bub = -2.0 * dchi / (wn * wn) + b.x / w4;
+                bub = (-2.0 * dchi + b.x * iw2) * iw2;
             }
             c1[n] -= 2.0 * wv * v * bub;
         }
