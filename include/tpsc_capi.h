@@ -155,6 +155,22 @@ int tpsc_keep_chi_tau(tpsc_ctx* ctx, int on);
 /* device properties useful to callers (HBM bytes of the context's buffers) */
 int tpsc_device_bytes(tpsc_ctx* ctx, double* bytes);
 
+/* ---- analytic continuation (host, FP64; needs no device) ---------------------------------
+ * generique::pade_cont_frac_coef_rec (generique.h:112, generique.cpp:1650-1663): coefficients
+ * of the Thiele continued-fraction Pade approximant through (z[i], f[i]), i < n.  z, f, coef:
+ * n complex values, interleaved re/im. */
+int tpsc_pade_coef(const double* z, const double* f, int n, double* coef);
+/* generique::pade (generique.cpp:1630-1648): the approximant at x[k], k < nx, into out
+ * (complex, interleaved). */
+int tpsc_pade_eval(const double* z, const double* coef, int n, const double* x, int nx, double* out);
+/* Optical conductivity sigma(w_k) = Im Lambda(w_k + i eta) / w_k, Lambda continued by the
+ * Pade approximant through Lambda(iq_n) = lam[n], q_n = 2 pi n T, n < npade (the bosonic
+ * continuation of chi.cpp:1055-1093 applied to Lambda_jj; the reference itself stops at
+ * Lambda_jj(iq_n)).  w_k = 0 is evaluated at w = 1e-6.  lam: Lambda_jj1 + Lambda_jj2 from
+ * tpsc_get. */
+int tpsc_pade_conductivity(const double* lam, int nlam, double T, int npade, const double* omega, int nw,
+                           double eta, double* sigma);
+
 #ifdef __cplusplus
 }
 #endif

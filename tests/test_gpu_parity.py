@@ -213,3 +213,28 @@ def test_headline_config_parity_and_invariants(oracle_mod):
     c1, c2 = g.get("chijj1"), g.get("chijj2")
     assert np.all(c1 >= 0)
     assert abs((c1[0] + c2[0] + g.scalar("kx0")) / g.scalar("kx0")) < 1e-3
+
+
+def test_pade_conductivity_from_the_solve(oracle_mod):
+    """sigma(w) from the HIP Lambda_jj(iq_n) equals the oracle's own Pade restatement applied to
+    the same values (to rounding), and agrees with the continuation of the oracle's Lambda_jj.
+    The second bound is loose on purpose: a 16-point Pade approximant amplifies its inputs'
+    1e-10 differences about 1e7-fold near w = 0 (measured 1.6e-3 at the DC point, <1e-5 above
+    w = 0.5)."""
+    from tpsc_conductivity_amd import capi
+
+    r = run_pair("dop_16x128", oracle_mod)
+    o, g, c = r["o"], r["g"], r["cfg"]
+    w = np.linspace(0.0, 8.0, 41)
+    eta = 0.05
+    sg = g.conductivity_pade(w, eta=eta, npade=16)
+    lam_o = o.get("chijj1") + o.get("chijj2")
+    so = capi.pade_conductivity(lam_o, c["T"], 16, w, eta)
+    assert relerr(sg, so) < 1e-2
+    assert relerr(sg[w > 0.5], so[w > 0.5]) < 1e-4
+    lam_g = g.get("chijj1") + g.get("chijj2")
+    n = np.arange(16)
+    z = 1j * (2.0 * n * np.pi * c["T"])  # the library's node arithmetic, ((2n) pi) T
+    w_eff = np.where(w == 0.0, 1e-6, w)
+    ref = oracle_mod.pade(lam_g[:16].astype(np.complex128), z, w_eff + 1j * eta).imag / w_eff
+    assert relerr(sg, ref) < 1e-11

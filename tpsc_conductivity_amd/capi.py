@@ -51,7 +51,7 @@ EXPORTS = [
     "tpsc_calc_self_energy_full", "tpsc_trace_self_G", "tpsc_trace_diff_G", "tpsc_find_mu",
     "tpsc_chijj", "tpsc_solve", "tpsc_get", "tpsc_field_size", "tpsc_get_scalar",
     "tpsc_prof_enable", "tpsc_prof_reset", "tpsc_prof_count", "tpsc_prof_get",
-    "tpsc_keep_chi_tau", "tpsc_device_bytes",
+    "tpsc_keep_chi_tau", "tpsc_device_bytes", "tpsc_pade_coef", "tpsc_pade_eval", "tpsc_pade_conductivity",
 ]
 
 
@@ -92,6 +92,9 @@ def lib():
     L.tpsc_prof_get.argtypes = [vp, C.c_int, C.POINTER(C.c_char_p), dp, ip]
     L.tpsc_keep_chi_tau.argtypes = [vp, C.c_int]
     L.tpsc_device_bytes.argtypes = [vp, dp]
+    L.tpsc_pade_coef.argtypes = [dp, dp, C.c_int, dp]
+    L.tpsc_pade_eval.argtypes = [dp, dp, C.c_int, dp, C.c_int, dp]
+    L.tpsc_pade_conductivity.argtypes = [dp, C.c_int, C.c_double, C.c_int, dp, C.c_int, C.c_double, dp]
     _lib = L
     return L
 
@@ -158,3 +161,45 @@ class Context:
         if name in ("Self", "Sigma_kw"):
             return out.view(np.complex128)
         return out
+
+
+# ---- analytic continuation (host FP64, no device needed) ---------------------------------
+def _cplx(a):
+    a = np.ascontiguousarray(a, dtype=np.complex128)
+    return a, a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def pade_coef(z, f):
+    """Thiele continued-fraction coefficients through (z_i, f_i) (generique.h:112)."""
+    z, zp = _cplx(z)
+    f, fp = _cplx(f)
+    coef = np.zeros(len(z), dtype=np.complex128)
+    rc = lib().tpsc_pade_coef(zp, fp, len(z), coef.ctypes.data_as(C.POINTER(C.c_double)))
+    if rc:
+        raise TPSCError(f"tpsc_pade_coef failed ({ERRORS.get(rc, rc)})")
+    return coef
+
+
+def pade_eval(z, coef, x):
+    """The approximant of pade_coef at the points x (generique.cpp:1630-1648)."""
+    z, zp = _cplx(z)
+    coef, cp = _cplx(coef)
+    x, xp = _cplx(np.atleast_1d(x))
+    out = np.zeros(len(x), dtype=np.complex128)
+    rc = lib().tpsc_pade_eval(zp, cp, len(z), xp, len(x), out.ctypes.data_as(C.POINTER(C.c_double)))
+    if rc:
+        raise TPSCError(f"tpsc_pade_eval failed ({ERRORS.get(rc, rc)})")
+    return out
+
+
+def pade_conductivity(lam, T, npade, omega, eta):
+    """sigma(w) = Im Lambda(w + i eta)/w, Lambda continued from Lambda(iq_n) = lam[n], n < npade."""
+    lam = np.ascontiguousarray(lam, dtype=np.float64)
+    omega = np.ascontiguousarray(np.atleast_1d(omega), dtype=np.float64)
+    sig = np.zeros(len(omega))
+    dpp = C.POINTER(C.c_double)
+    rc = lib().tpsc_pade_conductivity(lam.ctypes.data_as(dpp), len(lam), T, npade, omega.ctypes.data_as(dpp),
+                                      len(omega), eta, sig.ctypes.data_as(dpp))
+    if rc:
+        raise TPSCError(f"tpsc_pade_conductivity failed ({ERRORS.get(rc, rc)})")
+    return sig

@@ -7,7 +7,7 @@ reference reports on stdout (a root not found, a stage called too early) raise T
 """
 import ctypes as C
 
-from .capi import PRECISIONS, Context, TPSCError, lib
+from .capi import PRECISIONS, Context, TPSCError, lib, pade_conductivity
 
 __all__ = ["CondOpt", "TPSCError"]
 
@@ -105,6 +105,14 @@ class CondOpt:
 
     def scalar(self, name):
         return self.ctx.scalar(name)
+
+    def conductivity_pade(self, omega, eta=None, npade=None):
+        """sigma(w) = Im Lambda_jj(w + i eta)/w by the bosonic Pade continuation of
+        Lambda_jj1 + Lambda_jj2 (chi.cpp:1055-1093 applied to Lambda_jj); eta defaults to the
+        constructor's eta, npade to all Nw/2 + 1 Matsubara points."""
+        lam = self.get("chijj1") + self.get("chijj2")
+        return pade_conductivity(lam, self.tem, len(lam) if npade is None else npade, omega,
+                                 self.eta if eta is None else eta)
 
     def keep_chi_tau(self, on=True):
         self.ctx.check(lib().tpsc_keep_chi_tau(self.ctx.handle, int(on)), "keep_chi_tau")
