@@ -75,16 +75,22 @@ enum tpsc_field {
     TPSC_F_CHIJJ2 = 14,   /* Maki-Thompson Lambda_jj2(iqn), nwb         (cond_opt.cpp:3790-3807) */
     TPSC_F_DTAU_SIGMA = 15, /* 2*nk8                                    (green.cpp:1743-1817)   */
     TPSC_F_CHI0_TAU = 16, /* chi0(q,tau) before the tau->iqn transform (kept only after tpsc_chi0) */
-    TPSC_F_SIGMA_KW = 17  /* Sigma(k,iwn), n = 0..Nw/2-1, (Nw/2)*nk8 complex interleaved re/im:
+    TPSC_F_SIGMA_KW = 17, /* Sigma(k,iwn), n = 0..Nw/2-1, (Nw/2)*nk8 complex interleaved re/im:
                              Sigma_inf/z + C2/z^2 + C3/z^3 + R/z^4, z = iwn (green.cpp:3039);
                              the only Sigma(k,iwn) field of TPSC_PREC_MIXED                  */
+    TPSC_F_CHIJJ3 = 18,   /* Aslamazov-Larkin Lambda_jj3 on the sparse boson set, al_N values,
+                             entry 0 = 0                                (cond_opt.cpp:6396-6398) */
+    TPSC_F_CHIJJ_ALL = 19, /* Lambda_jj1 + Lambda_jj2 + Lambda_jj3 on that set, al_N values
+                                                                        (cond_opt.cpp:6460-6467) */
+    TPSC_F_AL_FREQ = 20   /* the sparse boson indices n (as doubles), al_N values
+                                                                        (cond_opt.cpp:5123-5156) */
 };
 enum tpsc_scalar {
     TPSC_S_MU0 = 1, TPSC_S_CHI0MAX, TPSC_S_QMAX_L, TPSC_S_QMAX_M, TPSC_S_USP, TPSC_S_UCH,
     TPSC_S_DBLOCC, TPSC_S_DIFREL_SP, TPSC_S_DIFREL_CH, TPSC_S_SIGMA_INF, TPSC_S_TRSG,
     TPSC_S_TRSG_CORR, TPSC_S_MU, TPSC_S_TRSG2, TPSC_S_DBLOCC2, TPSC_S_KX0, TPSC_S_CHIJJ1_INF2,
     TPSC_S_CHIJJ2_INF2, TPSC_S_CHIJJ2_INF4, TPSC_S_NEVAL_SP, TPSC_S_NEVAL_CH, TPSC_S_NEVAL_MU,
-    TPSC_S_NK, TPSC_S_NK8, TPSC_S_NWB, TPSC_S_NQ
+    TPSC_S_NK, TPSC_S_NK8, TPSC_S_NWB, TPSC_S_NQ, TPSC_S_AL_N
 };
 
 /* constructor / destructor (cond_opt.cpp:10-62 and the base-class constructors). Builds
@@ -128,8 +134,10 @@ int tpsc_trace_diff_G(tpsc_ctx* ctx, double mu, double* value);
 int tpsc_find_mu(tpsc_ctx* ctx, double* mu);
 
 /* cond_opt::calc_chijj_vertex_corr_all_optim(chi3, mmin, N_theta_k, k_chijj, nk_chijj),
- * cond_opt.cpp:2537-4290.  chi3 must be 0 (Aslamazov-Larkin not in this build) and
- * N_theta_k 1.  Runs the missing upstream stages first, as cond_opt.cpp:2550-2560 does. */
+ * cond_opt.cpp:2537-6467.  Bubble + Maki-Thompson + kx0; with chi3 != 0 also the
+ * Aslamazov-Larkin term on the sparse boson set of parameter mmin >= 0 (TPSC_F_CHIJJ3,
+ * TPSC_F_CHIJJ_ALL, TPSC_F_AL_FREQ).  N_theta_k must be 1 (the reference driver's value,
+ * tpsc_main.cpp:243).  Runs the missing upstream stages first, as cond_opt.cpp:2550-2560 does. */
 int tpsc_chijj(tpsc_ctx* ctx, int chi3, int mmin, int N_theta_k);
 
 /* one full solve: constructor state -> calc_Self_FFT_spline -> traceSelfG -> find_mu ->

@@ -54,6 +54,8 @@ class Orc(C.Structure):
         ("chijj1_inf2", C.c_double), ("chijj2_inf2", C.c_double), ("chijj2_inf4", C.c_double),
         ("N_theta", C.c_int),
         ("chijj1_k", _dp), ("chijj2_k", _dp),
+        ("al_N", C.c_int), ("al_N_theta", C.c_int), ("al_freq", C.POINTER(C.c_long)),
+        ("chijj3", _dp), ("chijj_all", _dp), ("chijj3_k", _dp), ("al_hw", _dp), ("al_hg2", _dp),
     ]
 
 
@@ -79,6 +81,10 @@ def lib():
         L.orc_chi0.restype = C.c_int
         L.orc_chijj.argtypes = [P, C.c_int]
         L.orc_chijj.restype = C.c_int
+        L.orc_chijj_al.argtypes = [P, C.c_int, C.c_int]
+        L.orc_chijj_al.restype = C.c_int
+        L.orc_al_freqs.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_long)]
+        L.orc_al_freqs.restype = C.c_int
         L.orc_sum_rule.argtypes = [P, C.c_int, C.c_double]
         L.orc_sum_rule.restype = C.c_double
         L.orc_find_Usp_Uch_grid.argtypes = [P, C.c_int, C.c_int, _dp]
@@ -103,6 +109,9 @@ def lib():
         L.orc_dft.argtypes = [_dp, C.c_int, C.c_int]
         L.orc_dispk.argtypes = [P, C.c_double, C.c_double]
         L.orc_dispk.restype = C.c_double
+        L.orc_grad_ek_x.argtypes = [P, C.c_double, C.c_double]
+        L.orc_grad_ek_x.restype = C.c_double
+        L.orc_al_tail_out.argtypes = [C.c_double] * 5 + [C.c_int, _dp]
         _lib = L
     return _lib
 
@@ -187,6 +196,14 @@ class Oracle:
     def calc_chijj(self, N_theta=1):
         return lib().orc_chijj(self._o, N_theta)
 
+    def calc_chijj_al(self, mmin, N_theta=1):
+        """Aslamazov-Larkin Lambda_jj3 on the sparse boson set (cond_opt.cpp:4292-6467)."""
+        return lib().orc_chijj_al(self._o, mmin, N_theta)
+
+    def al_freqs(self):
+        s = self.s
+        return np.array([s.al_freq[i] for i in range(s.al_N)], dtype=np.int64)
+
     def solve(self):
         return lib().orc_solve(self._o)
 
@@ -197,6 +214,12 @@ class Oracle:
         corr = C.c_double(0.0)
         tr = lib().orc_trace_chi0(self._o, C.byref(corr))
         return tr, corr.value
+
+    def dispersion(self, kx, ky):
+        return lib().orc_dispk(self._o, kx, ky)
+
+    def velocity_x(self, kx, ky):
+        return lib().orc_grad_ek_x(self._o, kx, ky)
 
     def self_kw_brute(self, kx, ky, n):
         r = (C.c_double * 2)()
@@ -213,6 +236,8 @@ class Oracle:
             "chi0_inf2": nk8, "V": nwb * nk8, "Sigma_tau": (s.nbw + 1) * nk8, "dtau_Sigma": 2 * nk8,
             "Sigma_inf2": nk8, "Sigma_inf3": nk8, "Sigma_inf4": nk8, "chijj1": nwb, "chijj2": nwb,
             "chijj1_k": nwb * s.N_theta, "chijj2_k": nwb * s.N_theta,
+            "chijj3": s.al_N, "chijj_all": s.al_N, "chijj3_k": s.al_N * s.al_N_theta,
+            "al_hw": self.nq * (s.nbw // 2), "al_hg2": self.nq,
         }
         if name == "Self":
             return _arr(s.Self, 2 * nwb * nk8, np.complex128)
@@ -245,6 +270,20 @@ def dft(a, sign):
     a = np.ascontiguousarray(a, dtype=np.complex128).copy()
     lib().orc_dft(a.ctypes.data_as(_dp), a.size, sign)
     return a
+
+
+def al_freqs(nbw, mmin):
+    n = lib().orc_al_freqs(nbw, mmin, None)
+    out = (C.c_long * n)()
+    lib().orc_al_freqs(nbw, mmin, out)
+    return np.array(list(out), dtype=np.int64)
+
+
+def al_tail(Usp, Uch, chi0inf, T, tau, kind):
+    """Imaginary-time two-pole tail of the AL vertex (oracle orc_al_tail)."""
+    out = (C.c_double * 2)()
+    lib().orc_al_tail_out(Usp, Uch, chi0inf, T, tau, kind, out)
+    return complex(out[0], out[1])
 
 
 def spline_coeffs_rel(x, v, d0, dN):

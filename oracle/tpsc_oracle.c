@@ -197,6 +197,12 @@ void orc_destroy(orc *o)
     free(o->chijj2);
     free(o->chijj1_k);
     free(o->chijj2_k);
+    free(o->al_freq);
+    free(o->chijj3);
+    free(o->chijj_all);
+    free(o->chijj3_k);
+    free(o->al_hw);
+    free(o->al_hg2);
     free(o);
 }
 
@@ -1411,4 +1417,509 @@ void orc_self_kw_brute_out(orc *o, int kx, int ky, int n, double out[2])
     const double complex s = orc_self_kw_brute(o, kx, ky, n);
     out[0] = creal(s);
     out[1] = cimag(s);
+}
+
+/* ================================================================ a11: Aslamazov-Larkin */
+/* Sparse boson set of the AL term (cond_opt.cpp:5123-5156): n = 0 .. (N0-1)/2^m, then m
+   groups of (N0-1)/2^(m+1) frequencies with steps 2, 4, ..., 2^m; N0 = Nw/2 + 1.
+   Returns the count; writes the indices to out when out != NULL. */
+int orc_al_freqs(int nbw, int mmin, long *out)
+{
+    const int N0 = nbw / 2 + 1;
+    const long pw = 1L << mmin;
+    const int jf0 = (int)((N0 - 1) / pw);
+    const int Nj = (int)((N0 - 1) / (2 * pw));
+    const int N = mmin * Nj + jf0 + 1;
+    if (out) {
+        for (int j = 0; j <= jf0; j++) out[j] = j;
+        long n = jf0, d = 1;
+        int jf = jf0;
+        for (int l = 0; l < mmin; l++) {
+            const int j0 = jf + 1;
+            jf = j0 + Nj - 1;
+            d *= 2;
+            for (int j = j0; j <= jf; j++) {
+                n += d;
+                out[j] = n;
+            }
+        }
+    }
+    return N;
+}
+
+/* chi0(q,iq_p) with the 1/qn^2 + 1/qn^4 tail beyond the grid (cond_opt.cpp:5383-5392) */
+static inline double al_chi0(const orc *o, long r, long p, long nk8, int H, double T)
+{
+    if (p < 0) p = -p;
+    if (p <= H) return o->chiqw[r + p * nk8];
+    const double wn = 2 * PI * p * T;
+    return o->chi0_inf[r] / (wn * wn) + o->chi0_inf2[r] / (wn * wn * wn * wn);
+}
+/* hG(q,iq_p) from its transform samples and 1/qn^2 coefficient (cond_opt.cpp:5396-5401) */
+static inline double al_hG(const double *hw, const double *hg2, long q, long p, long nq, int H, double T)
+{
+    if (p < 0) p = -p;
+    if (p == 0) return hw[q];
+    const double wn = 2 * PI * p * T;
+    const double c4 = (p < H) ? hw[q + p * nq] : hw[q + (long)(H - 1) * nq];
+    return -hg2[q] / (wn * wn) + c4 / (wn * wn * wn * wn);
+}
+/* Sigma(k,iw_f) from the moments and the remainder, conjugated for f < 0
+   (cond_opt.cpp:6314-6377) */
+static inline double complex al_sigma(const orc *o, long ki, long f, long nk8, int H)
+{
+    const double kn = (2 * f + 1) * PI * o->p.T;
+    const double C2 = o->Sigma_inf2[ki], C3 = o->Sigma_inf3[ki];
+    double re = -C2 / (kn * kn), im = -o->Sigma_inf / kn + C3 / (kn * kn * kn);
+    const long g = (f >= 0) ? f : -f - 1;
+    if (g < H) {
+        const double complex R = o->Self[ki + g * nk8];
+        re += creal(R) / (kn * kn * kn * kn);
+        im += ((f >= 0) ? cimag(R) : -cimag(R)) / (kn * kn * kn * kn);
+    }
+    return re + I * im;
+}
+
+/* Imaginary-time form of the two-pole tail Hinf(iq_p)/hg2 = 3 Usp/(-w^2 + a^2) + Uch/(-w^2 - c^2),
+   a^2 = Usp chi0_inf/2, c^2 = Uch chi0_inf/2, i.e. T sum_p e^{-i w_p tau} Hinf(iq_p)/hg2, in the
+   pole-sum form of cond_opt.cpp:5556-5618 (poles zsp = +-i a, zch = +-c).  kind 0: value at
+   0 <= tau < beta (:5562-5575); 1: value at tau = beta (:5577-5590); 2: slope at tau = 0+
+   (:5592-5604); 3: slope at tau = beta- (:5606-5618). */
+double complex orc_al_tail(double Usp, double Uch, double chi0inf, double T, double tau, int kind)
+{
+    const double rt = sqrt(Usp * chi0inf / 2.0);
+    const double complex zs0 = I * rt, zs1 = -zs0;
+    const double zc0 = sqrt(Uch * chi0inf / 2.0), zc1 = -zc0;
+    const double b = 1.0 / T;
+    double complex tsp;
+    double tch;
+    if (kind == 0) {
+        tsp = cexp(-zs0 * tau) / ((cexp(-zs0 * b) - 1.0) * (zs0 - zs1)) + cexp(-zs1 * tau) / ((cexp(-zs1 * b) - 1.0) * (zs1 - zs0));
+        tch = exp(-zc0 * tau) / ((exp(-zc0 * b) - 1.0) * (zc0 - zc1)) + exp(zc1 * (b - tau)) / ((1.0 - exp(zc1 * b)) * (zc1 - zc0));
+    } else if (kind == 1) {
+        const double complex e0 = cexp(zs0 * b), e1 = cexp(zs1 * b);
+        tsp = 1.0 / ((1.0 - e0) * (zs0 - zs1)) + 1.0 / ((1.0 - e1) * (zs1 - zs0));
+        const double ec = exp(-zc0 * b);
+        tch = ec / ((ec - 1.0) * (zc0 - zc1)) + 1.0 / ((1.0 - exp(zc1 * b)) * (zc1 - zc0));
+    } else if (kind == 2) {
+        tsp = -zs0 / ((cexp(-zs0 * b) - 1.0) * (zs0 - zs1)) - zs1 / ((cexp(-zs1 * b) - 1.0) * (zs1 - zs0));
+        const double e1 = exp(zc1 * b);
+        tch = -zc0 / ((exp(-zc0 * b) - 1.0) * (zc0 - zc1)) - zc1 * e1 / ((1.0 - e1) * (zc1 - zc0));
+    } else {
+        const double complex e0 = cexp(-zs0 * b), e1 = cexp(-zs1 * b);
+        tsp = -zs0 * e0 / ((e0 - 1.0) * (zs0 - zs1)) - zs1 * e1 / ((e1 - 1.0) * (zs1 - zs0));
+        const double ec = exp(-zc0 * b);
+        tch = -zc0 * ec / ((ec - 1.0) * (zc0 - zc1)) - zc1 / ((1.0 - exp(zc1 * b)) * (zc1 - zc0));
+    }
+    return 3.0 * Usp * tsp + Uch * tch;
+}
+void orc_al_tail_out(double Usp, double Uch, double chi0inf, double T, double tau, int kind, double out[2])
+{
+    const double complex v = orc_al_tail(Usp, Uch, chi0inf, T, tau, kind);
+    out[0] = creal(v);
+    out[1] = cimag(v);
+}
+
+/* spline + shifted fermionic transform of a complex tau-pencil z_j, j = 0..nbw, with
+   boundary slopes d0, dN (cond_opt.cpp:5907-5995 for HG, :6172-6245 for HG2):
+   out[j] = value at w_(f(j)+shift), f(j) = j (j < nbw/2 - nc) or j - nbw */
+static void al_k_pencil(const double complex *z, double complex d0, double complex dN, int nbw, double T,
+                        long shift, int nc, double *tau, double *v, double *cf, double complex *ft, sws *ws,
+                        double complex *out)
+{
+    const int H = nbw / 2;
+    double complex d2S0, d2Sb;
+    for (int part = 0; part < 2; part++) {
+        for (int j = 0; j <= nbw; j++) {
+            tau[j] = j / (nbw * T);
+            v[j] = part ? cimag(z[j]) : creal(z[j]);
+        }
+        cf[0] = part ? cimag(d0) : creal(d0);
+        cf[1] = part ? cimag(dN) : creal(dN);
+        orc_spline_coeffs_rel_ws(tau, v, nbw + 1, cf, ws->B, ws->r);
+        const double s0 = 2 * cf[1];
+        const double sb = 6.0 * cf[4 * nbw - 4] / (nbw * T) + 2.0 * cf[4 * nbw - 3];
+        if (!part) {
+            d2S0 = s0;
+            d2Sb = sb;
+            for (int j = 0; j < nbw; j++) ft[j] = cexp(-I * PI * j / nbw) * (6.0 * cf[4 * j]);
+        } else {
+            d2S0 += I * s0;
+            d2Sb += I * sb;
+            for (int j = 0; j < nbw; j++) ft[j] += I * cexp(-I * PI * j / nbw) * (6.0 * cf[4 * j]);
+        }
+    }
+    orc_dft(ft, nbw, -1);
+    const double complex cw = z[0] + z[nbw], ds = d0 + dN, d2 = d2S0 + d2Sb;
+    for (int j = 0; j < nbw; j++) {
+        const long f = (j < H - nc) ? j : j - nbw;
+        const long s = f + shift;
+        const double wn = (2 * s + 1) * PI * T;
+        double complex val = -I * cw / wn - ds / (wn * wn) + I * d2 / (wn * wn * wn);
+        if (s < nbw) {
+            const long fi = (s < 0) ? s + nbw : s;
+            val += (1.0 - cexp(-I * (2 * (j + shift) + 1) * PI / nbw)) * ft[fi] / (wn * wn * wn * wn);
+        }
+        out[j] = val;
+    }
+}
+
+/* cond_opt.cpp:4292-6467 with chi3 = true: the Aslamazov-Larkin term Lambda_jj3 on the
+   sparse boson set, after orc_chijj.  Per-k contributions are summed in k order. */
+int orc_chijj_al(orc *o, int mmin, int N_theta)
+{
+    if (!o->chijj1 || !o->Self || !o->chiqw || mmin < 0 || mmin > 24 || N_theta < 1) return -1;
+    const int nbk = o->nbk, nbw = o->nbw, H = nbw / 2, nk0 = 2 * (nbk - 1);
+    const long nq = (long)nbk * (nbk - 2), nk8 = (long)nbk * (nbk + 1) / 2;
+    const double T = o->p.T, mu0 = o->mu0, mu = o->mu, U = o->p.U, Usp = o->Usp, Uch = o->Uch;
+    const double nf = 1.0 / ((double)nk0 * nk0);
+    const double Dtheta = PI / (2.0 * N_theta);
+    const int Ns = orc_al_freqs(nbw, mmin, NULL);
+
+    free(o->al_freq);
+    free(o->chijj3);
+    free(o->chijj_all);
+    free(o->chijj3_k);
+    free(o->al_hw);
+    free(o->al_hg2);
+    o->al_freq = (long *)xmalloc(sizeof(long) * (size_t)Ns);
+    orc_al_freqs(nbw, mmin, o->al_freq);
+    o->al_N = Ns;
+    o->al_N_theta = N_theta;
+    o->chijj3 = dcalloc((size_t)Ns);
+    o->chijj_all = dcalloc((size_t)Ns);
+    o->chijj3_k = dcalloc((size_t)Ns * N_theta);
+
+    double *dek = dalloc((size_t)nq);
+    for (int l = 0; l < nbk - 2; l++)
+        for (int m = 0; m < nbk; m++) dek[m + l * nbk] = orc_grad_ek_x(o, (l + 1) * PI / (nbk - 1), m * PI / (nbk - 1));
+
+    /* h(r,tau) (cond_opt.cpp:4302-4330) and its slopes dh (:2745-2781) */
+    double *h = dalloc((size_t)nq * (nbw + 1));
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j <= nbw; j++) {
+        const double t = (double)j / (T * nbw);
+        double *fh = h + (size_t)j * nq;
+        for (int l = 0; l < nbk - 2; l++)
+            for (int m = 0; m < nbk; m++) {
+                const double e = o->ek[m + (l + 1) * nbk], v = dek[m + l * nbk];
+                if (e > mu0)
+                    fh[m + l * nbk] = nf * v * exp((mu0 - e) * t) / (exp((mu0 - e) / T) + 1.0);
+                else
+                    fh[m + l * nbk] = nf * v * exp((e - mu0) * (1.0 / T - t)) / (exp((e - mu0) / T) + 1.0);
+            }
+        orc_r2r_2d(fh, nbk - 2, nbk, 1, 0);
+    }
+    double *dh = dalloc(2 * (size_t)nq);
+    for (int l = 0; l < nbk - 2; l++)
+        for (int m = 0; m < nbk; m++) {
+            const long q = m + l * nbk;
+            const double e = o->ek[m + (l + 1) * nbk], v = dek[q];
+            if (e > mu0) {
+                const double ex = exp((mu0 - e) / T);
+                dh[q] = -nf * v * (e - mu0) / (ex + 1.0);
+                dh[q + nq] = -nf * v * (e - mu0) * ex / (ex + 1.0);
+            } else {
+                const double ex = exp((e - mu0) / T);
+                dh[q] = -nf * v * (e - mu0) * ex / (ex + 1.0);
+                dh[q + nq] = -nf * v * (e - mu0) / (ex + 1.0);
+            }
+        }
+    orc_r2r_2d(dh, nbk - 2, nbk, 1, 0);
+    orc_r2r_2d(dh + nq, nbk - 2, nbk, 1, 0);
+
+    /* G0(r,-tau) on the wedge (cond_opt.cpp:4374-4413) and its slopes (:4415-4445) */
+    double *Grt = dalloc((size_t)nk8 * (nbw + 1)), *dGrt = dalloc(2 * (size_t)nk8);
+#pragma omp parallel
+    {
+        double *sq = dalloc((size_t)nbk * nbk);
+#pragma omp for schedule(static)
+        for (int j = 0; j <= nbw; j++) {
+            const double t = (double)j / (T * nbw);
+            for (int l = 0; l < nbk; l++)
+                for (int m = 0; m <= l; m++) {
+                    const double x = o->ek[m + l * nbk] - mu0;
+                    const double g = (x < 0) ? nf * exp(x * t) / (exp(x / T) + 1.0)
+                                             : nf * exp(x * (t - 1.0 / T)) / (exp(-x / T) + 1.0);
+                    sq[m + l * nbk] = g;
+                    sq[l + m * nbk] = g;
+                }
+            orc_r2r_2d(sq, nbk, nbk, 0, 0);
+            for (int l = 0; l < nbk; l++)
+                for (int m = 0; m <= l; m++) Grt[widx(l, m) + (size_t)j * nk8] = sq[m + l * nbk];
+        }
+        free(sq);
+    }
+    {
+        double *sq = dalloc((size_t)nbk * nbk);
+        for (int l = 0; l < nbk; l++)
+            for (int m = 0; m <= l; m++) {
+                const double x = o->ek[m + l * nbk] - mu0;
+                const double g = (x < 0) ? nf * x / (exp(x / T) + 1.0) : nf * x * exp(-x / T) / (exp(-x / T) + 1.0);
+                sq[m + l * nbk] = g;
+                sq[l + m * nbk] = g;
+            }
+        orc_r2r_2d(sq, nbk, nbk, 0, 0);
+        for (int l = 0; l < nbk; l++)
+            for (int m = 0; m <= l; m++) dGrt[widx(l, m)] = sq[m + l * nbk];
+        for (long w = 0; w < nk8; w++) dGrt[nk8 + w] = ((w == 0) ? -mu0 : o->er[w]) - dGrt[w];
+        free(sq);
+    }
+
+    /* slopes of h(r,tau) G(r,-tau) -> q (cond_opt.cpp:5159-5229), then hG(q,tau) (:5231-5261) */
+    double *dhG = dalloc(2 * (size_t)nq);
+    for (int side = 0; side < 2; side++) {
+        const int jj = side ? nbw : 0;
+        double *d = dhG + side * nq;
+        for (int l = 0; l < nbk - 2; l++)
+            for (int m = 0; m < nbk; m++) {
+                const long q = m + l * nbk, r = fold_idx(l + 1, m, nk0);
+                d[q] = dh[q + side * nq] * Grt[r + (size_t)jj * nk8] + h[q + (size_t)jj * nq] * dGrt[r + side * nk8];
+            }
+        orc_r2r_2d(d, nbk - 2, nbk, 1, 0);
+    }
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j <= nbw; j++) {
+        double *fh = h + (size_t)j * nq;
+        for (int l = 0; l < nbk - 2; l++)
+            for (int m = 0; m < nbk; m++) fh[m + l * nbk] *= Grt[fold_idx(l + 1, m, nk0) + (size_t)j * nk8];
+        orc_r2r_2d(fh, nbk - 2, nbk, 1, 0);
+    }
+
+    /* hG(q,iq_p), p < nbw/2, and its 1/qn^2 coefficient hg2 (cond_opt.cpp:5266-5330) */
+    double *hw = dalloc((size_t)nq * H), *hg2 = dalloc((size_t)nq);
+#pragma omp parallel
+    {
+        double *tau = dalloc((size_t)nbw + 1), *v = dalloc((size_t)nbw + 1), *cf = dalloc(4 * (size_t)nbw);
+        double complex *ft = (double complex *)xmalloc(sizeof(double complex) * (size_t)nbw);
+        sws ws = sws_new(nbw + 1);
+#pragma omp for schedule(static)
+        for (long q = 0; q < nq; q++) {
+            for (int j = 0; j <= nbw; j++) {
+                tau[j] = j / (nbw * T);
+                v[j] = h[q + (size_t)j * nq];
+            }
+            cf[0] = dhG[q];
+            cf[1] = dhG[q + nq];
+            orc_spline_coeffs_rel_ws(tau, v, nbw + 1, cf, ws.B, ws.r);
+            for (int j = 0; j < nbw; j++) ft[j] = 6.0 * cf[4 * j];
+            orc_dft(ft, nbw, -1);
+            double integ = 0;
+            for (int j = 0; j < nbw; j++) {
+                const double a = cf[4 * j], b = cf[4 * j + 1], c = cf[4 * j + 2], d = cf[4 * j + 3];
+                const double x1 = tau[j + 1] - tau[j];
+                integ += a * x1 * x1 * x1 * x1 / 4.0 + b * x1 * x1 * x1 / 3.0 + c * x1 * x1 / 2.0 + d * x1;
+            }
+            hw[q] = 2 * integ;
+            hg2[q] = 2 * (dhG[q] - dhG[q + nq]);
+            for (int j = 1; j < H; j++) hw[q + (size_t)j * nq] = 2 * creal((1.0 - cexp(-I * 2.0 * j * PI / nbw)) * ft[j]);
+        }
+        free(tau);
+        free(v);
+        free(cf);
+        free(ft);
+        sws_free(&ws);
+    }
+    o->al_hw = hw;
+    o->al_hg2 = hg2;
+    free(dh);
+
+    /* per boson frequency of the sparse set (cond_opt.cpp:5350-6400) */
+    const size_t tsz = (size_t)nq * (nbw + 1);
+    double complex *HG = (double complex *)xmalloc(sizeof(double complex) * tsz);
+    double complex *HR = (double complex *)xmalloc(sizeof(double complex) * tsz);  /* HG(r,tau) */
+    double complex *HK = (double complex *)xmalloc(sizeof(double complex) * tsz);  /* products, k space */
+    double complex *HK2 = (double complex *)xmalloc(sizeof(double complex) * tsz);
+    double complex *dHn = (double complex *)xmalloc(sizeof(double complex) * 2 * nq);
+    double complex *dHr = (double complex *)xmalloc(sizeof(double complex) * 2 * nq);
+    double complex *dK = (double complex *)xmalloc(sizeof(double complex) * 2 * nq);
+    double complex *dK2 = (double complex *)xmalloc(sizeof(double complex) * 2 * nq);
+    double complex *contrib = (double complex *)xmalloc(sizeof(double complex) * nq);
+    for (int n = 1; n < Ns; n++) {
+        const long nn = o->al_freq[n];
+        const double qn = 2.0 * nn * PI * T;
+        const int nc = (int)(nn / 2);
+        /* (a) HG(q,iq_p) on the window p = j (j < nbw/2 - nc), j - nbw (:5350-5490) and (b) its
+           transform to tau with the two-pole tails (:5492-5620) */
+#pragma omp parallel
+        {
+            double complex *ft = (double complex *)xmalloc(sizeof(double complex) * (size_t)nbw);
+            double *Dv = dalloc((size_t)nbw);
+#pragma omp for schedule(static)
+            for (long q = 0; q < nq; q++) {
+                const int l = (int)(q / nbk), m = (int)(q % nbk);
+                const long r = fold_idx(l + 1, m, nk0);
+                double complex *Hq = HG + q;  /* stride nq */
+                for (int j = 0; j < nbw; j++) {
+                    const long p1 = (j < H - nc) ? j : j - nbw, p2 = p1 + nn;
+                    const double c1 = al_chi0(o, r, p1, nk8, H, T), c2 = al_chi0(o, r, p2, nk8, H, T);
+                    const double g1 = al_hG(hw, hg2, q, p1, nq, H, T), g2 = al_hG(hw, hg2, q, p2, nq, H, T);
+                    Dv[j] = (g1 - g2) / (c1 - c2);
+                }
+                for (int j = 0; j < nbw; j++) {
+                    const long p1 = (j < H - nc) ? j : j - nbw;
+                    const double c1 = al_chi0(o, r, p1, nk8, H, T);
+                    const double chisp = c1 / (1.0 - Usp * c1 / 2), chich = c1 / (1.0 + Uch * c1 / 2);
+                    const double G = 3 * Usp * chisp + Uch * chich;
+                    if (nn % 2 == 0 && p1 == -nn / 2)
+                        Hq[(size_t)j * nq] = G * (4 * Dv[j - 1] - Dv[j - 2]) / 3.0;
+                    else
+                        Hq[(size_t)j * nq] = G * Dv[j];
+                }
+                /* (b) */
+                const double cq = -o->chi0_inf[r];
+                double complex dtauH0 = 0;
+                for (int j = 0; j < nbw; j++) {
+                    const long p1 = (j < H - nc) ? j : j - nbw;
+                    const double wn = 2 * p1 * PI * T;
+                    const double csp = 1.0 / (-wn * wn - Usp * cq / 2.0), cch = 1.0 / (-wn * wn + Uch * cq / 2.0);
+                    const double Hinf = hg2[q] * (3.0 * Usp * csp + Uch * cch);
+                    ft[j] = T * (Hq[(size_t)j * nq] - Hinf);
+                    dtauH0 -= wn * ft[j];
+                }
+                orc_dft(ft, nbw, -1);
+                const double ci = o->chi0_inf[r], g2q = hg2[q];
+                for (int j = 0; j < nbw; j++)
+                    Hq[(size_t)j * nq] = ft[j] + g2q * orc_al_tail(Usp, Uch, ci, T, j / (nbw * T), 0);
+                Hq[(size_t)nbw * nq] = ft[0] + g2q * orc_al_tail(Usp, Uch, ci, T, 1.0 / T, 1);
+                dHn[q] = I * dtauH0 + g2q * orc_al_tail(Usp, Uch, ci, T, 0.0, 2);
+                dHn[q + nq] = I * dtauH0 + g2q * orc_al_tail(Usp, Uch, ci, T, 1.0 / T, 3);
+            }
+            free(ft);
+            free(Dv);
+        }
+        /* (c) q -> r of every tau slice and of the slopes (:5622-5720) */
+#pragma omp parallel
+        {
+            double *sr = dalloc((size_t)nq), *si = dalloc((size_t)nq);
+#pragma omp for schedule(static)
+            for (int j = 0; j <= nbw + 2; j++) {
+                const double complex *src = (j <= nbw) ? HG + (size_t)j * nq : dHn + (size_t)(j - nbw - 1) * nq;
+                double complex *dst = (j <= nbw) ? HR + (size_t)j * nq : dHr + (size_t)(j - nbw - 1) * nq;
+                for (long q = 0; q < nq; q++) {
+                    sr[q] = nf * creal(src[q]);
+                    si[q] = nf * cimag(src[q]);
+                }
+                orc_r2r_2d(sr, nbk - 2, nbk, 1, 0);
+                orc_r2r_2d(si, nbk - 2, nbk, 1, 0);
+                for (long q = 0; q < nq; q++) dst[q] = sr[q] + I * si[q];
+            }
+            free(sr);
+            free(si);
+        }
+        /* (d)-(f) slopes of HG(r,tau) G(r,-tau) and of HG2(r,tau) G(r,-tau), HG2(r,tau) =
+           HG(r,beta-tau), then the products back to k (:5722-5905, :6000-6170) */
+#pragma omp parallel
+        {
+            double *sr = dalloc((size_t)nq), *si = dalloc((size_t)nq);
+#pragma omp for schedule(static)
+            for (int j = 0; j <= 2 * nbw + 1 + 4; j++) {
+                const double complex *A, *B;
+                const double *ga, *gb;
+                double complex *dst;
+                double sgn = 1.0;
+                if (j <= nbw) { /* HG(r,tau_j) G(r,-tau_j) */
+                    A = HR + (size_t)j * nq; ga = Grt + (size_t)j * nk8; B = NULL; gb = NULL;
+                    dst = HK + (size_t)j * nq;
+                } else if (j <= 2 * nbw + 1) { /* HG2(r,tau_i) G(r,-tau_i) */
+                    const int i = j - nbw - 1;
+                    A = HR + (size_t)(nbw - i) * nq; ga = Grt + (size_t)i * nk8; B = NULL; gb = NULL;
+                    dst = HK2 + (size_t)i * nq;
+                } else { /* slopes: c = 0: HG side 0, 1: HG side beta, 2: HG2 side 0, 3: HG2 side beta */
+                    const int c = j - 2 * nbw - 2, side = c & 1;
+                    const int which = c >> 1;
+                    const int jt = which ? (side ? 0 : nbw) : (side ? nbw : 0);  /* HG time index */
+                    A = HR + (size_t)jt * nq;
+                    ga = dGrt + (size_t)side * nk8;
+                    B = which ? dHr + (size_t)(1 - side) * nq : dHr + (size_t)side * nq;
+                    sgn = which ? -1.0 : 1.0;
+                    gb = Grt + (size_t)(side ? nbw : 0) * nk8;
+                    dst = (which ? dK2 : dK) + (size_t)side * nq;
+                }
+                for (int l = 0; l < nbk - 2; l++)
+                    for (int m = 0; m < nbk; m++) {
+                        const long q = m + l * nbk, r = fold_idx(l + 1, m, nk0);
+                        double vr = creal(A[q]) * ga[r], vi = cimag(A[q]) * ga[r];
+                        if (B) {
+                            vr += sgn * creal(B[q]) * gb[r];
+                            vi += sgn * cimag(B[q]) * gb[r];
+                        }
+                        sr[q] = vr;
+                        si[q] = vi;
+                    }
+                orc_r2r_2d(sr, nbk - 2, nbk, 1, 0);
+                orc_r2r_2d(si, nbk - 2, nbk, 1, 0);
+                for (long q = 0; q < nq; q++) dst[q] = sr[q] + I * si[q];
+            }
+            free(sr);
+            free(si);
+        }
+        /* (g) per k: HG(k, iw_(f+n)), HG2(k, iw_f) by splines and (h) the sum over the
+           fermionic window with G(k,iw_f) G(k,iw_(f+n)) (:5907-6400) */
+#pragma omp parallel
+        {
+            double *tau = dalloc((size_t)nbw + 1), *v = dalloc((size_t)nbw + 1), *cf = dalloc(4 * (size_t)nbw);
+            double complex *ft = (double complex *)xmalloc(sizeof(double complex) * (size_t)nbw);
+            double complex *z = (double complex *)xmalloc(sizeof(double complex) * ((size_t)nbw + 1));
+            double complex *A = (double complex *)xmalloc(sizeof(double complex) * (size_t)nbw);
+            double complex *B = (double complex *)xmalloc(sizeof(double complex) * (size_t)nbw);
+            sws ws = sws_new(nbw + 1);
+#pragma omp for schedule(static)
+            for (long q = 0; q < nq; q++) {
+                const int l = (int)(q / nbk), m = (int)(q % nbk);
+                const long ki = fold_idx(l + 1, m, nk0);
+                for (int j = 0; j <= nbw; j++) z[j] = HK[q + (size_t)j * nq];
+                al_k_pencil(z, dK[q], dK[q + nq], nbw, T, nn, nc, tau, v, cf, ft, &ws, A);
+                for (int j = 0; j <= nbw; j++) z[j] = HK2[q + (size_t)j * nq];
+                al_k_pencil(z, dK2[q], dK2[q + nq], nbw, T, 0, nc, tau, v, cf, ft, &ws, B);
+                const double e = o->ek[m + (l + 1) * nbk];
+                const double wt = (m == 0 || m == nbk - 1) ? 2.0 : 4.0;
+                double complex acc = 0;
+                for (int j = 0; j < nbw; j++) {
+                    const long f = (j < H - nc) ? j : j - nbw;
+                    const double k1 = (2 * f + 1) * PI * T, k2 = (2 * (f + nn) + 1) * PI * T;
+                    const double complex G1 = 1.0 / (I * k1 - e + mu - al_sigma(o, ki, f, nk8, H));
+                    const double complex G2 = 1.0 / (I * k2 - e + mu - al_sigma(o, ki, f + nn, nk8, H));
+                    acc -= I * U * T * nf * wt * dek[q] * G1 * G2 * (A[j] - B[j]) / (2 * qn);
+                }
+                contrib[q] = acc;
+            }
+            free(tau);
+            free(v);
+            free(cf);
+            free(ft);
+            free(z);
+            free(A);
+            free(B);
+            sws_free(&ws);
+        }
+        double complex s = 0;
+        for (long q = 0; q < nq; q++) {
+            s += contrib[q];
+            const int l = (int)(q / nbk), m = (int)(q % nbk);
+            const double th = atan((1.0 * (nk0 / 2 - m)) / (nk0 / 2 - l - 1));
+            const int it = (int)floor(th / Dtheta);
+            if (it >= 0 && it < N_theta) o->chijj3_k[it + (size_t)n * N_theta] += creal(contrib[q]);
+        }
+        o->chijj3[n] = creal(s);
+    }
+    /* Lambda_jj1 + Lambda_jj2 + Lambda_jj3 on the sparse set (:6460-6467) */
+    o->chijj_all[0] = o->chijj1[0] + o->chijj2[0];
+    for (int j = 1; j < Ns; j++) o->chijj_all[j] = o->chijj1[o->al_freq[j]] + o->chijj2[o->al_freq[j]] + o->chijj3[j];
+
+    free(HG);
+    free(HR);
+    free(HK);
+    free(HK2);
+    free(dHn);
+    free(dHr);
+    free(dK);
+    free(dK2);
+    free(contrib);
+    free(h);
+    free(dhG);
+    free(Grt);
+    free(dGrt);
+    free(dek);
+    return 0;
 }

@@ -7,7 +7,8 @@
  *
  * It restates, in plain C99 + OpenMP, the algorithm of dbergeron1/TPSC_conductivity for
  * the chain  eps(k) -> mu0 -> chi0(q,iqn) -> (Usp,Uch) -> V(r,tau) -> Sigma(k,iwn) -> mu
- * -> Lambda_jj(iqn) (bubble + Maki-Thompson + diamagnetic kx0).  Every function cites the
+ * -> Lambda_jj(iqn) (bubble + Maki-Thompson + diamagnetic kx0), and the Aslamazov-Larkin
+ * term Lambda_jj3 on its sparse boson set.  Every function cites the
  * reference file:line it follows.  The FFTW r2r/dft calls of the reference are replaced
  * by direct-definition transforms (REDFT00 / RODFT00 by cosine/sine matrices, complex DFT
  * by a radix-2 FFT), LAPACK dgbsv_ by a restated banded LU with partial pivoting.
@@ -68,6 +69,12 @@ typedef struct orc {
     double chijj1_inf2, chijj2_inf2, chijj2_inf4;
     int N_theta;
     double *chijj1_k, *chijj2_k;    /* (nwb)*N_theta angle-binned contributions */
+    /* Aslamazov-Larkin term (orc_chijj_al) */
+    int al_N, al_N_theta;
+    long *al_freq;                  /* al_N sparse boson indices                   */
+    double *chijj3, *chijj_all;     /* al_N each: Lambda_jj3, Lambda_jj1+2+3       */
+    double *chijj3_k;               /* al_N*al_N_theta angle-binned Lambda_jj3     */
+    double *al_hw, *al_hg2;         /* hG(q,iq_p) p < Nw/2 ([q + p*nq]), 1/qn^2 coefficient */
 } orc;
 
 /* ---- lifecycle / model (hamiltonien.cpp:14-34,49-68; green0.cpp:13-37) ---- */
@@ -93,6 +100,12 @@ int orc_find_mu(orc *o);                                /* green.cpp:3185-3282 *
 int orc_trace_self_G2(orc *o);                          /* green.cpp:2929-2997 */
 int orc_chijj(orc *o, int N_theta);                     /* cond_opt.cpp:2537-4290 */
 int orc_solve(orc *o);                                  /* tpsc_main.cpp:61-75 + E4 */
+int orc_al_freqs(int nbw, int mmin, long *out);         /* cond_opt.cpp:5123-5156 */
+int orc_chijj_al(orc *o, int mmin, int N_theta);        /* cond_opt.cpp:4292-6467 (chi3) */
+double complex orc_al_tail(double Usp, double Uch, double chi0inf, double T, double tau,
+                           int kind);                   /* cond_opt.cpp:5556-5618 */
+void orc_al_tail_out(double Usp, double Uch, double chi0inf, double T, double tau, int kind,
+                     double out[2]);
 
 /* ---- self-checks of the reference, as oracle functions ---- */
 double orc_chiqw_discret(orc *o, int qx, int qy, int n);   /* chi0.cpp:1588-1651 */

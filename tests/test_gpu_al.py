@@ -1,0 +1,92 @@
+"""GPU parity of the Aslamazov-Larkin term Lambda_jj3(iq_n) (cond_opt.cpp:4292-6467) against
+the FP64 oracle, through the C ABI.
+
+Tolerance: normwise relative 1e-10 on Lambda_jj3 over the sparse boson set and on the combined
+Lambda_jj1 + Lambda_jj2 + Lambda_jj3; the index set itself must match exactly.  The mixed path
+(FP32 tau arrays upstream, FP64 AL arithmetic) must agree with the FP64 path to 1e-6.
+"""
+import numpy as np
+import pytest
+
+from conftest import relerr
+
+pytestmark = pytest.mark.gpu
+
+CONFIGS = {
+    "al_8x64": dict(N=8, Nw=64, n=0.9, T=0.25, U=2.5, tp=-0.1, tpp=0.0, t3=0.0, coarse=True, mmin=2),
+    "al_16x128": dict(N=16, Nw=128, n=0.875, T=0.1, U=4.0, tp=-0.175, tpp=0.0, t3=0.0, coarse=True, mmin=2),
+    "al_32x256": dict(N=32, Nw=256, n=1.1, T=0.15, U=3.0, tp=-0.2, tpp=0.1, t3=0.05, coarse=False, mmin=3),
+}
+TIGHT = 1e-10
+MIXED = 1e-6
+
+_cache = {}
+
+
+def run_pair(name, oracle_mod):
+    if name in _cache:
+        return _cache[name]
+    from tpsc_conductivity_amd.tpsc import CondOpt
+
+    c = CONFIGS[name]
+    print(f"[al parity] {name}: oracle + HIP", flush=True)
+    o = oracle_mod.Oracle(c["N"], c["Nw"], c["n"], c["T"], c["U"], tp=c["tp"], tpp=c["tpp"], t3=c["t3"],
+                          coarse_pass=c["coarse"])
+    assert o.solve()
+    assert o.calc_chijj_al(c["mmin"], 1) == 0
+    g = CondOpt(c["N"], c["Nw"], c["n"], c["T"], params=(c["tp"], c["tpp"], c["t3"], c["U"]),
+                coarse_pass=c["coarse"])
+    g.solve()
+    g.calc_chijj_vertex_corr_all_optim(chi3=True, mmin=c["mmin"], N_theta_k=1)
+    _cache[name] = (o, g)
+    return o, g
+
+
+@pytest.mark.parametrize("name", list(CONFIGS))
+def test_al_frequency_set(name, oracle_mod):
+    o, g = run_pair(name, oracle_mod)
+    f = g.get("al_freq")
+    assert np.array_equal(f.astype(np.int64), o.al_freqs())
+    assert int(g.scalar("al_N")) == len(f)
+
+
+@pytest.mark.parametrize("name", list(CONFIGS))
+def test_al_lambda3(name, oracle_mod):
+    o, g = run_pair(name, oracle_mod)
+    ref, got = o.get("chijj3"), g.get("chijj3")
+    assert got[0] == 0.0
+    e3, ea = relerr(got[1:], ref[1:]), relerr(g.get("chijj_all"), o.get("chijj_all"))
+    print(f"[al parity] {name}: Lambda_jj3 rel err {e3:.2e}, Lambda_jj total {ea:.2e}", flush=True)
+    assert e3 <= TIGHT
+    assert ea <= TIGHT
+
+
+def test_al_mixed_matches_fp64():
+    from tpsc_conductivity_amd.tpsc import CondOpt
+
+    c = CONFIGS["al_16x128"]
+    out = {}
+    for prec in ("fp64", "mixed"):
+        print(f"[al mixed] al_16x128 {prec}", flush=True)
+        g = CondOpt(c["N"], c["Nw"], c["n"], c["T"], params=(c["tp"], c["tpp"], c["t3"], c["U"]),
+                    coarse_pass=c["coarse"], precision=prec)
+        g.solve()
+        g.calc_chijj_vertex_corr_all_optim(chi3=True, mmin=c["mmin"], N_theta_k=1)
+        out[prec] = g.get("chijj3")[1:]
+        g.close()
+    e = relerr(out["mixed"], out["fp64"])
+    print(f"[al mixed] Lambda_jj3 mixed vs fp64 rel err {e:.2e}", flush=True)
+    assert e <= MIXED
+
+
+def test_al_bad_mmin():
+    from tpsc_conductivity_amd.capi import TPSCError
+    from tpsc_conductivity_amd.tpsc import CondOpt
+
+    g = CondOpt(8, 32, 0.9, 0.25, params=(-0.1, 0.0, 0.0, 2.5))
+    g.solve()
+    with pytest.raises(TPSCError):
+        g.calc_chijj_vertex_corr_all_optim(chi3=True, mmin=-1, N_theta_k=1)
+    with pytest.raises(TPSCError):
+        g.get("chijj3")
+    g.close()

@@ -18,6 +18,7 @@ FIELDS = {
     "ek": 1, "chi0": 2, "chirt": 3, "chi0_inf": 4, "chi0_inf2": 5, "dtau_chirt0": 6, "V": 7,
     "Sigma_tau": 8, "Self": 9, "Sigma_inf2": 10, "Sigma_inf3": 11, "Sigma_inf4": 12,
     "chijj1": 13, "chijj2": 14, "dtau_Sigma": 15, "chi0_tau": 16, "Sigma_kw": 17,
+    "chijj3": 18, "chijj_all": 19, "al_freq": 20,
 }
 PREC_FP64 = 0
 PREC_MIXED = 1
@@ -27,7 +28,7 @@ SCALARS = {
     "difRelsp": 8, "difRelch": 9, "Sigma_inf": 10, "trSG": 11, "trSG_corr": 12, "mu": 13,
     "trSG2": 14, "dblOcc2": 15, "kx0": 16, "chijj1_inf2": 17, "chijj2_inf2": 18,
     "chijj2_inf4": 19, "nevals_sp": 20, "nevals_ch": 21, "nevals_mu": 22, "nk": 23, "nk8": 24,
-    "nwb": 25, "nq": 26,
+    "nwb": 25, "nq": 26, "al_N": 27,
 }
 
 

@@ -46,6 +46,8 @@ struct tpsc_ctx {
     bool mu_ok = false;
     int nev_sp = 0, nev_ch = 0, nev_mu = 0;
     std::vector<double> chijj1, chijj2;
+    std::vector<long> al_freq;                 // Aslamazov-Larkin sparse boson set
+    std::vector<double> chijj3, chijj_all;     // Lambda_jj3 and Lambda_jj1+2+3 on it
     double kx0 = 0.0, jj_inf1 = 0.0, jj_inf2 = 0.0, jj_inf4 = 0.0;
     // stage state
     bool have_chi0 = false, have_U = false, have_V = false, have_sigma = false, have_jj = false;
@@ -57,6 +59,7 @@ struct tpsc_ctx {
     hipEvent_t ev_tau0 = nullptr, ev_hf0 = nullptr;
     bool hf0_pending = false;  // h(r,tau), f0(r,tau) queued on the side stream at the chi0 stage
     DevArrays d{};
+    ALArrays al{};
     double* chi_tau = nullptr;
     double* host_scalar = nullptr;  // pinned, mapped: kernels write root-finder values here
     double* dev_scalar = nullptr;   // device alias of host_scalar
@@ -762,6 +765,91 @@ int stage_chijj(tpsc_ctx* c)
     return 0;
 }
 
+// sparse boson set of the Aslamazov-Larkin term (cond_opt.cpp:5123-5156): n = 0 ..
+// (N0-1)/2^m, then m groups of (N0-1)/2^(m+1) indices with steps 2, 4, ..., 2^m; N0 = Nw/2+1
+std::vector<long> al_freqs(int Nw, int mmin)
+{
+    const long N0 = Nw / 2 + 1, pw = 1L << mmin;
+    const long jf0 = (N0 - 1) / pw, Nj = (N0 - 1) / (2 * pw);
+    std::vector<long> f;
+    for (long j = 0; j <= jf0; j++) f.push_back(j);
+    long n = jf0, d = 1;
+    for (int l = 0; l < mmin; l++) {
+        d *= 2;
+        for (long j = 0; j < Nj; j++) {
+            n += d;
+            f.push_back(n);
+        }
+    }
+    return f;
+}
+
+int al_alloc(tpsc_ctx* c)
+{
+    const Grid& g = c->g;
+    ALArrays& a = c->al;
+    if (a.N == g.N && a.Nw == g.Nw) return 0;
+    if (a.N) return fail(c, TPSC_E_INVALID, "Aslamazov-Larkin arrays were sized for another grid");
+    const size_t plane = (size_t)(g.Nw + 1) * g.nq;
+    int rc = 0;
+    rc |= dalloc(c, &a.Grt, (size_t)(g.Nw + 1) * g.nk8);
+    rc |= dalloc(c, &a.dGrt, 2 * (size_t)g.nk8);
+    rc |= dalloc(c, &a.dhG, 2 * (size_t)g.nq);
+    rc |= dalloc(c, &a.hw, (size_t)g.nq * (g.Nw / 2));
+    rc |= dalloc(c, &a.hg2, (size_t)g.nq);
+    rc |= dalloc(c, &a.chiwP, (size_t)g.nk8 * g.nwb);
+    rc |= dalloc(c, &a.P1p, 2 * plane);
+    rc |= dalloc(c, &a.P2p, 2 * plane);
+    rc |= dalloc(c, &a.HGs, 2 * plane);
+    rc |= dalloc(c, &a.P1s, 2 * plane);
+    rc |= dalloc(c, &a.P2s, 2 * plane);
+    rc |= dalloc(c, &a.dHq, 4 * (size_t)g.nq);
+    rc |= dalloc(c, &a.dP, 8 * (size_t)g.nq);
+    rc |= dalloc(c, &a.contrib, (size_t)g.nwb * g.nq);
+    rc |= dalloc(c, &a.red, (size_t)g.nwb);
+    if (rc) return rc;
+    a.N = g.N;
+    a.Nw = g.Nw;
+    return 0;
+}
+
+// cond_opt.cpp:4292-6467 (chi3 = true): Lambda_jj3 on the sparse boson set, after the
+// bubble and Maki-Thompson terms.  One launch sequence per boson frequency, no host sync
+// until the fixed-order reduction of the per-k contributions at the end.
+int stage_al(tpsc_ctx* c, int mmin)
+{
+    if (!c->have_jj) return fail(c, TPSC_E_ORDER, "Aslamazov-Larkin term needs Lambda_jj1, Lambda_jj2 first");
+    if (mmin < 0 || mmin > 30) return fail(c, TPSC_E_INVALID, "mmin must be in 0..30");
+    const Grid& g = c->g;
+    int rc = al_alloc(c);
+    if (rc) return rc;
+    const std::vector<long> fr = al_freqs(g.Nw, mmin);
+    const int Ns = (int)fr.size();
+    hipStream_t s = c->stream;
+    prof_begin(c, "al_setup");
+    HIPCHK(c, launch_al_setup(g, c->d, c->al, c->mu0, s));
+    prof_end(c);
+    for (int n = 1; n < Ns; n++) {
+        prof_begin(c, "al_freq");
+        HIPCHK(c, launch_al_freq(g, c->d, c->al, c->Usp, c->Uch, c->mu, c->p.U, fr[n],
+                                 c->al.contrib + (size_t)n * g.nq, s));
+        prof_end(c);
+    }
+    std::vector<double> r(Ns, 0.0);
+    if (Ns > 1) {
+        HIPCHK(c, launch_sum_rows(c->al.contrib + g.nq, Ns - 1, (int)g.nq, c->al.red + 1, s));
+        HIPCHK(c, hipMemcpyAsync(r.data() + 1, c->al.red + 1, sizeof(double) * (Ns - 1), hipMemcpyDeviceToHost, s));
+    }
+    HIPCHK(c, hipStreamSynchronize(s));
+    prof_flush(c);
+    c->al_freq = fr;
+    c->chijj3 = r;
+    c->chijj_all.assign(Ns, 0.0);
+    c->chijj_all[0] = c->chijj1[0] + c->chijj2[0];
+    for (int n = 1; n < Ns; n++) c->chijj_all[n] = c->chijj1[fr[n]] + c->chijj2[fr[n]] + r[n];
+    return 0;
+}
+
 size_t field_count(tpsc_ctx* c, int field)
 {
     const Grid& g = c->g;
@@ -783,6 +871,9 @@ size_t field_count(tpsc_ctx* c, int field)
         case TPSC_F_SIGMA_KW: return 2 * (size_t)(g.Nw / 2) * g.nk8;
         case TPSC_F_CHIJJ1:
         case TPSC_F_CHIJJ2: return (size_t)g.nwb;
+        case TPSC_F_CHIJJ3:
+        case TPSC_F_CHIJJ_ALL:
+        case TPSC_F_AL_FREQ: return c->chijj3.size();
         default: return 0;
     }
 }
@@ -1055,12 +1146,13 @@ int tpsc_find_mu(tpsc_ctx* c, double* mu)
 
 int tpsc_chijj(tpsc_ctx* c, int chi3, int mmin, int N_theta_k)
 {
-    (void)mmin;
     if (!c) return TPSC_E_INVALID;
-    if (chi3) return fail(c, TPSC_E_INVALID, "Aslamazov-Larkin term (chi3) is not available in this build");
     if (N_theta_k != 1) return fail(c, TPSC_E_INVALID, "N_theta_k must be 1");
     if (c->g.Nw > 2048) return fail(c, TPSC_E_INVALID, "Lambda_jj needs Nw <= 2048");
-    return stage_chijj(c);
+    if (chi3 && (mmin < 0 || mmin > 30)) return fail(c, TPSC_E_INVALID, "mmin must be in 0..30");
+    int rc = stage_chijj(c);
+    if (rc || !chi3) return rc;
+    return stage_al(c, mmin);
 }
 
 int tpsc_solve(tpsc_ctx* c)
@@ -1079,6 +1171,15 @@ int tpsc_get(tpsc_ctx* c, int field, void* dst, size_t bytes)
 {
     if (!c || !dst) return TPSC_E_INVALID;
     const size_t n = field_count(c, field);
+    if (field == TPSC_F_CHIJJ3 || field == TPSC_F_CHIJJ_ALL || field == TPSC_F_AL_FREQ) {
+        if (n == 0) return fail(c, TPSC_E_ORDER, "Aslamazov-Larkin term not computed");
+        if (bytes < n * sizeof(double)) return fail(c, TPSC_E_INVALID, "destination too small");
+        double* o = (double*)dst;
+        for (size_t i = 0; i < n; i++)
+            o[i] = (field == TPSC_F_CHIJJ3) ? c->chijj3[i] : (field == TPSC_F_CHIJJ_ALL) ? c->chijj_all[i]
+                                                                                             : (double)c->al_freq[i];
+        return 0;
+    }
     if (n == 0) return fail(c, TPSC_E_INVALID, "unknown field");
     if (bytes < n * sizeof(double)) return fail(c, TPSC_E_INVALID, "destination too small");
     const void* src = nullptr;
@@ -1159,6 +1260,7 @@ int tpsc_get_scalar(tpsc_ctx* c, int which, double* v)
         case TPSC_S_NK8: *v = (double)c->g.nk8; break;
         case TPSC_S_NWB: *v = c->g.nwb; break;
         case TPSC_S_NQ: *v = (double)c->g.nq; break;
+        case TPSC_S_AL_N: *v = (double)c->chijj3.size(); break;
         default: return fail(c, TPSC_E_INVALID, "unknown scalar");
     }
     return 0;

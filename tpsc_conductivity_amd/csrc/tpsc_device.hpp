@@ -757,4 +757,90 @@ __device__ inline double block_sum(double v, double* scratch)
     return s;
 }
 
+
+// ---------------------------------------------------------------------------------
+// slab layout helpers, Sigma(k,iwn) storage forms and the Lambda-grid integrand h / f0
+// (shared by tpsc_kernels.hip and tpsc_al.hip)
+// ---------------------------------------------------------------------------------
+// wedge (l,m) of a packed index, m <= l
+__device__ __forceinline__ void wedge_lm(long idx, int& l, int& m)
+{
+    int L = (int)((sqrt(8.0 * (double)idx + 1.0) - 1.0) * 0.5);
+    while (widx(L + 1, 0) <= idx) L++;
+    while (widx(L, 0) > idx) L--;
+    l = L;
+    m = (int)(idx - widx(L, 0));
+}
+
+// write a wedge value to both symmetric positions of an nk x nk LDS slab
+__device__ __forceinline__ void put_sym(double* slab, int nk, int l, int m, double v)
+{
+    slab[m + l * nk] = v;
+    slab[l + m * nk] = v;
+}
+
+// twiddle table placed after an n-element slab in LDS, 16-byte aligned
+template <typename Real>
+__device__ __forceinline__ typename cplx_of<Real>::type* slab_tw(Real* slab, long n)
+{
+    const long bytes = (n * (long)sizeof(Real) + 15) & ~15L;
+    return (typename cplx_of<Real>::type*)((char*)slab + bytes);
+}
+
+template <typename Real>
+__device__ __forceinline__ void put_sym_t(Real* slab, int nk, int l, int m, Real v)
+{
+    slab[m + l * nk] = v;
+    slab[l + m * nk] = v;
+}
+
+__device__ __forceinline__ cd sigma_mom(double Sinf, double s2, double s3, cd R, double wn)
+{
+    // Sinf/z + s2/z^2 + s3/z^3 + R/z^4, z = i wn (green.cpp:3039), with one reciprocal of wn
+    const double iw = 1.0 / wn, iw2 = iw * iw;
+    return cd{(-s2 + R.x * iw2) * iw2, (-Sinf + (s3 + R.y * iw) * iw2) * iw};
+}
+
+// Sigma(k, i w_n), w_n > 0, at offset o of the stored form: the remainder R (FP64 path,
+// assembled with the moments) or the assembled value itself (mixed path)
+__device__ __forceinline__ cd sigma_kw(const cd* __restrict__ Self, long o, double Sinf, double s2, double s3,
+                                       double wn)
+{
+    return sigma_mom(Sinf, s2, s3, Self[o], wn);
+}
+__device__ __forceinline__ cd sigma_kw(const cf* __restrict__ Self, long o, double, double, double, double)
+{
+    const cf v = Self[o];
+    return cd{(double)v.x, (double)v.y};
+}
+// k-space integrand of h (which = 0) or f0 (which = 1) at tau = t (cond_opt.cpp:2697-2735,
+// 2795-2827); nf = 1/N^2, v = v_x(k)
+__device__ __forceinline__ double hf0_val(int which, double e, double v, double mu0, double T, double t, double nf)
+{
+    if (which == 0) {
+        const double x = e - mu0;
+        return (x > 0) ? nf * v * exp(-x * t) / (exp(-x / T) + 1.0)
+                       : nf * v * exp(x * (1.0 / T - t)) / (exp(x / T) + 1.0);
+    }
+    if (e > mu0) {
+        const double ex = exp((mu0 - e) / T);
+        return nf * v * exp((mu0 - e) * t) * (t / (ex + 1.0) - ex / (T * (ex + 1.0) * (ex + 1.0)));
+    }
+    const double ex = exp((e - mu0) / T);
+    return nf * v * exp((e - mu0) * (1.0 / T - t)) * (t / (ex + 1.0) - 1.0 / (T * (ex + 1.0) * (ex + 1.0)));
+}
+
+
+// tau-derivative of the h integrand at tau = 0+ (side 0) or beta- (side 1)
+// (cond_opt.cpp:2745-2781)
+__device__ __forceinline__ double h_slope_val(int side, double e, double v, double mu0, double T, double nf)
+{
+    if (e > mu0) {
+        const double ex = exp((mu0 - e) / T);
+        return (side == 0) ? -nf * v * (e - mu0) / (ex + 1.0) : -nf * v * (e - mu0) * ex / (ex + 1.0);
+    }
+    const double ex = exp((e - mu0) / T);
+    return (side == 0) ? -nf * v * (e - mu0) * ex / (ex + 1.0) : -nf * v * (e - mu0) / (ex + 1.0);
+}
+
 }  // namespace tpsc

@@ -73,38 +73,6 @@ __global__ void k_finalize_k(const double* __restrict__ part, int n, double* out
     if (threadIdx.x == 0) out[blockIdx.x] = a;
 }
 
-// wedge (l,m) of a packed index, m <= l
-__device__ __forceinline__ void wedge_lm(long idx, int& l, int& m)
-{
-    int L = (int)((sqrt(8.0 * (double)idx + 1.0) - 1.0) * 0.5);
-    while (widx(L + 1, 0) <= idx) L++;
-    while (widx(L, 0) > idx) L--;
-    l = L;
-    m = (int)(idx - widx(L, 0));
-}
-
-// write a wedge value to both symmetric positions of an nk x nk LDS slab
-__device__ __forceinline__ void put_sym(double* slab, int nk, int l, int m, double v)
-{
-    slab[m + l * nk] = v;
-    slab[l + m * nk] = v;
-}
-
-// twiddle table placed after an n-element slab in LDS, 16-byte aligned
-template <typename Real>
-__device__ __forceinline__ typename cplx_of<Real>::type* slab_tw(Real* slab, long n)
-{
-    const long bytes = (n * (long)sizeof(Real) + 15) & ~15L;
-    return (typename cplx_of<Real>::type*)((char*)slab + bytes);
-}
-
-template <typename Real>
-__device__ __forceinline__ void put_sym_t(Real* slab, int nk, int l, int m, Real v)
-{
-    slab[m + l * nk] = v;
-    slab[l + m * nk] = v;
-}
-
 // ================================================================== chi0 slabs
 // G(k,t)/N^2, chi0.cpp:468-471 (form==0) and :535-538 (form==1, tau = beta/2)
 __device__ __forceinline__ double g0_plus(double x, double t, double T, double nk02, int form)
@@ -525,25 +493,6 @@ __global__ void __launch_bounds__(PEN_THREADS)
 }
 
 // ================================================================== self-energy
-__device__ __forceinline__ cd sigma_mom(double Sinf, double s2, double s3, cd R, double wn)
-{
-    // Sinf/z + s2/z^2 + s3/z^3 + R/z^4, z = i wn (green.cpp:3039), with one reciprocal of wn
-    const double iw = 1.0 / wn, iw2 = iw * iw;
-    return cd{(-s2 + R.x * iw2) * iw2, (-Sinf + (s3 + R.y * iw) * iw2) * iw};
-}
-
-// Sigma(k, i w_n), w_n > 0, at offset o of the stored form: the remainder R (FP64 path,
-// assembled with the moments) or the assembled value itself (mixed path)
-__device__ __forceinline__ cd sigma_kw(const cd* __restrict__ Self, long o, double Sinf, double s2, double s3,
-                                       double wn)
-{
-    return sigma_mom(Sinf, s2, s3, Self[o], wn);
-}
-__device__ __forceinline__ cd sigma_kw(const cf* __restrict__ Self, long o, double, double, double, double)
-{
-    const cf v = Self[o];
-    return cd{(double)v.x, (double)v.y};
-}
 __device__ __forceinline__ void store_self(cd* Self, long o, cd R, double, double, double, double) { Self[o] = R; }
 __device__ __forceinline__ void store_self(cf* Self, long o, cd R, double Sinf, double s2, double s3, double wn)
 {
@@ -910,23 +859,6 @@ __global__ void k_model(int N, double tp, double tpp, double t3, double* ek, dou
 }
 
 // ================================================================== Lambda_jj slabs
-// k-space integrand of h (which = 0) or f0 (which = 1) at tau = t (cond_opt.cpp:2697-2735,
-// 2795-2827); nf = 1/N^2, v = v_x(k)
-__device__ __forceinline__ double hf0_val(int which, double e, double v, double mu0, double T, double t, double nf)
-{
-    if (which == 0) {
-        const double x = e - mu0;
-        return (x > 0) ? nf * v * exp(-x * t) / (exp(-x / T) + 1.0)
-                       : nf * v * exp(x * (1.0 / T - t)) / (exp(x / T) + 1.0);
-    }
-    if (e > mu0) {
-        const double ex = exp((mu0 - e) / T);
-        return nf * v * exp((mu0 - e) * t) * (t / (ex + 1.0) - ex / (T * (ex + 1.0) * (ex + 1.0)));
-    }
-    const double ex = exp((e - mu0) / T);
-    return nf * v * exp((e - mu0) * (1.0 / T - t)) * (t / (ex + 1.0) - 1.0 / (T * (ex + 1.0) * (ex + 1.0)));
-}
-
 // h(r,tau_j) and f0(r,tau_j), j = 0..Nw (cond_opt.cpp:2697-2735, 2795-2827):
 // RODFT00 along kx (index l+1), REDFT00 along ky.
 template <int N, typename S>
@@ -996,13 +928,7 @@ __global__ void __launch_bounds__(SLAB_THREADS)
         const double e = ek[m + (l + 1) * nk], v = dek[s];
         double val;
         if (which == 0) {
-            if (e > mu0) {
-                const double ex = exp((mu0 - e) / T);
-                val = (side == 0) ? -nf * v * (e - mu0) / (ex + 1.0) : -nf * v * (e - mu0) * ex / (ex + 1.0);
-            } else {
-                const double ex = exp((e - mu0) / T);
-                val = (side == 0) ? -nf * v * (e - mu0) * ex / (ex + 1.0) : -nf * v * (e - mu0) / (ex + 1.0);
-            }
+            val = h_slope_val(side, e, v, mu0, T, nf);
         } else {
             if (e > mu0) {
                 const double ex = exp((mu0 - e) / T);
@@ -1825,6 +1751,22 @@ hipError_t launch_jj_pencil(const Grid& g, const DevArrays& d, double mu, hipStr
                            (const JJPoles*)d.poles, (const ST*)d.Self, d.Sinf2, d.Sinf3, d.Sinf, g.N, g.T, mu, d.twN,
                            d.twX, d.twS, d.twM, d.ph, d.jjpart, d.jjscal);
     }));
+    return hipGetLastError();
+}
+
+// [r][c] (nrows x ncols) -> [c][r]
+hipError_t launch_transpose_d(const double* in, double* out, long ncols, int nrows, hipStream_t s)
+{
+    const dim3 gr((unsigned)((ncols + 31) / 32), (unsigned)((nrows + 31) / 32));
+    hipLaunchKernelGGL(k_transpose<double>, gr, dim3(256), 0, s, in, out, ncols, nrows);
+    return hipGetLastError();
+}
+
+// out[r] = sum of part[r*n .. r*n+n) in a fixed order, r < nrows
+hipError_t launch_sum_rows(const double* part, int nrows, int n, double* out, hipStream_t s)
+{
+    if (nrows <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_finalize_k, dim3(nrows), dim3(256), 0, s, part, n, out);
     return hipGetLastError();
 }
 

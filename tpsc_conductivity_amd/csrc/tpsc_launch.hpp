@@ -63,6 +63,27 @@ struct DevArrays {
     int* imax = nullptr;         // argmax result
 };
 
+// Aslamazov-Larkin work arrays (FP64 in both precisions; allocated at the first chi3 call).
+// A plane is (Nw+1)*nq doubles; complex tau arrays are two planes (re, im).
+struct ALArrays {
+    double* Grt = nullptr;      // (Nw+1)*nk8  G0(r,-tau_j) on the wedge
+    double* dGrt = nullptr;     // 2*nk8       its tau-slopes at 0, beta
+    double* dhG = nullptr;      // 2*nq        slopes of hG(q,tau) at 0, beta
+    double* hw = nullptr;       // nq*(Nw/2)   hG(q,iq_p), point-major [q][p]
+    double* hg2 = nullptr;      // nq          1/qn^2 coefficient of hG
+    double* chiwP = nullptr;    // nk8*nwb     chi0(q,iq_p), point-major [w][p]
+    double* P1p = nullptr;      // 2 planes    point-major [c][q][t]: HG(q,tau), then P1(k,tau)
+    double* P2p = nullptr;      // 2 planes    point-major P2(k,tau)
+    double* HGs = nullptr;      // 2 planes    slab-major HG(q,tau) [c][t][q]
+    double* P1s = nullptr;      // 2 planes    slab-major P1 (and hG(q,tau) at set-up)
+    double* P2s = nullptr;      // 2 planes    slab-major P2
+    double* dHq = nullptr;      // 4*nq        HG slopes [c][side][q]
+    double* dP = nullptr;       // 8*nq        P1/P2 slopes [which][side][c][k]
+    double* contrib = nullptr;  // nwb*nq      per-frequency, per-k contributions [n][k]
+    double* red = nullptr;      // nwb         Lambda_jj3 on the sparse set
+    int N = 0, Nw = 0;          // grid the arrays were sized for
+};
+
 struct Grid {
     int N, nk, Nw, nwb;
     long nk8, nq;
@@ -107,6 +128,12 @@ hipError_t launch_jj_transpose(const Grid& g, const DevArrays& d, hipStream_t s)
 hipError_t launch_jj_pencil(const Grid& g, const DevArrays& d, double mu, hipStream_t s);
 hipError_t launch_kx0_rows(const Grid& g, const DevArrays& d, double mu, hipStream_t s);
 hipError_t launch_finalize(const double* part, int n, double* out, hipStream_t s);
+
+hipError_t launch_transpose_d(const double* in, double* out, long ncols, int nrows, hipStream_t s);
+hipError_t launch_sum_rows(const double* part, int nrows, int n, double* out, hipStream_t s);
+hipError_t launch_al_setup(const Grid& g, const DevArrays& d, const ALArrays& a, double mu0, hipStream_t s);
+hipError_t launch_al_freq(const Grid& g, const DevArrays& d, const ALArrays& a, double Usp, double Uch, double mu,
+                          double U, long nn, double* contrib, hipStream_t s);
 
 bool supported_N(int N);
 bool supported_Nw(int Nw);
