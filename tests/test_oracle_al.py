@@ -9,9 +9,13 @@ mathematics fixes, independently of the reference's code path.
 * The imaginary-time two-pole tail equals the direct Matsubara sum of its frequency form;
   its tau = beta value and its end slopes are the limits / derivatives of the general form.
 * The sparse boson set follows the reference's rule; the angle bins add up to the total.
+* The k-space, fermionic-frequency stage (q -> r, products with G0(r,-tau), r -> k, splines,
+  shifted transforms, the G G sum) converges to the direct Matsubara-convolution form of the
+  same expression (al_brute) as Nw grows.
 """
 import numpy as np
 import pytest
+from scipy.signal import fftconvolve
 
 
 def lattice(o, N):
@@ -44,6 +48,98 @@ def lindhard(o, N, T, mu0, p_list):
                     r = (f - fq) * D / (D * D + w * w)
                 hw[i, q] = -2.0 / N ** 2 * np.sum(v * r)
     return hg2, hw
+
+
+def al_brute(o, n_idx, P):
+    """Lambda_jj3 at the n_idx-th sparse frequency by direct Matsubara convolutions:
+    HG(k,iw) = -(T/N^2) sum_{q,p} HG(q,iq_p) / (i(w + q_p) - xi_k+q),
+    HG2(k,iw) = -(T/N^2) sum_{q,p} HG(q,iq_p) / (i(w - q_p) - xi_k+q),  |p| <= P,
+    i.e. the k-space, fermionic-frequency form of [HG(r,tau) G0(r,-tau)] and
+    [HG(r,beta-tau) G0(r,-tau)] (DST x DCT = -i x lattice Fourier sum for odd-in-x functions),
+    with HG(q,iq_p) from the oracle's chi0(q,iq_p), hG(q,iq_p), hg2 and tails on the full
+    lattice (odd in qx).  No tau grid, no splines, no two-pole tail model."""
+    s = o.s; N = s.p.nk0; nbw = s.nbw; H = nbw // 2; T = s.p.T; mu0 = s.mu0; mu = s.mu
+    nk = N // 2 + 1; nk8 = nk * (nk + 1) // 2; nq = nk * (nk - 2); Nsites = N * N
+    Usp, Uch, U = s.Usp, s.Uch, s.p.U
+    chiqw = o.get("chiqw"); ci = o.get("chi0_inf"); ci2 = o.get("chi0_inf2")
+    hw = o.get("al_hw").reshape(H, nq); hg2 = o.get("al_hg2")
+    Self = o.get("Self"); S2 = o.get("Sigma_inf2"); S3 = o.get("Sigma_inf3"); Sinf = s.Sigma_inf
+    nn = int(o.al_freqs()[n_idx]); nc = nn // 2; qn = 2 * np.pi * nn * T
+    def widx(l, m): return m + l * (l + 1) // 2
+    def fold(x, y):
+        if x > N // 2: x = N - x
+        if y > N // 2: y = N - y
+        if y > x: x, y = y, x
+        return widx(x, y)
+    ks = 2 * np.pi * np.arange(N) / N
+    eps = np.array([[o.dispersion(a, b) for b in ks] for a in ks])
+    p = np.arange(-P, P + 1)
+    ap = np.abs(p); w = 2 * np.pi * ap * T
+    # A(q_full, p)
+    A = np.zeros((N, N, 2 * P + 1))
+    for qx in range(N):
+        if qx == 0 or qx == N // 2: continue
+        sg = 1.0 if qx < N // 2 else -1.0
+        lx = qx if qx < N // 2 else N - qx
+        for qy in range(N):
+            ly = qy if qy <= N // 2 else N - qy
+            qro = ly + (lx - 1) * nk
+            r = fold(lx, ly)
+            def chi0(pp):
+                a = np.abs(pp); ww = 2 * np.pi * a * T
+                out = np.empty(len(a))
+                m = a <= H
+                out[m] = chiqw[r + a[m] * nk8]
+                ww2 = ww[~m]
+                out[~m] = ci[r] / ww2**2 + ci2[r] / ww2**4
+                return out
+            def hG(pp):
+                a = np.abs(pp); ww = 2 * np.pi * a * T
+                out = np.empty(len(a))
+                z = a == 0
+                out[z] = hw[0, qro]
+                nz = ~z
+                c4 = np.where(a[nz] < H, hw[np.minimum(a[nz], H - 1), qro], hw[H - 1, qro])
+                out[nz] = -hg2[qro] / ww[nz]**2 + c4 / ww[nz]**4
+                return out
+            pe = np.arange(-P - 2, P + 1)
+            with np.errstate(divide='ignore', invalid='ignore'):
+                D = (hG(pe) - hG(pe + nn)) / (chi0(pe) - chi0(pe + nn))
+            Dp = D[2:]
+            if nn % 2 == 0:
+                i = -nn // 2 + P
+                Dp[i] = (4 * D[i + 1] - D[i]) / 3.0   # D[i+2-1], D[i+2-2]
+            c1 = chi0(p)
+            G = 3 * Usp * c1 / (1 - Usp * c1 / 2) + Uch * c1 / (1 + Uch * c1 / 2)
+            A[qx, qy] = sg * G * Dp
+    A = A.reshape(N * N, 2 * P + 1)
+    jj = np.arange(nbw); f = np.where(jj < H - nc, jj, jj - nbw)
+    smin, smax = f.min(), f.max() + nn
+    umin, umax = smin - P, smax + P
+    u = np.arange(umin, umax + 1)
+    tot = 0.0
+    for l in range(nk - 2):
+        for m in range(nk):
+            kx, ky = l + 1, m
+            xi = np.array([eps[(kx + qx) % N, (ky + qy) % N] - mu0 for qx in range(N) for qy in range(N)])
+            b = 1.0 / (1j * (2 * u[None, :] + 1) * np.pi * T - xi[:, None])
+            c1 = fftconvolve(b, A[:, ::-1], axes=1).sum(0)
+            c2 = fftconvolve(b, A, axes=1).sum(0)
+            HGk = -T / Nsites * c1[(f + nn) + P - umin]
+            HG2k = -T / Nsites * c2[f + P - umin]
+            ki = fold(kx, ky); e = eps[kx, ky]
+            def sigma(ff):
+                kn = (2 * ff + 1) * np.pi * T
+                g = np.where(ff >= 0, ff, -ff - 1)
+                R = np.where(g < H, Self[ki + np.minimum(g, H - 1) * nk8], 0)
+                R = np.where(ff >= 0, R, np.conj(R))
+                return (-S2[ki] / kn**2 + R.real / kn**4) + 1j * (-Sinf / kn + S3[ki] / kn**3 + R.imag / kn**4)
+            k1 = (2 * f + 1) * np.pi * T; k2 = (2 * (f + nn) + 1) * np.pi * T
+            G1 = 1 / (1j * k1 - e + mu - sigma(f)); G2 = 1 / (1j * k2 - e + mu - sigma(f + nn))
+            wt = 2.0 if (m == 0 or m == nk - 1) else 4.0
+            v = o.velocity_x(kx * np.pi / (nk - 1), ky * np.pi / (nk - 1))
+            tot += np.real(-1j * U * T / Nsites * wt * v * np.sum(G1 * G2 * (HGk - HG2k)) / (2 * qn))
+    return tot
 
 
 def solved(oracle_mod, N, Nw, mmin=2, T=0.1):
@@ -117,3 +213,19 @@ def test_al_angle_bins_and_total(oracle_mod):
     tot = o.get("chijj1")[f] + o.get("chijj2")[f] + c3
     tot[0] = o.get("chijj1")[0] + o.get("chijj2")[0]
     assert np.allclose(o.get("chijj_all"), tot, rtol=1e-14)
+
+
+def test_al_lambda3_converges_to_matsubara_convolution(oracle_mod):
+    errs = {}
+    for Nw in (64, 128):
+        o = oracle_mod.Oracle(8, Nw, 0.9, 0.2, 2.0, tp=-0.2, tpp=0.05, coarse_pass=False)
+        assert o.solve()
+        assert o.calc_chijj_al(1, 1) == 0
+        c3, fr = o.get("chijj3"), o.al_freqs()
+        for n_idx in (1, 2, len(fr) - 1):  # odd n, even n (Richardson point), n = Nw/2
+            b = al_brute(o, n_idx, 4096)
+            errs[(Nw, n_idx if n_idx < 3 else -1)] = abs(c3[n_idx] - b) / abs(c3[n_idx])
+    print("AL vs Matsubara convolution:", {k: f"{v:.1e}" for k, v in errs.items()})
+    assert errs[(128, 1)] < 1e-6 and errs[(128, 2)] < 1e-6 and errs[(128, -1)] < 1e-3
+    for n in (1, 2, -1):
+        assert errs[(128, n)] < errs[(64, n)] / 3
