@@ -99,6 +99,36 @@ def profile(g, N, Nw, precision):
     return table, per, kb
 
 
+def bench_al(cfg, args):
+    """Lambda_jj including the Aslamazov-Larkin term (chi3) after one solve: wall time of the
+    call and, from the library's HIP events, the AL set-up and per-frequency times."""
+    g = make(cfg, args.precision)
+    g.solve()
+    for i in range(max(args.warmup, 1)):
+        g.calc_chijj_vertex_corr_all_optim(chi3=True, mmin=args.al)
+        print(f"[bench] al warmup {i + 1} done", file=sys.stderr, flush=True)
+    steps = max(1, min(args.steps, 3))
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.calc_chijj_vertex_corr_all_optim(chi3=True, mmin=args.al)
+    ms = 1e3 * (time.perf_counter() - t0) / steps
+    g.profile(True)
+    g.profile_reset()
+    g.calc_chijj_vertex_corr_all_optim(chi3=True, mmin=args.al)
+    table = g.profile_table()
+    g.profile(False)
+    nf = int(g.scalar("al_N")) - 1
+    al_ms = table["al_setup"][0] + table["al_freq"][0]
+    print(json.dumps({"metric": f"Aslamazov-Larkin Lambda_jj3 boson frequencies/s at {cfg['N']}²×{cfg['Nw']} iωn",
+                      "value": nf / (al_ms * 1e-3), "unit": "frequencies/s", "higher_is_better": True,
+                      "n_freq": nf, "mmin": args.al, "ms_per_freq": table["al_freq"][0] / max(nf, 1),
+                      "al_setup_ms": table["al_setup"][0], "al_ms": al_ms, "chijj_call_ms": ms,
+                      "steps": steps, "dtype": "f64", "precision": args.precision, "data": "synthetic",
+                      "config": {"workload": f"TPSC {cfg['N']}x{cfg['N']} lattice, {cfg['Nw']} Matsubara "
+                                             f"frequencies, n={cfg['n']}, T={cfg['T']}, U={cfg['U']}"}}),
+          flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -114,6 +144,9 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run an elementwise copy of a 1 GiB device buffer 5 times (reference kernel "
                          "for the FETCH_SIZE / WRITE_SIZE scale of the traffic figure) and exit")
+    ap.add_argument("--al", type=int, default=None, metavar="MMIN",
+                    help="instead of the headline line, time the Aslamazov-Larkin term on the sparse boson set "
+                         "of parameter MMIN (one solve first, then Lambda_jj with chi3) and print its own line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -130,6 +163,9 @@ def main():
         print(json.dumps({"copy_bytes_read": src.numel() * 8, "copy_bytes_written": dst.numel() * 8,
                           "launches": 5}), flush=True)
         return
+
+    if args.al is not None:
+        return bench_al(dict(WORKLOAD, N=args.N, Nw=args.Nw), args)
 
     from tpsc_conductivity_amd import roofline
 

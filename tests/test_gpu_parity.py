@@ -169,7 +169,7 @@ def test_error_paths():
     with pytest.raises(TPSCError, match="ORDER"):
         g.traceDiffG(0.0)
     with pytest.raises(TPSCError, match="INVALID"):
-        g.calc_chijj_vertex_corr_all_optim(chi3=True)
+        g.calc_chijj_vertex_corr_all_optim(chi3=True, mmin=-1)
     with pytest.raises(TPSCError, match="INVALID"):
         g.calc_chijj_vertex_corr_all_optim(N_theta_k=2)
     with pytest.raises(TPSCError, match="INVALID"):
