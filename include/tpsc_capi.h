@@ -90,7 +90,14 @@ enum tpsc_scalar {
     TPSC_S_DBLOCC, TPSC_S_DIFREL_SP, TPSC_S_DIFREL_CH, TPSC_S_SIGMA_INF, TPSC_S_TRSG,
     TPSC_S_TRSG_CORR, TPSC_S_MU, TPSC_S_TRSG2, TPSC_S_DBLOCC2, TPSC_S_KX0, TPSC_S_CHIJJ1_INF2,
     TPSC_S_CHIJJ2_INF2, TPSC_S_CHIJJ2_INF4, TPSC_S_NEVAL_SP, TPSC_S_NEVAL_CH, TPSC_S_NEVAL_MU,
-    TPSC_S_NK, TPSC_S_NK8, TPSC_S_NWB, TPSC_S_NQ, TPSC_S_AL_N
+    TPSC_S_NK, TPSC_S_NK8, TPSC_S_NWB, TPSC_S_NQ, TPSC_S_AL_N, TPSC_S_AL_MMIN,
+    TPSC_S_STAGES  /* bitmask of the completed stages, TPSC_STAGE_* (no reference counterpart:
+                      the reference keeps this state in its members) */
+};
+enum tpsc_stage {
+    TPSC_STAGE_CHI0 = 1, TPSC_STAGE_USP_UCH = 2, TPSC_STAGE_VERTEX = 4, TPSC_STAGE_SIGMA = 8,
+    TPSC_STAGE_TRACE_SG = 16, TPSC_STAGE_MU = 32, TPSC_STAGE_TRACE_SG2 = 64, TPSC_STAGE_CHIJJ = 128,
+    TPSC_STAGE_AL = 256
 };
 
 /* constructor / destructor (cond_opt.cpp:10-62 and the base-class constructors). Builds
@@ -143,6 +150,23 @@ int tpsc_chijj(tpsc_ctx* ctx, int chi3, int mmin, int N_theta_k);
 /* one full solve: constructor state -> calc_Self_FFT_spline -> traceSelfG -> find_mu ->
  * calc_chijj_vertex_corr_all_optim (tpsc_main.cpp:61-75 and :243) */
 int tpsc_solve(tpsc_ctx* ctx);
+
+/* the parameters the context was created with */
+int tpsc_get_params(tpsc_ctx* ctx, tpsc_params* out);
+
+/* Writes, into directory dir (NULL or "" = the current directory), the text files the
+ * reference writes along the same call sequence, for the stages this context has completed,
+ * with the reference's file names and stream formats:
+ *   e_k_*                                    hamiltonien.cpp:72-82
+ *   chi0_qmax_wn_*, chi0_wn0_q_*             chi0.cpp:720-737
+ *   TPSC_infos_*, TPSCparams_*               chi.cpp:375-415, green.cpp:3088-3103 (traceSelfG),
+ *                                            green.cpp:3259-3266 (find_mu), :2978-2994 (traceSelfG2)
+ *   mu0_mu_D_*                               green.cpp:3268-3276
+ *   kx0_*, kx0_bin_*, chijj1_qn_*, chijj2_qn_*, chijj_with_vc1_qn_*   cond_opt.cpp:4195-4279
+ *   chijj3_qn_*, chijj_with_all_vc_qn_*      cond_opt.cpp:6447-6481
+ * The files the reference builds in several appends are written whole.  *nfiles (optional)
+ * receives the number of files written. */
+int tpsc_write_files(tpsc_ctx* ctx, const char* dir, int* nfiles);
 
 /* copy-out in the reference layouts; bytes must be >= the field size */
 int tpsc_get(tpsc_ctx* ctx, int field, void* dst, size_t bytes);

@@ -1,0 +1,111 @@
+"""The input.dat driver and tpsc_write_files on the GPU: the reference's file set, names and
+formats (hamiltonien.cpp:72-82, chi0.cpp:720-737, chi.cpp:375-415, green.cpp:2978-2994,
+3088-3103, 3259-3276, cond_opt.cpp:4195-4279, 6447-6481), with the same numbers the C ABI
+returns for the same parameter point."""
+import os
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+RUN = os.path.join(ROOT, "tpsc_conductivity_amd", "tpsc_run")
+U, tp, tpp, t3, n, T, N, Nw, mmin = 4.0, -0.175, 0.0, 0.0, 0.875, 0.1, 16, 128, 2
+
+
+def names(qx, qy):
+    a8 = (U, tp, tpp, t3, n, T, N, Nw)
+    return {
+        "ek": "e_k_%5.3f_%5.3f_%5.3f_%1d.dat" % (tp, tpp, t3, N),
+        "chi0w": "chi0_qmax_wn_%5.3f_%5.3f_%5.3f_%7.5f_%6.4f_%d_%d_%d_%d.dat" % (tp, tpp, t3, n, T, N, Nw, qx, qy),
+        "chi0q": "chi0_wn0_q_%5.3f_%5.3f_%5.3f_%7.5f_%6.4f_%d_%d.dat" % (tp, tpp, t3, n, T, N, Nw),
+        "infos": "TPSC_infos_%4.2f_%5.3f_%5.3f_%5.3f_%7.5f_%6.4f_%d_%d.dat" % a8,
+        "params": "TPSCparams_%4.2f_%5.3f_%5.3f_%5.3f_%7.5f_%6.4f_%d_%d.dat" % a8,
+        "mu": "mu0_mu_D_%4.2f_%5.3f_%5.3f_%5.3f_%7.5f_%6.4f_%1d_%1d.dat" % a8,
+        "kx0": "kx0_%5.3f_%5.3f_%5.3f_%5.3f_%7.5f_%6.4f_%1d_%1d.dat" % a8,
+        "kx0b": "kx0_bin_%5.3f_%5.3f_%5.3f_%5.3f_%7.5f_%6.4f_%1d_%1d.dat" % a8,
+        "jj1": "chijj1_qn_%5.3f_%5.3f_%5.3f_%5.3f_%7.5f_%6.4f_%1d_%1d.dat" % a8,
+        "jj2": "chijj2_qn_%5.3f_%5.3f_%5.3f_%5.3f_%7.5f_%6.4f_%1d_%1d.dat" % a8,
+        "jjv": "chijj_with_vc1_qn_%5.3f_%5.3f_%5.3f_%5.3f_%7.5f_%6.4f_%1d_%1d.dat" % a8,
+        "jj3": "chijj3_qn_%5.3f_%5.3f_%5.3f_%5.3f_%7.5f_%6.4f_%1d_%1d_%1d.dat" % (a8 + (mmin,)),
+        "jja": "chijj_with_all_vc_qn_%5.3f_%5.3f_%5.3f_%5.3f_%7.5f_%6.4f_%1d_%1d_%1d.dat" % (a8 + (mmin,)),
+    }
+
+
+def table(path, cols):
+    rows = [ln.split() for ln in open(path) if ln.strip()]
+    assert all(len(r) == cols for r in rows)
+    return rows
+
+
+def test_driver_writes_the_reference_file_set(tmp_path):
+    from tpsc_conductivity_amd.tpsc import CondOpt
+
+    (tmp_path / "input.dat").write_text(f"{U} {tp} {tpp} {t3} {n} {T} {N} {Nw} 1 {mmin}\n")
+    out = tmp_path / "out"
+    out.mkdir()
+    print("[files] tpsc_run 16x128 chijj3=1", flush=True)
+    r = subprocess.run([RUN, str(tmp_path / "input.dat"), "--out", str(out)], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.startswith(f"U: 4  tp:  -0.175  tpp:  0  t3:  0  n:  0.875  T:  0.1  nk0:  16  Nw0:  128")
+    # the same point in-process through the C ABI
+    g = CondOpt(N, Nw, n, T, params=(tp, tpp, t3, U), coarse_pass=True)
+    g.solve()
+    g.calc_chijj_vertex_corr_all_optim(chi3=True, mmin=mmin)
+    qx, qy = int(g.scalar("qmax_l")), int(g.scalar("qmax_m"))
+    want = names(qx, qy)
+    assert sorted(os.listdir(out)) == sorted(want.values())
+    # e_k: l, m, eps (precision 15)
+    ek = np.array([float(r[2]) for r in table(out / want["ek"], 3)])
+    assert np.allclose(ek, g.get("ek"), rtol=1e-14, atol=1e-15)
+    # chi0 at (qxmax, qymax) for every n, and at n = 0 for every wedge q
+    c = g.get("chi0")
+    nk = N // 2 + 1
+    nk8 = nk * (nk + 1) // 2
+    w = np.array([float(r[1]) for r in table(out / want["chi0w"], 2)])
+    assert np.allclose(w, c[qy + qx * (qx + 1) // 2::nk8][: Nw // 2 + 1], rtol=1e-15, atol=0)
+    assert np.allclose([float(r[2]) for r in table(out / want["chi0q"], 3)], c[:nk8], rtol=1e-15, atol=0)
+    # parameter files
+    infos = open(out / want["infos"]).read().splitlines()
+    assert infos[0].startswith("Usp:  ") and abs(float(infos[0].split()[1]) - g.scalar("Usp")) < 1e-14
+    assert [ln.split(":")[0] for ln in infos] == [
+        "Usp", "Uch", "chi0max", "double occupation (ansatz)", "qxmax, qymax", "U*dblOcc-U*n^2/4", "trace Self*G",
+        "trace Self*G2", "double occupation (Self*G2)", "mu0", "mu"]
+    params = [ln.split() for ln in open(out / want["params"])]
+    assert [p[0] for p in params] == ["Usp", "Uch", "nbq", "nbw", "dblOcc_ansatz", "qxmax", "qymax",
+                                      "chispmax_over_chi0max", "mu0", "mu"]
+    assert int(params[2][1]) == nk and int(params[3][1]) == Nw // 2 + 1
+    assert abs(float(params[9][1]) - g.scalar("mu")) <= 1e-12 * max(1.0, abs(g.scalar("mu")))
+    mu_line = open(out / want["mu"]).read().split()
+    assert abs(float(mu_line[1]) - g.scalar("mu")) <= 1e-13 * max(1.0, abs(g.scalar("mu")))
+    # Lambda_jj
+    assert abs(float(open(out / want["kx0"]).read()) - g.scalar("kx0")) <= 1e-15 * abs(g.scalar("kx0"))
+    assert struct.unpack("d", open(out / want["kx0b"], "rb").read())[0] == pytest.approx(g.scalar("kx0"), rel=1e-14)
+    j1, j2 = g.get("chijj1"), g.get("chijj2")
+    for key, ref in (("jj1", j1), ("jj2", j2), ("jjv", j1 + j2)):
+        rows = table(out / want[key], 2)
+        assert [int(r[0]) for r in rows] == list(range(Nw // 2 + 1))
+        assert "e" in rows[1][1]  # scientific, as the reference's stream
+        assert np.allclose([float(r[1]) for r in rows], ref, rtol=1e-14, atol=0)
+    # Aslamazov-Larkin on the sparse set
+    f = g.get("al_freq").astype(int)
+    for key, ref in (("jj3", g.get("chijj3")), ("jja", g.get("chijj_all"))):
+        rows = table(out / want[key], 2)
+        assert [int(r[0]) for r in rows] == list(f)
+        assert np.allclose([float(r[1]) for r in rows], ref, rtol=1e-14, atol=1e-300)
+    g.close()
+
+
+def test_write_files_follows_the_completed_stages(tmp_path):
+    from tpsc_conductivity_amd.tpsc import CondOpt
+
+    g = CondOpt(8, 32, 0.9, 0.25, params=(-0.1, 0.0, 0.0, 2.5), coarse_pass=False)
+    assert g.write_files(str(tmp_path)) == 1           # e_k only
+    g.find_Usp_Uch()
+    assert g.write_files(str(tmp_path)) == 5           # + chi0 files, TPSC_infos, TPSCparams
+    assert not any(p.startswith("mu0_mu_D") for p in os.listdir(tmp_path))
+    g.close()

@@ -28,7 +28,7 @@ SCALARS = {
     "difRelsp": 8, "difRelch": 9, "Sigma_inf": 10, "trSG": 11, "trSG_corr": 12, "mu": 13,
     "trSG2": 14, "dblOcc2": 15, "kx0": 16, "chijj1_inf2": 17, "chijj2_inf2": 18,
     "chijj2_inf4": 19, "nevals_sp": 20, "nevals_ch": 21, "nevals_mu": 22, "nk": 23, "nk8": 24,
-    "nwb": 25, "nq": 26, "al_N": 27,
+    "nwb": 25, "nq": 26, "al_N": 27, "al_mmin": 28, "stages": 29,
 }
 
 
@@ -53,6 +53,7 @@ EXPORTS = [
     "tpsc_chijj", "tpsc_solve", "tpsc_get", "tpsc_field_size", "tpsc_get_scalar",
     "tpsc_prof_enable", "tpsc_prof_reset", "tpsc_prof_count", "tpsc_prof_get",
     "tpsc_keep_chi_tau", "tpsc_device_bytes", "tpsc_pade_coef", "tpsc_pade_eval", "tpsc_pade_conductivity",
+    "tpsc_get_params", "tpsc_write_files",
 ]
 
 
@@ -96,6 +97,8 @@ def lib():
     L.tpsc_pade_coef.argtypes = [dp, dp, C.c_int, dp]
     L.tpsc_pade_eval.argtypes = [dp, dp, C.c_int, dp, C.c_int, dp]
     L.tpsc_pade_conductivity.argtypes = [dp, C.c_int, C.c_double, C.c_int, dp, C.c_int, C.c_double, dp]
+    L.tpsc_get_params.argtypes = [vp, C.POINTER(Params)]
+    L.tpsc_write_files.argtypes = [vp, C.c_char_p, ip]
     _lib = L
     return L
 
@@ -153,6 +156,13 @@ class Context:
         v = C.c_double(0.0)
         self.check(lib().tpsc_get_scalar(self._h, SCALARS[name], C.byref(v)), f"tpsc_get_scalar({name})")
         return v.value
+
+    def write_files(self, directory=None):
+        """The reference's text outputs for the completed stages (tpsc_write_files); returns the count."""
+        n = C.c_int(0)
+        d = directory.encode() if directory else None
+        self.check(lib().tpsc_write_files(self._h, d, C.byref(n)), "tpsc_write_files")
+        return n.value
 
     def get(self, name):
         f = FIELDS[name]

@@ -51,6 +51,8 @@ struct tpsc_ctx {
     double kx0 = 0.0, jj_inf1 = 0.0, jj_inf2 = 0.0, jj_inf4 = 0.0;
     // stage state
     bool have_chi0 = false, have_U = false, have_V = false, have_sigma = false, have_jj = false;
+    bool have_trSG = false, have_trSG2 = false;
+    int al_mmin = -1;
     bool keep_chi_tau = false;
     // device
     hipStream_t stream = nullptr;
@@ -643,6 +645,7 @@ int stage_trace_selfG(tpsc_ctx* c)
     if (!rc) rc = reduce_trace(c, 1, &s1);
     if (rc) return rc;
     c->trSG = s0;
+    c->have_trSG = true;
     c->trSG_corr = s0 + s1;
     return 0;
 }
@@ -654,6 +657,7 @@ int stage_trace_selfG2(tpsc_ctx* c)
     int rc = reduce_trace(c, 2, &s);
     if (rc) return rc;
     c->trSG2 = s;
+    c->have_trSG2 = true;
     c->dblOcc2 = (s + c->p.U * c->p.n * c->p.n / 4.0) / c->p.U;
     return 0;
 }
@@ -843,6 +847,7 @@ int stage_al(tpsc_ctx* c, int mmin)
     HIPCHK(c, hipStreamSynchronize(s));
     prof_flush(c);
     c->al_freq = fr;
+    c->al_mmin = mmin;
     c->chijj3 = r;
     c->chijj_all.assign(Ns, 0.0);
     c->chijj_all[0] = c->chijj1[0] + c->chijj2[0];
@@ -1230,6 +1235,13 @@ int tpsc_get(tpsc_ctx* c, int field, void* dst, size_t bytes)
     return 0;
 }
 
+int tpsc_get_params(tpsc_ctx* c, tpsc_params* out)
+{
+    if (!c || !out) return TPSC_E_INVALID;
+    *out = c->p;
+    return 0;
+}
+
 int tpsc_get_scalar(tpsc_ctx* c, int which, double* v)
 {
     if (!c || !v) return TPSC_E_INVALID;
@@ -1261,6 +1273,14 @@ int tpsc_get_scalar(tpsc_ctx* c, int which, double* v)
         case TPSC_S_NWB: *v = c->g.nwb; break;
         case TPSC_S_NQ: *v = (double)c->g.nq; break;
         case TPSC_S_AL_N: *v = (double)c->chijj3.size(); break;
+        case TPSC_S_AL_MMIN: *v = c->al_mmin; break;
+        case TPSC_S_STAGES:
+            *v = (double)((c->have_chi0 ? TPSC_STAGE_CHI0 : 0) | (c->have_U ? TPSC_STAGE_USP_UCH : 0) |
+                          (c->have_V ? TPSC_STAGE_VERTEX : 0) | (c->have_sigma ? TPSC_STAGE_SIGMA : 0) |
+                          (c->have_trSG ? TPSC_STAGE_TRACE_SG : 0) | (c->mu_ok ? TPSC_STAGE_MU : 0) |
+                          (c->have_trSG2 ? TPSC_STAGE_TRACE_SG2 : 0) | (c->have_jj ? TPSC_STAGE_CHIJJ : 0) |
+                          (c->chijj3.empty() ? 0 : TPSC_STAGE_AL));
+            break;
         default: return fail(c, TPSC_E_INVALID, "unknown scalar");
     }
     return 0;

@@ -114,6 +114,11 @@ class CondOpt:
         return pade_conductivity(lam, self.tem, len(lam) if npade is None else npade, omega,
                                  self.eta if eta is None else eta)
 
+    def write_files(self, directory=None):
+        """Write the reference's text outputs (TPSCparams_*, TPSC_infos_*, chijj*_qn_*, kx0_*, ...)
+        for the stages run so far into directory (default: the current directory)."""
+        return self.ctx.write_files(directory)
+
     def keep_chi_tau(self, on=True):
         self.ctx.check(lib().tpsc_keep_chi_tau(self.ctx.handle, int(on)), "keep_chi_tau")
 
