@@ -151,6 +151,11 @@ int tpsc_chijj(tpsc_ctx* ctx, int chi3, int mmin, int N_theta_k);
  * calc_chijj_vertex_corr_all_optim (tpsc_main.cpp:61-75 and :243) */
 int tpsc_solve(tpsc_ctx* ctx);
 
+/* green::Gwn_local (green.cpp:209-266): the local Green function
+ * G_loc(iw_n) = (1/N^2) sum_k 1/(iw_n - e_k + mu - Sigma(k,iw_n)), n = 0..Nw/2-1, written to out
+ * as Nw/2 complex values interleaved re/im.  Needs the self-energy and mu (TPSC_E_ORDER). */
+int tpsc_gwn_local(tpsc_ctx* ctx, double* out);
+
 /* the parameters the context was created with */
 int tpsc_get_params(tpsc_ctx* ctx, tpsc_params* out);
 
@@ -164,6 +169,7 @@ int tpsc_get_params(tpsc_ctx* ctx, tpsc_params* out);
  *   mu0_mu_D_*                               green.cpp:3268-3276
  *   kx0_*, kx0_bin_*, chijj1_qn_*, chijj2_qn_*, chijj_with_vc1_qn_*   cond_opt.cpp:4195-4279
  *   chijj3_qn_*, chijj_with_all_vc_qn_*      cond_opt.cpp:6447-6481
+ *   Gwn_loc_*                                green.cpp:209-266 (after find_mu)
  * The files the reference builds in several appends are written whole.  *nfiles (optional)
  * receives the number of files written. */
 int tpsc_write_files(tpsc_ctx* ctx, const char* dir, int* nfiles);

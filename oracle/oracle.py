@@ -81,6 +81,8 @@ def lib():
         L.orc_chi0.restype = C.c_int
         L.orc_chijj.argtypes = [P, C.c_int]
         L.orc_chijj.restype = C.c_int
+        L.orc_gwn_local.argtypes = [P, _dp]
+        L.orc_gwn_local.restype = C.c_int
         L.orc_chijj_al.argtypes = [P, C.c_int, C.c_int]
         L.orc_chijj_al.restype = C.c_int
         L.orc_al_freqs.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_long)]
@@ -199,6 +201,12 @@ class Oracle:
     def calc_chijj_al(self, mmin, N_theta=1):
         """Aslamazov-Larkin Lambda_jj3 on the sparse boson set (cond_opt.cpp:4292-6467)."""
         return lib().orc_chijj_al(self._o, mmin, N_theta)
+
+    def Gwn_local(self):
+        """G_loc(iw_n), n < Nw/2 (green.cpp:209-266)."""
+        out = np.zeros(self.s.nbw // 2, dtype=np.complex128)
+        assert lib().orc_gwn_local(self._o, out.ctypes.data_as(_dp)) == 0
+        return out
 
     def al_freqs(self):
         s = self.s

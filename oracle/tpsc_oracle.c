@@ -1923,3 +1923,30 @@ int orc_chijj_al(orc *o, int mmin, int N_theta)
     free(dek);
     return 0;
 }
+
+/* ---------------------------------------------------------------- local G (green.cpp:209-266) */
+int orc_gwn_local(orc *o, double complex *out)
+{
+    if (!o->Self || !o->mu_ok) return -1;
+    const int nk = o->nbk, H = o->nbw / 2;
+    const long nk8 = (long)nk * (nk + 1) / 2;
+    const double nk02 = 4.0 * (nk - 1) * (nk - 1);
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < H; j++) {
+        const double complex z = I * ((2 * j + 1) * PI * o->p.T);
+        double complex g = 0;
+        for (int l = 0; l < nk; l++)
+            for (int m = 0; m <= l; m++) {
+                int w = 8;
+                if (l == 0 || l == nk - 1) w /= 2;
+                if (m == 0 || m == nk - 1) w /= 2;
+                if (m == l) w /= 2;
+                const long k = m + (l * (l + 1)) / 2;
+                const double complex s = o->Sigma_inf / z + o->Sigma_inf2[k] / (z * z) + o->Sigma_inf3[k] / (z * z * z) +
+                                         o->Self[k + (size_t)j * nk8] / (z * z * z * z);
+                g += ((double)w / nk02) / (z - o->ek[m + l * nk] + o->mu - s);
+            }
+        out[j] = g;
+    }
+    return 0;
+}

@@ -100,6 +100,7 @@ int orc_find_mu(orc *o);                                /* green.cpp:3185-3282 *
 int orc_trace_self_G2(orc *o);                          /* green.cpp:2929-2997 */
 int orc_chijj(orc *o, int N_theta);                     /* cond_opt.cpp:2537-4290 */
 int orc_solve(orc *o);                                  /* tpsc_main.cpp:61-75 + E4 */
+int orc_gwn_local(orc *o, double complex *out);          /* green.cpp:209-266 */
 int orc_al_freqs(int nbw, int mmin, long *out);         /* cond_opt.cpp:5123-5156 */
 int orc_chijj_al(orc *o, int mmin, int N_theta);        /* cond_opt.cpp:4292-6467 (chi3) */
 double complex orc_al_tail(double Usp, double Uch, double chi0inf, double T, double tau,

@@ -25,6 +25,7 @@ def names(qx, qy):
         "infos": "TPSC_infos_%4.2f_%5.3f_%5.3f_%5.3f_%7.5f_%6.4f_%d_%d.dat" % a8,
         "params": "TPSCparams_%4.2f_%5.3f_%5.3f_%5.3f_%7.5f_%6.4f_%d_%d.dat" % a8,
         "mu": "mu0_mu_D_%4.2f_%5.3f_%5.3f_%5.3f_%7.5f_%6.4f_%1d_%1d.dat" % a8,
+        "gloc": "Gwn_loc_%4.2f_%5.3f_%5.3f_%5.3f_%7.5f_%6.4f_%d_%d.dat" % a8,
         "kx0": "kx0_%5.3f_%5.3f_%5.3f_%5.3f_%7.5f_%6.4f_%1d_%1d.dat" % a8,
         "kx0b": "kx0_bin_%5.3f_%5.3f_%5.3f_%5.3f_%7.5f_%6.4f_%1d_%1d.dat" % a8,
         "jj1": "chijj1_qn_%5.3f_%5.3f_%5.3f_%5.3f_%7.5f_%6.4f_%1d_%1d.dat" % a8,
@@ -82,6 +83,9 @@ def test_driver_writes_the_reference_file_set(tmp_path):
     assert abs(float(params[9][1]) - g.scalar("mu")) <= 1e-12 * max(1.0, abs(g.scalar("mu")))
     mu_line = open(out / want["mu"]).read().split()
     assert abs(float(mu_line[1]) - g.scalar("mu")) <= 1e-13 * max(1.0, abs(g.scalar("mu")))
+    # local Green function (green.cpp:209-266)
+    gl = np.array([complex(float(r[1]), float(r[2])) for r in table(out / want["gloc"], 3)])
+    assert np.allclose(gl, g.Gwn_local(), rtol=1e-9, atol=1e-12)
     # Lambda_jj
     assert abs(float(open(out / want["kx0"]).read()) - g.scalar("kx0")) <= 1e-15 * abs(g.scalar("kx0"))
     assert struct.unpack("d", open(out / want["kx0b"], "rb").read())[0] == pytest.approx(g.scalar("kx0"), rel=1e-14)

@@ -238,3 +238,9 @@ def test_pade_conductivity_from_the_solve(oracle_mod):
     w_eff = np.where(w == 0.0, 1e-6, w)
     ref = oracle_mod.pade(lam_g[:16].astype(np.complex128), z, w_eff + 1j * eta).imag / w_eff
     assert relerr(sg, ref) < 1e-11
+
+
+@pytest.mark.parametrize("name", ["dop_16x128", "elec_64x256"])
+def test_gwn_local(name, oracle_mod):
+    r = run_pair(name, oracle_mod)
+    assert relerr(r["g"].Gwn_local(), r["o"].Gwn_local()) <= TIGHT

@@ -53,7 +53,7 @@ EXPORTS = [
     "tpsc_chijj", "tpsc_solve", "tpsc_get", "tpsc_field_size", "tpsc_get_scalar",
     "tpsc_prof_enable", "tpsc_prof_reset", "tpsc_prof_count", "tpsc_prof_get",
     "tpsc_keep_chi_tau", "tpsc_device_bytes", "tpsc_pade_coef", "tpsc_pade_eval", "tpsc_pade_conductivity",
-    "tpsc_get_params", "tpsc_write_files",
+    "tpsc_get_params", "tpsc_write_files", "tpsc_gwn_local",
 ]
 
 
@@ -99,6 +99,7 @@ def lib():
     L.tpsc_pade_conductivity.argtypes = [dp, C.c_int, C.c_double, C.c_int, dp, C.c_int, C.c_double, dp]
     L.tpsc_get_params.argtypes = [vp, C.POINTER(Params)]
     L.tpsc_write_files.argtypes = [vp, C.c_char_p, ip]
+    L.tpsc_gwn_local.argtypes = [vp, dp]
     _lib = L
     return L
 

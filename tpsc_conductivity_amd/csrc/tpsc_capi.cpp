@@ -53,6 +53,7 @@ struct tpsc_ctx {
     bool have_chi0 = false, have_U = false, have_V = false, have_sigma = false, have_jj = false;
     bool have_trSG = false, have_trSG2 = false;
     int al_mmin = -1;
+    double* gloc_buf = nullptr;  // k_gloc partials + result, allocated at the first tpsc_gwn_local
     bool keep_chi_tau = false;
     // device
     hipStream_t stream = nullptr;
@@ -1231,6 +1232,23 @@ int tpsc_get(tpsc_ctx* c, int field, void* dst, size_t bytes)
         return 0;
     }
     HIPCHK(c, hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int tpsc_gwn_local(tpsc_ctx* c, double* out)
+{
+    if (!c || !out) return TPSC_E_INVALID;
+    if (!c->have_sigma || !c->mu_ok) return fail(c, TPSC_E_ORDER, "G_local(): mu not computed or no self-energy");
+    const Grid& g = c->g;
+    const int H = g.Nw / 2;
+    if (!c->gloc_buf) {
+        int rc = dalloc(c, &c->gloc_buf, (size_t)(kGlocChunks + 1) * 2 * H);
+        if (rc) return rc;
+    }
+    double* res = c->gloc_buf + (size_t)kGlocChunks * 2 * H;
+    HIPCHK(c, launch_gloc(g, c->d, c->mu, c->gloc_buf, res, c->stream));
+    HIPCHK(c, hipMemcpyAsync(out, res, sizeof(double) * 2 * H, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;
 }

@@ -151,6 +151,18 @@ extern "C" int tpsc_write_files(tpsc_ctx* c, const char* dir, int* nfiles)
             f << std::setw(25) << mu0 << std::setw(25) << mu << dblOcc << '\n';
         }
     }
+    // green::Gwn_local (green.cpp:209-266), which the reference's main runs after find_mu
+    if ((stages & TPSC_STAGE_MU) && (stages & TPSC_STAGE_SIGMA)) {
+        std::vector<double> gl((size_t)nbw);
+        std::ofstream f;
+        if (tpsc_gwn_local(c, gl.data()) == TPSC_OK &&
+            w.open(f, w.path("./Gwn_loc_%4.2f_%5.3f_%5.3f_%5.3f_%7.5f_%6.4f_%d_%d.dat", U, tp, tpp, t3, density, tem,
+                             nk0, nbw))) {
+            f << std::setprecision(10) << std::setiosflags(std::ios::left);
+            for (int j = 0; j < nbw / 2; j++)
+                f << std::setw(10) << j << std::setw(25) << gl[2 * j] << gl[2 * j + 1] << std::endl;
+        }
+    }
     // cond_opt::calc_chijj_vertex_corr_all_optim (cond_opt.cpp:4195-4279)
     if (stages & TPSC_STAGE_CHIJJ) {
         const double kx0 = w.scalar(TPSC_S_KX0);

@@ -1432,6 +1432,34 @@ __global__ void k_kx0_rows(const double* __restrict__ ek, const double* __restri
     }
 }
 
+// ================================================================== local Green function
+// G_loc(iw_j) = sum over the wedge of (w_k/N^2) / (iw_j - e_k + mu - Sigma(k,iw_j)), j < Nw/2
+// (green.cpp:209-266).  Thread = frequency j (Self is point-major, so a warp reads one k's
+// row contiguously); blockIdx.y = k chunk: k = y, y + KCH, ...; partials
+// part[y*2H + 2j + (0 re, 1 im)], y < kGlocChunks, summed over y in a fixed order by
+// k_reduce_cols.
+template <typename SelfT>
+__global__ void __launch_bounds__(256)
+    k_gloc(const double* __restrict__ ekw, const double* __restrict__ wq, const SelfT* __restrict__ Self,
+           const double* __restrict__ Sinf2, const double* __restrict__ Sinf3, const double* __restrict__ Sinfp,
+           long nk8, int H, double T, double mu, double nk02, double* part)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= H) return;
+    const double wn = (2 * j + 1) * kPi * T;
+    const double Sinf = Sinfp[0];
+    double re = 0.0, im = 0.0;
+    for (long k = blockIdx.y; k < nk8; k += gridDim.y) {
+        const cd sg = sigma_kw(Self, k * H + j, Sinf, Sinf2[k], Sinf3[k], wn);
+        const cd g = (wq[k] / nk02) / cd{mu - ekw[k] - sg.x, wn - sg.y};
+        re += g.x;
+        im += g.y;
+    }
+    double* o = part + (long)blockIdx.y * 2 * H;
+    o[2 * j] = re;
+    o[2 * j + 1] = im;
+}
+
 // ================================================================== launchers
 // LDS bytes of a slab kernel: the slab (nk x nk, or (nk-2) x nk for Lambda_jj) of elem-byte
 // values, 16-byte aligned, then the N-point twiddle table of the same precision
@@ -1752,6 +1780,17 @@ hipError_t launch_jj_pencil(const Grid& g, const DevArrays& d, double mu, hipStr
                            d.twX, d.twS, d.twM, d.ph, d.jjpart, d.jjscal);
     }));
     return hipGetLastError();
+}
+
+hipError_t launch_gloc(const Grid& g, const DevArrays& d, double mu, double* part, double* out, hipStream_t s)
+{
+    const int H = g.Nw / 2;
+    const double nk02 = 4.0 * (g.nk - 1) * (g.nk - 1);
+    DISPATCH_S(g.prec, {
+        hipLaunchKernelGGL(k_gloc<ST>, dim3((H + 255) / 256, kGlocChunks), dim3(256), 0, s, d.ekw, d.wq,
+                           (const ST*)d.Self, d.Sinf2, d.Sinf3, d.Sinf, g.nk8, H, g.T, mu, nk02, part);
+    });
+    return launch_reduce_cols(part, kGlocChunks, 2 * H, out, s);
 }
 
 // [r][c] (nrows x ncols) -> [c][r]

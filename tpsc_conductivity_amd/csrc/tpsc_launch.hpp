@@ -129,6 +129,8 @@ hipError_t launch_jj_pencil(const Grid& g, const DevArrays& d, double mu, hipStr
 hipError_t launch_kx0_rows(const Grid& g, const DevArrays& d, double mu, hipStream_t s);
 hipError_t launch_finalize(const double* part, int n, double* out, hipStream_t s);
 
+constexpr int kGlocChunks = 64;  // k chunks of k_gloc (partials: kGlocChunks x Nw doubles)
+hipError_t launch_gloc(const Grid& g, const DevArrays& d, double mu, double* part, double* out, hipStream_t s);
 hipError_t launch_transpose_d(const double* in, double* out, long ncols, int nrows, hipStream_t s);
 hipError_t launch_sum_rows(const double* part, int nrows, int n, double* out, hipStream_t s);
 hipError_t launch_al_setup(const Grid& g, const DevArrays& d, const ALArrays& a, double mu0, hipStream_t s);

@@ -114,6 +114,16 @@ class CondOpt:
         return pade_conductivity(lam, self.tem, len(lam) if npade is None else npade, omega,
                                  self.eta if eta is None else eta)
 
+    def Gwn_local(self):
+        """green::Gwn_local (green.cpp:209-266): G_loc(iw_n) for n < Nw/2 (complex array)."""
+        import numpy as np
+
+        nw = int(self.scalar("nwb")) - 1
+        out = np.zeros(nw, dtype=np.complex128)
+        self.ctx.check(lib().tpsc_gwn_local(self.ctx.handle, out.ctypes.data_as(C.POINTER(C.c_double))),
+                       "Gwn_local")
+        return out
+
     def write_files(self, directory=None):
         """Write the reference's text outputs (TPSCparams_*, TPSC_infos_*, chijj*_qn_*, kx0_*, ...)
         for the stages run so far into directory (default: the current directory)."""
