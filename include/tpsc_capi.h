@@ -44,6 +44,8 @@ typedef struct tpsc_params {
                         reference does (chi.cpp:285-310); 0: full grid only      */
     int precision;   /* TPSC_PREC_FP64 or TPSC_PREC_MIXED (no reference counterpart:
                         the reference computes in FP64 only)                     */
+    double t;        /* nearest-neighbour hopping; 0 means 1, the reference's fixed value
+                        (hamiltonien.cpp:26,121).  A free t makes d/dt defined.  */
 } tpsc_params;
 
 /* precision values.  MIXED: the slab transforms (2D DCT/DST per tau or iqn slice) and their
@@ -155,6 +157,25 @@ int tpsc_solve(tpsc_ctx* ctx);
  * G_loc(iw_n) = (1/N^2) sum_k 1/(iw_n - e_k + mu - Sigma(k,iw_n)), n = 0..Nw/2-1, written to out
  * as Nw/2 complex values interleaved re/im.  Needs the self-energy and mu (TPSC_E_ORDER). */
 int tpsc_gwn_local(tpsc_ctx* ctx, double* out);
+
+/* mu0 as an input parameter: every later stage uses this mu0 instead of solving the density
+ * equation (green0.cpp:240-262); NaN returns to the solved mu0.  Resets the computed stages.
+ * No reference counterpart: it makes d/dmu0 a partial derivative at fixed n. */
+int tpsc_fix_mu0(tpsc_ctx* ctx, double mu0);
+
+/* Gradient of L = sum over the packed wedge of w_k Re Sigma(k, i w_0) (FP64 path only) with
+ * respect to (t, t', mu0, U), as a forward / backward pair.  No reference counterpart.
+ * forward: runs calc_Self_FFT_spline (chi0 -> Usp, Uch -> V -> Sigma, green.cpp:1673-2031) at the
+ *   context's parameters (mu0 fixed if tpsc_fix_mu0 was called) and returns L; w: nk8 weights.
+ * backward: grad[4] = dloss * dL/d(t, t', mu0, U) by tangent-linear propagation through the
+ *   same chain: one pass per parameter, the tau slabs recomputed instead of stored, the
+ *   Usp / Uch roots differentiated implicitly through their sum rules (chi.cpp:833-843). */
+int tpsc_sigma_loss_forward(tpsc_ctx* ctx, const double* w, double* loss);
+int tpsc_sigma_loss_backward(tpsc_ctx* ctx, double dloss, double* grad);
+
+/* n weights u_k ~ U(0,1) from std::mt19937_64(seed) and std::uniform_real_distribution
+ * (the loss weights of the gradient tests); needs no context */
+int tpsc_random_weights(unsigned long long seed, size_t n, double* out);
 
 /* the parameters the context was created with */
 int tpsc_get_params(tpsc_ctx* ctx, tpsc_params* out);

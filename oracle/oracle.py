@@ -25,7 +25,7 @@ class OrcParams(C.Structure):
     _fields_ = [
         ("nk0", C.c_int), ("Nw0", C.c_int), ("n", C.c_double), ("T", C.c_double),
         ("eta", C.c_double), ("tp", C.c_double), ("tpp", C.c_double), ("t3", C.c_double),
-        ("U", C.c_double), ("coarse_pass", C.c_int),
+        ("U", C.c_double), ("coarse_pass", C.c_int), ("t", C.c_double),
     ]
 
 
@@ -56,6 +56,7 @@ class Orc(C.Structure):
         ("chijj1_k", _dp), ("chijj2_k", _dp),
         ("al_N", C.c_int), ("al_N_theta", C.c_int), ("al_freq", C.POINTER(C.c_long)),
         ("chijj3", _dp), ("chijj_all", _dp), ("chijj3_k", _dp), ("al_hw", _dp), ("al_hg2", _dp),
+        ("mu0_fixed", C.c_int),
     ]
 
 
@@ -81,6 +82,9 @@ def lib():
         L.orc_chi0.restype = C.c_int
         L.orc_chijj.argtypes = [P, C.c_int]
         L.orc_chijj.restype = C.c_int
+        L.orc_fix_mu0.argtypes = [P, C.c_double]
+        L.orc_sigma_loss.argtypes = [P, _dp]
+        L.orc_sigma_loss.restype = C.c_double
         L.orc_gwn_local.argtypes = [P, _dp]
         L.orc_gwn_local.restype = C.c_int
         L.orc_chijj_al.argtypes = [P, C.c_int, C.c_int]
@@ -128,9 +132,11 @@ def _arr(ptr, n, dtype=np.float64):
 class Oracle:
     """One parameter point of the TPSC chain, mirroring the reference's object methods."""
 
-    def __init__(self, N, Nw, n, T, U, tp=0.0, tpp=0.0, t3=0.0, eta=0.1, coarse_pass=True):
-        self.params = OrcParams(N, Nw, n, T, eta, tp, tpp, t3, U, int(coarse_pass))
+    def __init__(self, N, Nw, n, T, U, tp=0.0, tpp=0.0, t3=0.0, eta=0.1, coarse_pass=True, t=1.0, mu0=None):
+        self.params = OrcParams(N, Nw, n, T, eta, tp, tpp, t3, U, int(coarse_pass), t)
         self._o = lib().orc_create(C.byref(self.params))
+        if mu0 is not None:
+            lib().orc_fix_mu0(self._o, mu0)
 
     def __del__(self):
         try:
@@ -201,6 +207,12 @@ class Oracle:
     def calc_chijj_al(self, mmin, N_theta=1):
         """Aslamazov-Larkin Lambda_jj3 on the sparse boson set (cond_opt.cpp:4292-6467)."""
         return lib().orc_chijj_al(self._o, mmin, N_theta)
+
+    def sigma_loss(self, w):
+        """sum_k w_k Re Sigma(k, i w_0) after chi0 -> Usp, Uch -> V -> Sigma (NaN if a root fails)."""
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        assert w.size == self.nk8
+        return lib().orc_sigma_loss(self._o, w.ctypes.data_as(_dp))
 
     def Gwn_local(self):
         """G_loc(iw_n), n < Nw/2 (green.cpp:209-266)."""

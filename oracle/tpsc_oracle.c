@@ -77,22 +77,23 @@ static void sws_free(sws *w)
 }
 
 /* ---------------------------------------------------------------- model (a1) */
+static inline double hop_t(const orc *o) { return o->p.t != 0.0 ? o->p.t : 1.0; }
 double orc_dispk(const orc *o, double kx, double ky)
 {
-    const double tp = o->p.tp, tpp = o->p.tpp, t3 = o->p.t3;
-    return -2.0 * (cos(kx) + cos(ky)) - 4.0 * tp * cos(kx) * cos(ky) - 2.0 * tpp * (cos(2 * kx) + cos(2 * ky)) -
+    const double tp = o->p.tp, tpp = o->p.tpp, t3 = o->p.t3, t = hop_t(o);
+    return -2.0 * t * (cos(kx) + cos(ky)) - 4.0 * tp * cos(kx) * cos(ky) - 2.0 * tpp * (cos(2 * kx) + cos(2 * ky)) -
            4 * t3 * (cos(2 * kx) * cos(ky) + cos(2 * ky) * cos(kx));
 }
 double orc_grad_ek_x(const orc *o, double kx, double ky)
 {
-    const double tp = o->p.tp, tpp = o->p.tpp, t3 = o->p.t3;
-    return 2.0 * sin(kx) + 4.0 * tp * sin(kx) * cos(ky) + 4.0 * tpp * sin(2 * kx) +
+    const double tp = o->p.tp, tpp = o->p.tpp, t3 = o->p.t3, t = hop_t(o);
+    return 2.0 * t * sin(kx) + 4.0 * tp * sin(kx) * cos(ky) + 4.0 * tpp * sin(2 * kx) +
            4 * t3 * (2.0 * sin(2 * kx) * cos(ky) + cos(2 * ky) * sin(kx));
 }
 double orc_hess_ek_xx(const orc *o, double kx, double ky)
 {
-    const double tp = o->p.tp, tpp = o->p.tpp, t3 = o->p.t3;
-    return 2.0 * cos(kx) + 4.0 * tp * cos(kx) * cos(ky) + 8.0 * tpp * cos(2 * kx) +
+    const double tp = o->p.tp, tpp = o->p.tpp, t3 = o->p.t3, t = hop_t(o);
+    return 2.0 * t * cos(kx) + 4.0 * tp * cos(kx) * cos(ky) + 8.0 * tpp * cos(2 * kx) +
            4 * t3 * (4.0 * cos(2 * kx) * cos(ky) + cos(2 * ky) * cos(kx));
 }
 
@@ -112,7 +113,7 @@ static void calc_ek(orc *o)
     }
     const long nk8 = (long)nbk * (nbk + 1) / 2;
     o->er = dcalloc((size_t)(nk8 > 5 ? nk8 : 5));
-    o->er[1] = -1;
+    o->er[1] = -hop_t(o);
     o->er[2] = -o->p.tp;
     o->er[3] = -o->p.tpp;
     o->er[4] = -o->p.t3;
@@ -139,6 +140,7 @@ static double fz_density(void *ctx, double x) { return orc_ddensity_mu_PBC((orc 
 /* green0.cpp:240-262 */
 int orc_find_mu0(orc *o)
 {
+    if (o->mu0_fixed) return 1;
     if (o->p.n == 1.0 && o->p.tp == 0 && o->p.tpp == 0) {
         o->mu0 = 0.0;
         return 1;
@@ -1949,4 +1951,31 @@ int orc_gwn_local(orc *o, double complex *out)
         out[j] = g;
     }
     return 0;
+}
+
+/* ---------------------------------------------------------------- loss of the gradient tests */
+/* mu0 becomes an input: every later set_grid keeps it instead of solving the density */
+void orc_fix_mu0(orc *o, double mu0)
+{
+    o->mu0_fixed = 1;
+    o->mu0 = mu0;
+}
+
+/* L = sum over the packed wedge of w_k Re Sigma(k, i w_0), Sigma(k,iw_n) = Sigma_inf/z + C2/z^2
+   + C3/z^3 + R/z^4 (green.cpp:3039), after the chain chi0 -> Usp, Uch -> V -> Sigma
+   (chi.cpp:210-417, 37-206; green.cpp:1673-2031).  Returns NAN if a root is not found. */
+double orc_sigma_loss(orc *o, const double *w)
+{
+    if (!orc_find_Usp_Uch(o)) return NAN;
+    orc_vertex(o);
+    orc_self_energy(o);
+    const long nk8 = (long)o->nbk * (o->nbk + 1) / 2;
+    const double complex z = I * (PI * o->p.T);
+    double L = 0.0;
+    for (long k = 0; k < nk8; k++) {
+        const double complex s = o->Sigma_inf / z + o->Sigma_inf2[k] / (z * z) + o->Sigma_inf3[k] / (z * z * z) +
+                                 o->Self[k] / (z * z * z * z);
+        L += w[k] * creal(s);
+    }
+    return L;
 }

@@ -36,6 +36,8 @@ typedef struct orc_params {
     double eta;       /* broadening (kept for interface parity; unused here)  */
     double tp, tpp, t3, U;
     int coarse_pass;  /* 1: run the coarse (N/2, Nw/2) Usp/Uch pass first    */
+    double t;         /* nearest-neighbour hopping; 0 means 1 (the reference's fixed
+                         value, hamiltonien.cpp:26,121)                        */
 } orc_params;
 
 typedef struct orc {
@@ -75,6 +77,7 @@ typedef struct orc {
     double *chijj3, *chijj_all;     /* al_N each: Lambda_jj3, Lambda_jj1+2+3       */
     double *chijj3_k;               /* al_N*al_N_theta angle-binned Lambda_jj3     */
     double *al_hw, *al_hg2;         /* hG(q,iq_p) p < Nw/2 ([q + p*nq]), 1/qn^2 coefficient */
+    int mu0_fixed;                  /* 1: mu0 is an input (orc_fix_mu0), not solved from n */
 } orc;
 
 /* ---- lifecycle / model (hamiltonien.cpp:14-34,49-68; green0.cpp:13-37) ---- */
@@ -101,6 +104,8 @@ int orc_trace_self_G2(orc *o);                          /* green.cpp:2929-2997 *
 int orc_chijj(orc *o, int N_theta);                     /* cond_opt.cpp:2537-4290 */
 int orc_solve(orc *o);                                  /* tpsc_main.cpp:61-75 + E4 */
 int orc_gwn_local(orc *o, double complex *out);          /* green.cpp:209-266 */
+void orc_fix_mu0(orc *o, double mu0);                    /* mu0 as an input parameter   */
+double orc_sigma_loss(orc *o, const double *w);          /* sum_k w_k Re Sigma(k,iw_0)  */
 int orc_al_freqs(int nbw, int mmin, long *out);         /* cond_opt.cpp:5123-5156 */
 int orc_chijj_al(orc *o, int mmin, int N_theta);        /* cond_opt.cpp:4292-6467 (chi3) */
 double complex orc_al_tail(double Usp, double Uch, double chi0inf, double T, double tau,

@@ -282,3 +282,21 @@ def test_coarse_pass_only_changes_diagnostics(oracle_mod):
     sa, sb = a.scalars(), b.scalars()
     assert sa["Usp"] == sb["Usp"] and sa["Uch"] == sb["Uch"] and sa["mu0"] == sb["mu0"]
     assert sa["difRelsp"] > 0 and sa["Usp_coarse"] != sa["Usp"]
+
+
+def test_hopping_t_is_an_energy_scale(oracle_mod):
+    """eps(k; t, t') = t eps(k; 1, t'/t), and the whole chain is homogeneous in its energies:
+    Sigma at (t, t', U, T, mu0) is t x Sigma at (1, t'/t, U/t, T/t, mu0/t) on the same Matsubara
+    index, so L = sum_k w_k Re Sigma(k, i w_0) scales the same way.  Pins the generalization of
+    eps(k), eps(r) (er[1] = -t) and the derivatives to a free hopping t."""
+    import numpy as np
+
+    N, Nw, n, T, U, tp, mu0, t = 16, 128, 0.9, 0.15, 2.5, -0.2, -0.3, 1.3
+    w = np.linspace(0.2, 1.0, (N // 2 + 1) * (N // 2 + 2) // 2)
+    a = oracle_mod.Oracle(N, Nw, n, T, U, tp=tp, coarse_pass=False, t=t, mu0=mu0).sigma_loss(w)
+    b = oracle_mod.Oracle(N, Nw, n, T / t, U / t, tp=tp / t, coarse_pass=False, t=1.0, mu0=mu0 / t).sigma_loss(w)
+    assert abs(a - t * b) <= 1e-9 * abs(a)
+    # t = 0 in the parameters means the reference's t = 1
+    c = oracle_mod.Oracle(N, Nw, n, T, U, tp=tp, coarse_pass=False, t=0.0, mu0=mu0).sigma_loss(w)
+    d = oracle_mod.Oracle(N, Nw, n, T, U, tp=tp, coarse_pass=False, t=1.0, mu0=mu0).sigma_loss(w)
+    assert c == d

@@ -40,7 +40,7 @@ class Params(C.Structure):
     _fields_ = [
         ("nk0", C.c_int), ("Nw0", C.c_int), ("n", C.c_double), ("T", C.c_double),
         ("eta", C.c_double), ("tp", C.c_double), ("tpp", C.c_double), ("t3", C.c_double),
-        ("U", C.c_double), ("coarse_pass", C.c_int), ("precision", C.c_int),
+        ("U", C.c_double), ("coarse_pass", C.c_int), ("precision", C.c_int), ("t", C.c_double),
     ]
 
 
@@ -53,7 +53,8 @@ EXPORTS = [
     "tpsc_chijj", "tpsc_solve", "tpsc_get", "tpsc_field_size", "tpsc_get_scalar",
     "tpsc_prof_enable", "tpsc_prof_reset", "tpsc_prof_count", "tpsc_prof_get",
     "tpsc_keep_chi_tau", "tpsc_device_bytes", "tpsc_pade_coef", "tpsc_pade_eval", "tpsc_pade_conductivity",
-    "tpsc_get_params", "tpsc_write_files", "tpsc_gwn_local",
+    "tpsc_get_params", "tpsc_write_files", "tpsc_gwn_local", "tpsc_fix_mu0", "tpsc_sigma_loss_forward",
+    "tpsc_sigma_loss_backward", "tpsc_random_weights",
 ]
 
 
@@ -100,6 +101,10 @@ def lib():
     L.tpsc_get_params.argtypes = [vp, C.POINTER(Params)]
     L.tpsc_write_files.argtypes = [vp, C.c_char_p, ip]
     L.tpsc_gwn_local.argtypes = [vp, dp]
+    L.tpsc_fix_mu0.argtypes = [vp, C.c_double]
+    L.tpsc_sigma_loss_forward.argtypes = [vp, dp, dp]
+    L.tpsc_sigma_loss_backward.argtypes = [vp, C.c_double, dp]
+    L.tpsc_random_weights.argtypes = [C.c_ulonglong, C.c_size_t, dp]
     _lib = L
     return L
 
@@ -113,9 +118,9 @@ def exported_symbols():
 class Context:
     """Owns one tpsc_ctx (device buffers + stream for one parameter point)."""
 
-    def __init__(self, nk0, Nw0, n, T, U, tp=0.0, tpp=0.0, t3=0.0, eta=0.1, coarse_pass=True, precision=0):
+    def __init__(self, nk0, Nw0, n, T, U, tp=0.0, tpp=0.0, t3=0.0, eta=0.1, coarse_pass=True, precision=0, t=1.0):
         self._h = C.c_void_p(None)
-        self.params = Params(nk0, Nw0, n, T, eta, tp, tpp, t3, U, int(bool(coarse_pass)), precision)
+        self.params = Params(nk0, Nw0, n, T, eta, tp, tpp, t3, U, int(bool(coarse_pass)), precision, t)
         rc = lib().tpsc_create(C.byref(self.params), C.byref(self._h))
         if rc != TPSC_OK:
             msg = self.last_error()
@@ -215,3 +220,12 @@ def pade_conductivity(lam, T, npade, omega, eta):
     if rc:
         raise TPSCError(f"tpsc_pade_conductivity failed ({ERRORS.get(rc, rc)})")
     return sig
+
+
+def random_weights(seed, n):
+    """n weights ~ U(0,1) from std::mt19937_64(seed) (tpsc_random_weights)."""
+    out = np.zeros(n)
+    rc = lib().tpsc_random_weights(seed, n, out.ctypes.data_as(C.POINTER(C.c_double)))
+    if rc:
+        raise TPSCError(f"tpsc_random_weights failed ({ERRORS.get(rc, rc)})")
+    return out

@@ -12,6 +12,7 @@
 #include <cstring>
 #include <functional>
 #include <map>
+#include <random>
 #include <string>
 #include <vector>
 
@@ -54,6 +55,9 @@ struct tpsc_ctx {
     bool have_trSG = false, have_trSG2 = false;
     int al_mmin = -1;
     double* gloc_buf = nullptr;  // k_gloc partials + result, allocated at the first tpsc_gwn_local
+    bool mu0_fixed = false;      // tpsc_fix_mu0: mu0 is an input, not solved from n
+    TanArrays tan{};             // tangent arrays of the Sigma-loss gradient
+    bool grad_ready = false;     // forward of the Sigma loss done, weights on the device
     bool keep_chi_tau = false;
     // device
     hipStream_t stream = nullptr;
@@ -77,6 +81,9 @@ struct tpsc_ctx {
 };
 
 namespace {
+
+// nearest-neighbour hopping: tpsc_params.t, with 0 meaning the reference's fixed value 1
+double hop_t(const tpsc_ctx* c) { return c->p.t != 0.0 ? c->p.t : 1.0; }
 
 int fail(tpsc_ctx* c, int code, const std::string& msg)
 {
@@ -382,6 +389,7 @@ int join_side(tpsc_ctx* c)
 // green0::find_mu (green0.cpp:240-262) with the density evaluated on the device
 int find_mu0(tpsc_ctx* c)
 {
+    if (c->mu0_fixed) return 0;
     if (c->p.n == 1.0 && c->p.tp == 0 && c->p.tpp == 0) {
         c->mu0 = 0.0;
         return 0;
@@ -423,7 +431,7 @@ int set_grid(tpsc_ctx* c, int N, int Nw)
     g.T = c->p.T;
     g.prec = c->p.precision;
     hipStream_t s = c->stream;
-    HIPCHK(c, launch_model(g, c->d, c->p.tp, c->p.tpp, c->p.t3, s));
+    HIPCHK(c, launch_model(g, c->d, hop_t(c), c->p.tp, c->p.tpp, c->p.t3, s));
     HIPCHK(c, launch_fill_tw(c->d, Nw, s));
     HIPCHK(c, hipStreamSynchronize(s));
     c->have_chi0 = false;
@@ -856,6 +864,130 @@ int stage_al(tpsc_ctx* c, int mmin)
     return 0;
 }
 
+// ------------------------------------------------------------------ Sigma-loss gradient
+int tan_alloc(tpsc_ctx* c)
+{
+    TanArrays& t = c->tan;
+    if (t.chi) return 0;
+    const Grid& g = c->g;
+    const size_t nk8 = (size_t)g.nk8, nwb = (size_t)g.nwb;
+    const size_t nblk = (size_t)((g.nk8 + 255) / 256) * kRowGroups;
+    int rc = 0;
+    rc |= dalloc(c, &t.dek, (size_t)g.nk * g.nk);
+    rc |= dalloc(c, &t.dekw, nk8);
+    rc |= dalloc(c, &t.der, nk8);
+    rc |= dalloc(c, &t.chi, nwb * nk8);
+    rc |= dalloc(c, &t.chirt, nwb * nk8);
+    rc |= dalloc(c, &t.chirt0, nk8);
+    rc |= dalloc(c, &t.dchirt0, nk8);
+    rc |= dalloc(c, &t.dGrt0p, nk8);
+    rc |= dalloc(c, &t.dchiqt0, nk8);
+    rc |= dalloc(c, &t.chi0inf, nk8);
+    rc |= dalloc(c, &t.chi0inf2, nk8);
+    rc |= dalloc(c, &t.V, nwb * nk8);
+    rc |= dalloc(c, &t.V0, nk8);
+    rc |= dalloc(c, &t.dSig, 2 * nk8);
+    rc |= dalloc(c, &t.Sb, 2 * nk8);
+    rc |= dalloc(c, &t.Stau, (size_t)(g.Nw + 1) * nk8);
+    rc |= dalloc(c, &t.Self, (size_t)(g.Nw / 2) * nk8);
+    rc |= dalloc(c, &t.Sinf2, nk8);
+    rc |= dalloc(c, &t.Sinf3, nk8);
+    rc |= dalloc(c, &t.Sinf4, nk8);
+    rc |= dalloc(c, &t.Sinf, 1);
+    rc |= dalloc(c, &t.scr, 6 * nk8);
+    rc |= dalloc(c, &t.part, 4 * nblk);
+    rc |= dalloc(c, &t.red, 8);
+    rc |= dalloc(c, &t.w, nk8);
+    return rc;
+}
+
+// tangent of eps(k) and eps(r) for parameter p (0: t, 1: t', 2: mu0, 3: U) on the quadrant
+// and the wedge; eps = -2t(cx+cy) - 4t' cx cy - 2t''(c2x+c2y) - 4t3(c2x cy + c2y cx),
+// eps(r) = -t at r = (1,0), -t' at (1,1) (hamiltonien.cpp:22-29,119-122)
+void tangent_model(const Grid& g, int p, std::vector<double>& dek, std::vector<double>& dekw, std::vector<double>& der)
+{
+    const int nk = g.nk;
+    dek.assign((size_t)nk * nk, 0.0);
+    dekw.assign((size_t)g.nk8, 0.0);
+    der.assign((size_t)g.nk8, 0.0);
+    if (p > 1) return;
+    const double dk = M_PI / (nk - 1);
+    for (int l = 0; l < nk; l++)
+        for (int m = 0; m < nk; m++) {
+            const double cx = cos(l * dk), cy = cos(m * dk);
+            const double v = (p == 0) ? -2.0 * (cx + cy) : -4.0 * cx * cy;
+            dek[(size_t)m + (size_t)l * nk] = v;
+            if (m <= l) dekw[(size_t)m + (size_t)l * (l + 1) / 2] = v;
+        }
+    if (g.nk8 > 2) der[(p == 0) ? 1 : 2] = -1.0;
+}
+
+// sum_k w_k Re Sigma(k, i w_0) from the current (FP64) Sigma
+int sigma_loss_value(tpsc_ctx* c, const double* Sinf2, const cd* Self, double* out)
+{
+    HIPCHK(c, launch_sigma_loss(c->g, c->tan.w, Sinf2, Self, c->tan.red, c->stream));
+    HIPCHK(c, hipMemcpyAsync(out, c->tan.red, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+// one tangent pass of the chain for parameter p; *dL receives d/dp sum_k w_k Re Sigma(k, iw_0)
+int tangent_pass(tpsc_ctx* c, int p, double* dL)
+{
+    const Grid& g = c->g;
+    TanArrays& t = c->tan;
+    hipStream_t s = c->stream;
+    std::vector<double> dek, dekw, der;
+    tangent_model(g, p, dek, dekw, der);
+    HIPCHK(c, hipMemcpy(t.dek, dek.data(), sizeof(double) * dek.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(t.dekw, dekw.data(), sizeof(double) * dekw.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(t.der, der.data(), sizeof(double) * der.size(), hipMemcpyHostToDevice));
+    const double dmu0 = (p == 2) ? 1.0 : 0.0, dU = (p == 3) ? 1.0 : 0.0;
+    const double U = c->p.U, Usp = c->Usp, Uch = c->Uch, nsr = c->nsr;
+    // chi0 -> tangent chi0(q,tau), chi0(r,tau) and tau = 0 data, then the (linear) pencil
+    HIPCHK(c, launch_chi0_tan(g, c->d, t, c->mu0, dmu0, s));
+    DevArrays v = c->d;
+    v.chi = t.chi;
+    v.chiw = t.chi;
+    v.dchiqt0 = t.dchiqt0;
+    v.chi0inf = t.chi0inf;
+    v.chi0inf2 = t.chi0inf2;
+    HIPCHK(c, launch_chi0_pencil(g, v, s));
+    // implicit derivatives of the sum-rule roots (chi.cpp:833-843)
+    double A[4];
+    HIPCHK(c, launch_sum_rule_tan(g, c->d, t, Usp, Uch, t.red, s));
+    HIPCHK(c, hipMemcpyAsync(A, t.red, sizeof A, hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    const double k2 = nsr * nsr / (2.0 * U);
+    const double dUsp = (Usp * k2 / U * dU - A[0]) / (A[1] + k2);
+    const double dUch = -(A[2] - k2 * dUsp + Usp * k2 / U * dU) / A[3];
+    // V: tangent integrand, the linear tau transform, then + dfact chi0(r,tau)
+    const double fact = U * (3.0 * Usp + Uch) / 8.0;
+    const double dfact = (dU * (3.0 * Usp + Uch) + U * (3.0 * dUsp + dUch)) / 8.0;
+    HIPCHK(c, launch_vertex_slab_tan(g, c->d, t, U, dU, Usp, dUsp, Uch, dUch, s));
+    v = c->d;
+    v.V = t.V;
+    v.chirt = t.chirt;
+    v.chirt0 = t.chirt0;
+    v.V0 = t.V0;
+    HIPCHK(c, launch_vertex_pencil(g, v, U, Usp, Uch, s));
+    HIPCHK(c, launch_axpy((long)g.nwb * g.nk8, dfact, c->d.chirt, t.V, s));
+    HIPCHK(c, launch_axpy(g.nk8, dfact, c->d.chirt0, t.V0, s));
+    // Sigma: boundary data and slabs, then the (linear) pencil
+    HIPCHK(c, launch_sigma_tan(g, c->d, t, c->mu0, dmu0, fact, dfact, s));
+    v = c->d;
+    v.Stau = t.Stau;
+    v.Sb = t.Sb;
+    v.dSig = t.dSig;
+    v.Self = t.Self;
+    v.Sinf2 = t.Sinf2;
+    v.Sinf3 = t.Sinf3;
+    v.Sinf4 = t.Sinf4;
+    v.Sinf = t.Sinf;
+    HIPCHK(c, launch_sigma_pencil(g, v, s));
+    return sigma_loss_value(c, t.Sinf2, t.Self, dL);
+}
+
 size_t field_count(tpsc_ctx* c, int field)
 {
     const Grid& g = c->g;
@@ -1233,6 +1365,58 @@ int tpsc_get(tpsc_ctx* c, int field, void* dst, size_t bytes)
     }
     HIPCHK(c, hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int tpsc_fix_mu0(tpsc_ctx* c, double mu0)
+{
+    if (!c) return TPSC_E_INVALID;
+    if (std::isnan(mu0)) {
+        c->mu0_fixed = false;
+        return find_mu0(c);
+    }
+    c->mu0_fixed = true;
+    c->mu0 = mu0;
+    c->have_chi0 = c->have_U = c->have_V = c->have_sigma = c->have_jj = false;
+    c->mu_ok = false;
+    c->grad_ready = false;
+    return 0;
+}
+
+int tpsc_sigma_loss_forward(tpsc_ctx* c, const double* w, double* loss)
+{
+    if (!c || !w || !loss) return TPSC_E_INVALID;
+    if (c->p.precision != TPSC_PREC_FP64) return fail(c, TPSC_E_INVALID, "the gradient runs in the FP64 path only");
+    c->grad_ready = false;
+    int rc = tpsc_calc_self_energy_full(c);
+    if (rc) return rc;
+    if ((rc = tan_alloc(c))) return rc;
+    HIPCHK(c, hipMemcpy(c->tan.w, w, sizeof(double) * c->g.nk8, hipMemcpyHostToDevice));
+    if ((rc = sigma_loss_value(c, c->d.Sinf2, c->d.Self, loss))) return rc;
+    c->grad_ready = true;
+    return 0;
+}
+
+int tpsc_sigma_loss_backward(tpsc_ctx* c, double dloss, double* grad)
+{
+    if (!c || !grad) return TPSC_E_INVALID;
+    if (!c->grad_ready || !c->have_sigma)
+        return fail(c, TPSC_E_ORDER, "tpsc_sigma_loss_backward needs tpsc_sigma_loss_forward first");
+    for (int p = 0; p < 4; p++) {
+        double dL = 0.0;
+        int rc = tangent_pass(c, p, &dL);
+        if (rc) return rc;
+        grad[p] = dloss * dL;
+    }
+    return 0;
+}
+
+int tpsc_random_weights(unsigned long long seed, size_t n, double* out)
+{
+    if (!out) return TPSC_E_INVALID;
+    std::mt19937_64 gen(seed);
+    std::uniform_real_distribution<double> u(0.0, 1.0);
+    for (size_t i = 0; i < n; i++) out[i] = u(gen);
     return 0;
 }
 

@@ -825,8 +825,8 @@ __global__ void __launch_bounds__(1024) k_reduce_cols(const double* __restrict__
 }
 
 // dispersion tables on the device (hamiltonien.cpp:49-68,125-155; green0 wedge weights)
-__global__ void k_model(int N, double tp, double tpp, double t3, double* ek, double* ekw, double* wq, double* er,
-                        double* d2ek, double* dek)
+__global__ void k_model(int N, double t, double tp, double tpp, double t3, double* ek, double* ekw, double* wq,
+                        double* er, double* d2ek, double* dek)
 {
     const int nk = N / 2 + 1;
     const long nk8 = (long)nk * (nk + 1) / 2, nq = (long)nk * (nk - 2);
@@ -836,8 +836,8 @@ __global__ void k_model(int N, double tp, double tpp, double t3, double* ek, dou
         const int l = (int)(i / nk), m = (int)(i % nk);
         const double kx = l * dk, ky = m * dk;
         const double cx = cos(kx), cy = cos(ky), c2x = cos(2 * kx), c2y = cos(2 * ky);
-        ek[i] = -2.0 * (cx + cy) - 4.0 * tp * cx * cy - 2.0 * tpp * (c2x + c2y) - 4 * t3 * (c2x * cy + c2y * cx);
-        d2ek[i] = 2.0 * cx + 4.0 * tp * cx * cy + 8.0 * tpp * c2x + 4 * t3 * (4.0 * c2x * cy + c2y * cx);
+        ek[i] = -2.0 * t * (cx + cy) - 4.0 * tp * cx * cy - 2.0 * tpp * (c2x + c2y) - 4 * t3 * (c2x * cy + c2y * cx);
+        d2ek[i] = 2.0 * t * cx + 4.0 * tp * cx * cy + 8.0 * tpp * c2x + 4 * t3 * (4.0 * c2x * cy + c2y * cx);
         if (m <= l) {
             const long w = widx(l, m);
             ekw[w] = ek[i];
@@ -846,13 +846,13 @@ __global__ void k_model(int N, double tp, double tpp, double t3, double* ek, dou
             if (m == 0 || m == nk - 1) ww /= 2;
             if (m == l) ww /= 2;
             wq[w] = ww;
-            er[w] = (w == 1) ? -1.0 : (w == 2) ? -tp : (w == 3) ? -tpp : (w == 4) ? -t3 : 0.0;
+            er[w] = (w == 1) ? -t : (w == 2) ? -tp : (w == 3) ? -tpp : (w == 4) ? -t3 : 0.0;
         }
     }
     if (i < nq) {
         const int l = (int)(i / nk), m = (int)(i % nk);
         const double kx = (l + 1) * dk, ky = m * dk;
-        dek[i] = 2.0 * sin(kx) + 4.0 * tp * sin(kx) * cos(ky) + 4.0 * tpp * sin(2 * kx) +
+        dek[i] = 2.0 * t * sin(kx) + 4.0 * tp * sin(kx) * cos(ky) + 4.0 * tpp * sin(2 * kx) +
                  4 * t3 * (2.0 * sin(2 * kx) * cos(ky) + cos(2 * ky) * sin(kx));
     }
     (void)nk8;
@@ -1607,10 +1607,11 @@ hipError_t launch_sum_rule3(const Grid& g, const DevArrays& d, double sgn, doubl
     return hipGetLastError();
 }
 
-hipError_t launch_model(const Grid& g, const DevArrays& d, double tp, double tpp, double t3, hipStream_t s)
+hipError_t launch_model(const Grid& g, const DevArrays& d, double t, double tp, double tpp, double t3,
+                        hipStream_t s)
 {
     const long n = std::max<long>((long)g.nk * g.nk, g.nq);
-    hipLaunchKernelGGL(k_model, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g.N, tp, tpp, t3, d.ek, d.ekw,
+    hipLaunchKernelGGL(k_model, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g.N, t, tp, tpp, t3, d.ek, d.ekw,
                        d.wq, d.er, d.d2ek, d.dek);
     return hipGetLastError();
 }

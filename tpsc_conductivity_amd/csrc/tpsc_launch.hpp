@@ -84,6 +84,36 @@ struct ALArrays {
     int N = 0, Nw = 0;          // grid the arrays were sized for
 };
 
+// Tangent arrays of the chain eps, mu0 -> chi0 -> Usp, Uch -> V -> Sigma (FP64; allocated at
+// the first gradient call).  Layouts are those of the forward arrays of the same name.
+struct TanArrays {
+    double* dek = nullptr;     // nk*nk   d eps(k) on the quadrant
+    double* dekw = nullptr;    // nk8     d eps(k) on the wedge
+    double* der = nullptr;     // nk8     d eps(r)
+    double* chi = nullptr;     // nwb*nk8 dchi0(q,tau), then dchi0(q,iqn) in place
+    double* chirt = nullptr;   // nwb*nk8 dchi0(r,tau)
+    double* chirt0 = nullptr;  // nk8
+    double* dchirt0 = nullptr; // nk8
+    double* dGrt0p = nullptr;  // nk8
+    double* dchiqt0 = nullptr; // nk8
+    double* chi0inf = nullptr; // nk8
+    double* chi0inf2 = nullptr;// nk8
+    double* V = nullptr;       // nwb*nk8
+    double* V0 = nullptr;      // nk8
+    double* dSig = nullptr;    // 2*nk8
+    double* Sb = nullptr;      // 2*nk8
+    double* Stau = nullptr;    // (Nw+1)*nk8
+    cd* Self = nullptr;        // nk8*(Nw/2)
+    double* Sinf2 = nullptr;   // nk8
+    double* Sinf3 = nullptr;
+    double* Sinf4 = nullptr;
+    double* Sinf = nullptr;    // 1
+    double* scr = nullptr;     // 6*nk8
+    double* part = nullptr;    // 4 * sum-rule grid
+    double* red = nullptr;     // 8
+    double* w = nullptr;       // nk8 loss weights
+};
+
 struct Grid {
     int N, nk, Nw, nwb;
     long nk8, nq;
@@ -109,7 +139,8 @@ hipError_t launch_sum_rule3(const Grid& g, const DevArrays& d, double sgn, doubl
 hipError_t launch_trace_diffG3(const Grid& g, const DevArrays& d, double pmu, double mu0, double* out,
                                hipStream_t s);
 hipError_t launch_density(const Grid& g, const DevArrays& d, double mu, double n, double* out, hipStream_t s);
-hipError_t launch_model(const Grid& g, const DevArrays& d, double tp, double tpp, double t3, hipStream_t s);
+hipError_t launch_model(const Grid& g, const DevArrays& d, double t, double tp, double tpp, double t3,
+                        hipStream_t s);
 hipError_t launch_reduce_cols(const double* part, int nrows, int ncols, double* out, hipStream_t s);
 hipError_t launch_vertex_slab(const Grid& g, const DevArrays& d, double U, double Usp, double Uch, hipStream_t s);
 hipError_t launch_vertex_pencil(const Grid& g, const DevArrays& d, double U, double Usp, double Uch, hipStream_t s);
@@ -131,6 +162,17 @@ hipError_t launch_finalize(const double* part, int n, double* out, hipStream_t s
 
 constexpr int kGlocChunks = 64;  // k chunks of k_gloc (partials: kGlocChunks x Nw doubles)
 hipError_t launch_gloc(const Grid& g, const DevArrays& d, double mu, double* part, double* out, hipStream_t s);
+hipError_t launch_chi0_tan(const Grid& g, const DevArrays& d, const TanArrays& t, double mu0, double dmu0,
+                           hipStream_t s);
+hipError_t launch_sum_rule_tan(const Grid& g, const DevArrays& d, const TanArrays& t, double Usp, double Uch,
+                               double* out4, hipStream_t s);
+hipError_t launch_vertex_slab_tan(const Grid& g, const DevArrays& d, const TanArrays& t, double U, double dU,
+                                  double Usp, double dUsp, double Uch, double dUch, hipStream_t s);
+hipError_t launch_axpy(long n, double a, const double* x, double* y, hipStream_t s);
+hipError_t launch_sigma_tan(const Grid& g, const DevArrays& d, const TanArrays& t, double mu0, double dmu0,
+                            double fact, double dfact, hipStream_t s);
+hipError_t launch_sigma_loss(const Grid& g, const double* w, const double* Sinf2, const cd* Self, double* out,
+                             hipStream_t s);
 hipError_t launch_transpose_d(const double* in, double* out, long ncols, int nrows, hipStream_t s);
 hipError_t launch_sum_rows(const double* part, int nrows, int n, double* out, hipStream_t s);
 hipError_t launch_al_setup(const Grid& g, const DevArrays& d, const ALArrays& a, double mu0, hipStream_t s);

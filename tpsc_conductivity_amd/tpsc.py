@@ -19,11 +19,12 @@ class CondOpt:
     tau/iqn arrays, FP64 boundary data, moments, roots and Matsubara sums).
     """
 
-    def __init__(self, nk0, Nw0, n, T, eta=0.1, params=(0.0, 0.0, 0.0, 1.0), coarse_pass=True, precision="fp64"):
+    def __init__(self, nk0, Nw0, n, T, eta=0.1, params=(0.0, 0.0, 0.0, 1.0), coarse_pass=True, precision="fp64",
+                 t=1.0):
         tp, tpp, t3, U = params
         self.precision = precision
         self.ctx = Context(nk0, Nw0, n, T, U, tp=tp, tpp=tpp, t3=t3, eta=eta, coarse_pass=coarse_pass,
-                           precision=PRECISIONS[precision])
+                           precision=PRECISIONS[precision], t=t)
         self.nk0, self.Nw0, self.density, self.tem, self.eta = nk0, Nw0, n, T, eta
         self.tp, self.tpp, self.t3, self.U = tp, tpp, t3, U
 
@@ -123,6 +124,30 @@ class CondOpt:
         self.ctx.check(lib().tpsc_gwn_local(self.ctx.handle, out.ctypes.data_as(C.POINTER(C.c_double))),
                        "Gwn_local")
         return out
+
+    def fix_mu0(self, mu0):
+        """Make mu0 an input (every later stage uses it instead of solving the density); NaN releases."""
+        self.ctx.check(lib().tpsc_fix_mu0(self.ctx.handle, float(mu0)), "tpsc_fix_mu0")
+
+    def sigma_loss_forward(self, w):
+        """L = sum_k w_k Re Sigma(k, i w_0) after calc_Self_FFT_spline (FP64 path)."""
+        import numpy as np
+
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        L = C.c_double(0.0)
+        self.ctx.check(lib().tpsc_sigma_loss_forward(self.ctx.handle, w.ctypes.data_as(C.POINTER(C.c_double)),
+                                                     C.byref(L)), "tpsc_sigma_loss_forward")
+        return L.value
+
+    def sigma_loss_backward(self, dloss=1.0):
+        """dloss * dL/d(t, t', mu0, U) for the last sigma_loss_forward."""
+        import numpy as np
+
+        g = np.zeros(4)
+        self.ctx.check(lib().tpsc_sigma_loss_backward(self.ctx.handle, float(dloss),
+                                                      g.ctypes.data_as(C.POINTER(C.c_double))),
+                       "tpsc_sigma_loss_backward")
+        return g
 
     def write_files(self, directory=None):
         """Write the reference's text outputs (TPSCparams_*, TPSC_infos_*, chijj*_qn_*, kx0_*, ...)
