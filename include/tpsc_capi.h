@@ -158,6 +158,13 @@ int tpsc_solve(tpsc_ctx* ctx);
  * as Nw/2 complex values interleaved re/im.  Needs the self-energy and mu (TPSC_E_ORDER). */
 int tpsc_gwn_local(tpsc_ctx* ctx, double* out);
 
+/* chi::sp_corr_length (chi.cpp:451-640, run by green::calc_Self_FFT_spline, green.cpp:1680):
+ * out = {vF, xi_y, xi_x, xi_as_y, xi_as_x}: the Fermi velocity at the hot spot of the
+ * non-interacting Fermi surface (chi.cpp:419-447, green0.cpp:626-675), the inverse half width
+ * at half maximum of chi_sp(q, 0) along y and x through qmax, and the asymptotic
+ * xi0 sqrt(Usp/(2/chi0max - Usp)).  Needs Usp and chi0 on the full grid (TPSC_E_ORDER). */
+int tpsc_sp_corr_length(tpsc_ctx* ctx, double* out);
+
 /* mu0 as an input parameter: every later stage uses this mu0 instead of solving the density
  * equation (green0.cpp:240-262); NaN returns to the solved mu0.  Resets the computed stages.
  * No reference counterpart: it makes d/dmu0 a partial derivative at fixed n. */
@@ -185,7 +192,8 @@ int tpsc_get_params(tpsc_ctx* ctx, tpsc_params* out);
  * with the reference's file names and stream formats:
  *   e_k_*                                    hamiltonien.cpp:72-82
  *   chi0_qmax_wn_*, chi0_wn0_q_*             chi0.cpp:720-737
- *   TPSC_infos_*, TPSCparams_*               chi.cpp:375-415, green.cpp:3088-3103 (traceSelfG),
+ *   TPSC_infos_*, TPSCparams_*               chi.cpp:375-415, :476-560 (sp_corr_length),
+ *                                            green.cpp:3088-3103 (traceSelfG),
  *                                            green.cpp:3259-3266 (find_mu), :2978-2994 (traceSelfG2)
  *   mu0_mu_D_*                               green.cpp:3268-3276
  *   kx0_*, kx0_bin_*, chijj1_qn_*, chijj2_qn_*, chijj_with_vc1_qn_*   cond_opt.cpp:4195-4279

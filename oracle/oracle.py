@@ -83,6 +83,11 @@ def lib():
         L.orc_chijj.argtypes = [P, C.c_int]
         L.orc_chijj.restype = C.c_int
         L.orc_fix_mu0.argtypes = [P, C.c_double]
+        L.orc_hot_spot_vF.argtypes = [P]
+        L.orc_hot_spot_vF.restype = C.c_double
+        L.orc_sp_corr_from.argtypes = [_dp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, _dp]
+        L.orc_sp_corr_length.argtypes = [P, _dp]
+        L.orc_sp_corr_length.restype = C.c_int
         L.orc_sigma_loss.argtypes = [P, _dp]
         L.orc_sigma_loss.restype = C.c_double
         L.orc_gwn_local.argtypes = [P, _dp]
@@ -214,6 +219,12 @@ class Oracle:
         assert w.size == self.nk8
         return lib().orc_sigma_loss(self._o, w.ctypes.data_as(_dp))
 
+    def sp_corr_length(self):
+        """(vF, xi_y, xi_x, xi_as_y, xi_as_x) of chi::sp_corr_length (chi.cpp:451-640)."""
+        out = np.zeros(5)
+        assert lib().orc_sp_corr_length(self._o, out.ctypes.data_as(_dp)) == 0
+        return out
+
     def Gwn_local(self):
         """G_loc(iw_n), n < Nw/2 (green.cpp:209-266)."""
         out = np.zeros(self.s.nbw // 2, dtype=np.complex128)
@@ -290,6 +301,13 @@ def dft(a, sign):
     a = np.ascontiguousarray(a, dtype=np.complex128).copy()
     lib().orc_dft(a.ctypes.data_as(_dp), a.size, sign)
     return a
+
+
+def sp_corr_from(c0, nbq, qx, qy, chi0max, Usp):
+    c0 = np.ascontiguousarray(c0, dtype=np.float64)
+    out = np.zeros(4)
+    lib().orc_sp_corr_from(c0.ctypes.data_as(_dp), nbq, qx, qy, chi0max, Usp, out.ctypes.data_as(_dp))
+    return out
 
 
 def al_freqs(nbw, mmin):

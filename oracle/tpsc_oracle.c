@@ -1530,7 +1530,7 @@ static void al_k_pencil(const double complex *z, double complex d0, double compl
                         double complex *out)
 {
     const int H = nbw / 2;
-    double complex d2S0, d2Sb;
+    double complex d2S0 = 0, d2Sb = 0;
     for (int part = 0; part < 2; part++) {
         for (int j = 0; j <= nbw; j++) {
             tau[j] = j / (nbw * T);
@@ -1978,4 +1978,119 @@ double orc_sigma_loss(orc *o, const double *w)
         L += w[k] * creal(s);
     }
     return L;
+}
+
+/* ---------------------------------------------------------------- spin correlation length */
+typedef struct {
+    const orc *o;
+    double kx0;
+} fs_ctx;
+static double fs_diag(void *ctx, double kx)
+{
+    const fs_ctx *f = (const fs_ctx *)ctx;
+    return orc_dispk(f->o, kx, kx - f->kx0) - f->o->mu0; /* green0::ekmu_diag, green0.cpp:621-624 */
+}
+
+/* green0::find_FS(10001) (green0.cpp:626-675) + chi::vF_hot_spots (chi.cpp:419-447) */
+double orc_hot_spot_vF(orc *o)
+{
+    const int nk = 10001;
+    double *FS = dalloc(2 * (size_t)nk);
+    for (int j = 0; j < 2 * nk; j++) FS[j] = -1;
+    fs_ctx fc = {o, 0.0};
+    for (int j = 0; j < nk; j++) {
+        fc.kx0 = j * PI / (nk - 1);
+        const double init[3] = {0.5 * (fc.kx0 + PI), fc.kx0, PI}, lims[2] = {fc.kx0, PI};
+        double root = 0.0;
+        if (!orc_find_zero(fs_diag, &fc, init, &root, lims, NULL)) break;
+        FS[2 * j] = root;
+        FS[2 * j + 1] = root - fc.kx0;
+    }
+    double kx1 = FS[0], ky1 = FS[1], kx2 = FS[2], ky2 = FS[3];
+    int j = 1;
+    while ((kx1 + ky1 - PI) * (kx2 + ky2 - PI) > 0 && j < nk - 1) {
+        j++;
+        kx1 = kx2;
+        ky1 = ky2;
+        kx2 = FS[2 * j];
+        ky2 = FS[2 * j + 1];
+    }
+    free(FS);
+    const double kx = (kx1 + kx2) / 2, ky = (ky1 + ky2) / 2;
+    const double gx = orc_grad_ek_x(o, kx, ky), gy = orc_grad_ek_x(o, ky, kx); /* grad_y by symmetry of eps */
+    return sqrt(gx * gx + gy * gy);
+}
+
+/* chi::sp_corr_length (chi.cpp:451-640) on chi0(q, iq_0) given as the packed wedge c0:
+   out = {xi_y, xi_x, xi_as_y, xi_as_x}.  A value the reference reads before setting it
+   (chi_sp(qmax_x, 0) already above half maximum) is taken as 0. */
+void orc_sp_corr_from(const double *c0, int nbq, int qx, int qy, double chi0max, double Usp, double out[4])
+{
+#define W(l, m) c0[(m) + ((l) * ((l) + 1)) / 2]
+#define SP(x) ((x) / (1.0 - Usp * (x) / 2.0))
+    const double chisp_max = chi0max / (1.0 - Usp * chi0max / 2.0), dq = PI / (nbq - 1);
+    int l = qx, m = 0;
+    double s = SP(W(l, 0)), s2 = 0.0;
+    while (s < chisp_max / 2.0 && m <= qy) {
+        m++;
+        s2 = s;
+        s = SP(W(l, m));
+    }
+    double qa = (m - 1) * dq, qm = qy * dq, qHM = dq * (chisp_max / 2.0 - s2) / (s - s2) + qa;
+    out[0] = 1.0 / (qm - qHM);
+    m = qy;
+    l = m;
+    s = SP(W(l, m));
+    if (s < chisp_max / 2.0) {
+        while (s < chisp_max / 2.0 && l <= qx) {
+            l++;
+            s2 = s;
+            s = SP(W(l, m));
+        }
+        qa = (l - 1) * dq;
+        qm = qx * dq;
+        qHM = dq * (chisp_max / 2.0 - s2) / (s - s2) + qa;
+        out[1] = 1.0 / (qm - qHM);
+    } else
+        out[1] = out[0];
+    const double r = sqrt(Usp / (2.0 / chi0max - Usp));
+    if (qx == nbq - 1 && qy == nbq - 1) {
+        const double d2 = 2.0 * (W(nbq - 1, nbq - 2) - chi0max) * (nbq - 1) * (nbq - 1) / (PI * PI);
+        out[3] = sqrt(-0.5 * d2 / chi0max) * r;
+        out[2] = out[3];
+    } else {
+        int x = qx, y = qy - 1;
+        if (y < 0) y = qy + 1;
+        const double cl = W(x, y), cm = W(qx, qy);
+        y = qy + 1;
+        if (y > x) {
+            x = qy + 1;
+            y = qx;
+        }
+        const double cr = W(x, y);
+        out[2] = sqrt(-0.5 * (cr - 2 * cm + cl) * (nbq - 1) * (nbq - 1) / (PI * PI) / chi0max) * r;
+        x = qx - 1;
+        y = qy;
+        if (y > x) {
+            x = qy;
+            y = qx - 1;
+            if (y < 0) y = qx + 1;
+        }
+        const double cl2 = W(x, y);
+        x = qx + 1;
+        if (x > nbq - 1) x = qx - 1;
+        const double cr2 = W(x, qy);
+        out[3] = sqrt(-0.5 * (cr2 - 2 * cm + cl2) * (nbq - 1) * (nbq - 1) / (PI * PI) / chi0max) * r;
+    }
+#undef W
+#undef SP
+}
+
+/* out = {vF, xi_y, xi_x, xi_as_y, xi_as_x} after orc_find_Usp_Uch (full grid) */
+int orc_sp_corr_length(orc *o, double out[5])
+{
+    if (!o->chiqw || o->Usp == 0.0) return -1;
+    out[0] = orc_hot_spot_vF(o);
+    orc_sp_corr_from(o->chiqw, o->nbk, o->qmax[0], o->qmax[1], o->chi0max, o->Usp, out + 1);
+    return 0;
 }

@@ -105,6 +105,10 @@ int orc_chijj(orc *o, int N_theta);                     /* cond_opt.cpp:2537-429
 int orc_solve(orc *o);                                  /* tpsc_main.cpp:61-75 + E4 */
 int orc_gwn_local(orc *o, double complex *out);          /* green.cpp:209-266 */
 void orc_fix_mu0(orc *o, double mu0);                    /* mu0 as an input parameter   */
+double orc_hot_spot_vF(orc *o);                          /* chi.cpp:419-447, green0.cpp:626-675 */
+void orc_sp_corr_from(const double *c0, int nbq, int qx, int qy, double chi0max, double Usp,
+                      double out[4]);                    /* chi.cpp:451-640 */
+int orc_sp_corr_length(orc *o, double out[5]);
 double orc_sigma_loss(orc *o, const double *w);          /* sum_k w_k Re Sigma(k,iw_0)  */
 int orc_al_freqs(int nbw, int mmin, long *out);         /* cond_opt.cpp:5123-5156 */
 int orc_chijj_al(orc *o, int mmin, int N_theta);        /* cond_opt.cpp:4292-6467 (chi3) */

@@ -73,9 +73,15 @@ def test_driver_writes_the_reference_file_set(tmp_path):
     # parameter files
     infos = open(out / want["infos"]).read().splitlines()
     assert infos[0].startswith("Usp:  ") and abs(float(infos[0].split()[1]) - g.scalar("Usp")) < 1e-14
+    sp = ["vitesse de Fermi aux points chauds"]
+    for kind, r1, r2 in (("(1/(larg.mi-haut.))", "xi_sp/xi_th", "xi_sp*T"),
+                         ("(formule asymptotique)", "xi_sp_As/xi_th", "xi_sp_As*T")):
+        for ax in ("y", "x"):
+            sp += [f"longueur de correlation de spin {kind} (axe {ax})", r1, r2]
     assert [ln.split(":")[0] for ln in infos] == [
-        "Usp", "Uch", "chi0max", "double occupation (ansatz)", "qxmax, qymax", "U*dblOcc-U*n^2/4", "trace Self*G",
-        "trace Self*G2", "double occupation (Self*G2)", "mu0", "mu"]
+        "Usp", "Uch", "chi0max", "double occupation (ansatz)", "qxmax, qymax"] + sp + [
+        "U*dblOcc-U*n^2/4", "trace Self*G", "trace Self*G2", "double occupation (Self*G2)", "mu0", "mu"]
+    assert abs(float(infos[5].split()[-1]) - g.sp_corr_length()[0]) < 1e-13
     params = [ln.split() for ln in open(out / want["params"])]
     assert [p[0] for p in params] == ["Usp", "Uch", "nbq", "nbw", "dblOcc_ansatz", "qxmax", "qymax",
                                       "chispmax_over_chi0max", "mu0", "mu"]

@@ -244,3 +244,11 @@ def test_pade_conductivity_from_the_solve(oracle_mod):
 def test_gwn_local(name, oracle_mod):
     r = run_pair(name, oracle_mod)
     assert relerr(r["g"].Gwn_local(), r["o"].Gwn_local()) <= TIGHT
+
+
+@pytest.mark.parametrize("name", ["dop_16x128", "cfg1_32x512", "elec_64x256"])
+def test_sp_corr_length(name, oracle_mod):
+    r = run_pair(name, oracle_mod)
+    got, ref = r["g"].sp_corr_length(), r["o"].sp_corr_length()
+    assert got[0] == pytest.approx(ref[0], rel=1e-12)       # hot-spot vF (host, same algorithm)
+    assert np.allclose(got[1:], ref[1:], rtol=1e-8, atol=0)  # from chi0(q, 0) at 1e-10

@@ -54,7 +54,7 @@ EXPORTS = [
     "tpsc_prof_enable", "tpsc_prof_reset", "tpsc_prof_count", "tpsc_prof_get",
     "tpsc_keep_chi_tau", "tpsc_device_bytes", "tpsc_pade_coef", "tpsc_pade_eval", "tpsc_pade_conductivity",
     "tpsc_get_params", "tpsc_write_files", "tpsc_gwn_local", "tpsc_fix_mu0", "tpsc_sigma_loss_forward",
-    "tpsc_sigma_loss_backward", "tpsc_random_weights",
+    "tpsc_sigma_loss_backward", "tpsc_random_weights", "tpsc_sp_corr_length",
 ]
 
 
@@ -105,6 +105,7 @@ def lib():
     L.tpsc_sigma_loss_forward.argtypes = [vp, dp, dp]
     L.tpsc_sigma_loss_backward.argtypes = [vp, C.c_double, dp]
     L.tpsc_random_weights.argtypes = [C.c_ulonglong, C.c_size_t, dp]
+    L.tpsc_sp_corr_length.argtypes = [vp, dp]
     _lib = L
     return L
 

@@ -1368,6 +1368,139 @@ int tpsc_get(tpsc_ctx* c, int field, void* dst, size_t bytes)
     return 0;
 }
 
+// ------------------------------------------------------------------ spin correlation length
+// eps(k) and its gradient on the host (hamiltonien.cpp:119-131), with the hopping t
+double host_dispk(const tpsc_ctx* c, double kx, double ky)
+{
+    const double t = hop_t(c), tp = c->p.tp, tpp = c->p.tpp, t3 = c->p.t3;
+    return -2.0 * t * (cos(kx) + cos(ky)) - 4.0 * tp * cos(kx) * cos(ky) - 2.0 * tpp * (cos(2 * kx) + cos(2 * ky)) -
+           4 * t3 * (cos(2 * kx) * cos(ky) + cos(2 * ky) * cos(kx));
+}
+
+// Fermi velocity at the hot spot: the non-interacting Fermi surface is traced along kx - ky =
+// kx0 for 10001 values of kx0 in [0, pi] (green0::find_FS, green0.cpp:626-675), then the
+// segment that crosses kx + ky = pi gives the point where |grad eps| is taken
+// (chi::vF_hot_spots, chi.cpp:419-447)
+double hot_spot_vF(const tpsc_ctx* c)
+{
+    const int nk = 10001;
+    std::vector<double> FS(2 * (size_t)nk, -1.0);
+    for (int j = 0; j < nk; j++) {
+        const double kx0 = j * M_PI / (nk - 1);
+        const double init[3] = {0.5 * (kx0 + M_PI), kx0, M_PI}, lims[2] = {kx0, M_PI};
+        double root = 0.0;
+        auto f = [&](double kx) { return host_dispk(c, kx, kx - kx0) - c->mu0; };
+        if (!find_zero(f, init, &root, lims, nullptr)) break;
+        FS[2 * j] = root;
+        FS[2 * j + 1] = root - kx0;
+    }
+    double kx1 = FS[0], ky1 = FS[1], kx2 = FS[2], ky2 = FS[3];
+    int j = 1;
+    while ((kx1 + ky1 - M_PI) * (kx2 + ky2 - M_PI) > 0 && j < nk - 1) {
+        j++;
+        kx1 = kx2;
+        ky1 = ky2;
+        kx2 = FS[2 * j];
+        ky2 = FS[2 * j + 1];
+    }
+    const double kx = (kx1 + kx2) / 2, ky = (ky1 + ky2) / 2;
+    const double t = hop_t(c), tp = c->p.tp, tpp = c->p.tpp, t3 = c->p.t3;
+    const double gx = 2.0 * t * sin(kx) + 4.0 * tp * sin(kx) * cos(ky) + 4.0 * tpp * sin(2 * kx) +
+                      4 * t3 * (2.0 * sin(2 * kx) * cos(ky) + cos(2 * ky) * sin(kx));
+    const double gy = 2.0 * t * sin(ky) + 4.0 * tp * cos(kx) * sin(ky) + 4.0 * tpp * sin(2 * ky) +
+                      4 * t3 * (cos(2 * kx) * sin(ky) + 2.0 * sin(2 * ky) * cos(kx));
+    return sqrt(gx * gx + gy * gy);
+}
+
+// chi::sp_corr_length (chi.cpp:451-640) on chi0(q, iq_0) (the packed wedge c0): the inverse
+// half width at half maximum of chi_sp(q, 0) along y through qmax and along x, and the
+// asymptotic xi = xi0 sqrt(Usp/(2/chi0max - Usp)) with xi0^2 from the discrete curvature of chi0
+// at qmax.  out = {vF, xi_y, xi_x, xi_as_y, xi_as_x}.  Where the reference reads a value it
+// never set (chi_sp(qmax_x, 0) already above half maximum) 0 is used.
+void sp_corr_from(const double* c0, int nbq, int q0x, int q0y, double chi0max, double Usp, double* out)
+{
+    auto w = [&](int l, int m) { return c0[m + (l * (l + 1)) / 2]; };
+    auto sp = [&](double x) { return x / (1.0 - Usp * x / 2.0); };
+    const double chisp_max = chi0max / (1.0 - Usp * chi0max / 2.0);
+    const double dq = M_PI / (nbq - 1);
+    int l = q0x, m = 0;
+    double s = sp(w(l, 0)), s2 = 0.0;
+    while (s < chisp_max / 2.0 && m <= q0y) {
+        m++;
+        s2 = s;
+        s = sp(w(l, m));
+    }
+    double qa = (m - 1) * dq, qm = q0y * dq;
+    double qHM = dq * (chisp_max / 2.0 - s2) / (s - s2) + qa;
+    const double xi_y = 1.0 / (qm - qHM);
+    double xi_x;
+    m = q0y;
+    l = m;
+    s = sp(w(l, m));
+    if (s < chisp_max / 2.0) {
+        while (s < chisp_max / 2.0 && l <= q0x) {
+            l++;
+            s2 = s;
+            s = sp(w(l, m));
+        }
+        qa = (l - 1) * dq;
+        qm = q0x * dq;
+        qHM = dq * (chisp_max / 2.0 - s2) / (s - s2) + qa;
+        xi_x = 1.0 / (qm - qHM);
+    } else
+        xi_x = xi_y;
+    double as_x, as_y;
+    const double r = sqrt(Usp / (2.0 / chi0max - Usp));
+    if (q0x == nbq - 1 && q0y == nbq - 1) {
+        const double d2 = 2.0 * (w(nbq - 1, nbq - 2) - chi0max) * (nbq - 1) * (nbq - 1) / (M_PI * M_PI);
+        as_x = sqrt(-0.5 * d2 / chi0max) * r;
+        as_y = as_x;
+    } else {
+        int x = q0x, y = q0y - 1;
+        if (y < 0) y = q0y + 1;
+        const double cl = w(x, y), cm0 = w(q0x, q0y);
+        y = q0y + 1;
+        if (y > x) {
+            x = q0y + 1;
+            y = q0x;
+        }
+        const double cr = w(x, y);
+        double d2 = (cr - 2 * cm0 + cl) * (nbq - 1) * (nbq - 1) / (M_PI * M_PI);
+        as_y = sqrt(-0.5 * d2 / chi0max) * r;
+        x = q0x - 1;
+        y = q0y;
+        if (y > x) {
+            x = q0y;
+            y = q0x - 1;
+            if (y < 0) y = q0x + 1;
+        }
+        const double cl2 = w(x, y);
+        x = q0x + 1;
+        if (x > nbq - 1) x = q0x - 1;
+        const double cr2 = w(x, q0y);
+        d2 = (cr2 - 2 * cm0 + cl2) * (nbq - 1) * (nbq - 1) / (M_PI * M_PI);
+        as_x = sqrt(-0.5 * d2 / chi0max) * r;
+    }
+    out[1] = xi_y;
+    out[2] = xi_x;
+    out[3] = as_y;
+    out[4] = as_x;
+}
+
+int tpsc_sp_corr_length(tpsc_ctx* c, double* out)
+{
+    if (!c || !out) return TPSC_E_INVALID;
+    if (!c->have_U || !c->have_chi0 || !full_grid(c))
+        return fail(c, TPSC_E_ORDER, "sp_corr_length(): Usp and chi0 on the full grid are needed");
+    const Grid& g = c->g;
+    std::vector<double> c0((size_t)g.nk8);
+    HIPCHK(c, hipMemcpyAsync(c0.data(), c->d.chiw, sizeof(double) * g.nk8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    out[0] = hot_spot_vF(c);
+    sp_corr_from(c0.data(), g.nk, c->qmax[0], c->qmax[1], c->chi0max, c->Usp, out);
+    return 0;
+}
+
 int tpsc_fix_mu0(tpsc_ctx* c, double mu0)
 {
     if (!c) return TPSC_E_INVALID;

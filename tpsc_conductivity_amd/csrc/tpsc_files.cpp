@@ -5,6 +5,7 @@
 // (precision, left / scientific flags, setw), so tools that read the reference's files read
 // these.  Files the reference builds in several appends (TPSC_infos_*, TPSCparams_*) are
 // written whole, with the lines in the reference's order.
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -117,6 +118,23 @@ extern "C" int tpsc_write_files(tpsc_ctx* c, const char* dir, int* nfiles)
             f << "chi0max: " << chi0max << "   chispmax: " << chispmax << "   rapport: " << chispmax / chi0max << '\n';
             f << "double occupation (ansatz): " << dblOcc << '\n';
             f << "qxmax, qymax:   " << q0 << ",  " << q1 << '\n';
+            double xi[5];
+            if ((stages & TPSC_STAGE_VERTEX) && tpsc_sp_corr_length(c, xi) == TPSC_OK) {
+                // chi::sp_corr_length (chi.cpp:476-560), which calc_Self_FFT_spline runs before V
+                const double vF = xi[0];
+                f << "vitesse de Fermi aux points chauds:  " << vF << std::endl;
+                const char* ax[2] = {"(axe y)", "(axe x)"};
+                for (int a = 0; a < 2; a++) {
+                    f << "longueur de correlation de spin (1/(larg.mi-haut.)) " << ax[a] << ":  " << xi[1 + a] << '\n';
+                    f << "xi_sp/xi_th:  " << xi[1 + a] * M_PI * tem / vF << '\n';
+                    f << "xi_sp*T:  " << xi[1 + a] * tem << '\n';
+                }
+                for (int a = 0; a < 2; a++) {
+                    f << "longueur de correlation de spin (formule asymptotique) " << ax[a] << ":  " << xi[3 + a] << '\n';
+                    f << "xi_sp_As/xi_th:  " << xi[3 + a] * M_PI * tem / vF << '\n';
+                    f << "xi_sp_As*T:  " << xi[3 + a] * tem << '\n';
+                }
+            }
             if (stages & TPSC_STAGE_TRACE_SG) {
                 f << "U*dblOcc-U*n^2/4:   " << U * dblOcc - U * density * density / 4.0 << '\n';
                 f << "trace Self*G:  " << w.scalar(TPSC_S_TRSG) << '\n';

@@ -125,6 +125,15 @@ class CondOpt:
                        "Gwn_local")
         return out
 
+    def sp_corr_length(self):
+        """chi::sp_corr_length (chi.cpp:451-640): (vF, xi_y, xi_x, xi_as_y, xi_as_x)."""
+        import numpy as np
+
+        out = np.zeros(5)
+        self.ctx.check(lib().tpsc_sp_corr_length(self.ctx.handle, out.ctypes.data_as(C.POINTER(C.c_double))),
+                       "sp_corr_length")
+        return out
+
     def fix_mu0(self, mu0):
         """Make mu0 an input (every later stage uses it instead of solving the density); NaN releases."""
         self.ctx.check(lib().tpsc_fix_mu0(self.ctx.handle, float(mu0)), "tpsc_fix_mu0")
