@@ -129,6 +129,36 @@ def bench_al(cfg, args):
           flush=True)
 
 
+def bench_batch(args):
+    """Throughput of tpsc_solve_batch on the 64 random points (N in {32..128}, T, n, U, t'),
+    tuned worker count against one worker (the points one after another)."""
+    from tpsc_conductivity_amd import capi
+
+    pts = capi.batch_points(64064, 64)
+    w, rates = capi.batch_tune(pts)
+    capi.solve_batch(pts, workers=w)  # warm-up (code objects, allocator)
+    out = {}
+    for tag, nw in (("tuned", w), ("one_worker", 1)):
+        t0 = time.perf_counter()
+        res, _ = capi.solve_batch(pts, workers=nw)
+        dt = time.perf_counter() - t0
+        st = [r.status for r in res]
+        out[tag] = {"workers": nw, "seconds": dt, "solved": st.count(0),
+                    "unsupported": st.count(-1), "failed": len(st) - st.count(0) - st.count(-1)}
+        print(f"[bench] batch {tag}: {nw} workers, {dt:.3f} s", file=sys.stderr, flush=True)
+    solved = out["tuned"]["solved"]
+    print(json.dumps({"metric": "TPSC batched solves/s, 64 random (N, T, n, U, t') points, one MI355X",
+                      "value": solved / out["tuned"]["seconds"], "unit": "solves/s", "higher_is_better": True,
+                      "speedup_over_one_worker": out["one_worker"]["seconds"] / out["tuned"]["seconds"],
+                      "tuner_points_per_s": {str(k): float(v) for k, v in zip((1, 2, 4, 8), rates)},
+                      "runs": out, "dtype": "f64", "data": "synthetic",
+                      "config": {"workload": "tpsc_batch_points(64064, 64): N in {32,48,64,96,128}, "
+                                             "T in [0.05,0.5], n in [0.8,1], U in [1,6], t' in [-0.3,0], "
+                                             "Nw = smallest power of two with (Nw-1) pi T >= 20; "
+                                             "N = 48, 96 unsupported (power-of-two lattices only)"}}),
+          flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -147,6 +177,9 @@ def main():
     ap.add_argument("--al", type=int, default=None, metavar="MMIN",
                     help="instead of the headline line, time the Aslamazov-Larkin term on the sparse boson set "
                          "of parameter MMIN (one solve first, then Lambda_jj with chi3) and print its own line")
+    ap.add_argument("--batch", action="store_true",
+                    help="instead of the headline line, solve the 64 random points of tpsc_batch_points(64064) "
+                         "with the tuned worker count and with one worker, and print their own line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -164,6 +197,8 @@ def main():
                           "launches": 5}), flush=True)
         return
 
+    if args.batch:
+        return bench_batch(args)
     if args.al is not None:
         return bench_al(dict(WORKLOAD, N=args.N, Nw=args.Nw), args)
 

@@ -184,7 +184,34 @@ int tpsc_sigma_loss_backward(tpsc_ctx* ctx, double dloss, double* grad);
  * (the loss weights of the gradient tests); needs no context */
 int tpsc_random_weights(unsigned long long seed, size_t n, double* out);
 
-/* the parameters the context was created with */
+/* ---- batches of independent parameter points (no reference counterpart) ---- */
+typedef struct tpsc_batch_result {
+    int status;                                  /* TPSC_OK or the failing call's code      */
+    double mu0, Usp, Uch, mu, dblOcc, kx0;
+    double seconds;                              /* wall time of the point's create + solve */
+} tpsc_batch_result;
+
+/* Solves npts independent points (tpsc_create + tpsc_solve + tpsc_destroy each) with `workers`
+ * host threads, each driving its own contexts and streams, so that the points' kernels
+ * interleave on the GPU; largest N^2 Nw first.  workers = 0: chosen by tpsc_batch_tune.
+ * chijj (optional, npts*ldc doubles): Lambda_jj1 + Lambda_jj2 at n = 0..min(Nw0/2, ldc-1). */
+int tpsc_solve_batch(const tpsc_params* pts, int npts, int workers, tpsc_batch_result* out, double* chijj,
+                     int ldc);
+/* times the solve of the 8 largest points at 1, 2, 4 and 8 workers (points/s in rates[4]
+ * if not NULL) and returns the fastest worker count */
+int tpsc_batch_tune(const tpsc_params* pts, int npts, int* workers, double* rates);
+/* npts random points drawn from std::mt19937_64(seed) in the order N in {32,48,64,96,128},
+ * T ~ U[0.05,0.5], n ~ U[0.8,1], U ~ U[1,6], t' ~ U[-0.3,0]; Nw the smallest power of two
+ * with (Nw-1) pi T >= 20; FP64, no coarse pass.  N = 48, 96 are not supported by tpsc_create
+ * (power-of-two lattices only) and come back with status TPSC_E_INVALID from a batch. */
+int tpsc_batch_points(unsigned long long seed, int npts, tpsc_params* out);
+
+/* re-targets a context to another parameter point on the same grid and precision (n, T,
+ * hoppings, U, coarse_pass may change): the constructors' eps(k) and mu0 are redone and every
+ * computed stage is dropped; the device arrays are kept.  No reference counterpart. */
+int tpsc_set_params(tpsc_ctx* ctx, const tpsc_params* p);
+
+/* the parameters the context was created with (or last set) */
 int tpsc_get_params(tpsc_ctx* ctx, tpsc_params* out);
 
 /* Writes, into directory dir (NULL or "" = the current directory), the text files the

@@ -44,6 +44,11 @@ class Params(C.Structure):
     ]
 
 
+class BatchResult(C.Structure):
+    _fields_ = [("status", C.c_int), ("mu0", C.c_double), ("Usp", C.c_double), ("Uch", C.c_double),
+                ("mu", C.c_double), ("dblOcc", C.c_double), ("kx0", C.c_double), ("seconds", C.c_double)]
+
+
 _lib = None
 
 EXPORTS = [
@@ -54,7 +59,8 @@ EXPORTS = [
     "tpsc_prof_enable", "tpsc_prof_reset", "tpsc_prof_count", "tpsc_prof_get",
     "tpsc_keep_chi_tau", "tpsc_device_bytes", "tpsc_pade_coef", "tpsc_pade_eval", "tpsc_pade_conductivity",
     "tpsc_get_params", "tpsc_write_files", "tpsc_gwn_local", "tpsc_fix_mu0", "tpsc_sigma_loss_forward",
-    "tpsc_sigma_loss_backward", "tpsc_random_weights", "tpsc_sp_corr_length",
+    "tpsc_sigma_loss_backward", "tpsc_random_weights", "tpsc_sp_corr_length", "tpsc_solve_batch",
+    "tpsc_batch_tune", "tpsc_batch_points", "tpsc_set_params",
 ]
 
 
@@ -106,6 +112,10 @@ def lib():
     L.tpsc_sigma_loss_backward.argtypes = [vp, C.c_double, dp]
     L.tpsc_random_weights.argtypes = [C.c_ulonglong, C.c_size_t, dp]
     L.tpsc_sp_corr_length.argtypes = [vp, dp]
+    L.tpsc_solve_batch.argtypes = [C.POINTER(Params), C.c_int, C.c_int, C.POINTER(BatchResult), dp, C.c_int]
+    L.tpsc_batch_tune.argtypes = [C.POINTER(Params), C.c_int, ip, dp]
+    L.tpsc_batch_points.argtypes = [C.c_ulonglong, C.c_int, C.POINTER(Params)]
+    L.tpsc_set_params.argtypes = [vp, C.POINTER(Params)]
     _lib = L
     return L
 
@@ -230,3 +240,33 @@ def random_weights(seed, n):
     if rc:
         raise TPSCError(f"tpsc_random_weights failed ({ERRORS.get(rc, rc)})")
     return out
+
+
+def batch_points(seed, n):
+    """The random parameter points of tpsc_batch_points (a ctypes Params array)."""
+    pts = (Params * n)()
+    rc = lib().tpsc_batch_points(seed, n, pts)
+    if rc:
+        raise TPSCError(f"tpsc_batch_points failed ({ERRORS.get(rc, rc)})")
+    return pts
+
+
+def solve_batch(pts, workers=0, ldc=0):
+    """tpsc_solve_batch over a Params array: (results array, Lambda_jj rows or None)."""
+    n = len(pts)
+    res = (BatchResult * n)()
+    jj = np.zeros((n, ldc)) if ldc else None
+    jp = jj.ctypes.data_as(C.POINTER(C.c_double)) if ldc else None
+    rc = lib().tpsc_solve_batch(pts, n, workers, res, jp, ldc)
+    if rc:
+        raise TPSCError(f"tpsc_solve_batch failed ({ERRORS.get(rc, rc)})")
+    return res, jj
+
+
+def batch_tune(pts):
+    w = C.c_int(0)
+    rates = np.zeros(4)
+    rc = lib().tpsc_batch_tune(pts, len(pts), C.byref(w), rates.ctypes.data_as(C.POINTER(C.c_double)))
+    if rc:
+        raise TPSCError(f"tpsc_batch_tune failed ({ERRORS.get(rc, rc)})")
+    return w.value, rates

@@ -1,0 +1,174 @@
+// tpsc_batch.cpp -- many independent parameter points on one GPU, through the public C ABI.
+//
+// Each host worker thread keeps one context (device arrays, two streams) per grid and
+// re-targets it to each of its points (tpsc_set_params) before tpsc_solve; with several
+// workers the points' kernels and root passes interleave on the GPU, which a single small
+// point (N <= 128) leaves mostly idle.  Points are taken in
+// decreasing cost N^2 Nw (longest first, from a shared counter).  The worker count is either
+// given or chosen by tpsc_batch_tune, which times a probe subset at 1, 2, 4 and 8 workers.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <numeric>
+#include <random>
+#include <thread>
+#include <tuple>
+#include <vector>
+
+#include "../../include/tpsc_capi.h"
+
+namespace {
+
+double now_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// contexts of one worker, one per (N, Nw, precision), re-targeted point by point
+struct Pool {
+    std::map<std::tuple<int, int, int>, tpsc_ctx*> ctx;
+    ~Pool()
+    {
+        for (auto& kv : ctx) tpsc_destroy(kv.second);
+    }
+    int get(const tpsc_params& p, tpsc_ctx** out)
+    {
+        const auto key = std::make_tuple(p.nk0, p.Nw0, p.precision);
+        auto it = ctx.find(key);
+        if (it == ctx.end()) {
+            tpsc_ctx* c = nullptr;
+            const int rc = tpsc_create(&p, &c);
+            if (rc) {
+                tpsc_destroy(c);
+                return rc;
+            }
+            ctx[key] = c;
+            *out = c;
+            return TPSC_OK;
+        }
+        *out = it->second;
+        return tpsc_set_params(it->second, &p);
+    }
+};
+
+void solve_point(Pool& pool, const tpsc_params& p, tpsc_batch_result& r, double* jj, int ldc)
+{
+    const double t0 = now_s();
+    std::memset(&r, 0, sizeof r);
+    tpsc_ctx* c = nullptr;
+    int rc = pool.get(p, &c);
+    if (!rc) rc = tpsc_solve(c);
+    if (c) {
+        tpsc_get_scalar(c, TPSC_S_MU0, &r.mu0);
+        tpsc_get_scalar(c, TPSC_S_USP, &r.Usp);
+        tpsc_get_scalar(c, TPSC_S_UCH, &r.Uch);
+        tpsc_get_scalar(c, TPSC_S_MU, &r.mu);
+        tpsc_get_scalar(c, TPSC_S_DBLOCC, &r.dblOcc);
+        tpsc_get_scalar(c, TPSC_S_KX0, &r.kx0);
+        if (!rc && jj && ldc > 0) {
+            const int nwb = p.Nw0 / 2 + 1;
+            std::vector<double> a((size_t)nwb), b((size_t)nwb);
+            if (!tpsc_get(c, TPSC_F_CHIJJ1, a.data(), a.size() * sizeof(double)) &&
+                !tpsc_get(c, TPSC_F_CHIJJ2, b.data(), b.size() * sizeof(double)))
+                for (int n = 0; n < std::min(nwb, ldc); n++) jj[n] = a[n] + b[n];
+        }
+    }
+    r.status = rc;
+    r.seconds = now_s() - t0;
+}
+
+std::vector<int> by_cost(const tpsc_params* pts, int n)
+{
+    std::vector<int> order((size_t)n);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        return (double)pts[a].nk0 * pts[a].nk0 * pts[a].Nw0 > (double)pts[b].nk0 * pts[b].nk0 * pts[b].Nw0;
+    });
+    return order;
+}
+
+void run(const tpsc_params* pts, const std::vector<int>& order, int workers, tpsc_batch_result* out, double* jj,
+         int ldc)
+{
+    std::atomic<size_t> next{0};
+    auto work = [&]() {
+        Pool pool;
+        for (size_t k; (k = next.fetch_add(1)) < order.size();) {
+            const int i = order[k];
+            solve_point(pool, pts[i], out[i], jj ? jj + (size_t)i * ldc : nullptr, ldc);
+        }
+    };
+    std::vector<std::thread> th;
+    for (int w = 1; w < workers; w++) th.emplace_back(work);
+    work();
+    for (auto& t : th) t.join();
+}
+
+}  // namespace
+
+extern "C" int tpsc_batch_tune(const tpsc_params* pts, int npts, int* workers, double* rates)
+{
+    if (!pts || npts <= 0 || !workers) return TPSC_E_INVALID;
+    std::vector<int> order = by_cost(pts, npts);
+    order.resize((size_t)std::min(npts, 8));
+    std::vector<tpsc_batch_result> tmp((size_t)npts);
+    const int cand[4] = {1, 2, 4, 8};
+    run(pts, order, 1, tmp.data(), nullptr, 0);  // warm-up: code objects and allocator pools
+    int best = 1;
+    double best_rate = 0.0;
+    for (int k = 0; k < 4; k++) {
+        const double t0 = now_s();
+        run(pts, order, cand[k], tmp.data(), nullptr, 0);
+        const double rate = order.size() / (now_s() - t0);
+        if (rates) rates[k] = rate;
+        if (rate > best_rate) {
+            best_rate = rate;
+            best = cand[k];
+        }
+    }
+    *workers = best;
+    return TPSC_OK;
+}
+
+extern "C" int tpsc_solve_batch(const tpsc_params* pts, int npts, int workers, tpsc_batch_result* out, double* chijj,
+                                int ldc)
+{
+    if (!pts || !out || npts < 0 || workers < 0 || workers > 64 || (chijj && ldc <= 0)) return TPSC_E_INVALID;
+    if (npts == 0) return TPSC_OK;
+    if (workers == 0) {
+        int rc = tpsc_batch_tune(pts, npts, &workers, nullptr);
+        if (rc) return rc;
+    }
+    run(pts, by_cost(pts, npts), std::min(workers, npts), out, chijj, ldc);
+    return TPSC_OK;
+}
+
+extern "C" int tpsc_batch_points(unsigned long long seed, int npts, tpsc_params* out)
+{
+    if (!out || npts < 0) return TPSC_E_INVALID;
+    std::mt19937_64 gen(seed);
+    std::uniform_int_distribution<int> pick(0, 4);
+    std::uniform_real_distribution<double> uT(0.05, 0.5), un(0.8, 1.0), uU(1.0, 6.0), utp(-0.3, 0.0);
+    const int Ns[5] = {32, 48, 64, 96, 128};
+    for (int i = 0; i < npts; i++) {
+        tpsc_params& p = out[i];
+        std::memset(&p, 0, sizeof p);
+        p.nk0 = Ns[pick(gen)];
+        p.T = uT(gen);
+        p.n = un(gen);
+        p.U = uU(gen);
+        p.tp = utp(gen);
+        // smallest power-of-two Nw with (Nw - 1) pi T >= 20
+        int Nw = 16;
+        while ((Nw - 1) * M_PI * p.T < 20.0) Nw *= 2;
+        p.Nw0 = Nw;
+        p.eta = 0.1;
+        p.coarse_pass = 0;
+        p.precision = TPSC_PREC_FP64;
+        p.t = 1.0;
+    }
+    return TPSC_OK;
+}

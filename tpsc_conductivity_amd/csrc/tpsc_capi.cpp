@@ -1570,6 +1570,28 @@ int tpsc_gwn_local(tpsc_ctx* c, double* out)
     return 0;
 }
 
+int tpsc_set_params(tpsc_ctx* c, const tpsc_params* p)
+{
+    if (!c || !p) return TPSC_E_INVALID;
+    if (p->nk0 != c->p.nk0 || p->Nw0 != c->p.Nw0 || p->precision != c->p.precision)
+        return fail(c, TPSC_E_INVALID, "tpsc_set_params: the grid and the precision must stay those of the context");
+    if (!(p->T > 0.0) || (p->coarse_pass && (p->nk0 < 8 || p->Nw0 < 32)))
+        return fail(c, TPSC_E_INVALID, "tpsc_set_params: invalid parameters");
+    c->p = *p;
+    c->nsr = (p->n > 1.0) ? 2.0 - p->n : p->n;  // chi.cpp:18-19
+    c->mu0_fixed = false;
+    c->have_U = c->have_V = c->have_sigma = c->have_jj = c->have_trSG = c->have_trSG2 = false;
+    c->mu_ok = false;
+    c->grad_ready = false;
+    c->hf0_pending = false;
+    c->chijj1.clear();
+    c->chijj2.clear();
+    c->chijj3.clear();
+    c->chijj_all.clear();
+    c->al_freq.clear();
+    return set_grid(c, p->nk0, p->Nw0);  // the constructors' eps(k) and mu0 for the new point
+}
+
 int tpsc_get_params(tpsc_ctx* c, tpsc_params* out)
 {
     if (!c || !out) return TPSC_E_INVALID;
