@@ -325,8 +325,10 @@ __device__ __forceinline__ void slab_r2r_2d(Real* slab, int n0, int n1, int ld, 
 // The twiddles w^r come from three table reads (w, w^2, w^4) and products, issued before
 // the LDS reads and the barrier so their latency overlaps them.
 // ---------------------------------------------------------------------------------
-template <int RAD, int L, int NT>
-__device__ __forceinline__ void fft_pass(cd* buf, int sign, const cd* __restrict__ twg, int Ns)
+// PP (ping-pong): the pass reads src and writes a different buffer dst, so the barrier
+// between its reads and its writes is not needed; in place (src == dst) it is.
+template <int RAD, int L, int NT, bool PP = false>
+__device__ __forceinline__ void fft_pass(const cd* src, cd* dst, int sign, const cd* __restrict__ twg, int Ns)
 {
     constexpr int NB = L / RAD;
     constexpr int PER = (NB + NT - 1) / NT;
@@ -358,10 +360,10 @@ __device__ __forceinline__ void fft_pass(cd* buf, int sign, const cd* __restrict
         const int b = tid + i * NT;
         if (b < NB) {
 #pragma unroll
-            for (int r = 0; r < RAD; r++) v[i][r] = buf[b + r * NB];
+            for (int r = 0; r < RAD; r++) v[i][r] = src[b + r * NB];
         }
     }
-    __syncthreads();
+    if constexpr (!PP) __syncthreads();
 #pragma unroll
     for (int i = 0; i < PER; i++) {
         const int b = tid + i * NT;
@@ -390,9 +392,9 @@ __device__ __forceinline__ void fft_pass(cd* buf, int sign, const cd* __restrict
 #pragma unroll
                 for (int r = 0; r < RAD; r++) v[i][r].y = -v[i][r].y;
             }
-            const int dst = (b / Ns) * Ns * RAD + k;
+            const int o = (b / Ns) * Ns * RAD + k;
 #pragma unroll
-            for (int r = 0; r < RAD; r++) buf[dst + r * Ns] = v[i][r];
+            for (int r = 0; r < RAD; r++) dst[o + r * Ns] = v[i][r];
         }
     }
     __syncthreads();
@@ -407,11 +409,40 @@ __device__ __forceinline__ void block_fft(cd* buf, int sign, const cd* __restric
     int Ns = 1;
 #pragma unroll
     for (int pass = 0; pass < P8; pass++) {
-        fft_pass<8, L, NT>(buf, sign, twg, Ns);
+        fft_pass<8, L, NT>(buf, buf, sign, twg, Ns);
         Ns *= 8;
     }
-    if constexpr (REM == 2) fft_pass<4, L, NT>(buf, sign, twg, Ns);
-    if constexpr (REM == 1) fft_pass<2, L, NT>(buf, sign, twg, Ns);
+    if constexpr (REM == 2) fft_pass<4, L, NT>(buf, buf, sign, twg, Ns);
+    if constexpr (REM == 1) fft_pass<2, L, NT>(buf, buf, sign, twg, Ns);
+}
+
+// The same transform with a second LDS buffer tmp of L values: the passes alternate between
+// buf and tmp and take one barrier each instead of two.  buf must be complete on entry (the
+// caller's barrier); the result is in buf (after an odd number of passes it is copied back).
+template <int L, int NT>
+__device__ __forceinline__ void block_fft_pp(cd* buf, cd* tmp, int sign, const cd* __restrict__ twg)
+{
+    constexpr int LOG2 = __builtin_ctz(L);
+    constexpr int P8 = LOG2 / 3;
+    constexpr int REM = LOG2 - 3 * P8;
+    constexpr int NP = P8 + (REM > 0 ? 1 : 0);
+    cd* a = buf;
+    cd* b = tmp;
+    int Ns = 1;
+#pragma unroll
+    for (int pass = 0; pass < P8; pass++) {
+        fft_pass<8, L, NT, true>(a, b, sign, twg, Ns);
+        cd* t = a;
+        a = b;
+        b = t;
+        Ns *= 8;
+    }
+    if constexpr (REM == 2) fft_pass<4, L, NT, true>(a, b, sign, twg, Ns);
+    if constexpr (REM == 1) fft_pass<2, L, NT, true>(a, b, sign, twg, Ns);
+    if constexpr (NP % 2 == 1) {
+        for (int j = threadIdx.x; j < L; j += NT) buf[j] = tmp[j];
+        __syncthreads();
+    }
 }
 
 // ---------------------------------------------------------------------------------

@@ -384,7 +384,18 @@ __global__ void __launch_bounds__(256)
 
 // The same trace and its first two derivatives with respect to Ux, where sU = sgn * Ux:
 // d/dUx [c/(1 + sU c/2)] = -sgn c^2/2 / (1 + sU c/2)^2, d2/dUx2 = c^3/2 / (1 + sU c/2)^3.
-// Partials: part[k * nblk + blk], k = 0, 1, 2.
+// Partials: part[k * nblk + blk], k = 0, 1, 2.  A thread sums its rows j = y, y + RG, ... in
+// increasing j: the stored rows j <= nwmax in batches of kSR loads issued before any use (one
+// dependent load per row would leave HBM idle), then the asymptotic rows, whose 1/wn^2 is
+// common to the block.
+constexpr int kSR = 8;
+__device__ __forceinline__ void sum_rule3_term(double c, double w, double sU, double sgn, double acc[3])
+{
+    const double r = 1.0 / (1 + sU * c / 2.0);
+    acc[0] += w * (c * r - c);
+    acc[1] -= w * sgn * 0.5 * c * c * r * r;
+    acc[2] += w * 0.5 * c * c * c * r * r * r;
+}
 __global__ void __launch_bounds__(256)
     k_sum_rule3(const double* __restrict__ chi, const double* __restrict__ inf1, const double* __restrict__ inf2,
                 const double* __restrict__ wq, int nk8, int nwmax, double T, double Nq, double sgn, double Ux,
@@ -392,25 +403,25 @@ __global__ void __launch_bounds__(256)
 {
     __shared__ double sc[3 * 4];
     const int q = blockIdx.x * 256 + threadIdx.x;
+    const int RG = gridDim.y;
     const double sU = sgn * Ux;
     double acc[3] = {0.0, 0.0, 0.0};
     if (q < nk8) {
-        const double w16 = 2.0 * wq[q];
+        const double wb = 2.0 * wq[q] * T / Nq;
         const double a1 = inf1[q], a2 = inf2[q];
-        for (int j = blockIdx.y; j <= 2 * nwmax; j += gridDim.y) {
-            const double w = ((j == 0) ? 0.5 * w16 : w16) * T / Nq;
-            double c;
-            if (j <= nwmax)
-                c = chi[(long)j * nk8 + q];
-            else {
-                const double wn = 2 * j * kPi * T;
-                const double i2 = 1.0 / (wn * wn);
-                c = (a1 + a2 * i2) * i2;
-            }
-            const double r = 1.0 / (1 + sU * c / 2.0);
-            acc[0] += w * (c * r - c);
-            acc[1] -= w * sgn * 0.5 * c * c * r * r;
-            acc[2] += w * 0.5 * c * c * c * r * r * r;
+        int j = blockIdx.y;
+        for (; j + (kSR - 1) * RG <= nwmax; j += kSR * RG) {
+            double c[kSR];
+#pragma unroll
+            for (int u = 0; u < kSR; u++) c[u] = chi[(long)(j + u * RG) * nk8 + q];
+#pragma unroll
+            for (int u = 0; u < kSR; u++) sum_rule3_term(c[u], (j + u * RG == 0) ? 0.5 * wb : wb, sU, sgn, acc);
+        }
+        for (; j <= nwmax; j += RG) sum_rule3_term(chi[(long)j * nk8 + q], (j == 0) ? 0.5 * wb : wb, sU, sgn, acc);
+        for (; j <= 2 * nwmax; j += RG) {
+            const double wn = 2 * j * kPi * T;
+            const double i2 = 1.0 / (wn * wn);
+            sum_rule3_term((a1 + a2 * i2) * i2, wb, sU, sgn, acc);
         }
     }
     block_sum_k<3>(acc, sc);
@@ -715,6 +726,20 @@ __global__ void __launch_bounds__(256)
 // traceDiffG and its first two derivatives with respect to pmu.  With a = e - pmu + Re Sigma,
 // b = wn - Im Sigma: Re G = -a/(a^2+b^2), d/dpmu Re G = (b^2-a^2)/(a^2+b^2)^2,
 // d2/dpmu2 Re G = (6ab^2 - 2a^3)/(a^2+b^2)^3.  Partials part[k * nblk + blk], k = 0, 1, 2.
+// A lane sums its frequencies j = lane, lane + 64, ... in increasing j: the stored ones
+// (j < Nw/2) in batches of kTD loads issued before any use, then the asymptotic ones.
+constexpr int kTD = 4;
+__device__ __forceinline__ void trace_diffG3_term(double wn, double rS, double iS, double e, double c, double w,
+                                                  double mu0, double pmu, double acc[3])
+{
+    const double a = e - pmu + rS, b = wn - iS;
+    const double a2 = a * a, b2 = b * b, r = 1.0 / (a2 + b2);
+    const double r0 = 1.0 / (wn * wn + c * c);
+    // Re G - Re G0 = [c (a^2+b^2) - a (c^2+wn^2)] r r0, written without cancellation
+    acc[0] += w * r * r0 * ((mu0 - pmu + rS) * a * c + iS * (iS - 2 * wn) * c - wn * wn * (mu0 - pmu + rS));
+    acc[1] += w * (b2 - a2) * r * r;
+    acc[2] += w * (6.0 * a * b2 - 2.0 * a * a2) * r * r * r;
+}
 template <typename SelfT>
 __global__ void __launch_bounds__(256)
     k_trace_diffG3(const double* __restrict__ ekw, const double* __restrict__ wq, const SelfT* __restrict__ Self,
@@ -733,25 +758,27 @@ __global__ void __launch_bounds__(256)
         const double s2 = Sinf2[q], s3 = Sinf3[q], s4 = Sinf4[q];
         const SelfT* Sq = Self + (long)q * H;
         const double c = e - mu0;
-        for (int j = lane; j <= Nw; j += 64) {
-            const double wn = (2 * j + 1) * kPi * T;
-            double rS, iS;
-            if (j >= H) {
-                const double iw = 1.0 / wn, iw2 = iw * iw;
-                rS = (-s2 + s4 * iw2) * iw2;
-                iS = (-Si + s3 * iw2) * iw;
-            } else {
-                const cd s = sigma_kw(Sq, j, Si, s2, s3, wn);
-                rS = s.x;
-                iS = s.y;
+        int j = lane;
+        for (; j + (kTD - 1) * 64 < H; j += kTD * 64) {
+            SelfT v[kTD];
+#pragma unroll
+            for (int u = 0; u < kTD; u++) v[u] = Sq[j + u * 64];
+#pragma unroll
+            for (int u = 0; u < kTD; u++) {
+                const double wn = (2 * (j + u * 64) + 1) * kPi * T;
+                const cd s = sigma_kw(v + u, 0, Si, s2, s3, wn);
+                trace_diffG3_term(wn, s.x, s.y, e, c, w, mu0, pmu, acc);
             }
-            const double a = e - pmu + rS, b = wn - iS;
-            const double a2 = a * a, b2 = b * b, r = 1.0 / (a2 + b2);
-            const double r0 = 1.0 / (wn * wn + c * c);
-            // Re G - Re G0 = [c (a^2+b^2) - a (c^2+wn^2)] r r0, written without cancellation
-            acc[0] += w * r * r0 * ((mu0 - pmu + rS) * a * c + iS * (iS - 2 * wn) * c - wn * wn * (mu0 - pmu + rS));
-            acc[1] += w * (b2 - a2) * r * r;
-            acc[2] += w * (6.0 * a * b2 - 2.0 * a * a2) * r * r * r;
+        }
+        for (; j < H; j += 64) {
+            const double wn = (2 * j + 1) * kPi * T;
+            const cd s = sigma_kw(Sq, j, Si, s2, s3, wn);
+            trace_diffG3_term(wn, s.x, s.y, e, c, w, mu0, pmu, acc);
+        }
+        for (; j <= Nw; j += 64) {
+            const double wn = (2 * j + 1) * kPi * T;
+            const double iw = 1.0 / wn, iw2 = iw * iw;
+            trace_diffG3_term(wn, (-s2 + s4 * iw2) * iw2, (-Si + s3 * iw2) * iw, e, c, w, mu0, pmu, acc);
         }
     }
     block_sum_k<3>(acc, sc);
@@ -1049,17 +1076,24 @@ __device__ __forceinline__ Poles make_poles(double e, double mu, double Sinf, do
 }
 
 // Dyson G at Matsubara slot s (n = s - Nw/2), with the conjugate Sigma for n < 0
-// (cond_opt.cpp:3354-3407)
+// (cond_opt.cpp:3354-3407).  dyson_n: the stored frequency index of slot s.
+__device__ __forceinline__ int dyson_n(int s, int H) { return (s < H) ? H - s - 1 : s - H; }
+// from the stored value raw = Self[ki * H + dyson_n(s, H)]
+template <typename SelfT>
+__device__ __forceinline__ cd dyson_G_raw(SelfT raw, int s, int H, double C2, double C3, double Sinf, double e,
+                                          double mu, double T, double& kn)
+{
+    kn = (2.0 * (s - H) + 1) * kPi * T;
+    cd sg = sigma_kw(&raw, 0, Sinf, C2, C3, fabs(kn));
+    if (s < H) sg = conjg(sg);
+    const cd den = cd{0.0 - e, kn} + cd{mu, 0.0} - sg;
+    return T / den;
+}
 template <typename SelfT>
 __device__ __forceinline__ cd dyson_G(const SelfT* __restrict__ Self, long ki, long nk8, int s, int H, double C2,
                                       double C3, double Sinf, double e, double mu, double T, double& kn)
 {
-    kn = (2.0 * (s - H) + 1) * kPi * T;
-    const int n = (s < H) ? H - s - 1 : s - H;
-    cd sg = sigma_kw(Self, ki * H + n, Sinf, C2, C3, fabs(kn));
-    if (s < H) sg = conjg(sg);
-    const cd den = cd{0.0 - e, kn} + cd{mu, 0.0} - sg;
-    return T / den;
+    return dyson_G_raw(Self[ki * H + dyson_n(s, H)], s, H, C2, C3, Sinf, e, mu, T, kn);
 }
 
 // Two-pole asymptote data of every Lambda-grid point (cond_opt.cpp:3409-3467): the roots of
@@ -1075,6 +1109,24 @@ struct JJPoles {
     cd a1d, a2d, b1d, b2d;  // a1/den1, a2/den2, b1/den1, b2/den2
 };
 static_assert(sizeof(JJPoles) <= 64 * sizeof(double), "DevArrays::poles holds 64 doubles per point");
+
+// The two factors of pole r in the tau = t asymptote (cond_opt.cpp:3409-3467), both of modulus
+// <= 1 on [0, beta]: e(t) = exp(-t r) (Re r > 0) or exp((beta - t) r), and its partner
+// ej(t) = exp(r (t - beta)) or exp(r t), taken as C / e(t) when rec is set (e ej = C).
+__device__ __forceinline__ cd pole_e(cd r, double t, double T)
+{
+    return (r.x > 0) ? cexpd(-t * r) : cexpd((1.0 / T - t) * r);
+}
+__device__ __forceinline__ cd pole_ej(cd r, double t, double T, int rec, cd C, cd e)
+{
+    if (rec) return C / e;
+    return (r.x > 0) ? cexpd(r * (t - 1.0 / T)) : cexpd(r * t);
+}
+__device__ __forceinline__ bool pole_off(cd v)
+{
+    const double a = fabs(v.x) + fabs(v.y);
+    return !(a >= 1.0e-280 && a <= 2.0);
+}
 
 __global__ void k_jj_poles(const double* __restrict__ ek, const double* __restrict__ hVb,
                            const double* __restrict__ Sinfp, int N, double T, double mu, JJPoles* out)
@@ -1212,6 +1264,33 @@ __global__ void __launch_bounds__(NT, JJ_WAVES_PER_SIMD)
         c2[n] = 0.0;
     }
     double k0acc = 0.0, inf1 = 0.0, inf2 = 0.0, inf4 = 0.0;
+    // The interior samples j = 1..L-1 of the f0V and hV pencils of the next point go from HBM
+    // straight into LDS (global_load_lds, no registers) while the current point's last
+    // accumulation runs: they are issued once Yb is free (after the last transform, which
+    // uses it as its second buffer).  FP64 rows land in Y1 + 1 and Y2 + 1 themselves;
+    // FP32 rows land in the upper half of Yb (stage0, stage1) and are widened at the top of
+    // the next point.  Each wave copies 256-byte chunks c = wave, wave + NT/64, ... of a row,
+    // one dword per lane.
+    constexpr bool kWide = sizeof(S) == sizeof(double);
+    char* const stage0 = kWide ? (char*)(Y1 + 1) : (char*)Y2;
+    char* const stage1 = kWide ? (char*)(Y2 + 1) : (char*)Y2 + 4 * L;
+    auto dma_rows = [&](long qq) {
+        constexpr int NDW = (L - 1) * (int)sizeof(S) / 4;  // dwords per row
+        // the LDS destination (M0) must be wave-uniform: the wave index is made provably so
+        const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+        const char* src0 = (const char*)(f0T + qq * (L + 1) + 1);
+        const char* src1 = (const char*)(hT + qq * (L + 1) + 1);
+        for (int c = wv; c * 64 < NDW; c += NT / 64) {
+            const int i = c * 64 + lane;
+            if (i < NDW) {
+                __builtin_amdgcn_global_load_lds((const void*)(src0 + 4 * i), (void*)(stage0 + 256 * c), 4, 0, 0);
+                __builtin_amdgcn_global_load_lds((const void*)(src1 + 4 * i), (void*)(stage1 + 256 * c), 4, 0, 0);
+            }
+        }
+    };
+    if (blk < nq) dma_rows(blk);
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's LDS DMA has landed (the barrier below covers the others)
+    __syncthreads();
     // point kk of workgroup blk is q = blk + kk * nblk: the workgroups resident on one XCD
     // (consecutive blk) work on consecutive q at the same time and share the cache lines of
     // the tau-major hV / f0V rows and of Sigma
@@ -1223,18 +1302,25 @@ __global__ void __launch_bounds__(NT, JJ_WAVES_PER_SIMD)
         const double e = ek[m + (l + 1) * nk];
         const double v = dek[q];
         const double wt = (m == 0 || m == nk - 1) ? 2.0 : 4.0;
-        // (1) f0V and hV pencils, their splines, one packed FFT
-        {
-            // all loads of the two rows first, then the LDS stores
-            const S* f0q = f0T + q * (L + 1);
-            const S* hq = hT + q * (L + 1);
-            S fa[CS], ha[CS];
+        // (1) f0V and hV pencils (interior samples already in LDS), their splines, one packed FFT
+        if constexpr (kWide) {
+            if (tid == 0) {
+                Y1[0] = f0Vb[q];
+                Y1[L] = f0Vb[nq + q];
+                Y2[0] = hVb[q];
+                Y2[L] = hVb[nq + q];
+            }
+        } else {
+            const float* sf = (const float*)stage0;
+            const float* sh = (const float*)stage1;
+            float fa[CS], ha[CS];
 #pragma unroll
             for (int c = 0; c < CS; c++) {
                 const int j = tid + c * NT;
-                fa[c] = (j > 0 && j < L) ? f0q[j] : (S)0;
-                ha[c] = (j > 0 && j < L) ? hq[j] : (S)0;
+                fa[c] = (j > 0 && j < L) ? sf[j - 1] : 0.0f;
+                ha[c] = (j > 0 && j < L) ? sh[j - 1] : 0.0f;
             }
+            __syncthreads();  // the staged rows overlap Y2
 #pragma unroll
             for (int c = 0; c < CS; c++) {
                 const int j = tid + c * NT;
@@ -1257,28 +1343,38 @@ __global__ void __launch_bounds__(NT, JJ_WAVES_PER_SIMD)
         }
         const double cj03 = -(Y1[0] + Y1[L]), cj3 = -(Y2[0] + Y2[L]);
         __syncthreads();
-        block_fft<L, NT>(X, +1, tw);
+        block_fft_pp<L, NT>(X, W, +1, tw);  // Y1, Y2 (W) are free: their last reads are above
         // (2) per Matsubara slot s (n = s - L/2): Dyson G, J0 (f0V), J (hV) -> W
         double r6[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // qn0 (re, im), sum Gn (re, im), dGp, dJ0
         {
             const double C2 = Sinf2[ki], C3 = Sinf3[ki];
             const double cj01 = -f0Vb[q] - f0Vb[nq + q], cj02 = df0V[q] + df0V[q + nq];
             const double cj2 = dhV[q] + dhV[q + nq];
+            // the global reads of slot s + NT (Sigma, twS) are issued before the work of slot s
+            const SelfT* Sk = Self + ki * H;
+            auto slot_jx = [&](int s) { return (s >= H) ? s - H : s + H; };  // transform index of slot s
+            SelfT sv_next = Sk[dyson_n(tid, H)];
+            cd ts_next = twS[slot_jx(tid)];
 #pragma unroll 1
             for (int s = tid; s < L; s += NT) {
-                const int jx = (s >= H) ? s - H : s + H;  // transform index of slot s
+                const SelfT sv = sv_next;
+                const cd ts = ts_next;
+                if (s + NT < L) {
+                    sv_next = Sk[dyson_n(s + NT, H)];
+                    ts_next = twS[slot_jx(s + NT)];
+                }
+                const int jx = slot_jx(s);
                 const cd Fj = X[jx], Fc = conjg(X[L - 1 - jx]);
                 const cd Af = cd{0.5 * (Fj.x + Fc.x), 0.5 * (Fj.y + Fc.y)};
                 const cd Bh = cd{0.5 * (Fj.y - Fc.y), -0.5 * (Fj.x - Fc.x)};
                 const double knj = (jx < H) ? (2 * jx + 1) * kPi * T : (2 * (jx - L) + 1) * kPi * T;
                 // -i c1/k - c2/k^2 + i c3/k^3 + a/k^4 with one reciprocal of k (cond_opt.cpp:3302-3311)
                 const double ik = 1.0 / knj, ik2 = ik * ik;
-                const cd ts = twS[jx];
                 const cd af = ts * Af, ah = ts * Bh;
                 const cd J0 = cd{(-cj02 + af.x * ik2) * ik2, (-cj01 + (cj03 + af.y * ik) * ik2) * ik};
                 const cd Jv = cd{(-cj2 + ah.x * ik2) * ik2, (-CJ1 + (cj3 + ah.y * ik) * ik2) * ik};
                 double kn;
-                const cd Gv = dyson_G(Self, ki, nk8, s, H, C2, C3, Sinf, e, mu, T, kn);
+                const cd Gv = dyson_G_raw(sv, s, H, C2, C3, Sinf, e, mu, T, kn);
                 const cd t = (Gv * Gv * J0) * iT;
                 r6[0] += t.x;
                 r6[1] += t.y;
@@ -1304,7 +1400,7 @@ __global__ void __launch_bounds__(NT, JJ_WAVES_PER_SIMD)
         const double dJ0 = r6[5] + (pp.fj1 + pp.fj2).x;
         const double dJb = -r6[5] + (pp.fj1b + pp.fj2b).x;
         // (4) to imaginary time: X_j <- (G(tau_j), J(tau_j))
-        block_fft<L, NT>(W, -1, tw);
+        block_fft_pp<L, NT>(W, X, -1, tw);  // X is free: its last reads (step 2) precede block_sum_k
         if (tid == 0) {
             const double t = 1.0 / T;
             const cd ej1 = (P.r1.x > 0) ? cd{1.0, 0.0} : cexpd(P.r1 * t);
@@ -1312,28 +1408,38 @@ __global__ void __launch_bounds__(NT, JJ_WAVES_PER_SIMD)
             const double Jinf = (P.b1 * ej1 / P.den1 + P.b2 * ej2 / P.den2).x;
             Jbeta_s = (cd{W[0].y, 0.0} / phT[L]).x + Jinf;  // W[0].y = FFT_+(Jn)_0 (ph_0 = 1)
         }
+        {
+            // the pole factors are geometric in j: from tau_j to tau_{j+NT} e_p gains
+            // exp(-dt r_p) and ej_p exp(+dt r_p), dt = NT/(L T); a factor that leaves
+            // [1e-280, 2] on the way (underflow of a factor that grows again, overflow of a
+            // step) is evaluated directly again
+            const double dt = NT / (L * T);
+            double t = tid / (L * T);
+            cd e1 = pole_e(P.r1, t, T), e2 = pole_e(P.r2, t, T);
+            cd ej1 = pole_ej(P.r1, t, T, pp.rec1, pp.C1, e1), ej2 = pole_ej(P.r2, t, T, pp.rec2, pp.C2, e2);
+            const cd s1 = cexpd(-dt * P.r1), sj1 = cexpd(dt * P.r1);
+            const cd s2 = cexpd(-dt * P.r2), sj2 = cexpd(dt * P.r2);
+            cd ph_next = phT[tid];
 #pragma unroll 1
-        for (int j = tid; j < L; j += NT) {
-            const double t = j / (L * T);
-            cd e1, ej1, e2, ej2;
-            if (P.r1.x > 0) {
-                e1 = cexpd(-t * P.r1);
-                ej1 = pp.rec1 ? pp.C1 / e1 : cexpd(P.r1 * (t - 1.0 / T));
-            } else {
-                e1 = cexpd((1.0 / T - t) * P.r1);
-                ej1 = pp.rec1 ? pp.C1 / e1 : cexpd(P.r1 * t);
+            for (int j = tid; j < L; j += NT) {
+                const cd ph = ph_next;
+                if (j + NT < L) ph_next = phT[j + NT];
+                const double G0t = (-(pp.a1d * e1) - pp.a2d * e2).x;
+                const double Jinf = (pp.b1d * ej1 + pp.b2d * ej2).x;
+                const cd R = ph * W[j];
+                X[j] = cd{R.x + G0t, R.y + Jinf};
+                if (j + NT < L) {
+                    t = (j + NT) / (L * T);
+                    e1 = e1 * s1;
+                    e2 = e2 * s2;
+                    ej1 = ej1 * sj1;
+                    ej2 = ej2 * sj2;
+                    if (pole_off(e1)) e1 = pole_e(P.r1, t, T);
+                    if (pole_off(e2)) e2 = pole_e(P.r2, t, T);
+                    if (pole_off(ej1)) ej1 = pole_ej(P.r1, t, T, pp.rec1, pp.C1, e1);
+                    if (pole_off(ej2)) ej2 = pole_ej(P.r2, t, T, pp.rec2, pp.C2, e2);
+                }
             }
-            if (P.r2.x > 0) {
-                e2 = cexpd(-t * P.r2);
-                ej2 = pp.rec2 ? pp.C2 / e2 : cexpd(P.r2 * (t - 1.0 / T));
-            } else {
-                e2 = cexpd((1.0 / T - t) * P.r2);
-                ej2 = pp.rec2 ? pp.C2 / e2 : cexpd(P.r2 * t);
-            }
-            const double G0t = (-(pp.a1d * e1) - pp.a2d * e2).x;
-            const double Jinf = (pp.b1d * ej1 + pp.b2d * ej2).x;
-            const cd R = phT[j] * W[j];
-            X[j] = cd{R.x + G0t, R.y + Jinf};
         }
         __syncthreads();
         const double G0 = X[0].x, J0t = X[0].y, Jbeta = Jbeta_s;
@@ -1360,8 +1466,9 @@ __global__ void __launch_bounds__(NT, JJ_WAVES_PER_SIMD)
             const double b = (j < H) ? (Y2[j + 1] - Y2[j]) / hstep : -(Y2[L - j] - Y2[L - 1 - j]) / hstep;
             X[j] = cd{a, b};
         }
-        __syncthreads();
-        block_fft<L, NT>(X, -1, tw);
+        __syncthreads();  // Y1, Y2 read for the last time
+        block_fft_pp<L, NT>(X, W, -1, tw);
+        if (kk + 1 < kJJK && q + nblk < nq) dma_rows(q + nblk);  // W is free again
         const double wv = nf * wt * v;
 #pragma unroll 1
         for (int n = tid; n <= H; n += NT) {
@@ -1386,6 +1493,7 @@ __global__ void __launch_bounds__(NT, JJ_WAVES_PER_SIMD)
         inf1 += 4.0 * dchi * wv * v;
         inf2 += wv * 2.0 * MJ1;
         inf4 -= wv * 2.0 * MJ3;
+        __builtin_amdgcn_s_waitcnt(0);  // the next point's rows have landed in LDS
         __syncthreads();
     }
     if (tid == 0) {
