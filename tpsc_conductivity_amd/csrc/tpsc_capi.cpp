@@ -137,148 +137,11 @@ void prof_flush(tpsc_ctx* c)
 }
 
 // ------------------------------------------------------------------ root finder
-// generique.cpp:604-913 (lims == nullptr) and :917-1234 (lims given): bracketed inverse
-// quadratic steps with secant fallback and bisection on stalls; secant search first when
-// the initial interval has no sign change.  Returns the abscissa of the smallest |f| kept.
-// The reference leaves the secant start unset when f(init[0]) is the smallest; it starts
-// from init[0] here.
+// the bracketing search of tpsc_roots.hpp (find_zero_t) over a host-evaluated function
 bool find_zero(const std::function<double(double)>& f, const double init[3], double* root, const double* lims,
                int* nevals)
 {
-    const double tol = lims ? 1.0e-15 : 1.0e-16;
-    const double eps = (lims ? 2.0 : 10.0) * DBL_EPSILON;
-    const int itmax = lims ? 12 : 20;
-    const double dytol = 0.01;
-    int it = 0, ne = 0;
-    double a = 0.0, b = 0.0, c = 0.0;
-    const double xmin = init[1], xmax = init[2];
-    double x[3] = {init[0], xmin, xmax}, y[3] = {0.0, 0.0, 0.0};
-    auto F = [&](double v) {
-        ne++;
-        return f(v);
-    };
-    auto done = [&](double v, bool ok) {
-        *root = v;
-        if (nevals) *nevals = ne;
-        return ok;
-    };
-    y[1] = F(xmin);
-    y[2] = F(xmax);
-    if (fabs(y[1]) < tol) return done(xmin, true);
-    if (fabs(y[2]) < tol) return done(xmax, true);
-    if (y[1] * y[2] > 0) {
-        y[0] = F(x[0]);
-        if (fabs(y[0]) < tol) return done(x[0], true);
-        if (y[0] * y[2] < 0) {
-            const double xt = x[0];
-            x[0] = (x[0] + x[2]) / 2;
-            x[1] = xt;
-            y[1] = y[0];
-        } else {
-            double yt = y[0], xt = x[0];
-            if (fabs(y[1]) < fabs(yt)) yt = y[1], xt = x[1];
-            if (fabs(y[2]) < fabs(yt)) yt = y[2], xt = x[2];
-            double xt1 = xt + 10 * eps, yt1 = F(xt1);
-            xt1 = xt - yt * ((xt1 - xt) / (yt1 - yt));
-            yt1 = F(xt1);
-            it++;
-            while (fabs(yt1) > tol && it < itmax && yt1 * yt > 0 && std::isfinite(xt1)) {
-                xt = xt1;
-                yt = yt1;
-                xt1 = xt + 10 * eps;
-                yt1 = F(xt1);
-                xt1 = xt - yt * ((xt1 - xt) / (yt1 - yt));
-                yt1 = F(xt1);
-                it++;
-            }
-            if (fabs(yt1) < tol) return done(xt1, true);
-            if (yt1 * yt < 0) {
-                x[0] = (xt + xt1) / 2;
-                x[1] = xt;
-                y[1] = yt;
-                x[2] = xt1;
-                y[2] = yt1;
-            }
-            if (it >= itmax || !std::isfinite(xt1)) return done(xt1, false);
-        }
-    }
-    if (lims && (x[1] < lims[0] || x[1] > lims[1] || x[2] < lims[0] || x[2] > lims[1])) return done(x[0], false);
-    if (fabs(xmax - xmin) < eps) return done(xmin, true);
-    y[0] = F(x[0]);
-    if (fabs(y[0]) < tol) return done(x[0], true);
-
-    int i3, imid, ifar, im, ip;
-    auto sides = [&]() {
-        int i1, i2;
-        if (y[0] * y[1] > 0)
-            i1 = 0, i2 = 1;
-        else if (y[0] * y[2] > 0)
-            i1 = 0, i2 = 2;
-        else
-            i1 = 1, i2 = 2;
-        i3 = 3 - i1 - i2;
-        if (x[i3] > x[i1]) {
-            if (x[i1] > x[i2])
-                ifar = i2, imid = i1;
-            else
-                ifar = i1, imid = i2;
-            im = imid, ip = i3;
-        } else {
-            if (x[i1] < x[i2])
-                ifar = i2, imid = i1;
-            else
-                ifar = i1, imid = i2;
-            im = i3, ip = imid;
-        }
-    };
-    sides();
-    x[ifar] = -y[imid] * (x[i3] - x[imid]) / (y[i3] - y[imid]) + x[imid];
-    y[ifar] = F(x[ifar]);
-    double dy = y[ifar];
-    while (fabs(dy) > tol && it < itmax) {
-        sides();
-        const double den = -x[0] * x[0] * x[2] + x[0] * x[0] * x[1] - x[0] * x[1] * x[1] + x[0] * x[2] * x[2] -
-                           x[2] * x[2] * x[1] + x[2] * x[1] * x[1];
-        if (fabs(den) > 0.0) {
-            a = (x[0] * y[2] - x[0] * y[1] + x[2] * y[1] - y[2] * x[1] - y[0] * x[2] + y[0] * x[1]) / den;
-            b = -(-y[2] * x[1] * x[1] + x[1] * x[1] * y[0] - x[2] * x[2] * y[0] - y[1] * x[0] * x[0] +
-                  y[2] * x[0] * x[0] + y[1] * x[2] * x[2]) / den;
-            c = (y[2] * x[0] * x[0] * x[1] - y[1] * x[0] * x[0] * x[2] - x[1] * x[1] * x[0] * y[2] +
-                 x[2] * x[2] * x[0] * y[1] + x[1] * x[1] * y[0] * x[2] - x[2] * x[2] * y[0] * x[1]) / den;
-        }
-        auto secant_or_stop = [&](bool& stop) {
-            if (fabs(y[i3] - y[imid]) > tol)
-                x[ifar] = -y[imid] * (x[i3] - x[imid]) / (y[i3] - y[imid]) + x[imid];
-            else
-                stop = true;
-        };
-        bool stop = false;
-        if ((b * b - 4.0 * a * c) > 0.0 && fabs(den) > 0.0) {
-            const double xp = (-b + sqrt(b * b - 4.0 * a * c)) / (2 * a);
-            const double xm = (-b - sqrt(b * b - 4.0 * a * c)) / (2 * a);
-            if (xp > x[im] && xp < x[ip])
-                x[ifar] = xp;
-            else if (xm > x[im] && xm < x[ip])
-                x[ifar] = xm;
-            else
-                secant_or_stop(stop);
-        } else
-            secant_or_stop(stop);
-        if (stop) return done(x[imid], true);
-        y[ifar] = F(x[ifar]);
-        double dyprec = fabs(dy);
-        dy = y[ifar];
-        if (fabs(dyprec - fabs(dy)) < dytol * dyprec || dyprec < fabs(dy)) {
-            x[ifar] = (x[i3] + x[imid]) / 2.0;
-            y[ifar] = F(x[ifar]);
-            dy = y[ifar];
-        }
-        it++;
-    }
-    int best = 0;
-    if (fabs(y[1]) < fabs(y[best])) best = 1;
-    if (fabs(y[2]) < fabs(y[best])) best = 2;
-    return done(x[best], true);
+    return find_zero_t(f, init, root, lims, nevals);
 }
 
 // TPSC_TRACE_ROOTS=1 prints every root-finder pass to stderr
@@ -286,55 +149,6 @@ bool trace_roots()
 {
     static const bool on = getenv("TPSC_TRACE_ROOTS") && atoi(getenv("TPSC_TRACE_ROOTS")) != 0;
     return on;
-}
-
-// Root of a monotone f inside (lo, hi) from x by Halley steps, with the bracket narrowed by the
-// sign of every evaluation and a bisection whenever a step leaves it.  fgh(x, v) evaluates
-// v = {f, f', f''} (one pass over the device array) and returns a TPSC status.  Converged when
-// a step is below 1e-13 max(|x|, 1): the cubic order leaves the last iterate at the rounding
-// level of f.  Returns false (and the caller falls back to find_zero) if f' changes sign, the
-// root lands on an end of the interval, or itmax passes are spent.  A step that leaves the
-// bracket is replaced by the Newton step, and that by a bisection.
-bool halley_root(const std::function<int(double, double*)>& fgh, double x, double lo, double hi, int itmax,
-                 double* root, int* nevals, int* rc)
-{
-    int ne = 0, dir = 0;
-    const double lo0 = lo, hi0 = hi;
-    for (int it = 0; it < itmax; it++) {
-        double v[3] = {0.0, 0.0, 0.0};
-        *rc = fgh(x, v);
-        ne++;
-        if (nevals) *nevals = ne;
-        if (*rc) return false;
-        const double f = v[0], f1 = v[1], f2 = v[2];
-        if (trace_roots()) fprintf(stderr, "[tpsc root] pass %d x=%.17g f=%.6e f1=%.6e f2=%.6e [%.17g, %.17g]\n", ne, x, f, f1, f2, lo, hi);
-        if (!std::isfinite(f) || !std::isfinite(f1) || f1 == 0.0) return false;
-        if (dir == 0) dir = (f1 > 0) ? 1 : -1;
-        if ((f1 > 0 ? 1 : -1) != dir) return false;
-        if (f == 0.0) {
-            *root = x;
-            return true;
-        }
-        if ((f > 0) == (dir > 0))
-            hi = x;
-        else
-            lo = x;
-        const double den = 2.0 * f1 * f1 - f * f2;
-        const double dn = -f / f1;  // Newton step
-        const double dh = (den != 0.0 && std::isfinite(den)) ? -2.0 * f * f1 / den : dn;
-        if (fabs(dh) <= 1.0e-13 * std::max(fabs(x), 1.0)) {  // converged: the step is below the tolerance
-            double xn = x + dh;
-            if (!(xn >= lo && xn <= hi)) xn = x;
-            if (xn <= lo0 || xn >= hi0) return false;
-            *root = xn;
-            return true;
-        }
-        double xn = x + dh;
-        if (!(xn > lo && xn < hi)) xn = x + dn;
-        if (!(xn > lo && xn < hi)) xn = 0.5 * (lo + hi);
-        x = xn;
-    }
-    return false;
 }
 
 // ------------------------------------------------------------------ device helpers
@@ -371,6 +185,33 @@ int wait_mapped(tpsc_ctx* c, double* out, int n = 1)
     return 0;
 }
 
+// Halley root search (tpsc_roots.hpp) run on the device: pass() queues one evaluation and the
+// step kernel that advances the state in d.root; kRootChunk passes are queued per host round
+// trip (a settled search turns the remaining ones into early exits).  The outcome is read from
+// the mirror in mapped host memory, host_scalar[8..15].
+constexpr int kRootChunk = 6;
+int device_halley(tpsc_ctx* c, const std::function<hipError_t()>& pass, const HalleyState& s0, double* root,
+                  int* nevals, bool* ok)
+{
+    HIPCHK(c, launch_halley_init(c->d.root, s0, c->stream));
+    volatile const double* m = c->host_scalar + 8;
+    int issued = 0;
+    for (;;) {
+        const int n = std::min(kRootChunk, s0.itmax - issued);
+        for (int k = 0; k < n; k++) HIPCHK(c, pass());
+        issued += n;
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (trace_roots())
+            fprintf(stderr, "[tpsc root] after %d passes: x=%.17g f=%.6e f1=%.6e [%.17g, %.17g] status %d\n",
+                    (int)m[3], m[2], m[4], m[5], m[6], m[7], (int)m[0]);
+        if ((int)m[0] != kRootRunning || issued >= s0.itmax) break;
+    }
+    *ok = (int)m[0] == kRootConverged;
+    *root = m[1];
+    if (nevals) *nevals = (int)m[3];
+    return 0;
+}
+
 // the side stream starts after the work queued so far on the main stream
 int fork_side(tpsc_ctx* c)
 {
@@ -386,7 +227,7 @@ int join_side(tpsc_ctx* c)
     return 0;
 }
 
-// green0::find_mu (green0.cpp:240-262) with the density evaluated on the device
+// green0::find_mu (green0.cpp:240-262) on the device
 int find_mu0(tpsc_ctx* c)
 {
     if (c->mu0_fixed) return 0;
@@ -394,21 +235,12 @@ int find_mu0(tpsc_ctx* c)
         c->mu0 = 0.0;
         return 0;
     }
-    int rc = 0;
-    auto f = [&](double m) {
-        double v = 0.0;
-        if (rc == 0) {
-            hipError_t e = launch_density(c->g, c->d, m, c->p.n, c->dev_scalar, c->stream);
-            if (e != hipSuccess) rc = fail(c, TPSC_E_DEVICE, hipGetErrorString(e));
-            if (rc == 0) rc = wait_mapped(c, &v);
-        }
-        return v;
-    };
-    const double init[3] = {0.0, -8.0, 8.0};
-    double root = 0.0;
-    const bool ok = find_zero(f, init, &root, nullptr, nullptr);
+    // the whole search in one launch (find_zero_t over the device density sums)
+    HIPCHK(c, launch_mu0_root(c->g, c->d, c->p.n, c->dev_scalar, c->stream));
+    double v[3];
+    const int rc = wait_mapped(c, v, 3);
     if (rc) return rc;
-    if (ok) c->mu0 = root;  // on failure the previous mu0 is kept, as in the reference
+    if (v[0] != 0.0) c->mu0 = v[1];  // on failure the previous mu0 is kept, as in the reference
     return 0;
 }
 
@@ -521,25 +353,19 @@ int stage_UspUch_grid(tpsc_ctx* c, int nq0, int nw0, double Usc[2], bool* ok_out
         if (rc == 0) rc = eval_sum_rule(c, 1, x, &v);
         return v;
     };
-    // sum rule, d/dUx and d2/dUx2 in one pass; channel 0: sU = -Ux, channel 1: sU = +Ux
-    auto gsr = [&](int channel, double x, double* v) -> int {
-        HIPCHK(c, launch_sum_rule3(c->g, c->d, channel == 0 ? -1.0 : 1.0, x, c->dev_scalar, c->stream));
-        int r = wait_mapped(c, v, 3);
-        if (r) return r;
-        const double n = c->p.n, nsr = c->nsr;
-        if (channel == 0) {
-            v[0] += n - n * n / 2.0 - nsr + x * nsr * nsr / (2.0 * U);
-            v[1] += nsr * nsr / (2.0 * U);
-        } else
-            v[0] += n - n * n / 2.0 - (nsr + c->Usp * nsr * nsr / (2.0 * U) - nsr * nsr);
-        return 0;
-    };
+    // sum rule, d/dUx and d2/dUx2 in one device pass; channel 0: sU = -Ux, channel 1: sU = +Ux.
+    // The host-known part of each channel's function is a0 + a1 Ux.
+    const double n = c->p.n, nsr = c->nsr;
+    auto pass_sp = [&]() { return launch_sum_rule3_pass(c->g, c->d, -1.0, c->dev_scalar + 8, c->stream); };
+    auto pass_ch = [&]() { return launch_sum_rule3_pass(c->g, c->d, 1.0, c->dev_scalar + 8, c->stream); };
     double root = 0.0;
     double init[3] = {U / 2.0, 0.0, 2.0 / c->chi0max - 2 * DBL_EPSILON};
     bool ok = true;
     int nh = 0;
-    bool hok = halley_root([&](double x, double* v) { return gsr(0, x, v); }, init[0], init[1], init[2], 40, &root,
-                           &nh, &rc);
+    bool hok = false;
+    rc = device_halley(c, pass_sp,
+                       halley_start(init[0], init[1], init[2], 40, n - n * n / 2.0 - nsr, nsr * nsr / (2.0 * U)),
+                       &root, &nh, &hok);
     if (rc) return rc;
     if (!hok) hok = find_zero(fsp, init, &root, nullptr, &c->nev_sp);
     else c->nev_sp = nh;
@@ -552,8 +378,10 @@ int stage_UspUch_grid(tpsc_ctx* c, int nq0, int nw0, double Usc[2], bool* ok_out
     root = 0.0;
     init[0] = 0;
     init[2] = 1000.0 * U;
-    hok = halley_root([&](double x, double* v) { return gsr(1, x, v); }, init[0], init[1], init[2], 40, &root, &nh,
-                      &rc);
+    rc = device_halley(c, pass_ch,
+                       halley_start(init[0], init[1], init[2], 40,
+                                    n - n * n / 2.0 - (nsr + c->Usp * nsr * nsr / (2.0 * U) - nsr * nsr), 0.0),
+                       &root, &nh, &hok);
     if (rc) return rc;
     if (!hok) hok = find_zero(fch, init, &root, nullptr, &c->nev_ch);
     else c->nev_ch = nh;
@@ -690,12 +518,10 @@ int stage_mu(tpsc_ctx* c)
     };
     const double init[3] = {0.0, -10.0, 10.0}, lims[2] = {-20, 20};
     double root = 0.0;
-    auto g3 = [&](double m, double* v) -> int {
-        HIPCHK(c, launch_trace_diffG3(c->g, c->d, m, c->mu0, c->dev_scalar, c->stream));
-        return wait_mapped(c, v, 3);
-    };
+    auto pass = [&]() { return launch_trace_diffG3_pass(c->g, c->d, c->mu0, c->dev_scalar + 8, c->stream); };
     int nh = 0;
-    bool ok = halley_root(g3, init[0], lims[0], lims[1], 40, &root, &nh, &rc);
+    bool ok = false;
+    rc = device_halley(c, pass, halley_start(init[0], lims[0], lims[1], 40, 0.0, 0.0), &root, &nh, &ok);
     if (rc) return rc;
     if (ok)
         c->nev_mu = nh;
@@ -1162,6 +988,7 @@ int tpsc_create(const tpsc_params* p, tpsc_ctx** out)
     rc |= dalloc(c, &d.part, kPartCap);
     rc |= dalloc(c, &d.jjred, 2 * (size_t)nwb + 4);
     rc |= dalloc(c, &d.red, 16);
+    rc |= dalloc(c, &d.root, 1);
     rc |= dalloc(c, &d.twN, (size_t)Nw);
     rc |= dalloc(c, &d.twX, (size_t)Nw);
     rc |= dalloc(c, &d.twS, (size_t)Nw);

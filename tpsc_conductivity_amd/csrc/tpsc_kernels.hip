@@ -4,7 +4,7 @@
 //   k_chi0_tau0, k_chi0_slab     chi0.cpp:310-559   G0(k,+-tau) -> G(r,+-tau) -> chi0(r,tau) -> chi0(q,tau)
 //   k_chi0_pencil                chi0.cpp:566-698   clamped spline in tau + DFT -> chi0(q,iqn), tails
 //   k_sum_rule                   chi.cpp:738-830    sum-rule traces (one candidate U per launch)
-//   k_density                    green0.cpp:182-203 finite-lattice density for mu0
+//   k_mu0_root                   green0.cpp:182-203, 240-262 finite-lattice density and the mu0 root
 //   k_vertex_slab/_pencil        chi.cpp:85-170     V(r,tau)
 //   k_sigma_bnd/_slab/_pencil    green.cpp:1743-2018 Sigma(k,iwn) moments + remainder
 //   k_trace_diffG, k_trace_selfG green.cpp:2929-3182 traces for mu and consistency checks
@@ -71,6 +71,39 @@ __global__ void k_finalize_k(const double* __restrict__ part, int n, double* out
     for (int i = threadIdx.x; i < n; i += blockDim.x) a += p[i];
     a = block_sum(a, sc);
     if (threadIdx.x == 0) out[blockIdx.x] = a;
+}
+
+// ================================================================== device-driven root search
+__global__ void k_halley_init(HalleyState* st, HalleyState s0) { *st = s0; }
+
+// One Halley step after a pass: the three partial sums of part[k * n + i] (k = 0, 1, 2) in the
+// order of k_finalize_k, the update of st, and a copy of its outcome in mirror (mapped host
+// memory): {status, root, x, passes, f, f', lo, hi}.  A settled search is left as it is.
+__global__ void __launch_bounds__(256) k_halley_step(const double* __restrict__ part, int n, HalleyState* st,
+                                                     double* mirror)
+{
+    __shared__ double sc[3][16];
+    const HalleyState s0 = *st;
+    if (s0.status != kRootRunning) return;
+    double v[3];
+    for (int k = 0; k < 3; k++) {
+        double a = 0.0;
+        for (int i = threadIdx.x; i < n; i += blockDim.x) a += part[(long)k * n + i];
+        v[k] = block_sum(a, sc[k]);
+    }
+    if (threadIdx.x == 0) {
+        HalleyState s = s0;
+        halley_update(s, v);
+        *st = s;
+        mirror[0] = s.status;
+        mirror[1] = s.root;
+        mirror[2] = s.x;
+        mirror[3] = s.ne;
+        mirror[4] = s.f;
+        mirror[5] = s.f1;
+        mirror[6] = s.lo;
+        mirror[7] = s.hi;
+    }
 }
 
 // ================================================================== chi0 slabs
@@ -396,12 +429,15 @@ __device__ __forceinline__ void sum_rule3_term(double c, double w, double sU, do
     acc[1] -= w * sgn * 0.5 * c * c * r * r;
     acc[2] += w * 0.5 * c * c * c * r * r * r;
 }
+// Ux is the current iterate of the root search st; a settled search leaves nothing to do.
 __global__ void __launch_bounds__(256)
     k_sum_rule3(const double* __restrict__ chi, const double* __restrict__ inf1, const double* __restrict__ inf2,
-                const double* __restrict__ wq, int nk8, int nwmax, double T, double Nq, double sgn, double Ux,
-                double* part)
+                const double* __restrict__ wq, int nk8, int nwmax, double T, double Nq, double sgn,
+                const HalleyState* __restrict__ st, double* part)
 {
     __shared__ double sc[3 * 4];
+    if (st->status != kRootRunning) return;
+    const double Ux = st->x;
     const int q = blockIdx.x * 256 + threadIdx.x;
     const int RG = gridDim.y;
     const double sU = sgn * Ux;
@@ -430,15 +466,28 @@ __global__ void __launch_bounds__(256)
         for (int k = 0; k < 3; k++) part[k * nblk + b] = acc[k];
 }
 
-// density for the mu0 root (green0.cpp:182-203), one block
-__global__ void k_density(const double* __restrict__ ekw, const double* __restrict__ wq, long nk8, double Nk2,
-                          double T, double mup, double dens, double* out)
+// green0::find_mu (green0.cpp:240-262) in one workgroup: every thread runs the same bracketing
+// search (find_zero_t) on block-wide sums of the finite-lattice density (green0.cpp:182-203),
+// so the control flow stays uniform.  out = {found, mu0, evaluations}.
+__global__ void __launch_bounds__(1024) k_mu0_root(const double* __restrict__ ekw, const double* __restrict__ wq,
+                                                   long nk8, double Nk2, double T, double dens, double* out)
 {
     __shared__ double sc[16];
-    double acc = 0.0;
-    for (long i = threadIdx.x; i < nk8; i += blockDim.x) acc += 2 * wq[i] / ((exp((ekw[i] - mup) / T) + 1.0) * Nk2);
-    acc = block_sum(acc, sc);
-    if (threadIdx.x == 0) out[0] = acc - dens;
+    auto density = [&](double mup) {
+        double acc = 0.0;
+        for (long i = threadIdx.x; i < nk8; i += blockDim.x)
+            acc += 2 * wq[i] / ((exp((ekw[i] - mup) / T) + 1.0) * Nk2);
+        return block_sum(acc, sc) - dens;  // the same total in every thread
+    };
+    const double init[3] = {0.0, -8.0, 8.0};
+    double root = 0.0;
+    int ne = 0;
+    const bool ok = find_zero_t(density, init, &root, nullptr, &ne);
+    if (threadIdx.x == 0) {
+        out[0] = ok ? 1.0 : 0.0;
+        out[1] = root;
+        out[2] = ne;
+    }
 }
 
 // ================================================================== vertex V(r,tau)
@@ -740,14 +789,17 @@ __device__ __forceinline__ void trace_diffG3_term(double wn, double rS, double i
     acc[1] += w * (b2 - a2) * r * r;
     acc[2] += w * (6.0 * a * b2 - 2.0 * a * a2) * r * r * r;
 }
+// pmu is the current iterate of the root search st; a settled search leaves nothing to do.
 template <typename SelfT>
 __global__ void __launch_bounds__(256)
     k_trace_diffG3(const double* __restrict__ ekw, const double* __restrict__ wq, const SelfT* __restrict__ Self,
                    const double* __restrict__ Sinf2, const double* __restrict__ Sinf3,
                    const double* __restrict__ Sinf4, const double* __restrict__ Sinf, int nk8, int Nw, double T,
-                   double mu0, double pmu, double* part)
+                   double mu0, const HalleyState* __restrict__ st, double* part)
 {
     __shared__ double sc[3 * 4];
+    if (st->status != kRootRunning) return;
+    const double pmu = st->x;
     const int lane = threadIdx.x & 63;
     const int q = blockIdx.x * kTraceK + (threadIdx.x >> 6);
     const int H = Nw / 2;
@@ -1250,7 +1302,7 @@ __global__ void __launch_bounds__(NT, JJ_WAVES_PER_SIMD)
     double* Y2 = Yb + (L + 1);
     cd* W = (cd*)Yb;  // L complex values, used while Y1, Y2 are free
     const int nk = N / 2 + 1;
-    const long nq = (long)nk * (nk - 2), nk8 = (long)nk * (nk + 1) / 2;
+    const long nq = (long)nk * (nk - 2);
     const double nf = 1.0 / ((double)N * N);
     const double Sinf = Sinfp[0];
     const double hstep = 1.0 / (L * T), iT = 1.0 / T;
@@ -1705,13 +1757,19 @@ hipError_t launch_sum_rule(const Grid& g, const DevArrays& d, double sU, double*
     return hipGetLastError();
 }
 
-hipError_t launch_sum_rule3(const Grid& g, const DevArrays& d, double sgn, double Ux, double* out, hipStream_t s)
+hipError_t launch_halley_init(HalleyState* st, const HalleyState& s0, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_halley_init, dim3(1), dim3(1), 0, s, st, s0);
+    return hipGetLastError();
+}
+
+hipError_t launch_sum_rule3_pass(const Grid& g, const DevArrays& d, double sgn, double* mirror, hipStream_t s)
 {
     const double Nq = 4.0 * (g.nk - 1) * (g.nk - 1);
     const dim3 gr = red_grid(g);
     hipLaunchKernelGGL(k_sum_rule3, gr, dim3(256), 0, s, d.chiw, d.chi0inf, d.chi0inf2, d.wq, (int)g.nk8, g.Nw / 2,
-                       g.T, Nq, sgn, Ux, d.part);
-    hipLaunchKernelGGL(k_finalize_k, dim3(3), dim3(256), 0, s, d.part, (int)(gr.x * gr.y), out);
+                       g.T, Nq, sgn, (const HalleyState*)d.root, d.part);
+    hipLaunchKernelGGL(k_halley_step, dim3(1), dim3(256), 0, s, d.part, (int)(gr.x * gr.y), d.root, mirror);
     return hipGetLastError();
 }
 
@@ -1730,10 +1788,10 @@ hipError_t launch_reduce_cols(const double* part, int nrows, int ncols, double* 
     return hipGetLastError();
 }
 
-hipError_t launch_density(const Grid& g, const DevArrays& d, double mu, double n, double* out, hipStream_t s)
+hipError_t launch_mu0_root(const Grid& g, const DevArrays& d, double n, double* out, hipStream_t s)
 {
     const double Nk2 = 4.0 * (g.nk - 1) * (g.nk - 1);
-    hipLaunchKernelGGL(k_density, dim3(1), dim3(1024), 0, s, d.ekw, d.wq, g.nk8, Nk2, g.T, mu, n, out);
+    hipLaunchKernelGGL(k_mu0_root, dim3(1), dim3(1024), 0, s, d.ekw, d.wq, g.nk8, Nk2, g.T, n, out);
     return hipGetLastError();
 }
 
@@ -1804,15 +1862,14 @@ hipError_t launch_trace_diffG(const Grid& g, const DevArrays& d, double pmu, dou
     return hipGetLastError();
 }
 
-hipError_t launch_trace_diffG3(const Grid& g, const DevArrays& d, double pmu, double mu0, double* out,
-                               hipStream_t s)
+hipError_t launch_trace_diffG3_pass(const Grid& g, const DevArrays& d, double mu0, double* mirror, hipStream_t s)
 {
     const dim3 gr = trace_grid(g);
     DISPATCH_S(g.prec, {
         hipLaunchKernelGGL(k_trace_diffG3<ST>, gr, dim3(256), 0, s, d.ekw, d.wq, (const ST*)d.Self, d.Sinf2, d.Sinf3,
-                           d.Sinf4, d.Sinf, (int)g.nk8, g.Nw, g.T, mu0, pmu, d.part);
+                           d.Sinf4, d.Sinf, (int)g.nk8, g.Nw, g.T, mu0, (const HalleyState*)d.root, d.part);
     });
-    hipLaunchKernelGGL(k_finalize_k, dim3(3), dim3(256), 0, s, d.part, (int)(gr.x * gr.y), out);
+    hipLaunchKernelGGL(k_halley_step, dim3(1), dim3(256), 0, s, d.part, (int)(gr.x * gr.y), d.root, mirror);
     return hipGetLastError();
 }
 

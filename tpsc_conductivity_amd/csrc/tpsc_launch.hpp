@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include "tpsc_device.hpp"
+#include "tpsc_roots.hpp"
 
 namespace tpsc {
 
@@ -55,6 +56,7 @@ struct DevArrays {
     double* tmp = nullptr;       // scratch, >= 4*nk8
     double* part = nullptr;      // reduction partials
     double* red = nullptr;       // reduction result(s)
+    HalleyState* root = nullptr;  // state of the device-driven root search
     cd* twN = nullptr;           // Nw   exp(-2 pi i t/Nw)
     cd* twX = nullptr;           // Nw   phase tables of the pencil transforms (k_fill_tables)
     cd* twS = nullptr;
@@ -133,12 +135,15 @@ hipError_t launch_chi0_slab(const Grid& g, const DevArrays& d, double mu0, hipSt
 hipError_t launch_chi0_pencil(const Grid& g, const DevArrays& d, hipStream_t s);
 hipError_t launch_chi0max(const Grid& g, const DevArrays& d, hipStream_t s);
 hipError_t launch_sum_rule(const Grid& g, const DevArrays& d, double sU, double* out, hipStream_t s);
-// trace, d/dUx and d2/dUx2 at sU = sgn*Ux into out[0..3)
-hipError_t launch_sum_rule3(const Grid& g, const DevArrays& d, double sgn, double Ux, double* out, hipStream_t s);
-// traceDiffG(pmu) and its first two pmu-derivatives into out[0..3)
-hipError_t launch_trace_diffG3(const Grid& g, const DevArrays& d, double pmu, double mu0, double* out,
-                               hipStream_t s);
-hipError_t launch_density(const Grid& g, const DevArrays& d, double mu, double n, double* out, hipStream_t s);
+// Device-driven Halley search (tpsc_roots.hpp) in d.root: initialise it, then one pass of
+// the sum rule (channel sign sgn, Ux = the current iterate) or of traceDiffG (pmu = the current
+// iterate) with its first two derivatives, followed by the step; mirror (mapped host memory)
+// receives {status, root, x, passes, f, f', lo, hi} after each step.
+hipError_t launch_halley_init(HalleyState* st, const HalleyState& s0, hipStream_t s);
+hipError_t launch_sum_rule3_pass(const Grid& g, const DevArrays& d, double sgn, double* mirror, hipStream_t s);
+hipError_t launch_trace_diffG3_pass(const Grid& g, const DevArrays& d, double mu0, double* mirror, hipStream_t s);
+// green0::find_mu on the device (one workgroup, the whole bracketing search): out = {found, mu0, evaluations}
+hipError_t launch_mu0_root(const Grid& g, const DevArrays& d, double n, double* out, hipStream_t s);
 hipError_t launch_model(const Grid& g, const DevArrays& d, double t, double tp, double tpp, double t3,
                         hipStream_t s);
 hipError_t launch_reduce_cols(const double* part, int nrows, int ncols, double* out, hipStream_t s);
