@@ -154,8 +154,7 @@ def bench_batch(args):
                       "runs": out, "dtype": "f64", "data": "synthetic",
                       "config": {"workload": "tpsc_batch_points(64064, 64): N in {32,48,64,96,128}, "
                                              "T in [0.05,0.5], n in [0.8,1], U in [1,6], t' in [-0.3,0], "
-                                             "Nw = smallest power of two with (Nw-1) pi T >= 20; "
-                                             "N = 48, 96 unsupported (power-of-two lattices only)"}}),
+                                             "Nw = smallest power of two with (Nw-1) pi T >= 20"}}),
           flush=True)
 
 

@@ -34,7 +34,7 @@ extern "C" {
  * cond_opt(int nk0, int Nw0, double n, double T, double eta, double params[4]={t',t'',t3,U})
  * (cond_opt.h:25, cond_opt.cpp:10; green.h:26; chi.h:25; hamiltonien.cpp:14-34). */
 typedef struct tpsc_params {
-    int nk0;         /* N, lattice N x N; power of two, 4..256                  */
+    int nk0;         /* N, lattice N x N; 2^k (4..256) or 3*2^k (12..192)       */
     int Nw0;         /* total number of fermionic +-iwn; power of two, 16..4096 */
     double n;        /* density                                                */
     double T;        /* temperature                                            */
@@ -202,8 +202,7 @@ int tpsc_solve_batch(const tpsc_params* pts, int npts, int workers, tpsc_batch_r
 int tpsc_batch_tune(const tpsc_params* pts, int npts, int* workers, double* rates);
 /* npts random points drawn from std::mt19937_64(seed) in the order N in {32,48,64,96,128},
  * T ~ U[0.05,0.5], n ~ U[0.8,1], U ~ U[1,6], t' ~ U[-0.3,0]; Nw the smallest power of two
- * with (Nw-1) pi T >= 20; FP64, no coarse pass.  N = 48, 96 are not supported by tpsc_create
- * (power-of-two lattices only) and come back with status TPSC_E_INVALID from a batch. */
+ * with (Nw-1) pi T >= 20; FP64, no coarse pass. */
 int tpsc_batch_points(unsigned long long seed, int npts, tpsc_params* out);
 
 /* re-targets a context to another parameter point on the same grid and precision (n, T,

@@ -16,6 +16,7 @@ CONFIGS = {
     "al_8x64": dict(N=8, Nw=64, n=0.9, T=0.25, U=2.5, tp=-0.1, tpp=0.0, t3=0.0, coarse=True, mmin=2),
     "al_16x128": dict(N=16, Nw=128, n=0.875, T=0.1, U=4.0, tp=-0.175, tpp=0.0, t3=0.0, coarse=True, mmin=2),
     "al_32x256": dict(N=32, Nw=256, n=1.1, T=0.15, U=3.0, tp=-0.2, tpp=0.1, t3=0.05, coarse=False, mmin=3),
+    "al_24x128": dict(N=24, Nw=128, n=0.9, T=0.15, U=3.0, tp=-0.15, tpp=0.0, t3=0.0, coarse=False, mmin=2),
 }
 TIGHT = 1e-10
 MIXED = 1e-6

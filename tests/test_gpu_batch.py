@@ -14,7 +14,8 @@ def params(N, Nw, n, T, U, tp):
 
 
 POINTS = [(32, 128, 0.9, 0.15, 3.0, -0.2), (16, 64, 0.875, 0.1, 4.0, -0.175), (8, 32, 0.95, 0.25, 2.0, 0.0),
-          (32, 64, 0.85, 0.3, 5.0, -0.1), (16, 256, 1.0, 0.05, 2.5, 0.0), (24, 64, 0.9, 0.2, 2.0, 0.0)]
+          (32, 64, 0.85, 0.3, 5.0, -0.1), (16, 256, 1.0, 0.05, 2.5, 0.0), (24, 64, 0.9, 0.2, 2.0, 0.0),
+          (20, 64, 0.9, 0.2, 2.0, 0.0)]
 
 
 def test_batch_matches_single_solves():
@@ -22,10 +23,10 @@ def test_batch_matches_single_solves():
     from tpsc_conductivity_amd.tpsc import CondOpt
 
     pts = (capi.Params * len(POINTS))(*[params(*p) for p in POINTS])
-    print("[batch] 6 points, 3 workers", flush=True)
+    print("[batch] 7 points, 3 workers", flush=True)
     res, jj = capi.solve_batch(pts, workers=3, ldc=129)
-    assert res[5].status == -1          # N = 24: not a power of two
-    for i, (N, Nw, n, T, U, tp) in enumerate(POINTS[:5]):
+    assert res[6].status == -1          # N = 20: neither 2^k nor 3*2^k
+    for i, (N, Nw, n, T, U, tp) in enumerate(POINTS[:6]):
         assert res[i].status == 0, i
         g = CondOpt(N, Nw, n, T, params=(tp, 0.0, 0.0, U), coarse_pass=False)
         g.solve()
@@ -45,6 +46,5 @@ def test_batch_tuner_and_the_random_point_set():
     print(f"[batch] tuner: workers {w}, points/s {rates}", flush=True)
     res, _ = capi.solve_batch(pts, workers=w)
     st = np.array([r.status for r in res])
-    bad = np.array([p.nk0 in (48, 96) for p in pts])
-    assert np.all(st[bad] == -1)
-    assert np.all(st[~bad] == 0)
+    assert {p.nk0 for p in pts} == {32, 48, 64, 96, 128}
+    assert np.all(st == 0)

@@ -28,7 +28,7 @@ def oracle_loss(oracle_mod, N, theta, w, Nw=CFG["Nw"], n=CFG["n"], T=CFG["T"]):
     return o.sigma_loss(w)
 
 
-@pytest.mark.parametrize("N", [64, 128])
+@pytest.mark.parametrize("N", [48, 64, 128])
 def test_gradient_matches_oracle_finite_differences(N, oracle_mod):
     from tpsc_conductivity_amd.capi import random_weights
     from tpsc_conductivity_amd.tpsc import CondOpt

@@ -22,6 +22,11 @@ CONFIGS = {
     "cfg1_32x512": dict(N=32, Nw=512, n=1.0, T=0.2, U=2.0, tp=0.0, tpp=0.0, t3=0.0, coarse=True),
     "elec_64x256": dict(N=64, Nw=256, n=1.1, T=0.15, U=3.0, tp=-0.2, tpp=0.1, t3=0.05, coarse=False),
     "cfg2_128x1024": dict(N=128, Nw=1024, n=0.875, T=0.1, U=4.0, tp=-0.175, tpp=0.0, t3=0.0, coarse=False),
+    # lattices 3*2^k (radix-3 register DFT in the slab transforms); the coarse passes run at 24, 96
+    "tri_12x64": dict(N=12, Nw=64, n=0.95, T=0.25, U=2.0, tp=-0.1, tpp=0.0, t3=0.0, coarse=False),
+    "tri_48x256": dict(N=48, Nw=256, n=0.9, T=0.12, U=3.0, tp=-0.15, tpp=0.02, t3=0.0, coarse=True),
+    "tri_96x128": dict(N=96, Nw=128, n=1.05, T=0.2, U=2.5, tp=-0.1, tpp=0.0, t3=0.02, coarse=False),
+    "tri_192x256": dict(N=192, Nw=256, n=0.85, T=0.1, U=4.0, tp=-0.2, tpp=0.05, t3=0.0, coarse=True),
 }
 TIGHT = 1e-10
 REMAINDER = 1e-8

@@ -503,6 +503,11 @@ static size_t al_slab_lds(int N, bool lambda)
     switch (N) {                                                    \
         case 4: { constexpr int NN = 4; __VA_ARGS__; } break;       \
         case 8: { constexpr int NN = 8; __VA_ARGS__; } break;       \
+        case 12: { constexpr int NN = 12; __VA_ARGS__; } break;     \
+        case 24: { constexpr int NN = 24; __VA_ARGS__; } break;     \
+        case 48: { constexpr int NN = 48; __VA_ARGS__; } break;     \
+        case 96: { constexpr int NN = 96; __VA_ARGS__; } break;     \
+        case 192: { constexpr int NN = 192; __VA_ARGS__; } break;   \
         case 16: { constexpr int NN = 16; __VA_ARGS__; } break;     \
         case 32: { constexpr int NN = 32; __VA_ARGS__; } break;     \
         case 64: { constexpr int NN = 64; __VA_ARGS__; } break;     \

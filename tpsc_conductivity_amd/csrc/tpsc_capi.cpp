@@ -914,7 +914,8 @@ int tpsc_create(const tpsc_params* p, tpsc_ctx** out)
         *out = c;
         return fail(c, TPSC_E_INVALID, "precision must be TPSC_PREC_FP64 (0) or TPSC_PREC_MIXED (1)");
     }
-    if (!supported_N(p->nk0) || !supported_Nw(p->Nw0) || (p->coarse_pass && (p->nk0 < 8 || p->Nw0 < 32))) {
+    if (!supported_N(p->nk0) || !supported_Nw(p->Nw0) ||
+        (p->coarse_pass && (p->nk0 < 8 || p->Nw0 < 32 || !supported_N(p->nk0 / 2)))) {
         *out = c;
         return fail(c, TPSC_E_INVALID, "unsupported grid");
     }
@@ -1402,7 +1403,7 @@ int tpsc_set_params(tpsc_ctx* c, const tpsc_params* p)
     if (!c || !p) return TPSC_E_INVALID;
     if (p->nk0 != c->p.nk0 || p->Nw0 != c->p.Nw0 || p->precision != c->p.precision)
         return fail(c, TPSC_E_INVALID, "tpsc_set_params: the grid and the precision must stay those of the context");
-    if (!(p->T > 0.0) || (p->coarse_pass && (p->nk0 < 8 || p->Nw0 < 32)))
+    if (!(p->T > 0.0) || (p->coarse_pass && (p->nk0 < 8 || p->Nw0 < 32 || !supported_N(p->nk0 / 2))))
         return fail(c, TPSC_E_INVALID, "tpsc_set_params: invalid parameters");
     c->p = *p;
     c->nsr = (p->n > 1.0) ? 2.0 - p->n : p->n;  // chi.cpp:18-19

@@ -154,7 +154,7 @@ __device__ __forceinline__ cf shfl_xor_c(cf v)
 }
 
 // ---------------------------------------------------------------------------------
-// in-register DFT of length R (R = 1, 2, 4, 8), forward sign, natural order output
+// in-register DFT of length R (R = 1, 2, 3, 4, 8), forward sign, natural order output
 // ---------------------------------------------------------------------------------
 template <typename C>
 __device__ inline C mul_mi(C a) { return C{a.y, -a.x}; }  // a * (-i)
@@ -168,6 +168,15 @@ __device__ __forceinline__ void reg_dft(C* v)
         const C a = v[0], b = v[1];
         v[0] = a + b;
         v[1] = a - b;
+    } else if constexpr (R == 3) {
+        // w = exp(-2 pi i/3) = -1/2 - i sqrt(3)/2: X1,2 = a - (b + c)/2 -/+ i (sqrt(3)/2) (b - c)
+        using F = decltype(C{}.x);
+        const F h = (F)0.86602540378443864676;
+        const C a = v[0], s = v[1] + v[2], d = v[1] - v[2];
+        const C m = C{a.x - (F)0.5 * s.x, a.y - (F)0.5 * s.y};
+        v[0] = a + s;
+        v[1] = m + mul_mi(C{h * d.x, h * d.y});
+        v[2] = m - mul_mi(C{h * d.x, h * d.y});
     } else if constexpr (R == 4) {
         const C a0 = v[0], a1 = v[1], a2 = v[2], a3 = v[3];
         const C s02 = a0 + a2, d02 = a0 - a2, s13 = a1 + a3, d13 = a1 - a3;
@@ -200,7 +209,8 @@ __device__ __forceinline__ void reg_dft(C* v)
 //                     z_j = x_{j-1}, z_{L-j} = -x_{j-1}
 // Row r element e lives at base[r*rs + e*es].  The DFT of a real even (odd) sequence is
 // real (imaginary), so packing rows r and r+1 as real and imaginary parts yields both
-// transforms from one FFT.  L = R*G, G = min(L,64) lanes per FFT.  The arithmetic runs
+// transforms from one FFT.  L = R*G with G (lanes per FFT) the largest power of two that
+// divides L, at most 64, and R in {1, 2, 3, 4, 8} (L = 2^k or 3*2^k).  The arithmetic runs
 // in the precision of Real (FP64 or FP32).
 // Must be called by every thread of the block (uniform control flow); caller syncs.
 // tw: LDS table tw[t] = exp(-2 pi i t / L), t < L, in the precision of Real.
@@ -308,8 +318,9 @@ __device__ __forceinline__ void fill_twiddles(C* tw, int L)
 template <typename Real, int N, int KIND0, int KIND1>
 __device__ __forceinline__ void slab_r2r_2d(Real* slab, int n0, int n1, int ld, const typename cplx_of<Real>::type* tw)
 {
-    constexpr int G = (N < 64) ? N : 64;
+    constexpr int G = ((N & -N) < 64) ? (N & -N) : 64;  // lanes: the power-of-two part of N, at most 64
     constexpr int R = N / G;
+    static_assert(R == 1 || R == 2 || R == 3 || R == 4 || R == 8, "transform length 2^k or 3*2^k");
     wave_r2r_rows<Real, R, G, KIND1>(slab, n0, ld, 1, tw);  // rows (axis 1)
     __syncthreads();
     wave_r2r_rows<Real, R, G, KIND0>(slab, n1, 1, ld, tw);  // columns (axis 0)

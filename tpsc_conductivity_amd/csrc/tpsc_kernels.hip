@@ -1630,7 +1630,16 @@ static size_t slab_lds(int N, bool lambda, size_t elem)
     return ((n * elem + 15) & ~(size_t)15) + 2 * elem * (size_t)N;
 }
 
-bool supported_N(int N) { return N == 4 || N == 8 || N == 16 || N == 32 || N == 64 || N == 128 || N == 256; }
+// lattice sizes with compiled slab transforms: 2^k (4..256) and 3*2^k (12..192)
+bool supported_N(int N)
+{
+    switch (N) {
+        case 4: case 8: case 12: case 16: case 24: case 32: case 48: case 64: case 96: case 128: case 192: case 256:
+            return true;
+        default:
+            return false;
+    }
+}
 bool supported_Nw(int Nw)
 {
     return Nw == 16 || Nw == 32 || Nw == 64 || Nw == 128 || Nw == 256 || Nw == 512 || Nw == 1024 || Nw == 2048 ||
@@ -1641,6 +1650,11 @@ bool supported_Nw(int Nw)
     switch (N) {                                    \
         case 4: { constexpr int NN = 4; __VA_ARGS__; } break;     \
         case 8: { constexpr int NN = 8; __VA_ARGS__; } break;     \
+        case 12: { constexpr int NN = 12; __VA_ARGS__; } break;   \
+        case 24: { constexpr int NN = 24; __VA_ARGS__; } break;   \
+        case 48: { constexpr int NN = 48; __VA_ARGS__; } break;   \
+        case 96: { constexpr int NN = 96; __VA_ARGS__; } break;   \
+        case 192: { constexpr int NN = 192; __VA_ARGS__; } break; \
         case 16: { constexpr int NN = 16; __VA_ARGS__; } break;   \
         case 32: { constexpr int NN = 32; __VA_ARGS__; } break;   \
         case 64: { constexpr int NN = 64; __VA_ARGS__; } break;   \
