@@ -215,17 +215,23 @@ __device__ __forceinline__ void reg_dft(C* v)
 // Must be called by every thread of the block (uniform control flow); caller syncs.
 // tw: LDS table tw[t] = exp(-2 pi i t / L), t < L, in the precision of Real.
 // ---------------------------------------------------------------------------------
-// cross-lane DIF stages of span HH, HH/2, ..., 1 (lane p of a G-lane group)
+// cross-lane DIF stages of span HH, HH/2, ..., 1 (lane p of a G-lane group).  The lower
+// lane of a pair keeps v + o, the upper one (o - v) w; both are fma(s, v, o) w' with
+// (s, w') = (1, 1) or (-1, w), so no lane evaluates both branches.  Exact: fma(+-1, v, o) rounds
+// as o +- v, and the product with 1 + 0i returns its operand.
 template <int HH, int R, int L, typename C>
 __device__ __forceinline__ void dif_stages(C* v, int p, const C* tw)
 {
     if constexpr (HH >= 1) {
+        using F = decltype(C{}.x);
         const bool upper = (p & HH) != 0;
-        const C w = tw[(p & (HH - 1)) * (L / (2 * HH))];
+        const C wt = tw[(p & (HH - 1)) * (L / (2 * HH))];
+        const C w = upper ? wt : C{(F)1, (F)0};
+        const F sg = upper ? (F)-1 : (F)1;
 #pragma unroll
         for (int k1 = 0; k1 < R; k1++) {
             const C o = shfl_xor_c<HH>(v[k1]);
-            v[k1] = upper ? (o - v[k1]) * w : (v[k1] + o);
+            v[k1] = C{fma(sg, v[k1].x, o.x), fma(sg, v[k1].y, o.y)} * w;
         }
         dif_stages<HH / 2, R, L>(v, p, tw);
     }
