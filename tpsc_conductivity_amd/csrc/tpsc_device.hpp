@@ -811,13 +811,16 @@ __device__ inline double block_sum(double v, double* scratch)
 // (shared by tpsc_kernels.hip and tpsc_al.hip)
 // ---------------------------------------------------------------------------------
 // wedge (l,m) of a packed index, m <= l
+// The first guess for l comes from the hardware FP32 square root (idx < 2^24 is exact in FP32,
+// the guess is within one of the row); the two loops make it exact.
 __device__ __forceinline__ void wedge_lm(long idx, int& l, int& m)
 {
-    int L = (int)((sqrt(8.0 * (double)idx + 1.0) - 1.0) * 0.5);
-    while (widx(L + 1, 0) <= idx) L++;
-    while (widx(L, 0) > idx) L--;
+    const int id = (int)idx;
+    int L = (int)((__builtin_amdgcn_sqrtf(8.0f * (float)id + 1.0f) - 1.0f) * 0.5f);
+    while ((L + 1) * (L + 2) / 2 <= id) L++;
+    while (L * (L + 1) / 2 > id) L--;
     l = L;
-    m = (int)(idx - widx(L, 0));
+    m = id - L * (L + 1) / 2;
 }
 
 // write a wedge value to both symmetric positions of an nk x nk LDS slab
