@@ -881,6 +881,24 @@ __device__ __forceinline__ double hf0_val(int which, double e, double v, double 
     return nf * v * exp((e - mu0) * (1.0 / T - t)) * (t / (ex + 1.0) - 1.0 / (T * (ex + 1.0) * (ex + 1.0)));
 }
 
+// h and f0 of the same point at once: they share both exponentials, exp(-x t) (x = e - mu0 > 0)
+// or exp(x (1/T - t)), and exp(-|x|/T), whose arguments are the same numbers (mu0 - e = -x
+// exactly), so hv, fv equal hf0_val(0, ...), hf0_val(1, ...) bit for bit.
+__device__ __forceinline__ void hf0_pair(double e, double v, double mu0, double T, double t, double nf, double& hv,
+                                         double& fv)
+{
+    const double x = e - mu0;
+    if (x > 0) {
+        const double E = exp(-x * t), ex = exp(-x / T);
+        hv = nf * v * E / (ex + 1.0);
+        fv = nf * v * E * (t / (ex + 1.0) - ex / (T * (ex + 1.0) * (ex + 1.0)));
+    } else {
+        const double E = exp(x * (1.0 / T - t)), ex = exp(x / T);
+        hv = nf * v * E / (ex + 1.0);
+        fv = nf * v * E * (t / (ex + 1.0) - 1.0 / (T * (ex + 1.0) * (ex + 1.0)));
+    }
+}
+
 
 // tau-derivative of the h integrand at tau = 0+ (side 0) or beta- (side 1)
 // (cond_opt.cpp:2745-2781)

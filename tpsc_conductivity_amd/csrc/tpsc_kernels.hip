@@ -303,25 +303,42 @@ __global__ void __launch_bounds__(PEN_THREADS)
                   double* chi0inf2, long nk8, double T, const cd* __restrict__ tw)
 {
     constexpr int NS = L / 2;  // intervals
-    __shared__ double y[NS + 1], M[NS + 1];
+    constexpr int PJ = (NS + PEN_THREADS - 1) / PEN_THREADS;  // intervals per thread
+    // The spline arrays (scratch rr, samples y, moments M: 3 (NS + 1) doubles) live in the FFT
+    // buffer until the transform input is formed: 36 KB of LDS per workgroup, four per CU.
+    static_assert(3 * (NS + 1) <= 2 * L, "the spline arrays fit in the transform buffer");
     __shared__ cd X[L];
     __shared__ double ES[2 * PEN_THREADS];
     __shared__ double sc[16];
     const long q = xcd_remap(blockIdx.x, gridDim.x);
     if (q >= nk8) return;
     const int tid = threadIdx.x, nt = blockDim.x;
-    double* rr = (double*)X;  // spline scratch aliases the FFT buffer
+    double* rr = (double*)X;
+    double* y = rr + (NS + 1);
+    double* M = y + (NS + 1);
     for (int j = tid; j <= NS; j += nt) y[j] = (double)chit[q + (long)j * nk8];
     const double d0 = dchiqt0[q];
     const double h = 1.0 / (L * T);
     __syncthreads();
     block_spline_moments<NS + 1, PEN_THREADS>(y, M, rr, NS, h, d0, 0.0, ES);
-    double integ = 0.0;
-    for (int j = tid; j < NS; j += nt) {
-        const double a6 = (M[j + 1] - M[j]) / h;
-        X[j] = cd{a6, 0.0};
-        X[L - 1 - j] = cd{-a6, 0.0};
-        integ += h * (y[j] + y[j + 1]) * 0.5 - h * h * h * (M[j] + M[j + 1]) / 24.0;
+    double integ = 0.0, a6v[PJ];
+#pragma unroll
+    for (int c = 0; c < PJ; c++) {
+        const int j = tid + c * nt;
+        a6v[c] = 0.0;
+        if (j < NS) {
+            a6v[c] = (M[j + 1] - M[j]) / h;
+            integ += h * (y[j] + y[j + 1]) * 0.5 - h * h * h * (M[j] + M[j + 1]) / 24.0;
+        }
+    }
+    __syncthreads();  // every read of y and M is done: X may be written
+#pragma unroll
+    for (int c = 0; c < PJ; c++) {
+        const int j = tid + c * nt;
+        if (j < NS) {
+            X[j] = cd{a6v[c], 0.0};
+            X[L - 1 - j] = cd{-a6v[c], 0.0};
+        }
     }
     integ = block_sum(integ, sc);
     block_fft<L, PEN_THREADS>(X, +1, tw);
@@ -330,9 +347,10 @@ __global__ void __launch_bounds__(PEN_THREADS)
         const double inf = -2.0 * d0;
         for (int j = 1 + tid; j <= NS; j += nt) {
             const double wn = 2 * j * kPi * T;
-            const double ang = 2 * j * kPi / L;
+            // cos, sin of 2 pi j / L from the twiddle table tw[j] = exp(-2 pi i j / L)
+            const cd e = tw[j];
             chiw[q + (long)j * nk8] =
-                inf / (wn * wn) + ((1.0 - cos(ang)) * X[j].x + sin(ang) * X[j].y) / (wn * wn * wn * wn);
+                inf / (wn * wn) + ((1.0 - e.x) * X[j].x - e.y * X[j].y) / (wn * wn * wn * wn);
         }
         if (tid == 0) {
             chi0inf[q] = inf;
@@ -955,17 +973,25 @@ __global__ void __launch_bounds__(SLAB_THREADS, SlabOcc<S>::waves)
     const int j = blockIdx.x;
     const double t = (double)j / (T * Nw);
     const int tid = threadIdx.x, nt = blockDim.x;
-    for (int which = 0; which < 2; which++) {
-        for (long s = tid; s < nq; s += nt) {
-            const int l = (int)(s / nk), m = (int)(s % nk);
-            slab[s] = (S)hf0_val(which, ek[m + (l + 1) * nk], dek[s], mu0, T, t, nf);
-        }
-        __syncthreads();
-        slab_r2r_2d<S, N, 1, 0>(slab, nk - 2, nk, nk, tw);
-        S* o = (which == 0 ? h : f0) + (long)j * nq;
-        for (long s = tid; s < nq; s += nt) o[s] = slab[s];
-        __syncthreads();
+    S* const oh = h + (long)j * nq;
+    S* const of = f0 + (long)j * nq;
+    // h and f0 together (hf0_pair); f0 waits in its output row, read back by the thread that
+    // wrote it, while h is transformed
+    for (int s = tid; s < (int)nq; s += nt) {
+        const int l = s / nk, m = s % nk;
+        double hv, fv;
+        hf0_pair(ek[m + (l + 1) * nk], dek[s], mu0, T, t, nf, hv, fv);
+        slab[s] = (S)hv;
+        of[s] = (S)fv;
     }
+    __syncthreads();
+    slab_r2r_2d<S, N, 1, 0>(slab, nk - 2, nk, nk, tw);
+    for (int s = tid; s < (int)nq; s += nt) oh[s] = slab[s];
+    __syncthreads();
+    for (int s = tid; s < (int)nq; s += nt) slab[s] = of[s];
+    __syncthreads();
+    slab_r2r_2d<S, N, 1, 0>(slab, nk - 2, nk, nk, tw);
+    for (int s = tid; s < (int)nq; s += nt) of[s] = slab[s];
 }
 
 // Boundary data of hV and f0V in FP64 (cond_opt.cpp:2697-2735, 2745-3114 at tau = 0, beta).
