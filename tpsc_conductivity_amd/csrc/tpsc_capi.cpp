@@ -188,12 +188,12 @@ int wait_mapped(tpsc_ctx* c, double* out, int n = 1)
 // Halley root search (tpsc_roots.hpp) run on the device: pass() queues one evaluation and the
 // step kernel that advances the state in d.root; kRootChunk passes are queued per host round
 // trip (a settled search turns the remaining ones into early exits).  The outcome is read from
-// the mirror in mapped host memory, host_scalar[8..15].
+// the mirror in mapped host memory, host_scalar[8..15], which says "running" until it settles.
 constexpr int kRootChunk = 6;
 int device_halley(tpsc_ctx* c, const std::function<hipError_t()>& pass, const HalleyState& s0, double* root,
                   int* nevals, bool* ok)
 {
-    HIPCHK(c, launch_halley_init(c->d.root, s0, c->stream));
+    HIPCHK(c, launch_halley_init(c->d.root, s0, c->dev_scalar + 8, c->stream));
     volatile const double* m = c->host_scalar + 8;
     int issued = 0;
     for (;;) {
@@ -201,8 +201,8 @@ int device_halley(tpsc_ctx* c, const std::function<hipError_t()>& pass, const Ha
         for (int k = 0; k < n; k++) HIPCHK(c, pass());
         issued += n;
         HIPCHK(c, hipStreamSynchronize(c->stream));
-        if (trace_roots())
-            fprintf(stderr, "[tpsc root] after %d passes: x=%.17g f=%.6e f1=%.6e [%.17g, %.17g] status %d\n",
+        if (trace_roots() && (int)m[0] != kRootRunning)
+            fprintf(stderr, "[tpsc root] settled after %d passes: x=%.17g f=%.6e f1=%.6e [%.17g, %.17g] status %d\n",
                     (int)m[3], m[2], m[4], m[5], m[6], m[7], (int)m[0]);
         if ((int)m[0] != kRootRunning || issued >= s0.itmax) break;
     }

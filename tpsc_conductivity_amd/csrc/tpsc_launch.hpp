@@ -138,8 +138,9 @@ hipError_t launch_sum_rule(const Grid& g, const DevArrays& d, double sU, double*
 // Device-driven Halley search (tpsc_roots.hpp) in d.root: initialise it, then one pass of
 // the sum rule (channel sign sgn, Ux = the current iterate) or of traceDiffG (pmu = the current
 // iterate) with its first two derivatives, followed by the step; mirror (mapped host memory)
-// receives {status, root, x, passes, f, f', lo, hi} after each step.
-hipError_t launch_halley_init(HalleyState* st, const HalleyState& s0, hipStream_t s);
+// holds {status, root, x, passes, f, f', lo, hi}: "running" from the initialisation, the
+// outcome once the search has settled.
+hipError_t launch_halley_init(HalleyState* st, const HalleyState& s0, double* mirror, hipStream_t s);
 hipError_t launch_sum_rule3_pass(const Grid& g, const DevArrays& d, double sgn, double* mirror, hipStream_t s);
 hipError_t launch_trace_diffG3_pass(const Grid& g, const DevArrays& d, double mu0, double* mirror, hipStream_t s);
 // green0::find_mu on the device (one workgroup, the whole bracketing search): out = {found, mu0, evaluations}
