@@ -74,35 +74,48 @@ __global__ void k_finalize_k(const double* __restrict__ part, int n, double* out
 }
 
 // ================================================================== device-driven root search
-__global__ void k_halley_init(HalleyState* st, HalleyState s0) { *st = s0; }
+// mirror (mapped host memory) = {status, root, x, passes, f, f', lo, hi} of the search: written
+// here with status "running", and by the step that settles it (one system-scope write per
+// search instead of one per pass)
+__global__ void k_halley_init(HalleyState* st, HalleyState s0, double* mirror)
+{
+    *st = s0;
+    mirror[0] = s0.status;
+    mirror[1] = s0.root;
+    mirror[2] = s0.x;
+    mirror[3] = s0.ne;
+}
 
-// One Halley step after a pass: the three partial sums of part[k * n + i] (k = 0, 1, 2) in the
-// order of k_finalize_k, the update of st, and a copy of its outcome in mirror (mapped host
-// memory): {status, root, x, passes, f, f', lo, hi}.  A settled search is left as it is.
+// One Halley step after a pass: the three partial sums of part[k * n + i] (k = 0, 1, 2), each
+// in the order of k_finalize_k, the update of st, and the outcome in mirror once the search has
+// settled.  A settled search is left as it is.
 __global__ void __launch_bounds__(256) k_halley_step(const double* __restrict__ part, int n, HalleyState* st,
                                                      double* mirror)
 {
-    __shared__ double sc[3][16];
+    __shared__ double sc[3 * 4];
     const HalleyState s0 = *st;
     if (s0.status != kRootRunning) return;
-    double v[3];
-    for (int k = 0; k < 3; k++) {
-        double a = 0.0;
-        for (int i = threadIdx.x; i < n; i += blockDim.x) a += part[(long)k * n + i];
-        v[k] = block_sum(a, sc[k]);
+    double v[3] = {0.0, 0.0, 0.0};
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        v[0] += part[i];
+        v[1] += part[(long)n + i];
+        v[2] += part[2L * n + i];
     }
+    block_sum_k<3>(v, sc);
     if (threadIdx.x == 0) {
         HalleyState s = s0;
         halley_update(s, v);
         *st = s;
-        mirror[0] = s.status;
-        mirror[1] = s.root;
-        mirror[2] = s.x;
-        mirror[3] = s.ne;
-        mirror[4] = s.f;
-        mirror[5] = s.f1;
-        mirror[6] = s.lo;
-        mirror[7] = s.hi;
+        if (s.status != kRootRunning) {
+            mirror[0] = s.status;
+            mirror[1] = s.root;
+            mirror[2] = s.x;
+            mirror[3] = s.ne;
+            mirror[4] = s.f;
+            mirror[5] = s.f1;
+            mirror[6] = s.lo;
+            mirror[7] = s.hi;
+        }
     }
 }
 
@@ -1797,9 +1810,9 @@ hipError_t launch_sum_rule(const Grid& g, const DevArrays& d, double sU, double*
     return hipGetLastError();
 }
 
-hipError_t launch_halley_init(HalleyState* st, const HalleyState& s0, hipStream_t s)
+hipError_t launch_halley_init(HalleyState* st, const HalleyState& s0, double* mirror, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_halley_init, dim3(1), dim3(1), 0, s, st, s0);
+    hipLaunchKernelGGL(k_halley_init, dim3(1), dim3(1), 0, s, st, s0, mirror);
     return hipGetLastError();
 }
 
