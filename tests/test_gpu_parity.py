@@ -258,3 +258,22 @@ def test_sp_corr_length(name, oracle_mod):
     got, ref = r["g"].sp_corr_length(), r["o"].sp_corr_length()
     assert got[0] == pytest.approx(ref[0], rel=1e-12)       # hot-spot vF (host, same algorithm)
     assert np.allclose(got[1:], ref[1:], rtol=1e-8, atol=0)  # from chi0(q, 0) at 1e-10
+
+
+def test_queued_lambda_work_then_a_new_point():
+    """The Lambda stage's hV/f0V slabs are queued on a third stream at the end of the vertex
+    stage. A context left after the self-energy (that work never waited for) and re-targeted to
+    another point must give that point's solve bit for bit, as a fresh context does."""
+    from tpsc_conductivity_amd import capi
+    from tpsc_conductivity_amd.tpsc import CondOpt
+
+    a = CondOpt(32, 256, 0.9, 0.15, params=(-0.1, 0.0, 0.0, 3.0), coarse_pass=False)
+    a.calc_Self_FFT_spline()          # queues the Lambda prep work and returns
+    p = capi.Params(32, 256, 0.85, 0.2, 0.1, -0.2, 0.02, 0.0, 2.5, 0, 0, 1.0)
+    a.ctx.check(capi.lib().tpsc_set_params(a.ctx.handle, p), "set_params")
+    a.solve()
+    b = CondOpt(32, 256, 0.85, 0.2, params=(-0.2, 0.02, 0.0, 2.5), coarse_pass=False)
+    b.solve()
+    for f in ("chijj1", "chijj2", "Self", "V"):
+        assert np.array_equal(a.get(f), b.get(f)), f
+    assert a.scalar("mu") == b.scalar("mu") and a.scalar("kx0") == b.scalar("kx0")

@@ -65,6 +65,11 @@ struct tpsc_ctx {
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     hipEvent_t ev_tau0 = nullptr, ev_hf0 = nullptr;
     bool hf0_pending = false;  // h(r,tau), f0(r,tau) queued on the side stream at the chi0 stage
+    // third stream: the Lambda stage's hV/f0V slabs, transposes and boundary data, queued at the
+    // end of the vertex stage and overlapped with the self-energy and mu stages
+    hipStream_t prep = nullptr;
+    hipEvent_t ev_hv = nullptr;
+    bool hv_pending = false;
     DevArrays d{};
     ALArrays al{};
     double* chi_tau = nullptr;
@@ -212,6 +217,17 @@ int device_halley(tpsc_ctx* c, const std::function<hipError_t()>& pass, const Ha
     return 0;
 }
 
+// the side stream waits for the Lambda-stage work queued on the prep stream (which reads and
+// rewrites h, f0 and the transposed rows); nothing is left pending afterwards
+int settle_prep(tpsc_ctx* c)
+{
+    if (!c->hv_pending) return 0;
+    HIPCHK(c, hipStreamWaitEvent(c->side, c->ev_hv, 0));
+    HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_hv, 0));
+    c->hv_pending = false;
+    return 0;
+}
+
 // the side stream starts after the work queued so far on the main stream
 int fork_side(tpsc_ctx* c)
 {
@@ -277,7 +293,10 @@ bool full_grid(const tpsc_ctx* c) { return c->g.N == c->p.nk0 && c->g.Nw == c->p
 int stage_chi0(tpsc_ctx* c, int nq0, int nw0)
 {
     if (nq0 > c->p.nk0 || nw0 > c->p.Nw0) return fail(c, TPSC_E_INVALID, "chi0 grid larger than the context grid");
-    int rc = set_grid(c, nq0, nw0);
+    // Lambda-stage work of an earlier point (prep stream) still reads eps(k), V, h and f0
+    int rc = settle_prep(c);
+    if (rc) return rc;
+    rc = set_grid(c, nq0, nw0);
     if (rc) return rc;
     hipStream_t s = c->stream;
     if ((rc = fork_side(c))) return rc;
@@ -430,6 +449,8 @@ int stage_vertex(tpsc_ctx* c)
         int rc = stage_UspUch(c);
         if (rc) return rc;
     }
+    int rc = settle_prep(c);  // queued Lambda work still reads V
+    if (rc) return rc;
     prof_begin(c, "vertex_slab");
     HIPCHK(c, launch_vertex_slab(c->g, c->d, c->p.U, c->Usp, c->Uch, c->stream));
     prof_end(c);
@@ -439,6 +460,26 @@ int stage_vertex(tpsc_ctx* c)
     HIPCHK(c, hipStreamSynchronize(c->stream));
     prof_flush(c);
     c->have_V = true;
+    // The Lambda stage's hV, f0V slabs (h, f0 times V), their point-major transposes and the
+    // boundary data need V, mu0, Usp, Uch and the h, f0 slabs of the chi0 stage only: they start
+    // here on the prep stream and overlap the self-energy and mu stages.
+    if (c->hf0_pending) {
+        const Grid& g = c->g;
+        const double fact4 = c->p.U * (3.0 * c->Usp + c->Uch) / 4;
+        HIPCHK(c, hipStreamWaitEvent(c->prep, c->ev_hf0, 0));
+        c->hf0_pending = false;
+        prof_begin(c, "jj_hVf0V_slab", c->prep);
+        HIPCHK(c, launch_hVf0V_slab(g, c->d, c->prep));
+        prof_end(c);
+        prof_begin(c, "jj_transpose", c->prep);
+        HIPCHK(c, launch_jj_transpose(g, c->d, c->prep));
+        prof_end(c);
+        prof_begin(c, "jj_hf0_bnd", c->prep);
+        HIPCHK(c, launch_hf0_bnd(g, c->d, c->mu0, fact4, c->prep));
+        prof_end(c);
+        HIPCHK(c, hipEventRecord(c->ev_hv, c->prep));
+        c->hv_pending = true;
+    }
     return 0;
 }
 
@@ -553,31 +594,39 @@ int stage_chijj(tpsc_ctx* c)
     const Grid& g = c->g;
     const double fact4 = c->p.U * (3.0 * c->Usp + c->Uch) / 4;
     hipStream_t s = c->stream;
-    int rc = fork_side(c);
+    // hV, f0V, their transposes and the boundary data were queued at the vertex stage (prep
+    // stream); otherwise they are made here
+    const bool prepared = c->hv_pending;
+    int rc = settle_prep(c);
     if (rc) return rc;
-    prof_begin(c, "jj_hf0_bnd", c->side);
-    HIPCHK(c, launch_hf0_bnd(g, c->d, c->mu0, fact4, c->side));
-    prof_end(c);
+    if ((rc = fork_side(c))) return rc;
+    if (!prepared) {
+        prof_begin(c, "jj_hf0_bnd", c->side);
+        HIPCHK(c, launch_hf0_bnd(g, c->d, c->mu0, fact4, c->side));
+        prof_end(c);
+    }
     prof_begin(c, "jj_kx0_rows", c->side);
     HIPCHK(c, launch_kx0_rows(g, c->d, c->mu, c->side));
     prof_end(c);
     prof_begin(c, "jj_poles", c->side);
     HIPCHK(c, launch_jj_poles(g, c->d, c->mu, c->side));
     prof_end(c);
-    if (c->hf0_pending) {
-        HIPCHK(c, hipStreamWaitEvent(s, c->ev_hf0, 0));
-        c->hf0_pending = false;
-    } else {
-        prof_begin(c, "jj_hf0_slab");
-        HIPCHK(c, launch_hf0_slab(g, c->d, c->mu0, s));
+    if (!prepared) {
+        if (c->hf0_pending) {
+            HIPCHK(c, hipStreamWaitEvent(s, c->ev_hf0, 0));
+            c->hf0_pending = false;
+        } else {
+            prof_begin(c, "jj_hf0_slab");
+            HIPCHK(c, launch_hf0_slab(g, c->d, c->mu0, s));
+            prof_end(c);
+        }
+        prof_begin(c, "jj_hVf0V_slab");
+        HIPCHK(c, launch_hVf0V_slab(g, c->d, s));
+        prof_end(c);
+        prof_begin(c, "jj_transpose");
+        HIPCHK(c, launch_jj_transpose(g, c->d, s));
         prof_end(c);
     }
-    prof_begin(c, "jj_hVf0V_slab");
-    HIPCHK(c, launch_hVf0V_slab(g, c->d, s));
-    prof_end(c);
-    prof_begin(c, "jj_transpose");
-    HIPCHK(c, launch_jj_transpose(g, c->d, s));
-    prof_end(c);
     if ((rc = join_side(c))) return rc;
     prof_begin(c, "jj_pencil");
     HIPCHK(c, launch_jj_pencil(g, c->d, c->mu, s));
@@ -930,6 +979,8 @@ int tpsc_create(const tpsc_params* p, tpsc_ctx** out)
         return fail(c, TPSC_E_DEVICE, "no HIP device available");
     HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     HIPCHK(c, hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+    HIPCHK(c, hipStreamCreateWithFlags(&c->prep, hipStreamNonBlocking));
+    HIPCHK(c, hipEventCreateWithFlags(&c->ev_hv, hipEventDisableTiming));
     HIPCHK(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
     HIPCHK(c, hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
     HIPCHK(c, hipEventCreateWithFlags(&c->ev_tau0, hipEventDisableTiming));
@@ -1005,6 +1056,7 @@ void tpsc_destroy(tpsc_ctx* c)
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->side) (void)hipStreamSynchronize(c->side);
+    if (c->prep) (void)hipStreamSynchronize(c->prep);
     for (auto& pp : c->pending) {
         (void)hipEventDestroy(pp.a);
         (void)hipEventDestroy(pp.b);
@@ -1016,6 +1068,8 @@ void tpsc_destroy(tpsc_ctx* c)
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
     if (c->ev_tau0) (void)hipEventDestroy(c->ev_tau0);
     if (c->ev_hf0) (void)hipEventDestroy(c->ev_hf0);
+    if (c->ev_hv) (void)hipEventDestroy(c->ev_hv);
+    if (c->prep) (void)hipStreamDestroy(c->prep);
     if (c->side) (void)hipStreamDestroy(c->side);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1412,6 +1466,10 @@ int tpsc_set_params(tpsc_ctx* c, const tpsc_params* p)
     c->mu_ok = false;
     c->grad_ready = false;
     c->hf0_pending = false;
+    if (c->hv_pending) {  // queued Lambda work of the previous point still reads h, f0
+        HIPCHK(c, hipStreamSynchronize(c->prep));
+        c->hv_pending = false;
+    }
     c->chijj1.clear();
     c->chijj2.clear();
     c->chijj3.clear();
