@@ -68,8 +68,9 @@ struct tpsc_ctx {
     // third stream: the Lambda stage's hV/f0V slabs, transposes and boundary data, queued at the
     // end of the vertex stage and overlapped with the self-energy and mu stages
     hipStream_t prep = nullptr;
-    hipEvent_t ev_hv = nullptr;
+    hipEvent_t ev_hv = nullptr, ev_v = nullptr;
     bool hv_pending = false;
+    bool sinf_async = false;  // Sigma_inf copied to host_scalar[4], not yet read
     DevArrays d{};
     ALArrays al{};
     double* chi_tau = nullptr;
@@ -228,6 +229,16 @@ int settle_prep(tpsc_ctx* c)
     return 0;
 }
 
+// the host value of Sigma_inf (its copy was queued by the self-energy stage)
+int sigma_inf_ready(tpsc_ctx* c)
+{
+    if (!c->sinf_async) return 0;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->Sigma_inf = ((volatile double*)c->host_scalar)[4];
+    c->sinf_async = false;
+    return 0;
+}
+
 // the side stream starts after the work queued so far on the main stream
 int fork_side(tpsc_ctx* c)
 {
@@ -281,7 +292,6 @@ int set_grid(tpsc_ctx* c, int N, int Nw)
     hipStream_t s = c->stream;
     HIPCHK(c, launch_model(g, c->d, hop_t(c), c->p.tp, c->p.tpp, c->p.t3, s));
     HIPCHK(c, launch_fill_tw(c->d, Nw, s));
-    HIPCHK(c, hipStreamSynchronize(s));
     c->have_chi0 = false;
     return find_mu0(c);
 }
@@ -457,7 +467,7 @@ int stage_vertex(tpsc_ctx* c)
     prof_begin(c, "vertex_pencil");
     HIPCHK(c, launch_vertex_pencil(c->g, c->d, c->p.U, c->Usp, c->Uch, c->stream));
     prof_end(c);
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipEventRecord(c->ev_v, c->stream));  // V and V(r,0) are complete here
     prof_flush(c);
     c->have_V = true;
     // The Lambda stage's hV, f0V slabs (h, f0 times V), their point-major transposes and the
@@ -467,6 +477,7 @@ int stage_vertex(tpsc_ctx* c)
         const Grid& g = c->g;
         const double fact4 = c->p.U * (3.0 * c->Usp + c->Uch) / 4;
         HIPCHK(c, hipStreamWaitEvent(c->prep, c->ev_hf0, 0));
+        HIPCHK(c, hipStreamWaitEvent(c->prep, c->ev_v, 0));
         c->hf0_pending = false;
         prof_begin(c, "jj_hVf0V_slab", c->prep);
         HIPCHK(c, launch_hVf0V_slab(g, c->d, c->prep));
@@ -500,8 +511,9 @@ int stage_sigma(tpsc_ctx* c)
     prof_begin(c, "sigma_pencil");
     HIPCHK(c, launch_sigma_pencil(c->g, c->d, c->stream));
     prof_end(c);
-    rc = read_scalar(c, c->d.Sinf, &c->Sigma_inf);
-    if (rc) return rc;
+    // Sigma_inf is read by the host only for reports: its copy completes with the next wait
+    HIPCHK(c, hipMemcpyAsync(c->host_scalar + 4, c->d.Sinf, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    c->sinf_async = true;
     prof_flush(c);
     c->have_sigma = true;
     c->mu_ok = false;
@@ -518,10 +530,14 @@ int reduce_trace(tpsc_ctx* c, int which, double* out)
 int stage_trace_selfG(tpsc_ctx* c)
 {
     if (!c->have_sigma) return fail(c, TPSC_E_ORDER, "traceSelfG before the self-energy");
-    double s0 = 0.0, s1 = 0.0;
-    int rc = reduce_trace(c, 0, &s0);
-    if (!rc) rc = reduce_trace(c, 1, &s1);
+    // both reductions queued, one wait (each ends in its own mapped slot)
+    HIPCHK(c, launch_trace_selfG(c->g, c->d, c->mu0, c->mu, 0, c->dev_scalar, c->stream));
+    HIPCHK(c, launch_trace_selfG(c->g, c->d, c->mu0, c->mu, 1, c->dev_scalar + 1, c->stream));
+    double v[2];
+    int rc = wait_mapped(c, v, 2);
     if (rc) return rc;
+    if ((rc = sigma_inf_ready(c))) return rc;
+    const double s0 = v[0], s1 = v[1];
     c->trSG = s0;
     c->have_trSG = true;
     c->trSG_corr = s0 + s1;
@@ -914,6 +930,7 @@ int get_self_remainder(tpsc_ctx* c, double* dst)
 int get_sigma_kw(tpsc_ctx* c, double* dst)
 {
     if (!c->have_sigma) return fail(c, TPSC_E_ORDER, "self-energy not computed");
+    if (int rc = sigma_inf_ready(c)) return rc;
     const Grid& g = c->g;
     const int H = g.Nw / 2;
     const size_t n = (size_t)H * g.nk8;
@@ -981,6 +998,7 @@ int tpsc_create(const tpsc_params* p, tpsc_ctx** out)
     HIPCHK(c, hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
     HIPCHK(c, hipStreamCreateWithFlags(&c->prep, hipStreamNonBlocking));
     HIPCHK(c, hipEventCreateWithFlags(&c->ev_hv, hipEventDisableTiming));
+    HIPCHK(c, hipEventCreateWithFlags(&c->ev_v, hipEventDisableTiming));
     HIPCHK(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
     HIPCHK(c, hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
     HIPCHK(c, hipEventCreateWithFlags(&c->ev_tau0, hipEventDisableTiming));
@@ -1069,6 +1087,7 @@ void tpsc_destroy(tpsc_ctx* c)
     if (c->ev_tau0) (void)hipEventDestroy(c->ev_tau0);
     if (c->ev_hf0) (void)hipEventDestroy(c->ev_hf0);
     if (c->ev_hv) (void)hipEventDestroy(c->ev_hv);
+    if (c->ev_v) (void)hipEventDestroy(c->ev_v);
     if (c->prep) (void)hipStreamDestroy(c->prep);
     if (c->side) (void)hipStreamDestroy(c->side);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1488,6 +1507,8 @@ int tpsc_get_params(tpsc_ctx* c, tpsc_params* out)
 int tpsc_get_scalar(tpsc_ctx* c, int which, double* v)
 {
     if (!c || !v) return TPSC_E_INVALID;
+    if (which == TPSC_S_SIGMA_INF)
+        if (int rc = sigma_inf_ready(c)) return rc;
     switch (which) {
         case TPSC_S_MU0: *v = c->mu0; break;
         case TPSC_S_CHI0MAX: *v = c->chi0max; break;
