@@ -392,8 +392,11 @@ int stage_UspUch_grid(tpsc_ctx* c, int nq0, int nw0, double Usc[2], bool* ok_out
     bool ok = true;
     int nh = 0;
     bool hok = false;
+    // the reference's start U/2 can lie beyond the Stoner bound 2/chi0max, where the function has
+    // its pole: the derivative-based search starts inside the bracket instead (same root)
+    const double x0 = (init[0] > init[1] && init[0] < init[2]) ? init[0] : 0.5 * (init[1] + init[2]);
     rc = device_halley(c, pass_sp,
-                       halley_start(init[0], init[1], init[2], 40, n - n * n / 2.0 - nsr, nsr * nsr / (2.0 * U)),
+                       halley_start(x0, init[1], init[2], 40, n - n * n / 2.0 - nsr, nsr * nsr / (2.0 * U)),
                        &root, &nh, &hok);
     if (rc) return rc;
     if (!hok) hok = find_zero(fsp, init, &root, nullptr, &c->nev_sp);
@@ -647,7 +650,7 @@ int stage_chijj(tpsc_ctx* c)
     prof_begin(c, "jj_pencil");
     HIPCHK(c, launch_jj_pencil(g, c->d, c->mu, s));
     prof_end(c);
-    const int nblk = (int)((g.nq + kJJK - 1) / kJJK);
+    const int nblk = (int)((g.nq + jj_points(g.Nw) - 1) / jj_points(g.Nw));
     const int nwb = g.nwb;
     HIPCHK(c, launch_reduce_cols(c->d.jjpart, nblk, 2 * nwb, c->d.jjred, c->stream));
     HIPCHK(c, launch_reduce_cols(c->d.jjscal, nblk, 4, c->d.jjred + 2 * nwb, c->stream));
@@ -1050,7 +1053,7 @@ int tpsc_create(const tpsc_params* p, tpsc_ctx** out)
     rc |= dalloc(c, &d.hT, (size_t)(Nw + 1) * std::max<long>(nq, 1), es);
     rc |= dalloc(c, &d.f0T, (size_t)(Nw + 1) * std::max<long>(nq, 1), es);
     rc |= dalloc(c, &d.poles, 64 * (size_t)std::max<long>(nq, 1));
-    const long nblk = (nq + kJJK - 1) / kJJK;
+    const long nblk = (nq + jj_points(Nw) - 1) / jj_points(Nw);
     rc |= dalloc(c, &d.jjpart, (size_t)std::max<long>(nblk, 1) * 2 * nwb);
     rc |= dalloc(c, &d.jjscal, (size_t)std::max<long>(nblk, 1) * 4);
     rc |= dalloc(c, &d.kx0part, 2 * (size_t)nk);

@@ -37,9 +37,6 @@ struct SlabOcc<float> {
     static constexpr int waves = 8;
 };
 #define PEN_THREADS 256
-#ifndef JJ_THREADS
-#define JJ_THREADS 256
-#endif
 
 // ================================================================== small helpers
 __global__ void k_fill_tw(cd* tw, int L)
@@ -1296,7 +1293,7 @@ __global__ void __launch_bounds__(256) k_transpose(const S* __restrict__ in, S* 
 
 // Per-k Lambda_jj contributions (cond_opt.cpp:3179-3822).  One workgroup of NT threads per
 // Lambda-grid point at a time; workgroup blk handles the points q = blk + kk*nblk,
-// kk < kJJK, and accumulates into its own partial rows part[blk*2*nwb + n] (bubble) and
+// kk < jj_points(L), and accumulates into its own partial rows part[blk*2*nwb + n] (bubble) and
 // part[blk*2*nwb + nwb + n] (Maki-Thompson).
 //
 // The six length-L transforms of the reference run as three complex FFTs of pairs of
@@ -1385,7 +1382,8 @@ __global__ void __launch_bounds__(NT, JJ_WAVES_PER_SIMD)
     // point kk of workgroup blk is q = blk + kk * nblk: the workgroups resident on one XCD
     // (consecutive blk) work on consecutive q at the same time and share the cache lines of
     // the tau-major hV / f0V rows and of Sigma
-    for (int kk = 0; kk < kJJK; kk++) {
+    constexpr int KP = jj_points(L);
+    for (int kk = 0; kk < KP; kk++) {
         const long q = (long)blk + kk * nblk;
         if (q >= nq) break;
         const int l = (int)(q / nk), m = (int)(q % nk);
@@ -1559,7 +1557,7 @@ __global__ void __launch_bounds__(NT, JJ_WAVES_PER_SIMD)
         }
         __syncthreads();  // Y1, Y2 read for the last time
         block_fft_pp<L, NT>(X, W, -1, tw);
-        if (kk + 1 < kJJK && q + nblk < nq) dma_rows(q + nblk);  // W is free again
+        if (kk + 1 < KP && q + nblk < nq) dma_rows(q + nblk);  // W is free again
         const double wv = nf * wt * v;
 #pragma unroll 1
         for (int n = tid; n <= H; n += NT) {
@@ -1991,9 +1989,10 @@ hipError_t launch_jj_transpose(const Grid& g, const DevArrays& d, hipStream_t s)
 
 hipError_t launch_jj_pencil(const Grid& g, const DevArrays& d, double mu, hipStream_t s)
 {
-    const unsigned nblk = (unsigned)((g.nq + kJJK - 1) / kJJK);
+    const unsigned nblk = (unsigned)((g.nq + jj_points(g.Nw) - 1) / jj_points(g.Nw));
     DISPATCH_S(g.prec, DISPATCH_LJJ(g.Nw, {
-        hipLaunchKernelGGL((k_jj_pencil<LL, JJ_THREADS, SS, ST>), dim3(nblk), dim3(JJ_THREADS), 0, s, d.ek, d.dek,
+        constexpr int NTJ = jj_threads(LL);
+        hipLaunchKernelGGL((k_jj_pencil<LL, NTJ, SS, ST>), dim3(nblk), dim3(NTJ), 0, s, d.ek, d.dek,
                            d.d2ek, (const SS*)d.hT, (const SS*)d.f0T, d.dhV, d.df0V, d.hVb, d.f0Vb,
                            (const JJPoles*)d.poles, (const ST*)d.Self, d.Sinf2, d.Sinf3, d.Sinf, g.N, g.T, mu, d.twN,
                            d.twX, d.twS, d.twM, d.ph, d.jjpart, d.jjscal);

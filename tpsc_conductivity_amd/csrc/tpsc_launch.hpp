@@ -126,7 +126,12 @@ struct Grid {
 constexpr int kRedBlocks = 1024;
 constexpr int kRowGroups = 32;       // frequency-row groups of the reduction grids
 constexpr int kPartCap = 8192;       // capacity of DevArrays::part
-constexpr int kJJK = 16;  // k-points per Lambda workgroup
+constexpr int kJJK = 16;  // k-points per Lambda workgroup for Nw > 256
+// Lambda pencil shape: up to Nw = 256 one wavefront per workgroup (its barriers are then free)
+// and 4 points per workgroup, so that small grids still spread over the CUs; beyond, 256
+// threads and kJJK points
+constexpr int jj_points(int Nw) { return Nw <= 256 ? 4 : kJJK; }
+constexpr int jj_threads(int Nw) { return Nw <= 256 ? 64 : 256; }
 
 // ---- launchers; each returns hipGetLastError() of its launch
 hipError_t launch_fill_tw(const DevArrays& d, int L, hipStream_t s);
