@@ -68,6 +68,13 @@ struct tpsc_ctx {
     // third stream: the Lambda stage's hV/f0V slabs, transposes and boundary data, queued at the
     // end of the vertex stage and overlapped with the self-energy and mu stages
     hipStream_t prep = nullptr;
+    // captured chunks of root-search passes, one per (grid, precision, search kind), made once
+    // the chunk has been launched TPSC_GRAPH_AFTER times (default 16)
+    struct RootGraph {
+        int N, Nw, prec, kind, uses;
+        hipGraphExec_t exec;
+    };
+    std::vector<RootGraph> root_graphs;
     hipEvent_t ev_hv = nullptr, ev_v = nullptr;
     bool hv_pending = false;
     bool sinf_async = false;  // Sigma_inf copied to host_scalar[4], not yet read
@@ -195,16 +202,65 @@ int wait_mapped(tpsc_ctx* c, double* out, int n = 1)
 // step kernel that advances the state in d.root; kRootChunk passes are queued per host round
 // trip (a settled search turns the remaining ones into early exits).  The outcome is read from
 // the mirror in mapped host memory, host_scalar[8..15], which says "running" until it settles.
+// A full chunk of a grid, precision and search kind that has already been launched
+// TPSC_GRAPH_AFTER times (default 16) is captured into a graph and replayed from
+// then on: the pass and step kernels see the point only through d.root.  Capture and
+// instantiation cost ~1.2 ms per graph and a replay saves ~15 us per chunk on small grids, so
+// only a grid that keeps being solved gets one.  TPSC_NO_GRAPHS=1 never captures.
 constexpr int kRootChunk = 6;
-int device_halley(tpsc_ctx* c, const std::function<hipError_t()>& pass, const HalleyState& s0, double* root,
+enum { kRootUsp = 0, kRootUch = 1, kRootMu = 2 };
+int root_chunk_graph(tpsc_ctx* c, int kind, const std::function<hipError_t()>& pass, hipGraphExec_t* out)
+{
+    static const int after = getenv("TPSC_GRAPH_AFTER") ? atoi(getenv("TPSC_GRAPH_AFTER")) : 16;
+    const Grid& g = c->g;
+    *out = nullptr;
+    tpsc_ctx::RootGraph* rg = nullptr;
+    for (auto& r : c->root_graphs)
+        if (r.N == g.N && r.Nw == g.Nw && r.prec == g.prec && r.kind == kind) rg = &r;
+    if (!rg) {
+        c->root_graphs.push_back({g.N, g.Nw, g.prec, kind, 0, nullptr});
+        rg = &c->root_graphs.back();
+    }
+    if (rg->exec) {
+        *out = rg->exec;
+        return 0;
+    }
+    if (rg->uses++ < after) return 0;  // launched one by one until then
+    hipGraph_t graph = nullptr;
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < kRootChunk && e == hipSuccess; k++) e = pass();
+    const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
+    if (e == hipSuccess) e = ec;
+    hipGraphExec_t exec = nullptr;
+    if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(c, TPSC_E_DEVICE, std::string("root-search graph: ") + hipGetErrorString(e));
+    rg->exec = exec;
+    *out = exec;
+    return 0;
+}
+
+int device_halley(tpsc_ctx* c, int kind, const std::function<hipError_t()>& pass, HalleyState s0, double* root,
                   int* nevals, bool* ok)
 {
+    static const bool graphs = !(getenv("TPSC_NO_GRAPHS") && atoi(getenv("TPSC_NO_GRAPHS")) != 0);
+    s0.T = c->g.T;
+    s0.mu0 = c->mu0;
     HIPCHK(c, launch_halley_init(c->d.root, s0, c->dev_scalar + 8, c->stream));
     volatile const double* m = c->host_scalar + 8;
     int issued = 0;
     for (;;) {
         const int n = std::min(kRootChunk, s0.itmax - issued);
-        for (int k = 0; k < n; k++) HIPCHK(c, pass());
+        hipGraphExec_t exec = nullptr;
+        if (graphs && n == kRootChunk) {
+            int rc = root_chunk_graph(c, kind, pass, &exec);
+            if (rc) return rc;
+        }
+        if (exec)
+            HIPCHK(c, hipGraphLaunch(exec, c->stream));
+        else
+            for (int k = 0; k < n; k++) HIPCHK(c, pass());
         issued += n;
         HIPCHK(c, hipStreamSynchronize(c->stream));
         if (trace_roots() && (int)m[0] != kRootRunning)
@@ -395,7 +451,7 @@ int stage_UspUch_grid(tpsc_ctx* c, int nq0, int nw0, double Usc[2], bool* ok_out
     // the reference's start U/2 can lie beyond the Stoner bound 2/chi0max, where the function has
     // its pole: the derivative-based search starts inside the bracket instead (same root)
     const double x0 = (init[0] > init[1] && init[0] < init[2]) ? init[0] : 0.5 * (init[1] + init[2]);
-    rc = device_halley(c, pass_sp,
+    rc = device_halley(c, kRootUsp, pass_sp,
                        halley_start(x0, init[1], init[2], 40, n - n * n / 2.0 - nsr, nsr * nsr / (2.0 * U)),
                        &root, &nh, &hok);
     if (rc) return rc;
@@ -410,7 +466,7 @@ int stage_UspUch_grid(tpsc_ctx* c, int nq0, int nw0, double Usc[2], bool* ok_out
     root = 0.0;
     init[0] = 0;
     init[2] = 1000.0 * U;
-    rc = device_halley(c, pass_ch,
+    rc = device_halley(c, kRootUch, pass_ch,
                        halley_start(init[0], init[1], init[2], 40,
                                     n - n * n / 2.0 - (nsr + c->Usp * nsr * nsr / (2.0 * U) - nsr * nsr), 0.0),
                        &root, &nh, &hok);
@@ -578,10 +634,10 @@ int stage_mu(tpsc_ctx* c)
     };
     const double init[3] = {0.0, -10.0, 10.0}, lims[2] = {-20, 20};
     double root = 0.0;
-    auto pass = [&]() { return launch_trace_diffG3_pass(c->g, c->d, c->mu0, c->dev_scalar + 8, c->stream); };
+    auto pass = [&]() { return launch_trace_diffG3_pass(c->g, c->d, c->dev_scalar + 8, c->stream); };
     int nh = 0;
     bool ok = false;
-    rc = device_halley(c, pass, halley_start(init[0], lims[0], lims[1], 40, 0.0, 0.0), &root, &nh, &ok);
+    rc = device_halley(c, kRootMu, pass, halley_start(init[0], lims[0], lims[1], 40, 0.0, 0.0), &root, &nh, &ok);
     if (rc) return rc;
     if (ok)
         c->nev_mu = nh;
@@ -1089,6 +1145,9 @@ void tpsc_destroy(tpsc_ctx* c)
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
     if (c->ev_tau0) (void)hipEventDestroy(c->ev_tau0);
     if (c->ev_hf0) (void)hipEventDestroy(c->ev_hf0);
+    for (auto& rg : c->root_graphs)
+        if (rg.exec) (void)hipGraphExecDestroy(rg.exec);
+    c->root_graphs.clear();
     if (c->ev_hv) (void)hipEventDestroy(c->ev_hv);
     if (c->ev_v) (void)hipEventDestroy(c->ev_v);
     if (c->prep) (void)hipStreamDestroy(c->prep);

@@ -457,15 +457,16 @@ __device__ __forceinline__ void sum_rule3_term(double c, double w, double sU, do
     acc[1] -= w * sgn * 0.5 * c * c * r * r;
     acc[2] += w * 0.5 * c * c * c * r * r * r;
 }
-// Ux is the current iterate of the root search st; a settled search leaves nothing to do.
+// Ux (and T) come from the root search st, so that a captured pass graph serves every point;
+// a settled search leaves nothing to do.
 __global__ void __launch_bounds__(256)
     k_sum_rule3(const double* __restrict__ chi, const double* __restrict__ inf1, const double* __restrict__ inf2,
-                const double* __restrict__ wq, int nk8, int nwmax, double T, double Nq, double sgn,
+                const double* __restrict__ wq, int nk8, int nwmax, double Nq, double sgn,
                 const HalleyState* __restrict__ st, double* part)
 {
     __shared__ double sc[3 * 4];
     if (st->status != kRootRunning) return;
-    const double Ux = st->x;
+    const double Ux = st->x, T = st->T;
     const int q = blockIdx.x * 256 + threadIdx.x;
     const int RG = gridDim.y;
     const double sU = sgn * Ux;
@@ -817,17 +818,18 @@ __device__ __forceinline__ void trace_diffG3_term(double wn, double rS, double i
     acc[1] += w * (b2 - a2) * r * r;
     acc[2] += w * (6.0 * a * b2 - 2.0 * a * a2) * r * r * r;
 }
-// pmu is the current iterate of the root search st; a settled search leaves nothing to do.
+// pmu (and T, mu0) come from the root search st, so that a captured pass graph serves every
+// point; a settled search leaves nothing to do.
 template <typename SelfT>
 __global__ void __launch_bounds__(256)
     k_trace_diffG3(const double* __restrict__ ekw, const double* __restrict__ wq, const SelfT* __restrict__ Self,
                    const double* __restrict__ Sinf2, const double* __restrict__ Sinf3,
-                   const double* __restrict__ Sinf4, const double* __restrict__ Sinf, int nk8, int Nw, double T,
-                   double mu0, const HalleyState* __restrict__ st, double* part)
+                   const double* __restrict__ Sinf4, const double* __restrict__ Sinf, int nk8, int Nw,
+                   const HalleyState* __restrict__ st, double* part)
 {
     __shared__ double sc[3 * 4];
     if (st->status != kRootRunning) return;
-    const double pmu = st->x;
+    const double pmu = st->x, T = st->T, mu0 = st->mu0;
     const int lane = threadIdx.x & 63;
     const int q = blockIdx.x * kTraceK + (threadIdx.x >> 6);
     const int H = Nw / 2;
@@ -1819,7 +1821,7 @@ hipError_t launch_sum_rule3_pass(const Grid& g, const DevArrays& d, double sgn, 
     const double Nq = 4.0 * (g.nk - 1) * (g.nk - 1);
     const dim3 gr = red_grid(g);
     hipLaunchKernelGGL(k_sum_rule3, gr, dim3(256), 0, s, d.chiw, d.chi0inf, d.chi0inf2, d.wq, (int)g.nk8, g.Nw / 2,
-                       g.T, Nq, sgn, (const HalleyState*)d.root, d.part);
+                       Nq, sgn, (const HalleyState*)d.root, d.part);
     hipLaunchKernelGGL(k_halley_step, dim3(1), dim3(256), 0, s, d.part, (int)(gr.x * gr.y), d.root, mirror);
     return hipGetLastError();
 }
@@ -1913,12 +1915,12 @@ hipError_t launch_trace_diffG(const Grid& g, const DevArrays& d, double pmu, dou
     return hipGetLastError();
 }
 
-hipError_t launch_trace_diffG3_pass(const Grid& g, const DevArrays& d, double mu0, double* mirror, hipStream_t s)
+hipError_t launch_trace_diffG3_pass(const Grid& g, const DevArrays& d, double* mirror, hipStream_t s)
 {
     const dim3 gr = trace_grid(g);
     DISPATCH_S(g.prec, {
         hipLaunchKernelGGL(k_trace_diffG3<ST>, gr, dim3(256), 0, s, d.ekw, d.wq, (const ST*)d.Self, d.Sinf2, d.Sinf3,
-                           d.Sinf4, d.Sinf, (int)g.nk8, g.Nw, g.T, mu0, (const HalleyState*)d.root, d.part);
+                           d.Sinf4, d.Sinf, (int)g.nk8, g.Nw, (const HalleyState*)d.root, d.part);
     });
     hipLaunchKernelGGL(k_halley_step, dim3(1), dim3(256), 0, s, d.part, (int)(gr.x * gr.y), d.root, mirror);
     return hipGetLastError();

@@ -142,12 +142,13 @@ hipError_t launch_chi0max(const Grid& g, const DevArrays& d, hipStream_t s);
 hipError_t launch_sum_rule(const Grid& g, const DevArrays& d, double sU, double* out, hipStream_t s);
 // Device-driven Halley search (tpsc_roots.hpp) in d.root: initialise it, then one pass of
 // the sum rule (channel sign sgn, Ux = the current iterate) or of traceDiffG (pmu = the current
-// iterate) with its first two derivatives, followed by the step; mirror (mapped host memory)
+// iterate) with its first two derivatives, followed by the step.  T and mu0 are read from the
+// state, so a pass depends on the point only through d.root.  mirror (mapped host memory)
 // holds {status, root, x, passes, f, f', lo, hi}: "running" from the initialisation, the
 // outcome once the search has settled.
 hipError_t launch_halley_init(HalleyState* st, const HalleyState& s0, double* mirror, hipStream_t s);
 hipError_t launch_sum_rule3_pass(const Grid& g, const DevArrays& d, double sgn, double* mirror, hipStream_t s);
-hipError_t launch_trace_diffG3_pass(const Grid& g, const DevArrays& d, double mu0, double* mirror, hipStream_t s);
+hipError_t launch_trace_diffG3_pass(const Grid& g, const DevArrays& d, double* mirror, hipStream_t s);
 // green0::find_mu on the device (one workgroup, the whole bracketing search): out = {found, mu0, evaluations}
 hipError_t launch_mu0_root(const Grid& g, const DevArrays& d, double n, double* out, hipStream_t s);
 hipError_t launch_model(const Grid& g, const DevArrays& d, double t, double tp, double tpp, double t3,

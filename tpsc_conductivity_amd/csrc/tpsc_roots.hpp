@@ -29,6 +29,7 @@ struct HalleyState {
     double x, lo, hi, lo0, hi0, root;
     double a0, a1;  // f += a0 + a1 x, f' += a1
     double f, f1;   // last evaluation (diagnostics)
+    double T, mu0;  // inputs of the pass kernels that change from point to point
     int dir, ne, status, itmax;
 };
 
