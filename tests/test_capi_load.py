@@ -41,7 +41,7 @@ def _params(**kw):
     return Params(**d)
 
 
-@pytest.mark.parametrize("bad", [dict(nk0=24), dict(nk0=512), dict(Nw0=100), dict(precision=2), dict(T=0.0),
+@pytest.mark.parametrize("bad", [dict(nk0=20), dict(nk0=512), dict(Nw0=100), dict(precision=2), dict(T=0.0),
                                  dict(coarse_pass=1, nk0=4)])
 def test_invalid_parameters_are_rejected_without_a_device(hip_lib, bad):
     h = C.c_void_p(None)
