@@ -1534,22 +1534,33 @@ __global__ void __launch_bounds__(NT, JJ_WAVES_PER_SIMD)
         }
         __syncthreads();
         const double G0 = X[0].x, J0t = X[0].y, Jbeta = Jbeta_s;
-        // (5) Maki-Thompson samples G(tau) J(-tau) -> Y1; bubble samples -G(tau) G(-tau) -> Y2
+        // (5) Maki-Thompson samples G(tau) J(-tau) -> Y1; bubble samples -G(tau) G(-tau) -> Y2 over
+        // the whole of [0, beta].  They are symmetric about beta/2 (the product is the same in
+        // both orders), so the clamped spline over [0, beta] with end slopes (dchi, -dchi) is, on
+        // [0, beta/2], the reference's spline with slopes (dchi, 0), and its integral is twice
+        // that of the half: the two splines run in one call.
         for (int j = tid; j < L; j += NT) {
             const cd gj = X[j];
             Y1[j] = gj.x * gj.y;
-            if (j <= H) Y2[j] = (j == 0) ? G0 * (G0 + 1.0) : -(gj.x * X[L - j].x);
+            Y2[j] = (j == 0) ? G0 * (G0 + 1.0) : -(gj.x * X[L - j].x);
         }
-        if (tid == 0) Y1[L] = (-G0 - 1.0) * Jbeta;
+        if (tid == 0) {
+            Y1[L] = (-G0 - 1.0) * Jbeta;
+            Y2[L] = G0 * (G0 + 1.0);
+        }
         __syncthreads();
         const double gd0 = dGp * J0t + G0 * dJ0;
         const double gdN = dGm * Jbeta - (G0 + 1.0) * dJb;
         const double MJ1 = Y1[0] - Y1[L];
         const double dchi = dGp * (G0 + 1.0) + G0 * dGm;
-        block_spline_inplace<CS, NT>(Y1, L, hstep, gd0, gdN, ES, nullptr);
-        double integ[1] = {0.0};
-        block_spline_inplace<CS, NT>(Y2, H, hstep, dchi, 0.0, ES, integ);
-        block_sum_k<1>(integ, sc);
+        double integ[2] = {0.0, 0.0};
+        {
+            double* const ys[2] = {Y1, Y2};
+            const double d0s[2] = {gd0, dchi}, dNs[2] = {gdN, -dchi};
+            block_spline_inplace_k<CS, NT, 2>(ys, L, hstep, d0s, dNs, ES, integ);
+        }
+        double ibub[1] = {integ[1]};  // the bubble over [0, beta]
+        block_sum_k<1>(ibub, sc);
         const double MJ3 = Y1[0] - Y1[L];
         // (6) packed transform of the two moment sequences
         for (int j = tid; j < L; j += NT) {
@@ -1566,7 +1577,7 @@ __global__ void __launch_bounds__(NT, JJ_WAVES_PER_SIMD)
             const cd Fn = X[n], Fc = conjg(X[(L - n) & (L - 1)]);
             double bub;
             if (n == 0) {
-                bub = 2 * integ[0];
+                bub = ibub[0];
                 c2[0] -= wv * qn0;
             } else {
                 const double iw = 1.0 / (2 * n * kPi * T), iw2 = iw * iw;
