@@ -77,6 +77,9 @@ struct tpsc_ctx {
     std::vector<RootGraph> root_graphs;
     hipEvent_t ev_hv = nullptr, ev_v = nullptr;
     bool hv_pending = false;
+    bool hv_slabs_due = false;  // the hV, f0V slabs are still to be queued (after the Sigma slab)
+    bool hv_dropped = false;    // they were dropped: the Lambda stage makes them
+    hipEvent_t ev_sig = nullptr;
     bool sinf_async = false;  // Sigma_inf copied to host_scalar[4], not yet read
     DevArrays d{};
     ALArrays al{};
@@ -276,9 +279,19 @@ int device_halley(tpsc_ctx* c, int kind, const std::function<hipError_t()>& pass
 
 // the side stream waits for the Lambda-stage work queued on the prep stream (which reads and
 // rewrites h, f0 and the transposed rows); nothing is left pending afterwards
-int settle_prep(tpsc_ctx* c)
+int queue_hv_slabs(tpsc_ctx* c);
+// need_slabs: the caller is the Lambda stage, which wants the hV, f0V slabs (queued now if they
+// were not yet); any other caller drops slabs that were not queued, and the Lambda stage then
+// makes them itself
+int settle_prep(tpsc_ctx* c, bool need_slabs)
 {
     if (!c->hv_pending) return 0;
+    if (need_slabs) {
+        if (int rc = queue_hv_slabs(c)) return rc;
+    } else if (c->hv_slabs_due) {
+        c->hv_slabs_due = false;
+        c->hv_dropped = true;
+    }
     HIPCHK(c, hipStreamWaitEvent(c->side, c->ev_hv, 0));
     HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_hv, 0));
     c->hv_pending = false;
@@ -360,7 +373,7 @@ int stage_chi0(tpsc_ctx* c, int nq0, int nw0)
 {
     if (nq0 > c->p.nk0 || nw0 > c->p.Nw0) return fail(c, TPSC_E_INVALID, "chi0 grid larger than the context grid");
     // Lambda-stage work of an earlier point (prep stream) still reads eps(k), V, h and f0
-    int rc = settle_prep(c);
+    int rc = settle_prep(c, false);
     if (rc) return rc;
     rc = set_grid(c, nq0, nw0);
     if (rc) return rc;
@@ -518,7 +531,7 @@ int stage_vertex(tpsc_ctx* c)
         int rc = stage_UspUch(c);
         if (rc) return rc;
     }
-    int rc = settle_prep(c);  // queued Lambda work still reads V
+    int rc = settle_prep(c, false);  // queued Lambda work still reads V
     if (rc) return rc;
     prof_begin(c, "vertex_slab");
     HIPCHK(c, launch_vertex_slab(c->g, c->d, c->p.U, c->Usp, c->Uch, c->stream));
@@ -529,27 +542,42 @@ int stage_vertex(tpsc_ctx* c)
     HIPCHK(c, hipEventRecord(c->ev_v, c->stream));  // V and V(r,0) are complete here
     prof_flush(c);
     c->have_V = true;
-    // The Lambda stage's hV, f0V slabs (h, f0 times V), their point-major transposes and the
-    // boundary data need V, mu0, Usp, Uch and the h, f0 slabs of the chi0 stage only: they start
-    // here on the prep stream and overlap the self-energy and mu stages.
+    // The Lambda stage's boundary data, hV and f0V slabs (h, f0 times V) and their point-major
+    // transposes need V, mu0, Usp, Uch and the h, f0 slabs of the chi0 stage only.  They run on
+    // the prep stream: the boundary kernel (four workgroups) from here, the slabs once the
+    // self-energy slab is done (queued by stage_sigma), beside the latency-bound self-energy
+    // pencil, traces and mu search.
     if (c->hf0_pending) {
-        const Grid& g = c->g;
         const double fact4 = c->p.U * (3.0 * c->Usp + c->Uch) / 4;
         HIPCHK(c, hipStreamWaitEvent(c->prep, c->ev_hf0, 0));
         HIPCHK(c, hipStreamWaitEvent(c->prep, c->ev_v, 0));
         c->hf0_pending = false;
-        prof_begin(c, "jj_hVf0V_slab", c->prep);
-        HIPCHK(c, launch_hVf0V_slab(g, c->d, c->prep));
-        prof_end(c);
-        prof_begin(c, "jj_transpose", c->prep);
-        HIPCHK(c, launch_jj_transpose(g, c->d, c->prep));
-        prof_end(c);
         prof_begin(c, "jj_hf0_bnd", c->prep);
-        HIPCHK(c, launch_hf0_bnd(g, c->d, c->mu0, fact4, c->prep));
+        HIPCHK(c, launch_hf0_bnd(c->g, c->d, c->mu0, fact4, c->prep));
         prof_end(c);
+        c->hv_slabs_due = true;
+        c->hv_dropped = false;
         HIPCHK(c, hipEventRecord(c->ev_hv, c->prep));
         c->hv_pending = true;
     }
+    return 0;
+}
+
+// the rest of the Lambda preparation (hV, f0V slabs and transposes) on the prep stream, after
+// the work queued so far on the main stream
+int queue_hv_slabs(tpsc_ctx* c)
+{
+    if (!c->hv_slabs_due) return 0;
+    c->hv_slabs_due = false;
+    HIPCHK(c, hipEventRecord(c->ev_sig, c->stream));
+    HIPCHK(c, hipStreamWaitEvent(c->prep, c->ev_sig, 0));
+    prof_begin(c, "jj_hVf0V_slab", c->prep);
+    HIPCHK(c, launch_hVf0V_slab(c->g, c->d, c->prep));
+    prof_end(c);
+    prof_begin(c, "jj_transpose", c->prep);
+    HIPCHK(c, launch_jj_transpose(c->g, c->d, c->prep));
+    prof_end(c);
+    HIPCHK(c, hipEventRecord(c->ev_hv, c->prep));
     return 0;
 }
 
@@ -566,6 +594,7 @@ int stage_sigma(tpsc_ctx* c)
     prof_begin(c, "sigma_slab");
     HIPCHK(c, launch_sigma_slab(c->g, c->d, c->mu0, c->stream));
     prof_end(c);
+    if ((rc = queue_hv_slabs(c))) return rc;
     if ((rc = join_side(c))) return rc;
     prof_begin(c, "sigma_pencil");
     HIPCHK(c, launch_sigma_pencil(c->g, c->d, c->stream));
@@ -671,8 +700,9 @@ int stage_chijj(tpsc_ctx* c)
     hipStream_t s = c->stream;
     // hV, f0V, their transposes and the boundary data were queued at the vertex stage (prep
     // stream); otherwise they are made here
-    const bool prepared = c->hv_pending;
-    int rc = settle_prep(c);
+    const bool prepared = c->hv_pending && !c->hv_dropped;
+    c->hv_dropped = false;
+    int rc = settle_prep(c, true);
     if (rc) return rc;
     if ((rc = fork_side(c))) return rc;
     if (!prepared) {
@@ -1058,6 +1088,7 @@ int tpsc_create(const tpsc_params* p, tpsc_ctx** out)
     HIPCHK(c, hipStreamCreateWithFlags(&c->prep, hipStreamNonBlocking));
     HIPCHK(c, hipEventCreateWithFlags(&c->ev_hv, hipEventDisableTiming));
     HIPCHK(c, hipEventCreateWithFlags(&c->ev_v, hipEventDisableTiming));
+    HIPCHK(c, hipEventCreateWithFlags(&c->ev_sig, hipEventDisableTiming));
     HIPCHK(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
     HIPCHK(c, hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
     HIPCHK(c, hipEventCreateWithFlags(&c->ev_tau0, hipEventDisableTiming));
@@ -1150,6 +1181,7 @@ void tpsc_destroy(tpsc_ctx* c)
     c->root_graphs.clear();
     if (c->ev_hv) (void)hipEventDestroy(c->ev_hv);
     if (c->ev_v) (void)hipEventDestroy(c->ev_v);
+    if (c->ev_sig) (void)hipEventDestroy(c->ev_sig);
     if (c->prep) (void)hipStreamDestroy(c->prep);
     if (c->side) (void)hipStreamDestroy(c->side);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1551,6 +1583,7 @@ int tpsc_set_params(tpsc_ctx* c, const tpsc_params* p)
         HIPCHK(c, hipStreamSynchronize(c->prep));
         c->hv_pending = false;
     }
+    c->hv_slabs_due = c->hv_dropped = false;
     c->chijj1.clear();
     c->chijj2.clear();
     c->chijj3.clear();
