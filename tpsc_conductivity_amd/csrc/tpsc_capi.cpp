@@ -29,6 +29,10 @@ struct ProfPair {
     hipStream_t s;
 };
 
+// host_scalar: [0..2] reductions read by wait_mapped, [4] Sigma_inf, [8..15] root-search mirror,
+// [16..18] mu0 search {found, mu0, evaluations}, [19] chi0max, [20] its wedge index (int)
+constexpr int kHostScalars = 32;
+
 }  // namespace
 
 struct tpsc_ctx {
@@ -64,7 +68,7 @@ struct tpsc_ctx {
     hipStream_t side = nullptr;  // boundary kernels, overlapped with the slab kernels
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     hipEvent_t ev_tau0 = nullptr, ev_hf0 = nullptr;
-    bool hf0_pending = false;  // h(r,tau), f0(r,tau) queued on the side stream at the chi0 stage
+    bool hf0_pending = false;  // h(r,tau), f0(r,tau) queued on the prep stream at the chi0 stage
     // third stream: the Lambda stage's hV/f0V slabs, transposes and boundary data, queued at the
     // end of the vertex stage and overlapped with the self-energy and mu stages
     hipStream_t prep = nullptr;
@@ -81,10 +85,17 @@ struct tpsc_ctx {
     bool hv_dropped = false;    // they were dropped: the Lambda stage makes them
     hipEvent_t ev_sig = nullptr;
     bool sinf_async = false;  // Sigma_inf copied to host_scalar[4], not yet read
+    // mu0 is found on the device and read there by the chi0 and h/f0 kernels; the host copy
+    // (host_scalar[16..18]) is taken when it is first needed, after ev_mu0
+    hipEvent_t ev_mu0 = nullptr;
+    bool mu0_async = false;
+    // chi0max and its index, copied to host_scalar[19..20] behind the chi0 stage, taken after
+    // the first round trip of the Usp search, whose bracket the device sets from chi0max
+    bool chi0_async = false;
     DevArrays d{};
     ALArrays al{};
     double* chi_tau = nullptr;
-    double* host_scalar = nullptr;  // pinned, mapped: kernels write root-finder values here
+    double* host_scalar = nullptr;  // pinned, mapped (kHostScalars): kernels write root-finder values here
     double* dev_scalar = nullptr;   // device alias of host_scalar
     size_t dev_bytes = 0;
     std::vector<void*> allocs;
@@ -244,13 +255,17 @@ int root_chunk_graph(tpsc_ctx* c, int kind, const std::function<hipError_t()>& p
     return 0;
 }
 
+int mu0_ready(tpsc_ctx* c);
+// chi0max (device) != nullptr: the bracket is (0, 2/chi0max - 2 eps), set on the device.
 int device_halley(tpsc_ctx* c, int kind, const std::function<hipError_t()>& pass, HalleyState s0, double* root,
-                  int* nevals, bool* ok)
+                  int* nevals, bool* ok, const double* chi0max = nullptr)
 {
     static const bool graphs = !(getenv("TPSC_NO_GRAPHS") && atoi(getenv("TPSC_NO_GRAPHS")) != 0);
+    if (kind == kRootMu)
+        if (int rc = mu0_ready(c)) return rc;
     s0.T = c->g.T;
     s0.mu0 = c->mu0;
-    HIPCHK(c, launch_halley_init(c->d.root, s0, c->dev_scalar + 8, c->stream));
+    HIPCHK(c, launch_halley_init(c->d.root, s0, chi0max, c->dev_scalar + 8, c->stream));
     volatile const double* m = c->host_scalar + 8;
     int issued = 0;
     for (;;) {
@@ -323,20 +338,31 @@ int join_side(tpsc_ctx* c)
     return 0;
 }
 
-// green0::find_mu (green0.cpp:240-262) on the device
+// green0::find_mu (green0.cpp:240-262) on the device: the whole search in one launch
+// (find_zero_t over the device density sums), queued without a wait.  The root lands in d.mu0v
+// for the kernels and in host_scalar[16..18] for mu0_ready.  A fixed mu0, or mu0 = 0 at half
+// filling without t', t'' (green0.cpp:245), is stored as a preset.
 int find_mu0(tpsc_ctx* c)
 {
-    if (c->mu0_fixed) return 0;
-    if (c->p.n == 1.0 && c->p.tp == 0 && c->p.tpp == 0) {
-        c->mu0 = 0.0;
-        return 0;
-    }
-    // the whole search in one launch (find_zero_t over the device density sums)
-    HIPCHK(c, launch_mu0_root(c->g, c->d, c->p.n, c->dev_scalar, c->stream));
-    double v[3];
-    const int rc = wait_mapped(c, v, 3);
-    if (rc) return rc;
+    double preset = std::nan("");
+    if (c->mu0_fixed)
+        preset = c->mu0;
+    else if (c->p.n == 1.0 && c->p.tp == 0 && c->p.tpp == 0)
+        preset = 0.0;
+    HIPCHK(c, launch_mu0_root(c->g, c->d, c->p.n, preset, c->dev_scalar + 16, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev_mu0, c->stream));
+    c->mu0_async = true;
+    return 0;
+}
+
+// the host copy of mu0 once the search has ended
+int mu0_ready(tpsc_ctx* c)
+{
+    if (!c->mu0_async) return 0;
+    HIPCHK(c, hipEventSynchronize(c->ev_mu0));
+    volatile const double* v = c->host_scalar + 16;
     if (v[0] != 0.0) c->mu0 = v[1];  // on failure the previous mu0 is kept, as in the reference
+    c->mu0_async = false;
     return 0;
 }
 
@@ -369,35 +395,46 @@ int set_grid(tpsc_ctx* c, int N, int Nw)
 bool full_grid(const tpsc_ctx* c) { return c->g.N == c->p.nk0 && c->g.Nw == c->p.Nw0; }
 
 // chi0::calc_chiqw_FFT_spline, chi0.cpp:279-718
-int stage_chi0(tpsc_ctx* c, int nq0, int nw0)
+int chi0_ready(tpsc_ctx* c);
+// defer: chi0max and its index are copied to the host behind the stage without a wait, and
+// chi0_ready takes them (the Usp search, whose bracket the device sets, syncs first)
+int stage_chi0(tpsc_ctx* c, int nq0, int nw0, bool defer = false)
 {
     if (nq0 > c->p.nk0 || nw0 > c->p.Nw0) return fail(c, TPSC_E_INVALID, "chi0 grid larger than the context grid");
     // Lambda-stage work of an earlier point (prep stream) still reads eps(k), V, h and f0
     int rc = settle_prep(c, false);
     if (rc) return rc;
+    hipStream_t s = c->stream;
+    if (c->hf0_pending) {  // h, f0 of an earlier chain, still read on the prep stream, use eps(k)
+        HIPCHK(c, hipStreamWaitEvent(s, c->ev_hf0, 0));
+        c->hf0_pending = false;
+    }
     rc = set_grid(c, nq0, nw0);
     if (rc) return rc;
-    hipStream_t s = c->stream;
+    // Nothing waits for mu0 on the host.  The tau = 0 slice (one workgroup of 1024 threads)
+    // follows the mu0 search in the main stream, so that it takes its CU before the slab kernel
+    // on the side stream fills the chip: alone it runs ~240 us at N = 256, behind the slab
+    // kernel's workgroups more than twice that, and the chi0 pencils wait for it.
     if ((rc = fork_side(c))) return rc;
-    prof_begin(c, "chi0_tau0", c->side);
-    HIPCHK(c, launch_chi0_tau0(c->g, c->d, c->mu0, c->side));
+    prof_begin(c, "chi0_slab", c->side);
+    HIPCHK(c, launch_chi0_slab(c->g, c->d, c->side));
     prof_end(c);
-    HIPCHK(c, hipEventRecord(c->ev_tau0, c->side));
+    prof_begin(c, "chi0_tau0");
+    HIPCHK(c, launch_chi0_tau0(c->g, c->d, s));
+    prof_end(c);
+    HIPCHK(c, hipEventRecord(c->ev_tau0, s));
+    if ((rc = join_side(c))) return rc;
     // h(r,tau) and f0(r,tau) of the Lambda stage need only mu0 and eps(k): on the full grid
-    // they are queued now, behind the tau = 0 slice, and run during the chi0 pencils and the
-    // sum-rule passes; the Lambda stage waits for ev_hf0
-    c->hf0_pending = false;
+    // they are queued now on the prep stream, behind the tau = 0 slice, and run beside the chi0
+    // slab and pencils and the sum-rule passes; the Lambda stage waits for ev_hf0
     if (full_grid(c) && c->g.Nw <= 2048) {
-        prof_begin(c, "jj_hf0_slab", c->side);
-        HIPCHK(c, launch_hf0_slab(c->g, c->d, c->mu0, c->side));
+        HIPCHK(c, hipStreamWaitEvent(c->prep, c->ev_tau0, 0));
+        prof_begin(c, "jj_hf0_slab", c->prep);
+        HIPCHK(c, launch_hf0_slab(c->g, c->d, c->prep));
         prof_end(c);
-        HIPCHK(c, hipEventRecord(c->ev_hf0, c->side));
+        HIPCHK(c, hipEventRecord(c->ev_hf0, c->prep));
         c->hf0_pending = true;
     }
-    prof_begin(c, "chi0_slab");
-    HIPCHK(c, launch_chi0_slab(c->g, c->d, c->mu0, s));
-    prof_end(c);
-    HIPCHK(c, hipStreamWaitEvent(s, c->ev_tau0, 0));
     if (c->keep_chi_tau && c->chi_tau)
         HIPCHK(c, hipMemcpyAsync(c->chi_tau, c->d.chi, elem_bytes(c) * c->g.nwb * c->g.nk8, hipMemcpyDeviceToDevice,
                                  s));
@@ -405,18 +442,29 @@ int stage_chi0(tpsc_ctx* c, int nq0, int nw0)
     HIPCHK(c, launch_chi0_pencil(c->g, c->d, s));
     prof_end(c);
     HIPCHK(c, launch_chi0max(c->g, c->d, s));
-    int imax = 0;
-    HIPCHK(c, hipMemcpyAsync(c->host_scalar, c->d.red, sizeof(double), hipMemcpyDeviceToHost, s));
-    HIPCHK(c, hipMemcpyAsync(&imax, c->d.imax, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipMemcpyAsync(c->host_scalar + 19, c->d.red, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipMemcpyAsync(c->host_scalar + 20, c->d.imax, sizeof(int), hipMemcpyDeviceToHost, s));
+    c->chi0_async = true;
+    if (defer) return 0;
     HIPCHK(c, hipStreamSynchronize(s));
+    return chi0_ready(c);
+}
+
+int chi0_ready(tpsc_ctx* c)
+{
+    if (!c->chi0_async) return 0;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     prof_flush(c);
-    c->chi0max = c->host_scalar[0];
+    volatile const double* h = c->host_scalar;
+    c->chi0max = h[19];
+    const int imax = *(volatile const int*)(h + 20);
     int l = 0;
     while ((long)(l + 1) * (l + 2) / 2 <= imax) l++;
     c->qmax[0] = l;
     c->qmax[1] = imax - l * (l + 1) / 2;
+    c->chi0_async = false;
     c->have_chi0 = true;
-    return 0;
+    return mu0_ready(c);
 }
 
 int eval_sum_rule(tpsc_ctx* c, int channel, double Ux, double* out)
@@ -437,7 +485,7 @@ int eval_sum_rule(tpsc_ctx* c, int channel, double Ux, double* out)
 // chi::find_Usp_Uch(nq0, nw0, Usc), chi.cpp:846-876
 int stage_UspUch_grid(tpsc_ctx* c, int nq0, int nw0, double Usc[2], bool* ok_out)
 {
-    int rc = stage_chi0(c, nq0, nw0);
+    int rc = stage_chi0(c, nq0, nw0, true);
     if (rc) return rc;
     prof_begin(c, "sum_rules");
     const double U = c->p.U;
@@ -457,17 +505,19 @@ int stage_UspUch_grid(tpsc_ctx* c, int nq0, int nw0, double Usc[2], bool* ok_out
     auto pass_sp = [&]() { return launch_sum_rule3_pass(c->g, c->d, -1.0, c->dev_scalar + 8, c->stream); };
     auto pass_ch = [&]() { return launch_sum_rule3_pass(c->g, c->d, 1.0, c->dev_scalar + 8, c->stream); };
     double root = 0.0;
-    double init[3] = {U / 2.0, 0.0, 2.0 / c->chi0max - 2 * DBL_EPSILON};
     bool ok = true;
     int nh = 0;
     bool hok = false;
-    // the reference's start U/2 can lie beyond the Stoner bound 2/chi0max, where the function has
-    // its pole: the derivative-based search starts inside the bracket instead (same root)
-    const double x0 = (init[0] > init[1] && init[0] < init[2]) ? init[0] : 0.5 * (init[1] + init[2]);
+    // bracket (0, 2/chi0max - 2 eps) with the reference's start U/2.  That start can lie beyond
+    // the Stoner bound 2/chi0max, where the function has its pole: the derivative-based search
+    // starts at the midpoint instead (same root).  The device sets both from its chi0max, so the
+    // first passes follow the chi0 stage without a host round trip.
     rc = device_halley(c, kRootUsp, pass_sp,
-                       halley_start(x0, init[1], init[2], 40, n - n * n / 2.0 - nsr, nsr * nsr / (2.0 * U)),
-                       &root, &nh, &hok);
+                       halley_start(U / 2.0, 0.0, 0.0, 40, n - n * n / 2.0 - nsr, nsr * nsr / (2.0 * U)), &root,
+                       &nh, &hok, c->d.red);
+    if (!rc) rc = chi0_ready(c);
     if (rc) return rc;
+    double init[3] = {U / 2.0, 0.0, 2.0 / c->chi0max - 2 * DBL_EPSILON};
     if (!hok) hok = find_zero(fsp, init, &root, nullptr, &c->nev_sp);
     else c->nev_sp = nh;
     if (rc) return rc;
@@ -722,7 +772,7 @@ int stage_chijj(tpsc_ctx* c)
             c->hf0_pending = false;
         } else {
             prof_begin(c, "jj_hf0_slab");
-            HIPCHK(c, launch_hf0_slab(g, c->d, c->mu0, s));
+            HIPCHK(c, launch_hf0_slab(g, c->d, s));
             prof_end(c);
         }
         prof_begin(c, "jj_hVf0V_slab");
@@ -1093,7 +1143,8 @@ int tpsc_create(const tpsc_params* p, tpsc_ctx** out)
     HIPCHK(c, hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
     HIPCHK(c, hipEventCreateWithFlags(&c->ev_tau0, hipEventDisableTiming));
     HIPCHK(c, hipEventCreateWithFlags(&c->ev_hf0, hipEventDisableTiming));
-    HIPCHK(c, hipHostMalloc((void**)&c->host_scalar, 16 * sizeof(double), hipHostMallocMapped));
+    HIPCHK(c, hipEventCreateWithFlags(&c->ev_mu0, hipEventDisableTiming));
+    HIPCHK(c, hipHostMalloc((void**)&c->host_scalar, kHostScalars * sizeof(double), hipHostMallocMapped));
     HIPCHK(c, hipHostGetDevicePointer((void**)&c->dev_scalar, c->host_scalar, 0));
     const int N = p->nk0, Nw = p->Nw0, nk = N / 2 + 1;
     const long nk8 = (long)nk * (nk + 1) / 2, nq = (long)nk * (nk - 2);
@@ -1155,7 +1206,9 @@ int tpsc_create(const tpsc_params* p, tpsc_ctx** out)
     rc |= dalloc(c, &d.twM, (size_t)Nw);
     rc |= dalloc(c, &d.ph, (size_t)Nw + 1);
     rc |= dalloc(c, &d.imax, 4);
+    rc |= dalloc(c, &d.mu0v, 1);
     if (rc) return TPSC_E_NOMEM;
+    HIPCHK(c, hipMemset(d.mu0v, 0, sizeof(double)));  // = c->mu0 before any search
     return set_grid(c, N, Nw);  // hamiltonien + green0 constructors
 }
 
@@ -1176,6 +1229,7 @@ void tpsc_destroy(tpsc_ctx* c)
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
     if (c->ev_tau0) (void)hipEventDestroy(c->ev_tau0);
     if (c->ev_hf0) (void)hipEventDestroy(c->ev_hf0);
+    if (c->ev_mu0) (void)hipEventDestroy(c->ev_mu0);
     for (auto& rg : c->root_graphs)
         if (rg.exec) (void)hipGraphExecDestroy(rg.exec);
     c->root_graphs.clear();
@@ -1194,6 +1248,7 @@ int tpsc_find_mu0(tpsc_ctx* c, double* mu0)
 {
     if (!c) return TPSC_E_INVALID;
     int rc = find_mu0(c);
+    if (!rc) rc = mu0_ready(c);
     if (mu0) *mu0 = c->mu0;
     return rc;
 }
@@ -1501,7 +1556,8 @@ int tpsc_fix_mu0(tpsc_ctx* c, double mu0)
     if (!c) return TPSC_E_INVALID;
     if (std::isnan(mu0)) {
         c->mu0_fixed = false;
-        return find_mu0(c);
+        int rc = find_mu0(c);
+        return rc ? rc : mu0_ready(c);
     }
     c->mu0_fixed = true;
     c->mu0 = mu0;
@@ -1578,9 +1634,9 @@ int tpsc_set_params(tpsc_ctx* c, const tpsc_params* p)
     c->have_U = c->have_V = c->have_sigma = c->have_jj = c->have_trSG = c->have_trSG2 = false;
     c->mu_ok = false;
     c->grad_ready = false;
-    c->hf0_pending = false;
-    if (c->hv_pending) {  // queued Lambda work of the previous point still reads h, f0
+    if (c->hv_pending || c->hf0_pending) {  // queued work of the previous point still reads eps(k), h, f0
         HIPCHK(c, hipStreamSynchronize(c->prep));
+        c->hf0_pending = false;
         c->hv_pending = false;
     }
     c->hv_slabs_due = c->hv_dropped = false;
@@ -1604,6 +1660,8 @@ int tpsc_get_scalar(tpsc_ctx* c, int which, double* v)
     if (!c || !v) return TPSC_E_INVALID;
     if (which == TPSC_S_SIGMA_INF)
         if (int rc = sigma_inf_ready(c)) return rc;
+    if (int rc = chi0_ready(c)) return rc;
+    if (int rc = mu0_ready(c)) return rc;
     switch (which) {
         case TPSC_S_MU0: *v = c->mu0; break;
         case TPSC_S_CHI0MAX: *v = c->chi0max; break;

@@ -74,13 +74,24 @@ __global__ void k_finalize_k(const double* __restrict__ part, int n, double* out
 // mirror (mapped host memory) = {status, root, x, passes, f, f', lo, hi} of the search: written
 // here with status "running", and by the step that settles it (one system-scope write per
 // search instead of one per pass)
-__global__ void k_halley_init(HalleyState* st, HalleyState s0, double* mirror)
+// chi0max != nullptr (the Usp search): the bracket is (0, 2/chi0max - 2 eps) from the device
+// maximum of chi0, with the start s0.x kept when it lies inside and the midpoint otherwise, so
+// that the host need not wait for chi0max before the first pass
+__global__ void k_halley_init(HalleyState* st, HalleyState s0, const double* chi0max, double* mirror)
 {
-    *st = s0;
-    mirror[0] = s0.status;
-    mirror[1] = s0.root;
-    mirror[2] = s0.x;
-    mirror[3] = s0.ne;
+    HalleyState s = s0;
+    if (chi0max) {
+        const double lo = 0.0, hi = 2.0 / chi0max[0] - 2 * DBL_EPSILON;
+        const double x0 = (s0.x > lo && s0.x < hi) ? s0.x : 0.5 * (lo + hi);
+        s = halley_start(x0, lo, hi, s0.itmax, s0.a0, s0.a1);
+        s.T = s0.T;
+        s.mu0 = s0.mu0;
+    }
+    *st = s;
+    mirror[0] = s.status;
+    mirror[1] = s.root;
+    mirror[2] = s.x;
+    mirror[3] = s.ne;
 }
 
 // One Halley step after a pass: the three partial sums of part[k * n + i] (k = 0, 1, 2), each
@@ -137,9 +148,11 @@ __device__ __forceinline__ double g0_minus(double x, double t, double T, double 
 // only rounds the slice-0 values it stores into the S arrays.
 template <int N, typename S>
 __global__ void __launch_bounds__(SLAB_THREADS)
-    k_chi0_tau0(const double* __restrict__ ekw, const double* __restrict__ er, double mu0, double T, S* chi,
-                S* chirt, double* dchirt0, double* dGrt0p, double* dchiqt0, double* chirt0, double* tmp)
+    k_chi0_tau0(const double* __restrict__ ekw, const double* __restrict__ er, const double* __restrict__ mu0p,
+                double T, S* chi, S* chirt, double* dchirt0, double* dGrt0p, double* dchiqt0, double* chirt0,
+                double* tmp)
 {
+    const double mu0 = *mu0p;
     constexpr int nk = N / 2 + 1;
     constexpr long nk8 = (long)nk * (nk + 1) / 2;
     const double nk02 = (double)N * N;
@@ -228,8 +241,9 @@ __global__ void __launch_bounds__(SLAB_THREADS)
 // slices j = 1 .. nwb-1 (chi0.cpp:454-559); block b handles j = b + 1
 template <int N, typename S>
 __global__ void __launch_bounds__(SLAB_THREADS, SlabOcc<S>::waves)
-    k_chi0_slab(const double* __restrict__ ekw, double mu0, double T, int Nw, S* chi, S* chirt)
+    k_chi0_slab(const double* __restrict__ ekw, const double* __restrict__ mu0p, double T, int Nw, S* chi, S* chirt)
 {
+    const double mu0 = *mu0p;
     constexpr int nk = N / 2 + 1;
     constexpr long nk8 = (long)nk * (nk + 1) / 2;
     const double nk02 = (double)N * N;
@@ -495,17 +509,43 @@ __global__ void __launch_bounds__(256)
         for (int k = 0; k < 3; k++) part[k * nblk + b] = acc[k];
 }
 
+// wedge points per thread in k_mu0_root: 16 waves hold the 8385 points of N = 256
+constexpr int kMu0K = 9;
+
 // green0::find_mu (green0.cpp:240-262) in one workgroup: every thread runs the same bracketing
 // search (find_zero_t) on block-wide sums of the finite-lattice density (green0.cpp:182-203),
 // so the control flow stays uniform.  out = {found, mu0, evaluations}.
+// Each thread keeps its kMu0K strided wedge points (eps, weight) in registers for all the
+// evaluations; a padded slot has weight 0 and adds +0.  The launcher sizes the workgroup so
+// that blockDim.x * kMu0K >= nk8.
+// The root goes to out (mapped host memory) and to the device slot mu0v, which the chi0 and
+// h/f0 kernels read, so that the host need not wait for it; a failed search leaves mu0v as it
+// was (the previous mu0).  A preset value (not NaN) is stored without a search.
 __global__ void __launch_bounds__(1024) k_mu0_root(const double* __restrict__ ekw, const double* __restrict__ wq,
-                                                   long nk8, double Nk2, double T, double dens, double* out)
+                                                   long nk8, double Nk2, double T, double dens, double preset,
+                                                   double* out, double* mu0v)
 {
+    if (!isnan(preset)) {
+        if (threadIdx.x == 0) {
+            out[0] = 1.0;
+            out[1] = preset;
+            out[2] = 0.0;
+            *mu0v = preset;
+        }
+        return;
+    }
     __shared__ double sc[16];
+    double e[kMu0K], w[kMu0K];
+#pragma unroll
+    for (int k = 0; k < kMu0K; k++) {
+        const long i = threadIdx.x + (long)k * blockDim.x;
+        e[k] = (i < nk8) ? ekw[i] : 0.0;
+        w[k] = (i < nk8) ? wq[i] : 0.0;
+    }
     auto density = [&](double mup) {
         double acc = 0.0;
-        for (long i = threadIdx.x; i < nk8; i += blockDim.x)
-            acc += 2 * wq[i] / ((exp((ekw[i] - mup) / T) + 1.0) * Nk2);
+#pragma unroll
+        for (int k = 0; k < kMu0K; k++) acc += 2 * w[k] / ((exp((e[k] - mup) / T) + 1.0) * Nk2);
         return block_sum(acc, sc) - dens;  // the same total in every thread
     };
     const double init[3] = {0.0, -8.0, 8.0};
@@ -516,6 +556,7 @@ __global__ void __launch_bounds__(1024) k_mu0_root(const double* __restrict__ ek
         out[0] = ok ? 1.0 : 0.0;
         out[1] = root;
         out[2] = ne;
+        if (ok) *mu0v = root;
     }
 }
 
@@ -972,9 +1013,10 @@ __global__ void k_model(int N, double t, double tp, double tpp, double t3, doubl
 // RODFT00 along kx (index l+1), REDFT00 along ky.
 template <int N, typename S>
 __global__ void __launch_bounds__(SLAB_THREADS, SlabOcc<S>::waves)
-    k_hf0_slab(const double* __restrict__ ek, const double* __restrict__ dek, double mu0, double T, int Nw,
-               S* h, S* f0)
+    k_hf0_slab(const double* __restrict__ ek, const double* __restrict__ dek, const double* __restrict__ mu0p,
+               double T, int Nw, S* h, S* f0)
 {
+    const double mu0 = *mu0p;
     constexpr int nk = N / 2 + 1;
     constexpr long nq = (long)nk * (nk - 2);
     const double nf = 1.0 / ((double)N * N);
@@ -1771,23 +1813,23 @@ hipError_t launch_finalize(const double* part, int n, double* out, hipStream_t s
     return hipGetLastError();
 }
 
-hipError_t launch_chi0_tau0(const Grid& g, const DevArrays& d, double mu0, hipStream_t s)
+hipError_t launch_chi0_tau0(const Grid& g, const DevArrays& d, hipStream_t s)
 {
     const size_t lds = slab_lds(g.N, false, sizeof(double));
     DISPATCH_S(g.prec, DISPATCH_N(g.N, {
         set_lds(k_chi0_tau0<NN, SS>, lds);
-        hipLaunchKernelGGL((k_chi0_tau0<NN, SS>), dim3(1), dim3(SLAB_THREADS), lds, s, d.ekw, d.er, mu0, g.T,
+        hipLaunchKernelGGL((k_chi0_tau0<NN, SS>), dim3(1), dim3(SLAB_THREADS), lds, s, d.ekw, d.er, d.mu0v, g.T,
                            (SS*)d.chi, (SS*)d.chirt, d.dchirt0, d.dGrt0p, d.dchiqt0, d.chirt0, d.tmp);
     }));
     return hipGetLastError();
 }
 
-hipError_t launch_chi0_slab(const Grid& g, const DevArrays& d, double mu0, hipStream_t s)
+hipError_t launch_chi0_slab(const Grid& g, const DevArrays& d, hipStream_t s)
 {
     DISPATCH_S(g.prec, DISPATCH_N(g.N, {
         const size_t lds = slab_lds(g.N, false, sizeof(SS));
         set_lds(k_chi0_slab<NN, SS>, lds);
-        hipLaunchKernelGGL((k_chi0_slab<NN, SS>), dim3(g.nwb - 1), dim3(SLAB_THREADS), lds, s, d.ekw, mu0, g.T, g.Nw,
+        hipLaunchKernelGGL((k_chi0_slab<NN, SS>), dim3(g.nwb - 1), dim3(SLAB_THREADS), lds, s, d.ekw, d.mu0v, g.T, g.Nw,
                            (SS*)d.chi, (SS*)d.chirt);
     }));
     return hipGetLastError();
@@ -1821,9 +1863,10 @@ hipError_t launch_sum_rule(const Grid& g, const DevArrays& d, double sU, double*
     return hipGetLastError();
 }
 
-hipError_t launch_halley_init(HalleyState* st, const HalleyState& s0, double* mirror, hipStream_t s)
+hipError_t launch_halley_init(HalleyState* st, const HalleyState& s0, const double* chi0max, double* mirror,
+                              hipStream_t s)
 {
-    hipLaunchKernelGGL(k_halley_init, dim3(1), dim3(1), 0, s, st, s0, mirror);
+    hipLaunchKernelGGL(k_halley_init, dim3(1), dim3(1), 0, s, st, s0, chi0max, mirror);
     return hipGetLastError();
 }
 
@@ -1852,10 +1895,14 @@ hipError_t launch_reduce_cols(const double* part, int nrows, int ncols, double* 
     return hipGetLastError();
 }
 
-hipError_t launch_mu0_root(const Grid& g, const DevArrays& d, double n, double* out, hipStream_t s)
+hipError_t launch_mu0_root(const Grid& g, const DevArrays& d, double n, double preset, double* out, hipStream_t s)
 {
     const double Nk2 = 4.0 * (g.nk - 1) * (g.nk - 1);
-    hipLaunchKernelGGL(k_mu0_root, dim3(1), dim3(1024), 0, s, d.ekw, d.wq, g.nk8, Nk2, g.T, n, out);
+    // fewest waves that hold the wedge at kMu0K points per thread (barriers get cheaper)
+    const long waves = (g.nk8 + 64L * kMu0K - 1) / (64L * kMu0K);
+    if (waves > 16) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_mu0_root, dim3(1), dim3((unsigned)(64 * waves)), 0, s, d.ekw, d.wq, g.nk8, Nk2, g.T, n,
+                       preset, out, d.mu0v);
     return hipGetLastError();
 }
 
@@ -1950,12 +1997,12 @@ hipError_t launch_trace_selfG(const Grid& g, const DevArrays& d, double mu0, dou
     return hipGetLastError();
 }
 
-hipError_t launch_hf0_slab(const Grid& g, const DevArrays& d, double mu0, hipStream_t s)
+hipError_t launch_hf0_slab(const Grid& g, const DevArrays& d, hipStream_t s)
 {
     DISPATCH_S(g.prec, DISPATCH_N(g.N, {
         const size_t lds = slab_lds(g.N, true, sizeof(SS));
         set_lds(k_hf0_slab<NN, SS>, lds);
-        hipLaunchKernelGGL((k_hf0_slab<NN, SS>), dim3(g.Nw + 1), dim3(SLAB_THREADS), lds, s, d.ek, d.dek, mu0, g.T,
+        hipLaunchKernelGGL((k_hf0_slab<NN, SS>), dim3(g.Nw + 1), dim3(SLAB_THREADS), lds, s, d.ek, d.dek, d.mu0v, g.T,
                            g.Nw, (SS*)d.h, (SS*)d.f0);
     }));
     return hipGetLastError();

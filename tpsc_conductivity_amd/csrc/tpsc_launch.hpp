@@ -63,6 +63,7 @@ struct DevArrays {
     cd* twM = nullptr;
     cd* ph = nullptr;            // Nw+1
     int* imax = nullptr;         // argmax result
+    double* mu0v = nullptr;      // 1  mu0 of the current grid (k_mu0_root), read by the chi0 and h/f0 kernels
 };
 
 // Aslamazov-Larkin work arrays (FP64 in both precisions; allocated at the first chi3 call).
@@ -135,8 +136,8 @@ constexpr int jj_threads(int Nw) { return Nw <= 256 ? 64 : 256; }
 
 // ---- launchers; each returns hipGetLastError() of its launch
 hipError_t launch_fill_tw(const DevArrays& d, int L, hipStream_t s);
-hipError_t launch_chi0_tau0(const Grid& g, const DevArrays& d, double mu0, hipStream_t s);
-hipError_t launch_chi0_slab(const Grid& g, const DevArrays& d, double mu0, hipStream_t s);
+hipError_t launch_chi0_tau0(const Grid& g, const DevArrays& d, hipStream_t s);
+hipError_t launch_chi0_slab(const Grid& g, const DevArrays& d, hipStream_t s);
 hipError_t launch_chi0_pencil(const Grid& g, const DevArrays& d, hipStream_t s);
 hipError_t launch_chi0max(const Grid& g, const DevArrays& d, hipStream_t s);
 hipError_t launch_sum_rule(const Grid& g, const DevArrays& d, double sU, double* out, hipStream_t s);
@@ -146,11 +147,12 @@ hipError_t launch_sum_rule(const Grid& g, const DevArrays& d, double sU, double*
 // state, so a pass depends on the point only through d.root.  mirror (mapped host memory)
 // holds {status, root, x, passes, f, f', lo, hi}: "running" from the initialisation, the
 // outcome once the search has settled.
-hipError_t launch_halley_init(HalleyState* st, const HalleyState& s0, double* mirror, hipStream_t s);
+hipError_t launch_halley_init(HalleyState* st, const HalleyState& s0, const double* chi0max, double* mirror,
+                              hipStream_t s);
 hipError_t launch_sum_rule3_pass(const Grid& g, const DevArrays& d, double sgn, double* mirror, hipStream_t s);
 hipError_t launch_trace_diffG3_pass(const Grid& g, const DevArrays& d, double* mirror, hipStream_t s);
 // green0::find_mu on the device (one workgroup, the whole bracketing search): out = {found, mu0, evaluations}
-hipError_t launch_mu0_root(const Grid& g, const DevArrays& d, double n, double* out, hipStream_t s);
+hipError_t launch_mu0_root(const Grid& g, const DevArrays& d, double n, double preset, double* out, hipStream_t s);
 hipError_t launch_model(const Grid& g, const DevArrays& d, double t, double tp, double tpp, double t3,
                         hipStream_t s);
 hipError_t launch_reduce_cols(const double* part, int nrows, int ncols, double* out, hipStream_t s);
@@ -163,7 +165,7 @@ hipError_t launch_trace_diffG(const Grid& g, const DevArrays& d, double pmu, dou
                               hipStream_t s);
 hipError_t launch_trace_selfG(const Grid& g, const DevArrays& d, double mu0, double mu, int which, double* out,
                               hipStream_t s);
-hipError_t launch_hf0_slab(const Grid& g, const DevArrays& d, double mu0, hipStream_t s);
+hipError_t launch_hf0_slab(const Grid& g, const DevArrays& d, hipStream_t s);
 hipError_t launch_hf0_bnd(const Grid& g, const DevArrays& d, double mu0, double fact4, hipStream_t s);
 hipError_t launch_hVf0V_slab(const Grid& g, const DevArrays& d, hipStream_t s);
 hipError_t launch_jj_poles(const Grid& g, const DevArrays& d, double mu, hipStream_t s);
