@@ -1522,18 +1522,28 @@ __global__ void __launch_bounds__(NT, JJ_WAVES_PER_SIMD)
                 W[s] = cd{Gn.x + Jn.y, Gn.y + Jn.x};  // Gn + i conj(Jn)
             }
         }
-        block_sum_k<6>(r6, sc);
-        const double qn0 = r6[0];
-        // (3) two-pole asymptote: pole data from k_jj_poles, sums of this point
+        // the six sums: wave sums parked in sc[k * nw + wave], added in the order of block_sum_k
+        // where they are used (after the transform: fewer registers live across it)
+        constexpr int nw = NT / 64;
+        {
+            const int lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+            for (int k = 0; k < 6; k++) r6[k] = wave_sum(r6[k]);
+            if (lane == 0)
+#pragma unroll
+                for (int k = 0; k < 6; k++) sc[k * nw + wave] = r6[k];
+        }
+        __syncthreads();  // the W writes and the parked sums; X's last reads (step 2) are above
+        auto sum6 = [&](int k) {
+            double t = 0.0;
+            for (int i = 0; i < nw; i++) t += sc[k * nw + i];
+            return t;
+        };
+        // (3) two-pole asymptote: pole data from k_jj_poles
         const JJPoles& pp = poles[q];
         const Poles& P = pp.P;
-        const double k0r = r6[2] + (pp.f1m + pp.f2m).x;
-        const double dGp = r6[4] + (P.r1 * pp.f1p + P.r2 * pp.f2p).x;
-        const double dGm = -r6[4] + (P.r1 * pp.f1m + P.r2 * pp.f2m).x;
-        const double dJ0 = r6[5] + (pp.fj1 + pp.fj2).x;
-        const double dJb = -r6[5] + (pp.fj1b + pp.fj2b).x;
         // (4) to imaginary time: X_j <- (G(tau_j), J(tau_j))
-        block_fft_pp<L, NT>(W, X, -1, tw);  // X is free: its last reads (step 2) precede block_sum_k
+        block_fft_pp<L, NT>(W, X, -1, tw);
         if (tid == 0) {
             const double t = 1.0 / T;
             const cd ej1 = (P.r1.x > 0) ? cd{1.0, 0.0} : cexpd(P.r1 * t);
@@ -1576,6 +1586,13 @@ __global__ void __launch_bounds__(NT, JJ_WAVES_PER_SIMD)
         }
         __syncthreads();
         const double G0 = X[0].x, J0t = X[0].y, Jbeta = Jbeta_s;
+        // sums of this point with the pole parts
+        const double k0r = sum6(2) + (pp.f1m + pp.f2m).x;
+        const double s4 = sum6(4), s5 = sum6(5);
+        const double dGp = s4 + (P.r1 * pp.f1p + P.r2 * pp.f2p).x;
+        const double dGm = -s4 + (P.r1 * pp.f1m + P.r2 * pp.f2m).x;
+        const double dJ0 = s5 + (pp.fj1 + pp.fj2).x;
+        const double dJb = -s5 + (pp.fj1b + pp.fj2b).x;
         // (5) Maki-Thompson samples G(tau) J(-tau) -> Y1; bubble samples -G(tau) G(-tau) -> Y2 over
         // the whole of [0, beta].  They are symmetric about beta/2 (the product is the same in
         // both orders), so the clamped spline over [0, beta] with end slopes (dchi, -dchi) is, on
@@ -1601,8 +1618,12 @@ __global__ void __launch_bounds__(NT, JJ_WAVES_PER_SIMD)
             const double d0s[2] = {gd0, dchi}, dNs[2] = {gdN, -dchi};
             block_spline_inplace_k<CS, NT, 2>(ys, L, hstep, d0s, dNs, ES, integ);
         }
-        double ibub[1] = {integ[1]};  // the bubble over [0, beta]
-        block_sum_k<1>(ibub, sc);
+        // the bubble over [0, beta] is used by thread 0 only (n = 0): each wave parks its sum in
+        // sc[6 nw + wave], which thread 0 adds in the accumulation below (after two barriers)
+        {
+            const double ws = wave_sum(integ[1]);
+            if ((tid & 63) == 0) sc[6 * (NT / 64) + (tid >> 6)] = ws;
+        }
         const double MJ3 = Y1[0] - Y1[L];
         // (6) packed transform of the two moment sequences
         for (int j = tid; j < L; j += NT) {
@@ -1610,7 +1631,7 @@ __global__ void __launch_bounds__(NT, JJ_WAVES_PER_SIMD)
             const double b = (j < H) ? (Y2[j + 1] - Y2[j]) / hstep : -(Y2[L - j] - Y2[L - 1 - j]) / hstep;
             X[j] = cd{a, b};
         }
-        __syncthreads();  // Y1, Y2 read for the last time
+        __syncthreads();  // Y1, Y2 read for the last time; the bubble's wave sums are in sc
         block_fft_pp<L, NT>(X, W, -1, tw);
         if (kk + 1 < KP && q + nblk < nq) dma_rows(q + nblk);  // W is free again
         const double wv = nf * wt * v;
@@ -1619,8 +1640,8 @@ __global__ void __launch_bounds__(NT, JJ_WAVES_PER_SIMD)
             const cd Fn = X[n], Fc = conjg(X[(L - n) & (L - 1)]);
             double bub;
             if (n == 0) {
-                bub = ibub[0];
-                c2[0] -= wv * qn0;
+                bub = sum6(6);  // the bubble's wave sums
+                c2[0] -= wv * sum6(0);
             } else {
                 const double iw = 1.0 / (2 * n * kPi * T), iw2 = iw * iw;
                 const cd tm = twM[n];
