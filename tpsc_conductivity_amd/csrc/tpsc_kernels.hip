@@ -1139,22 +1139,26 @@ __global__ void __launch_bounds__(SLAB_THREADS, SlabOcc<S>::waves)
     S* slab = (S*)smem;
     auto* tw = slab_tw(slab, nq);
     fill_twiddles(tw, N);
-    const int j = blockIdx.x;
-    const int jv = (j > Nw / 2) ? Nw - j : j;
-    const S* Vj = V + (long)jv * nk8;
     const int tid = threadIdx.x, nt = blockDim.x;
-    for (int which = 0; which < 2; which++) {
-        S* a = ((which == 0) ? h : f0) + (long)j * nq;
-        for (long s = tid; s < nq; s += nt) {
-            const int l = (int)(s / nk), m = (int)(s % nk);
-            slab[s] = (S)2 * a[s] * Vj[fold_idx(l + 1, m, N)];
+    // slabs j = blockIdx.x, + gridDim.x, ...: the launch may hold fewer workgroups than slabs
+    for (int j = blockIdx.x; j <= Nw; j += gridDim.x) {
+        const int jv = (j > Nw / 2) ? Nw - j : j;
+        const S* Vj = V + (long)jv * nk8;
+        for (int which = 0; which < 2; which++) {
+            S* a = ((which == 0) ? h : f0) + (long)j * nq;
+            for (long s = tid; s < nq; s += nt) {
+                const int l = (int)(s / nk), m = (int)(s % nk);
+                slab[s] = (S)2 * a[s] * Vj[fold_idx(l + 1, m, N)];
+            }
+            __syncthreads();
+            slab_r2r_2d<S, N, 1, 0>(slab, nk - 2, nk, nk, tw);
+            for (long s = tid; s < nq; s += nt) a[s] = slab[s];
+            __syncthreads();
         }
-        __syncthreads();
-        slab_r2r_2d<S, N, 1, 0>(slab, nk - 2, nk, nk, tw);
-        for (long s = tid; s < nq; s += nt) a[s] = slab[s];
-        __syncthreads();
     }
 }
+
+constexpr int kHVBlocks = 256;  // workgroups of the FP32 hV/f0V slab kernel
 
 // ================================================================== Lambda_jj pencils
 struct Poles {
@@ -2045,7 +2049,13 @@ hipError_t launch_hVf0V_slab(const Grid& g, const DevArrays& d, hipStream_t s)
     DISPATCH_S(g.prec, DISPATCH_N(g.N, {
         const size_t lds = slab_lds(g.N, true, sizeof(SS));
         set_lds(k_hVf0V_slab<NN, SS>, lds);
-        hipLaunchKernelGGL((k_hVf0V_slab<NN, SS>), dim3(g.Nw + 1), dim3(SLAB_THREADS), lds, s, (const SS*)d.V, g.Nw,
+        // FP32: at most kHVBlocks workgroups, each looping over slabs.  This kernel runs beside
+        // the latency-bound self-energy pencil and mu search; two of its workgroups fill a CU's
+        // waves, one leaves room for theirs.  An FP64 slab (133 KB of LDS) holds a CU alone
+        // either way, and fewer of them only lengthen the kernel (measured at 256^2 x 2048:
+        // -0.05 ms per solve in FP32, +0.2 ms in FP64).
+        const unsigned nb = (unsigned)((sizeof(SS) == sizeof(float)) ? std::min(g.Nw + 1, kHVBlocks) : g.Nw + 1);
+        hipLaunchKernelGGL((k_hVf0V_slab<NN, SS>), dim3(nb), dim3(SLAB_THREADS), lds, s, (const SS*)d.V, g.Nw,
                            (SS*)d.h, (SS*)d.f0);
     }));
     return hipGetLastError();
