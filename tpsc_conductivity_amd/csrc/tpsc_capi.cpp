@@ -27,6 +27,7 @@ struct ProfPair {
     std::string name;
     hipEvent_t a, b;
     hipStream_t s;
+    bool open;  // b not yet recorded
 };
 
 // host_scalar: [0..2] reductions read by wait_mapped, [4] Sigma_inf, [8..15] root-search mirror,
@@ -134,19 +135,28 @@ void prof_begin(tpsc_ctx* c, const char* name, hipStream_t st = nullptr)
     (void)hipEventCreate(&pp.a);
     (void)hipEventCreate(&pp.b);
     (void)hipEventRecord(pp.a, pp.s);
+    pp.open = true;
     c->pending.push_back(pp);
 }
 void prof_end(tpsc_ctx* c)
 {
-    if (!c->prof || c->pending.empty()) return;
+    if (!c->prof || c->pending.empty() || !c->pending.back().open) return;
     (void)hipEventRecord(c->pending.back().b, c->pending.back().s);
+    c->pending.back().open = false;
 }
+// the closed regions (all three streams drained first); a region still open stays pending
 void prof_flush(tpsc_ctx* c)
 {
     if (c->pending.empty()) return;
+    (void)hipStreamSynchronize(c->prep);
     (void)hipStreamSynchronize(c->side);
     (void)hipStreamSynchronize(c->stream);
+    std::vector<ProfPair> keep;
     for (auto& pp : c->pending) {
+        if (pp.open) {
+            keep.push_back(pp);
+            continue;
+        }
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, pp.a, pp.b);
         auto it = c->prof_acc.find(pp.name);
@@ -160,7 +170,7 @@ void prof_flush(tpsc_ctx* c)
         (void)hipEventDestroy(pp.a);
         (void)hipEventDestroy(pp.b);
     }
-    c->pending.clear();
+    c->pending.swap(keep);
 }
 
 // ------------------------------------------------------------------ root finder
