@@ -466,7 +466,7 @@ __global__ void __launch_bounds__(256)
 constexpr int kSR = 8;
 __device__ __forceinline__ void sum_rule3_term(double c, double w, double sU, double sgn, double acc[3])
 {
-    const double r = 1.0 / (1 + sU * c / 2.0);
+    const double r = rcp_nr(1 + sU * c / 2.0);
     acc[0] += w * (c * r - c);
     acc[1] -= w * sgn * 0.5 * c * c * r * r;
     acc[2] += w * 0.5 * c * c * c * r * r * r;
@@ -499,7 +499,7 @@ __global__ void __launch_bounds__(256)
         for (; j <= nwmax; j += RG) sum_rule3_term(chi[(long)j * nk8 + q], (j == 0) ? 0.5 * wb : wb, sU, sgn, acc);
         for (; j <= 2 * nwmax; j += RG) {
             const double wn = 2 * j * kPi * T;
-            const double i2 = 1.0 / (wn * wn);
+            const double i2 = rcp_nr(wn * wn);
             sum_rule3_term((a1 + a2 * i2) * i2, wb, sU, sgn, acc);
         }
     }
@@ -852,8 +852,8 @@ __device__ __forceinline__ void trace_diffG3_term(double wn, double rS, double i
                                                   double mu0, double pmu, double acc[3])
 {
     const double a = e - pmu + rS, b = wn - iS;
-    const double a2 = a * a, b2 = b * b, r = 1.0 / (a2 + b2);
-    const double r0 = 1.0 / (wn * wn + c * c);
+    const double a2 = a * a, b2 = b * b, r = rcp_nr(a2 + b2);
+    const double r0 = rcp_nr(wn * wn + c * c);
     // Re G - Re G0 = [c (a^2+b^2) - a (c^2+wn^2)] r r0, written without cancellation
     acc[0] += w * r * r0 * ((mu0 - pmu + rS) * a * c + iS * (iS - 2 * wn) * c - wn * wn * (mu0 - pmu + rS));
     acc[1] += w * (b2 - a2) * r * r;
