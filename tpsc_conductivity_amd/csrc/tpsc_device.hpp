@@ -779,6 +779,13 @@ __device__ __forceinline__ double rcp_nr(double x)
     return fma(fma(-x, r, 1.0), r, r);
 }
 
+// s / b (real s, complex b != 0) with one rcp_nr of |b|^2
+__device__ __forceinline__ cd sdiv_rcp(double s, cd b)
+{
+    const double r = s * rcp_nr(b.x * b.x + b.y * b.y);
+    return cd{b.x * r, -b.y * r};
+}
+
 // block-wide sums of K values at once; every thread gets the results.
 // scratch: LDS double[>= K * nwaves]
 template <int K>

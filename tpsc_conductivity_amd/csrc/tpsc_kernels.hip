@@ -1223,7 +1223,7 @@ __device__ __forceinline__ cd dyson_G_raw(SelfT raw, int s, int H, double C2, do
     cd sg = sigma_kw(&raw, 0, Sinf, C2, C3, fabs(kn));
     if (s < H) sg = conjg(sg);
     const cd den = cd{0.0 - e, kn} + cd{mu, 0.0} - sg;
-    return T / den;
+    return sdiv_rcp(T, den);
 }
 template <typename SelfT>
 __device__ __forceinline__ cd dyson_G(const SelfT* __restrict__ Self, long ki, long nk8, int s, int H, double C2,
@@ -1506,7 +1506,7 @@ __global__ void __launch_bounds__(NT, JJ_WAVES_PER_SIMD)
                 const cd Bh = cd{0.5 * (Fj.y - Fc.y), -0.5 * (Fj.x - Fc.x)};
                 const double knj = (jx < H) ? (2 * jx + 1) * kPi * T : (2 * (jx - L) + 1) * kPi * T;
                 // -i c1/k - c2/k^2 + i c3/k^3 + a/k^4 with one reciprocal of k (cond_opt.cpp:3302-3311)
-                const double ik = 1.0 / knj, ik2 = ik * ik;
+                const double ik = rcp_nr(knj), ik2 = ik * ik;
                 const cd af = ts * Af, ah = ts * Bh;
                 const cd J0 = cd{(-cj02 + af.x * ik2) * ik2, (-cj01 + (cj03 + af.y * ik) * ik2) * ik};
                 const cd Jv = cd{(-cj2 + ah.x * ik2) * ik2, (-CJ1 + (cj3 + ah.y * ik) * ik2) * ik};
@@ -1515,8 +1515,8 @@ __global__ void __launch_bounds__(NT, JJ_WAVES_PER_SIMD)
                 const cd t = (Gv * Gv * J0) * iT;
                 r6[0] += t.x;
                 r6[1] += t.y;
-                const double ikn = 1.0 / kn;
-                const cd G2 = T / cd{-e + mu, kn + Sinf * ikn};
+                const double ikn = rcp_nr(kn);
+                const cd G2 = sdiv_rcp(T, cd{-e + mu, kn + Sinf * ikn});
                 const cd Jn = Gv * Jv - G2 * cd{0.0, -CJ1 * ikn};
                 const cd Gn = Gv - G2;
                 r6[2] += Gn.x;
