@@ -20,6 +20,7 @@
 // and the Lambda_jj Matsubara sums stay in FP64.  Self holds the remainder R(k,n) as FP64
 // complex (S = double) or the assembled Sigma(k,iwn) as FP32 complex (S = float).
 #include <algorithm>
+#include <unordered_map>
 
 #include "tpsc_launch.hpp"
 
@@ -1818,11 +1819,17 @@ bool supported_Nw(int Nw)
         __VA_ARGS__;              \
     }
 
+// the dynamic-LDS limit of a kernel, raised once to the largest size asked for (the attribute
+// holds for the process; each host thread keeps its own record of what it has set)
 template <typename K>
 static void set_lds(K kern, size_t bytes)
 {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)bytes);
+    thread_local std::unordered_map<const void*, size_t> done;
+    const void* f = reinterpret_cast<const void*>(kern);
+    size_t& have = done[f];
+    if (have >= bytes) return;
+    (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    have = bytes;
 }
 
 hipError_t launch_fill_tw(const DevArrays& d, int L, hipStream_t s)
