@@ -1316,26 +1316,35 @@ __global__ void k_jj_poles(const double* __restrict__ ek, const double* __restri
     out[q] = o;
 }
 
-// [j][q] (j < nj, q < nq) -> [q][j]: 32 x 32 tiles through LDS, both sides coalesced.  The
-// Lambda pencils then read each tau-pencil as one contiguous row.
+// [j][q] (j < nj, q < nq) -> [q][j]: kTT x kTT tiles through LDS, both sides coalesced (a
+// wave moves 64 consecutive elements of one row per access; 16 accesses per thread in flight).
+// Rows of odd length (nq = nk (nk - 2), nj = Nw + 1) rule out vector accesses.  The Lambda
+// pencils then read each tau-pencil as one contiguous row.
+constexpr int kTT = 64;
 template <typename S>
 __global__ void __launch_bounds__(256) k_transpose(const S* __restrict__ in, S* __restrict__ out, long nq, int nj)
 {
-    __shared__ S t[32][33];
-    const long q0 = (long)blockIdx.x * 32;
-    const int j0 = blockIdx.y * 32;
-    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
-#pragma unroll
-    for (int r = 0; r < 32; r += 8) {
-        const int j = j0 + ty + r;
+    __shared__ S t[kTT][kTT + 1];
+    const long q0 = (long)blockIdx.x * kTT;
+    const int j0 = blockIdx.y * kTT;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;  // 64 x 4
+    {
+        // all 16 loads issued before the first LDS write
         const long q = q0 + tx;
-        if (j < nj && q < nq) t[ty + r][tx] = in[(long)j * nq + q];
+        S v[kTT / 4];
+#pragma unroll
+        for (int r = 0; r < kTT / 4; r++) {
+            const int j = j0 + ty + 4 * r;
+            v[r] = (j < nj && q < nq) ? in[(long)j * nq + q] : S(0);
+        }
+#pragma unroll
+        for (int r = 0; r < kTT / 4; r++) t[ty + 4 * r][tx] = v[r];
     }
     __syncthreads();
+    const int j = j0 + tx;
 #pragma unroll
-    for (int r = 0; r < 32; r += 8) {
+    for (int r = 0; r < kTT; r += 4) {
         const long q = q0 + ty + r;
-        const int j = j0 + tx;
         if (j < nj && q < nq) out[q * nj + j] = t[tx][ty + r];
     }
 }
@@ -2077,7 +2086,7 @@ hipError_t launch_jj_poles(const Grid& g, const DevArrays& d, double mu, hipStre
 
 hipError_t launch_jj_transpose(const Grid& g, const DevArrays& d, hipStream_t s)
 {
-    const dim3 gr((unsigned)((g.nq + 31) / 32), (unsigned)((g.Nw + 1 + 31) / 32));
+    const dim3 gr((unsigned)((g.nq + kTT - 1) / kTT), (unsigned)((g.Nw + 1 + kTT - 1) / kTT));
     DISPATCH_S(g.prec, {
         hipLaunchKernelGGL(k_transpose<SS>, gr, dim3(256), 0, s, (const SS*)d.h, (SS*)d.hT, g.nq, g.Nw + 1);
         hipLaunchKernelGGL(k_transpose<SS>, gr, dim3(256), 0, s, (const SS*)d.f0, (SS*)d.f0T, g.nq, g.Nw + 1);
@@ -2112,7 +2121,7 @@ hipError_t launch_gloc(const Grid& g, const DevArrays& d, double mu, double* par
 // [r][c] (nrows x ncols) -> [c][r]
 hipError_t launch_transpose_d(const double* in, double* out, long ncols, int nrows, hipStream_t s)
 {
-    const dim3 gr((unsigned)((ncols + 31) / 32), (unsigned)((nrows + 31) / 32));
+    const dim3 gr((unsigned)((ncols + kTT - 1) / kTT), (unsigned)((nrows + kTT - 1) / kTT));
     hipLaunchKernelGGL(k_transpose<double>, gr, dim3(256), 0, s, in, out, ncols, nrows);
     return hipGetLastError();
 }
