@@ -31,7 +31,8 @@ struct ProfPair {
 };
 
 // host_scalar: [0..2] reductions read by wait_mapped, [4] Sigma_inf, [8..15] root-search mirror,
-// [16..18] mu0 search {found, mu0, evaluations}, [19] chi0max, [20] its wedge index (int)
+// [16..18] mu0 search {found, mu0, evaluations}, [19] chi0max, [20] its wedge index (int),
+// [21..22] Tr(Sigma G) main sum and correction, [23] Tr(Sigma G) at the final mu
 constexpr int kHostScalars = 32;
 
 }  // namespace
@@ -93,6 +94,7 @@ struct tpsc_ctx {
     // chi0max and its index, copied to host_scalar[19..20] behind the chi0 stage, taken after
     // the first round trip of the Usp search, whose bracket the device sets from chi0max
     bool chi0_async = false;
+    int trace_async = 0;  // trace reductions queued, not yet read: 1 Tr(Sigma G), 2 at the final mu
     DevArrays d{};
     ALArrays al{};
     double* chi_tau = nullptr;
@@ -668,39 +670,44 @@ int stage_sigma(tpsc_ctx* c)
     return 0;
 }
 
-int reduce_trace(tpsc_ctx* c, int which, double* out)
-{
-    HIPCHK(c, launch_trace_selfG(c->g, c->d, c->mu0, c->mu, which, c->dev_scalar, c->stream));
-    return wait_mapped(c, out);
-}
-
-// green::traceSelfG, green.cpp:3001-3104
+// The trace reductions are reported values (nothing downstream reads them): each is queued
+// into its own mapped slot and read at the next wait (traces_ready), so neither costs a host
+// round trip of its own.
+// green::traceSelfG, green.cpp:3001-3104: host_scalar[21], [22]
 int stage_trace_selfG(tpsc_ctx* c)
 {
     if (!c->have_sigma) return fail(c, TPSC_E_ORDER, "traceSelfG before the self-energy");
-    // both reductions queued, one wait (each ends in its own mapped slot)
-    HIPCHK(c, launch_trace_selfG(c->g, c->d, c->mu0, c->mu, 0, c->dev_scalar, c->stream));
-    HIPCHK(c, launch_trace_selfG(c->g, c->d, c->mu0, c->mu, 1, c->dev_scalar + 1, c->stream));
-    double v[2];
-    int rc = wait_mapped(c, v, 2);
-    if (rc) return rc;
-    if ((rc = sigma_inf_ready(c))) return rc;
-    const double s0 = v[0], s1 = v[1];
-    c->trSG = s0;
+    HIPCHK(c, launch_trace_selfG(c->g, c->d, c->mu0, c->mu, 0, c->dev_scalar + 21, c->stream));
+    HIPCHK(c, launch_trace_selfG(c->g, c->d, c->mu0, c->mu, 1, c->dev_scalar + 22, c->stream));
+    c->trace_async |= 1;
     c->have_trSG = true;
-    c->trSG_corr = s0 + s1;
     return 0;
 }
 
-// green::traceSelfG2, green.cpp:2929-2997
+// green::traceSelfG2, green.cpp:2929-2997: host_scalar[23]
 int stage_trace_selfG2(tpsc_ctx* c)
 {
-    double s = 0.0;
-    int rc = reduce_trace(c, 2, &s);
-    if (rc) return rc;
-    c->trSG2 = s;
+    HIPCHK(c, launch_trace_selfG(c->g, c->d, c->mu0, c->mu, 2, c->dev_scalar + 23, c->stream));
+    c->trace_async |= 2;
     c->have_trSG2 = true;
-    c->dblOcc2 = (s + c->p.U * c->p.n * c->p.n / 4.0) / c->p.U;
+    return 0;
+}
+
+// the values of the trace reductions queued since the last read
+int traces_ready(tpsc_ctx* c)
+{
+    if (!c->trace_async) return 0;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    volatile const double* h = c->host_scalar;
+    if (c->trace_async & 1) {
+        c->trSG = h[21];
+        c->trSG_corr = h[21] + h[22];
+    }
+    if (c->trace_async & 2) {
+        c->trSG2 = h[23];
+        c->dblOcc2 = (h[23] + c->p.U * c->p.n * c->p.n / 4.0) / c->p.U;
+    }
+    c->trace_async = 0;
     return 0;
 }
 
@@ -1322,6 +1329,7 @@ int tpsc_trace_self_G(tpsc_ctx* c, double* trace, double* trace_corr)
 {
     if (!c) return TPSC_E_INVALID;
     int rc = stage_trace_selfG(c);
+    if (!rc) rc = traces_ready(c);
     if (trace) *trace = c->trSG;
     if (trace_corr) *trace_corr = c->trSG_corr;
     return rc;
@@ -1642,6 +1650,7 @@ int tpsc_set_params(tpsc_ctx* c, const tpsc_params* p)
     c->nsr = (p->n > 1.0) ? 2.0 - p->n : p->n;  // chi.cpp:18-19
     c->mu0_fixed = false;
     c->have_U = c->have_V = c->have_sigma = c->have_jj = c->have_trSG = c->have_trSG2 = false;
+    c->trace_async = 0;
     c->mu_ok = false;
     c->grad_ready = false;
     if (c->hv_pending || c->hf0_pending) {  // queued work of the previous point still reads eps(k), h, f0
@@ -1670,6 +1679,7 @@ int tpsc_get_scalar(tpsc_ctx* c, int which, double* v)
     if (!c || !v) return TPSC_E_INVALID;
     if (which == TPSC_S_SIGMA_INF)
         if (int rc = sigma_inf_ready(c)) return rc;
+    if (int rc = traces_ready(c)) return rc;
     if (int rc = chi0_ready(c)) return rc;
     if (int rc = mu0_ready(c)) return rc;
     switch (which) {
