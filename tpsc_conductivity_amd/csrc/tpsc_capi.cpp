@@ -1577,6 +1577,9 @@ int tpsc_fix_mu0(tpsc_ctx* c, double mu0)
         int rc = find_mu0(c);
         return rc ? rc : mu0_ready(c);
     }
+    // a search still in flight (queued without a wait) is taken first, so that its result
+    // cannot replace the fixed value when it is read later
+    if (int rc = mu0_ready(c)) return rc;
     c->mu0_fixed = true;
     c->mu0 = mu0;
     c->have_chi0 = c->have_U = c->have_V = c->have_sigma = c->have_jj = false;
