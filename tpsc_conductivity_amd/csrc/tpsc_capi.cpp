@@ -233,7 +233,8 @@ int wait_mapped(tpsc_ctx* c, double* out, int n = 1)
 // then on: the pass and step kernels see the point only through d.root.  Capture and
 // instantiation cost ~1.2 ms per graph and a replay saves ~15 us per chunk on small grids, so
 // only a grid that keeps being solved gets one.  TPSC_NO_GRAPHS=1 never captures.
-constexpr int kRootChunk = 6;
+// Five: most searches settle in four or five passes, and each pass queued after that costs ~8 us.
+constexpr int kRootChunk = 5;
 enum { kRootUsp = 0, kRootUch = 1, kRootMu = 2 };
 int root_chunk_graph(tpsc_ctx* c, int kind, const std::function<hipError_t()>& pass, hipGraphExec_t* out)
 {
