@@ -248,6 +248,16 @@ int tpsc_keep_chi_tau(tpsc_ctx* ctx, int on);
 /* device properties useful to callers (HBM bytes of the context's buffers) */
 int tpsc_device_bytes(tpsc_ctx* ctx, double* bytes);
 
+/* generique::find_zero (generique.cpp:604-913; with limits, :917-1234): a root of f inside
+ * init = {start, lo, hi}; lims NULL or {min, max}, which a widened bracket may not leave.
+ * The |f| tolerance is 1e-16 without limits and 1e-15 with them, as in the reference; the
+ * method is a bracketed hybrid of inverse quadratic interpolation and bisection (the
+ * library's host fallback of the sum-rule and mu searches and its Fermi-surface traces).
+ * Host only; needs no device.  Returns 0 with *root set, or TPSC_E_NOROOT with *root the best
+ * point seen; *nevals (optional) receives the number of evaluations of f. */
+int tpsc_find_zero(double (*f)(double x, void* user), void* user, const double init[3], const double* lims,
+                   double* root, int* nevals);
+
 /* ---- analytic continuation (host, FP64; needs no device) ---------------------------------
  * generique::pade_cont_frac_coef_rec (generique.h:112, generique.cpp:1650-1663): coefficients
  * of the Thiele continued-fraction Pade approximant through (z[i], f[i]), i < n.  z, f, coef:

@@ -42,7 +42,8 @@ def _params(**kw):
 
 
 @pytest.mark.parametrize("bad", [dict(nk0=20), dict(nk0=512), dict(Nw0=100), dict(precision=2), dict(T=0.0),
-                                 dict(coarse_pass=1, nk0=4)])
+                                 dict(coarse_pass=1, nk0=4), dict(n=0.0), dict(n=2.0), dict(n=2.5),
+                                 dict(n=-0.1)])
 def test_invalid_parameters_are_rejected_without_a_device(hip_lib, bad):
     h = C.c_void_p(None)
     p = _params(**bad)

@@ -31,3 +31,33 @@ def test_fixed_mu0_holds_over_the_queued_search():
     g.fix_mu0(math.nan)
     assert g.scalar("mu0") == root
     g.close()
+
+
+def test_fix_mu0_drops_every_stage_and_release_matches_fresh_context():
+    """fix_mu0 (fixed value or NaN) resets every computed stage; a solve after the release
+    gives the fresh context's results bit for bit."""
+    import numpy as np
+
+    g0 = make()
+    g0.solve()
+    ref = {k: g0.scalar(k) for k in ("mu0", "Usp", "Uch", "mu", "kx0")}
+    ref_jj = (g0.get("chijj1"), g0.get("chijj2"))
+    g0.close()
+    g = make()
+    g.solve()
+    assert g.scalar("stages") != 0
+    g.fix_mu0(ref["mu0"] + 0.02)
+    assert g.scalar("stages") == 0
+    g.solve()
+    assert g.scalar("stages") != 0
+    assert g.scalar("Usp") != ref["Usp"]
+    g.fix_mu0(math.nan)
+    assert g.scalar("stages") == 0
+    assert g.scalar("mu0") == ref["mu0"]
+    g.solve()
+    for k, v in ref.items():
+        assert g.scalar(k) == v, k
+    assert np.array_equal(g.get("chijj1"), ref_jj[0])
+    assert np.array_equal(g.get("chijj2"), ref_jj[1])
+    g.close()
+

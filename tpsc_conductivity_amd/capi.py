@@ -60,8 +60,11 @@ EXPORTS = [
     "tpsc_keep_chi_tau", "tpsc_device_bytes", "tpsc_pade_coef", "tpsc_pade_eval", "tpsc_pade_conductivity",
     "tpsc_get_params", "tpsc_write_files", "tpsc_gwn_local", "tpsc_fix_mu0", "tpsc_sigma_loss_forward",
     "tpsc_sigma_loss_backward", "tpsc_random_weights", "tpsc_sp_corr_length", "tpsc_solve_batch",
-    "tpsc_batch_tune", "tpsc_batch_points", "tpsc_set_params",
+    "tpsc_batch_tune", "tpsc_batch_points", "tpsc_set_params", "tpsc_find_zero",
 ]
+
+# double f(double x, void* user) of tpsc_find_zero
+ROOT_FN = C.CFUNCTYPE(C.c_double, C.c_double, C.c_void_p)
 
 
 def lib():
@@ -116,6 +119,7 @@ def lib():
     L.tpsc_batch_tune.argtypes = [C.POINTER(Params), C.c_int, ip, dp]
     L.tpsc_batch_points.argtypes = [C.c_ulonglong, C.c_int, C.POINTER(Params)]
     L.tpsc_set_params.argtypes = [vp, C.POINTER(Params)]
+    L.tpsc_find_zero.argtypes = [ROOT_FN, vp, dp, dp, dp, ip]
     _lib = L
     return L
 
@@ -231,6 +235,20 @@ def pade_conductivity(lam, T, npade, omega, eta):
     if rc:
         raise TPSCError(f"tpsc_pade_conductivity failed ({ERRORS.get(rc, rc)})")
     return sig
+
+
+def find_zero(f, init, lims=None):
+    """tpsc_find_zero (host root search, no device): (found, root, evaluations)."""
+    cb = ROOT_FN(lambda x, _u: float(f(x)))
+    dpp = C.POINTER(C.c_double)
+    ini = np.ascontiguousarray(init, dtype=np.float64)
+    lim = np.ascontiguousarray(lims, dtype=np.float64) if lims is not None else None
+    root, ne = C.c_double(0.0), C.c_int(0)
+    rc = lib().tpsc_find_zero(cb, None, ini.ctypes.data_as(dpp), lim.ctypes.data_as(dpp) if lim is not None else None,
+                              C.byref(root), C.byref(ne))
+    if rc not in (TPSC_OK, -5):
+        raise TPSCError(f"tpsc_find_zero failed ({ERRORS.get(rc, rc)})")
+    return rc == TPSC_OK, root.value, ne.value
 
 
 def random_weights(seed, n):

@@ -62,6 +62,7 @@ struct tpsc_ctx {
     int al_mmin = -1;
     double* gloc_buf = nullptr;  // k_gloc partials + result, allocated at the first tpsc_gwn_local
     bool mu0_fixed = false;      // tpsc_fix_mu0: mu0 is an input, not solved from n
+    bool mu0_failed = false;     // the last mu0 search found no root (mu0 kept its previous value)
     TanArrays tan{};             // tangent arrays of the Sigma-loss gradient
     bool grad_ready = false;     // forward of the Sigma loss done, weights on the device
     bool keep_chi_tau = false;
@@ -176,11 +177,12 @@ void prof_flush(tpsc_ctx* c)
 }
 
 // ------------------------------------------------------------------ root finder
-// the bracketing search of tpsc_roots.hpp (find_zero_t) over a host-evaluated function
+// the derivative-free bracketed search of tpsc_roots.hpp over a host-evaluated function, in the
+// reference's calling form {start, lo, hi} + optional limits
 bool find_zero(const std::function<double(double)>& f, const double init[3], double* root, const double* lims,
                int* nevals)
 {
-    return find_zero_t(f, init, root, lims, nevals);
+    return find_root(f, init, root, lims, nevals);
 }
 
 // TPSC_TRACE_ROOTS=1 prints every root-finder pass to stderr
@@ -352,7 +354,7 @@ int join_side(tpsc_ctx* c)
 }
 
 // green0::find_mu (green0.cpp:240-262) on the device: the whole search in one launch
-// (find_zero_t over the device density sums), queued without a wait.  The root lands in d.mu0v
+// (Halley steps on the device density sums), queued without a wait.  The root lands in d.mu0v
 // for the kernels and in host_scalar[16..18] for mu0_ready.  A fixed mu0, or mu0 = 0 at half
 // filling without t', t'' (green0.cpp:245), is stored as a preset.
 int find_mu0(tpsc_ctx* c)
@@ -374,8 +376,36 @@ int mu0_ready(tpsc_ctx* c)
     if (!c->mu0_async) return 0;
     HIPCHK(c, hipEventSynchronize(c->ev_mu0));
     volatile const double* v = c->host_scalar + 16;
-    if (v[0] != 0.0) c->mu0 = v[1];  // on failure the previous mu0 is kept, as in the reference
     c->mu0_async = false;
+    c->mu0_failed = v[0] == 0.0;
+    if (c->mu0_failed)  // the previous mu0 is kept, as in the reference ("mu0 non trouve")
+        return fail(c, TPSC_E_NOROOT, "mu0 not found: the density equation has no root; mu0 keeps its previous value");
+    c->mu0 = v[1];
+    return 0;
+}
+
+// Drops every computed stage and result (chi0 through Lambda_jj, traces, gradients), after the
+// Lambda-stage work still queued on the prep stream (it reads eps(k), mu0, h and f0) has ended.
+// The device arrays are kept.
+int invalidate_stages(tpsc_ctx* c)
+{
+    if (c->hv_pending || c->hf0_pending) {
+        HIPCHK(c, hipStreamSynchronize(c->prep));
+        c->hf0_pending = false;
+        c->hv_pending = false;
+    }
+    c->have_chi0 = c->have_U = c->have_V = c->have_sigma = c->have_jj = false;
+    c->have_trSG = c->have_trSG2 = false;
+    c->trace_async = 0;
+    c->mu_ok = false;
+    c->grad_ready = false;
+    c->hv_slabs_due = c->hv_dropped = false;
+    c->chijj1.clear();
+    c->chijj2.clear();
+    c->chijj3.clear();
+    c->chijj_all.clear();
+    c->al_freq.clear();
+    c->al_mmin = -1;
     return 0;
 }
 
@@ -476,8 +506,10 @@ int chi0_ready(tpsc_ctx* c)
     c->qmax[0] = l;
     c->qmax[1] = imax - l * (l + 1) / 2;
     c->chi0_async = false;
+    if (int rc = mu0_ready(c)) return rc;
+    if (c->mu0_failed) return fail(c, TPSC_E_NOROOT, "chi0 computed at a mu0 whose search failed");
     c->have_chi0 = true;
-    return mu0_ready(c);
+    return 0;
 }
 
 int eval_sum_rule(tpsc_ctx* c, int channel, double Ux, double* out)
@@ -1146,6 +1178,10 @@ int tpsc_create(const tpsc_params* p, tpsc_ctx** out)
         *out = c;
         return fail(c, TPSC_E_INVALID, "temperature must be positive");
     }
+    if (!(p->n > 0.0 && p->n < 2.0)) {  // the density equation has a root only for 0 < n < 2
+        *out = c;
+        return fail(c, TPSC_E_INVALID, "density n must lie in (0, 2)");
+    }
     *out = c;
     c->nsr = (p->n > 1.0) ? 2.0 - p->n : p->n;  // chi.cpp:18-19
     int dev_count = 0;
@@ -1573,19 +1609,19 @@ int tpsc_sp_corr_length(tpsc_ctx* c, double* out)
 int tpsc_fix_mu0(tpsc_ctx* c, double mu0)
 {
     if (!c) return TPSC_E_INVALID;
+    // a search still in flight (queued without a wait) is taken first, so that its result
+    // cannot replace the value set here when it is read later
+    int rc = mu0_ready(c);
+    if (rc && rc != TPSC_E_NOROOT) return rc;
+    if ((rc = invalidate_stages(c))) return rc;
     if (std::isnan(mu0)) {
         c->mu0_fixed = false;
-        int rc = find_mu0(c);
+        rc = find_mu0(c);
         return rc ? rc : mu0_ready(c);
     }
-    // a search still in flight (queued without a wait) is taken first, so that its result
-    // cannot replace the fixed value when it is read later
-    if (int rc = mu0_ready(c)) return rc;
     c->mu0_fixed = true;
+    c->mu0_failed = false;
     c->mu0 = mu0;
-    c->have_chi0 = c->have_U = c->have_V = c->have_sigma = c->have_jj = false;
-    c->mu_ok = false;
-    c->grad_ready = false;
     return 0;
 }
 
@@ -1615,6 +1651,14 @@ int tpsc_sigma_loss_backward(tpsc_ctx* c, double dloss, double* grad)
         grad[p] = dloss * dL;
     }
     return 0;
+}
+
+int tpsc_find_zero(double (*f)(double, void*), void* user, const double init[3], const double* lims, double* root,
+                   int* nevals)
+{
+    if (!f || !init || !root) return TPSC_E_INVALID;
+    const bool ok = find_root([&](double x) { return f(x, user); }, init, root, lims, nevals);
+    return ok ? TPSC_OK : TPSC_E_NOROOT;
 }
 
 int tpsc_random_weights(unsigned long long seed, size_t n, double* out)
@@ -1648,26 +1692,16 @@ int tpsc_set_params(tpsc_ctx* c, const tpsc_params* p)
     if (!c || !p) return TPSC_E_INVALID;
     if (p->nk0 != c->p.nk0 || p->Nw0 != c->p.Nw0 || p->precision != c->p.precision)
         return fail(c, TPSC_E_INVALID, "tpsc_set_params: the grid and the precision must stay those of the context");
-    if (!(p->T > 0.0) || (p->coarse_pass && (p->nk0 < 8 || p->Nw0 < 32 || !supported_N(p->nk0 / 2))))
+    if (!(p->T > 0.0) || !(p->n > 0.0 && p->n < 2.0) ||
+        (p->coarse_pass && (p->nk0 < 8 || p->Nw0 < 32 || !supported_N(p->nk0 / 2))))
         return fail(c, TPSC_E_INVALID, "tpsc_set_params: invalid parameters");
     c->p = *p;
     c->nsr = (p->n > 1.0) ? 2.0 - p->n : p->n;  // chi.cpp:18-19
     c->mu0_fixed = false;
-    c->have_U = c->have_V = c->have_sigma = c->have_jj = c->have_trSG = c->have_trSG2 = false;
-    c->trace_async = 0;
-    c->mu_ok = false;
-    c->grad_ready = false;
-    if (c->hv_pending || c->hf0_pending) {  // queued work of the previous point still reads eps(k), h, f0
-        HIPCHK(c, hipStreamSynchronize(c->prep));
-        c->hf0_pending = false;
-        c->hv_pending = false;
-    }
-    c->hv_slabs_due = c->hv_dropped = false;
-    c->chijj1.clear();
-    c->chijj2.clear();
-    c->chijj3.clear();
-    c->chijj_all.clear();
-    c->al_freq.clear();
+    // a mu0 search of the previous point still in flight is taken (and its outcome dropped)
+    int rc = mu0_ready(c);
+    if (rc && rc != TPSC_E_NOROOT) return rc;
+    if ((rc = invalidate_stages(c))) return rc;
     return set_grid(c, p->nk0, p->Nw0);  // the constructors' eps(k) and mu0 for the new point
 }
 

@@ -513,9 +513,15 @@ __global__ void __launch_bounds__(256)
 // wedge points per thread in k_mu0_root: 16 waves hold the 8385 points of N = 256
 constexpr int kMu0K = 9;
 
-// green0::find_mu (green0.cpp:240-262) in one workgroup: every thread runs the same bracketing
-// search (find_zero_t) on block-wide sums of the finite-lattice density (green0.cpp:182-203),
-// so the control flow stays uniform.  out = {found, mu0, evaluations}.
+// green0::find_mu (green0.cpp:240-262) in one workgroup, on block-wide sums of the
+// finite-lattice density n(mu) = (2/N^2) sum_wedge w f(eps - mu) (green0.cpp:182-203): every
+// thread sees the same sums, so the control flow stays uniform.  One evaluation gives n - dens
+// and its first two derivatives from the same exponential (f' = f(1-f)/T,
+// f'' = f(1-f)(1-2f)/T^2), and the Halley iteration of tpsc_roots.hpp runs from the reference's
+// start 0 in its bracket [-8, 8] (green0.cpp:248-257).  n(mu) is increasing, so it fails only
+// when the root lies outside the bracket or the derivative underflows (no level within ~700 T
+// of the iterate); the derivative-free bracketed search (bracket_root, which widens the
+// interval) then takes over.  out = {found, mu0, evaluations}.
 // Each thread keeps its kMu0K strided wedge points (eps, weight) in registers for all the
 // evaluations; a padded slot has weight 0 and adds +0.  The launcher sizes the workgroup so
 // that blockDim.x * kMu0K >= nk8.
@@ -535,24 +541,47 @@ __global__ void __launch_bounds__(1024) k_mu0_root(const double* __restrict__ ek
         }
         return;
     }
-    __shared__ double sc[16];
+    __shared__ double sc[3 * 16];
     double e[kMu0K], w[kMu0K];
 #pragma unroll
     for (int k = 0; k < kMu0K; k++) {
         const long i = threadIdx.x + (long)k * blockDim.x;
         e[k] = (i < nk8) ? ekw[i] : 0.0;
-        w[k] = (i < nk8) ? wq[i] : 0.0;
+        w[k] = (i < nk8) ? 2.0 * wq[i] / Nk2 : 0.0;
     }
-    auto density = [&](double mup) {
-        double acc = 0.0;
+    const double iT = 1.0 / T;
+    // v = {n(mu) - dens, n'(mu), n''(mu)}
+    auto density3 = [&](double mup, double* v) {
+        v[0] = v[1] = v[2] = 0.0;
 #pragma unroll
-        for (int k = 0; k < kMu0K; k++) acc += 2 * w[k] / ((exp((e[k] - mup) / T) + 1.0) * Nk2);
-        return block_sum(acc, sc) - dens;  // the same total in every thread
+        for (int k = 0; k < kMu0K; k++) {
+            const double f = 1.0 / (exp((e[k] - mup) * iT) + 1.0), g = f * (1.0 - f) * iT;
+            v[0] += w[k] * f;
+            v[1] += w[k] * g;
+            v[2] += w[k] * g * (1.0 - 2.0 * f) * iT;
+        }
+        block_sum_k<3>(v, sc);
+        v[0] -= dens;
     };
-    const double init[3] = {0.0, -8.0, 8.0};
-    double root = 0.0;
-    int ne = 0;
-    const bool ok = find_zero_t(density, init, &root, nullptr, &ne);
+    HalleyState hs = halley_start(0.0, -8.0, 8.0, 60, 0.0, 0.0);
+    while (hs.status == kRootRunning) {
+        double v[3];
+        density3(hs.x, v);
+        halley_update(hs, v);
+    }
+    bool ok = hs.status == kRootConverged;
+    double root = hs.root;
+    int ne = hs.ne;
+    if (!ok) {
+        int nb = 0;
+        auto density = [&](double mup) {
+            double v[3];
+            density3(mup, v);
+            return v[0];
+        };
+        ok = bracket_root(density, 0.0, -8.0, 8.0, nullptr, 1.0e-16, &root, &nb);
+        ne += nb;
+    }
     if (threadIdx.x == 0) {
         out[0] = ok ? 1.0 : 0.0;
         out[1] = root;

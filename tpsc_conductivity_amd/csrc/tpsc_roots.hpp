@@ -1,6 +1,6 @@
 // tpsc_roots.hpp -- root searches shared by host and device code: the Halley iteration (host
-// loop or device-driven passes) and the bracketing search find_zero_t (host, or every thread of
-// one workgroup in step).
+// loop or device-driven passes) and the derivative-free bracketed search bracket_root (host, or
+// every thread of one workgroup in step).
 //
 // Root of a monotone f inside (lo, hi) from x by Halley steps, with the bracket narrowed by the
 // sign of every evaluation and a bisection whenever a step leaves it.  One evaluation gives
@@ -93,148 +93,110 @@ __host__ __device__ inline void halley_update(HalleyState& s, const double v[3])
     if (s.ne >= s.itmax) s.status = kRootFailed;
 }
 
-// generique.cpp:604-913 (lims == nullptr) and :917-1234 (lims given): bracketed inverse
-// quadratic steps with secant fallback and bisection on stalls; secant search first when
-// the initial interval has no sign change.  Returns the abscissa of the smallest |f| kept.
-// The reference leaves the secant start unset when f(init[0]) is the smallest; it starts
-// from init[0] here.
+// Derivative-free bracketed search, for the roots whose function has no cheap derivative (the
+// host fallbacks of the sum-rule and mu searches, the Fermi-surface traces) and for the mu0
+// search when its Halley iteration fails.  Replaces the reference's find_zero
+// (generique.cpp:604-913, and :917-1234 with limits): same inputs {start, lo, hi}, the same
+// optional limits and |f| tolerance, a different method.
+//
+// 1. Bracket.  f is taken at lo and hi, and at the start when it lies strictly inside; the
+//    part of [lo, hi] whose ends differ in sign and that has the start as an end is kept
+//    ([start, hi] when both parts do, as in the reference).  Without a
+//    sign change the interval grows: the end with the smaller |f| moves outward by the current
+//    width (the width doubles each time), clamped to the limits when given; kRootWiden moves at
+//    most.  No sign change then, or a limit reached without one, is a failure.
+// 2. Shrink.  Chandrupatla's hybrid of inverse quadratic interpolation and bisection
+//    (T. R. Chandrupatla, Advances in Engineering Software 28 (1997) 145-149): the next point
+//    is a + t (b - a) on the bracket [a, b] (a the newest point, c the end just dropped).  The
+//    quadratic through (a, b, c) is used when its inverse is monotone on the bracket, i.e.
+//    phi^2 < xi and (1 - phi)^2 < 1 - xi with xi = (a - b)/(c - b), phi = (fa - fb)/(fc - fb);
+//    otherwise t = 1/2.  t is kept at least tol/|b - a| away from both ends so that every
+//    step moves by more than the tolerance.
+// Ends when |f| <= ftol at the best point, f is exactly zero, or the bracket is narrower than
+// 2 tol, tol = 2 eps |x| + xabs; the point of smaller |f| is returned.  maxev bounds the number
+// of evaluations (a bisection halves the bracket, so 200 is never reached in double).
+constexpr int kRootWiden = 64;
+
 template <typename Fn>
-__host__ __device__ bool find_zero_t(const Fn& f, const double init[3], double* root, const double* lims, int* nevals)
+__host__ __device__ bool bracket_root(const Fn& f, double x0, double lo, double hi, const double* lims, double ftol,
+                                      double* root, int* nevals, int maxev = 200, double xabs = 1.0e-15)
 {
-    const double tol = lims ? 1.0e-15 : 1.0e-16;
-    const double eps = (lims ? 2.0 : 10.0) * DBL_EPSILON;
-    const int itmax = lims ? 12 : 20;
-    const double dytol = 0.01;
-    int it = 0, ne = 0;
-    double a = 0.0, b = 0.0, c = 0.0;
-    const double xmin = init[1], xmax = init[2];
-    double x[3] = {init[0], xmin, xmax}, y[3] = {0.0, 0.0, 0.0};
-    auto F = [&](double v) {
+    int ne = 0;
+    auto F = [&](double x) {
         ne++;
-        return f(v);
+        return f(x);
     };
-    auto done = [&](double v, bool ok) {
-        *root = v;
+    auto finish = [&](double x, bool ok) {
+        *root = x;
         if (nevals) *nevals = ne;
         return ok;
     };
-    y[1] = F(xmin);
-    y[2] = F(xmax);
-    if (fabs(y[1]) < tol) return done(xmin, true);
-    if (fabs(y[2]) < tol) return done(xmax, true);
-    if (y[1] * y[2] > 0) {
-        y[0] = F(x[0]);
-        if (fabs(y[0]) < tol) return done(x[0], true);
-        if (y[0] * y[2] < 0) {
-            const double xt = x[0];
-            x[0] = (x[0] + x[2]) / 2;
-            x[1] = xt;
-            y[1] = y[0];
+    auto opposite = [](double u, double v) { return (u < 0.0 && v > 0.0) || (u > 0.0 && v < 0.0); };
+    double a = lo, b = hi;
+    double fa = F(a), fb = F(b);
+    if (!(fabs(fa) > ftol)) return finish(a, std::isfinite(fa));
+    if (!(fabs(fb) > ftol)) return finish(b, std::isfinite(fb));
+    if (x0 > lo && x0 < hi) {
+        const double f0 = F(x0);
+        if (!(fabs(f0) > ftol)) return finish(x0, std::isfinite(f0));
+        if (opposite(f0, fb))  // [x0, hi] first: the side the reference searches when both halves bracket
+            a = x0, fa = f0;
+        else if (opposite(f0, fa))
+            b = x0, fb = f0;
+    }
+    for (int k = 0; !opposite(fa, fb); k++) {
+        if (k == kRootWiden || !std::isfinite(fa) || !std::isfinite(fb)) return finish(fabs(fa) < fabs(fb) ? a : b, false);
+        const double w = b - a;
+        if (fabs(fa) <= fabs(fb)) {
+            double na = a - w;
+            if (lims && na < lims[0]) na = lims[0];
+            if (!(na < a)) return finish(a, false);
+            b = a, fb = fa;
+            a = na, fa = F(a);
+            if (!(fabs(fa) > ftol)) return finish(a, std::isfinite(fa));
         } else {
-            double yt = y[0], xt = x[0];
-            if (fabs(y[1]) < fabs(yt)) yt = y[1], xt = x[1];
-            if (fabs(y[2]) < fabs(yt)) yt = y[2], xt = x[2];
-            double xt1 = xt + 10 * eps, yt1 = F(xt1);
-            xt1 = xt - yt * ((xt1 - xt) / (yt1 - yt));
-            yt1 = F(xt1);
-            it++;
-            while (fabs(yt1) > tol && it < itmax && yt1 * yt > 0 && std::isfinite(xt1)) {
-                xt = xt1;
-                yt = yt1;
-                xt1 = xt + 10 * eps;
-                yt1 = F(xt1);
-                xt1 = xt - yt * ((xt1 - xt) / (yt1 - yt));
-                yt1 = F(xt1);
-                it++;
-            }
-            if (fabs(yt1) < tol) return done(xt1, true);
-            if (yt1 * yt < 0) {
-                x[0] = (xt + xt1) / 2;
-                x[1] = xt;
-                y[1] = yt;
-                x[2] = xt1;
-                y[2] = yt1;
-            }
-            if (it >= itmax || !std::isfinite(xt1)) return done(xt1, false);
+            double nb = b + w;
+            if (lims && nb > lims[1]) nb = lims[1];
+            if (!(nb > b)) return finish(b, false);
+            a = b, fa = fb;
+            b = nb, fb = F(b);
+            if (!(fabs(fb) > ftol)) return finish(b, std::isfinite(fb));
         }
     }
-    if (lims && (x[1] < lims[0] || x[1] > lims[1] || x[2] < lims[0] || x[2] > lims[1])) return done(x[0], false);
-    if (fabs(xmax - xmin) < eps) return done(xmin, true);
-    y[0] = F(x[0]);
-    if (fabs(y[0]) < tol) return done(x[0], true);
-
-    int i3, imid, ifar, im, ip;
-    auto sides = [&]() {
-        int i1, i2;
-        if (y[0] * y[1] > 0)
-            i1 = 0, i2 = 1;
-        else if (y[0] * y[2] > 0)
-            i1 = 0, i2 = 2;
+    double c = b, fc = fb;  // no third point yet: the first step bisects
+    double t = 0.5;
+    for (;;) {
+        const double xt = a + t * (b - a), ft = F(xt);
+        if (opposite(ft, fa)) {  // root in [xt, a]: a becomes the far end, b is dropped
+            c = b, fc = fb;
+            b = a, fb = fa;
+        } else {  // root in [xt, b]: a is dropped
+            c = a, fc = fa;
+        }
+        a = xt, fa = ft;
+        const bool aBest = fabs(fa) < fabs(fb);
+        const double xm = aBest ? a : b, fm = aBest ? fa : fb;
+        if (!std::isfinite(fm)) return finish(xm, false);
+        if (!(fabs(fm) > ftol)) return finish(xm, true);
+        const double tol = 2.0 * DBL_EPSILON * fabs(xm) + xabs, width = fabs(b - a);
+        if (width <= 2.0 * tol) return finish(xm, true);
+        if (ne >= maxev) return finish(xm, false);
+        const double tl = tol / width;
+        const double xi = (a - b) / (c - b), phi = (fa - fb) / (fc - fb);
+        if (phi * phi < xi && (1.0 - phi) * (1.0 - phi) < 1.0 - xi)
+            t = fa / (fb - fa) * fc / (fb - fc) + (c - a) / (b - a) * fa / (fc - fa) * fb / (fc - fb);
         else
-            i1 = 1, i2 = 2;
-        i3 = 3 - i1 - i2;
-        if (x[i3] > x[i1]) {
-            if (x[i1] > x[i2])
-                ifar = i2, imid = i1;
-            else
-                ifar = i1, imid = i2;
-            im = imid, ip = i3;
-        } else {
-            if (x[i1] < x[i2])
-                ifar = i2, imid = i1;
-            else
-                ifar = i1, imid = i2;
-            im = i3, ip = imid;
-        }
-    };
-    sides();
-    x[ifar] = -y[imid] * (x[i3] - x[imid]) / (y[i3] - y[imid]) + x[imid];
-    y[ifar] = F(x[ifar]);
-    double dy = y[ifar];
-    while (fabs(dy) > tol && it < itmax) {
-        sides();
-        const double den = -x[0] * x[0] * x[2] + x[0] * x[0] * x[1] - x[0] * x[1] * x[1] + x[0] * x[2] * x[2] -
-                           x[2] * x[2] * x[1] + x[2] * x[1] * x[1];
-        if (fabs(den) > 0.0) {
-            a = (x[0] * y[2] - x[0] * y[1] + x[2] * y[1] - y[2] * x[1] - y[0] * x[2] + y[0] * x[1]) / den;
-            b = -(-y[2] * x[1] * x[1] + x[1] * x[1] * y[0] - x[2] * x[2] * y[0] - y[1] * x[0] * x[0] +
-                  y[2] * x[0] * x[0] + y[1] * x[2] * x[2]) / den;
-            c = (y[2] * x[0] * x[0] * x[1] - y[1] * x[0] * x[0] * x[2] - x[1] * x[1] * x[0] * y[2] +
-                 x[2] * x[2] * x[0] * y[1] + x[1] * x[1] * y[0] * x[2] - x[2] * x[2] * y[0] * x[1]) / den;
-        }
-        auto secant_or_stop = [&](bool& stop) {
-            if (fabs(y[i3] - y[imid]) > tol)
-                x[ifar] = -y[imid] * (x[i3] - x[imid]) / (y[i3] - y[imid]) + x[imid];
-            else
-                stop = true;
-        };
-        bool stop = false;
-        if ((b * b - 4.0 * a * c) > 0.0 && fabs(den) > 0.0) {
-            const double xp = (-b + sqrt(b * b - 4.0 * a * c)) / (2 * a);
-            const double xm = (-b - sqrt(b * b - 4.0 * a * c)) / (2 * a);
-            if (xp > x[im] && xp < x[ip])
-                x[ifar] = xp;
-            else if (xm > x[im] && xm < x[ip])
-                x[ifar] = xm;
-            else
-                secant_or_stop(stop);
-        } else
-            secant_or_stop(stop);
-        if (stop) return done(x[imid], true);
-        y[ifar] = F(x[ifar]);
-        double dyprec = fabs(dy);
-        dy = y[ifar];
-        if (fabs(dyprec - fabs(dy)) < dytol * dyprec || dyprec < fabs(dy)) {
-            x[ifar] = (x[i3] + x[imid]) / 2.0;
-            y[ifar] = F(x[ifar]);
-            dy = y[ifar];
-        }
-        it++;
+            t = 0.5;
+        t = fmin(1.0 - tl, fmax(tl, t));
     }
-    int best = 0;
-    if (fabs(y[1]) < fabs(y[best])) best = 1;
-    if (fabs(y[2]) < fabs(y[best])) best = 2;
-    return done(x[best], true);
+}
+
+// The reference's calling form of its root search: init = {start, lo, hi}, optional limits,
+// |f| tolerance 1e-16 without limits and 1e-15 with them (generique.cpp:604-607,917-920).
+template <typename Fn>
+__host__ __device__ bool find_root(const Fn& f, const double init[3], double* root, const double* lims, int* nevals)
+{
+    return bracket_root(f, init[0], init[1], init[2], lims, lims ? 1.0e-15 : 1.0e-16, root, nevals);
 }
 
 }  // namespace tpsc
