@@ -29,20 +29,43 @@ TOL = 1e-6
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 
 
-def cpu_baseline(cfg):
-    """One full solve of the same workload in the FP64 C oracle on the host cores."""
+def cpu_baseline(cfg, repeats=3):
+    """The same workload in the FP64 C oracle on the host cores: built -O3 -march=native on this
+    host, one warm-up solve, then the median of `repeats` timed solves, with the median time of
+    each stage (A mu0 + chi0, B Usp/Uch roots, C vertex, D self-energy, E traceSelfG + mu,
+    F Lambda_jj)."""
+    import statistics
+
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    build = "-O3 -march=native"
+    try:
+        oracle.use_native()
+    except Exception as e:  # no compiler on this host: the portable x86-64-v3 build
+        build = f"-O3 -march=x86-64-v3 (native build failed: {e})"
+    threads = oracle.omp_max_threads()
+    nproc = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     o = oracle.Oracle(cfg["N"], cfg["Nw"], cfg["n"], cfg["T"], cfg["U"], tp=cfg["tp"], tpp=cfg["tpp"],
                       t3=cfg["t3"], coarse_pass=False)
-    t0 = time.perf_counter()
-    o.solve()
-    dt = time.perf_counter() - t0
-    return {"value": 1.0 / dt, "unit": "solves/s", "cores": threads, "kind": "port",
-            "sample": f"one full {cfg['N']}^2 x {cfg['Nw']} solve (mu0..Lambda_jj) of the FP64 C oracle, "
-                      f"OpenMP {threads} threads, {dt:.2f} s"}
+    o.solve_timed()
+    print("[bench] cpu baseline warm-up done", file=sys.stderr, flush=True)
+    totals, stages = [], []
+    for r in range(repeats):
+        t0 = time.perf_counter()
+        ok, st = o.solve_timed()
+        totals.append(time.perf_counter() - t0)
+        stages.append(st)
+        print(f"[bench] cpu baseline solve {r + 1}/{repeats}: {totals[-1]:.2f} s", file=sys.stderr, flush=True)
+    med = statistics.median(totals)
+    names = ("A_mu0_chi0", "B_Usp_Uch", "C_vertex", "D_self_energy", "E_traces_mu", "F_Lambda_jj")
+    return {"value": 1.0 / med, "unit": "solves/s", "cores": threads, "kind": "port",
+            "nproc": nproc, "omp_max_threads": threads, "build": build,
+            "stages_s": {nm: round(statistics.median(s[i] for s in stages), 4) for i, nm in enumerate(names)},
+            "solves_s": [round(t, 3) for t in totals],
+            "sample": f"median of {repeats} full {cfg['N']}^2 x {cfg['Nw']} solves (mu0..Lambda_jj) of the FP64 C "
+                      f"oracle ({build}) after one warm-up solve, OpenMP {threads} threads on {nproc} host cores, "
+                      f"{med:.2f} s per solve"}
 
 
 def make(cfg, precision):

@@ -21,6 +21,17 @@ def build(force=False):
     return _LIB_PATH
 
 
+def use_native():
+    """Build the oracle with -march=native for this host (oracle/_build_native) and load that copy
+    instead of the portable one; must run before the first lib() call.  Returns its path."""
+    global _LIB_PATH
+    if _lib is not None:
+        raise RuntimeError("oracle library already loaded")
+    subprocess.run(["make", "-s", "-C", _HERE, "native"], check=True)
+    _LIB_PATH = os.path.join(_HERE, "_build_native", "libtpsc_oracle.so")
+    return _LIB_PATH
+
+
 class OrcParams(C.Structure):
     _fields_ = [
         ("nk0", C.c_int), ("Nw0", C.c_int), ("n", C.c_double), ("T", C.c_double),
@@ -73,6 +84,9 @@ def lib():
         L.orc_create.restype = P
         L.orc_create.argtypes = [C.POINTER(OrcParams)]
         L.orc_destroy.argtypes = [P]
+        L.orc_solve_timed.argtypes = [P, _dp]
+        L.orc_solve_timed.restype = C.c_int
+        L.orc_omp_max_threads.restype = C.c_int
         for name in ("orc_find_mu0", "orc_find_Usp_Uch", "orc_vertex", "orc_self_energy",
                      "orc_trace_self_G", "orc_find_mu", "orc_trace_self_G2", "orc_solve"):
             getattr(L, name).argtypes = [P]
@@ -125,6 +139,10 @@ def lib():
         L.orc_al_tail_out.argtypes = [C.c_double] * 5 + [C.c_int, _dp]
         _lib = L
     return _lib
+
+
+def omp_max_threads():
+    return lib().orc_omp_max_threads()
 
 
 def _arr(ptr, n, dtype=np.float64):
@@ -237,6 +255,13 @@ class Oracle:
 
     def solve(self):
         return lib().orc_solve(self._o)
+
+    def solve_timed(self):
+        """A full solve without the coarse pass; (ok, seconds of the stages A mu0 + chi0, B roots,
+        C vertex, D self-energy, E traceSelfG + mu, F Lambda_jj)."""
+        t = np.zeros(6)
+        ok = lib().orc_solve_timed(self._o, t.ctypes.data_as(_dp))
+        return bool(ok), t
 
     def chiqw_discret(self, qx, qy, n):
         return lib().orc_chiqw_discret(self._o, qx, qy, n)

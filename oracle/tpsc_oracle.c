@@ -11,6 +11,8 @@
  */
 #include "tpsc_oracle.h"
 
+#include <omp.h>
+
 #include <float.h>
 #include <math.h>
 #include <stdio.h>
@@ -538,9 +540,9 @@ static double fz_sp(void *ctx, double x) { return orc_sum_rule((orc *)ctx, 0, x)
 static double fz_ch(void *ctx, double x) { return orc_sum_rule((orc *)ctx, 1, x); }
 
 /* chi.cpp:846-876 */
-int orc_find_Usp_Uch_grid(orc *o, int nq0, int nw0, double Usc[2])
+/* the Usp and Uch roots on the chi0 already computed (chi.cpp:853-869) */
+static int orc_usp_uch_roots(orc *o, double Usc[2])
 {
-    orc_chi0(o, nq0, nw0);
     const double U = o->p.U;
     double root = 0.0;
     double init[3] = {U / 2.0, 0.0, 2.0 / o->chi0max - 2 * DBL_EPSILON};
@@ -560,18 +562,24 @@ int orc_find_Usp_Uch_grid(orc *o, int nq0, int nw0, double Usc[2])
     return ok;
 }
 
-/* chi.cpp:210-417 */
-int orc_find_Usp_Uch(orc *o)
+int orc_find_Usp_Uch_grid(orc *o, int nq0, int nw0, double Usc[2])
+{
+    orc_chi0(o, nq0, nw0);
+    return orc_usp_uch_roots(o, Usc);
+}
+
+/* chi.cpp:210-417; with chi0_done the full-grid chi0 is taken as computed (the stage-timed solve) */
+static int orc_find_Usp_Uch_impl(orc *o, int chi0_done)
 {
     double Usc[2] = {0.0, 0.0};
     const int N = o->p.nk0, Nw = o->p.Nw0;
     o->Usp_coarse = o->Uch_coarse = 0.0;
-    if (o->p.coarse_pass) {
+    if (o->p.coarse_pass && !chi0_done) {
         orc_find_Usp_Uch_grid(o, N / 2, Nw / 2, Usc);
         o->Usp_coarse = Usc[0];
         o->Uch_coarse = Usc[1];
     }
-    int ok = orc_find_Usp_Uch_grid(o, N, Nw, Usc);
+    int ok = chi0_done ? orc_usp_uch_roots(o, Usc) : orc_find_Usp_Uch_grid(o, N, Nw, Usc);
     o->difRelsp = fabs((Usc[0] - o->Usp_coarse) / Usc[0]);
     o->difRelch = fabs((Usc[1] - o->Uch_coarse) / Usc[1]);
     o->Usp = Usc[0];
@@ -583,6 +591,8 @@ int orc_find_Usp_Uch(orc *o)
         o->dblOcc = o->Usp / U * nsr * nsr / 4.0;
     return ok;
 }
+
+int orc_find_Usp_Uch(orc *o) { return orc_find_Usp_Uch_impl(o, 0); }
 
 /* ---------------------------------------------------------------- vertex (a6) */
 /* chi.cpp:37-206 */
@@ -1413,6 +1423,35 @@ int orc_solve(orc *o)
     orc_chijj(o, o->N_theta > 0 ? o->N_theta : 1);
     return ok;
 }
+
+/* orc_solve without the coarse pass, with the wall time of its stages in t[6] (seconds):
+   A mu0 + chi0, B Usp/Uch roots, C vertex, D self-energy, E traceSelfG + mu, F Lambda_jj */
+int orc_solve_timed(orc *o, double t[6])
+{
+    double w0 = omp_get_wtime();
+    orc_find_mu0(o);
+    orc_chi0(o, o->p.nk0, o->p.Nw0);
+    double w1 = omp_get_wtime();
+    t[0] = w1 - w0;
+    int ok = orc_find_Usp_Uch_impl(o, 1);
+    w0 = omp_get_wtime();
+    t[1] = w0 - w1;
+    orc_vertex(o);
+    w1 = omp_get_wtime();
+    t[2] = w1 - w0;
+    orc_self_energy(o);
+    w0 = omp_get_wtime();
+    t[3] = w0 - w1;
+    orc_trace_self_G(o);
+    ok &= orc_find_mu(o);
+    w1 = omp_get_wtime();
+    t[4] = w1 - w0;
+    orc_chijj(o, o->N_theta > 0 ? o->N_theta : 1);
+    t[5] = omp_get_wtime() - w1;
+    return ok;
+}
+
+int orc_omp_max_threads(void) { return omp_get_max_threads(); }
 
 void orc_self_kw_brute_out(orc *o, int kx, int ky, int n, double out[2])
 {

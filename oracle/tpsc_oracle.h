@@ -103,6 +103,8 @@ int orc_find_mu(orc *o);                                /* green.cpp:3185-3282 *
 int orc_trace_self_G2(orc *o);                          /* green.cpp:2929-2997 */
 int orc_chijj(orc *o, int N_theta);                     /* cond_opt.cpp:2537-4290 */
 int orc_solve(orc *o);                                  /* tpsc_main.cpp:61-75 + E4 */
+int orc_solve_timed(orc *o, double t[6]);               /* the same without the coarse pass, stage times */
+int orc_omp_max_threads(void);
 int orc_gwn_local(orc *o, double complex *out);          /* green.cpp:209-266 */
 void orc_fix_mu0(orc *o, double mu0);                    /* mu0 as an input parameter   */
 double orc_hot_spot_vF(orc *o);                          /* chi.cpp:419-447, green0.cpp:626-675 */
