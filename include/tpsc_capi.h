@@ -106,7 +106,8 @@ enum tpsc_stage {
  * eps(k) and mu0 (green0.cpp:18-23). */
 int tpsc_create(const tpsc_params* p, tpsc_ctx** out);
 void tpsc_destroy(tpsc_ctx* ctx);
-const char* tpsc_last_error(const tpsc_ctx* ctx);
+const char* tpsc_last_error(const tpsc_ctx* ctx); /* ctx NULL: the last failed context-free call
+                                                      of the calling thread */
 
 /* green0::find_mu, green0.cpp:240-262 (finite-lattice mu0, root of ddensity_mu_PBC) */
 int tpsc_find_mu0(tpsc_ctx* ctx, double* mu0);
@@ -118,6 +119,22 @@ int tpsc_chi0(tpsc_ctx* ctx, int nq0, int nw0);
 /* chi::sum_rule_sp / sum_rule_ch at one candidate vertex, chi.cpp:833-843
  * (channel 0: spin, 1: charge; the charge rule uses the current Usp) */
 int tpsc_sum_rule(tpsc_ctx* ctx, int channel, double Ux, double* value);
+
+/* The value of tpsc_sum_rule without a context and without any allocation.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Inputs are device arrays in
+ * the layouts tpsc_get copies out: chi0_qn = chi0(q,iqn) (TPSC_F_CHI0, (Nw/2+1)*nk8 doubles),
+ * chi0_inf and chi0_inf2 (nk8 doubles each), nk8 = (N/2+1)(N/2+2)/2; Usp is read by the charge
+ * channel only.  The value of chi::sum_rule_sp (channel 0) or sum_rule_ch (channel 1) at Ux
+ * (chi.cpp:833-843) goes to value[0], a device address, asynchronously on `stream` (a
+ * hipStream_t; NULL = the default stream).  The weights and partial sums live in a fixed
+ * 128 KiB array of the library on each GPU (held until the process ends), which limits N to
+ * 256; larger N gives TPSC_E_INVALID.  Calls that overlap on one GPU share that array and race.
+ * Errors: TPSC_E_INVALID, TPSC_E_DEVICE, with the message in tpsc_last_error(NULL) on the
+ * calling thread. */
+int tpsc_sum_rule_noalloc(const double* chi0_qn, const double* chi0_inf, const double* chi0_inf2, int N, int Nw,
+                          double T, double n, double U, double Usp, int channel, double Ux, double* value,
+                          void* stream);
 
 /* chi::find_Usp_Uch(int nq0, int nw0, double Usc[2]), chi.cpp:846-876 */
 int tpsc_find_Usp_Uch_grid(tpsc_ctx* ctx, int nq0, int nw0, double* Usp, double* Uch);

@@ -58,3 +58,18 @@ def test_null_arguments(hip_lib):
     assert hip_lib.tpsc_create(None, None) == -1
     assert hip_lib.tpsc_chi0(None, 16, 64) == -1
     assert hip_lib.tpsc_chijj(None, 0, 3, 1) == -1
+
+
+def test_noalloc_sum_rule_rejects_bad_sizes_without_a_device(hip_lib):
+    """tpsc_sum_rule_noalloc checks its sizes before any device call: N beyond its fixed
+    scratch array, unsupported grids and a bad channel give TPSC_E_INVALID and a message in
+    tpsc_last_error(NULL)."""
+    x = C.c_void_p(16)  # never dereferenced
+    f = hip_lib.tpsc_sum_rule_noalloc
+    assert f(x, x, x, 512, 256, 0.1, 0.9, 4.0, 0.0, 0, 1.0, x, None) == -1
+    assert b"N=512" in hip_lib.tpsc_last_error(None)
+    assert f(x, x, x, 20, 256, 0.1, 0.9, 4.0, 0.0, 0, 1.0, x, None) == -1
+    assert f(x, x, x, 64, 100, 0.1, 0.9, 4.0, 0.0, 0, 1.0, x, None) == -1
+    assert f(x, x, x, 64, 256, 0.1, 0.9, 4.0, 0.0, 2, 1.0, x, None) == -1
+    assert b"channel" in hip_lib.tpsc_last_error(None)
+    assert f(None, x, x, 64, 256, 0.1, 0.9, 4.0, 0.0, 0, 1.0, x, None) == -1

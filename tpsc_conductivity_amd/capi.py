@@ -60,7 +60,7 @@ EXPORTS = [
     "tpsc_keep_chi_tau", "tpsc_device_bytes", "tpsc_pade_coef", "tpsc_pade_eval", "tpsc_pade_conductivity",
     "tpsc_get_params", "tpsc_write_files", "tpsc_gwn_local", "tpsc_fix_mu0", "tpsc_sigma_loss_forward",
     "tpsc_sigma_loss_backward", "tpsc_random_weights", "tpsc_sp_corr_length", "tpsc_solve_batch",
-    "tpsc_batch_tune", "tpsc_batch_points", "tpsc_set_params", "tpsc_find_zero",
+    "tpsc_batch_tune", "tpsc_batch_points", "tpsc_set_params", "tpsc_find_zero", "tpsc_sum_rule_noalloc",
 ]
 
 # double f(double x, void* user) of tpsc_find_zero
@@ -120,6 +120,9 @@ def lib():
     L.tpsc_batch_points.argtypes = [C.c_ulonglong, C.c_int, C.POINTER(Params)]
     L.tpsc_set_params.argtypes = [vp, C.POINTER(Params)]
     L.tpsc_find_zero.argtypes = [ROOT_FN, vp, dp, dp, dp, ip]
+    # device addresses (chi0(q,iqn), its tails, the value) and the stream as plain pointers
+    L.tpsc_sum_rule_noalloc.argtypes = [vp, vp, vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                        C.c_double, C.c_int, C.c_double, vp, vp]
     _lib = L
     return L
 

@@ -21,6 +21,8 @@
 
 using namespace tpsc;
 
+const char* tpsc_noalloc_error();  // tpsc_noalloc.hip: message of the last failed call on this thread
+
 namespace {
 
 struct ProfPair {
@@ -1296,7 +1298,12 @@ void tpsc_destroy(tpsc_ctx* c)
     delete c;
 }
 
-const char* tpsc_last_error(const tpsc_ctx* c) { return c ? c->err.c_str() : "null context"; }
+const char* tpsc_last_error(const tpsc_ctx* c)
+{
+    if (c) return c->err.c_str();
+    const char* m = tpsc_noalloc_error();
+    return *m ? m : "null context";
+}
 
 int tpsc_find_mu0(tpsc_ctx* c, double* mu0)
 {

@@ -64,6 +64,16 @@ __device__ inline cd csqrtd(cd z)
 }
 
 __device__ inline long widx(long l, long m) { return m + (l * (l + 1)) / 2; }
+// weight of wedge point (l, m), m <= l, in sums over the full zone: 8 images, halved on each
+// of the edges l, m in {0, nk - 1} and on the diagonal m = l (green0.cpp:192-195)
+__device__ inline double wedge_weight(int l, int m, int nk)
+{
+    double ww = 8;
+    if (l == 0 || l == nk - 1) ww /= 2;
+    if (m == 0 || m == nk - 1) ww /= 2;
+    if (m == l) ww /= 2;
+    return ww;
+}
 __device__ inline long fold_idx(long x, long y, long N)
 {
     if (x > N / 2) x = N - x;

@@ -1021,11 +1021,7 @@ __global__ void k_model(int N, double t, double tp, double tpp, double t3, doubl
         if (m <= l) {
             const long w = widx(l, m);
             ekw[w] = ek[i];
-            double ww = 8;
-            if (l == 0 || l == nk - 1) ww /= 2;
-            if (m == 0 || m == nk - 1) ww /= 2;
-            if (m == l) ww /= 2;
-            wq[w] = ww;
+            wq[w] = wedge_weight(l, m, nk);
             er[w] = (w == 1) ? -t : (w == 2) ? -tp : (w == 3) ? -tpp : (w == 4) ? -t3 : 0.0;
         }
     }
@@ -1925,11 +1921,24 @@ static dim3 trace_grid(const Grid& g) { return dim3((unsigned)((g.nk8 + kTraceK 
 
 hipError_t launch_sum_rule(const Grid& g, const DevArrays& d, double sU, double* out, hipStream_t s)
 {
-    const double Nq = 4.0 * (g.nk - 1) * (g.nk - 1);
-    const dim3 gr = red_grid(g);
-    hipLaunchKernelGGL(k_sum_rule, gr, dim3(256), 0, s, d.chiw, d.chi0inf, d.chi0inf2, d.wq, (int)g.nk8, g.Nw / 2,
-                       g.T, Nq, sU, d.part);
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, d.part, (int)(gr.x * gr.y), out);
+    int np = 0;
+    const hipError_t e = launch_sum_rule_part(g.N, g.Nw, g.T, d.chiw, d.chi0inf, d.chi0inf2, d.wq, sU, d.part, &np, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, d.part, np, out);
+    return hipGetLastError();
+}
+
+int sum_rule_parts(int N) { return (int)((((long)(N / 2 + 1) * (N / 2 + 2) / 2) + 255) / 256) * kRowGroups; }
+
+hipError_t launch_sum_rule_part(int N, int Nw, double T, const double* chiw, const double* inf1, const double* inf2,
+                                const double* wq, double sU, double* part, int* nparts, hipStream_t s)
+{
+    const int nk = N / 2 + 1;
+    const long nk8 = (long)nk * (nk + 1) / 2;
+    const double Nq = 4.0 * (nk - 1) * (nk - 1);
+    const dim3 gr((unsigned)((nk8 + 255) / 256), kRowGroups);
+    hipLaunchKernelGGL(k_sum_rule, gr, dim3(256), 0, s, chiw, inf1, inf2, wq, (int)nk8, Nw / 2, T, Nq, sU, part);
+    *nparts = (int)(gr.x * gr.y);
     return hipGetLastError();
 }
 

@@ -141,6 +141,12 @@ hipError_t launch_chi0_slab(const Grid& g, const DevArrays& d, hipStream_t s);
 hipError_t launch_chi0_pencil(const Grid& g, const DevArrays& d, hipStream_t s);
 hipError_t launch_chi0max(const Grid& g, const DevArrays& d, hipStream_t s);
 hipError_t launch_sum_rule(const Grid& g, const DevArrays& d, double sU, double* out, hipStream_t s);
+// the partial sums of the sum-rule trace at sU over explicit arrays (chi0(q,iqn) nwb*nk8,
+// its two tail coefficients and the wedge weights, nk8 each): sum_rule_parts(N) doubles to part,
+// their count to *nparts
+int sum_rule_parts(int N);
+hipError_t launch_sum_rule_part(int N, int Nw, double T, const double* chiw, const double* inf1, const double* inf2,
+                                const double* wq, double sU, double* part, int* nparts, hipStream_t s);
 // Device-driven Halley search (tpsc_roots.hpp) in d.root: initialise it, then one pass of
 // the sum rule (channel sign sgn, Ux = the current iterate) or of traceDiffG (pmu = the current
 // iterate) with its first two derivatives, followed by the step.  T and mu0 are read from the
