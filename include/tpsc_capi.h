@@ -56,6 +56,11 @@ typedef struct tpsc_params {
  * every array to FP64. */
 #define TPSC_PREC_FP64 0
 #define TPSC_PREC_MIXED 1
+/* In the mixed path the chi0 stage (slabs and pencils) runs in FP64 for a point whose spin
+ * enhancement 1/(1 - Usp chi0max/2) exceeds 8: chi_sp multiplies the relative error of chi0 by
+ * that factor.  The point's first Usp search runs on the FP32 chi0; if it finds the enhancement
+ * above 8, chi0 and the roots are redone in FP64, and later solves of the point (the same
+ * parameters) start in FP64 (TPSC_S_CHI0_FP64). */
 
 typedef struct tpsc_ctx tpsc_ctx;
 
@@ -84,8 +89,14 @@ enum tpsc_field {
                              entry 0 = 0                                (cond_opt.cpp:6396-6398) */
     TPSC_F_CHIJJ_ALL = 19, /* Lambda_jj1 + Lambda_jj2 + Lambda_jj3 on that set, al_N values
                                                                         (cond_opt.cpp:6460-6467) */
-    TPSC_F_AL_FREQ = 20   /* the sparse boson indices n (as doubles), al_N values
+    TPSC_F_AL_FREQ = 20,  /* the sparse boson indices n (as doubles), al_N values
                                                                         (cond_opt.cpp:5123-5156) */
+    TPSC_F_CHIJJ1_K = 21, /* Lambda_jj1 by angle of k, nwb * N_theta_k values [b + n*N_theta_k]
+                                                        (cond_opt.cpp:3140-3174, 3792-3847) */
+    TPSC_F_CHIJJ2_K = 22, /* Lambda_jj2 by angle of k, the same layout  (cond_opt.cpp:3799-3809) */
+    TPSC_F_CHIJJ3_K = 23  /* Lambda_jj3 by angle of k on the sparse set, al_N * N_theta_k values
+                             [b + j*N_theta_k], entry j = 0 zero; only when N_theta_k > 1
+                                                                        (cond_opt.cpp:6261-6398) */
 };
 enum tpsc_scalar {
     TPSC_S_MU0 = 1, TPSC_S_CHI0MAX, TPSC_S_QMAX_L, TPSC_S_QMAX_M, TPSC_S_USP, TPSC_S_UCH,
@@ -93,8 +104,12 @@ enum tpsc_scalar {
     TPSC_S_TRSG_CORR, TPSC_S_MU, TPSC_S_TRSG2, TPSC_S_DBLOCC2, TPSC_S_KX0, TPSC_S_CHIJJ1_INF2,
     TPSC_S_CHIJJ2_INF2, TPSC_S_CHIJJ2_INF4, TPSC_S_NEVAL_SP, TPSC_S_NEVAL_CH, TPSC_S_NEVAL_MU,
     TPSC_S_NK, TPSC_S_NK8, TPSC_S_NWB, TPSC_S_NQ, TPSC_S_AL_N, TPSC_S_AL_MMIN,
-    TPSC_S_STAGES  /* bitmask of the completed stages, TPSC_STAGE_* (no reference counterpart:
+    TPSC_S_STAGES, /* bitmask of the completed stages, TPSC_STAGE_* (no reference counterpart:
                       the reference keeps this state in its members) */
+    TPSC_S_N_THETA, /* the angle bins N_theta_k of the last tpsc_chijj call */
+    TPSC_S_CHI0_FP64 /* 1 if the chi0 stage of the full grid runs in FP64: always in the FP64 path;
+                        in the mixed path once the point's spin enhancement 1/(1 - Usp chi0max/2)
+                        exceeded 8 (no reference counterpart) */
 };
 enum tpsc_stage {
     TPSC_STAGE_CHI0 = 1, TPSC_STAGE_USP_UCH = 2, TPSC_STAGE_VERTEX = 4, TPSC_STAGE_SIGMA = 8,
@@ -162,8 +177,10 @@ int tpsc_find_mu(tpsc_ctx* ctx, double* mu);
 /* cond_opt::calc_chijj_vertex_corr_all_optim(chi3, mmin, N_theta_k, k_chijj, nk_chijj),
  * cond_opt.cpp:2537-6467.  Bubble + Maki-Thompson + kx0; with chi3 != 0 also the
  * Aslamazov-Larkin term on the sparse boson set of parameter mmin >= 0 (TPSC_F_CHIJJ3,
- * TPSC_F_CHIJJ_ALL, TPSC_F_AL_FREQ).  N_theta_k must be 1 (the reference driver's value,
- * tpsc_main.cpp:243).  Runs the missing upstream stages first, as cond_opt.cpp:2550-2560 does. */
+ * TPSC_F_CHIJJ_ALL, TPSC_F_AL_FREQ; Nw <= 2048).  N_theta_k in 1..64: the contributions are also
+ * summed per angle bin b = floor(atan((N/2 - ky)/(N/2 - kx)) / (pi/(2 N_theta_k))) of k
+ * (TPSC_F_CHIJJ1_K, _CHIJJ2_K, _CHIJJ3_K; the reference driver uses 1, tpsc_main.cpp:243).
+ * Runs the missing upstream stages first, as cond_opt.cpp:2550-2560 does. */
 int tpsc_chijj(tpsc_ctx* ctx, int chi3, int mmin, int N_theta_k);
 
 /* one full solve: constructor state -> calc_Self_FFT_spline -> traceSelfG -> find_mu ->
@@ -242,9 +259,64 @@ int tpsc_get_params(tpsc_ctx* ctx, tpsc_params* out);
  *   kx0_*, kx0_bin_*, chijj1_qn_*, chijj2_qn_*, chijj_with_vc1_qn_*   cond_opt.cpp:4195-4279
  *   chijj3_qn_*, chijj_with_all_vc_qn_*      cond_opt.cpp:6447-6481
  *   Gwn_loc_*                                green.cpp:209-266 (after find_mu)
+ *   FS_*, FS_discret_AkF_*, FS_discret_*     green0.cpp:51-105, 626-676 (constructor)
+ *   FS_inter_AkF_*, FS_inter_*, FS_inter_AkF_Pade_*, FS_inter_Pade_*   green.cpp:2210-2341 (after find_mu)
+ *   Self_Pade_k_*, Akw_Pade_k_*              green.cpp:942-1065 (MDC of tpsc_main.cpp:101)
+ *   chijj1_k_qn_*, chijj1_k_tot_qn_*, chijj2_k_qn_*, chijj2_k_tot_qn_*   cond_opt.cpp:3889-3946
+ *   chijj3_k_qn_*, chijj3_k_tot_qn_*         cond_opt.cpp:6483-6520 (N_theta_k > 1)
  * The files the reference builds in several appends are written whole.  *nfiles (optional)
  * receives the number of files written. */
 int tpsc_write_files(tpsc_ctx* ctx, const char* dir, int* nfiles);
+
+/* ---- post-processing of a solved point (host FP64 over the device results, as the reference's
+ * main runs it after find_mu, tpsc_main.cpp:77-101) ---------------------------------------------
+ * green0::find_mu_inf_syst (green0.cpp:221-236, 266-287): mu0 of the infinite lattice, the root of
+ * (2/pi^2) int_[0,pi]^2 f(eps(k) - mu) dk = n by nested adaptive Gauss-Kronrod quadrature; 0 at
+ * half filling without t', t''.  Needs no computed stage. */
+int tpsc_mu0_infinite(tpsc_ctx* ctx, double* mu0);
+/* green0::find_FS(nk) (green0.cpp:626-676): for kx0 = j pi/(nk-1), the root kx in [kx0, pi] of
+ * eps(kx, kx - kx0) = mu0 (finite-lattice mu0); kxy[2j], kxy[2j+1] = (kx, kx - kx0), -1 from the
+ * first j without a root on; *count (optional) = the number found. */
+int tpsc_fermi_surface(tpsc_ctx* ctx, int nk, double* kxy, int* count);
+/* the lattice Fermi surface along the diagonals x = l + m, y = m: per l < N/2+1, the m of largest
+ * A(k, omega = 0) and that A.  kind 0: non-interacting, eta = 1e-5 (green0::find_FS_PBC,
+ * green0.cpp:51-105); 1: interacting, Sigma(k, i eta) by the 3-point polynomial of Sigma(k, iw_n),
+ * n < 3; 2: the 4-point Pade; eta = 1e-6 (green::find_FS_inter, green.cpp:2210-2341; needs mu). */
+int tpsc_fs_lattice(tpsc_ctx* ctx, int kind, int* m_idx, double* Ak);
+/* green::MDC(n, Nn, w, eta, Neta, use_Pade) (green.cpp:814-1068): Sigma(k, w + i eta_e) and
+ * A(k, w + i eta_e) at every wedge point k, continued from Sigma(k, i w_{n_j}), j < Nn (the stored
+ * frequencies j = 0..Nn-1 at the points n[j], as the reference indexes them) by the Nn-point Pade
+ * (use_pade) or the 4- or 3-point polynomial.  self_w: nk8*Neta complex [k*Neta + e] interleaved
+ * re/im; akw: nk8*Neta [k*Neta + e]; either may be NULL.  Needs Sigma and mu. */
+int tpsc_mdc(tpsc_ctx* ctx, const int* n, int Nn, double w, const double* eta, int Neta, int use_pade,
+             double* self_w, double* akw);
+/* green::EDC(kx, ky, Nk, w, nw0, m_freq, NP_max, eta, N_eta) with calc_spectral_weight
+ * (green.cpp:589-700, 1073-1137): A(k_p, w_i + i eta_e) from the Pade through the first
+ * NP = min(N_freq, NP_max) points of the non-uniform Matsubara set of parameter m_freq
+ * (create_non_uniform_ind_freq, green.cpp:556-583).  spectr: Nk*nw0*Neta [(p*nw0 + i)*Neta + e].
+ * k indices fold into [0, N/2] as the reference's. */
+int tpsc_edc(tpsc_ctx* ctx, const int* kx, const int* ky, int Nk, const double* w, int nw0, int m_freq, int NP_max,
+             const double* eta, int Neta, double* spectr);
+/* green::calc_Self_Re_w(k, n, Nn, w, Nw, eta, Neta) (green.cpp:408-551): at k = (k[0], k[1]),
+ * 0 <= k[i] <= N/2, Sigma(w_i + i eta_e) and A from the Nn-point Pade (Nn >= 4), and from the
+ * continued-fraction form Sigma = Sinf/(z + b1 + S(z)), S the Pade of the transformed remainder on
+ * max(Nn/8, 4) points ("Jfrac").  self_w, self_jfrac: Nw*Neta complex [i*Neta + e]; akw, akw_jfrac:
+ * Nw*Neta; any may be NULL. */
+int tpsc_self_real_freq(tpsc_ctx* ctx, const int* k, const int* n, int Nn, const double* w, int Nw, const double* eta,
+                        int Neta, double* self_w, double* akw, double* self_jfrac, double* akw_jfrac);
+/* green::save_Self_bin (green.cpp:1496-1514), FP64 path: self_bin_* in dir (NULL = cwd), the
+ * (Nw/2+1)*nk8 complex values of the reference's Self_kw_array after calc_Self_FFT_spline: the
+ * remainder R(k, n), n < Nw/2, then (Sigma(k, tau_{Nw/2}), Sigma(k, beta)). */
+int tpsc_save_self_bin(tpsc_ctx* ctx, const char* dir);
+/* chi0::save_chi (chi0.cpp:61-96): chi0_* in dir, int nbq = N/2+1, int nwmax = Nw/2, then
+ * chi0(q, iq_n), n = 0..nwmax, nk8 doubles each. */
+int tpsc_save_chi0_bin(tpsc_ctx* ctx, const char* dir);
+/* green::load_Self_bin (green.cpp:1516-1540): the (Nw/2+1)*nk8 complex values of a self_bin_*
+ * file of an N x N, Nw grid into out (interleaved).  Needs no context. */
+int tpsc_read_self_bin(const char* file, int N, int Nw, double* out);
+/* chi0::loadChi0 (chi0.cpp:99-156): the header of a chi0_* file and, when out is not NULL and cap
+ * (doubles) suffices, its (nwmax+1) * nbq(nbq+1)/2 values.  Needs no context. */
+int tpsc_read_chi0_bin(const char* file, int* nbq, int* nwmax, double* out, size_t cap);
 
 /* copy-out in the reference layouts; bytes must be >= the field size */
 int tpsc_get(tpsc_ctx* ctx, int field, void* dst, size_t bytes);

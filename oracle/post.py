@@ -237,7 +237,7 @@ def edc(ek, nbk, mu, sig_of, T, nbw, kxs, kys, w, m_freq, np_max, etas):
     return out
 
 
-def self_real_freq(ek, nbk, mu, sig_of, Sinf, C2, C3, R_of, T, k, nidx, w, etas):
+def self_real_freq(ek, nbk, mu, Sinf, C2, C3, R_of, T, k, nidx, w, etas):
     """green::calc_Self_Re_w(k, n, Nn, w, Nw, eta, Neta): Sigma(w + i eta) and A from the Nn-point
     Pade, then from the continued-fraction form Sigma = a1/(z + b1 + S(z)) with S the Pade of the
     transformed data on Nn/8 (at least 4) points.  Returns (self, akw, self_jfrac, akw_jfrac)."""
@@ -247,7 +247,15 @@ def self_real_freq(ek, nbk, mu, sig_of, Sinf, C2, C3, R_of, T, k, nidx, w, etas)
     kk = widx(x, y)
     Nn = len(nidx)
     z0 = 1j * (2 * np.asarray(nidx) + 1) * PI * T
-    func = np.array([sig_of(j, kk) for j in range(Nn)])
+    # Sinf/z + C2/z^2 + C3/z^3 + R/z^4 at z = i w in real arithmetic (green.cpp:428-433)
+    func = []
+    for j in range(Nn):
+        wn = (2 * nidx[j] + 1) * PI * T
+        w2 = wn * wn
+        w3, w4 = w2 * wn, w2 * w2
+        r = R_of(j, kk)
+        func.append(complex(-C2[kk] / w2 + r.real / w4, -Sinf / wn + C3[kk] / w3 + r.imag / w4))
+    func = np.array(func)
     ekmu = ek_val(ek, nbk, k[0], k[1]) - mu
     sw = np.zeros((len(w), len(etas)), dtype=complex)
     ak = np.zeros((len(w), len(etas)))

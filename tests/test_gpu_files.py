@@ -1,7 +1,7 @@
 """The input.dat driver and tpsc_write_files on the GPU: the reference's file set, names and
-formats (hamiltonien.cpp:72-82, chi0.cpp:720-737, chi.cpp:375-415, green.cpp:2978-2994,
-3088-3103, 3259-3276, cond_opt.cpp:4195-4279, 6447-6481), with the same numbers the C ABI
-returns for the same parameter point."""
+formats (hamiltonien.cpp:72-82, green0.cpp:51-105, 626-676, chi0.cpp:720-737, chi.cpp:375-415,
+green.cpp:942-1065, 2210-2341, 2978-2994, 3088-3103, 3259-3276, cond_opt.cpp:3889-3946, 4195-4279,
+6447-6481), with the same numbers the C ABI returns for the same parameter point."""
 import os
 import struct
 import subprocess
@@ -33,6 +33,19 @@ def names(qx, qy):
         "jjv": "chijj_with_vc1_qn_%5.3f_%5.3f_%5.3f_%5.3f_%7.5f_%6.4f_%1d_%1d.dat" % a8,
         "jj3": "chijj3_qn_%5.3f_%5.3f_%5.3f_%5.3f_%7.5f_%6.4f_%1d_%1d_%1d.dat" % (a8 + (mmin,)),
         "jja": "chijj_with_all_vc_qn_%5.3f_%5.3f_%5.3f_%5.3f_%7.5f_%6.4f_%1d_%1d_%1d.dat" % (a8 + (mmin,)),
+        "fs": "FS_%5.3f_%5.3f_%5.3f_%7.5f_%6.4f_%1d.dat" % (tp, tpp, t3, n, T, 10001),
+        "fsdA": "FS_discret_AkF_%5.3f_%5.3f_%5.3f_%7.5f_%6.4f_%1d.dat" % (tp, tpp, t3, n, T, N),
+        "fsd": "FS_discret_%5.3f_%5.3f_%5.3f_%7.5f_%6.4f_%1d.dat" % (tp, tpp, t3, n, T, N),
+        "fsiA": "FS_inter_AkF_%4.2f_%5.3f_%5.3f_%5.3f_%7.5f_%6.4f_%1d_%1d.dat" % a8,
+        "fsi": "FS_inter_%4.2f_%5.3f_%5.3f_%5.3f_%7.5f_%6.4f_%1d_%1d.dat" % a8,
+        "fsiAP": "FS_inter_AkF_Pade_%4.2f_%5.3f_%5.3f_%5.3f_%7.5f_%6.4f_%1d_%1d.dat" % a8,
+        "fsiP": "FS_inter_Pade_%4.2f_%5.3f_%5.3f_%5.3f_%7.5f_%6.4f_%1d_%1d.dat" % a8,
+        "mdcS": "Self_Pade_k_%4.2f_%5.3f_%5.3f_%5.3f_%7.5f_%6.4f_%5.3e_%1d_%1d.dat" % (a8[:6] + (0.0, N, Nw)),
+        "mdcA": "Akw_Pade_k_%4.2f_%5.3f_%5.3f_%5.3f_%7.5f_%6.4f_%5.3e_%1d_%1d.dat" % (a8[:6] + (0.0, N, Nw)),
+        "jj1k": "chijj1_k_qn_%5.3f_%5.3f_%5.3f_%5.3f_%7.5f_%6.4f_%1d_%1d.dat" % a8,
+        "jj1kt": "chijj1_k_tot_qn_%5.3f_%5.3f_%5.3f_%5.3f_%7.5f_%6.4f_%1d_%1d.dat" % a8,
+        "jj2k": "chijj2_k_qn_%5.3f_%5.3f_%5.3f_%5.3f_%7.5f_%6.4f_%1d_%1d.dat" % a8,
+        "jj2kt": "chijj2_k_tot_qn_%5.3f_%5.3f_%5.3f_%5.3f_%7.5f_%6.4f_%1d_%1d.dat" % a8,
     }
 
 
@@ -107,6 +120,28 @@ def test_driver_writes_the_reference_file_set(tmp_path):
         rows = table(out / want[key], 2)
         assert [int(r[0]) for r in rows] == list(f)
         assert np.allclose([float(r[1]) for r in rows], ref, rtol=1e-14, atol=1e-300)
+    # Fermi surfaces (green0.cpp:51-105, 626-676; green.cpp:2210-2341) and the MDC (green.cpp:942-1065)
+    fs = np.array([[float(x) for x in r] for r in table(out / want["fs"], 2)])
+    assert np.abs(fs - g.find_FS(10001)).max() < 1e-9  # setprecision(10)
+    nk8 = (N // 2 + 1) * (N // 2 + 2) // 2
+    m0, a0 = g.find_FS_PBC()
+    rows = table(out / want["fsdA"], 3)
+    assert [int(r[1]) for r in rows] == list(m0) and [int(r[0]) for r in rows] == [l + m for l, m in enumerate(m0)]
+    assert [int(r[1]) for r in table(out / want["fsd"], 2)] == list(m0)
+    (mp, ap), (mq, aq) = g.find_FS_inter()
+    assert [int(r[1]) for r in table(out / want["fsiA"], 3)] == list(mp)
+    assert [int(r[1]) for r in table(out / want["fsiAP"], 3)] == list(mq)
+    assert len(table(out / want["fsi"], 2)) == len(table(out / want["fsiP"], 2)) == N // 2 + 1
+    sw, ak = g.MDC(list(range(Nw // 2)), 10, 0.0, [0.01, 0.001, 0.0001, 0.00001])
+    rows = table(out / want["mdcS"], 2 + 8)
+    assert len(rows) == nk8
+    assert np.allclose([float(r[2]) for r in rows], sw[:, 0].real, rtol=1e-11, atol=1e-13)
+    rows = table(out / want["mdcA"], 2 + 4)
+    assert np.allclose([[float(x) for x in r[2:]] for r in rows], ak, rtol=1e-11, atol=1e-13)
+    # angle-binned Lambda_jj with one bin (cond_opt.cpp:3889-3946)
+    rows = table(out / want["jj1kt"], 4)
+    assert np.allclose([float(r[1]) for r in rows], j1, rtol=1e-14, atol=0)
+    assert [int(r[1]) for r in table(out / want["jj2k"], 3)] == [1] * (Nw // 2 + 1)
     g.close()
 
 
@@ -114,8 +149,8 @@ def test_write_files_follows_the_completed_stages(tmp_path):
     from tpsc_conductivity_amd.tpsc import CondOpt
 
     g = CondOpt(8, 32, 0.9, 0.25, params=(-0.1, 0.0, 0.0, 2.5), coarse_pass=False)
-    assert g.write_files(str(tmp_path)) == 1           # e_k only
+    assert g.write_files(str(tmp_path)) == 4           # e_k, the Fermi surfaces of the constructor
     g.find_Usp_Uch()
-    assert g.write_files(str(tmp_path)) == 5           # + chi0 files, TPSC_infos, TPSCparams
+    assert g.write_files(str(tmp_path)) == 8           # + chi0 files, TPSC_infos, TPSCparams
     assert not any(p.startswith("mu0_mu_D") for p in os.listdir(tmp_path))
     g.close()

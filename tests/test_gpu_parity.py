@@ -176,7 +176,9 @@ def test_error_paths():
     with pytest.raises(TPSCError, match="INVALID"):
         g.calc_chijj_vertex_corr_all_optim(chi3=True, mmin=-1)
     with pytest.raises(TPSCError, match="INVALID"):
-        g.calc_chijj_vertex_corr_all_optim(N_theta_k=2)
+        g.calc_chijj_vertex_corr_all_optim(N_theta_k=0)
+    with pytest.raises(TPSCError, match="INVALID"):
+        g.calc_chijj_vertex_corr_all_optim(N_theta_k=65)
     with pytest.raises(TPSCError, match="INVALID"):
         g.calc_chiqw_FFT_spline(32, 64)   # larger than the context grid
     # the lazy chain of cond_opt.cpp:2550-2560 runs the upstream stages itself

@@ -160,7 +160,7 @@ def test_self_real_freq_recovers_rational_self_energies_in_both_forms():
         def R_of(j, k):
             return np.sum(a * e ** 3 * z0[j] / (z0[j] - e))
 
-        sw, ak, sj, aj = post.self_real_freq(ek, nbk, mu, sig_of, np.sum(a), C2, C3, R_of, T, (1, 2), nidx, w, etas)
+        sw, ak, sj, aj = post.self_real_freq(ek, nbk, mu, np.sum(a), C2, C3, R_of, T, (1, 2), nidx, w, etas)
         ekmu = ek[2 + 1 * nbk] - mu
         for i, wi in enumerate(w):
             for q, eta in enumerate(etas):

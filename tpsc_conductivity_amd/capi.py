@@ -18,7 +18,7 @@ FIELDS = {
     "ek": 1, "chi0": 2, "chirt": 3, "chi0_inf": 4, "chi0_inf2": 5, "dtau_chirt0": 6, "V": 7,
     "Sigma_tau": 8, "Self": 9, "Sigma_inf2": 10, "Sigma_inf3": 11, "Sigma_inf4": 12,
     "chijj1": 13, "chijj2": 14, "dtau_Sigma": 15, "chi0_tau": 16, "Sigma_kw": 17,
-    "chijj3": 18, "chijj_all": 19, "al_freq": 20,
+    "chijj3": 18, "chijj_all": 19, "al_freq": 20, "chijj1_k": 21, "chijj2_k": 22, "chijj3_k": 23,
 }
 PREC_FP64 = 0
 PREC_MIXED = 1
@@ -28,7 +28,7 @@ SCALARS = {
     "difRelsp": 8, "difRelch": 9, "Sigma_inf": 10, "trSG": 11, "trSG_corr": 12, "mu": 13,
     "trSG2": 14, "dblOcc2": 15, "kx0": 16, "chijj1_inf2": 17, "chijj2_inf2": 18,
     "chijj2_inf4": 19, "nevals_sp": 20, "nevals_ch": 21, "nevals_mu": 22, "nk": 23, "nk8": 24,
-    "nwb": 25, "nq": 26, "al_N": 27, "al_mmin": 28, "stages": 29,
+    "nwb": 25, "nq": 26, "al_N": 27, "al_mmin": 28, "stages": 29, "n_theta": 30, "chi0_fp64": 31,
 }
 
 
@@ -61,6 +61,8 @@ EXPORTS = [
     "tpsc_get_params", "tpsc_write_files", "tpsc_gwn_local", "tpsc_fix_mu0", "tpsc_sigma_loss_forward",
     "tpsc_sigma_loss_backward", "tpsc_random_weights", "tpsc_sp_corr_length", "tpsc_solve_batch",
     "tpsc_batch_tune", "tpsc_batch_points", "tpsc_set_params", "tpsc_find_zero", "tpsc_sum_rule_noalloc",
+    "tpsc_mu0_infinite", "tpsc_fermi_surface", "tpsc_fs_lattice", "tpsc_mdc", "tpsc_edc", "tpsc_self_real_freq",
+    "tpsc_save_self_bin", "tpsc_save_chi0_bin", "tpsc_read_self_bin", "tpsc_read_chi0_bin",
 ]
 
 # double f(double x, void* user) of tpsc_find_zero
@@ -120,6 +122,16 @@ def lib():
     L.tpsc_batch_points.argtypes = [C.c_ulonglong, C.c_int, C.POINTER(Params)]
     L.tpsc_set_params.argtypes = [vp, C.POINTER(Params)]
     L.tpsc_find_zero.argtypes = [ROOT_FN, vp, dp, dp, dp, ip]
+    L.tpsc_mu0_infinite.argtypes = [vp, dp]
+    L.tpsc_fermi_surface.argtypes = [vp, C.c_int, dp, ip]
+    L.tpsc_fs_lattice.argtypes = [vp, C.c_int, ip, dp]
+    L.tpsc_mdc.argtypes = [vp, ip, C.c_int, C.c_double, dp, C.c_int, C.c_int, dp, dp]
+    L.tpsc_edc.argtypes = [vp, ip, ip, C.c_int, dp, C.c_int, C.c_int, C.c_int, dp, C.c_int, dp]
+    L.tpsc_self_real_freq.argtypes = [vp, ip, ip, C.c_int, dp, C.c_int, dp, C.c_int, dp, dp, dp, dp]
+    L.tpsc_save_self_bin.argtypes = [vp, C.c_char_p]
+    L.tpsc_save_chi0_bin.argtypes = [vp, C.c_char_p]
+    L.tpsc_read_self_bin.argtypes = [C.c_char_p, C.c_int, C.c_int, dp]
+    L.tpsc_read_chi0_bin.argtypes = [C.c_char_p, ip, ip, dp, C.c_size_t]
     # device addresses (chi0(q,iqn), its tails, the value) and the stream as plain pointers
     L.tpsc_sum_rule_noalloc.argtypes = [vp, vp, vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                         C.c_double, C.c_int, C.c_double, vp, vp]
@@ -291,3 +303,30 @@ def batch_tune(pts):
     if rc:
         raise TPSCError(f"tpsc_batch_tune failed ({ERRORS.get(rc, rc)})")
     return w.value, rates
+
+
+def load_Self_bin(path, N, Nw):
+    """green::load_Self_bin (green.cpp:1516-1540): the (Nw/2+1) x nk8 complex array of a self_bin_* file."""
+    nk = N // 2 + 1
+    nk8 = nk * (nk + 1) // 2
+    out = np.zeros((Nw // 2 + 1) * nk8, dtype=np.complex128)
+    rc = lib().tpsc_read_self_bin(str(path).encode(), N, Nw, out.ctypes.data_as(C.POINTER(C.c_double)))
+    if rc:
+        raise TPSCError(f"tpsc_read_self_bin failed ({ERRORS.get(rc, rc)}): {path}")
+    return out.reshape(Nw // 2 + 1, nk8)
+
+
+def loadChi0(path):
+    """chi0::loadChi0 (chi0.cpp:99-156): (nbq, nwmax, chi0 array (nwmax+1, nbq(nbq+1)/2)) of a chi0_* file."""
+    nbq, nwmax = C.c_int(0), C.c_int(0)
+    L = lib()
+    rc = L.tpsc_read_chi0_bin(str(path).encode(), C.byref(nbq), C.byref(nwmax), None, 0)
+    if rc:
+        raise TPSCError(f"tpsc_read_chi0_bin failed ({ERRORS.get(rc, rc)}): {path}")
+    nk8 = nbq.value * (nbq.value + 1) // 2
+    out = np.zeros((nwmax.value + 1) * nk8)
+    rc = L.tpsc_read_chi0_bin(str(path).encode(), C.byref(nbq), C.byref(nwmax), out.ctypes.data_as(C.POINTER(C.c_double)),
+                              out.size)
+    if rc:
+        raise TPSCError(f"tpsc_read_chi0_bin failed ({ERRORS.get(rc, rc)}): {path}")
+    return nbq.value, nwmax.value, out.reshape(nwmax.value + 1, nk8)

@@ -36,6 +36,11 @@ struct ProfPair {
 // [16..18] mu0 search {found, mu0, evaluations}, [19] chi0max, [20] its wedge index (int),
 // [21..22] Tr(Sigma G) main sum and correction, [23] Tr(Sigma G) at the final mu
 constexpr int kHostScalars = 32;
+constexpr int kMaxTheta = 64;  // angle bins of tpsc_chijj (N_theta_k)
+// Mixed path: chi_sp = chi0/(1 - Usp chi0/2) carries the relative error of chi0 multiplied by the
+// spin enhancement 1/(1 - Usp chi0max/2).  FP32 tau arrays leave ~6e-8 in chi0, so above this
+// enhancement the chi0 stage runs in FP64 (measured: 3.9e-7 on chi_sp at 7.2, 1.1e-6 at ~12).
+constexpr double kPromoteEnhancement = 8.0;
 
 }  // namespace
 
@@ -55,6 +60,16 @@ struct tpsc_ctx {
     bool mu_ok = false;
     int nev_sp = 0, nev_ch = 0, nev_mu = 0;
     std::vector<double> chijj1, chijj2;
+    int ntheta = 1;                            // angle bins of the last Lambda_jj call
+    std::vector<double> chijj1_k, chijj2_k;    // angle-binned Lambda_jj1, Lambda_jj2, [b + n*ntheta]
+    std::vector<double> chijj3_k;              // angle-binned Lambda_jj3 on the sparse set
+    int* jjbins = nullptr;                     // angle bin of each Lambda point (ntheta > 1)
+    int jjbins_ntheta = 0;
+    double* jjpart_t = nullptr;                // binned partial rows and their reduction (ntheta > 1)
+    double* jjred_t = nullptr;
+    size_t jjpart_t_cap = 0, jjred_t_cap = 0;
+    double* al_red_t = nullptr;                // binned Lambda_jj3 (ntheta > 1)
+    size_t al_red_t_cap = 0;
     std::vector<long> al_freq;                 // Aslamazov-Larkin sparse boson set
     std::vector<double> chijj3, chijj_all;     // Lambda_jj3 and Lambda_jj1+2+3 on it
     double kx0 = 0.0, jj_inf1 = 0.0, jj_inf2 = 0.0, jj_inf4 = 0.0;
@@ -68,6 +83,11 @@ struct tpsc_ctx {
     TanArrays tan{};             // tangent arrays of the Sigma-loss gradient
     bool grad_ready = false;     // forward of the Sigma loss done, weights on the device
     bool keep_chi_tau = false;
+    bool chi_tau_fp64 = false;   // the kept chi0(q,tau) came from an FP64 chi0 stage
+    // mixed path: the chi0 stage of the full grid runs in FP64 (chi0(q,tau) in chiw, chi0(r,tau) in
+    // chirt64, then narrowed into chirt) once the spin enhancement of the point is found large
+    bool chi0_fp64 = false;
+    double* chirt64 = nullptr;
     // device
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;  // boundary kernels, overlapped with the slab kernels
@@ -404,6 +424,9 @@ int invalidate_stages(tpsc_ctx* c)
     c->hv_slabs_due = c->hv_dropped = false;
     c->chijj1.clear();
     c->chijj2.clear();
+    c->chijj1_k.clear();
+    c->chijj2_k.clear();
+    c->chijj3_k.clear();
     c->chijj3.clear();
     c->chijj_all.clear();
     c->al_freq.clear();
@@ -456,23 +479,34 @@ int stage_chi0(tpsc_ctx* c, int nq0, int nw0, bool defer = false)
     }
     rc = set_grid(c, nq0, nw0);
     if (rc) return rc;
+    // the arrays and precision this stage computes in: FP64 throughout for a promoted full grid
+    // of the mixed path (chi0(q,tau) transformed in place in chiw, chi0(r,tau) in chirt64)
+    const bool f64 = c->p.precision && c->chi0_fp64 && full_grid(c);
+    Grid gx = c->g;
+    DevArrays dx = c->d;
+    if (f64) {
+        if (!c->chirt64 && (rc = dalloc(c, &c->chirt64, (size_t)c->g.nwb * c->g.nk8))) return rc;
+        gx.prec = 0;
+        dx.chi = c->d.chiw;
+        dx.chirt = c->chirt64;
+    }
     // Nothing waits for mu0 on the host.  The tau = 0 slice (one workgroup of 1024 threads)
     // follows the mu0 search in the main stream, so that it takes its CU before the slab kernel
     // on the side stream fills the chip: alone it runs ~240 us at N = 256, behind the slab
     // kernel's workgroups more than twice that, and the chi0 pencils wait for it.
     if ((rc = fork_side(c))) return rc;
     prof_begin(c, "chi0_slab", c->side);
-    HIPCHK(c, launch_chi0_slab(c->g, c->d, c->side));
+    HIPCHK(c, launch_chi0_slab(gx, dx, c->side));
     prof_end(c);
     prof_begin(c, "chi0_tau0");
-    HIPCHK(c, launch_chi0_tau0(c->g, c->d, s));
+    HIPCHK(c, launch_chi0_tau0(gx, dx, s));
     prof_end(c);
     HIPCHK(c, hipEventRecord(c->ev_tau0, s));
     if ((rc = join_side(c))) return rc;
     // h(r,tau) and f0(r,tau) of the Lambda stage need only mu0 and eps(k): on the full grid
     // they are queued now on the prep stream, behind the tau = 0 slice, and run beside the chi0
     // slab and pencils and the sum-rule passes; the Lambda stage waits for ev_hf0
-    if (full_grid(c) && c->g.Nw <= 2048) {
+    if (full_grid(c)) {
         HIPCHK(c, hipStreamWaitEvent(c->prep, c->ev_tau0, 0));
         prof_begin(c, "jj_hf0_slab", c->prep);
         HIPCHK(c, launch_hf0_slab(c->g, c->d, c->prep));
@@ -480,12 +514,15 @@ int stage_chi0(tpsc_ctx* c, int nq0, int nw0, bool defer = false)
         HIPCHK(c, hipEventRecord(c->ev_hf0, c->prep));
         c->hf0_pending = true;
     }
-    if (c->keep_chi_tau && c->chi_tau)
-        HIPCHK(c, hipMemcpyAsync(c->chi_tau, c->d.chi, elem_bytes(c) * c->g.nwb * c->g.nk8, hipMemcpyDeviceToDevice,
-                                 s));
+    if (c->keep_chi_tau && c->chi_tau) {
+        HIPCHK(c, hipMemcpyAsync(c->chi_tau, dx.chi, (f64 ? sizeof(double) : elem_bytes(c)) * c->g.nwb * c->g.nk8,
+                                 hipMemcpyDeviceToDevice, s));
+        c->chi_tau_fp64 = f64;
+    }
     prof_begin(c, "chi0_pencil");
-    HIPCHK(c, launch_chi0_pencil(c->g, c->d, s));
+    HIPCHK(c, launch_chi0_pencil(gx, dx, s));
     prof_end(c);
+    if (f64) HIPCHK(c, launch_narrow(c->chirt64, (float*)c->d.chirt, (long)c->g.nwb * c->g.nk8, s));
     HIPCHK(c, launch_chi0max(c->g, c->d, s));
     HIPCHK(c, hipMemcpyAsync(c->host_scalar + 19, c->d.red, sizeof(double), hipMemcpyDeviceToHost, s));
     HIPCHK(c, hipMemcpyAsync(c->host_scalar + 20, c->d.imax, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -609,6 +646,11 @@ int stage_UspUch(tpsc_ctx* c)
     }
     rc = stage_UspUch_grid(c, c->p.nk0, c->p.Nw0, Usc, &ok);
     if (rc) return rc;
+    if (c->p.precision && !c->chi0_fp64 && ok && 1.0 / (1.0 - Usc[0] * c->chi0max / 2.0) > kPromoteEnhancement) {
+        c->chi0_fp64 = true;  // kept for this point: later solves start in FP64
+        rc = stage_UspUch_grid(c, c->p.nk0, c->p.Nw0, Usc, &ok);
+        if (rc) return rc;
+    }
     c->difRelsp = fabs((Usc[0] - Us0) / Usc[0]);
     c->difRelch = fabs((Usc[1] - Uc0) / Usc[1]);
     c->Usp = Usc[0];
@@ -787,8 +829,56 @@ int stage_mu(tpsc_ctx* c)
     return 0;
 }
 
-// cond_opt::calc_chijj_vertex_corr_all_optim (chi3 = false), cond_opt.cpp:2537-4290
-int stage_chijj(tpsc_ctx* c)
+// a device buffer of at least n doubles, kept by the context (grown when a call needs more)
+int grow(tpsc_ctx* c, double** buf, size_t* cap, size_t n)
+{
+    if (*cap >= n) return 0;
+    if (*buf) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, hipFree(*buf));
+        c->dev_bytes -= *cap * sizeof(double);
+        *buf = nullptr;
+        *cap = 0;
+    }
+    if (hipMalloc((void**)buf, n * sizeof(double)) != hipSuccess) {
+        *buf = nullptr;
+        return fail(c, TPSC_E_NOMEM, "device allocation failed");
+    }
+    *cap = n;
+    c->dev_bytes += n * sizeof(double);
+    return 0;
+}
+
+// the angle bin of every Lambda point (kx index l + 1, ky index m): floor(theta / (pi/(2 ntheta)))
+// with theta = atan((N/2 - m)/(N/2 - l - 1)) (cond_opt.cpp:3174, 3792-3793), on the host
+int angle_bins(tpsc_ctx* c, int ntheta)
+{
+    if (c->jjbins && c->jjbins_ntheta == ntheta) return 0;
+    const Grid& g = c->g;
+    if (!c->jjbins) {
+        if (hipMalloc((void**)&c->jjbins, sizeof(int) * (size_t)std::max<long>(g.nq, 1)) != hipSuccess) {
+            c->jjbins = nullptr;
+            return fail(c, TPSC_E_NOMEM, "device allocation failed");
+        }
+        c->dev_bytes += sizeof(int) * (size_t)std::max<long>(g.nq, 1);
+    }
+    const int nk0 = g.N;
+    const double Dtheta = kPi / (2.0 * ntheta);
+    std::vector<int> b(g.nq);
+    for (long q = 0; q < g.nq; q++) {
+        const int l = (int)(q / g.nk), m = (int)(q % g.nk);
+        const double th = atan((1.0 * (nk0 / 2 - m)) / (nk0 / 2 - l - 1));
+        int it = (int)floor(th / Dtheta);
+        b[q] = std::min(std::max(it, 0), ntheta - 1);
+    }
+    HIPCHK(c, hipMemcpy(c->jjbins, b.data(), sizeof(int) * g.nq, hipMemcpyHostToDevice));
+    c->jjbins_ntheta = ntheta;
+    return 0;
+}
+
+// cond_opt::calc_chijj_vertex_corr_all_optim (chi3 = false), cond_opt.cpp:2537-4290, with the
+// contributions also binned by the angle of k in ntheta bins (cond_opt.cpp:3140-3174, 3792-3847)
+int stage_chijj(tpsc_ctx* c, int ntheta)
 {
     if (!c->have_sigma) {  // cond_opt.cpp:2550-2560
         int rc = tpsc_calc_self_energy_full(c);
@@ -834,23 +924,47 @@ int stage_chijj(tpsc_ctx* c)
         HIPCHK(c, launch_jj_transpose(g, c->d, s));
         prof_end(c);
     }
-    if ((rc = join_side(c))) return rc;
-    prof_begin(c, "jj_pencil");
-    HIPCHK(c, launch_jj_pencil(g, c->d, c->mu, s));
-    prof_end(c);
     const int nblk = (int)((g.nq + jj_points(g.Nw) - 1) / jj_points(g.Nw));
     const int nwb = g.nwb;
-    HIPCHK(c, launch_reduce_cols(c->d.jjpart, nblk, 2 * nwb, c->d.jjred, c->stream));
+    double* part = c->d.jjpart;
+    double* redrows = c->d.jjred;
+    const int* bins = nullptr;
+    if (ntheta > 1) {
+        if ((rc = angle_bins(c, ntheta))) return rc;
+        if ((rc = grow(c, &c->jjpart_t, &c->jjpart_t_cap, (size_t)nblk * 2 * nwb * ntheta))) return rc;
+        if ((rc = grow(c, &c->jjred_t, &c->jjred_t_cap, (size_t)2 * nwb * ntheta))) return rc;
+        part = c->jjpart_t;
+        redrows = c->jjred_t;
+        bins = c->jjbins;
+    }
+    if ((rc = join_side(c))) return rc;
+    prof_begin(c, "jj_pencil");
+    HIPCHK(c, launch_jj_pencil(g, c->d, c->mu, part, bins, ntheta, s));
+    prof_end(c);
+    HIPCHK(c, launch_reduce_cols(part, nblk, 2 * nwb * ntheta, redrows, c->stream));
     HIPCHK(c, launch_reduce_cols(c->d.jjscal, nblk, 4, c->d.jjred + 2 * nwb, c->stream));
-    std::vector<double> red(2 * nwb + 4), rows(2 * g.nk);
-    HIPCHK(c, hipMemcpyAsync(red.data(), c->d.jjred, sizeof(double) * red.size(), hipMemcpyDeviceToHost, c->stream));
+    std::vector<double> red(2 * nwb * ntheta), sc4(4), rows(2 * g.nk);
+    HIPCHK(c, hipMemcpyAsync(red.data(), redrows, sizeof(double) * red.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(sc4.data(), c->d.jjred + 2 * nwb, sizeof(double) * 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipMemcpyAsync(rows.data(), c->d.kx0part, sizeof(double) * rows.size(), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     prof_flush(c);
-    c->chijj1.assign(red.begin(), red.begin() + nwb);
-    c->chijj2.assign(red.begin() + nwb, red.begin() + 2 * nwb);
-    double k0 = red[2 * nwb + 0];
-    const double i1 = red[2 * nwb + 1], i2 = red[2 * nwb + 2], i4 = red[2 * nwb + 3];
+    // red: [b][2][nwb]; the totals add the bins in order
+    c->ntheta = ntheta;
+    c->chijj1.assign(nwb, 0.0);
+    c->chijj2.assign(nwb, 0.0);
+    c->chijj1_k.assign((size_t)nwb * ntheta, 0.0);
+    c->chijj2_k.assign((size_t)nwb * ntheta, 0.0);
+    for (int b = 0; b < ntheta; b++)
+        for (int n = 0; n < nwb; n++) {
+            const double v1 = red[(size_t)b * 2 * nwb + n], v2 = red[(size_t)b * 2 * nwb + nwb + n];
+            c->chijj1_k[b + (size_t)n * ntheta] = v1;
+            c->chijj2_k[b + (size_t)n * ntheta] = v2;
+            c->chijj1[n] += v1;
+            c->chijj2[n] += v2;
+        }
+    double k0 = sc4[0];
+    const double i1 = sc4[1], i2 = sc4[2], i4 = sc4[3];
     for (double v : rows) k0 += v;
     c->kx0 = k0;
     c->jj_inf1 = i1;
@@ -911,7 +1025,7 @@ int al_alloc(tpsc_ctx* c)
 // cond_opt.cpp:4292-6467 (chi3 = true): Lambda_jj3 on the sparse boson set, after the
 // bubble and Maki-Thompson terms.  One launch sequence per boson frequency, no host sync
 // until the fixed-order reduction of the per-k contributions at the end.
-int stage_al(tpsc_ctx* c, int mmin)
+int stage_al(tpsc_ctx* c, int mmin, int ntheta)
 {
     if (!c->have_jj) return fail(c, TPSC_E_ORDER, "Aslamazov-Larkin term needs Lambda_jj1, Lambda_jj2 first");
     if (mmin < 0 || mmin > 30) return fail(c, TPSC_E_INVALID, "mmin must be in 0..30");
@@ -930,12 +1044,21 @@ int stage_al(tpsc_ctx* c, int mmin)
                                  c->al.contrib + (size_t)n * g.nq, s));
         prof_end(c);
     }
-    std::vector<double> r(Ns, 0.0);
+    std::vector<double> r(Ns, 0.0), rk((size_t)Ns * ntheta, 0.0);
     if (Ns > 1) {
         HIPCHK(c, launch_sum_rows(c->al.contrib + g.nq, Ns - 1, (int)g.nq, c->al.red + 1, s));
         HIPCHK(c, hipMemcpyAsync(r.data() + 1, c->al.red + 1, sizeof(double) * (Ns - 1), hipMemcpyDeviceToHost, s));
+        if (ntheta > 1) {  // cond_opt.cpp:6261-6300, 6398
+            if ((rc = angle_bins(c, ntheta))) return rc;
+            if ((rc = grow(c, &c->al_red_t, &c->al_red_t_cap, (size_t)Ns * ntheta))) return rc;
+            HIPCHK(c, launch_sum_rows_binned(c->al.contrib + g.nq, Ns - 1, (int)g.nq, c->jjbins, ntheta,
+                                             c->al_red_t, s));
+            HIPCHK(c, hipMemcpyAsync(rk.data() + ntheta, c->al_red_t, sizeof(double) * (size_t)(Ns - 1) * ntheta,
+                                     hipMemcpyDeviceToHost, s));
+        }
     }
     HIPCHK(c, hipStreamSynchronize(s));
+    c->chijj3_k = (ntheta > 1) ? rk : std::vector<double>();
     prof_flush(c);
     c->al_freq = fr;
     c->al_mmin = mmin;
@@ -1094,6 +1217,9 @@ size_t field_count(tpsc_ctx* c, int field)
         case TPSC_F_CHIJJ3:
         case TPSC_F_CHIJJ_ALL:
         case TPSC_F_AL_FREQ: return c->chijj3.size();
+        case TPSC_F_CHIJJ1_K:
+        case TPSC_F_CHIJJ2_K: return c->chijj1_k.size();
+        case TPSC_F_CHIJJ3_K: return c->chijj3_k.size();
         default: return 0;
     }
 }
@@ -1264,7 +1390,10 @@ int tpsc_create(const tpsc_params* p, tpsc_ctx** out)
     rc |= dalloc(c, &d.imax, 4);
     rc |= dalloc(c, &d.mu0v, 1);
     if (rc) return TPSC_E_NOMEM;
-    HIPCHK(c, hipMemset(d.mu0v, 0, sizeof(double)));  // = c->mu0 before any search
+    // = c->mu0 before any search.  On the context's stream: the streams are non-blocking, so a
+    // memset on the null stream could land after the mu0 search of set_grid has written mu0v
+    // (seen as a batch point solved at mu0 = 0 while other host threads kept the null stream busy)
+    HIPCHK(c, hipMemsetAsync(d.mu0v, 0, sizeof(double), c->stream));
     return set_grid(c, N, Nw);  // hamiltonien + green0 constructors
 }
 
@@ -1280,6 +1409,10 @@ void tpsc_destroy(tpsc_ctx* c)
     }
     for (void* p : c->allocs) (void)hipFree(p);
     if (c->chi_tau) (void)hipFree(c->chi_tau);
+    if (c->jjbins) (void)hipFree(c->jjbins);
+    if (c->jjpart_t) (void)hipFree(c->jjpart_t);
+    if (c->jjred_t) (void)hipFree(c->jjred_t);
+    if (c->al_red_t) (void)hipFree(c->al_red_t);
     if (c->host_scalar) (void)hipHostFree(c->host_scalar);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
@@ -1397,12 +1530,14 @@ int tpsc_find_mu(tpsc_ctx* c, double* mu)
 int tpsc_chijj(tpsc_ctx* c, int chi3, int mmin, int N_theta_k)
 {
     if (!c) return TPSC_E_INVALID;
-    if (N_theta_k != 1) return fail(c, TPSC_E_INVALID, "N_theta_k must be 1");
-    if (c->g.Nw > 2048) return fail(c, TPSC_E_INVALID, "Lambda_jj needs Nw <= 2048");
+    if (N_theta_k < 1 || N_theta_k > kMaxTheta)
+        return fail(c, TPSC_E_INVALID, "N_theta_k must be in 1.." + std::to_string(kMaxTheta));
     if (chi3 && (mmin < 0 || mmin > 30)) return fail(c, TPSC_E_INVALID, "mmin must be in 0..30");
-    int rc = stage_chijj(c);
+    if (chi3 && c->p.Nw0 > 2048)
+        return fail(c, TPSC_E_INVALID, "the Aslamazov-Larkin term needs Nw <= 2048 (its pencils hold 4 tau rows in LDS)");
+    int rc = stage_chijj(c, N_theta_k);
     if (rc || !chi3) return rc;
-    return stage_al(c, mmin);
+    return stage_al(c, mmin, N_theta_k);
 }
 
 int tpsc_solve(tpsc_ctx* c)
@@ -1428,6 +1563,16 @@ int tpsc_get(tpsc_ctx* c, int field, void* dst, size_t bytes)
         for (size_t i = 0; i < n; i++)
             o[i] = (field == TPSC_F_CHIJJ3) ? c->chijj3[i] : (field == TPSC_F_CHIJJ_ALL) ? c->chijj_all[i]
                                                                                              : (double)c->al_freq[i];
+        return 0;
+    }
+    if (field == TPSC_F_CHIJJ1_K || field == TPSC_F_CHIJJ2_K || field == TPSC_F_CHIJJ3_K) {
+        if (n == 0)
+            return fail(c, TPSC_E_ORDER, field == TPSC_F_CHIJJ3_K ? "angle-binned Lambda_jj3 not computed (N_theta_k > 1)"
+                                                                  : "Lambda_jj not computed");
+        if (bytes < n * sizeof(double)) return fail(c, TPSC_E_INVALID, "destination too small");
+        const std::vector<double>& v =
+            (field == TPSC_F_CHIJJ1_K) ? c->chijj1_k : (field == TPSC_F_CHIJJ2_K) ? c->chijj2_k : c->chijj3_k;
+        memcpy(dst, v.data(), n * sizeof(double));
         return 0;
     }
     if (n == 0) return fail(c, TPSC_E_INVALID, "unknown field");
@@ -1463,7 +1608,7 @@ int tpsc_get(tpsc_ctx* c, int field, void* dst, size_t bytes)
         case TPSC_F_CHI0_TAU:
             if (!c->chi_tau) return fail(c, TPSC_E_ORDER, "chi0(q,tau) not kept");
             src = c->chi_tau;
-            stored = true;
+            stored = !c->chi_tau_fp64;
             break;
         default: return fail(c, TPSC_E_INVALID, "unknown field");
     }
@@ -1705,6 +1850,7 @@ int tpsc_set_params(tpsc_ctx* c, const tpsc_params* p)
     c->p = *p;
     c->nsr = (p->n > 1.0) ? 2.0 - p->n : p->n;  // chi.cpp:18-19
     c->mu0_fixed = false;
+    c->chi0_fp64 = false;  // the new point's enhancement decides again
     // a mu0 search of the previous point still in flight is taken (and its outcome dropped)
     int rc = mu0_ready(c);
     if (rc && rc != TPSC_E_NOROOT) return rc;
@@ -1763,6 +1909,8 @@ int tpsc_get_scalar(tpsc_ctx* c, int which, double* v)
                           (c->have_trSG2 ? TPSC_STAGE_TRACE_SG2 : 0) | (c->have_jj ? TPSC_STAGE_CHIJJ : 0) |
                           (c->chijj3.empty() ? 0 : TPSC_STAGE_AL));
             break;
+        case TPSC_S_N_THETA: *v = c->ntheta; break;
+        case TPSC_S_CHI0_FP64: *v = (c->p.precision == TPSC_PREC_FP64 || c->chi0_fp64) ? 1.0 : 0.0; break;
         default: return fail(c, TPSC_E_INVALID, "unknown scalar");
     }
     return 0;

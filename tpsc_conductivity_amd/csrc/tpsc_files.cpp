@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <fstream>
 #include <iomanip>
@@ -79,6 +80,36 @@ extern "C" int tpsc_write_files(tpsc_ctx* c, const char* dir, int* nfiles)
             f << std::setiosflags(std::ios::left) << std::setprecision(15);
             for (int l = 0; l < nk; l++)
                 for (int m = 0; m < nk; m++) f << std::setw(10) << l << std::setw(10) << m << ek[m + l * nk] << std::endl;
+        }
+    }
+    // the green0 constructor's Fermi surfaces: find_FS(10001) (green0.cpp:626-676) and
+    // find_FS_PBC (green0.cpp:51-105), at the finite-lattice mu0
+    {
+        const int nfs = 10001;
+        std::vector<double> kxy(2 * (size_t)nfs);
+        int cnt = 0;
+        std::ofstream f;
+        if (tpsc_fermi_surface(c, nfs, kxy.data(), &cnt) == TPSC_OK &&
+            w.open(f, w.path("./FS_%5.3f_%5.3f_%5.3f_%7.5f_%6.4f_%1d.dat", tp, tpp, t3, density, tem, nfs))) {
+            f << std::setiosflags(std::ios::left) << std::setprecision(10);
+            for (int j = 0; j < cnt; j++) f << std::setw(25) << kxy[2 * j] << kxy[2 * j + 1] << std::endl;
+            f.close();
+        }
+        std::vector<int> mi(nk);
+        std::vector<double> ak(nk);
+        if (tpsc_fs_lattice(c, 0, mi.data(), ak.data()) == TPSC_OK) {
+            const char* forms[2] = {"./FS_discret_AkF_%5.3f_%5.3f_%5.3f_%7.5f_%6.4f_%1d.dat",
+                                    "./FS_discret_%5.3f_%5.3f_%5.3f_%7.5f_%6.4f_%1d.dat"};
+            for (int k = 0; k < 2; k++) {
+                if (!w.open(f, w.path(forms[k], tp, tpp, t3, density, tem, nk0))) continue;
+                f << std::setiosflags(std::ios::left) << std::setprecision(8);
+                for (int l = 0; l < nk; l++) {
+                    f << std::setw(10) << l + mi[l] << std::setw(10) << mi[l];
+                    if (k == 0) f << ak[l];
+                    f << std::endl;
+                }
+                f.close();
+            }
         }
     }
     // chi0::calc_chiqw_FFT_spline with save_chi0_q (chi0.cpp:720-737)
@@ -181,6 +212,60 @@ extern "C" int tpsc_write_files(tpsc_ctx* c, const char* dir, int* nfiles)
                 f << std::setw(10) << j << std::setw(25) << gl[2 * j] << gl[2 * j + 1] << std::endl;
         }
     }
+    // what the reference's main runs after find_mu: find_FS_inter (green.cpp:2210-2341) and
+    // MDC(n = 0, 1, ..., NP = 10, w = 0, eta = {1e-2, 1e-3, 1e-4, 1e-5}, Pade) (tpsc_main.cpp:77-101,
+    // green.cpp:942-1065)
+    if ((stages & TPSC_STAGE_MU) && (stages & TPSC_STAGE_SIGMA)) {
+        std::vector<int> mp(nk), mq(nk);
+        std::vector<double> ap(nk), aq(nk);
+        if (tpsc_fs_lattice(c, 1, mp.data(), ap.data()) == TPSC_OK &&
+            tpsc_fs_lattice(c, 2, mq.data(), aq.data()) == TPSC_OK) {
+            const char* forms[4] = {"./FS_inter_AkF_%4.2f_%5.3f_%5.3f_%5.3f_%7.5f_%6.4f_%1d_%1d.dat",
+                                    "./FS_inter_%4.2f_%5.3f_%5.3f_%5.3f_%7.5f_%6.4f_%1d_%1d.dat",
+                                    "./FS_inter_AkF_Pade_%4.2f_%5.3f_%5.3f_%5.3f_%7.5f_%6.4f_%1d_%1d.dat",
+                                    "./FS_inter_Pade_%4.2f_%5.3f_%5.3f_%5.3f_%7.5f_%6.4f_%1d_%1d.dat"};
+            for (int k = 0; k < 4; k++) {
+                std::ofstream f;
+                if (!w.open(f, w.path(forms[k], U, tp, tpp, t3, density, tem, nk0, nbw))) continue;
+                f << std::setiosflags(std::ios::left) << std::setprecision(8);
+                const std::vector<int>& m = (k < 2) ? mp : mq;
+                const std::vector<double>& a = (k < 2) ? ap : aq;
+                for (int l = 0; l < nk; l++) {
+                    f << std::setw(10) << l + m[l] << std::setw(10) << m[l];
+                    if (k % 2 == 0) f << a[l];
+                    f << std::endl;
+                }
+            }
+        }
+        const int NP = std::min(10, nbw / 2), neta = 4;
+        const double eta_p[4] = {0.01, 0.001, 0.0001, 0.00001}, wr = 0.0;
+        std::vector<int> nq(NP);
+        for (int j = 0; j < NP; j++) nq[j] = j;
+        const long nk8 = (long)nk * (nk + 1) / 2;
+        std::vector<double> sw(2 * (size_t)nk8 * neta), akw((size_t)nk8 * neta);
+        if (NP >= 2 && tpsc_mdc(c, nq.data(), NP, wr, eta_p, neta, 1, sw.data(), akw.data()) == TPSC_OK) {
+            std::ofstream fs, fa;
+            if (w.open(fs, w.path("Self_Pade_k_%4.2f_%5.3f_%5.3f_%5.3f_%7.5f_%6.4f_%5.3e_%1d_%1d.dat", U, tp, tpp, t3,
+                                  density, tem, wr, nk0, nbw)) &&
+                w.open(fa, w.path("Akw_Pade_k_%4.2f_%5.3f_%5.3f_%5.3f_%7.5f_%6.4f_%5.3e_%1d_%1d.dat", U, tp, tpp, t3,
+                                  density, tem, wr, nk0, nbw))) {
+                fs << std::setprecision(12) << std::setiosflags(std::ios::left);
+                fa << std::setprecision(12) << std::setiosflags(std::ios::left);
+                for (int l = 0; l < nk; l++)
+                    for (int m = 0; m <= l; m++) {
+                        const long k = m + (long)l * (l + 1) / 2;
+                        fs << std::setw(10) << l << std::setw(20) << m;
+                        fa << std::setw(10) << l << std::setw(20) << m;
+                        for (int e = 0; e < neta; e++) {
+                            fs << std::setw(20) << sw[2 * (k * neta + e)] << std::setw(25) << sw[2 * (k * neta + e) + 1];
+                            fa << std::setw(20) << akw[k * neta + e];
+                        }
+                        fs << std::endl;
+                        fa << std::endl;
+                    }
+            }
+        }
+    }
     // cond_opt::calc_chijj_vertex_corr_all_optim (cond_opt.cpp:4195-4279)
     if (stages & TPSC_STAGE_CHIJJ) {
         const double kx0 = w.scalar(TPSC_S_KX0);
@@ -209,6 +294,28 @@ extern "C" int tpsc_write_files(tpsc_ctx* c, const char* dir, int* nfiles)
             }
             f.close();
         }
+        // the angle-binned contributions and their sums (cond_opt.cpp:3889-3946)
+        const int nth = (int)w.scalar(TPSC_S_N_THETA);
+        const std::vector<double> k1 = w.field(TPSC_F_CHIJJ1_K), k2 = w.field(TPSC_F_CHIJJ2_K);
+        const char* kforms[4] = {"./chijj1_k_qn_%5.3f_%5.3f_%5.3f_%5.3f_%7.5f_%6.4f_%1d_%1d.dat",
+                                 "./chijj1_k_tot_qn_%5.3f_%5.3f_%5.3f_%5.3f_%7.5f_%6.4f_%1d_%1d.dat",
+                                 "./chijj2_k_qn_%5.3f_%5.3f_%5.3f_%5.3f_%7.5f_%6.4f_%1d_%1d.dat",
+                                 "./chijj2_k_tot_qn_%5.3f_%5.3f_%5.3f_%5.3f_%7.5f_%6.4f_%1d_%1d.dat"};
+        for (int k = 0; k < 4 && nth >= 1 && k1.size() == (size_t)nth * (nbw / 2 + 1); k++) {
+            if (!w.open(f, w.path(kforms[k], U, tp, tpp, t3, density, tem, nk0, nbw))) continue;
+            f << std::setprecision(16) << std::setiosflags(std::ios::left) << std::setiosflags(std::ios::scientific);
+            const std::vector<double>& b = (k < 2) ? k1 : k2;
+            const std::vector<double>& qn = (k < 2) ? j1 : j2;
+            for (int j = 0; j <= nbw / 2; j++) {
+                if (k % 2 == 1) {
+                    double tot = 0.0;
+                    for (int p = 0; p < nth; p++) tot += b[p + (size_t)j * nth];
+                    f << std::setw(10) << j << std::setw(30) << tot << std::setw(30) << qn[j] << tot / qn[j] << '\n';
+                } else
+                    for (int p = 0; p < nth; p++) f << std::setw(10) << j << std::setw(10) << p + 1 << b[p + (size_t)j * nth] << '\n';
+            }
+            f.close();
+        }
     }
     // Aslamazov-Larkin outputs (cond_opt.cpp:6447-6481)
     if (stages & TPSC_STAGE_AL) {
@@ -222,6 +329,24 @@ extern "C" int tpsc_write_files(tpsc_ctx* c, const char* dir, int* nfiles)
             if (!w.open(f, w.path(forms[k], U, tp, tpp, t3, density, tem, nk0, nbw, mmin))) continue;
             f << std::setprecision(16) << std::setiosflags(std::ios::left) << std::setiosflags(std::ios::scientific);
             for (size_t j = 0; j < fr.size(); j++) f << std::setw(10) << (long)fr[j] << (k == 0 ? j3[j] : ja[j]) << '\n';
+        }
+        // with N_theta_k > 1, by angle bin (cond_opt.cpp:6483-6520)
+        const int nth = (int)w.scalar(TPSC_S_N_THETA);
+        const std::vector<double> k3 = (nth > 1) ? w.field(TPSC_F_CHIJJ3_K) : std::vector<double>();
+        const char* kforms[2] = {"./chijj3_k_qn_%5.3f_%5.3f_%5.3f_%5.3f_%7.5f_%6.4f_%1d_%1d_%1d.dat",
+                                 "./chijj3_k_tot_qn_%5.3f_%5.3f_%5.3f_%5.3f_%7.5f_%6.4f_%1d_%1d_%1d.dat"};
+        for (int k = 0; k < 2 && nth > 1 && k3.size() == (size_t)nth * fr.size(); k++) {
+            std::ofstream f;
+            if (!w.open(f, w.path(kforms[k], U, tp, tpp, t3, density, tem, nk0, nbw, mmin))) continue;
+            f << std::setprecision(16) << std::setiosflags(std::ios::left) << std::setiosflags(std::ios::scientific);
+            for (size_t j = 1; j < fr.size(); j++) {
+                if (k == 1) {
+                    double tot = 0.0;
+                    for (int p = 0; p < nth; p++) tot += k3[p + j * nth];
+                    f << std::setw(10) << (long)fr[j] << std::setw(30) << tot << std::setw(30) << j3[j] << tot / j3[j] << '\n';
+                } else
+                    for (int p = 0; p < nth; p++) f << std::setw(10) << (long)fr[j] << std::setw(10) << (p + 1) << k3[p + j * nth] << '\n';
+            }
         }
     }
     if (nfiles) *nfiles = w.count;

@@ -1376,8 +1376,10 @@ __global__ void __launch_bounds__(256) k_transpose(const S* __restrict__ in, S* 
 
 // Per-k Lambda_jj contributions (cond_opt.cpp:3179-3822).  One workgroup of NT threads per
 // Lambda-grid point at a time; workgroup blk handles the points q = blk + kk*nblk,
-// kk < jj_points(L), and accumulates into its own partial rows part[blk*2*nwb + n] (bubble) and
-// part[blk*2*nwb + nwb + n] (Maki-Thompson).
+// kk < jj_points(L), and accumulates into its own partial rows, one pair per angle bin b of the
+// point (bins[q], cond_opt.cpp:3792-3793; bin 0 when bins is null):
+// part[(blk*ntheta + b)*2*nwb + n] (bubble) and part[(blk*ntheta + b)*2*nwb + nwb + n]
+// (Maki-Thompson).
 //
 // The six length-L transforms of the reference run as three complex FFTs of pairs of
 // sequences whose transforms have a known symmetry:
@@ -1398,8 +1400,9 @@ __global__ void __launch_bounds__(256) k_transpose(const S* __restrict__ in, S* 
 #ifndef JJ_WAVES_PER_SIMD
 #define JJ_WAVES_PER_SIMD 2  // two 256-thread workgroups per CU (74 KB LDS each), no register cap
 #endif
+// L = 4096 takes 139 KB of LDS: one workgroup per CU
 template <int L, int NT, typename S, typename SelfT>
-__global__ void __launch_bounds__(NT, JJ_WAVES_PER_SIMD)
+__global__ void __launch_bounds__(NT, (L > 2048 ? 1 : JJ_WAVES_PER_SIMD))
     k_jj_pencil(const double* __restrict__ ek, const double* __restrict__ dek, const double* __restrict__ d2ek,
                 const S* __restrict__ hT, const S* __restrict__ f0T, const double* __restrict__ dhV,
                 const double* __restrict__ df0V, const double* __restrict__ hVb, const double* __restrict__ f0Vb,
@@ -1407,7 +1410,7 @@ __global__ void __launch_bounds__(NT, JJ_WAVES_PER_SIMD)
                 const double* __restrict__ Sinf3, const double* __restrict__ Sinfp, int N, double T, double mu,
                 const cd* __restrict__ tw,
                 const cd* __restrict__ twX, const cd* __restrict__ twS, const cd* __restrict__ twM,
-                const cd* __restrict__ phT, double* part, double* scal)
+                const cd* __restrict__ phT, double* part, double* scal, const int* __restrict__ bins, int ntheta)
 {
     constexpr int H = L / 2;
     constexpr int NWB = H + 1;
@@ -1428,12 +1431,8 @@ __global__ void __launch_bounds__(NT, JJ_WAVES_PER_SIMD)
     const int tid = threadIdx.x;
     const int blk = xcd_remap(blockIdx.x, gridDim.x);
     const long nblk = gridDim.x;
-    double* c1 = part + (long)blk * 2 * NWB;
-    double* c2 = c1 + NWB;
-    for (int n = tid; n < NWB; n += NT) {
-        c1[n] = 0.0;
-        c2[n] = 0.0;
-    }
+    double* const rows = part + (long)blk * 2 * NWB * ntheta;
+    for (int n = tid; n < 2 * NWB * ntheta; n += NT) rows[n] = 0.0;  // the barrier below orders it
     double k0acc = 0.0, inf1 = 0.0, inf2 = 0.0, inf4 = 0.0;
     // The interior samples j = 1..L-1 of the f0V and hV pencils of the next point go from HBM
     // straight into LDS (global_load_lds, no registers) while the current point's last
@@ -1474,6 +1473,8 @@ __global__ void __launch_bounds__(NT, JJ_WAVES_PER_SIMD)
         const double e = ek[m + (l + 1) * nk];
         const double v = dek[q];
         const double wt = (m == 0 || m == nk - 1) ? 2.0 : 4.0;
+        double* const c1 = rows + (long)(bins ? bins[q] : 0) * 2 * NWB;
+        double* const c2 = c1 + NWB;
         // (1) f0V and hV pencils (interior samples already in LDS), their splines, one packed FFT
         if constexpr (kWide) {
             if (tid == 0) {
@@ -1838,6 +1839,7 @@ bool supported_Nw(int Nw)
         case 512: { constexpr int LL = 512; __VA_ARGS__; } break;   \
         case 1024: { constexpr int LL = 1024; __VA_ARGS__; } break; \
         case 2048: { constexpr int LL = 2048; __VA_ARGS__; } break; \
+        case 4096: { constexpr int LL = 4096; __VA_ARGS__; } break; \
         default: return hipErrorInvalidValue;         \
     }
 
@@ -2132,7 +2134,8 @@ hipError_t launch_jj_transpose(const Grid& g, const DevArrays& d, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_jj_pencil(const Grid& g, const DevArrays& d, double mu, hipStream_t s)
+hipError_t launch_jj_pencil(const Grid& g, const DevArrays& d, double mu, double* part, const int* bins, int ntheta,
+                            hipStream_t s)
 {
     const unsigned nblk = (unsigned)((g.nq + jj_points(g.Nw) - 1) / jj_points(g.Nw));
     DISPATCH_S(g.prec, DISPATCH_LJJ(g.Nw, {
@@ -2140,8 +2143,44 @@ hipError_t launch_jj_pencil(const Grid& g, const DevArrays& d, double mu, hipStr
         hipLaunchKernelGGL((k_jj_pencil<LL, NTJ, SS, ST>), dim3(nblk), dim3(NTJ), 0, s, d.ek, d.dek,
                            d.d2ek, (const SS*)d.hT, (const SS*)d.f0T, d.dhV, d.df0V, d.hVb, d.f0Vb,
                            (const JJPoles*)d.poles, (const ST*)d.Self, d.Sinf2, d.Sinf3, d.Sinf, g.N, g.T, mu, d.twN,
-                           d.twX, d.twS, d.twM, d.ph, d.jjpart, d.jjscal);
+                           d.twX, d.twS, d.twM, d.ph, part, d.jjscal, bins, ntheta);
     }));
+    return hipGetLastError();
+}
+
+// out[i] = (float)in[i]
+__global__ void k_narrow(const double* __restrict__ in, float* __restrict__ out, long n)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        out[i] = (float)in[i];
+}
+
+hipError_t launch_narrow(const double* in, float* out, long n, hipStream_t s)
+{
+    const long nb = std::min<long>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_narrow, dim3((unsigned)std::max<long>(nb, 1)), dim3(256), 0, s, in, out, n);
+    return hipGetLastError();
+}
+
+// out[r*ntheta + b] = sum over k < n with bins[k] == b of part[r*n + k], in increasing k
+__global__ void __launch_bounds__(256) k_sum_rows_binned(const double* __restrict__ part, int n,
+                                                         const int* __restrict__ bins, int ntheta, double* out)
+{
+    __shared__ double sc[16];
+    const int r = blockIdx.x, b = blockIdx.y;
+    const double* row = part + (long)r * n;
+    double a = 0.0;
+    for (int k = threadIdx.x; k < n; k += 256)
+        if (bins[k] == b) a += row[k];
+    a = block_sum(a, sc);
+    if (threadIdx.x == 0) out[(long)r * ntheta + b] = a;
+}
+
+hipError_t launch_sum_rows_binned(const double* part, int nrows, int n, const int* bins, int ntheta, double* out,
+                                  hipStream_t s)
+{
+    if (nrows <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_sum_rows_binned, dim3(nrows, ntheta), dim3(256), 0, s, part, n, bins, ntheta, out);
     return hipGetLastError();
 }
 

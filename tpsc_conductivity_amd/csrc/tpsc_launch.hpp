@@ -176,7 +176,14 @@ hipError_t launch_hf0_bnd(const Grid& g, const DevArrays& d, double mu0, double 
 hipError_t launch_hVf0V_slab(const Grid& g, const DevArrays& d, hipStream_t s);
 hipError_t launch_jj_poles(const Grid& g, const DevArrays& d, double mu, hipStream_t s);
 hipError_t launch_jj_transpose(const Grid& g, const DevArrays& d, hipStream_t s);
-hipError_t launch_jj_pencil(const Grid& g, const DevArrays& d, double mu, hipStream_t s);
+// partial rows to part (nblk x ntheta x 2*nwb); bins: the angle bin of each Lambda point, or null
+hipError_t launch_jj_pencil(const Grid& g, const DevArrays& d, double mu, double* part, const int* bins, int ntheta,
+                            hipStream_t s);
+// out[i] = (float)in[i], i < n
+hipError_t launch_narrow(const double* in, float* out, long n, hipStream_t s);
+// out[r*ntheta + b]: the sum of row r of part (nrows x n) over the columns k with bins[k] == b
+hipError_t launch_sum_rows_binned(const double* part, int nrows, int n, const int* bins, int ntheta, double* out,
+                                  hipStream_t s);
 hipError_t launch_kx0_rows(const Grid& g, const DevArrays& d, double mu, hipStream_t s);
 hipError_t launch_finalize(const double* part, int n, double* out, hipStream_t s);
 

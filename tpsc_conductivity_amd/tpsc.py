@@ -158,6 +158,103 @@ class CondOpt:
                        "tpsc_sigma_loss_backward")
         return g
 
+    # ---- post-processing (host FP64 over the device results, tpsc_main.cpp:77-101)
+    def find_mu_inf_syst(self):
+        """green0::find_mu_inf_syst (green0.cpp:266-287): mu0 of the infinite lattice."""
+        v = C.c_double(0.0)
+        self.ctx.check(lib().tpsc_mu0_infinite(self.ctx.handle, C.byref(v)), "find_mu_inf_syst")
+        return v.value
+
+    def find_FS(self, nk=10001):
+        """green0::find_FS (green0.cpp:626-676): (kx, ky) on the non-interacting Fermi surface along
+        kx - ky = j pi/(nk-1), as an (count, 2) array."""
+        import numpy as np
+
+        out = np.zeros(2 * nk)
+        cnt = C.c_int(0)
+        self.ctx.check(lib().tpsc_fermi_surface(self.ctx.handle, nk, out.ctypes.data_as(C.POINTER(C.c_double)),
+                                                C.byref(cnt)), "find_FS")
+        return out[: 2 * cnt.value].reshape(-1, 2)
+
+    def _fs_lattice(self, kind, what):
+        import numpy as np
+
+        nk = self.nk0 // 2 + 1
+        m = np.zeros(nk, dtype=np.int32)
+        a = np.zeros(nk)
+        self.ctx.check(lib().tpsc_fs_lattice(self.ctx.handle, kind, m.ctypes.data_as(C.POINTER(C.c_int)),
+                                             a.ctypes.data_as(C.POINTER(C.c_double))), what)
+        return m, a
+
+    def find_FS_PBC(self):
+        """green0::find_FS_PBC (green0.cpp:51-105): per diagonal l, the m of largest A0(k, 0) and that A0."""
+        return self._fs_lattice(0, "find_FS_PBC")
+
+    def find_FS_inter(self):
+        """green::find_FS_inter (green.cpp:2210-2341): ((m, A) from the 3-point polynomial, (m, A) from the
+        4-point Pade) per diagonal l."""
+        return self._fs_lattice(1, "find_FS_inter"), self._fs_lattice(2, "find_FS_inter")
+
+    def MDC(self, n, NP, w, eta_p, use_Pade=True):
+        """green::MDC (green.cpp:814-1068): (Sigma(k, w + i eta) [nk8, Neta] complex, A [nk8, Neta])."""
+        import numpy as np
+
+        nn = np.ascontiguousarray(n[:max(NP, 2)], dtype=np.int32)
+        eta = np.ascontiguousarray(eta_p, dtype=np.float64)
+        nk8 = int(self.scalar("nk8"))
+        sw = np.zeros((nk8, len(eta)), dtype=np.complex128)
+        ak = np.zeros((nk8, len(eta)))
+        dp = C.POINTER(C.c_double)
+        self.ctx.check(lib().tpsc_mdc(self.ctx.handle, nn.ctypes.data_as(C.POINTER(C.c_int)), NP, w,
+                                      eta.ctypes.data_as(dp), len(eta), int(bool(use_Pade)), sw.ctypes.data_as(dp),
+                                      ak.ctypes.data_as(dp)), "MDC")
+        return sw, ak
+
+    def EDC(self, kx, ky, w, m_freq, NP_max, eta_p):
+        """green::EDC(kx, ky, Nk, w, nw0, m_freq, NP_max, eta, N_eta) (green.cpp:589-700): the spectral
+        weight [point, w, eta]."""
+        import numpy as np
+
+        kx = np.ascontiguousarray(kx, dtype=np.int32)
+        ky = np.ascontiguousarray(ky, dtype=np.int32)
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        eta = np.ascontiguousarray(eta_p, dtype=np.float64)
+        out = np.zeros((len(kx), len(w), len(eta)))
+        ip, dp = C.POINTER(C.c_int), C.POINTER(C.c_double)
+        self.ctx.check(lib().tpsc_edc(self.ctx.handle, kx.ctypes.data_as(ip), ky.ctypes.data_as(ip), len(kx),
+                                      w.ctypes.data_as(dp), len(w), m_freq, NP_max, eta.ctypes.data_as(dp), len(eta),
+                                      out.ctypes.data_as(dp)), "EDC")
+        return out
+
+    def calc_Self_Re_w(self, k, n, w, eta_p):
+        """green::calc_Self_Re_w (green.cpp:408-551): (Sigma, A, Sigma_Jfrac, A_Jfrac), each [w, eta]."""
+        import numpy as np
+
+        kk = np.ascontiguousarray(k, dtype=np.int32)
+        nn = np.ascontiguousarray(n, dtype=np.int32)
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        eta = np.ascontiguousarray(eta_p, dtype=np.float64)
+        s1 = np.zeros((len(w), len(eta)), dtype=np.complex128)
+        s2 = np.zeros_like(s1)
+        a1 = np.zeros((len(w), len(eta)))
+        a2 = np.zeros_like(a1)
+        ip, dp = C.POINTER(C.c_int), C.POINTER(C.c_double)
+        self.ctx.check(lib().tpsc_self_real_freq(self.ctx.handle, kk.ctypes.data_as(ip), nn.ctypes.data_as(ip),
+                                                 len(nn), w.ctypes.data_as(dp), len(w), eta.ctypes.data_as(dp),
+                                                 len(eta), s1.ctypes.data_as(dp), a1.ctypes.data_as(dp),
+                                                 s2.ctypes.data_as(dp), a2.ctypes.data_as(dp)), "calc_Self_Re_w")
+        return s1, a1, s2, a2
+
+    def save_Self_bin(self, directory=None):
+        """green::save_Self_bin (green.cpp:1496-1514), FP64 path."""
+        self.ctx.check(lib().tpsc_save_self_bin(self.ctx.handle, directory.encode() if directory else None),
+                       "save_Self_bin")
+
+    def save_chi(self, directory=None):
+        """chi0::save_chi (chi0.cpp:61-96)."""
+        self.ctx.check(lib().tpsc_save_chi0_bin(self.ctx.handle, directory.encode() if directory else None),
+                       "save_chi")
+
     def write_files(self, directory=None):
         """Write the reference's text outputs (TPSCparams_*, TPSC_infos_*, chijj*_qn_*, kx0_*, ...)
         for the stages run so far into directory (default: the current directory)."""
