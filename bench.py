@@ -27,6 +27,7 @@ METRIC = "TPSC Σ/χ/Λjj solves/s at 256²×2048 iωn: mixed vs FP64 GPU at 1e-
 WORKLOAD = dict(N=256, Nw=2048, n=0.9, T=0.05, U=5.0, tp=-0.2, tpp=0.1, t3=0.0)
 TOL = 1e-6
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
+FLOPS_FILE = os.path.join(ROOT, "profiles", "flops.json")
 
 
 def cpu_baseline(cfg, repeats=3):
@@ -243,13 +244,25 @@ def main():
     d = per[dominant]
     slab_ms = sum(table[k][0] for k in roofline.SLAB_KERNELS if k in table)
     slab_b = sum(kb[k] * table[k][1] for k in roofline.SLAB_KERNELS if k in table)
-    traffic = None
+    key = f"{prec}:{dominant}:{cfg['N']}x{cfg['Nw']}"
+    traffic, valu = None, None
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
             tr = json.load(f)
-        key = f"{prec}:{dominant}:{cfg['N']}x{cfg['Nw']}"
         if key in tr:
             traffic = tr[key]["bytes_per_launch"]
+    if os.path.exists(FLOPS_FILE):
+        with open(FLOPS_FILE) as f:
+            fl = json.load(f)
+        if key in fl:
+            # FP64 work of the same kernel: its counted VALU FP64 instructions (per launch, a
+            # fixed figure for this workload) over the launch time measured here
+            f64 = fl[key]["fp64_flops_per_launch"]
+            tf = f64 / (d["avg_ms"] * 1e-3) / 1e12
+            valu = {"bound": "fp64 valu", "kernel": dominant, "achieved": tf, "peak": roofline.FP64_PEAK_TFLOPS,
+                    "unit": "TFLOP/s", "frac": tf / roofline.FP64_PEAK_TFLOPS, "fp64_flops_per_launch": f64,
+                    "flops_source": "64 x (ADD + MUL + TRANS + 2 FMA) of SQ_INSTS_VALU_*_F64 per launch, "
+                                    "profiles/flops.json"}
     out = {
         "metric": METRIC,
         "value": 1e3 / ms,
@@ -270,6 +283,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": d["GBps"], "peak": roofline.HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": d["GBps"] / roofline.HBM_PEAK_GBPS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": kb[dominant], "avg_launch_ms": d["avg_ms"]},
+        "valu_roofline": valu,
         "slab_passes": {"kernels": list(roofline.SLAB_KERNELS), "ms": slab_ms,
                         "GBps": slab_b / (slab_ms * 1e-3) / 1e9 if slab_ms > 0 else None,
                         "frac": slab_b / (slab_ms * 1e-3) / 1e9 / roofline.HBM_PEAK_GBPS if slab_ms > 0 else None},

@@ -97,6 +97,8 @@ def lib():
         L.orc_chijj.argtypes = [P, C.c_int]
         L.orc_chijj.restype = C.c_int
         L.orc_fix_mu0.argtypes = [P, C.c_double]
+        L.orc_solve_from_roots.argtypes = [P, C.c_double, C.c_double]
+        L.orc_solve_from_roots.restype = C.c_int
         L.orc_hot_spot_vF.argtypes = [P]
         L.orc_hot_spot_vF.restype = C.c_double
         L.orc_sp_corr_from.argtypes = [_dp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, _dp]
@@ -255,6 +257,11 @@ class Oracle:
 
     def solve(self):
         return lib().orc_solve(self._o)
+
+    def solve_from_roots(self, Usp, Uch=float("nan")):
+        """After solve(): the chain again from the given Usp and Uch (Uch NaN: its root searched
+        at that Usp), then vertex, self-energy, mu and Lambda_jj, on the same chi0."""
+        return lib().orc_solve_from_roots(self._o, Usp, Uch)
 
     def solve_timed(self):
         """A full solve without the coarse pass; (ok, seconds of the stages A mu0 + chi0, B roots,

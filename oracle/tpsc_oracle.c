@@ -539,6 +539,17 @@ double orc_sum_rule(orc *o, int channel, double Ux)
 static double fz_sp(void *ctx, double x) { return orc_sum_rule((orc *)ctx, 0, x); }
 static double fz_ch(void *ctx, double x) { return orc_sum_rule((orc *)ctx, 1, x); }
 
+/* the Uch root on the chi0 already computed, given o->Usp (the charge sum rule reads Usp
+   through the double occupancy; chi.cpp:866-869) */
+static int orc_uch_root(orc *o, double *Uch)
+{
+    double root = 0.0;
+    const double init[3] = {0.0, 0.0, 1000.0 * o->p.U};
+    if (!orc_find_zero(fz_ch, o, init, &root, NULL, &o->nevals_ch)) return 0;
+    *Uch = root;
+    return 1;
+}
+
 /* chi.cpp:846-876 */
 /* the Usp and Uch roots on the chi0 already computed (chi.cpp:853-869) */
 static int orc_usp_uch_roots(orc *o, double Usc[2])
@@ -552,13 +563,7 @@ static int orc_usp_uch_roots(orc *o, double Usc[2])
         o->Usp = root;
     } else
         ok = 0;
-    root = 0.0;
-    init[0] = 0;
-    init[2] = 1000.0 * U;
-    if (orc_find_zero(fz_ch, o, init, &root, NULL, &o->nevals_ch))
-        Usc[1] = root;
-    else
-        ok = 0;
+    if (!orc_uch_root(o, &Usc[1])) ok = 0;
     return ok;
 }
 
@@ -1448,6 +1453,30 @@ int orc_solve_timed(orc *o, double t[6])
     t[4] = w1 - w0;
     orc_chijj(o, o->N_theta > 0 ? o->N_theta : 1);
     t[5] = omp_get_wtime() - w1;
+    return ok;
+}
+
+/* The chain after the sum rules from given roots, on the chi0 of a solve already done: Usp as
+   given, Uch as given or, if NaN, by the charge-rule search (chi.cpp:866-869), the double
+   occupancy (chi.cpp:370-371), then vertex, self-energy, traceSelfG, mu and Lambda_jj as
+   orc_solve runs them.  For checking a solve whose roots are not the ones orc_find_Usp_Uch
+   returns, such as where that search stops short of a root. */
+int orc_solve_from_roots(orc *o, double Usp, double Uch)
+{
+    o->Usp = Usp;
+    int ok = 1;
+    if (isnan(Uch)) ok = orc_uch_root(o, &Uch);
+    o->Uch = Uch;
+    const double U = o->p.U, nsr = o->nsr;
+    if (o->p.n > 1.0)
+        o->dblOcc = o->Usp / U * nsr * nsr / 4.0 + 1.0 - nsr;
+    else
+        o->dblOcc = o->Usp / U * nsr * nsr / 4.0;
+    orc_vertex(o);
+    orc_self_energy(o);
+    orc_trace_self_G(o);
+    ok &= orc_find_mu(o);
+    orc_chijj(o, o->N_theta > 0 ? o->N_theta : 1);
     return ok;
 }
 

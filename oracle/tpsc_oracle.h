@@ -104,6 +104,7 @@ int orc_trace_self_G2(orc *o);                          /* green.cpp:2929-2997 *
 int orc_chijj(orc *o, int N_theta);                     /* cond_opt.cpp:2537-4290 */
 int orc_solve(orc *o);                                  /* tpsc_main.cpp:61-75 + E4 */
 int orc_solve_timed(orc *o, double t[6]);               /* the same without the coarse pass, stage times */
+int orc_solve_from_roots(orc *o, double Usp, double Uch); /* the chain after the roots (Uch NaN: searched) */
 int orc_omp_max_threads(void);
 int orc_gwn_local(orc *o, double complex *out);          /* green.cpp:209-266 */
 void orc_fix_mu0(orc *o, double mu0);                    /* mu0 as an input parameter   */

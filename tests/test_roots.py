@@ -93,3 +93,55 @@ def test_same_roots_as_reference_search(oracle_mod):
     ok, r, _ = capi.find_zero(d, [0.0, -10.0, 10.0], lims=[-20.0, 20.0])
     ok2, r2, _ = oracle_mod.find_zero(d, [0.0, -10.0, 10.0], lims=[-20.0, 20.0])
     assert ok and ok2 and abs(r - r2) < 1e-12 * max(1.0, abs(r2))
+
+
+# batch point 41 of tpsc_batch_points(64064, 64): 1 - Usp chi0max/2 ~ 3e-5, the spin sum rule
+# rises by ~5.7e3 per unit of U just below its pole at 2/chi0max, the charge sum rule falls by
+# only 1.6e-3 per unit of U at Uch ~ 45
+NEAR_POLE = dict(N=128, Nw=128, n=0.9690175914152073, T=0.0672362960301115, U=4.676953119127109,
+                 tp=-0.040868229371670894)
+
+
+def newton_step(f, x):
+    h = 1e-6 * abs(x)
+    return f(x) / ((f(x + h) - f(x - h)) / (2 * h))
+
+
+def test_reference_search_stops_short_near_the_instability(oracle_mod):
+    """At NEAR_POLE the reference's search (generique.cpp:604-913: 20 iterations, then the point
+    of smallest |f| kept) ends 2.3e-7 below the Usp root (|f| ~ 1.8e-3), and, at the converged
+    Usp, one Newton step (2.4e-10 relative) short of the Uch root; the library's search
+    converges on both.  Device results there are checked against the oracle's chain rerun from
+    the converged roots (orc_solve_from_roots)."""
+    p = NEAR_POLE
+    o = oracle_mod.Oracle(p["N"], p["Nw"], p["n"], p["T"], p["U"], tp=p["tp"], coarse_pass=False)
+    assert o.solve()
+    s = o.scalars()
+    f = lambda u: o.sum_rule(0, u)
+    assert s["nevals_sp"] == 40 and abs(f(s["Usp"])) > 1e-3
+    init = [p["U"] / 2, 0.0, 2.0 / s["chi0max"] - 2 * np.finfo(float).eps]
+    ok, usp, _ = capi.find_zero(f, init)
+    assert ok and abs(newton_step(f, usp)) < 1e-14 * usp and 2e-7 < (usp - s["Usp"]) / usp < 3e-7
+    assert o.solve_from_roots(usp)  # Uch searched the reference's way at the converged Usp
+    g = lambda u: o.sum_rule(1, u)
+    uch_ref = o.scalars()["Uch"]
+    assert 1e-10 < abs(newton_step(g, uch_ref)) / uch_ref < 1e-9
+    ok, uch, _ = capi.find_zero(g, [0.0, 0.0, 1000.0 * p["U"]])
+    assert ok and abs(newton_step(g, uch)) < 1e-13 * uch
+
+
+def test_chain_from_given_roots_reproduces_the_solve(oracle_mod):
+    o = oracle_mod.Oracle(32, 128, 0.9, 0.1, 4.0, tp=-0.175, coarse_pass=False)
+    assert o.solve()
+    s0, j0 = o.scalars(), o.get("chijj1") + o.get("chijj2")
+    for args in ((s0["Usp"],), (s0["Usp"], s0["Uch"])):
+        assert o.solve_from_roots(*args)
+        s1, j1 = o.scalars(), o.get("chijj1") + o.get("chijj2")
+        for k in ("Usp", "Uch", "dblOcc", "mu", "kx0"):
+            assert s1[k] == s0[k], (k, args)
+        assert np.array_equal(j0, j1)
+    # a different Usp moves every later stage, Uch included when it is searched
+    assert o.solve_from_roots(s0["Usp"] * (1 + 1e-6))
+    assert o.scalars()["Uch"] != s0["Uch"] and o.scalars()["mu"] != s0["mu"]
+    assert o.solve_from_roots(s0["Usp"], s0["Uch"] * (1 + 1e-6))
+    assert o.scalars()["Uch"] == s0["Uch"] * (1 + 1e-6) and o.scalars()["mu"] != s0["mu"]

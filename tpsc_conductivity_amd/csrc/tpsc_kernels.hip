@@ -767,8 +767,11 @@ __global__ void __launch_bounds__(SLAB_THREADS, SlabOcc<S>::waves)
 
 // green.cpp:1925-2018: per wedge k, spline of Sigma(k,tau_j) (j=0..Nw) with slopes dSig,
 // F = BACKWARD DFT[exp(i pi j/Nw) 6a_j], R(k,n) = (1 - exp(i pi (2n+1)/Nw)) F_n
-template <int L, typename S, typename SelfT>
-__global__ void __launch_bounds__(PEN_THREADS)
+// threads of the self-energy pencil: 512 for the long transforms (at Nw = 2048 the 70 KB
+// workgroup spends less of its time in the serial parts of the spline solve than at 256)
+constexpr int sigma_pen_threads(int L) { return L >= 1024 ? 512 : PEN_THREADS; }
+template <int L, typename S, typename SelfT, int NT = sigma_pen_threads(L)>
+__global__ void __launch_bounds__(NT)
     k_sigma_pencil(const S* __restrict__ Stau, const double* __restrict__ Sb, const double* __restrict__ dSig,
                    long nk8, double T, SelfT* Self,
                    double* Sinf2, double* Sinf3, double* Sinf4, const double* __restrict__ Sinfp,
@@ -776,7 +779,7 @@ __global__ void __launch_bounds__(PEN_THREADS)
 {
     __shared__ double y[L + 1], M[L + 1];
     __shared__ cd X[L];
-    __shared__ double ES[2 * PEN_THREADS];
+    __shared__ double ES[2 * NT];
     const long k = xcd_remap(blockIdx.x, gridDim.x);
     if (k >= nk8) return;
     const int tid = threadIdx.x, nt = blockDim.x;
@@ -786,13 +789,13 @@ __global__ void __launch_bounds__(PEN_THREADS)
     const double d0 = dSig[k], dN = dSig[k + nk8];
     const double h = 1.0 / (L * T);
     __syncthreads();
-    block_spline_moments<L + 1, PEN_THREADS>(y, M, rr, L, h, d0, dN, ES);
+    block_spline_moments<L + 1, NT>(y, M, rr, L, h, d0, dN, ES);
     for (int j = tid; j < L; j += nt) {
         const double a6 = (M[j + 1] - M[j]) / h;
         X[j] = twX[j] * a6;
     }
     __syncthreads();
-    block_fft<L, PEN_THREADS>(X, +1, tw);
+    block_fft<L, NT>(X, +1, tw);
     const double s2 = d0 + dN, s3 = -(M[0] + M[L]), Si = Sinfp[0];
     for (int n = tid; n < L / 2; n += nt) {
         const cd R = twS[n] * X[n];
@@ -2035,7 +2038,7 @@ hipError_t launch_sigma_pencil(const Grid& g, const DevArrays& d, hipStream_t s)
 {
     hipLaunchKernelGGL(k_sigma_inf, dim3(1), dim3(64), 0, s, d.Sb, g.nk8, d.Sinf);
     DISPATCH_S(g.prec, DISPATCH_L(g.Nw, {
-        hipLaunchKernelGGL((k_sigma_pencil<LL, SS, ST>), dim3((unsigned)g.nk8), dim3(PEN_THREADS), 0, s,
+        hipLaunchKernelGGL((k_sigma_pencil<LL, SS, ST>), dim3((unsigned)g.nk8), dim3(sigma_pen_threads(LL)), 0, s,
                            (const SS*)d.Stau, d.Sb, d.dSig, g.nk8, g.T, (ST*)d.Self, d.Sinf2, d.Sinf3, d.Sinf4, d.Sinf,
                            d.twN, d.twX, d.twS);
     }));

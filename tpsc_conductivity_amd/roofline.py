@@ -10,6 +10,7 @@ both paths.
 """
 
 HBM_PEAK_GBPS = 8000.0  # MI355X HBM3E
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 (vector and matrix)
 
 SLAB_KERNELS = ("chi0_slab", "vertex_slab", "sigma_slab", "jj_hf0_slab", "jj_hVf0V_slab")
 
