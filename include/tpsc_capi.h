@@ -94,9 +94,16 @@ enum tpsc_field {
     TPSC_F_CHIJJ1_K = 21, /* Lambda_jj1 by angle of k, nwb * N_theta_k values [b + n*N_theta_k]
                                                         (cond_opt.cpp:3140-3174, 3792-3847) */
     TPSC_F_CHIJJ2_K = 22, /* Lambda_jj2 by angle of k, the same layout  (cond_opt.cpp:3799-3809) */
-    TPSC_F_CHIJJ3_K = 23  /* Lambda_jj3 by angle of k on the sparse set, al_N * N_theta_k values
+    TPSC_F_CHIJJ3_K = 23, /* Lambda_jj3 by angle of k on the sparse set, al_N * N_theta_k values
                              [b + j*N_theta_k], entry j = 0 zero; only when N_theta_k > 1
                                                                         (cond_opt.cpp:6261-6398) */
+    /* tangents of the last tpsc_loss_backward (no reference counterpart), parameters p = t, t',
+     * mu0, U: */
+    TPSC_F_GRAD_TERMS = 24,   /* 16 values [p*4 + k]: d/dp of loss term k (Re Sigma, Im Sigma,
+                                 chi_sp(pi,pi), Lambda_jj; 0 for terms not in the loss) */
+    TPSC_F_GRAD_DMU = 25,     /* 4 values: dmu/dp of the interacting mu (Lambda_jj term only) */
+    TPSC_F_GRAD_DLAMBDA = 26  /* 4*nwb values [p*nwb + n]: d/dp [Lambda_jj1 + Lambda_jj2](iq_n)
+                                 (Lambda_jj term only) */
 };
 enum tpsc_scalar {
     TPSC_S_MU0 = 1, TPSC_S_CHI0MAX, TPSC_S_QMAX_L, TPSC_S_QMAX_M, TPSC_S_USP, TPSC_S_UCH,
@@ -204,13 +211,25 @@ int tpsc_sp_corr_length(tpsc_ctx* ctx, double* out);
  * No reference counterpart: it makes d/dmu0 a partial derivative at fixed n. */
 int tpsc_fix_mu0(tpsc_ctx* ctx, double mu0);
 
-/* Gradient of L = sum over the packed wedge of w_k Re Sigma(k, i w_0) (FP64 path only) with
- * respect to (t, t', mu0, U), as a forward / backward pair.  No reference counterpart.
- * forward: runs calc_Self_FFT_spline (chi0 -> Usp, Uch -> V -> Sigma, green.cpp:1673-2031) at the
- *   context's parameters (mu0 fixed if tpsc_fix_mu0 was called) and returns L; w: nk8 weights.
+/* Gradient of a loss L(t, t', mu0, U) built from the solve's outputs (FP64 path only), as a
+ * forward / backward pair.  No reference counterpart.  L is the sum of the terms in `terms`
+ * (bitwise or of TPSC_LOSS_*); w: nk8 weights over the packed wedge (the Sigma terms).
+ * forward: runs the chain at the context's parameters (mu0 fixed if tpsc_fix_mu0 was called):
+ *   calc_Self_FFT_spline (chi0 -> Usp, Uch -> V -> Sigma, green.cpp:1673-2031), and for the
+ *   Lambda_jj term traceSelfG, find_mu and calc_chijj_vertex_corr_all_optim (tpsc_main.cpp:61-75,
+ *   243); returns L.
  * backward: grad[4] = dloss * dL/d(t, t', mu0, U) by tangent-linear propagation through the
- *   same chain: one pass per parameter, the tau slabs recomputed instead of stored, the
- *   Usp / Uch roots differentiated implicitly through their sum rules (chi.cpp:833-843). */
+ *   same chain: one pass per parameter, the tau slabs recomputed instead of stored, the Usp / Uch
+ *   roots differentiated implicitly through their sum rules (chi.cpp:833-843) and mu through
+ *   Tr(G - G0)(mu) = 0 (green.cpp:3185-3282).  The Lambda_jj term needs Nw <= 1024.
+ * The per-term derivatives, dmu/dp and d Lambda_jj(iq_n)/dp are in TPSC_F_GRAD_*. */
+#define TPSC_LOSS_RE_SIGMA 1 /* sum_k w_k Re Sigma(k, i pi T)                                   */
+#define TPSC_LOSS_IM_SIGMA 2 /* sum_k w_k Im Sigma(k, i pi T)                                   */
+#define TPSC_LOSS_CHISP_Q 4  /* chi_sp(q = (pi, pi), iq_0) = chi0/(1 - Usp chi0/2)              */
+#define TPSC_LOSS_LAMBDA 8   /* sum_{n=0}^{8} [Lambda_jj1 + Lambda_jj2](iq_n), bubble + MT      */
+int tpsc_loss_forward(tpsc_ctx* ctx, const double* w, int terms, double* loss);
+int tpsc_loss_backward(tpsc_ctx* ctx, double dloss, double* grad);
+/* the same pair for L = sum_k w_k Re Sigma(k, i pi T) (terms = TPSC_LOSS_RE_SIGMA) */
 int tpsc_sigma_loss_forward(tpsc_ctx* ctx, const double* w, double* loss);
 int tpsc_sigma_loss_backward(tpsc_ctx* ctx, double dloss, double* grad);
 

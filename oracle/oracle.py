@@ -106,6 +106,8 @@ def lib():
         L.orc_sp_corr_length.restype = C.c_int
         L.orc_sigma_loss.argtypes = [P, _dp]
         L.orc_sigma_loss.restype = C.c_double
+        L.orc_loss.argtypes = [P, _dp, C.c_int]
+        L.orc_loss.restype = C.c_double
         L.orc_gwn_local.argtypes = [P, _dp]
         L.orc_gwn_local.restype = C.c_int
         L.orc_chijj_al.argtypes = [P, C.c_int, C.c_int]
@@ -238,6 +240,15 @@ class Oracle:
         w = np.ascontiguousarray(w, dtype=np.float64)
         assert w.size == self.nk8
         return lib().orc_sigma_loss(self._o, w.ctypes.data_as(_dp))
+
+    def loss(self, w, terms):
+        """The loss terms of tpsc_loss_forward (bit 1 Re Sigma(k, i pi T), 2 Im, 4 chi_sp(pi,pi,0),
+        8 sum_{n <= 8} Lambda_jj(iq_n)) after the chain they need; NaN if a root fails."""
+        if w is None:
+            w = np.zeros(self.nk8)
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        assert w.size == self.nk8
+        return lib().orc_loss(self._o, w.ctypes.data_as(_dp), int(terms))
 
     def sp_corr_length(self):
         """(vF, xi_y, xi_x, xi_as_y, xi_as_x) of chi::sp_corr_length (chi.cpp:451-640)."""

@@ -2048,6 +2048,38 @@ double orc_sigma_loss(orc *o, const double *w)
     return L;
 }
 
+/* The loss terms of tpsc_loss_forward after chi0 -> Usp, Uch -> V -> Sigma (and, for the
+   Lambda_jj term, traceSelfG -> mu -> Lambda_jj): bit 1 sum_k w_k Re Sigma(k, i pi T), bit 2
+   the same for Im, bit 4 chi_sp(q = (pi, pi), iq_0) = chi0/(1 - Usp chi0/2), bit 8
+   sum_{n <= 8} [chijj1 + chijj2](iq_n).  NaN if a root fails. */
+double orc_loss(orc *o, const double *w, int terms)
+{
+    if (!orc_find_Usp_Uch(o)) return NAN;
+    orc_vertex(o);
+    orc_self_energy(o);
+    const long nk8 = (long)o->nbk * (o->nbk + 1) / 2;
+    const double complex z = I * (PI * o->p.T);
+    double L = 0.0;
+    if (terms & 3)
+        for (long k = 0; k < nk8; k++) {
+            const double complex s = o->Sigma_inf / z + o->Sigma_inf2[k] / (z * z) + o->Sigma_inf3[k] / (z * z * z) +
+                                     o->Self[k] / (z * z * z * z);
+            if (terms & 1) L += w[k] * creal(s);
+            if (terms & 2) L += w[k] * cimag(s);
+        }
+    if (terms & 4) {
+        const double c = o->chiqw[nk8 - 1];
+        L += c / (1.0 - o->Usp * c / 2.0);
+    }
+    if (terms & 8) {
+        orc_trace_self_G(o);
+        if (!orc_find_mu(o)) return NAN;
+        orc_chijj(o, 1);
+        for (int n = 0; n <= 8 && n <= o->nwmax; n++) L += o->chijj1[n] + o->chijj2[n];
+    }
+    return L;
+}
+
 /* ---------------------------------------------------------------- spin correlation length */
 typedef struct {
     const orc *o;

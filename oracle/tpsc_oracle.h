@@ -113,6 +113,7 @@ void orc_sp_corr_from(const double *c0, int nbq, int qx, int qy, double chi0max,
                       double out[4]);                    /* chi.cpp:451-640 */
 int orc_sp_corr_length(orc *o, double out[5]);
 double orc_sigma_loss(orc *o, const double *w);          /* sum_k w_k Re Sigma(k,iw_0)  */
+double orc_loss(orc *o, const double *w, int terms); /* the loss terms of tpsc_loss_forward */
 int orc_al_freqs(int nbw, int mmin, long *out);         /* cond_opt.cpp:5123-5156 */
 int orc_chijj_al(orc *o, int mmin, int N_theta);        /* cond_opt.cpp:4292-6467 (chi3) */
 double complex orc_al_tail(double Usp, double Uch, double chi0inf, double T, double tau,

@@ -19,7 +19,10 @@ FIELDS = {
     "Sigma_tau": 8, "Self": 9, "Sigma_inf2": 10, "Sigma_inf3": 11, "Sigma_inf4": 12,
     "chijj1": 13, "chijj2": 14, "dtau_Sigma": 15, "chi0_tau": 16, "Sigma_kw": 17,
     "chijj3": 18, "chijj_all": 19, "al_freq": 20, "chijj1_k": 21, "chijj2_k": 22, "chijj3_k": 23,
+    "grad_terms": 24, "grad_dmu": 25, "grad_dlambda": 26,
 }
+# loss terms of tpsc_loss_forward (bitwise or)
+LOSS_RE_SIGMA, LOSS_IM_SIGMA, LOSS_CHISP_Q, LOSS_LAMBDA = 1, 2, 4, 8
 PREC_FP64 = 0
 PREC_MIXED = 1
 PRECISIONS = {"fp64": PREC_FP64, "mixed": PREC_MIXED}
@@ -59,7 +62,7 @@ EXPORTS = [
     "tpsc_prof_enable", "tpsc_prof_reset", "tpsc_prof_count", "tpsc_prof_get",
     "tpsc_keep_chi_tau", "tpsc_device_bytes", "tpsc_pade_coef", "tpsc_pade_eval", "tpsc_pade_conductivity",
     "tpsc_get_params", "tpsc_write_files", "tpsc_gwn_local", "tpsc_fix_mu0", "tpsc_sigma_loss_forward",
-    "tpsc_sigma_loss_backward", "tpsc_random_weights", "tpsc_sp_corr_length", "tpsc_solve_batch",
+    "tpsc_sigma_loss_backward", "tpsc_loss_forward", "tpsc_loss_backward", "tpsc_random_weights", "tpsc_sp_corr_length", "tpsc_solve_batch",
     "tpsc_batch_tune", "tpsc_batch_points", "tpsc_set_params", "tpsc_find_zero", "tpsc_sum_rule_noalloc",
     "tpsc_mu0_infinite", "tpsc_fermi_surface", "tpsc_fs_lattice", "tpsc_mdc", "tpsc_edc", "tpsc_self_real_freq",
     "tpsc_save_self_bin", "tpsc_save_chi0_bin", "tpsc_read_self_bin", "tpsc_read_chi0_bin",
@@ -115,6 +118,8 @@ def lib():
     L.tpsc_fix_mu0.argtypes = [vp, C.c_double]
     L.tpsc_sigma_loss_forward.argtypes = [vp, dp, dp]
     L.tpsc_sigma_loss_backward.argtypes = [vp, C.c_double, dp]
+    L.tpsc_loss_forward.argtypes = [vp, dp, C.c_int, dp]
+    L.tpsc_loss_backward.argtypes = [vp, C.c_double, dp]
     L.tpsc_random_weights.argtypes = [C.c_ulonglong, C.c_size_t, dp]
     L.tpsc_sp_corr_length.argtypes = [vp, dp]
     L.tpsc_solve_batch.argtypes = [C.POINTER(Params), C.c_int, C.c_int, C.POINTER(BatchResult), dp, C.c_int]

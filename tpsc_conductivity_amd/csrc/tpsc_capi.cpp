@@ -81,7 +81,9 @@ struct tpsc_ctx {
     bool mu0_fixed = false;      // tpsc_fix_mu0: mu0 is an input, not solved from n
     bool mu0_failed = false;     // the last mu0 search found no root (mu0 kept its previous value)
     TanArrays tan{};             // tangent arrays of the Sigma-loss gradient
-    bool grad_ready = false;     // forward of the Sigma loss done, weights on the device
+    bool grad_ready = false;     // forward of the loss done, weights on the device
+    int loss_terms = 0;          // TPSC_LOSS_* of the last tpsc_loss_forward
+    std::vector<double> grad_terms, grad_dmu, grad_dlambda;  // per-parameter tangents of the last backward
     bool keep_chi_tau = false;
     bool chi_tau_fp64 = false;   // the kept chi0(q,tau) came from an FP64 chi0 stage
     // mixed path: the chi0 stage of the full grid runs in FP64 (chi0(q,tau) in chiw, chi0(r,tau) in
@@ -1127,21 +1129,73 @@ void tangent_model(const Grid& g, int p, std::vector<double>& dek, std::vector<d
     if (g.nk8 > 2) der[(p == 0) ? 1 : 2] = -1.0;
 }
 
-// sum_k w_k Re Sigma(k, i w_0) from the current (FP64) Sigma
-int sigma_loss_value(tpsc_ctx* c, const double* Sinf2, const cd* Self, double* out)
+// tangent arrays of the Lambda_jj stage
+int tan_alloc_jj(tpsc_ctx* c)
 {
-    HIPCHK(c, launch_sigma_loss(c->g, c->tan.w, Sinf2, Self, c->tan.red, c->stream));
+    TanArrays& t = c->tan;
+    if (t.vel) return 0;
+    const Grid& g = c->g;
+    const size_t nq = (size_t)g.nq, plane = (size_t)(g.Nw + 1) * nq;
+    t.jj_nblk = (int)std::min<long>(g.nq, 1024);
+    int rc = 0;
+    rc |= dalloc(c, &t.vel, nq);
+    rc |= dalloc(c, &t.h, plane);
+    rc |= dalloc(c, &t.f0, plane);
+    rc |= dalloc(c, &t.hT, plane);
+    rc |= dalloc(c, &t.f0T, plane);
+    rc |= dalloc(c, &t.dhV, 2 * nq);
+    rc |= dalloc(c, &t.df0V, 2 * nq);
+    rc |= dalloc(c, &t.hVb, 2 * nq);
+    rc |= dalloc(c, &t.f0Vb, 2 * nq);
+    rc |= dalloc(c, &t.bscr, 12 * nq);
+    rc |= dalloc(c, &t.jjpart, (size_t)t.jj_nblk * 2 * g.nwb);
+    rc |= dalloc(c, &t.jjred, (size_t)2 * g.nwb + 2);
+    rc |= dalloc(c, &t.jjtr, 2 * (size_t)((g.nk8 + 3) / 4));
+    return rc;
+}
+
+// tangent of v_x(k) on the Lambda grid (kx index l + 1, ky index m) for parameter p:
+// v_x = 2t sin kx + 4t' sin kx cos ky + ... (cond_opt.cpp:2645-2660 through k_model)
+void tangent_velocity(const Grid& g, int p, std::vector<double>& dv)
+{
+    dv.assign((size_t)g.nq, 0.0);
+    if (p > 1) return;
+    const int nk = g.nk;
+    const double dk = M_PI / (nk - 1);
+    for (long i = 0; i < g.nq; i++) {
+        const int l = (int)(i / nk), m = (int)(i % nk);
+        const double sx = sin((l + 1) * dk), cy = cos(m * dk);
+        dv[(size_t)i] = (p == 0) ? 2.0 * sx : 4.0 * sx * cy;
+    }
+}
+
+// sum_k w_k Re Sigma(k, i w_0) (which 0) or Im Sigma (which 1) from the FP64 Sigma arrays given
+int sigma_loss_value(tpsc_ctx* c, const double* Sinf, const double* Sinf2, const double* Sinf3, const cd* Self,
+                     int which, double* out)
+{
+    HIPCHK(c, launch_sigma_loss(c->g, c->tan.w, Sinf, Sinf2, Sinf3, Self, which, c->tan.red, c->stream));
     HIPCHK(c, hipMemcpyAsync(out, c->tan.red, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;
 }
 
-// one tangent pass of the chain for parameter p; *dL receives d/dp sum_k w_k Re Sigma(k, iw_0)
-int tangent_pass(tpsc_ctx* c, int p, double* dL)
+// the Lambda_jj term of the loss: sum_{n <= 8} [Lambda_jj1 + Lambda_jj2](iq_n)
+constexpr int kLossLambdaN = 8;
+double lambda_term(const std::vector<double>& a, const std::vector<double>& b)
+{
+    double L = 0.0;
+    for (size_t n = 0; n < a.size() && n <= (size_t)kLossLambdaN; n++) L += a[n] + b[n];
+    return L;
+}
+
+// one tangent pass of the chain for parameter p; term[k] receives the derivative of loss term
+// k (Re Sigma, Im Sigma, chi_sp(pi,pi), Lambda_jj) for the terms in c->loss_terms
+int tangent_pass(tpsc_ctx* c, int p, double term[4])
 {
     const Grid& g = c->g;
     TanArrays& t = c->tan;
     hipStream_t s = c->stream;
+    const int terms = c->loss_terms;
     std::vector<double> dek, dekw, der;
     tangent_model(g, p, dek, dekw, der);
     HIPCHK(c, hipMemcpy(t.dek, dek.data(), sizeof(double) * dek.size(), hipMemcpyHostToDevice));
@@ -1166,6 +1220,16 @@ int tangent_pass(tpsc_ctx* c, int p, double* dL)
     const double k2 = nsr * nsr / (2.0 * U);
     const double dUsp = (Usp * k2 / U * dU - A[0]) / (A[1] + k2);
     const double dUch = -(A[2] - k2 * dUsp + Usp * k2 / U * dU) / A[3];
+    for (int k = 0; k < 4; k++) term[k] = 0.0;
+    if (terms & TPSC_LOSS_CHISP_Q) {
+        // chi_sp(Q, 0) = chi0/(1 - Usp chi0/2) at the last wedge point Q = (pi, pi), n = 0
+        double x[2];
+        HIPCHK(c, hipMemcpy(&x[0], c->d.chiw + (g.nk8 - 1), sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(&x[1], t.chi + (g.nk8 - 1), sizeof(double), hipMemcpyDeviceToHost));
+        const double den = 1.0 - Usp * x[0] / 2.0;
+        term[2] = (x[1] + x[0] * x[0] * dUsp / 2.0) / (den * den);
+    }
+    if (!(terms & (TPSC_LOSS_RE_SIGMA | TPSC_LOSS_IM_SIGMA | TPSC_LOSS_LAMBDA))) return 0;
     // V: tangent integrand, the linear tau transform, then + dfact chi0(r,tau)
     const double fact = U * (3.0 * Usp + Uch) / 8.0;
     const double dfact = (dU * (3.0 * Usp + Uch) + U * (3.0 * dUsp + dUch)) / 8.0;
@@ -1190,7 +1254,42 @@ int tangent_pass(tpsc_ctx* c, int p, double* dL)
     v.Sinf4 = t.Sinf4;
     v.Sinf = t.Sinf;
     HIPCHK(c, launch_sigma_pencil(g, v, s));
-    return sigma_loss_value(c, t.Sinf2, t.Self, dL);
+    int rc = 0;
+    if ((terms & TPSC_LOSS_RE_SIGMA) && (rc = sigma_loss_value(c, t.Sinf, t.Sinf2, t.Sinf3, t.Self, 0, &term[0])))
+        return rc;
+    if ((terms & TPSC_LOSS_IM_SIGMA) && (rc = sigma_loss_value(c, t.Sinf, t.Sinf2, t.Sinf3, t.Self, 1, &term[1])))
+        return rc;
+    if (!(terms & TPSC_LOSS_LAMBDA)) return 0;
+    // mu from its root, Tr(G - G0)(mu) = 0 (green.cpp:3185-3282): dmu = -F_p / F_mu
+    double F2[2];
+    HIPCHK(c, launch_trace_diffG_tan(g, c->d, t, c->mu0, dmu0, c->mu, t.jjred, s));
+    HIPCHK(c, hipMemcpyAsync(F2, t.jjred, sizeof F2, hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    const double dmu = -F2[0] / F2[1];
+    // hV, f0V slabs and their boundary data, transposes, then the pencil
+    std::vector<double> dv;
+    tangent_velocity(g, p, dv);
+    HIPCHK(c, hipMemcpy(t.vel, dv.data(), sizeof(double) * dv.size(), hipMemcpyHostToDevice));
+    const double fact4 = 2.0 * fact, dfact4 = 2.0 * dfact;
+    HIPCHK(c, launch_hVf0V_tan(g, c->d, t, c->mu0, dmu0, s));
+    HIPCHK(c, launch_hf0_bnd_tan(g, c->d, t, c->mu0, dmu0, fact4, dfact4, s));
+    v = c->d;
+    v.h = t.h;
+    v.f0 = t.f0;
+    v.hT = t.hT;
+    v.f0T = t.f0T;
+    HIPCHK(c, launch_jj_transpose(g, v, s));
+    HIPCHK(c, launch_jj_pencil_tan(g, c->d, t, c->mu, dmu, t.jj_nblk, s));
+    const int nwb = g.nwb;
+    HIPCHK(c, launch_reduce_cols(t.jjpart, t.jj_nblk, 2 * nwb, t.jjred, s));
+    std::vector<double> red((size_t)2 * nwb);
+    HIPCHK(c, hipMemcpyAsync(red.data(), t.jjred, sizeof(double) * red.size(), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    std::vector<double> d1(red.begin(), red.begin() + nwb), d2(red.begin() + nwb, red.end());
+    term[3] = lambda_term(d1, d2);
+    c->grad_dmu[p] = dmu;
+    for (int n = 0; n < nwb; n++) c->grad_dlambda[(size_t)p * nwb + n] = d1[n] + d2[n];
+    return 0;
 }
 
 size_t field_count(tpsc_ctx* c, int field)
@@ -1220,6 +1319,9 @@ size_t field_count(tpsc_ctx* c, int field)
         case TPSC_F_CHIJJ1_K:
         case TPSC_F_CHIJJ2_K: return c->chijj1_k.size();
         case TPSC_F_CHIJJ3_K: return c->chijj3_k.size();
+        case TPSC_F_GRAD_TERMS: return c->grad_terms.size();
+        case TPSC_F_GRAD_DMU: return c->grad_dmu.size();
+        case TPSC_F_GRAD_DLAMBDA: return c->grad_dlambda.size();
         default: return 0;
     }
 }
@@ -1565,6 +1667,15 @@ int tpsc_get(tpsc_ctx* c, int field, void* dst, size_t bytes)
                                                                                              : (double)c->al_freq[i];
         return 0;
     }
+    if (field == TPSC_F_GRAD_TERMS || field == TPSC_F_GRAD_DMU || field == TPSC_F_GRAD_DLAMBDA) {
+        if (n == 0) return fail(c, TPSC_E_ORDER, "no gradient of a loss with these terms computed");
+        if (bytes < n * sizeof(double)) return fail(c, TPSC_E_INVALID, "destination too small");
+        const std::vector<double>& v = (field == TPSC_F_GRAD_TERMS) ? c->grad_terms
+                                       : (field == TPSC_F_GRAD_DMU) ? c->grad_dmu
+                                                                    : c->grad_dlambda;
+        memcpy(dst, v.data(), n * sizeof(double));
+        return 0;
+    }
     if (field == TPSC_F_CHIJJ1_K || field == TPSC_F_CHIJJ2_K || field == TPSC_F_CHIJJ3_K) {
         if (n == 0)
             return fail(c, TPSC_E_ORDER, field == TPSC_F_CHIJJ3_K ? "angle-binned Lambda_jj3 not computed (N_theta_k > 1)"
@@ -1777,33 +1888,78 @@ int tpsc_fix_mu0(tpsc_ctx* c, double mu0)
     return 0;
 }
 
-int tpsc_sigma_loss_forward(tpsc_ctx* c, const double* w, double* loss)
+int tpsc_loss_forward(tpsc_ctx* c, const double* w, int terms, double* loss)
 {
-    if (!c || !w || !loss) return TPSC_E_INVALID;
+    if (!c || !loss) return TPSC_E_INVALID;
+    const int all = TPSC_LOSS_RE_SIGMA | TPSC_LOSS_IM_SIGMA | TPSC_LOSS_CHISP_Q | TPSC_LOSS_LAMBDA;
+    if (terms == 0 || (terms & ~all)) return fail(c, TPSC_E_INVALID, "unknown loss terms");
+    if ((terms & (TPSC_LOSS_RE_SIGMA | TPSC_LOSS_IM_SIGMA)) && !w)
+        return fail(c, TPSC_E_INVALID, "the Sigma terms need the weights w");
     if (c->p.precision != TPSC_PREC_FP64) return fail(c, TPSC_E_INVALID, "the gradient runs in the FP64 path only");
+    if ((terms & TPSC_LOSS_LAMBDA) && !grad_jj_supported(c->g))
+        return fail(c, TPSC_E_INVALID, "the Lambda_jj term of the gradient needs Nw <= 1024 and N >= 8");
     c->grad_ready = false;
     int rc = tpsc_calc_self_energy_full(c);
     if (rc) return rc;
     if ((rc = tan_alloc(c))) return rc;
-    HIPCHK(c, hipMemcpy(c->tan.w, w, sizeof(double) * c->g.nk8, hipMemcpyHostToDevice));
-    if ((rc = sigma_loss_value(c, c->d.Sinf2, c->d.Self, loss))) return rc;
+    if ((terms & TPSC_LOSS_LAMBDA) && (rc = tan_alloc_jj(c))) return rc;
+    if (w) HIPCHK(c, hipMemcpy(c->tan.w, w, sizeof(double) * c->g.nk8, hipMemcpyHostToDevice));
+    double L = 0.0, v = 0.0;
+    if (terms & TPSC_LOSS_RE_SIGMA) {
+        if ((rc = sigma_loss_value(c, c->d.Sinf, c->d.Sinf2, c->d.Sinf3, c->d.Self, 0, &v))) return rc;
+        L += v;
+    }
+    if (terms & TPSC_LOSS_IM_SIGMA) {
+        if ((rc = sigma_loss_value(c, c->d.Sinf, c->d.Sinf2, c->d.Sinf3, c->d.Self, 1, &v))) return rc;
+        L += v;
+    }
+    if (terms & TPSC_LOSS_CHISP_Q) {
+        double x = 0.0;
+        HIPCHK(c, hipMemcpy(&x, c->d.chiw + (c->g.nk8 - 1), sizeof(double), hipMemcpyDeviceToHost));
+        L += x / (1.0 - c->Usp * x / 2.0);
+    }
+    if (terms & TPSC_LOSS_LAMBDA) {
+        if ((rc = tpsc_trace_self_G(c, nullptr, nullptr)) || (rc = tpsc_find_mu(c, nullptr)) || (rc = tpsc_chijj(c, 0, 0, 1)))
+            return rc;
+        L += lambda_term(c->chijj1, c->chijj2);
+    }
+    c->loss_terms = terms;
+    c->grad_terms.assign(16, 0.0);
+    c->grad_dmu.assign(4, 0.0);
+    c->grad_dlambda.assign((terms & TPSC_LOSS_LAMBDA) ? (size_t)4 * c->g.nwb : 0, 0.0);
     c->grad_ready = true;
+    *loss = L;
     return 0;
 }
 
-int tpsc_sigma_loss_backward(tpsc_ctx* c, double dloss, double* grad)
+int tpsc_loss_backward(tpsc_ctx* c, double dloss, double* grad)
 {
     if (!c || !grad) return TPSC_E_INVALID;
     if (!c->grad_ready || !c->have_sigma)
-        return fail(c, TPSC_E_ORDER, "tpsc_sigma_loss_backward needs tpsc_sigma_loss_forward first");
+        return fail(c, TPSC_E_ORDER, "the loss backward needs its forward first");
+    if ((c->loss_terms & TPSC_LOSS_LAMBDA) && !c->have_jj)
+        return fail(c, TPSC_E_ORDER, "Lambda_jj changed since the loss forward");
     for (int p = 0; p < 4; p++) {
-        double dL = 0.0;
-        int rc = tangent_pass(c, p, &dL);
+        double term[4];
+        int rc = tangent_pass(c, p, term);
         if (rc) return rc;
+        double dL = 0.0;
+        for (int k = 0; k < 4; k++) {
+            c->grad_terms[(size_t)p * 4 + k] = term[k];
+            dL += term[k];
+        }
         grad[p] = dloss * dL;
     }
     return 0;
 }
+
+int tpsc_sigma_loss_forward(tpsc_ctx* c, const double* w, double* loss)
+{
+    if (!c || !w || !loss) return TPSC_E_INVALID;
+    return tpsc_loss_forward(c, w, TPSC_LOSS_RE_SIGMA, loss);
+}
+
+int tpsc_sigma_loss_backward(tpsc_ctx* c, double dloss, double* grad) { return tpsc_loss_backward(c, dloss, grad); }
 
 int tpsc_find_zero(double (*f)(double, void*), void* user, const double init[3], const double* lims, double* root,
                    int* nevals)

@@ -418,16 +418,21 @@ __global__ void __launch_bounds__(GRAD_THREADS)
     }
 }
 
-// sum_k w_k Re Sigma(k, i w_0) = sum_k w_k (-C2_k / w0^2 + Re R_k(0) / w0^4): the 1/z and 1/z^3
-// terms are imaginary at z = i w0.  Self: FP64 remainder R, point-major [k*(Nw/2) + n].
-__global__ void k_sigma_loss(const double* __restrict__ w, const double* __restrict__ Sinf2, const cd* __restrict__ Self,
-                             long nk8, int H, double w0, double* out)
+// Sigma(k, i w0) = Sinf/z + C2/z^2 + C3/z^3 + R/z^4 at z = i w0 (green.cpp:3039):
+// which 0: sum_k w_k Re Sigma = sum_k w_k (-C2_k / w0^2 + Re R_k(0) / w0^4);
+// which 1: sum_k w_k Im Sigma = sum_k w_k (-Sinf / w0 + C3_k / w0^3 + Im R_k(0) / w0^4).
+// Self: FP64 remainder R, point-major [k*(Nw/2) + n].
+__global__ void k_sigma_loss(const double* __restrict__ w, const double* __restrict__ Sinfp,
+                             const double* __restrict__ Sinf2, const double* __restrict__ Sinf3,
+                             const cd* __restrict__ Self, long nk8, int H, double w0, int which, double* out)
 {
     __shared__ double sc[16];
     double acc = 0.0;
-    const double i2 = 1.0 / (w0 * w0);
+    const double i1 = 1.0 / w0, i2 = i1 * i1;
+    const double Si = Sinfp[0];
     for (long k = threadIdx.x; k < nk8; k += blockDim.x)
-        acc += w[k] * (-Sinf2[k] * i2 + Self[k * H].x * i2 * i2);
+        acc += (which == 0) ? w[k] * (-Sinf2[k] * i2 + Self[k * H].x * i2 * i2)
+                            : w[k] * (-Si * i1 + Sinf3[k] * i2 * i1 + Self[k * H].y * i2 * i2);
     acc = block_sum(acc, sc);
     if (threadIdx.x == 0) out[0] = acc;
 }
@@ -523,10 +528,11 @@ hipError_t launch_sigma_tan(const Grid& g, const DevArrays& d, const TanArrays& 
     return hipGetLastError();
 }
 
-hipError_t launch_sigma_loss(const Grid& g, const double* w, const double* Sinf2, const cd* Self, double* out,
-                             hipStream_t s)
+hipError_t launch_sigma_loss(const Grid& g, const double* w, const double* Sinf, const double* Sinf2,
+                             const double* Sinf3, const cd* Self, int which, double* out, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_sigma_loss, dim3(1), dim3(1024), 0, s, w, Sinf2, Self, g.nk8, g.Nw / 2, kPi * g.T, out);
+    hipLaunchKernelGGL(k_sigma_loss, dim3(1), dim3(1024), 0, s, w, Sinf, Sinf2, Sinf3, Self, g.nk8, g.Nw / 2,
+                       kPi * g.T, which, out);
     return hipGetLastError();
 }
 

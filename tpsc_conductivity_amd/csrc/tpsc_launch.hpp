@@ -115,6 +115,21 @@ struct TanArrays {
     double* part = nullptr;    // 4 * sum-rule grid
     double* red = nullptr;     // 8
     double* w = nullptr;       // nk8 loss weights
+    // the Lambda_jj stage (allocated at the first gradient with the Lambda_jj term)
+    double* vel = nullptr;     // nq          d v_x(k) on the Lambda grid
+    double* h = nullptr;       // (Nw+1)*nq   d hV(k,tau), tau-major
+    double* f0 = nullptr;      // (Nw+1)*nq   d f0V(k,tau)
+    double* hT = nullptr;      // (Nw+1)*nq   the same, point-major
+    double* f0T = nullptr;
+    double* dhV = nullptr;     // 2*nq        d of the tau-slopes at 0, beta
+    double* df0V = nullptr;
+    double* hVb = nullptr;     // 2*nq        d of the values at 0, beta
+    double* f0Vb = nullptr;
+    double* bscr = nullptr;    // 12*nq       staging of k_hf0_bnd_tan
+    double* jjpart = nullptr;  // jj_nblk*2*nwb  per-workgroup tangent rows
+    double* jjred = nullptr;   // 2*nwb + 2   their sums, then (dF/dp, dF/dmu) of the mu root
+    double* jjtr = nullptr;    // 2*ceil(nk8/4) partials of k_trace_diffG_tan
+    int jj_nblk = 0;
 };
 
 struct Grid {
@@ -198,8 +213,19 @@ hipError_t launch_vertex_slab_tan(const Grid& g, const DevArrays& d, const TanAr
 hipError_t launch_axpy(long n, double a, const double* x, double* y, hipStream_t s);
 hipError_t launch_sigma_tan(const Grid& g, const DevArrays& d, const TanArrays& t, double mu0, double dmu0,
                             double fact, double dfact, hipStream_t s);
-hipError_t launch_sigma_loss(const Grid& g, const double* w, const double* Sinf2, const cd* Self, double* out,
-                             hipStream_t s);
+// which 0: sum_k w_k Re Sigma(k, i pi T); 1: sum_k w_k Im Sigma(k, i pi T)
+hipError_t launch_sigma_loss(const Grid& g, const double* w, const double* Sinf, const double* Sinf2,
+                             const double* Sinf3, const cd* Self, int which, double* out, hipStream_t s);
+// tangent kernels of the Lambda_jj stage (tpsc_grad_jj.hip)
+bool grad_jj_supported(const Grid& g);
+hipError_t launch_trace_diffG_tan(const Grid& g, const DevArrays& d, const TanArrays& t, double mu0, double dmu0,
+                                  double mu, double* out2, hipStream_t s);
+hipError_t launch_hVf0V_tan(const Grid& g, const DevArrays& d, const TanArrays& t, double mu0, double dmu0,
+                            hipStream_t s);
+hipError_t launch_hf0_bnd_tan(const Grid& g, const DevArrays& d, const TanArrays& t, double mu0, double dmu0,
+                              double fact4, double dfact4, hipStream_t s);
+hipError_t launch_jj_pencil_tan(const Grid& g, const DevArrays& d, const TanArrays& t, double mu, double dmu,
+                                int nblk, hipStream_t s);
 hipError_t launch_transpose_d(const double* in, double* out, long ncols, int nrows, hipStream_t s);
 hipError_t launch_sum_rows(const double* part, int nrows, int n, double* out, hipStream_t s);
 hipError_t launch_al_setup(const Grid& g, const DevArrays& d, const ALArrays& a, double mu0, hipStream_t s);

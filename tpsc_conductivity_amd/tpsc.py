@@ -158,6 +158,28 @@ class CondOpt:
                        "tpsc_sigma_loss_backward")
         return g
 
+    def loss_forward(self, w, terms):
+        """L = the loss terms `terms` (capi.LOSS_*: Re / Im Sigma(k, i pi T) weighted by w,
+        chi_sp(pi, pi, 0), sum_{n <= 8} Lambda_jj(iq_n)) after the chain they need (FP64 path)."""
+        import numpy as np
+
+        wp = None
+        if w is not None:
+            w = np.ascontiguousarray(w, dtype=np.float64)
+            wp = w.ctypes.data_as(C.POINTER(C.c_double))
+        L = C.c_double(0.0)
+        self.ctx.check(lib().tpsc_loss_forward(self.ctx.handle, wp, int(terms), C.byref(L)), "tpsc_loss_forward")
+        return L.value
+
+    def loss_backward(self, dloss=1.0):
+        """dloss * dL/d(t, t', mu0, U) for the last loss_forward."""
+        import numpy as np
+
+        g = np.zeros(4)
+        self.ctx.check(lib().tpsc_loss_backward(self.ctx.handle, float(dloss), g.ctypes.data_as(C.POINTER(C.c_double))),
+                       "tpsc_loss_backward")
+        return g
+
     # ---- post-processing (host FP64 over the device results, tpsc_main.cpp:77-101)
     def find_mu_inf_syst(self):
         """green0::find_mu_inf_syst (green0.cpp:266-287): mu0 of the infinite lattice."""
