@@ -154,28 +154,34 @@ def bench_al(cfg, args):
 
 
 def bench_batch(args):
-    """Throughput of tpsc_solve_batch on the 64 random points (N in {32..128}, T, n, U, t'),
-    tuned worker count against one worker (the points one after another)."""
+    """Throughput of tpsc_solve_batch on the 64 random points (N in {32..128}, T, n, U, t'), in
+    mixed precision (value) and FP64, each with the tuned worker count and with one worker (the
+    points one after another)."""
     from tpsc_conductivity_amd import capi
 
     pts = capi.batch_points(64064, 64)
-    w, rates = capi.batch_tune(pts)
-    capi.solve_batch(pts, workers=w)  # warm-up (code objects, allocator)
-    out = {}
-    for tag, nw in (("tuned", w), ("one_worker", 1)):
-        t0 = time.perf_counter()
-        res, _ = capi.solve_batch(pts, workers=nw)
-        dt = time.perf_counter() - t0
-        st = [r.status for r in res]
-        out[tag] = {"workers": nw, "seconds": dt, "solved": st.count(0),
-                    "unsupported": st.count(-1), "failed": len(st) - st.count(0) - st.count(-1)}
-        print(f"[bench] batch {tag}: {nw} workers, {dt:.3f} s", file=sys.stderr, flush=True)
-    solved = out["tuned"]["solved"]
+    paths = {}
+    for prec, code in (("mixed", capi.PREC_MIXED), ("fp64", capi.PREC_FP64)):
+        for p in pts:
+            p.precision = code
+        w, rates = capi.batch_tune(pts)
+        capi.solve_batch(pts, workers=w)  # warm-up (code objects, allocator)
+        runs = {}
+        for tag, nw in (("tuned", w), ("one_worker", 1)):
+            t0 = time.perf_counter()
+            res, _ = capi.solve_batch(pts, workers=nw)
+            dt = time.perf_counter() - t0
+            st = [r.status for r in res]
+            runs[tag] = {"workers": nw, "seconds": dt, "solved": st.count(0),
+                         "unsupported": st.count(-1), "failed": len(st) - st.count(0) - st.count(-1)}
+            print(f"[bench] batch {prec} {tag}: {nw} workers, {dt:.3f} s", file=sys.stderr, flush=True)
+        paths[prec] = {"solves_per_s": runs["tuned"]["solved"] / runs["tuned"]["seconds"],
+                       "speedup_over_one_worker": runs["one_worker"]["seconds"] / runs["tuned"]["seconds"],
+                       "tuner_points_per_s": {str(k): float(v) for k, v in zip((1, 2, 4, 8), rates)},
+                       "runs": runs}
     print(json.dumps({"metric": "TPSC batched solves/s, 64 random (N, T, n, U, t') points, one MI355X",
-                      "value": solved / out["tuned"]["seconds"], "unit": "solves/s", "higher_is_better": True,
-                      "speedup_over_one_worker": out["one_worker"]["seconds"] / out["tuned"]["seconds"],
-                      "tuner_points_per_s": {str(k): float(v) for k, v in zip((1, 2, 4, 8), rates)},
-                      "runs": out, "dtype": "f64", "data": "synthetic",
+                      "value": paths["mixed"]["solves_per_s"], "unit": "solves/s", "higher_is_better": True,
+                      "mixed": paths["mixed"], "fp64": paths["fp64"], "dtype": "f32+f64", "data": "synthetic",
                       "config": {"workload": "tpsc_batch_points(64064, 64): N in {32,48,64,96,128}, "
                                              "T in [0.05,0.5], n in [0.8,1], U in [1,6], t' in [-0.3,0], "
                                              "Nw = smallest power of two with (Nw-1) pi T >= 20"}}),

@@ -102,8 +102,10 @@ enum tpsc_field {
     TPSC_F_GRAD_TERMS = 24,   /* 16 values [p*4 + k]: d/dp of loss term k (Re Sigma, Im Sigma,
                                  chi_sp(pi,pi), Lambda_jj; 0 for terms not in the loss) */
     TPSC_F_GRAD_DMU = 25,     /* 4 values: dmu/dp of the interacting mu (Lambda_jj term only) */
-    TPSC_F_GRAD_DLAMBDA = 26  /* 4*nwb values [p*nwb + n]: d/dp [Lambda_jj1 + Lambda_jj2](iq_n)
+    TPSC_F_GRAD_DLAMBDA = 26, /* 4*nwb values [p*nwb + n]: d/dp [Lambda_jj1 + Lambda_jj2](iq_n)
                                  (Lambda_jj term only) */
+    TPSC_F_SIGMA_BND = 27     /* Sigma(k, tau = 0) and Sigma(k, tau = beta), 2*nk8: the end samples
+                                 the self-energy spline uses (FP64 in both paths; green.cpp:1822-1892) */
 };
 enum tpsc_scalar {
     TPSC_S_MU0 = 1, TPSC_S_CHI0MAX, TPSC_S_QMAX_L, TPSC_S_QMAX_M, TPSC_S_USP, TPSC_S_UCH,

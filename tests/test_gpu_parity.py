@@ -5,9 +5,10 @@ the reference exposes -- mu0, chi0(q,iqn), chi0(r,tau), Usp, Uch, chi_sp, chi_ch
 Sigma(k,tau), Sigma(k,iw_n), mu, the traces, Lambda_jj1/2(iq_n), kx0.  The raw 1/z^4
 remainder R(k,n) of Sigma and the top-frequency tail chi0_inf2 are compared at 1e-8: they
 are third-derivative spline sums whose high-frequency entries amplify 1-ulp differences of
-their tau-space inputs ~1e5-1e6 fold (an exact-arithmetic solve of the GPU's own inputs
-agrees with the GPU to ~1e-11), and they enter Sigma(k,iw_n) and chi0 only divided by
-w_n^4, where the 1e-10 bound holds.
+their tau-space inputs ~1e5-1e6 fold, and they enter Sigma(k,iw_n) and chi0 only divided by
+w_n^4, where the 1e-10 bound holds.  The evidence: R recomputed in extended precision from the
+GPU's own inputs matches the GPU below 1e-11 (tests/test_gpu_remainder_precision.py), and from the
+oracle's inputs matches the oracle likewise (tests/test_oracle_numerics.py).
 """
 import numpy as np
 import pytest

@@ -148,3 +148,22 @@ def test_pade_interpolates_and_recovers_rational(oracle_mod):
     assert relerr(out, vals) < 1e-12
     zs = np.array([0.3 + 0.05j, -0.7 + 0.2j, 2.0 + 0.01j])
     assert relerr(oracle_mod.pade(vals, x, zs), f(zs)) < 1e-9
+
+
+@pytest.mark.parametrize("N,Nw,T", [(16, 256, 0.1), (32, 1024, 0.05)])
+def test_self_energy_remainder_is_the_exact_transform_of_its_inputs(N, Nw, T, oracle_mod):
+    """The raw remainder R(k, n) (green.cpp:1925-2018) equals its transform of Sigma(k, tau) and the
+    end slopes recomputed in x87 extended precision (tests/remainder_ld.py) to ~1e-12: the spline
+    solve and the phased DFT add no error of their own beyond FP64 rounding."""
+    import remainder_ld as rl
+
+    o = oracle_mod.Oracle(N, Nw, 0.875, T, 4.0, tp=-0.175, coarse_pass=False)
+    assert o.solve()
+    nk8 = o.nk8
+    St = o.get("Sigma_tau").reshape(Nw + 1, nk8)
+    dS = o.get("dtau_Sigma").reshape(2, nk8)
+    R = o.get("Self").reshape(-1, nk8)[: Nw // 2]
+    ks = np.arange(nk8) if nk8 < 60 else np.linspace(0, nk8 - 1, 8).astype(int)
+    Rl = rl.remainder(St[:, ks], dS[0, ks], dS[1, ks], T).astype(np.complex128)
+    e = np.max(np.abs(R[:, ks] - Rl), axis=0) / np.max(np.abs(Rl), axis=0)
+    assert e.max() < 1e-11

@@ -19,7 +19,7 @@ FIELDS = {
     "Sigma_tau": 8, "Self": 9, "Sigma_inf2": 10, "Sigma_inf3": 11, "Sigma_inf4": 12,
     "chijj1": 13, "chijj2": 14, "dtau_Sigma": 15, "chi0_tau": 16, "Sigma_kw": 17,
     "chijj3": 18, "chijj_all": 19, "al_freq": 20, "chijj1_k": 21, "chijj2_k": 22, "chijj3_k": 23,
-    "grad_terms": 24, "grad_dmu": 25, "grad_dlambda": 26,
+    "grad_terms": 24, "grad_dmu": 25, "grad_dlambda": 26, "Sigma_bnd": 27,
 }
 # loss terms of tpsc_loss_forward (bitwise or)
 LOSS_RE_SIGMA, LOSS_IM_SIGMA, LOSS_CHISP_Q, LOSS_LAMBDA = 1, 2, 4, 8

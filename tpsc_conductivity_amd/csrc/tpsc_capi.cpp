@@ -1308,6 +1308,7 @@ size_t field_count(tpsc_ctx* c, int field)
         case TPSC_F_SIGMA_INF3:
         case TPSC_F_SIGMA_INF4: return (size_t)g.nk8;
         case TPSC_F_DTAU_SIGMA: return 2 * (size_t)g.nk8;
+        case TPSC_F_SIGMA_BND: return 2 * (size_t)g.nk8;
         case TPSC_F_SIGMA_TAU: return (size_t)(g.Nw + 1) * g.nk8;
         case TPSC_F_SELF:
         case TPSC_F_SIGMA_KW: return 2 * (size_t)(g.Nw / 2) * g.nk8;
@@ -1716,6 +1717,7 @@ int tpsc_get(tpsc_ctx* c, int field, void* dst, size_t bytes)
         case TPSC_F_SIGMA_INF3: src = c->d.Sinf3; break;
         case TPSC_F_SIGMA_INF4: src = c->d.Sinf4; break;
         case TPSC_F_DTAU_SIGMA: src = c->d.dSig; break;
+        case TPSC_F_SIGMA_BND: src = c->d.Sb; break;
         case TPSC_F_CHI0_TAU:
             if (!c->chi_tau) return fail(c, TPSC_E_ORDER, "chi0(q,tau) not kept");
             src = c->chi_tau;
