@@ -242,13 +242,21 @@ int tpsc_random_weights(unsigned long long seed, size_t n, double* out);
 /* ---- batches of independent parameter points (no reference counterpart) ---- */
 typedef struct tpsc_batch_result {
     int status;                                  /* TPSC_OK or the failing call's code      */
+    int queued;                                  /* 1: solved on the queued path (workers =
+                                                    1), 0: by tpsc_solve                    */
     double mu0, Usp, Uch, mu, dblOcc, kx0;
-    double seconds;                              /* wall time of the point's create + solve */
+    double seconds;                              /* wall time of the point's solve (queued
+                                                    path: the queue's time per point)       */
 } tpsc_batch_result;
 
-/* Solves npts independent points (tpsc_create + tpsc_solve + tpsc_destroy each) with `workers`
- * host threads, each driving its own contexts and streams, so that the points' kernels
- * interleave on the GPU; largest N^2 Nw first.  workers = 0: chosen by tpsc_batch_tune.
+/* Solves npts independent points, each with the results of tpsc_create + tpsc_solve, largest
+ * N^2 Nw first.  workers = 1: one host thread queues every point on one context per grid
+ * without a host wait (root searches settled on the device in at most 8 passes each), drains
+ * the streams once and solves again with tpsc_solve the points the queue could not finish
+ * (a search unsettled after 8 passes, a failed mu0 search, a mixed-precision point that needs
+ * the FP64 chi0 stage).  workers >= 2: host threads, each driving its own contexts and
+ * streams with tpsc_set_params + tpsc_solve, so that the points' kernels interleave on the
+ * GPU.  workers = 0: chosen by tpsc_batch_tune.
  * chijj (optional, npts*ldc doubles): Lambda_jj1 + Lambda_jj2 at n = 0..min(Nw0/2, ldc-1). */
 int tpsc_solve_batch(const tpsc_params* pts, int npts, int workers, tpsc_batch_result* out, double* chijj,
                      int ldc);

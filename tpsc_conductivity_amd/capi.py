@@ -48,7 +48,7 @@ class Params(C.Structure):
 
 
 class BatchResult(C.Structure):
-    _fields_ = [("status", C.c_int), ("mu0", C.c_double), ("Usp", C.c_double), ("Uch", C.c_double),
+    _fields_ = [("status", C.c_int), ("queued", C.c_int), ("mu0", C.c_double), ("Usp", C.c_double), ("Uch", C.c_double),
                 ("mu", C.c_double), ("dblOcc", C.c_double), ("kx0", C.c_double), ("seconds", C.c_double)]
 
 

@@ -1,11 +1,17 @@
-// tpsc_batch.cpp -- many independent parameter points on one GPU, through the public C ABI.
+// tpsc_batch.cpp -- many independent parameter points on one GPU.
 //
-// Each host worker thread keeps one context (device arrays, two streams) per grid and
-// re-targets it to each of its points (tpsc_set_params) before tpsc_solve; with several
-// workers the points' kernels and root passes interleave on the GPU, which a single small
-// point (N <= 128) leaves mostly idle.  Points are taken in
-// decreasing cost N^2 Nw (longest first, from a shared counter).  The worker count is either
-// given or chosen by tpsc_batch_tune, which times a probe subset at 1, 2, 4 and 8 workers.
+// One worker (the default the tuner usually picks): every point is queued without a host wait
+// (tpsc_queue_solve: the root searches read and write device memory only), one context per grid,
+// its results copied into a pinned slot of its own; the host then drains the streams and reads
+// the slots.  Small points (N <= 128) are launch- and round-trip-bound, so queuing them back to
+// back keeps the GPU busy where one point at a time leaves it mostly idle.  Mixed-precision
+// points that need the FP64 chi0 stage are queued a second time with it, as tpsc_solve redoes
+// them; a point that the queue cannot finish as tpsc_solve would (a root search unsettled after
+// its passes, a mu0 search without a root) is solved again with tpsc_solve.
+// Several workers: host threads, each with one context per grid re-targeted point by point
+// (tpsc_set_params) before tpsc_solve.  Points are taken in decreasing cost N^2 Nw (longest
+// first).  The worker count is either given or chosen by tpsc_batch_tune, which times a probe
+// subset at 1, 2, 4 and 8 workers.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -18,7 +24,10 @@
 #include <tuple>
 #include <vector>
 
+#include <hip/hip_runtime.h>
+
 #include "../../include/tpsc_capi.h"
+#include "tpsc_queue.hpp"
 
 namespace {
 
@@ -34,23 +43,32 @@ struct Pool {
     {
         for (auto& kv : ctx) tpsc_destroy(kv.second);
     }
-    int get(const tpsc_params& p, tpsc_ctx** out)
+    // the context of p's grid, created at p when missing (not re-targeted)
+    int ctx_for(const tpsc_params& p, tpsc_ctx** out)
     {
         const auto key = std::make_tuple(p.nk0, p.Nw0, p.precision);
         auto it = ctx.find(key);
-        if (it == ctx.end()) {
-            tpsc_ctx* c = nullptr;
-            const int rc = tpsc_create(&p, &c);
-            if (rc) {
-                tpsc_destroy(c);
-                return rc;
-            }
-            ctx[key] = c;
-            *out = c;
+        if (it != ctx.end()) {
+            *out = it->second;
             return TPSC_OK;
         }
-        *out = it->second;
-        return tpsc_set_params(it->second, &p);
+        tpsc_ctx* c = nullptr;
+        const int rc = tpsc_create(&p, &c);
+        if (rc) {
+            tpsc_destroy(c);
+            return rc;
+        }
+        ctx[key] = c;
+        *out = c;
+        return TPSC_OK;
+    }
+    // the context of p's grid, re-targeted to p
+    int get(const tpsc_params& p, tpsc_ctx** out)
+    {
+        const bool fresh = !ctx.count(std::make_tuple(p.nk0, p.Nw0, p.precision));
+        const int rc = ctx_for(p, out);
+        if (rc || fresh) return rc;
+        return tpsc_set_params(*out, &p);
     }
 };
 
@@ -90,9 +108,64 @@ std::vector<int> by_cost(const tpsc_params* pts, int n)
     return order;
 }
 
+// one host thread: queue every point, drain, read the slots; queue the mixed-precision points
+// that need the FP64 chi0 stage again with it; solve what is left with tpsc_solve
+void run_queued(const tpsc_params* pts, const std::vector<int>& order, tpsc_batch_result* out, double* jj, int ldc)
+{
+    const double t0 = now_s();
+    Pool pool;
+    const size_t n = order.size();
+    std::vector<size_t> off(n + 1, 0);
+    std::vector<tpsc_ctx*> ctxs(n, nullptr);
+    for (size_t k = 0; k < n; k++) {
+        const int i = order[k];
+        std::memset(&out[i], 0, sizeof out[i]);
+        if (pool.ctx_for(pts[i], &ctxs[k])) ctxs[k] = nullptr;
+        off[k + 1] = off[k] + (ctxs[k] ? tpsc_queue_slot_doubles(ctxs[k]) : 0);
+    }
+    double* slots = nullptr;
+    if (off[n] && hipHostMalloc((void**)&slots, off[n] * sizeof(double), hipHostMallocDefault) != hipSuccess)
+        slots = nullptr;
+    std::vector<size_t> todo, redo;
+    for (size_t k = 0; k < n; k++) (slots && ctxs[k] ? todo : redo).push_back(k);
+    size_t done = 0;
+    for (const bool fp64 : {false, true}) {
+        std::vector<int> qrc(n, TPSC_E_INVALID);
+        for (size_t k : todo) qrc[k] = tpsc_queue_solve(ctxs[k], &pts[order[k]], slots + off[k], fp64);
+        for (auto& kv : pool.ctx) tpsc_queue_sync(kv.second);
+        std::vector<size_t> promote;
+        for (size_t k : todo) {
+            const int i = order[k];
+            const int rc = qrc[k] == TPSC_OK ? tpsc_queue_result(&pts[i], slots + off[k], fp64, &out[i],
+                                                                 jj ? jj + (size_t)i * ldc : nullptr, ldc)
+                                             : qrc[k];
+            if (rc == TPSC_OK) {
+                out[i].queued = 1;
+                done++;
+            } else if (rc == kQueuePromote)
+                promote.push_back(k);
+            else
+                redo.push_back(k);
+        }
+        todo.swap(promote);
+    }
+    const double per = done ? (now_s() - t0) / done : 0.0;
+    for (size_t k = 0; k < n; k++)
+        if (out[order[k]].queued) out[order[k]].seconds = per;
+    for (size_t k : redo) {
+        const int i = order[k];
+        solve_point(pool, pts[i], out[i], jj ? jj + (size_t)i * ldc : nullptr, ldc);
+    }
+    if (slots) (void)hipHostFree(slots);
+}
+
 void run(const tpsc_params* pts, const std::vector<int>& order, int workers, tpsc_batch_result* out, double* jj,
          int ldc)
 {
+    if (workers <= 1) {
+        run_queued(pts, order, out, jj, ldc);
+        return;
+    }
     std::atomic<size_t> next{0};
     auto work = [&]() {
         Pool pool;

@@ -78,7 +78,25 @@ __global__ void k_finalize_k(const double* __restrict__ part, int n, double* out
 // chi0max != nullptr (the Usp search): the bracket is (0, 2/chi0max - 2 eps) from the device
 // maximum of chi0, with the start s0.x kept when it lies inside and the midpoint otherwise, so
 // that the host need not wait for chi0max before the first pass
-__global__ void k_halley_init(HalleyState* st, HalleyState s0, const double* chi0max, double* mirror)
+// Point scalars formed on the device by the queued solve (tpsc_capi.cpp) where the host passes
+// them by value otherwise, with the host's rounding (no contraction into fused multiply-adds):
+// U (3 Usp + Uch) / div, and the charge sum rule's constant part
+// A0 - (nsr + Usp nsr^2 / (2U) - nsr^2), A0 = n - n^2/2 (chi.cpp:830-843).
+__device__ __forceinline__ double vfact(double U, double Usp, double Uch, double div)
+{
+#pragma clang fp contract(off)
+    return U * (3.0 * Usp + Uch) / div;
+}
+__device__ __forceinline__ double uch_a0(double A0, double nsr, double U, double Usp)
+{
+#pragma clang fp contract(off)
+    return A0 - (nsr + Usp * nsr * nsr / (2.0 * U) - nsr * nsr);
+}
+
+// a0src (device Usp, may be null): a0 = uch_a0(a0A, nsr, U, *a0src); mu0p (may be null): the
+// state's mu0 from the device
+__global__ void k_halley_init(HalleyState* st, HalleyState s0, const double* chi0max, double* mirror,
+                              const double* mu0p, const double* a0src, double a0A, double nsr, double U)
 {
     HalleyState s = s0;
     if (chi0max) {
@@ -88,11 +106,24 @@ __global__ void k_halley_init(HalleyState* st, HalleyState s0, const double* chi
         s.T = s0.T;
         s.mu0 = s0.mu0;
     }
+    if (mu0p) s.mu0 = *mu0p;
+    if (a0src) s.a0 = uch_a0(a0A, nsr, U, *a0src);
     *st = s;
     mirror[0] = s.status;
     mirror[1] = s.root;
     mirror[2] = s.x;
     mirror[3] = s.ne;
+}
+
+// the outcome of a settled (or exhausted) search into the queued solve's scalars:
+// psc[slot] = root, psc[3 + slot] = status, psc[6 + slot] = evaluations
+__global__ void k_save_root(const HalleyState* __restrict__ st, double* psc, int slot)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        psc[slot] = st->root;
+        psc[3 + slot] = st->status;
+        psc[6 + slot] = st->ne;
+    }
 }
 
 // One Halley step after a pass: the three partial sums of part[k * n + i] (k = 0, 1, 2), each
@@ -594,8 +625,13 @@ __global__ void __launch_bounds__(1024) k_mu0_root(const double* __restrict__ ek
 // chi.cpp:85-119, block j = iqn index
 template <int N, typename S>
 __global__ void __launch_bounds__(SLAB_THREADS, SlabOcc<S>::waves)
-    k_vertex_slab(const double* __restrict__ chi, S* V, double normFact, double U, double Usp, double Uch)
+    k_vertex_slab(const double* __restrict__ chi, S* V, double normFact, double U, double Usp, double Uch,
+                  const double* __restrict__ dsc)
 {
+    if (dsc) {
+        Usp = dsc[0];
+        Uch = dsc[1];
+    }
     constexpr int nk = N / 2 + 1;
     constexpr long nk8 = (long)nk * (nk + 1) / 2;
     extern __shared__ double smem[];
@@ -625,8 +661,9 @@ __global__ void __launch_bounds__(SLAB_THREADS, SlabOcc<S>::waves)
 template <int L, typename S>
 __global__ void __launch_bounds__(PEN_THREADS)
     k_vertex_pencil(S* V, const S* __restrict__ chirt, const double* __restrict__ chirt0, double* V0, long nk8,
-                    double fact, const cd* __restrict__ tw)
+                    double fact, const cd* __restrict__ tw, double U, const double* __restrict__ dsc)
 {
+    if (dsc) fact = vfact(U, dsc[0], dsc[1], 8.0);
     constexpr int H = L / 2;
     __shared__ cd X[L];
     const long q = 2 * (long)xcd_remap(blockIdx.x, gridDim.x);
@@ -674,8 +711,11 @@ template <int N>
 __global__ void __launch_bounds__(SLAB_THREADS)
     k_sigma_bnd(const double* __restrict__ ek, const double* __restrict__ er, const double* __restrict__ V0,
                 const double* __restrict__ dchirt0, const double* __restrict__ dGrt0p, double mu0, double T,
-                double fact, double* dSig, double* Sb, double* tmp)
+                double fact, double* dSig, double* Sb, double* tmp, const double* __restrict__ mu0p, double U,
+                const double* __restrict__ dsc)
 {
+    if (mu0p) mu0 = *mu0p;
+    if (dsc) fact = vfact(U, dsc[0], dsc[1], 8.0);
     constexpr int nk = N / 2 + 1;
     constexpr long nk8 = (long)nk * (nk + 1) / 2;
     const double nk02 = (double)N * N;
@@ -727,8 +767,10 @@ __global__ void __launch_bounds__(SLAB_THREADS)
 // Sigma(k,tau_j), j = 0..Nw (green.cpp:1822-1892): G0(k,tau) -> r, x V(r,|tau|), -> k
 template <int N, typename S>
 __global__ void __launch_bounds__(SLAB_THREADS, SlabOcc<S>::waves)
-    k_sigma_slab(const double* __restrict__ ek, const S* __restrict__ V, double mu0, double T, int Nw, S* Stau)
+    k_sigma_slab(const double* __restrict__ ek, const S* __restrict__ V, double mu0, double T, int Nw, S* Stau,
+                 const double* __restrict__ mu0p)
 {
+    if (mu0p) mu0 = *mu0p;
     constexpr int nk = N / 2 + 1;
     constexpr long nk8 = (long)nk * (nk + 1) / 2;
     const double nk02 = (double)N * N;
@@ -1088,8 +1130,11 @@ template <int N>
 __global__ void __launch_bounds__(SLAB_THREADS)
     k_hf0_bnd(const double* __restrict__ ek, const double* __restrict__ dek, const double* __restrict__ V0,
               const double* __restrict__ dchirt0, double mu0, double T, int Nw, double fact4, double* dhV,
-              double* df0V, double* hVb, double* f0Vb, double* bscr)
+              double* df0V, double* hVb, double* f0Vb, double* bscr, const double* __restrict__ mu0p, double U,
+              const double* __restrict__ dsc)
 {
+    if (mu0p) mu0 = *mu0p;
+    if (dsc) fact4 = vfact(U, dsc[0], dsc[1], 4.0);
     constexpr int nk = N / 2 + 1;
     constexpr long nq = (long)nk * (nk - 2);
     const double nf = 1.0 / ((double)N * N);
@@ -1294,8 +1339,10 @@ __device__ __forceinline__ bool pole_off(cd v)
 }
 
 __global__ void k_jj_poles(const double* __restrict__ ek, const double* __restrict__ hVb,
-                           const double* __restrict__ Sinfp, int N, double T, double mu, JJPoles* out)
+                           const double* __restrict__ Sinfp, int N, double T, double mu, JJPoles* out,
+                           const double* __restrict__ dsc)
 {
+    if (dsc) mu = dsc[2];
     const int nk = N / 2 + 1;
     const long nq = (long)nk * (nk - 2);
     const long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -1413,8 +1460,10 @@ __global__ void __launch_bounds__(NT, (L > 2048 ? 1 : JJ_WAVES_PER_SIMD))
                 const double* __restrict__ Sinf3, const double* __restrict__ Sinfp, int N, double T, double mu,
                 const cd* __restrict__ tw,
                 const cd* __restrict__ twX, const cd* __restrict__ twS, const cd* __restrict__ twM,
-                const cd* __restrict__ phT, double* part, double* scal, const int* __restrict__ bins, int ntheta)
+                const cd* __restrict__ phT, double* part, double* scal, const int* __restrict__ bins, int ntheta,
+                const double* __restrict__ dsc)
 {
+    if (dsc) mu = dsc[2];
     constexpr int H = L / 2;
     constexpr int NWB = H + 1;
     constexpr int CS = (L + 1 + NT - 1) / NT;  // spline chunk
@@ -1717,8 +1766,10 @@ __global__ void __launch_bounds__(NT, (L > 2048 ? 1 : JJ_WAVES_PER_SIMD))
 template <typename SelfT>
 __global__ void k_kx0_rows(const double* __restrict__ ek, const double* __restrict__ d2ek, const SelfT* __restrict__ Self,
                            const double* __restrict__ Sinf2, const double* __restrict__ Sinf3,
-                           const double* __restrict__ Sinfp, int N, int L, double T, double mu, double* out)
+                           const double* __restrict__ Sinfp, int N, int L, double T, double mu, double* out,
+                           const double* __restrict__ dsc)
 {
+    if (dsc) mu = dsc[2];
     __shared__ double sc[16];
     const int nk = N / 2 + 1;
     const long nk8 = (long)nk * (nk + 1) / 2;
@@ -1948,9 +1999,16 @@ hipError_t launch_sum_rule_part(int N, int Nw, double T, const double* chiw, con
 }
 
 hipError_t launch_halley_init(HalleyState* st, const HalleyState& s0, const double* chi0max, double* mirror,
-                              hipStream_t s)
+                              hipStream_t s, const double* mu0p, const double* a0src, double a0A, double nsr,
+                              double U)
 {
-    hipLaunchKernelGGL(k_halley_init, dim3(1), dim3(1), 0, s, st, s0, chi0max, mirror);
+    hipLaunchKernelGGL(k_halley_init, dim3(1), dim3(1), 0, s, st, s0, chi0max, mirror, mu0p, a0src, a0A, nsr, U);
+    return hipGetLastError();
+}
+
+hipError_t launch_save_root(const HalleyState* st, double* psc, int slot, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_save_root, dim3(1), dim3(64), 0, s, st, psc, slot);
     return hipGetLastError();
 }
 
@@ -1990,46 +2048,49 @@ hipError_t launch_mu0_root(const Grid& g, const DevArrays& d, double n, double p
     return hipGetLastError();
 }
 
-hipError_t launch_vertex_slab(const Grid& g, const DevArrays& d, double U, double Usp, double Uch, hipStream_t s)
+hipError_t launch_vertex_slab(const Grid& g, const DevArrays& d, double U, double Usp, double Uch, hipStream_t s,
+                              const double* dsc)
 {
     const double normFact = g.T / ((double)g.N * g.N);
     DISPATCH_S(g.prec, DISPATCH_N(g.N, {
         const size_t lds = slab_lds(g.N, false, sizeof(SS));
         set_lds(k_vertex_slab<NN, SS>, lds);
         hipLaunchKernelGGL((k_vertex_slab<NN, SS>), dim3(g.nwb), dim3(SLAB_THREADS), lds, s, d.chiw, (SS*)d.V,
-                           normFact, U, Usp, Uch);
+                           normFact, U, Usp, Uch, dsc);
     }));
     return hipGetLastError();
 }
 
-hipError_t launch_vertex_pencil(const Grid& g, const DevArrays& d, double U, double Usp, double Uch, hipStream_t s)
+hipError_t launch_vertex_pencil(const Grid& g, const DevArrays& d, double U, double Usp, double Uch, hipStream_t s,
+                                const double* dsc)
 {
     const double fact = U * (3.0 * Usp + Uch) / 8.0;
     DISPATCH_S(g.prec, DISPATCH_L(g.Nw, {
         hipLaunchKernelGGL((k_vertex_pencil<LL, SS>), dim3((unsigned)((g.nk8 + 1) / 2)), dim3(PEN_THREADS), 0, s,
-                           (SS*)d.V, (const SS*)d.chirt, d.chirt0, d.V0, g.nk8, fact, d.twN);
+                           (SS*)d.V, (const SS*)d.chirt, d.chirt0, d.V0, g.nk8, fact, d.twN, U, dsc);
     }));
     return hipGetLastError();
 }
 
-hipError_t launch_sigma_bnd(const Grid& g, const DevArrays& d, double mu0, double fact, hipStream_t s)
+hipError_t launch_sigma_bnd(const Grid& g, const DevArrays& d, double mu0, double fact, hipStream_t s,
+                            const double* mu0p, double U, const double* dsc)
 {
     const size_t lds = slab_lds(g.N, false, sizeof(double));
     DISPATCH_N(g.N, {
         set_lds(k_sigma_bnd<NN>, lds);
         hipLaunchKernelGGL(k_sigma_bnd<NN>, dim3(2), dim3(SLAB_THREADS), lds, s, d.ek, d.er, d.V0, d.dchirt0, d.dGrt0p,
-                           mu0, g.T, fact, d.dSig, d.Sb, d.tmp);
+                           mu0, g.T, fact, d.dSig, d.Sb, d.tmp, mu0p, U, dsc);
     });
     return hipGetLastError();
 }
 
-hipError_t launch_sigma_slab(const Grid& g, const DevArrays& d, double mu0, hipStream_t s)
+hipError_t launch_sigma_slab(const Grid& g, const DevArrays& d, double mu0, hipStream_t s, const double* mu0p)
 {
     DISPATCH_S(g.prec, DISPATCH_N(g.N, {
         const size_t lds = slab_lds(g.N, false, sizeof(SS));
         set_lds(k_sigma_slab<NN, SS>, lds);
         hipLaunchKernelGGL((k_sigma_slab<NN, SS>), dim3(g.Nw + 1), dim3(SLAB_THREADS), lds, s, d.ek, (const SS*)d.V,
-                           mu0, g.T, g.Nw, (SS*)d.Stau);
+                           mu0, g.T, g.Nw, (SS*)d.Stau, mu0p);
     }));
     return hipGetLastError();
 }
@@ -2092,13 +2153,14 @@ hipError_t launch_hf0_slab(const Grid& g, const DevArrays& d, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_hf0_bnd(const Grid& g, const DevArrays& d, double mu0, double fact4, hipStream_t s)
+hipError_t launch_hf0_bnd(const Grid& g, const DevArrays& d, double mu0, double fact4, hipStream_t s,
+                          const double* mu0p, double U, const double* dsc)
 {
     const size_t lds = slab_lds(g.N, true, sizeof(double));
     DISPATCH_N(g.N, {
         set_lds(k_hf0_bnd<NN>, lds);
         hipLaunchKernelGGL(k_hf0_bnd<NN>, dim3(4), dim3(SLAB_THREADS), lds, s, d.ek, d.dek, d.V0, d.dchirt0, mu0, g.T,
-                           g.Nw, fact4, d.dhV, d.df0V, d.hVb, d.f0Vb, d.bscr);
+                           g.Nw, fact4, d.dhV, d.df0V, d.hVb, d.f0Vb, d.bscr, mu0p, U, dsc);
     });
     return hipGetLastError();
 }
@@ -2120,10 +2182,10 @@ hipError_t launch_hVf0V_slab(const Grid& g, const DevArrays& d, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_jj_poles(const Grid& g, const DevArrays& d, double mu, hipStream_t s)
+hipError_t launch_jj_poles(const Grid& g, const DevArrays& d, double mu, hipStream_t s, const double* dsc)
 {
     hipLaunchKernelGGL(k_jj_poles, dim3((unsigned)((g.nq + 127) / 128)), dim3(128), 0, s, d.ek, d.hVb, d.Sinf, g.N,
-                       g.T, mu, (JJPoles*)d.poles);
+                       g.T, mu, (JJPoles*)d.poles, dsc);
     return hipGetLastError();
 }
 
@@ -2138,7 +2200,7 @@ hipError_t launch_jj_transpose(const Grid& g, const DevArrays& d, hipStream_t s)
 }
 
 hipError_t launch_jj_pencil(const Grid& g, const DevArrays& d, double mu, double* part, const int* bins, int ntheta,
-                            hipStream_t s)
+                            hipStream_t s, const double* dsc)
 {
     const unsigned nblk = (unsigned)((g.nq + jj_points(g.Nw) - 1) / jj_points(g.Nw));
     DISPATCH_S(g.prec, DISPATCH_LJJ(g.Nw, {
@@ -2146,7 +2208,7 @@ hipError_t launch_jj_pencil(const Grid& g, const DevArrays& d, double mu, double
         hipLaunchKernelGGL((k_jj_pencil<LL, NTJ, SS, ST>), dim3(nblk), dim3(NTJ), 0, s, d.ek, d.dek,
                            d.d2ek, (const SS*)d.hT, (const SS*)d.f0T, d.dhV, d.df0V, d.hVb, d.f0Vb,
                            (const JJPoles*)d.poles, (const ST*)d.Self, d.Sinf2, d.Sinf3, d.Sinf, g.N, g.T, mu, d.twN,
-                           d.twX, d.twS, d.twM, d.ph, part, d.jjscal, bins, ntheta);
+                           d.twX, d.twS, d.twM, d.ph, part, d.jjscal, bins, ntheta, dsc);
     }));
     return hipGetLastError();
 }
@@ -2214,11 +2276,11 @@ hipError_t launch_sum_rows(const double* part, int nrows, int n, double* out, hi
     return hipGetLastError();
 }
 
-hipError_t launch_kx0_rows(const Grid& g, const DevArrays& d, double mu, hipStream_t s)
+hipError_t launch_kx0_rows(const Grid& g, const DevArrays& d, double mu, hipStream_t s, const double* dsc)
 {
     DISPATCH_S(g.prec, {
         hipLaunchKernelGGL(k_kx0_rows<ST>, dim3(2 * g.nk), dim3(256), 0, s, d.ek, d.d2ek, (const ST*)d.Self, d.Sinf2,
-                           d.Sinf3, d.Sinf, g.N, g.Nw, g.T, mu, d.kx0part);
+                           d.Sinf3, d.Sinf, g.N, g.Nw, g.T, mu, d.kx0part, dsc);
     });
     return hipGetLastError();
 }

@@ -64,6 +64,7 @@ struct DevArrays {
     cd* ph = nullptr;            // Nw+1
     int* imax = nullptr;         // argmax result
     double* mu0v = nullptr;      // 1  mu0 of the current grid (k_mu0_root), read by the chi0 and h/f0 kernels
+    double* psc = nullptr;       // 32 scalars of a queued solve (tpsc_capi.cpp, queued_solve)
 };
 
 // Aslamazov-Larkin work arrays (FP64 in both precisions; allocated at the first chi3 call).
@@ -168,8 +169,13 @@ hipError_t launch_sum_rule_part(int N, int Nw, double T, const double* chiw, con
 // state, so a pass depends on the point only through d.root.  mirror (mapped host memory)
 // holds {status, root, x, passes, f, f', lo, hi}: "running" from the initialisation, the
 // outcome once the search has settled.
+// mu0p (device, may be null): the state's mu0; a0src (device Usp, may be null): the charge sum
+// rule's constant a0 = a0A - (nsr + Usp nsr^2/(2U) - nsr^2), formed on the device
 hipError_t launch_halley_init(HalleyState* st, const HalleyState& s0, const double* chi0max, double* mirror,
-                              hipStream_t s);
+                              hipStream_t s, const double* mu0p = nullptr, const double* a0src = nullptr,
+                              double a0A = 0.0, double nsr = 0.0, double U = 0.0);
+// psc[slot], psc[3 + slot], psc[6 + slot] = root, status, evaluations of the search in st
+hipError_t launch_save_root(const HalleyState* st, double* psc, int slot, hipStream_t s);
 hipError_t launch_sum_rule3_pass(const Grid& g, const DevArrays& d, double sgn, double* mirror, hipStream_t s);
 hipError_t launch_trace_diffG3_pass(const Grid& g, const DevArrays& d, double* mirror, hipStream_t s);
 // green0::find_mu on the device (one workgroup, the whole bracketing search): out = {found, mu0, evaluations}
@@ -177,29 +183,37 @@ hipError_t launch_mu0_root(const Grid& g, const DevArrays& d, double n, double p
 hipError_t launch_model(const Grid& g, const DevArrays& d, double t, double tp, double tpp, double t3,
                         hipStream_t s);
 hipError_t launch_reduce_cols(const double* part, int nrows, int ncols, double* out, hipStream_t s);
-hipError_t launch_vertex_slab(const Grid& g, const DevArrays& d, double U, double Usp, double Uch, hipStream_t s);
-hipError_t launch_vertex_pencil(const Grid& g, const DevArrays& d, double U, double Usp, double Uch, hipStream_t s);
-hipError_t launch_sigma_bnd(const Grid& g, const DevArrays& d, double mu0, double fact, hipStream_t s);
-hipError_t launch_sigma_slab(const Grid& g, const DevArrays& d, double mu0, hipStream_t s);
+// Optional device sources of the point scalars (the queued solve of tpsc_capi.cpp; null: the values
+// passed): dsc = {Usp, Uch, mu} (device), from which the kernels form U(3Usp+Uch)/8 and /4 with the
+// host's rounding; mu0p = the device mu0.
+hipError_t launch_vertex_slab(const Grid& g, const DevArrays& d, double U, double Usp, double Uch, hipStream_t s,
+                              const double* dsc = nullptr);
+hipError_t launch_vertex_pencil(const Grid& g, const DevArrays& d, double U, double Usp, double Uch, hipStream_t s,
+                                const double* dsc = nullptr);
+hipError_t launch_sigma_bnd(const Grid& g, const DevArrays& d, double mu0, double fact, hipStream_t s,
+                            const double* mu0p = nullptr, double U = 0.0, const double* dsc = nullptr);
+hipError_t launch_sigma_slab(const Grid& g, const DevArrays& d, double mu0, hipStream_t s,
+                             const double* mu0p = nullptr);
 hipError_t launch_sigma_pencil(const Grid& g, const DevArrays& d, hipStream_t s);
 hipError_t launch_trace_diffG(const Grid& g, const DevArrays& d, double pmu, double mu0, double* out,
                               hipStream_t s);
 hipError_t launch_trace_selfG(const Grid& g, const DevArrays& d, double mu0, double mu, int which, double* out,
                               hipStream_t s);
 hipError_t launch_hf0_slab(const Grid& g, const DevArrays& d, hipStream_t s);
-hipError_t launch_hf0_bnd(const Grid& g, const DevArrays& d, double mu0, double fact4, hipStream_t s);
+hipError_t launch_hf0_bnd(const Grid& g, const DevArrays& d, double mu0, double fact4, hipStream_t s,
+                          const double* mu0p = nullptr, double U = 0.0, const double* dsc = nullptr);
 hipError_t launch_hVf0V_slab(const Grid& g, const DevArrays& d, hipStream_t s);
-hipError_t launch_jj_poles(const Grid& g, const DevArrays& d, double mu, hipStream_t s);
+hipError_t launch_jj_poles(const Grid& g, const DevArrays& d, double mu, hipStream_t s, const double* dsc = nullptr);
 hipError_t launch_jj_transpose(const Grid& g, const DevArrays& d, hipStream_t s);
 // partial rows to part (nblk x ntheta x 2*nwb); bins: the angle bin of each Lambda point, or null
 hipError_t launch_jj_pencil(const Grid& g, const DevArrays& d, double mu, double* part, const int* bins, int ntheta,
-                            hipStream_t s);
+                            hipStream_t s, const double* dsc = nullptr);
 // out[i] = (float)in[i], i < n
 hipError_t launch_narrow(const double* in, float* out, long n, hipStream_t s);
 // out[r*ntheta + b]: the sum of row r of part (nrows x n) over the columns k with bins[k] == b
 hipError_t launch_sum_rows_binned(const double* part, int nrows, int n, const int* bins, int ntheta, double* out,
                                   hipStream_t s);
-hipError_t launch_kx0_rows(const Grid& g, const DevArrays& d, double mu, hipStream_t s);
+hipError_t launch_kx0_rows(const Grid& g, const DevArrays& d, double mu, hipStream_t s, const double* dsc = nullptr);
 hipError_t launch_finalize(const double* part, int n, double* out, hipStream_t s);
 
 constexpr int kGlocChunks = 64;  // k chunks of k_gloc (partials: kGlocChunks x Nw doubles)
