@@ -256,13 +256,17 @@ typedef struct tpsc_batch_result {
  * (a search unsettled after 8 passes, a failed mu0 search, a mixed-precision point that needs
  * the FP64 chi0 stage).  workers >= 2: host threads, each driving its own contexts and
  * streams with tpsc_set_params + tpsc_solve, so that the points' kernels interleave on the
- * GPU.  workers = 0: chosen by tpsc_batch_tune.
+ * GPU.  workers = 0: chosen by tpsc_batch_tune.  The queued path keeps its contexts (one per
+ * grid and precision) and its pinned result slots from call to call; tpsc_batch_release frees
+ * them.  Calls on the queued path from several host threads run one after another.
  * chijj (optional, npts*ldc doubles): Lambda_jj1 + Lambda_jj2 at n = 0..min(Nw0/2, ldc-1). */
 int tpsc_solve_batch(const tpsc_params* pts, int npts, int workers, tpsc_batch_result* out, double* chijj,
                      int ldc);
 /* times the solve of the 8 largest points at 1, 2, 4 and 8 workers (points/s in rates[4]
  * if not NULL) and returns the fastest worker count */
 int tpsc_batch_tune(const tpsc_params* pts, int npts, int* workers, double* rates);
+/* frees the contexts and result slots the queued batch path keeps between calls */
+int tpsc_batch_release(void);
 /* npts random points drawn from std::mt19937_64(seed) in the order N in {32,48,64,96,128},
  * T ~ U[0.05,0.5], n ~ U[0.8,1], U ~ U[1,6], t' ~ U[-0.3,0]; Nw the smallest power of two
  * with (Nw-1) pi T >= 20; FP64, no coarse pass. */

@@ -61,6 +61,13 @@ def test_queued_batch_equals_threaded_batch(precision, cap, monkeypatch):
         monkeypatch.setenv("TPSC_QUEUE_MAX_PASSES", str(cap))
     r1, j1 = capi.solve_batch(pts, workers=1, ldc=129)
     monkeypatch.delenv("TPSC_QUEUE_MAX_PASSES", raising=False)
+    if not cap:  # the queued path's contexts persist between calls; released, they are made again
+        capi.batch_release()
+        r2, j2 = capi.solve_batch(pts, workers=1, ldc=129)
+        r2b, j2b = capi.solve_batch(pts, workers=1, ldc=129)
+        for r in (r2, r2b):
+            assert all(getattr(a, k) == getattr(b, k) for a, b in zip(r1, r) for k in SCALARS)
+        assert np.array_equal(j1, j2) and np.array_equal(j1, j2b)
     r3, j3 = capi.solve_batch(pts, workers=3, ldc=129)
     queued = sum(r.queued for r in r1)
     print(f"[batch] precision {precision}, passes capped at {cap}: {queued} of 64 points finished on the "

@@ -63,7 +63,7 @@ EXPORTS = [
     "tpsc_keep_chi_tau", "tpsc_device_bytes", "tpsc_pade_coef", "tpsc_pade_eval", "tpsc_pade_conductivity",
     "tpsc_get_params", "tpsc_write_files", "tpsc_gwn_local", "tpsc_fix_mu0", "tpsc_sigma_loss_forward",
     "tpsc_sigma_loss_backward", "tpsc_loss_forward", "tpsc_loss_backward", "tpsc_random_weights", "tpsc_sp_corr_length", "tpsc_solve_batch",
-    "tpsc_batch_tune", "tpsc_batch_points", "tpsc_set_params", "tpsc_find_zero", "tpsc_sum_rule_noalloc",
+    "tpsc_batch_tune", "tpsc_batch_release", "tpsc_batch_points", "tpsc_set_params", "tpsc_find_zero", "tpsc_sum_rule_noalloc",
     "tpsc_mu0_infinite", "tpsc_fermi_surface", "tpsc_fs_lattice", "tpsc_mdc", "tpsc_edc", "tpsc_self_real_freq",
     "tpsc_save_self_bin", "tpsc_save_chi0_bin", "tpsc_read_self_bin", "tpsc_read_chi0_bin",
 ]
@@ -124,6 +124,7 @@ def lib():
     L.tpsc_sp_corr_length.argtypes = [vp, dp]
     L.tpsc_solve_batch.argtypes = [C.POINTER(Params), C.c_int, C.c_int, C.POINTER(BatchResult), dp, C.c_int]
     L.tpsc_batch_tune.argtypes = [C.POINTER(Params), C.c_int, ip, dp]
+    L.tpsc_batch_release.argtypes = []
     L.tpsc_batch_points.argtypes = [C.c_ulonglong, C.c_int, C.POINTER(Params)]
     L.tpsc_set_params.argtypes = [vp, C.POINTER(Params)]
     L.tpsc_find_zero.argtypes = [ROOT_FN, vp, dp, dp, dp, ip]
@@ -299,6 +300,11 @@ def solve_batch(pts, workers=0, ldc=0):
     if rc:
         raise TPSCError(f"tpsc_solve_batch failed ({ERRORS.get(rc, rc)})")
     return res, jj
+
+
+def batch_release():
+    """tpsc_batch_release: frees the contexts the queued batch path keeps between calls."""
+    lib().tpsc_batch_release()
 
 
 def batch_tune(pts):

@@ -4,10 +4,11 @@
 // (tpsc_queue_solve: the root searches read and write device memory only), one context per grid,
 // its results copied into a pinned slot of its own; the host then drains the streams and reads
 // the slots.  Small points (N <= 128) are launch- and round-trip-bound, so queuing them back to
-// back keeps the GPU busy where one point at a time leaves it mostly idle.  Mixed-precision
-// points that need the FP64 chi0 stage are queued a second time with it, as tpsc_solve redoes
-// them; a point that the queue cannot finish as tpsc_solve would (a root search unsettled after
-// its passes, a mu0 search without a root) is solved again with tpsc_solve.
+// back keeps the GPU busy where one point at a time leaves it mostly idle.  A point whose root
+// search has not settled in the queued passes is queued again with longer searches, a
+// mixed-precision point that needs the FP64 chi0 stage again with it (as tpsc_solve redoes
+// it); one that the queue cannot finish as tpsc_solve would (a failed search, a mu0 search
+// without a root) is solved again with tpsc_solve.
 // Several workers: host threads, each with one context per grid re-targeted point by point
 // (tpsc_set_params) before tpsc_solve.  Points are taken in decreasing cost N^2 Nw (longest
 // first).  The worker count is either given or chosen by tpsc_batch_tune, which times a probe
@@ -20,6 +21,7 @@
 #include <map>
 #include <numeric>
 #include <random>
+#include <mutex>
 #include <thread>
 #include <tuple>
 #include <vector>
@@ -108,12 +110,30 @@ std::vector<int> by_cost(const tpsc_params* pts, int n)
     return order;
 }
 
-// one host thread: queue every point, drain, read the slots; queue the mixed-precision points
-// that need the FP64 chi0 stage again with it; solve what is left with tpsc_solve
+// one host thread: queue every point, drain, read the slots; queue again, with longer root
+// searches or the FP64 chi0 stage, the points that need them; solve what is left with tpsc_solve
+// The queued path's contexts (one per grid and precision) and its pinned slots, kept across
+// calls: a context costs ~1-2 ms of allocations and stream set-up, more than a small point's
+// solve.  Calls on the queued path take turns on the mutex; tpsc_batch_release frees them.
+struct QueuedState {
+    Pool pool;
+    double* slots = nullptr;  // pinned, mapped host memory
+    double* slots_dev = nullptr;
+    size_t cap = 0;
+    ~QueuedState()
+    {
+        if (slots) (void)hipHostFree(slots);
+    }
+};
+std::mutex g_queued_mu;
+QueuedState* g_queued = nullptr;  // never freed at exit: the HIP runtime may be gone by then
+
 void run_queued(const tpsc_params* pts, const std::vector<int>& order, tpsc_batch_result* out, double* jj, int ldc)
 {
     const double t0 = now_s();
-    Pool pool;
+    std::lock_guard<std::mutex> lock(g_queued_mu);
+    if (!g_queued) g_queued = new QueuedState;
+    Pool& pool = g_queued->pool;
     const size_t n = order.size();
     std::vector<size_t> off(n + 1, 0);
     std::vector<tpsc_ctx*> ctxs(n, nullptr);
@@ -123,31 +143,60 @@ void run_queued(const tpsc_params* pts, const std::vector<int>& order, tpsc_batc
         if (pool.ctx_for(pts[i], &ctxs[k])) ctxs[k] = nullptr;
         off[k + 1] = off[k] + (ctxs[k] ? tpsc_queue_slot_doubles(ctxs[k]) : 0);
     }
-    double* slots = nullptr;
-    if (off[n] && hipHostMalloc((void**)&slots, off[n] * sizeof(double), hipHostMallocDefault) != hipSuccess)
-        slots = nullptr;
-    std::vector<size_t> todo, redo;
-    for (size_t k = 0; k < n; k++) (slots && ctxs[k] ? todo : redo).push_back(k);
+    // the slots: pinned host memory that the gather kernels write through its device address
+    QueuedState& q = *g_queued;
+    if (off[n] > q.cap) {
+        if (q.slots) (void)hipHostFree(q.slots);
+        q.slots = q.slots_dev = nullptr;
+        q.cap = 0;
+        if (hipHostMalloc((void**)&q.slots, off[n] * sizeof(double), hipHostMallocMapped) == hipSuccess &&
+            hipHostGetDevicePointer((void**)&q.slots_dev, q.slots, 0) == hipSuccess)
+            q.cap = off[n];
+        else if (q.slots) {
+            (void)hipHostFree(q.slots);
+            q.slots = nullptr;
+        }
+    }
+    double *slots = q.cap ? q.slots : nullptr, *slots_dev = q.cap ? q.slots_dev : nullptr;
+    struct Item {
+        size_t k;
+        bool fp64, longer;
+    };
+    std::vector<Item> todo;
+    std::vector<size_t> redo;
+    for (size_t k = 0; k < n; k++)
+        if (slots && ctxs[k])
+            todo.push_back({k, false, false});
+        else
+            redo.push_back(k);
     size_t done = 0;
-    for (const bool fp64 : {false, true}) {
-        std::vector<int> qrc(n, TPSC_E_INVALID);
-        for (size_t k : todo) qrc[k] = tpsc_queue_solve(ctxs[k], &pts[order[k]], slots + off[k], fp64);
+    while (!todo.empty()) {  // each item's flags only turn on: at most three rounds
+        std::vector<int> qrc(todo.size());
+        for (size_t t = 0; t < todo.size(); t++) {
+            const Item& it = todo[t];
+            qrc[t] = tpsc_queue_solve(ctxs[it.k], &pts[order[it.k]], slots_dev + off[it.k], it.fp64, it.longer);
+        }
         for (auto& kv : pool.ctx) tpsc_queue_sync(kv.second);
-        std::vector<size_t> promote;
-        for (size_t k : todo) {
-            const int i = order[k];
-            const int rc = qrc[k] == TPSC_OK ? tpsc_queue_result(&pts[i], slots + off[k], fp64, &out[i],
+        std::vector<Item> next;
+        for (size_t t = 0; t < todo.size(); t++) {
+            Item it = todo[t];
+            const int i = order[it.k];
+            const int rc = qrc[t] == TPSC_OK ? tpsc_queue_result(&pts[i], slots + off[it.k], it.fp64, &out[i],
                                                                  jj ? jj + (size_t)i * ldc : nullptr, ldc)
-                                             : qrc[k];
+                                             : qrc[t];
             if (rc == TPSC_OK) {
                 out[i].queued = 1;
                 done++;
-            } else if (rc == kQueuePromote)
-                promote.push_back(k);
-            else
-                redo.push_back(k);
+            } else if (rc == kQueuePromote) {
+                it.fp64 = true;
+                next.push_back(it);
+            } else if (rc == kQueueLonger && !it.longer) {
+                it.longer = true;
+                next.push_back(it);
+            } else
+                redo.push_back(it.k);
         }
-        todo.swap(promote);
+        todo.swap(next);
     }
     const double per = done ? (now_s() - t0) / done : 0.0;
     for (size_t k = 0; k < n; k++)
@@ -156,7 +205,6 @@ void run_queued(const tpsc_params* pts, const std::vector<int>& order, tpsc_batc
         const int i = order[k];
         solve_point(pool, pts[i], out[i], jj ? jj + (size_t)i * ldc : nullptr, ldc);
     }
-    if (slots) (void)hipHostFree(slots);
 }
 
 void run(const tpsc_params* pts, const std::vector<int>& order, int workers, tpsc_batch_result* out, double* jj,
@@ -216,6 +264,14 @@ extern "C" int tpsc_solve_batch(const tpsc_params* pts, int npts, int workers, t
         if (rc) return rc;
     }
     run(pts, by_cost(pts, npts), std::min(workers, npts), out, chijj, ldc);
+    return TPSC_OK;
+}
+
+extern "C" int tpsc_batch_release(void)
+{
+    std::lock_guard<std::mutex> lock(g_queued_mu);
+    delete g_queued;
+    g_queued = nullptr;
     return TPSC_OK;
 }
 

@@ -1493,8 +1493,10 @@ int tpsc_create(const tpsc_params* p, tpsc_ctx** out)
     rc |= dalloc(c, &d.ph, (size_t)Nw + 1);
     rc |= dalloc(c, &d.imax, 4);
     rc |= dalloc(c, &d.mu0v, 1);
-    rc |= dalloc(c, &d.psc, 32);
+    rc |= dalloc(c, &d.psc, kPscDoubles);
+    rc |= dalloc(c, &d.ticket, 1);
     if (rc) return TPSC_E_NOMEM;
+    HIPCHK(c, hipMemsetAsync(d.ticket, 0, sizeof(unsigned), c->stream));
     // = c->mu0 before any search.  On the context's stream: the streams are non-blocking, so a
     // memset on the null stream could land after the mu0 search of set_grid has written mu0v
     // (seen as a batch point solved at mu0 = 0 while other host threads kept the null stream busy)
@@ -2123,15 +2125,17 @@ int tpsc_keep_chi_tau(tpsc_ctx* c, int on)
 }  // extern "C"
 
 // ------------------------------------------------------------------ queued solves (batches)
-// d.psc of a queued solve: [0..2] Usp, Uch, mu; [3..5] their search statuses; [6..8] their
-// evaluations; [9] chi0max; [10..12] the mu0 search {found, mu0, evaluations}; [16..23] the
-// root-step mirror (not read).  A slot holds psc, then the reduced Lambda rows (2 nwb) and the
-// pencil's four scalar sums, then the kx = 0, pi rows of the diamagnetic term (2 nk).
-size_t tpsc_queue_slot_doubles(const tpsc_ctx* c) { return 32 + 2 * (size_t)c->g.nwb + 4 + 2 * (size_t)c->g.nk; }
-
-int tpsc_queue_solve(tpsc_ctx* c, const tpsc_params* p, double* slot, bool chi0_fp64)
+// A slot holds d.psc (kPscDoubles), then the reduced Lambda rows (2 nwb) and the pencil's four
+// scalar sums, then the kx = 0, pi rows of the diamagnetic term (2 nk); k_queue_gather writes it
+// through the slot's device address.
+size_t tpsc_queue_slot_doubles(const tpsc_ctx* c)
 {
-    if (!c || !p || !slot) return TPSC_E_INVALID;
+    return kPscDoubles + 2 * (size_t)c->g.nwb + 4 + 2 * (size_t)c->g.nk;
+}
+
+int tpsc_queue_solve(tpsc_ctx* c, const tpsc_params* p, double* slot_dev, bool chi0_fp64, bool long_searches)
+{
+    if (!c || !p || !slot_dev) return TPSC_E_INVALID;
     if (p->nk0 != c->p.nk0 || p->Nw0 != c->p.Nw0 || p->precision != c->p.precision || p->coarse_pass ||
         !full_grid(c))
         return fail(c, TPSC_E_INVALID, "a queued solve keeps the context's grid and precision, without a coarse pass");
@@ -2150,10 +2154,15 @@ int tpsc_queue_solve(tpsc_ctx* c, const tpsc_params* p, double* slot, bool chi0_
     hipStream_t s = c->stream;
     double* psc = d.psc;
     const double U = p->U, n = p->n, nsr = c->nsr;
-    // eps(k), then mu0 (into d.mu0v, the search's outcome into psc[10..12]), as set_grid does
+    // Halley passes per search (TPSC_QUEUE_MAX_PASSES caps them: fewer points settle)
+    const char* cap_env = getenv("TPSC_QUEUE_MAX_PASSES");
+    const int cap = cap_env ? std::max(1, atoi(cap_env)) : 1 << 20;
+    const int* np = long_searches ? kQueuedPassesLong : kQueuedPasses;
+    const int nsp = std::min(np[0], cap), nch = std::min(np[1], cap), nmu = std::min(np[2], cap);
+    // eps(k), then mu0 (into d.mu0v, the search's outcome into psc), as set_grid does
     HIPCHK(c, launch_model(g, d, hop_t(c), p->tp, p->tpp, p->t3, s));
     const double preset = (n == 1.0 && p->tp == 0 && p->tpp == 0) ? 0.0 : std::nan("");
-    HIPCHK(c, launch_mu0_root(g, d, n, preset, psc + 10, s));
+    HIPCHK(c, launch_mu0_root(g, d, n, preset, psc + kPscMu0, s));
     // chi0(q,iqn), chi0(r,tau) and chi0max (stage_chi0), in FP64 for a promoted mixed point
     Grid gx = g;
     DevArrays dx = d;
@@ -2167,24 +2176,18 @@ int tpsc_queue_solve(tpsc_ctx* c, const tpsc_params* p, double* slot, bool chi0_
     HIPCHK(c, launch_chi0_tau0(gx, dx, s));
     HIPCHK(c, launch_chi0_pencil(gx, dx, s));
     if (c->chi0_fp64) HIPCHK(c, launch_narrow(c->chirt64, (float*)d.chirt, (long)g.nwb * g.nk8, s));
-    HIPCHK(c, launch_chi0max(g, d, s));
-    HIPCHK(c, hipMemcpyAsync(psc + 9, d.red, sizeof(double), hipMemcpyDeviceToDevice, s));
+    HIPCHK(c, launch_chi0max(g, d, s, psc + kPscChi0max));
     // Usp and Uch (stage_UspUch_grid): the same starts, brackets and host-known parts, with Uch's
-    // constant formed on the device from Usp
-    double* mirror = psc + 16;
-    // TPSC_QUEUE_MAX_PASSES caps the passes of every search (fewer points settle on the queue)
-    const char* cap_env = getenv("TPSC_QUEUE_MAX_PASSES");
-    const int cap = cap_env ? std::max(1, atoi(cap_env)) : kQueuedPassesUsp;
+    // constant formed on the device from Usp; each search settles into its own mirror in psc
     HalleyState s0 = halley_start(U / 2.0, 0.0, 0.0, 40, n - n * n / 2.0 - nsr, nsr * nsr / (2.0 * U));
     s0.T = g.T;
-    HIPCHK(c, launch_halley_init(d.root, s0, d.red, mirror, s));
-    for (int k = 0; k < std::min(kQueuedPassesUsp, cap); k++) HIPCHK(c, launch_sum_rule3_pass(g, d, -1.0, mirror, s));
-    HIPCHK(c, launch_save_root(d.root, psc, 0, s));
+    HIPCHK(c, launch_halley_init(d.root, s0, d.red, psc + kPscUsp, s));
+    for (int k = 0; k < nsp; k++) HIPCHK(c, launch_sum_rule3_pass(g, d, -1.0, psc + kPscUsp, s));
     s0 = halley_start(0.0, 0.0, 1000.0 * U, 40, 0.0, 0.0);
     s0.T = g.T;
-    HIPCHK(c, launch_halley_init(d.root, s0, nullptr, mirror, s, nullptr, psc + 0, n - n * n / 2.0, nsr, U));
-    for (int k = 0; k < std::min(kQueuedPassesUch, cap); k++) HIPCHK(c, launch_sum_rule3_pass(g, d, 1.0, mirror, s));
-    HIPCHK(c, launch_save_root(d.root, psc, 1, s));
+    HIPCHK(c, launch_halley_init(d.root, s0, nullptr, psc + kPscUch, s, nullptr, psc + kPscUsp + 1, n - n * n / 2.0,
+                                 nsr, U));
+    for (int k = 0; k < nch; k++) HIPCHK(c, launch_sum_rule3_pass(g, d, 1.0, psc + kPscUch, s));
     // vertex and self-energy (stage_vertex, stage_sigma)
     HIPCHK(c, launch_vertex_slab(g, d, U, 0.0, 0.0, s, psc));
     HIPCHK(c, launch_vertex_pencil(g, d, U, 0.0, 0.0, s, psc));
@@ -2194,9 +2197,8 @@ int tpsc_queue_solve(tpsc_ctx* c, const tpsc_params* p, double* slot, bool chi0_
     // mu (stage_mu): start 0 in the limits (-20, 20), mu0 from the device
     s0 = halley_start(0.0, -20.0, 20.0, 40, 0.0, 0.0);
     s0.T = g.T;
-    HIPCHK(c, launch_halley_init(d.root, s0, nullptr, mirror, s, d.mu0v));
-    for (int k = 0; k < std::min(kQueuedPassesMu, cap); k++) HIPCHK(c, launch_trace_diffG3_pass(g, d, mirror, s));
-    HIPCHK(c, launch_save_root(d.root, psc, 2, s));
+    HIPCHK(c, launch_halley_init(d.root, s0, nullptr, psc + kPscMu, s, d.mu0v));
+    for (int k = 0; k < nmu; k++) HIPCHK(c, launch_trace_diffG3_pass(g, d, psc + kPscMu, s));
     // Lambda_jj bubble + Maki-Thompson and the diamagnetic rows (stage_chijj, N_theta_k = 1)
     HIPCHK(c, launch_hf0_slab(g, d, s));
     HIPCHK(c, launch_hf0_bnd(g, d, 0.0, 0.0, s, d.mu0v, U, psc));
@@ -2208,10 +2210,7 @@ int tpsc_queue_solve(tpsc_ctx* c, const tpsc_params* p, double* slot, bool chi0_
     HIPCHK(c, launch_jj_pencil(g, d, 0.0, d.jjpart, nullptr, 1, s, psc));
     HIPCHK(c, launch_reduce_cols(d.jjpart, nblk, 2 * g.nwb, d.jjred, s));
     HIPCHK(c, launch_reduce_cols(d.jjscal, nblk, 4, d.jjred + 2 * g.nwb, s));
-    HIPCHK(c, hipMemcpyAsync(slot, psc, 32 * sizeof(double), hipMemcpyDeviceToHost, s));
-    HIPCHK(c, hipMemcpyAsync(slot + 32, d.jjred, (2 * (size_t)g.nwb + 4) * sizeof(double), hipMemcpyDeviceToHost, s));
-    HIPCHK(c, hipMemcpyAsync(slot + 32 + 2 * g.nwb + 4, d.kx0part, 2 * (size_t)g.nk * sizeof(double),
-                             hipMemcpyDeviceToHost, s));
+    HIPCHK(c, launch_queue_gather(g, d, slot_dev, s));
     return 0;
 }
 
@@ -2227,19 +2226,24 @@ int tpsc_queue_result(const tpsc_params* p, const double* slot, bool chi0_fp64, 
 {
     const int nwb = p->Nw0 / 2 + 1, nk = p->nk0 / 2 + 1;
     const double* psc = slot;
-    const bool roots = psc[3] == kRootConverged && psc[4] == kRootConverged && psc[5] == kRootConverged;
-    if (!roots || psc[10] == 0.0) return TPSC_E_NOROOT;
-    const double Usp = psc[0], chi0max = psc[9];
+    // a failed search (or mu0 without a root) goes to tpsc_solve's bracketed fallback; one still
+    // running when its passes ran out, to a queue with longer searches
+    for (const int k : {kPscUsp, kPscUch, kPscMu})
+        if (psc[k] == kRootFailed) return TPSC_E_NOROOT;
+    if (psc[kPscMu0] == 0.0) return TPSC_E_NOROOT;
+    for (const int k : {kPscUsp, kPscUch, kPscMu})
+        if (psc[k] != kRootConverged) return kQueueLonger;
+    const double Usp = psc[kPscUsp + 1], chi0max = psc[kPscChi0max];
     if (p->precision && !chi0_fp64 && 1.0 / (1.0 - Usp * chi0max / 2.0) > kPromoteEnhancement) return kQueuePromote;
-    const double* red = slot + 32;
+    const double* red = slot + kPscDoubles;
     const double* sc4 = red + 2 * nwb;
     const double* rows = sc4 + 4;
     const double U = p->U, nsr = (p->n > 1.0) ? 2.0 - p->n : p->n;
     r->status = TPSC_OK;
-    r->mu0 = psc[11];
+    r->mu0 = psc[kPscMu0 + 1];
     r->Usp = Usp;
-    r->Uch = psc[1];
-    r->mu = psc[2];
+    r->Uch = psc[kPscUch + 1];
+    r->mu = psc[kPscMu + 1];
     r->dblOcc = (p->n > 1.0) ? Usp / U * nsr * nsr / 4.0 + 1.0 - nsr : Usp / U * nsr * nsr / 4.0;
     double k0 = sc4[0];
     for (int i = 0; i < 2 * nk; i++) k0 += rows[i];

@@ -115,24 +115,12 @@ __global__ void k_halley_init(HalleyState* st, HalleyState s0, const double* chi
     mirror[3] = s.ne;
 }
 
-// the outcome of a settled (or exhausted) search into the queued solve's scalars:
-// psc[slot] = root, psc[3 + slot] = status, psc[6 + slot] = evaluations
-__global__ void k_save_root(const HalleyState* __restrict__ st, double* psc, int slot)
+// One Halley step after a pass, by one workgroup of 256 threads: the three partial sums of
+// part[k * n + i] (k = 0, 1, 2), each in the order of k_finalize_k, the update of st, and the
+// outcome in mirror once the search has settled.  A settled search is left as it is.
+__device__ __forceinline__ void halley_step_block(const double* part, int n, HalleyState* st, double* mirror,
+                                                  double* sc)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        psc[slot] = st->root;
-        psc[3 + slot] = st->status;
-        psc[6 + slot] = st->ne;
-    }
-}
-
-// One Halley step after a pass: the three partial sums of part[k * n + i] (k = 0, 1, 2), each
-// in the order of k_finalize_k, the update of st, and the outcome in mirror once the search has
-// settled.  A settled search is left as it is.
-__global__ void __launch_bounds__(256) k_halley_step(const double* __restrict__ part, int n, HalleyState* st,
-                                                     double* mirror)
-{
-    __shared__ double sc[3 * 4];
     const HalleyState s0 = *st;
     if (s0.status != kRootRunning) return;
     double v[3] = {0.0, 0.0, 0.0};
@@ -157,6 +145,26 @@ __global__ void __launch_bounds__(256) k_halley_step(const double* __restrict__ 
             mirror[7] = s.hi;
         }
     }
+}
+
+// True in the workgroup of a pass that finishes last, once every workgroup's partial sums
+// (stored by its thread 0) are visible to it; that workgroup resets the ticket for the next pass.
+// No workgroup waits for another: the others return at once.  Only thread 0 releases (its
+// part[] stores, written back from its XCD's L2 before the ticket) and only the last workgroup
+// acquires: a full fence in every workgroup would also invalidate each XCD's L2 while the
+// other workgroups still stream the pass's input from it.
+__device__ __forceinline__ bool last_workgroup(unsigned* ticket, unsigned nblk)
+{
+    __shared__ unsigned last;
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        last = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == nblk - 1;
+    }
+    __syncthreads();
+    if (!last) return false;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    if (threadIdx.x == 0) __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return true;
 }
 
 // ================================================================== chi0 slabs
@@ -423,7 +431,8 @@ __global__ void __launch_bounds__(PEN_THREADS)
 
 // chi0max and its wedge position (chi0.cpp:703-717): largest value, ties to the
 // larger packed index (the reference scans l, m downwards with a strict >).
-__global__ void k_chi0max(const double* __restrict__ chi, long nk8, double* out, int* imax)
+// out2 (may be null): a second copy of the maximum
+__global__ void k_chi0max(const double* __restrict__ chi, long nk8, double* out, int* imax, double* out2)
 {
     __shared__ double sv[1024];
     __shared__ long si[1024];
@@ -456,6 +465,7 @@ __global__ void k_chi0max(const double* __restrict__ chi, long nk8, double* out,
     if (threadIdx.x == 0) {
         out[0] = (si[0] >= 0) ? sv[0] : 0.0;
         imax[0] = (int)((si[0] >= 0) ? si[0] : 0);
+        if (out2) out2[0] = out[0];
     }
 }
 
@@ -504,11 +514,11 @@ __device__ __forceinline__ void sum_rule3_term(double c, double w, double sU, do
     acc[2] += w * 0.5 * c * c * c * r * r * r;
 }
 // Ux (and T) come from the root search st, so that a captured pass graph serves every point;
-// a settled search leaves nothing to do.
+// a settled search leaves nothing to do.  The workgroup that finishes last takes the Halley step.
 __global__ void __launch_bounds__(256)
     k_sum_rule3(const double* __restrict__ chi, const double* __restrict__ inf1, const double* __restrict__ inf2,
-                const double* __restrict__ wq, int nk8, int nwmax, double Nq, double sgn,
-                const HalleyState* __restrict__ st, double* part)
+                const double* __restrict__ wq, int nk8, int nwmax, double Nq, double sgn, HalleyState* st,
+                double* part, unsigned* ticket, double* mirror)
 {
     __shared__ double sc[3 * 4];
     if (st->status != kRootRunning) return;
@@ -539,6 +549,7 @@ __global__ void __launch_bounds__(256)
     const int nblk = gridDim.x * gridDim.y, b = blockIdx.y * gridDim.x + blockIdx.x;
     if (threadIdx.x == 0)
         for (int k = 0; k < 3; k++) part[k * nblk + b] = acc[k];
+    if (last_workgroup(ticket, nblk)) halley_step_block(part, nblk, st, mirror, sc);
 }
 
 // wedge points per thread in k_mu0_root: 16 waves hold the 8385 points of N = 256
@@ -629,8 +640,8 @@ __global__ void __launch_bounds__(SLAB_THREADS, SlabOcc<S>::waves)
                   const double* __restrict__ dsc)
 {
     if (dsc) {
-        Usp = dsc[0];
-        Uch = dsc[1];
+        Usp = dsc[kPscUsp + 1];
+        Uch = dsc[kPscUch + 1];
     }
     constexpr int nk = N / 2 + 1;
     constexpr long nk8 = (long)nk * (nk + 1) / 2;
@@ -663,7 +674,7 @@ __global__ void __launch_bounds__(PEN_THREADS)
     k_vertex_pencil(S* V, const S* __restrict__ chirt, const double* __restrict__ chirt0, double* V0, long nk8,
                     double fact, const cd* __restrict__ tw, double U, const double* __restrict__ dsc)
 {
-    if (dsc) fact = vfact(U, dsc[0], dsc[1], 8.0);
+    if (dsc) fact = vfact(U, dsc[kPscUsp + 1], dsc[kPscUch + 1], 8.0);
     constexpr int H = L / 2;
     __shared__ cd X[L];
     const long q = 2 * (long)xcd_remap(blockIdx.x, gridDim.x);
@@ -715,7 +726,7 @@ __global__ void __launch_bounds__(SLAB_THREADS)
                 const double* __restrict__ dsc)
 {
     if (mu0p) mu0 = *mu0p;
-    if (dsc) fact = vfact(U, dsc[0], dsc[1], 8.0);
+    if (dsc) fact = vfact(U, dsc[kPscUsp + 1], dsc[kPscUch + 1], 8.0);
     constexpr int nk = N / 2 + 1;
     constexpr long nk8 = (long)nk * (nk + 1) / 2;
     const double nk02 = (double)N * N;
@@ -935,13 +946,14 @@ __device__ __forceinline__ void trace_diffG3_term(double wn, double rS, double i
     acc[2] += w * (6.0 * a * b2 - 2.0 * a * a2) * r * r * r;
 }
 // pmu (and T, mu0) come from the root search st, so that a captured pass graph serves every
-// point; a settled search leaves nothing to do.
+// point; a settled search leaves nothing to do.  The workgroup that finishes last takes the
+// Halley step.
 template <typename SelfT>
 __global__ void __launch_bounds__(256)
     k_trace_diffG3(const double* __restrict__ ekw, const double* __restrict__ wq, const SelfT* __restrict__ Self,
                    const double* __restrict__ Sinf2, const double* __restrict__ Sinf3,
                    const double* __restrict__ Sinf4, const double* __restrict__ Sinf, int nk8, int Nw,
-                   const HalleyState* __restrict__ st, double* part)
+                   HalleyState* st, double* part, unsigned* ticket, double* mirror)
 {
     __shared__ double sc[3 * 4];
     if (st->status != kRootRunning) return;
@@ -982,6 +994,7 @@ __global__ void __launch_bounds__(256)
     block_sum_k<3>(acc, sc);
     if (threadIdx.x == 0)
         for (int k = 0; k < 3; k++) part[k * gridDim.x + blockIdx.x] = acc[k];
+    if (last_workgroup(ticket, gridDim.x)) halley_step_block(part, gridDim.x, st, mirror, sc);
 }
 
 // which = 0: Tr Sigma G0 main sum (green.cpp:3018-3051); 1: its Sigma_inf correction
@@ -1134,7 +1147,7 @@ __global__ void __launch_bounds__(SLAB_THREADS)
               const double* __restrict__ dsc)
 {
     if (mu0p) mu0 = *mu0p;
-    if (dsc) fact4 = vfact(U, dsc[0], dsc[1], 4.0);
+    if (dsc) fact4 = vfact(U, dsc[kPscUsp + 1], dsc[kPscUch + 1], 4.0);
     constexpr int nk = N / 2 + 1;
     constexpr long nq = (long)nk * (nk - 2);
     const double nf = 1.0 / ((double)N * N);
@@ -1342,7 +1355,7 @@ __global__ void k_jj_poles(const double* __restrict__ ek, const double* __restri
                            const double* __restrict__ Sinfp, int N, double T, double mu, JJPoles* out,
                            const double* __restrict__ dsc)
 {
-    if (dsc) mu = dsc[2];
+    if (dsc) mu = dsc[kPscMu + 1];
     const int nk = N / 2 + 1;
     const long nq = (long)nk * (nk - 2);
     const long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -1463,7 +1476,7 @@ __global__ void __launch_bounds__(NT, (L > 2048 ? 1 : JJ_WAVES_PER_SIMD))
                 const cd* __restrict__ phT, double* part, double* scal, const int* __restrict__ bins, int ntheta,
                 const double* __restrict__ dsc)
 {
-    if (dsc) mu = dsc[2];
+    if (dsc) mu = dsc[kPscMu + 1];
     constexpr int H = L / 2;
     constexpr int NWB = H + 1;
     constexpr int CS = (L + 1 + NT - 1) / NT;  // spline chunk
@@ -1769,7 +1782,7 @@ __global__ void k_kx0_rows(const double* __restrict__ ek, const double* __restri
                            const double* __restrict__ Sinfp, int N, int L, double T, double mu, double* out,
                            const double* __restrict__ dsc)
 {
-    if (dsc) mu = dsc[2];
+    if (dsc) mu = dsc[kPscMu + 1];
     __shared__ double sc[16];
     const int nk = N / 2 + 1;
     const long nk8 = (long)nk * (nk + 1) / 2;
@@ -1966,9 +1979,9 @@ hipError_t launch_chi0_pencil(const Grid& g, const DevArrays& d, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_chi0max(const Grid& g, const DevArrays& d, hipStream_t s)
+hipError_t launch_chi0max(const Grid& g, const DevArrays& d, hipStream_t s, double* out2)
 {
-    hipLaunchKernelGGL(k_chi0max, dim3(1), dim3(1024), 0, s, d.chiw, g.nk8, d.red, d.imax);
+    hipLaunchKernelGGL(k_chi0max, dim3(1), dim3(1024), 0, s, d.chiw, g.nk8, d.red, d.imax, out2);
     return hipGetLastError();
 }
 
@@ -2006,9 +2019,20 @@ hipError_t launch_halley_init(HalleyState* st, const HalleyState& s0, const doub
     return hipGetLastError();
 }
 
-hipError_t launch_save_root(const HalleyState* st, double* psc, int slot, hipStream_t s)
+// the results of a queued solve into its slot (pinned host memory, written through its device
+// address): d.psc, the reduced Lambda rows with the pencil's four scalar sums, the kx = 0, pi
+// rows of the diamagnetic term
+__global__ void __launch_bounds__(256) k_queue_gather(const double* __restrict__ psc, const double* __restrict__ jjred,
+                                                      int njj, const double* __restrict__ rows, int nrows, double* out)
 {
-    hipLaunchKernelGGL(k_save_root, dim3(1), dim3(64), 0, s, st, psc, slot);
+    for (int i = threadIdx.x; i < kPscDoubles + njj + nrows; i += blockDim.x)
+        out[i] = i < kPscDoubles ? psc[i] : i < kPscDoubles + njj ? jjred[i - kPscDoubles] : rows[i - kPscDoubles - njj];
+}
+
+hipError_t launch_queue_gather(const Grid& g, const DevArrays& d, double* out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_queue_gather, dim3(1), dim3(256), 0, s, d.psc, d.jjred, 2 * g.nwb + 4, d.kx0part, 2 * g.nk,
+                       out);
     return hipGetLastError();
 }
 
@@ -2017,8 +2041,7 @@ hipError_t launch_sum_rule3_pass(const Grid& g, const DevArrays& d, double sgn, 
     const double Nq = 4.0 * (g.nk - 1) * (g.nk - 1);
     const dim3 gr = red_grid(g);
     hipLaunchKernelGGL(k_sum_rule3, gr, dim3(256), 0, s, d.chiw, d.chi0inf, d.chi0inf2, d.wq, (int)g.nk8, g.Nw / 2,
-                       Nq, sgn, (const HalleyState*)d.root, d.part);
-    hipLaunchKernelGGL(k_halley_step, dim3(1), dim3(256), 0, s, d.part, (int)(gr.x * gr.y), d.root, mirror);
+                       Nq, sgn, d.root, d.part, d.ticket, mirror);
     return hipGetLastError();
 }
 
@@ -2123,9 +2146,8 @@ hipError_t launch_trace_diffG3_pass(const Grid& g, const DevArrays& d, double* m
     const dim3 gr = trace_grid(g);
     DISPATCH_S(g.prec, {
         hipLaunchKernelGGL(k_trace_diffG3<ST>, gr, dim3(256), 0, s, d.ekw, d.wq, (const ST*)d.Self, d.Sinf2, d.Sinf3,
-                           d.Sinf4, d.Sinf, (int)g.nk8, g.Nw, (const HalleyState*)d.root, d.part);
+                           d.Sinf4, d.Sinf, (int)g.nk8, g.Nw, d.root, d.part, d.ticket, mirror);
     });
-    hipLaunchKernelGGL(k_halley_step, dim3(1), dim3(256), 0, s, d.part, (int)(gr.x * gr.y), d.root, mirror);
     return hipGetLastError();
 }
 

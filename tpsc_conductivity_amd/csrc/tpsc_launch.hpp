@@ -64,8 +64,14 @@ struct DevArrays {
     cd* ph = nullptr;            // Nw+1
     int* imax = nullptr;         // argmax result
     double* mu0v = nullptr;      // 1  mu0 of the current grid (k_mu0_root), read by the chi0 and h/f0 kernels
-    double* psc = nullptr;       // 32 scalars of a queued solve (tpsc_capi.cpp, queued_solve)
+    double* psc = nullptr;       // kPscDoubles scalars of a queued solve (layout below)
+    unsigned* ticket = nullptr;  // 1  arrival count of a root-search pass (zero between passes)
 };
+
+// d.psc of a queued solve: the mirrors of the Usp, Uch and mu searches (8 doubles each: status,
+// root, x, evaluations, f, f', lo, hi; the root read by later stages at [kPsc* + 1]), chi0max,
+// and the mu0 search {found, mu0, evaluations}
+constexpr int kPscUsp = 0, kPscUch = 8, kPscMu = 16, kPscChi0max = 24, kPscMu0 = 25, kPscDoubles = 32;
 
 // Aslamazov-Larkin work arrays (FP64 in both precisions; allocated at the first chi3 call).
 // A plane is (Nw+1)*nq doubles; complex tau arrays are two planes (re, im).
@@ -155,7 +161,7 @@ hipError_t launch_fill_tw(const DevArrays& d, int L, hipStream_t s);
 hipError_t launch_chi0_tau0(const Grid& g, const DevArrays& d, hipStream_t s);
 hipError_t launch_chi0_slab(const Grid& g, const DevArrays& d, hipStream_t s);
 hipError_t launch_chi0_pencil(const Grid& g, const DevArrays& d, hipStream_t s);
-hipError_t launch_chi0max(const Grid& g, const DevArrays& d, hipStream_t s);
+hipError_t launch_chi0max(const Grid& g, const DevArrays& d, hipStream_t s, double* out2 = nullptr);
 hipError_t launch_sum_rule(const Grid& g, const DevArrays& d, double sU, double* out, hipStream_t s);
 // the partial sums of the sum-rule trace at sU over explicit arrays (chi0(q,iqn) nwb*nk8,
 // its two tail coefficients and the wedge weights, nk8 each): sum_rule_parts(N) doubles to part,
@@ -175,7 +181,6 @@ hipError_t launch_halley_init(HalleyState* st, const HalleyState& s0, const doub
                               hipStream_t s, const double* mu0p = nullptr, const double* a0src = nullptr,
                               double a0A = 0.0, double nsr = 0.0, double U = 0.0);
 // psc[slot], psc[3 + slot], psc[6 + slot] = root, status, evaluations of the search in st
-hipError_t launch_save_root(const HalleyState* st, double* psc, int slot, hipStream_t s);
 hipError_t launch_sum_rule3_pass(const Grid& g, const DevArrays& d, double sgn, double* mirror, hipStream_t s);
 hipError_t launch_trace_diffG3_pass(const Grid& g, const DevArrays& d, double* mirror, hipStream_t s);
 // green0::find_mu on the device (one workgroup, the whole bracketing search): out = {found, mu0, evaluations}
@@ -183,6 +188,7 @@ hipError_t launch_mu0_root(const Grid& g, const DevArrays& d, double n, double p
 hipError_t launch_model(const Grid& g, const DevArrays& d, double t, double tp, double tpp, double t3,
                         hipStream_t s);
 hipError_t launch_reduce_cols(const double* part, int nrows, int ncols, double* out, hipStream_t s);
+hipError_t launch_queue_gather(const Grid& g, const DevArrays& d, double* out, hipStream_t s);
 // Optional device sources of the point scalars (the queued solve of tpsc_capi.cpp; null: the values
 // passed): dsc = {Usp, Uch, mu} (device), from which the kernels form U(3Usp+Uch)/8 and /4 with the
 // host's rounding; mu0p = the device mu0.
