@@ -167,6 +167,14 @@ __device__ __forceinline__ bool last_workgroup(unsigned* ticket, unsigned nblk)
     return true;
 }
 
+// the Halley step as a kernel of its own, launched after a pass of more than kFusedStepMax
+// workgroups
+__global__ void __launch_bounds__(256) k_halley_step(const double* part, int n, HalleyState* st, double* mirror)
+{
+    __shared__ double sc[3 * 4];
+    halley_step_block(part, n, st, mirror, sc);
+}
+
 // ================================================================== chi0 slabs
 // G(k,t)/N^2, chi0.cpp:468-471 (form==0) and :535-538 (form==1, tau = beta/2)
 __device__ __forceinline__ double g0_plus(double x, double t, double T, double nk02, int form)
@@ -514,7 +522,8 @@ __device__ __forceinline__ void sum_rule3_term(double c, double w, double sU, do
     acc[2] += w * 0.5 * c * c * c * r * r * r;
 }
 // Ux (and T) come from the root search st, so that a captured pass graph serves every point;
-// a settled search leaves nothing to do.  The workgroup that finishes last takes the Halley step.
+// a settled search leaves nothing to do.  ticket != null: the workgroup that finishes last takes
+// the Halley step.
 __global__ void __launch_bounds__(256)
     k_sum_rule3(const double* __restrict__ chi, const double* __restrict__ inf1, const double* __restrict__ inf2,
                 const double* __restrict__ wq, int nk8, int nwmax, double Nq, double sgn, HalleyState* st,
@@ -549,7 +558,7 @@ __global__ void __launch_bounds__(256)
     const int nblk = gridDim.x * gridDim.y, b = blockIdx.y * gridDim.x + blockIdx.x;
     if (threadIdx.x == 0)
         for (int k = 0; k < 3; k++) part[k * nblk + b] = acc[k];
-    if (last_workgroup(ticket, nblk)) halley_step_block(part, nblk, st, mirror, sc);
+    if (ticket && last_workgroup(ticket, nblk)) halley_step_block(part, nblk, st, mirror, sc);
 }
 
 // wedge points per thread in k_mu0_root: 16 waves hold the 8385 points of N = 256
@@ -946,8 +955,8 @@ __device__ __forceinline__ void trace_diffG3_term(double wn, double rS, double i
     acc[2] += w * (6.0 * a * b2 - 2.0 * a * a2) * r * r * r;
 }
 // pmu (and T, mu0) come from the root search st, so that a captured pass graph serves every
-// point; a settled search leaves nothing to do.  The workgroup that finishes last takes the
-// Halley step.
+// point; a settled search leaves nothing to do.  ticket != null: the workgroup that finishes
+// last takes the Halley step.
 template <typename SelfT>
 __global__ void __launch_bounds__(256)
     k_trace_diffG3(const double* __restrict__ ekw, const double* __restrict__ wq, const SelfT* __restrict__ Self,
@@ -994,7 +1003,7 @@ __global__ void __launch_bounds__(256)
     block_sum_k<3>(acc, sc);
     if (threadIdx.x == 0)
         for (int k = 0; k < 3; k++) part[k * gridDim.x + blockIdx.x] = acc[k];
-    if (last_workgroup(ticket, gridDim.x)) halley_step_block(part, gridDim.x, st, mirror, sc);
+    if (ticket && last_workgroup(ticket, gridDim.x)) halley_step_block(part, gridDim.x, st, mirror, sc);
 }
 
 // which = 0: Tr Sigma G0 main sum (green.cpp:3018-3051); 1: its Sigma_inf correction
@@ -2040,8 +2049,11 @@ hipError_t launch_sum_rule3_pass(const Grid& g, const DevArrays& d, double sgn, 
 {
     const double Nq = 4.0 * (g.nk - 1) * (g.nk - 1);
     const dim3 gr = red_grid(g);
+    const int nblk = (int)(gr.x * gr.y);
+    const bool fused = nblk <= kFusedStepMax;
     hipLaunchKernelGGL(k_sum_rule3, gr, dim3(256), 0, s, d.chiw, d.chi0inf, d.chi0inf2, d.wq, (int)g.nk8, g.Nw / 2,
-                       Nq, sgn, d.root, d.part, d.ticket, mirror);
+                       Nq, sgn, d.root, d.part, fused ? d.ticket : nullptr, mirror);
+    if (!fused) hipLaunchKernelGGL(k_halley_step, dim3(1), dim3(256), 0, s, d.part, nblk, d.root, mirror);
     return hipGetLastError();
 }
 
@@ -2144,10 +2156,13 @@ hipError_t launch_trace_diffG(const Grid& g, const DevArrays& d, double pmu, dou
 hipError_t launch_trace_diffG3_pass(const Grid& g, const DevArrays& d, double* mirror, hipStream_t s)
 {
     const dim3 gr = trace_grid(g);
+    const int nblk = (int)gr.x;
+    const bool fused = nblk <= kFusedStepMax;
     DISPATCH_S(g.prec, {
         hipLaunchKernelGGL(k_trace_diffG3<ST>, gr, dim3(256), 0, s, d.ekw, d.wq, (const ST*)d.Self, d.Sinf2, d.Sinf3,
-                           d.Sinf4, d.Sinf, (int)g.nk8, g.Nw, d.root, d.part, d.ticket, mirror);
+                           d.Sinf4, d.Sinf, (int)g.nk8, g.Nw, d.root, d.part, fused ? d.ticket : nullptr, mirror);
     });
+    if (!fused) hipLaunchKernelGGL(k_halley_step, dim3(1), dim3(256), 0, s, d.part, nblk, d.root, mirror);
     return hipGetLastError();
 }
 

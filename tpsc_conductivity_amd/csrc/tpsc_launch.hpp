@@ -149,6 +149,12 @@ struct Grid {
 constexpr int kRedBlocks = 1024;
 constexpr int kRowGroups = 32;       // frequency-row groups of the reduction grids
 constexpr int kPartCap = 8192;       // capacity of DevArrays::part
+// Root-search passes over at most this many workgroups take their Halley step in the workgroup
+// that finishes last (one launch per pass: small grids are launch-bound); larger passes launch
+// the step kernel after them, since each workgroup's release fence (a write-back of its XCD's
+// L2) costs more there than the launch it saves (256^2: sum-rule passes of 1056 workgroups,
+// trace passes of 2097).
+constexpr int kFusedStepMax = 640;
 constexpr int kJJK = 16;  // k-points per Lambda workgroup for Nw > 256
 // Lambda pencil shape: up to Nw = 256 one wavefront per workgroup (its barriers are then free)
 // and 4 points per workgroup, so that small grids still spread over the CUs; beyond, 256
