@@ -152,7 +152,6 @@ __global__ void __launch_bounds__(AL_PEN_THREADS)
 {
     constexpr int NT = AL_PEN_THREADS;
     constexpr int H = L / 2;
-    constexpr int CS = (L + 1 + NT - 1) / NT;
     __shared__ double Y[L + 1];
     __shared__ cd X[L];
     __shared__ double ES[2 * NT + 8];
@@ -165,7 +164,10 @@ __global__ void __launch_bounds__(AL_PEN_THREADS)
     const double d0 = dhG[q], dN = dhG[nq + q];
     __syncthreads();
     double integ[1] = {0.0};
-    block_spline_inplace<CS, NT>(Y, L, h, d0, dN, ES, integ);
+    {
+        double* const ys[1] = {Y};
+        block_spline_fixed<L, NT, 1>(ys, h, &d0, &dN, ES, integ);
+    }
     const double tot = block_sum(integ[0], sc);
     for (int j = tid; j < L; j += NT) X[j] = cd{(Y[j + 1] - Y[j]) / h, 0.0};
     __syncthreads();
@@ -407,7 +409,6 @@ __global__ void __launch_bounds__(NT)
                   const cd* __restrict__ twX, const cd* __restrict__ twS, double* contrib)
 {
     constexpr int H = L / 2;
-    constexpr int CS = (L + 1 + NT - 1) / NT;
     __shared__ __align__(16) double Yb[4 * (L + 1)];
     __shared__ cd X[L];
     __shared__ double ES[4 * (2 * NT + 8)];
@@ -436,7 +437,7 @@ __global__ void __launch_bounds__(NT)
     __syncthreads();
     const cd cw1 = cd{Y[0][0] + Y[0][L], Y[1][0] + Y[1][L]};
     const cd cw2 = cd{Y[2][0] + Y[2][L], Y[3][0] + Y[3][L]};
-    block_spline_inplace_k<CS, NT, 4>(Y, L, hstep, d0, dN, ES, nullptr);
+    block_spline_fixed<L, NT, 4>(Y, hstep, d0, dN, ES, nullptr);
     const cd dd1 = cd{Y[0][0] + Y[0][L], Y[1][0] + Y[1][L]};  // z''(0) + z''(beta)
     const cd dd2 = cd{Y[2][0] + Y[2][L], Y[3][0] + Y[3][L]};
     const cd ds1 = cd{d0[0] + dN[0], d0[1] + dN[1]};

@@ -608,167 +608,197 @@ __device__ __forceinline__ void block_spline_moments(const double* y, double* M,
     __syncthreads();
 }
 
-// powers z^e, e = 0..2 KW, of the spline pole z = sqrt(3) - 2, by repeated multiplication
-struct ZPow {
-    double v[2 * KW + 1];
-};
-constexpr ZPow make_zpow()
+// z^e of the spline pole z = sqrt(3) - 2 by repeated multiplication, z^(N-1) by repeated
+// squaring, and the number of neighbouring chunks whose
+// carries count (|z^(C (d-1))| > 1e-19, at most maxd)
+constexpr double kSplineZ = 1.7320508075688772935 - 2.0;
+constexpr double spline_zpow(int e)
 {
-    ZPow t{};
-    const double z = 1.7320508075688772935 - 2.0;
-    t.v[0] = 1.0;
-    for (int e = 1; e <= 2 * KW; e++) t.v[e] = t.v[e - 1] * z;
-    return t;
+    double p = 1.0;
+    for (int i = 0; i < e; i++) p *= kSplineZ;
+    return p;
 }
-__constant__ constexpr ZPow kZPow = make_zpow();
-
-// The same clamped spline as block_spline_moments, in place, for K sequences of the same
-// length at once (their latencies overlap; one barrier sequence for all): yM[k] holds
-// y_0..y_N on entry and the moments M_0..M_N on exit.  Each thread keeps its chunk of the
-// right-hand side in registers, so no scratch array is needed.  The two first-order
-// recursions run once from a zero state to get the chunk-end states, and again after the
-// exchange, seeded with the carries of the neighbouring chunks.  C: largest chunk,
-// >= ceil((N+1)/NT).
-// trap (optional): trap[k] receives this thread's part of sum_j h (y_j + y_{j+1})/2
-// - h^3 (M_j + M_{j+1})/24, the integral of spline k over [0, N h]; the caller adds the parts
-// (block_sum).  ES: LDS scratch of K*(2*NT + 8) doubles.
-template <int C, int NT, int K>
-__device__ __forceinline__ void block_spline_inplace_k(double* const* yM, int N, double h, const double* d0,
-                                                       const double* dN, double* ES, double* trap)
+constexpr double spline_zpow_sq(int e)
 {
+    double r = 1.0, zz = kSplineZ;
+    while (e > 0) {
+        if (e & 1) r *= zz;
+        zz *= zz;
+        e >>= 1;
+    }
+    return r;
+}
+constexpr int spline_carry_terms(double zc, int maxd)
+{
+    int d = 0;
+    double zd = 1.0;
+    while (d < maxd && (zd > 1.0e-19 || zd < -1.0e-19)) {
+        d++;
+        zd *= zc;
+    }
+    return d;
+}
+
+// The same clamped spline as block_spline_moments, in place, for K sequences of length N (known
+// at compile time) at once, their latencies overlapping: yM[k] holds y_0..y_N on entry and the
+// moments M_0..M_N on exit.  Each thread keeps its chunk of the right-hand side in registers, so
+// no scratch array is needed.  The two first-order recursions run once from a zero state to get
+// the chunk-end states, and again after the exchange, seeded with the carries of the
+// neighbouring chunks.  The chunk layout is fixed at compile time, with almost no divergent
+// branches: the sequences are padded with
+// zeros past N (the anti-causal recursion then reaches N in the zero state it starts from, so
+// the moments do not change), the carries take their compile-time number of terms with
+// out-of-range neighbours read as zero, full chunks store without per-element tests, and only
+// the chunks within 2 KW of an end take the boundary modes.
+// trap (optional): trap[k] receives this thread's part of sum_j h (y_j + y_{j+1})/2
+// - h^3 (M_j + M_{j+1})/24, the integral of spline k over [0, N h]; the caller adds the parts.
+// ES: LDS scratch of K*(2*NT + 8) doubles.
+template <int N, int NT, int K>
+__device__ __forceinline__ void block_spline_fixed(double* const* yM, double h, const double* d0, const double* dN,
+                                                   double* ES, double* trap)
+{
+    constexpr int C = (N + 1 + NT - 1) / NT;
+    constexpr double z = kSplineZ;
+    constexpr double A = 0.28867513459481288225;
+    constexpr double zc = spline_zpow(C);
+    constexpr int D = spline_carry_terms(zc, NT - 1);
+    constexpr int tN = N / C, cN = N % C;                  // thread and slot of j = N
+    constexpr int t1 = 1 / C, c1 = 1 % C;                  // of j = 1
+    constexpr int tM = (N - 1) / C, cM = (N - 1) % C;      // of j = N - 1
+    constexpr double zN1 = spline_zpow_sq(N - 1);
+    constexpr double dg = 2.0 + z, off = zN1 * (1.0 + 2.0 * z);
+    constexpr double idet = 1.0 / (dg * dg - off * off);
     const int tid = threadIdx.x;
-    const double z = 1.7320508075688772935 - 2.0;
-    const double A = 0.28867513459481288225;
+    const int j0 = tid * C;
+    const bool full = j0 + C - 1 <= N;
     const double ih = 1.0 / h;
-    const int Cr = (N + 1 + NT - 1) / NT;
-    const int nchunk = (N + 1 + Cr - 1) / Cr;
-    const int j0 = tid * Cr, j1 = min(N + 1, j0 + Cr);
     double* bnd = ES + 2 * K * NT;  // per sequence: r_0, r_N, M_0, M_1, M_{N-1}, M_N (8 slots)
     double ri[K][C], u[K][C];
     double tr[K];
 #pragma unroll
     for (int k = 0; k < K; k++) {
         const double* y = yM[k];
+        double yv[C + 2];  // y[j0 - 1 .. j0 + C], indices clamped to [0, N]
+#pragma unroll
+        for (int i = 0; i < C + 2; i++) yv[i] = y[min(max(j0 - 1 + i, 0), N)];
         tr[k] = 0.0;
 #pragma unroll
         for (int c = 0; c < C; c++) {
             const int j = j0 + c;
-            double r = 0.0;
-            if (j < j1) {
-                if (j == 0) {
-                    bnd[8 * k + 0] = 6.0 * ih * ((y[1] - y[0]) * ih - d0[k]);
-                } else if (j == N) {
-                    bnd[8 * k + 1] = 6.0 * ih * (dN[k] - (y[N] - y[N - 1]) * ih);
-                } else
-                    r = 6.0 * ih * ih * (y[j + 1] - 2.0 * y[j] + y[j - 1]);
-                if (trap && j < N) tr[k] += h * (y[j] + y[j + 1]) * 0.5;
-            }
-            ri[k][c] = r;
+            const double r = 6.0 * ih * ih * (yv[c + 2] - 2.0 * yv[c + 1] + yv[c]);
+            ri[k][c] = (j >= 1 && j <= N - 1) ? r : 0.0;
+            if (trap && j < N) tr[k] += h * (yv[c + 1] + yv[c + 2]) * 0.5;
         }
+        if (tid == 0) bnd[8 * k + 0] = 6.0 * ih * ((yv[2] - yv[1]) * ih - d0[k]);
+        if (tid == tN) bnd[8 * k + 1] = 6.0 * ih * (dN[k] - (yv[cN + 1] - yv[cN]) * ih);
     }
     // chunk-end states of the two recursions from a zero start
 #pragma unroll
     for (int k = 0; k < K; k++) {
-        double eu = 0.0, ew = 0.0;
-        if (j0 < j1) {
-            double sa = 0.0;
+        double sa = 0.0;
 #pragma unroll
-            for (int c = 0; c < C; c++)
-                if (j0 + c < j1) sa = ri[k][c] + z * sa;
-            eu = sa;
-            sa = 0.0;
+        for (int c = 0; c < C; c++) sa = ri[k][c] + z * sa;
+        ES[2 * k * NT + tid] = sa;
+        sa = 0.0;
 #pragma unroll
-            for (int c = C - 1; c >= 0; c--)
-                if (j0 + c < j1) sa = ri[k][c] + z * sa;
-            ew = sa;
-        }
-        ES[2 * k * NT + tid] = eu;
-        ES[2 * k * NT + NT + tid] = ew;
+        for (int c = C - 1; c >= 0; c--) sa = ri[k][c] + z * sa;
+        ES[2 * k * NT + NT + tid] = sa;
     }
     __syncthreads();  // all reads of y are done
-    if (j0 < j1) {
-        double zc = 1.0;
-        for (int c = 0; c < Cr; c++) zc *= z;
 #pragma unroll
-        for (int k = 0; k < K; k++) {
-            double Su = 0.0, Sw = 0.0, zd = 1.0;
-            for (int d = 1; d <= nchunk && fabs(zd) > 1.0e-19; d++) {
-                if (tid - d >= 0) Su += zd * ES[2 * k * NT + tid - d];
-                if (tid + d < nchunk) Sw += zd * ES[2 * k * NT + NT + tid + d];
-                zd *= zc;
-            }
-            // the recursions again, seeded with the carries from the neighbouring chunks
-            double sa = Su;
+    for (int k = 0; k < K; k++) {
+        double Su = 0.0, Sw = 0.0, zd = 1.0;
+#pragma unroll
+        for (int d = 1; d <= D; d++) {
+            const double el = ES[2 * k * NT + max(tid - d, 0)];
+            const double er = ES[2 * k * NT + NT + min(tid + d, NT - 1)];
+            Su += zd * (tid - d >= 0 ? el : 0.0);
+            Sw += zd * (tid + d < NT ? er : 0.0);
+            zd *= zc;
+        }
+        // the recursions again, seeded with the carries from the neighbouring chunks
+        double sa = Su;
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+            sa = ri[k][c] + z * sa;
+            u[k][c] = sa;
+        }
+        sa = Sw;
+#pragma unroll
+        for (int c = C - 1; c >= 0; c--) {
+            sa = ri[k][c] + z * sa;
+            u[k][c] = A * (u[k][c] + sa - ri[k][c]);
+        }
+        double* y = yM[k];
+        if (full) {
+#pragma unroll
+            for (int c = 0; c < C; c++) y[j0 + c] = u[k][c];
+        } else if (j0 <= N) {
+#pragma unroll
+            for (int c = 0; c < C; c++)
+                if (j0 + c <= N) y[j0 + c] = u[k][c];
+        }
+        // end moments for the boundary modes (read after the barrier, never rewritten)
+        if (tid == 0) bnd[8 * k + 2] = u[k][0];
+        if (tid == t1) bnd[8 * k + 3] = u[k][c1];
+        if (tid == tM) bnd[8 * k + 4] = u[k][cM];
+        if (tid == tN) bnd[8 * k + 5] = u[k][cN];
+    }
+    __syncthreads();
+    const bool edge = j0 <= N && (j0 <= 2 * KW || j0 + C - 1 >= N - 2 * KW);
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        if (edge) {
+            const double R0 = bnd[8 * k + 0] - 2.0 * bnd[8 * k + 2] - bnd[8 * k + 3];
+            const double RN = bnd[8 * k + 1] - bnd[8 * k + 4] - 2.0 * bnd[8 * k + 5];
+            const double alpha = (dg * R0 - off * RN) * idet;
+            const double beta = (dg * RN - off * R0) * idet;
+            // z^j and z^(N-j) of this chunk by repeated multiplication from 1
+            double zlo[C], zhi[C];
+            double zp = 1.0;
+            if (j0 <= 2 * KW)
+                for (int e = 0; e < j0; e++) zp *= z;
 #pragma unroll
             for (int c = 0; c < C; c++) {
-                sa = ri[k][c] + z * sa;
-                u[k][c] = sa;
+                zlo[c] = zp;
+                zp *= z;
             }
-            sa = Sw;
-            double* y = yM[k];
+            const int jl = min(j0 + C - 1, N);
+            zp = 1.0;
+            if (N - jl <= 2 * KW)
+                for (int e = 0; e < N - jl; e++) zp *= z;
 #pragma unroll
             for (int c = C - 1; c >= 0; c--) {
+                zhi[c] = zp;
+                if (j0 + c <= jl) zp *= z;
+            }
+            double* y = yM[k];
+#pragma unroll
+            for (int c = 0; c < C; c++) {
                 const int j = j0 + c;
-                if (j < j1) {
-                    sa = ri[k][c] + z * sa;
-                    const double m = A * (u[k][c] + sa - ri[k][c]);
+                double add = 0.0;
+                if (j <= 2 * KW) add += alpha * zlo[c];
+                if (N - j <= 2 * KW) add += beta * zhi[c];
+                if (add != 0.0 && j <= N) {
+                    const double m = u[k][c] + add;
                     y[j] = m;
                     u[k][c] = m;
-                    // end moments for the boundary modes (read after the barrier, never rewritten)
-                    if (j == 0) bnd[8 * k + 2] = m;
-                    if (j == 1) bnd[8 * k + 3] = m;
-                    if (j == N - 1) bnd[8 * k + 4] = m;
-                    if (j == N) bnd[8 * k + 5] = m;
                 }
             }
         }
-    }
-    __syncthreads();
-    double zN1 = 1.0;  // z^(N-1)
-    {
-        double zz = z;
-        int e = N - 1;
-        while (e > 0) {
-            if (e & 1) zN1 *= zz;
-            zz *= zz;
-            e >>= 1;
-        }
-    }
-    const double dg = 2.0 + z, off = zN1 * (1.0 + 2.0 * z);
-    const double det = dg * dg - off * off;
+        if (trap) {
+            double tm = 0.0;
 #pragma unroll
-    for (int k = 0; k < K; k++) {
-        const double R0 = bnd[8 * k + 0] - 2.0 * bnd[8 * k + 2] - bnd[8 * k + 3];
-        const double RN = bnd[8 * k + 1] - bnd[8 * k + 4] - 2.0 * bnd[8 * k + 5];
-        const double alpha = (dg * R0 - off * RN) / det;
-        const double beta = (dg * RN - off * R0) / det;
-        double* y = yM[k];
-        double tm = 0.0;
-#pragma unroll
-        for (int c = 0; c < C; c++) {
-            const int j = j0 + c;
-            if (j < j1) {
-                double add = 0.0;
-                if (j <= 2 * KW) add += alpha * kZPow.v[j];
-                if (N - j <= 2 * KW) add += beta * kZPow.v[N - j];
-                const double m = u[k][c] + add;
-                if (add != 0.0) y[j] = m;
-                if (trap) tm += (j == 0 || j == N) ? m : 2.0 * m;
+            for (int c = 0; c < C; c++) {
+                const int j = j0 + c;
+                tm += (j > N) ? 0.0 : (j == 0 || j == N) ? u[k][c] : 2.0 * u[k][c];
             }
+            trap[k] = tr[k] - h * h * h * tm * (1.0 / 24.0);
         }
-        if (trap) trap[k] = tr[k] - h * h * h * tm / 24.0;
     }
     __syncthreads();
 }
 
-// one sequence (see block_spline_inplace_k); ES: 2*NT + 8 doubles
-template <int C, int NT>
-__device__ __forceinline__ void block_spline_inplace(double* yM, int N, double h, double d0, double dN, double* ES,
-                                                     double* trap)
-{
-    double* const ys[1] = {yM};
-    block_spline_inplace_k<C, NT, 1>(ys, N, h, &d0, &dN, ES, trap);
-}
 
 // ---------------------------------------------------------------------------------
 // reductions

@@ -354,7 +354,6 @@ __global__ void __launch_bounds__(NT)
 {
     constexpr int H = L / 2;
     constexpr int NWB = H + 1;
-    constexpr int CS = (L + 1 + NT - 1) / NT;
     constexpr int nw = NT / 64;
     __shared__ double Y[2 * (L + 1)], dY[2 * (L + 1)];
     __shared__ cd X[L], dX[L];
@@ -402,10 +401,10 @@ __global__ void __launch_bounds__(NT)
         {
             double* const ys[2] = {Y1, Y2};
             const double d0s[2] = {df0V[q], dhV[q]}, dNs[2] = {df0V[q + nq], dhV[q + nq]};
-            block_spline_inplace_k<CS, NT, 2>(ys, L, hstep, d0s, dNs, ES, nullptr);
+            block_spline_fixed<L, NT, 2>(ys, hstep, d0s, dNs, ES, nullptr);
             double* const ts[2] = {dY1, dY2};
             const double t0s[2] = {tdf0V[q], tdhV[q]}, tNs[2] = {tdf0V[q + nq], tdhV[q + nq]};
-            block_spline_inplace_k<CS, NT, 2>(ts, L, hstep, t0s, tNs, ES, nullptr);
+            block_spline_fixed<L, NT, 2>(ts, hstep, t0s, tNs, ES, nullptr);
         }
         for (int j = tid; j < L; j += NT) {
             X[j] = twX[j] * cd{(Y1[j + 1] - Y1[j]) / hstep, (Y2[j + 1] - Y2[j]) / hstep};
@@ -558,10 +557,10 @@ __global__ void __launch_bounds__(NT)
         {
             double* const ys[2] = {Y1, Y2};
             const double d0s[2] = {gd0.v, dchi.v}, dNs[2] = {gdN.v, -dchi.v};
-            block_spline_inplace_k<CS, NT, 2>(ys, L, hstep, d0s, dNs, ES, integ);
+            block_spline_fixed<L, NT, 2>(ys, hstep, d0s, dNs, ES, integ);
             double* const ts[2] = {dY1, dY2};
             const double t0s[2] = {gd0.d, dchi.d}, tNs[2] = {gdN.d, -dchi.d};
-            block_spline_inplace_k<CS, NT, 2>(ts, L, hstep, t0s, tNs, ES, tinteg);
+            block_spline_fixed<L, NT, 2>(ts, hstep, t0s, tNs, ES, tinteg);
         }
         {
             const double a = wave_sum(integ[1]), b = wave_sum(tinteg[1]);

@@ -1339,6 +1339,10 @@ struct JJPoles {
     cd C1, C2;
     int rec1, rec2;
     cd a1d, a2d, b1d, b2d;  // a1/den1, a2/den2, b1/den1, b2/den2
+    // the pencil's per-step factors exp(-dt r), exp(dt r) of each pole (dt = NT/(L T), its
+    // threads' stride in tau), and the two-pole part of J at tau = beta
+    cd s1, sj1, s2, sj2;
+    double Jinf_beta;
 };
 static_assert(sizeof(JJPoles) <= 64 * sizeof(double), "DevArrays::poles holds 64 doubles per point");
 
@@ -1361,7 +1365,7 @@ __device__ __forceinline__ bool pole_off(cd v)
 }
 
 __global__ void k_jj_poles(const double* __restrict__ ek, const double* __restrict__ hVb,
-                           const double* __restrict__ Sinfp, int N, double T, double mu, JJPoles* out,
+                           const double* __restrict__ Sinfp, int N, double T, double mu, double dt, JJPoles* out,
                            const double* __restrict__ dsc)
 {
     if (dsc) mu = dsc[kPscMu + 1];
@@ -1384,6 +1388,15 @@ __global__ void k_jj_poles(const double* __restrict__ ek, const double* __restri
     o.a2d = P.a2 / P.den2;
     o.b1d = P.b1 / P.den1;
     o.b2d = P.b2 / P.den2;
+    o.s1 = cexpd(-dt * P.r1);
+    o.sj1 = cexpd(dt * P.r1);
+    o.s2 = cexpd(-dt * P.r2);
+    o.sj2 = cexpd(dt * P.r2);
+    {
+        const cd ej1 = (P.r1.x > 0) ? cd{1.0, 0.0} : cexpd(P.r1 * beta);
+        const cd ej2 = (P.r2.x > 0) ? cd{1.0, 0.0} : cexpd(P.r2 * beta);
+        o.Jinf_beta = (P.b1 * ej1 / P.den1 + P.b2 * ej2 / P.den2).x;
+    }
     if (P.r1.x > 0) {
         const cd ex = cexpd(-(P.r1 / T));
         o.f1p = P.a1 / P.den1;
@@ -1501,7 +1514,7 @@ __global__ void __launch_bounds__(NT, (L > 2048 ? 1 : JJ_WAVES_PER_SIMD))
     const long nq = (long)nk * (nk - 2);
     const double nf = 1.0 / ((double)N * N);
     const double Sinf = Sinfp[0];
-    const double hstep = 1.0 / (L * T), iT = 1.0 / T;
+    const double hstep = 1.0 / (L * T), ihstep = L * T, iT = 1.0 / T;
     const int tid = threadIdx.x;
     const int blk = xcd_remap(blockIdx.x, gridDim.x);
     const long nblk = gridDim.x;
@@ -1582,10 +1595,10 @@ __global__ void __launch_bounds__(NT, (L > 2048 ? 1 : JJ_WAVES_PER_SIMD))
         {
             double* const ys[2] = {Y1, Y2};
             const double d0s[2] = {df0V[q], dhV[q]}, dNs[2] = {df0V[q + nq], dhV[q + nq]};
-            block_spline_inplace_k<CS, NT, 2>(ys, L, hstep, d0s, dNs, ES, nullptr);
+            block_spline_fixed<L, NT, 2>(ys, hstep, d0s, dNs, ES, nullptr);
         }
         for (int j = tid; j < L; j += NT) {
-            const double a = (Y1[j + 1] - Y1[j]) / hstep, b = (Y2[j + 1] - Y2[j]) / hstep;
+            const double a = (Y1[j + 1] - Y1[j]) * ihstep, b = (Y2[j + 1] - Y2[j]) * ihstep;
             X[j] = twX[j] * cd{a, b};
         }
         const double cj03 = -(Y1[0] + Y1[L]), cj3 = -(Y2[0] + Y2[L]);
@@ -1658,24 +1671,16 @@ __global__ void __launch_bounds__(NT, (L > 2048 ? 1 : JJ_WAVES_PER_SIMD))
         const Poles& P = pp.P;
         // (4) to imaginary time: X_j <- (G(tau_j), J(tau_j))
         block_fft_pp<L, NT>(W, X, -1, tw);
-        if (tid == 0) {
-            const double t = 1.0 / T;
-            const cd ej1 = (P.r1.x > 0) ? cd{1.0, 0.0} : cexpd(P.r1 * t);
-            const cd ej2 = (P.r2.x > 0) ? cd{1.0, 0.0} : cexpd(P.r2 * t);
-            const double Jinf = (P.b1 * ej1 / P.den1 + P.b2 * ej2 / P.den2).x;
-            Jbeta_s = (cd{W[0].y, 0.0} / phT[L]).x + Jinf;  // W[0].y = FFT_+(Jn)_0 (ph_0 = 1)
-        }
+        if (tid == 0) Jbeta_s = (cd{W[0].y, 0.0} / phT[L]).x + pp.Jinf_beta;  // W[0].y = FFT_+(Jn)_0 (ph_0 = 1)
         {
             // the pole factors are geometric in j: from tau_j to tau_{j+NT} e_p gains
-            // exp(-dt r_p) and ej_p exp(+dt r_p), dt = NT/(L T); a factor that leaves
-            // [1e-280, 2] on the way (underflow of a factor that grows again, overflow of a
-            // step) is evaluated directly again
-            const double dt = NT / (L * T);
+            // exp(-dt r_p) and ej_p exp(+dt r_p), dt = NT/(L T) (the step factors come with the
+            // pole data); a factor that leaves [1e-280, 2] on the way (underflow of a factor
+            // that grows again, overflow of a step) is evaluated directly again
             double t = tid / (L * T);
             cd e1 = pole_e(P.r1, t, T), e2 = pole_e(P.r2, t, T);
             cd ej1 = pole_ej(P.r1, t, T, pp.rec1, pp.C1, e1), ej2 = pole_ej(P.r2, t, T, pp.rec2, pp.C2, e2);
-            const cd s1 = cexpd(-dt * P.r1), sj1 = cexpd(dt * P.r1);
-            const cd s2 = cexpd(-dt * P.r2), sj2 = cexpd(dt * P.r2);
+            const cd s1 = pp.s1, sj1 = pp.sj1, s2 = pp.s2, sj2 = pp.sj2;
             cd ph_next = phT[tid];
 #pragma unroll 1
             for (int j = tid; j < L; j += NT) {
@@ -1730,7 +1735,7 @@ __global__ void __launch_bounds__(NT, (L > 2048 ? 1 : JJ_WAVES_PER_SIMD))
         {
             double* const ys[2] = {Y1, Y2};
             const double d0s[2] = {gd0, dchi}, dNs[2] = {gdN, -dchi};
-            block_spline_inplace_k<CS, NT, 2>(ys, L, hstep, d0s, dNs, ES, integ);
+            block_spline_fixed<L, NT, 2>(ys, hstep, d0s, dNs, ES, integ);
         }
         // the bubble over [0, beta] is used by thread 0 only (n = 0): each wave parks its sum in
         // sc[6 nw + wave], which thread 0 adds in the accumulation below (after two barriers)
@@ -1741,8 +1746,8 @@ __global__ void __launch_bounds__(NT, (L > 2048 ? 1 : JJ_WAVES_PER_SIMD))
         const double MJ3 = Y1[0] - Y1[L];
         // (6) packed transform of the two moment sequences
         for (int j = tid; j < L; j += NT) {
-            const double a = (Y1[j + 1] - Y1[j]) / hstep;
-            const double b = (j < H) ? (Y2[j + 1] - Y2[j]) / hstep : -(Y2[L - j] - Y2[L - 1 - j]) / hstep;
+            const double a = (Y1[j + 1] - Y1[j]) * ihstep;
+            const double b = (j < H) ? (Y2[j + 1] - Y2[j]) * ihstep : -(Y2[L - j] - Y2[L - 1 - j]) * ihstep;
             X[j] = cd{a, b};
         }
         __syncthreads();  // Y1, Y2 read for the last time; the bubble's wave sums are in sc
@@ -2221,8 +2226,9 @@ hipError_t launch_hVf0V_slab(const Grid& g, const DevArrays& d, hipStream_t s)
 
 hipError_t launch_jj_poles(const Grid& g, const DevArrays& d, double mu, hipStream_t s, const double* dsc)
 {
+    const double dt = jj_threads(g.Nw) / (g.Nw * g.T);  // = NT / (L T) of k_jj_pencil
     hipLaunchKernelGGL(k_jj_poles, dim3((unsigned)((g.nq + 127) / 128)), dim3(128), 0, s, d.ek, d.hVb, d.Sinf, g.N,
-                       g.T, mu, (JJPoles*)d.poles, dsc);
+                       g.T, mu, dt, (JJPoles*)d.poles, dsc);
     return hipGetLastError();
 }
 
