@@ -481,136 +481,14 @@ __device__ __forceinline__ void block_fft_pp(cd* buf, cd* tmp, int sign, const c
 // A z^{|j|}, z = sqrt(3)-2, A = 1/(2 sqrt 3); P_j = A sum_{i=1}^{N-1} z^{|j-i|} r_i
 // = A (u_j + w_j - r_j) with u (w) the causal (anti-causal) recursion s <- r + z s.
 // M_j = P_j + alpha z^j + beta z^{N-j}; alpha, beta from the two boundary rows.
-// Each thread runs the recursions over a chunk with a KW-sample warm-up
-// (|z|^33 < 2e-19, so the truncation is below FP64 rounding).
-// y: LDS input (N+1); M: LDS output (N+1); rr: LDS scratch (N+1).
+// The boundary modes reach 2 KW samples into the sequence (|z|^66 < 1e-37, below FP64
+// rounding of the moments).
 // ---------------------------------------------------------------------------------
 constexpr int KW = 33;
 
-// NPMAX: largest N+1 the caller passes; NT: threads of the block.  Each thread owns a
-// chunk of Cr consecutive samples (Cr <= C, a compile-time bound, so the chunk stays in
-// registers).  The recursions are a chunked scan: local recursion from a zero state,
-// chunk-end (start) states exchanged through LDS, then the carry S from the preceding
-// chunks, S_{t-1} = sum_{d>=1} z^{Cr(d-1)} E_{t-d}, truncated once z^{Cr(d-1)} < 1e-19.
-// ES: LDS scratch of 2*NT doubles.
-template <int NPMAX, int NT>
-__device__ __forceinline__ void block_spline_moments(const double* y, double* M, double* rr, int N, double h,
-                                                     double d0, double dN, double* ES)
-{
-    constexpr int C = (NPMAX + NT - 1) / NT;
-    const int tid = threadIdx.x;
-    const double z = 1.7320508075688772935 - 2.0;   // sqrt(3) - 2
-    const double A = 0.28867513459481288225;         // 1 / (2 sqrt 3)
-    const double ih = 1.0 / h;
-    for (int j = tid; j <= N; j += NT) {
-        double r;
-        if (j == 0)
-            r = 6.0 * ih * ((y[1] - y[0]) * ih - d0);
-        else if (j == N)
-            r = 6.0 * ih * (dN - (y[N] - y[N - 1]) * ih);
-        else
-            r = 6.0 * ih * ih * (y[j + 1] - 2.0 * y[j] + y[j - 1]);
-        rr[j] = r;
-    }
-    __syncthreads();
-    const int Cr = (N + 1 + NT - 1) / NT;
-    const int nchunk = (N + 1 + Cr - 1) / Cr;
-    const int j0 = tid * Cr, j1 = min(N + 1, j0 + Cr);
-    double ri[C], u[C], w[C];
-    double eu = 0.0, ew = 0.0;
-    if (j0 < j1) {
-#pragma unroll
-        for (int c = 0; c < C; c++) {
-            const int i = j0 + c;
-            ri[c] = (i < j1 && i >= 1 && i <= N - 1) ? rr[i] : 0.0;
-        }
-        double s = 0.0;
-#pragma unroll
-        for (int c = 0; c < C; c++) {
-            s = ri[c] + z * s;
-            u[c] = s;
-        }
-#pragma unroll
-        for (int c = 0; c < C; c++)
-            if (c == Cr - 1) eu = u[c];
-        s = 0.0;
-#pragma unroll
-        for (int c = C - 1; c >= 0; c--) {
-            if (j0 + c < j1) s = ri[c] + z * s;
-            w[c] = s;
-        }
-        ew = s;
-    }
-    ES[tid] = eu;       // causal state at the chunk end
-    ES[NT + tid] = ew;  // anti-causal state at the chunk start
-    __syncthreads();
-    if (j0 < j1) {
-        double zc = 1.0;  // z^Cr
-        for (int c = 0; c < Cr; c++) zc *= z;
-        double Su = 0.0, Sw = 0.0, zd = 1.0;
-        for (int d = 1; d <= nchunk && fabs(zd) > 1.0e-19; d++) {
-            if (tid - d >= 0) Su += zd * ES[tid - d];
-            if (tid + d < nchunk) Sw += zd * ES[NT + tid + d];
-            zd *= zc;
-        }
-        double zp = z;
-#pragma unroll
-        for (int c = 0; c < C; c++) {
-            u[c] += zp * Su;
-            zp *= z;
-        }
-        // anti-causal carry enters from the right end of the chunk
-        zp = z;
-#pragma unroll
-        for (int c = C - 1; c >= 0; c--) {
-            if (j0 + c < j1) {
-                w[c] += zp * Sw;
-                zp *= z;
-            }
-        }
-#pragma unroll
-        for (int c = 0; c < C; c++)
-            if (j0 + c < j1) M[j0 + c] = A * (u[c] + w[c] - ri[c]);
-    }
-    __syncthreads();
-    // boundary modes
-    const double R0 = rr[0] - 2.0 * M[0] - M[1];
-    const double RN = rr[N] - M[N - 1] - 2.0 * M[N];
-    double zN1 = 1.0;  // z^(N-1)
-    {
-        double zz = z;
-        int e = N - 1;
-        while (e > 0) {
-            if (e & 1) zN1 *= zz;
-            zz *= zz;
-            e >>= 1;
-        }
-    }
-    const double dg = 2.0 + z, off = zN1 * (1.0 + 2.0 * z);
-    const double det = dg * dg - off * off;
-    const double alpha = (dg * R0 - off * RN) / det;
-    const double beta = (dg * RN - off * R0) / det;
-    __syncthreads();
-    for (int j = tid; j <= N; j += NT) {
-        double add = 0.0;
-        if (j <= 2 * KW) {
-            double zp = 1.0;
-            for (int e = 0; e < j; e++) zp *= z;
-            add += alpha * zp;
-        }
-        if (N - j <= 2 * KW) {
-            double zp = 1.0;
-            for (int e = 0; e < N - j; e++) zp *= z;
-            add += beta * zp;
-        }
-        M[j] += add;
-    }
-    __syncthreads();
-}
-
 // z^e of the spline pole z = sqrt(3) - 2 by repeated multiplication, z^(N-1) by repeated
-// squaring, and the number of neighbouring chunks whose
-// carries count (|z^(C (d-1))| > 1e-19, at most maxd)
+// squaring, and the number of neighbouring chunks whose carries count (|z^(C (d-1))| > 1e-19,
+// at most maxd)
 constexpr double kSplineZ = 1.7320508075688772935 - 2.0;
 constexpr double spline_zpow(int e)
 {
@@ -639,17 +517,17 @@ constexpr int spline_carry_terms(double zc, int maxd)
     return d;
 }
 
-// The same clamped spline as block_spline_moments, in place, for K sequences of length N (known
-// at compile time) at once, their latencies overlapping: yM[k] holds y_0..y_N on entry and the
+// The clamped spline above, in place, for K sequences of length N (known at compile time) at
+// once, their latencies overlapping: yM[k] holds y_0..y_N on entry and the
 // moments M_0..M_N on exit.  Each thread keeps its chunk of the right-hand side in registers, so
 // no scratch array is needed.  The two first-order recursions run once from a zero state to get
 // the chunk-end states, and again after the exchange, seeded with the carries of the
-// neighbouring chunks.  The chunk layout is fixed at compile time, with almost no divergent
-// branches: the sequences are padded with
-// zeros past N (the anti-causal recursion then reaches N in the zero state it starts from, so
-// the moments do not change), the carries take their compile-time number of terms with
-// out-of-range neighbours read as zero, full chunks store without per-element tests, and only
-// the chunks within 2 KW of an end take the boundary modes.
+// neighbouring chunks, S_{t-1} = sum_{d>=1} z^{C(d-1)} E_{t-d}, truncated once z^{C(d-1)} <
+// 1e-19.  The chunk layout is fixed at compile time, with almost no divergent branches: the
+// sequences are padded with zeros past N (the anti-causal recursion then reaches N in the zero
+// state it starts from, so the moments do not change), the carries take their compile-time
+// number of terms with out-of-range neighbours read as zero, full chunks store without
+// per-element tests, and only the chunks within 2 KW of an end take the boundary modes.
 // trap (optional): trap[k] receives this thread's part of sum_j h (y_j + y_{j+1})/2
 // - h^3 (M_j + M_{j+1})/24, the integral of spline k over [0, N h]; the caller adds the parts.
 // ES: LDS scratch of K*(2*NT + 8) doubles.

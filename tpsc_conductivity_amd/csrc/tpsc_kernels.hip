@@ -376,32 +376,31 @@ __global__ void __launch_bounds__(PEN_THREADS)
 {
     constexpr int NS = L / 2;  // intervals
     constexpr int PJ = (NS + PEN_THREADS - 1) / PEN_THREADS;  // intervals per thread
-    // The spline arrays (scratch rr, samples y, moments M: 3 (NS + 1) doubles) live in the FFT
-    // buffer until the transform input is formed: 36 KB of LDS per workgroup, four per CU.
-    static_assert(3 * (NS + 1) <= 2 * L, "the spline arrays fit in the transform buffer");
+    // The spline samples, then in place their moments (NS + 1 doubles), live in the FFT buffer
+    // until the transform input is formed: 36 KB of LDS per workgroup, four per CU.
+    static_assert(NS + 1 <= 2 * L, "the spline array fits in the transform buffer");
     __shared__ cd X[L];
-    __shared__ double ES[2 * PEN_THREADS];
+    __shared__ double ES[2 * PEN_THREADS + 8];
     __shared__ double sc[16];
     const long q = xcd_remap(blockIdx.x, gridDim.x);
     if (q >= nk8) return;
     const int tid = threadIdx.x, nt = blockDim.x;
-    double* rr = (double*)X;
-    double* y = rr + (NS + 1);
-    double* M = y + (NS + 1);
+    double* y = (double*)X;
     for (int j = tid; j <= NS; j += nt) y[j] = (double)chit[q + (long)j * nk8];
-    const double d0 = dchiqt0[q];
+    const double d0 = dchiqt0[q], dN = 0.0;
     const double h = 1.0 / (L * T);
     __syncthreads();
-    block_spline_moments<NS + 1, PEN_THREADS>(y, M, rr, NS, h, d0, 0.0, ES);
+    // y -> moments M in place; integ: this thread's part of the spline's integral
     double integ = 0.0, a6v[PJ];
+    {
+        double* const ys[1] = {y};
+        block_spline_fixed<NS, PEN_THREADS, 1>(ys, h, &d0, &dN, ES, &integ);
+    }
+    const double* M = y;
 #pragma unroll
     for (int c = 0; c < PJ; c++) {
         const int j = tid + c * nt;
-        a6v[c] = 0.0;
-        if (j < NS) {
-            a6v[c] = (M[j + 1] - M[j]) / h;
-            integ += h * (y[j] + y[j + 1]) * 0.5 - h * h * h * (M[j] + M[j + 1]) / 24.0;
-        }
+        a6v[c] = (j < NS) ? (M[j + 1] - M[j]) / h : 0.0;
     }
     __syncthreads();  // every read of y and M is done: X may be written
 #pragma unroll
@@ -839,19 +838,22 @@ __global__ void __launch_bounds__(NT)
                    double* Sinf2, double* Sinf3, double* Sinf4, const double* __restrict__ Sinfp,
                    const cd* __restrict__ tw, const cd* __restrict__ twX, const cd* __restrict__ twS)
 {
-    __shared__ double y[L + 1], M[L + 1];
+    __shared__ double y[L + 1];  // the samples, then in place their spline moments
     __shared__ cd X[L];
-    __shared__ double ES[2 * NT];
+    __shared__ double ES[2 * NT + 8];
     const long k = xcd_remap(blockIdx.x, gridDim.x);
     if (k >= nk8) return;
     const int tid = threadIdx.x, nt = blockDim.x;
-    double* rr = (double*)X;
     for (int j = tid; j <= L; j += nt)
         y[j] = (j == 0) ? Sb[k] : (j == L) ? Sb[nk8 + k] : (double)Stau[k + (long)j * nk8];
     const double d0 = dSig[k], dN = dSig[k + nk8];
     const double h = 1.0 / (L * T);
     __syncthreads();
-    block_spline_moments<L + 1, NT>(y, M, rr, L, h, d0, dN, ES);
+    {
+        double* const ys[1] = {y};
+        block_spline_fixed<L, NT, 1>(ys, h, &d0, &dN, ES, nullptr);
+    }
+    const double* M = y;
     for (int j = tid; j < L; j += nt) {
         const double a6 = (M[j + 1] - M[j]) / h;
         X[j] = twX[j] * a6;
