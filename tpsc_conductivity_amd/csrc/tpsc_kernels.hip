@@ -828,10 +828,12 @@ __global__ void __launch_bounds__(SLAB_THREADS, SlabOcc<S>::waves)
 
 // green.cpp:1925-2018: per wedge k, spline of Sigma(k,tau_j) (j=0..Nw) with slopes dSig,
 // F = BACKWARD DFT[exp(i pi j/Nw) 6a_j], R(k,n) = (1 - exp(i pi (2n+1)/Nw)) F_n
-// threads of the self-energy pencil: 512 for the long transforms (at Nw = 2048 the 70 KB
-// workgroup spends less of its time in the serial parts of the spline solve than at 256)
-constexpr int sigma_pen_threads(int L) { return L >= 1024 ? 512 : PEN_THREADS; }
-template <int L, typename S, typename SelfT, int NT = sigma_pen_threads(L)>
+// threads of the self-energy pencil: 512 for the long transforms of the mixed path, which
+// stores the assembled FP32 Sigma(k,iwn) (at Nw = 2048: 0.59 against 0.63 ms at 256); 256 in
+// FP64, which stores the remainder (0.48 against 0.57 ms at 512)
+template <typename SelfT>
+constexpr int sigma_pen_threads(int L) { return (L >= 1024 && sizeof(SelfT) == sizeof(cf)) ? 512 : PEN_THREADS; }
+template <int L, typename S, typename SelfT, int NT = sigma_pen_threads<SelfT>(L)>
 __global__ void __launch_bounds__(NT)
     k_sigma_pencil(const S* __restrict__ Stau, const double* __restrict__ Sb, const double* __restrict__ dSig,
                    long nk8, double T, SelfT* Self,
@@ -2141,7 +2143,7 @@ hipError_t launch_sigma_pencil(const Grid& g, const DevArrays& d, hipStream_t s)
 {
     hipLaunchKernelGGL(k_sigma_inf, dim3(1), dim3(64), 0, s, d.Sb, g.nk8, d.Sinf);
     DISPATCH_S(g.prec, DISPATCH_L(g.Nw, {
-        hipLaunchKernelGGL((k_sigma_pencil<LL, SS, ST>), dim3((unsigned)g.nk8), dim3(sigma_pen_threads(LL)), 0, s,
+        hipLaunchKernelGGL((k_sigma_pencil<LL, SS, ST>), dim3((unsigned)g.nk8), dim3(sigma_pen_threads<ST>(LL)), 0, s,
                            (const SS*)d.Stau, d.Sb, d.dSig, g.nk8, g.T, (ST*)d.Self, d.Sinf2, d.Sinf3, d.Sinf4, d.Sinf,
                            d.twN, d.twX, d.twS);
     }));
