@@ -721,11 +721,13 @@ __global__ void k_sigma_inf(const double* __restrict__ Sb, long nk8, double* Sin
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) Sinf[0] = -(Sb[0] + Sb[nk8]);
 }
-// Boundary data of Sigma(k,tau) in FP64, block `side` = 0 (tau = 0) or 1 (tau = beta):
-// the slope dSigma/dtau (green.cpp:1743-1817) and the value Sigma(k,tau_b)
-// = DCT[G0(r,tau_b) V(r,0)] (green.cpp:1822-1892 at j = 0, Nw; V(r,beta) = V(r,0)).
+// Boundary data of Sigma(k,tau) in FP64, block b: side = b % 2 (0: tau = 0, 1: tau = beta),
+// pass = b / 2: the slope dSigma/dtau (pass 0, green.cpp:1743-1817) or the value
+// Sigma(k,tau_b) = DCT[G0(r,tau_b) V(r,0)] (pass 1, green.cpp:1822-1892 at j = 0, Nw;
+// V(r,beta) = V(r,0)).  Each block transforms G0(k,tau_b) to r itself (the two passes of a
+// side run side by side: two transforms in a row per block instead of three).
 // G0(k,0+) = -1/(e^{-x/T}+1) and G0(k,beta-) = -1/(e^{x/T}+1), both over N^2.
-// tmp: staging, 2*nk8 per side.
+// tmp: staging, nk8 per block.
 template <int N>
 __global__ void __launch_bounds__(SLAB_THREADS)
     k_sigma_bnd(const double* __restrict__ ek, const double* __restrict__ er, const double* __restrict__ V0,
@@ -743,8 +745,8 @@ __global__ void __launch_bounds__(SLAB_THREADS)
     cd* tw = slab_tw(slab, (long)nk * nk);
     fill_twiddles(tw, N);
     const int tid = threadIdx.x, nt = blockDim.x;
-    const int side = blockIdx.x;
-    double* stg = tmp + 2 * side * nk8;
+    const int side = blockIdx.x & 1, pass = blockIdx.x >> 1;
+    double* stg = tmp + blockIdx.x * nk8;
     for (int s = tid; s < nk * nk; s += nt) {
         const double x = ek[s] - mu0;
         slab[s] = (side == 0) ? -1.0 / (exp(-x / T) + 1.0) / nk02 : -1.0 / (exp(x / T) + 1.0) / nk02;
@@ -756,30 +758,28 @@ __global__ void __launch_bounds__(SLAB_THREADS)
         wedge_lm(i, l, m);
         const double g = slab[m + l * nk];
         const double v = V0[i];
-        if (side == 0)
+        if (pass == 1)
+            stg[i] = g * v;
+        else if (side == 0)
             stg[i] = fact * dchirt0[i] * g + v * dGrt0p[i];
         else {
             const double xi = (i == 0) ? -mu0 : er[i];
             stg[i] = -fact * dchirt0[i] * g + v * (-dGrt0p[i] + xi);
         }
-        stg[nk8 + i] = g * v;
     }
     __syncthreads();
-    for (int pass = 0; pass < 2; pass++) {
-        for (long i = tid; i < nk8; i += nt) {
-            int l, m;
-            wedge_lm(i, l, m);
-            put_sym(slab, nk, l, m, stg[pass * nk8 + i]);
-        }
-        __syncthreads();
-        slab_r2r_2d<double, N, 0, 0>(slab, nk, nk, nk, tw);
-        double* o = (pass == 0) ? dSig : Sb;
-        for (long i = tid; i < nk8; i += nt) {
-            int l, m;
-            wedge_lm(i, l, m);
-            o[i + side * nk8] = slab[m + l * nk];
-        }
-        __syncthreads();
+    for (long i = tid; i < nk8; i += nt) {
+        int l, m;
+        wedge_lm(i, l, m);
+        put_sym(slab, nk, l, m, stg[i]);
+    }
+    __syncthreads();
+    slab_r2r_2d<double, N, 0, 0>(slab, nk, nk, nk, tw);
+    double* o = (pass == 0) ? dSig : Sb;
+    for (long i = tid; i < nk8; i += nt) {
+        int l, m;
+        wedge_lm(i, l, m);
+        o[i + side * nk8] = slab[m + l * nk];
     }
 }
 
@@ -1146,12 +1146,14 @@ __global__ void __launch_bounds__(SLAB_THREADS, SlabOcc<S>::waves)
 }
 
 // Boundary data of hV and f0V in FP64 (cond_opt.cpp:2697-2735, 2745-3114 at tau = 0, beta).
-// Block b: which = b/2 (0: h, 1: f0), side = b%2 (0: tau = 0, 1: tau = beta).
+// Block b: which = (b/2)%2 (0: h, 1: f0), side = b%2 (0: tau = 0, 1: tau = beta), part = b/4:
 //   (1) h(r,tau_b) (or f0) by DST x DCT of the k-space integrand at t = 0 or beta
-//   (2) the tau-derivative integrand -> r, combined with V(r,0), dchi0(r,0) -> k: slopes
-//       dhV / df0V (cond_opt.cpp:2745-3014)
-//   (3) 2 h(r,tau_b) V(r,0) -> k: hVb / f0Vb, the tau = 0, beta values of hV / f0V
-// Independent of the tau slabs.  bscr: staging, nq per block.
+//   part 1: (2) the tau-derivative integrand -> r, combined with V(r,0), dchi0(r,0) -> k:
+//       slopes dhV / df0V (cond_opt.cpp:2745-3014)
+//   part 0: (3) 2 h(r,tau_b) V(r,0) -> k: hVb / f0Vb, the tau = 0, beta values of hV / f0V
+// Both parts of a (which, side) run (1) themselves and then side by side: three transforms in
+// a row per block instead of four.  Independent of the tau slabs.  bscr: staging, nq per
+// (which, side).
 template <int N>
 __global__ void __launch_bounds__(SLAB_THREADS)
     k_hf0_bnd(const double* __restrict__ ek, const double* __restrict__ dek, const double* __restrict__ V0,
@@ -1168,9 +1170,9 @@ __global__ void __launch_bounds__(SLAB_THREADS)
     double* slab = smem;
     cd* tw = slab_tw(slab, nq);
     fill_twiddles(tw, N);
-    const int which = blockIdx.x >> 1, side = blockIdx.x & 1;
+    const int which = (blockIdx.x >> 1) & 1, side = blockIdx.x & 1, part = blockIdx.x >> 2;
     const int tid = threadIdx.x, nt = blockDim.x;
-    double* hr = bscr + blockIdx.x * nq;
+    double* hr = bscr + (blockIdx.x & 3) * nq;
     const double tb = (side == 0) ? 0.0 : (double)Nw / (T * Nw);  // tau_j of k_hf0_slab at j = 0, Nw
     // (1)
     for (long s = tid; s < nq; s += nt) {
@@ -1179,6 +1181,18 @@ __global__ void __launch_bounds__(SLAB_THREADS)
     }
     __syncthreads();
     slab_r2r_2d<double, N, 1, 0>(slab, nk - 2, nk, nk, tw);
+    if (part == 0) {
+        // (3), from h(r,tau_b) in place
+        for (long s = tid; s < nq; s += nt) {
+            const int l = (int)(s / nk), m = (int)(s % nk);
+            slab[s] = 2.0 * slab[s] * V0[fold_idx(l + 1, m, N)];
+        }
+        __syncthreads();
+        slab_r2r_2d<double, N, 1, 0>(slab, nk - 2, nk, nk, tw);
+        double* ob = ((which == 0) ? hVb : f0Vb) + side * nq;
+        for (long s = tid; s < nq; s += nt) ob[s] = slab[s];
+        return;
+    }
     for (long s = tid; s < nq; s += nt) hr[s] = slab[s];
     __syncthreads();
     // (2)
@@ -1215,16 +1229,6 @@ __global__ void __launch_bounds__(SLAB_THREADS)
     slab_r2r_2d<double, N, 1, 0>(slab, nk - 2, nk, nk, tw);
     double* o = ((which == 0) ? dhV : df0V) + side * nq;
     for (long s = tid; s < nq; s += nt) o[s] = slab[s];
-    __syncthreads();
-    // (3)
-    for (long s = tid; s < nq; s += nt) {
-        const int l = (int)(s / nk), m = (int)(s % nk);
-        slab[s] = 2.0 * hr[s] * V0[fold_idx(l + 1, m, N)];
-    }
-    __syncthreads();
-    slab_r2r_2d<double, N, 1, 0>(slab, nk - 2, nk, nk, tw);
-    double* ob = ((which == 0) ? hVb : f0Vb) + side * nq;
-    for (long s = tid; s < nq; s += nt) ob[s] = slab[s];
 }
 
 // hV(k,tau) = DST x DCT [2 h(r,tau) V(r,|tau|)], same for f0, in place (cond_opt.cpp:3029-3114)
@@ -2122,7 +2126,7 @@ hipError_t launch_sigma_bnd(const Grid& g, const DevArrays& d, double mu0, doubl
     const size_t lds = slab_lds(g.N, false, sizeof(double));
     DISPATCH_N(g.N, {
         set_lds(k_sigma_bnd<NN>, lds);
-        hipLaunchKernelGGL(k_sigma_bnd<NN>, dim3(2), dim3(SLAB_THREADS), lds, s, d.ek, d.er, d.V0, d.dchirt0, d.dGrt0p,
+        hipLaunchKernelGGL(k_sigma_bnd<NN>, dim3(4), dim3(SLAB_THREADS), lds, s, d.ek, d.er, d.V0, d.dchirt0, d.dGrt0p,
                            mu0, g.T, fact, d.dSig, d.Sb, d.tmp, mu0p, U, dsc);
     });
     return hipGetLastError();
@@ -2205,7 +2209,7 @@ hipError_t launch_hf0_bnd(const Grid& g, const DevArrays& d, double mu0, double 
     const size_t lds = slab_lds(g.N, true, sizeof(double));
     DISPATCH_N(g.N, {
         set_lds(k_hf0_bnd<NN>, lds);
-        hipLaunchKernelGGL(k_hf0_bnd<NN>, dim3(4), dim3(SLAB_THREADS), lds, s, d.ek, d.dek, d.V0, d.dchirt0, mu0, g.T,
+        hipLaunchKernelGGL(k_hf0_bnd<NN>, dim3(8), dim3(SLAB_THREADS), lds, s, d.ek, d.dek, d.V0, d.dchirt0, mu0, g.T,
                            g.Nw, fact4, d.dhV, d.df0V, d.hVb, d.f0Vb, d.bscr, mu0p, U, dsc);
     });
     return hipGetLastError();
