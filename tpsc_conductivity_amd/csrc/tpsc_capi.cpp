@@ -108,6 +108,7 @@ struct tpsc_ctx {
     };
     std::vector<RootGraph> root_graphs;
     hipEvent_t ev_hv = nullptr, ev_v = nullptr;
+    hipEvent_t ev_bnd = nullptr;  // the Lambda boundary kernel (hVb, f0Vb, slopes) done on prep
     bool hv_pending = false;
     bool hv_slabs_due = false;  // the hV, f0V slabs are still to be queued (after the Sigma slab)
     bool hv_dropped = false;    // they were dropped: the Lambda stage makes them
@@ -696,6 +697,7 @@ int stage_vertex(tpsc_ctx* c)
         c->hf0_pending = false;
         prof_begin(c, "jj_hf0_bnd", c->prep);
         HIPCHK(c, launch_hf0_bnd(c->g, c->d, c->mu0, fact4, c->prep));
+        HIPCHK(c, hipEventRecord(c->ev_bnd, c->prep));
         prof_end(c);
         c->hv_slabs_due = true;
         c->hv_dropped = false;
@@ -897,6 +899,18 @@ int stage_chijj(tpsc_ctx* c, int ntheta)
     // stream); otherwise they are made here
     const bool prepared = c->hv_pending && !c->hv_dropped;
     c->hv_dropped = false;
+    if (prepared) {
+        // the kx = 0, pi rows and the pole data need no slab of the prep stream: they go on
+        // the main stream while the hV/f0V slabs and transposes finish there (the poles read
+        // the boundary kernel's hVb)
+        prof_begin(c, "jj_kx0_rows");
+        HIPCHK(c, launch_kx0_rows(g, c->d, c->mu, s));
+        prof_end(c);
+        HIPCHK(c, hipStreamWaitEvent(s, c->ev_bnd, 0));
+        prof_begin(c, "jj_poles");
+        HIPCHK(c, launch_jj_poles(g, c->d, c->mu, s));
+        prof_end(c);
+    }
     int rc = settle_prep(c, true);
     if (rc) return rc;
     if ((rc = fork_side(c))) return rc;
@@ -904,13 +918,13 @@ int stage_chijj(tpsc_ctx* c, int ntheta)
         prof_begin(c, "jj_hf0_bnd", c->side);
         HIPCHK(c, launch_hf0_bnd(g, c->d, c->mu0, fact4, c->side));
         prof_end(c);
+        prof_begin(c, "jj_kx0_rows", c->side);
+        HIPCHK(c, launch_kx0_rows(g, c->d, c->mu, c->side));
+        prof_end(c);
+        prof_begin(c, "jj_poles", c->side);
+        HIPCHK(c, launch_jj_poles(g, c->d, c->mu, c->side));
+        prof_end(c);
     }
-    prof_begin(c, "jj_kx0_rows", c->side);
-    HIPCHK(c, launch_kx0_rows(g, c->d, c->mu, c->side));
-    prof_end(c);
-    prof_begin(c, "jj_poles", c->side);
-    HIPCHK(c, launch_jj_poles(g, c->d, c->mu, c->side));
-    prof_end(c);
     if (!prepared) {
         if (c->hf0_pending) {
             HIPCHK(c, hipStreamWaitEvent(s, c->ev_hf0, 0));
@@ -1428,6 +1442,7 @@ int tpsc_create(const tpsc_params* p, tpsc_ctx** out)
     HIPCHK(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
     HIPCHK(c, hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
     HIPCHK(c, hipEventCreateWithFlags(&c->ev_tau0, hipEventDisableTiming));
+    HIPCHK(c, hipEventCreateWithFlags(&c->ev_bnd, hipEventDisableTiming));
     HIPCHK(c, hipEventCreateWithFlags(&c->ev_hf0, hipEventDisableTiming));
     HIPCHK(c, hipEventCreateWithFlags(&c->ev_mu0, hipEventDisableTiming));
     HIPCHK(c, hipHostMalloc((void**)&c->host_scalar, kHostScalars * sizeof(double), hipHostMallocMapped));
@@ -1524,6 +1539,7 @@ void tpsc_destroy(tpsc_ctx* c)
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
     if (c->ev_tau0) (void)hipEventDestroy(c->ev_tau0);
+    if (c->ev_bnd) (void)hipEventDestroy(c->ev_bnd);
     if (c->ev_hf0) (void)hipEventDestroy(c->ev_hf0);
     if (c->ev_mu0) (void)hipEventDestroy(c->ev_mu0);
     for (auto& rg : c->root_graphs)
