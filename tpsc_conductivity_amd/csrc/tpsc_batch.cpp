@@ -143,7 +143,7 @@ void run_queued(const tpsc_params* pts, const std::vector<int>& order, tpsc_batc
         if (pool.ctx_for(pts[i], &ctxs[k])) ctxs[k] = nullptr;
         off[k + 1] = off[k] + (ctxs[k] ? tpsc_queue_slot_doubles(ctxs[k]) : 0);
     }
-    // the slots: pinned host memory that the gather kernels write through its device address
+    // the slots: pinned host memory that the kernels write through its device address
     QueuedState& q = *g_queued;
     if (off[n] > q.cap) {
         if (q.slots) (void)hipHostFree(q.slots);

@@ -958,8 +958,8 @@ int stage_chijj(tpsc_ctx* c, int ntheta)
     prof_begin(c, "jj_pencil");
     HIPCHK(c, launch_jj_pencil(g, c->d, c->mu, part, bins, ntheta, s));
     prof_end(c);
-    HIPCHK(c, launch_reduce_cols(part, nblk, 2 * nwb * ntheta, redrows, c->stream));
-    HIPCHK(c, launch_reduce_cols(c->d.jjscal, nblk, 4, c->d.jjred + 2 * nwb, c->stream));
+    HIPCHK(c, launch_reduce_cols2(part, 2 * nwb * ntheta, redrows, c->d.jjscal, 4, c->d.jjred + 2 * nwb, nblk,
+                                  c->stream));
     std::vector<double> red(2 * nwb * ntheta), sc4(4), rows(2 * g.nk);
     HIPCHK(c, hipMemcpyAsync(red.data(), redrows, sizeof(double) * red.size(), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipMemcpyAsync(sc4.data(), c->d.jjred + 2 * nwb, sizeof(double) * 4, hipMemcpyDeviceToHost, c->stream));
@@ -2142,8 +2142,8 @@ int tpsc_keep_chi_tau(tpsc_ctx* c, int on)
 
 // ------------------------------------------------------------------ queued solves (batches)
 // A slot holds d.psc (kPscDoubles), then the reduced Lambda rows (2 nwb) and the pencil's four
-// scalar sums, then the kx = 0, pi rows of the diamagnetic term (2 nk); k_queue_gather writes it
-// through the slot's device address.
+// scalar sums, then the kx = 0, pi rows of the diamagnetic term (2 nk); the kernels that make
+// them write them there through the slot's device address.
 size_t tpsc_queue_slot_doubles(const tpsc_ctx* c)
 {
     return kPscDoubles + 2 * (size_t)c->g.nwb + 4 + 2 * (size_t)c->g.nk;
@@ -2220,13 +2220,14 @@ int tpsc_queue_solve(tpsc_ctx* c, const tpsc_params* p, double* slot_dev, bool c
     HIPCHK(c, launch_hf0_bnd(g, d, 0.0, 0.0, s, d.mu0v, U, psc));
     HIPCHK(c, launch_hVf0V_slab(g, d, s));
     HIPCHK(c, launch_jj_transpose(g, d, s));
-    HIPCHK(c, launch_kx0_rows(g, d, 0.0, s, psc));
+    double* const slot_rows = slot_dev + kPscDoubles;  // reduced Lambda rows, scalar sums, kx0 rows
+    HIPCHK(c, launch_kx0_rows(g, d, 0.0, s, psc, slot_rows + 2 * g.nwb + 4));
     HIPCHK(c, launch_jj_poles(g, d, 0.0, s, psc));
     const int nblk = (int)((g.nq + jj_points(g.Nw) - 1) / jj_points(g.Nw));
     HIPCHK(c, launch_jj_pencil(g, d, 0.0, d.jjpart, nullptr, 1, s, psc));
-    HIPCHK(c, launch_reduce_cols(d.jjpart, nblk, 2 * g.nwb, d.jjred, s));
-    HIPCHK(c, launch_reduce_cols(d.jjscal, nblk, 4, d.jjred + 2 * g.nwb, s));
-    HIPCHK(c, launch_queue_gather(g, d, slot_dev, s));
+    // the column sums straight into the slot, and the point scalars with them (block 0)
+    HIPCHK(c, launch_reduce_cols2(d.jjpart, 2 * g.nwb, slot_rows, d.jjscal, 4, slot_rows + 2 * g.nwb, nblk, s, psc,
+                                  kPscDoubles, slot_dev));
     return 0;
 }
 

@@ -717,10 +717,6 @@ __device__ __forceinline__ void store_self(cf* Self, long o, cd R, double Sinf, 
 }
 
 // Sigma_inf = -(Sigma(k=0,0) + Sigma(k=0,beta)) (green.cpp:1996); one thread
-__global__ void k_sigma_inf(const double* __restrict__ Sb, long nk8, double* Sinf)
-{
-    if (threadIdx.x == 0 && blockIdx.x == 0) Sinf[0] = -(Sb[0] + Sb[nk8]);
-}
 // Boundary data of Sigma(k,tau) in FP64, block b: side = b % 2 (0: tau = 0, 1: tau = beta),
 // pass = b / 2: the slope dSigma/dtau (pass 0, green.cpp:1743-1817) or the value
 // Sigma(k,tau_b) = DCT[G0(r,tau_b) V(r,0)] (pass 1, green.cpp:1822-1892 at j = 0, Nw;
@@ -837,7 +833,7 @@ template <int L, typename S, typename SelfT, int NT = sigma_pen_threads<SelfT>(L
 __global__ void __launch_bounds__(NT)
     k_sigma_pencil(const S* __restrict__ Stau, const double* __restrict__ Sb, const double* __restrict__ dSig,
                    long nk8, double T, SelfT* Self,
-                   double* Sinf2, double* Sinf3, double* Sinf4, const double* __restrict__ Sinfp,
+                   double* Sinf2, double* Sinf3, double* Sinf4, double* Sinfp,
                    const cd* __restrict__ tw, const cd* __restrict__ twX, const cd* __restrict__ twS)
 {
     __shared__ double y[L + 1];  // the samples, then in place their spline moments
@@ -862,7 +858,9 @@ __global__ void __launch_bounds__(NT)
     }
     __syncthreads();
     block_fft<L, NT>(X, +1, tw);
-    const double s2 = d0 + dN, s3 = -(M[0] + M[L]), Si = Sinfp[0];
+    // Sigma_inf = -(Sigma(k=0, 0+) + Sigma(k=0, beta-)) from the boundary values; stored by k = 0
+    const double s2 = d0 + dN, s3 = -(M[0] + M[L]), Si = -(Sb[0] + Sb[nk8]);
+    if (k == 0 && tid == 0) Sinfp[0] = Si;
     for (int n = tid; n < L / 2; n += nt) {
         const cd R = twS[n] * X[n];
         store_self(Self, k * (L / 2) + n, R, Si, s2, s3, (2 * n + 1) * kPi * T);
@@ -1057,12 +1055,12 @@ __global__ void __launch_bounds__(256)
 
 // column sums of a row-major (nrows x ncols) partial table in a fixed order: a block of
 // 64 columns x 16 row groups, row group partials added in order
-__global__ void __launch_bounds__(1024) k_reduce_cols(const double* __restrict__ part, int nrows, int ncols,
-                                                      double* out)
+__device__ __forceinline__ void reduce_cols_block(const double* __restrict__ part, int nrows, int ncols, double* out,
+                                                  int blk)
 {
     __shared__ double t[16][65];
     const int cx = threadIdx.x & 63, ry = threadIdx.x >> 6;
-    const int c = blockIdx.x * 64 + cx;
+    const int c = blk * 64 + cx;
     double a = 0.0;
     if (c < ncols)
         for (int r = ry; r < nrows; r += 16) a += part[(long)r * ncols + c];
@@ -1073,6 +1071,26 @@ __global__ void __launch_bounds__(1024) k_reduce_cols(const double* __restrict__
         for (int i = 0; i < 16; i++) sum += t[i][cx];
         out[c] = sum;
     }
+}
+__global__ void __launch_bounds__(1024) k_reduce_cols(const double* __restrict__ part, int nrows, int ncols,
+                                                      double* out)
+{
+    reduce_cols_block(part, nrows, ncols, out, blockIdx.x);
+}
+// two tables of the same row count in one launch (blocks of the first, then of the second);
+// block 0 also copies ncopy doubles from csrc to cdst (null: nothing)
+__global__ void __launch_bounds__(1024) k_reduce_cols2(const double* __restrict__ p1, int ncols1, double* o1,
+                                                       const double* __restrict__ p2, int ncols2, double* o2,
+                                                       int nrows, const double* __restrict__ csrc, int ncopy,
+                                                       double* cdst)
+{
+    const int b1 = (ncols1 + 63) / 64;
+    if ((int)blockIdx.x < b1)
+        reduce_cols_block(p1, nrows, ncols1, o1, blockIdx.x);
+    else
+        reduce_cols_block(p2, nrows, ncols2, o2, blockIdx.x - b1);
+    if (csrc && blockIdx.x == 0)
+        for (int i = threadIdx.x; i < ncopy; i += blockDim.x) cdst[i] = csrc[i];
 }
 
 // dispersion tables on the device (hamiltonien.cpp:49-68,125-155; green0 wedge weights)
@@ -1437,11 +1455,14 @@ __global__ void k_jj_poles(const double* __restrict__ ek, const double* __restri
 // [j][q] (j < nj, q < nq) -> [q][j]: kTT x kTT tiles through LDS, both sides coalesced (a
 // wave moves 64 consecutive elements of one row per access; 16 accesses per thread in flight).
 // Rows of odd length (nq = nk (nk - 2), nj = Nw + 1) rule out vector accesses.  The Lambda
-// pencils then read each tau-pencil as one contiguous row.
+// pencils then read each tau-pencil as one contiguous row.  Two arrays per launch (grid z).
 constexpr int kTT = 64;
 template <typename S>
-__global__ void __launch_bounds__(256) k_transpose(const S* __restrict__ in, S* __restrict__ out, long nq, int nj)
+__global__ void __launch_bounds__(256) k_transpose(const S* __restrict__ in0, S* __restrict__ out0,
+                                                   const S* __restrict__ in1, S* __restrict__ out1, long nq, int nj)
 {
+    const S* __restrict__ in = blockIdx.z ? in1 : in0;  // z: which of the two arrays
+    S* __restrict__ out = blockIdx.z ? out1 : out0;
     __shared__ S t[kTT][kTT + 1];
     const long q0 = (long)blockIdx.x * kTT;
     const int j0 = blockIdx.y * kTT;
@@ -2041,23 +2062,6 @@ hipError_t launch_halley_init(HalleyState* st, const HalleyState& s0, const doub
     return hipGetLastError();
 }
 
-// the results of a queued solve into its slot (pinned host memory, written through its device
-// address): d.psc, the reduced Lambda rows with the pencil's four scalar sums, the kx = 0, pi
-// rows of the diamagnetic term
-__global__ void __launch_bounds__(256) k_queue_gather(const double* __restrict__ psc, const double* __restrict__ jjred,
-                                                      int njj, const double* __restrict__ rows, int nrows, double* out)
-{
-    for (int i = threadIdx.x; i < kPscDoubles + njj + nrows; i += blockDim.x)
-        out[i] = i < kPscDoubles ? psc[i] : i < kPscDoubles + njj ? jjred[i - kPscDoubles] : rows[i - kPscDoubles - njj];
-}
-
-hipError_t launch_queue_gather(const Grid& g, const DevArrays& d, double* out, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_queue_gather, dim3(1), dim3(256), 0, s, d.psc, d.jjred, 2 * g.nwb + 4, d.kx0part, 2 * g.nk,
-                       out);
-    return hipGetLastError();
-}
-
 hipError_t launch_sum_rule3_pass(const Grid& g, const DevArrays& d, double sgn, double* mirror, hipStream_t s)
 {
     const double Nq = 4.0 * (g.nk - 1) * (g.nk - 1);
@@ -2082,6 +2086,15 @@ hipError_t launch_model(const Grid& g, const DevArrays& d, double t, double tp, 
 hipError_t launch_reduce_cols(const double* part, int nrows, int ncols, double* out, hipStream_t s)
 {
     hipLaunchKernelGGL(k_reduce_cols, dim3((ncols + 63) / 64), dim3(1024), 0, s, part, nrows, ncols, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_cols2(const double* p1, int ncols1, double* o1, const double* p2, int ncols2, double* o2,
+                               int nrows, hipStream_t s, const double* csrc, int ncopy, double* cdst)
+{
+    const unsigned nb = (unsigned)((ncols1 + 63) / 64 + (ncols2 + 63) / 64);
+    hipLaunchKernelGGL(k_reduce_cols2, dim3(nb), dim3(1024), 0, s, p1, ncols1, o1, p2, ncols2, o2, nrows, csrc, ncopy,
+                       cdst);
     return hipGetLastError();
 }
 
@@ -2145,7 +2158,6 @@ hipError_t launch_sigma_slab(const Grid& g, const DevArrays& d, double mu0, hipS
 
 hipError_t launch_sigma_pencil(const Grid& g, const DevArrays& d, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_sigma_inf, dim3(1), dim3(64), 0, s, d.Sb, g.nk8, d.Sinf);
     DISPATCH_S(g.prec, DISPATCH_L(g.Nw, {
         hipLaunchKernelGGL((k_sigma_pencil<LL, SS, ST>), dim3((unsigned)g.nk8), dim3(sigma_pen_threads<ST>(LL)), 0, s,
                            (const SS*)d.Stau, d.Sb, d.dSig, g.nk8, g.T, (ST*)d.Self, d.Sinf2, d.Sinf3, d.Sinf4, d.Sinf,
@@ -2242,10 +2254,10 @@ hipError_t launch_jj_poles(const Grid& g, const DevArrays& d, double mu, hipStre
 
 hipError_t launch_jj_transpose(const Grid& g, const DevArrays& d, hipStream_t s)
 {
-    const dim3 gr((unsigned)((g.nq + kTT - 1) / kTT), (unsigned)((g.Nw + 1 + kTT - 1) / kTT));
+    const dim3 gr((unsigned)((g.nq + kTT - 1) / kTT), (unsigned)((g.Nw + 1 + kTT - 1) / kTT), 2);
     DISPATCH_S(g.prec, {
-        hipLaunchKernelGGL(k_transpose<SS>, gr, dim3(256), 0, s, (const SS*)d.h, (SS*)d.hT, g.nq, g.Nw + 1);
-        hipLaunchKernelGGL(k_transpose<SS>, gr, dim3(256), 0, s, (const SS*)d.f0, (SS*)d.f0T, g.nq, g.Nw + 1);
+        hipLaunchKernelGGL(k_transpose<SS>, gr, dim3(256), 0, s, (const SS*)d.h, (SS*)d.hT, (const SS*)d.f0,
+                           (SS*)d.f0T, g.nq, g.Nw + 1);
     });
     return hipGetLastError();
 }
@@ -2315,7 +2327,7 @@ hipError_t launch_gloc(const Grid& g, const DevArrays& d, double mu, double* par
 hipError_t launch_transpose_d(const double* in, double* out, long ncols, int nrows, hipStream_t s)
 {
     const dim3 gr((unsigned)((ncols + kTT - 1) / kTT), (unsigned)((nrows + kTT - 1) / kTT));
-    hipLaunchKernelGGL(k_transpose<double>, gr, dim3(256), 0, s, in, out, ncols, nrows);
+    hipLaunchKernelGGL(k_transpose<double>, gr, dim3(256), 0, s, in, out, in, out, ncols, nrows);  // grid z = 1
     return hipGetLastError();
 }
 
@@ -2327,11 +2339,12 @@ hipError_t launch_sum_rows(const double* part, int nrows, int n, double* out, hi
     return hipGetLastError();
 }
 
-hipError_t launch_kx0_rows(const Grid& g, const DevArrays& d, double mu, hipStream_t s, const double* dsc)
+hipError_t launch_kx0_rows(const Grid& g, const DevArrays& d, double mu, hipStream_t s, const double* dsc,
+                           double* out)
 {
     DISPATCH_S(g.prec, {
         hipLaunchKernelGGL(k_kx0_rows<ST>, dim3(2 * g.nk), dim3(256), 0, s, d.ek, d.d2ek, (const ST*)d.Self, d.Sinf2,
-                           d.Sinf3, d.Sinf, g.N, g.Nw, g.T, mu, d.kx0part, dsc);
+                           d.Sinf3, d.Sinf, g.N, g.Nw, g.T, mu, out ? out : d.kx0part, dsc);
     });
     return hipGetLastError();
 }

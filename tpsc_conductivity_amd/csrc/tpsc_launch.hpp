@@ -194,7 +194,9 @@ hipError_t launch_mu0_root(const Grid& g, const DevArrays& d, double n, double p
 hipError_t launch_model(const Grid& g, const DevArrays& d, double t, double tp, double tpp, double t3,
                         hipStream_t s);
 hipError_t launch_reduce_cols(const double* part, int nrows, int ncols, double* out, hipStream_t s);
-hipError_t launch_queue_gather(const Grid& g, const DevArrays& d, double* out, hipStream_t s);
+hipError_t launch_reduce_cols2(const double* p1, int ncols1, double* o1, const double* p2, int ncols2, double* o2,
+                               int nrows, hipStream_t s, const double* csrc = nullptr, int ncopy = 0,
+                               double* cdst = nullptr);
 // Optional device sources of the point scalars (the queued solve of tpsc_capi.cpp; null: the values
 // passed): dsc = {Usp, Uch, mu} (device), from which the kernels form U(3Usp+Uch)/8 and /4 with the
 // host's rounding; mu0p = the device mu0.
@@ -225,7 +227,8 @@ hipError_t launch_narrow(const double* in, float* out, long n, hipStream_t s);
 // out[r*ntheta + b]: the sum of row r of part (nrows x n) over the columns k with bins[k] == b
 hipError_t launch_sum_rows_binned(const double* part, int nrows, int n, const int* bins, int ntheta, double* out,
                                   hipStream_t s);
-hipError_t launch_kx0_rows(const Grid& g, const DevArrays& d, double mu, hipStream_t s, const double* dsc = nullptr);
+hipError_t launch_kx0_rows(const Grid& g, const DevArrays& d, double mu, hipStream_t s, const double* dsc = nullptr,
+                           double* out = nullptr);
 hipError_t launch_finalize(const double* part, int n, double* out, hipStream_t s);
 
 constexpr int kGlocChunks = 64;  // k chunks of k_gloc (partials: kGlocChunks x Nw doubles)
