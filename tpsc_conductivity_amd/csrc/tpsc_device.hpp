@@ -506,6 +506,20 @@ constexpr double spline_zpow_sq(int e)
     }
     return r;
 }
+// z^(S0 + C i), i = 0..I, as compile-time constants: v[i] is the product of S0 + C i factors z
+// taken in turn from 1, as a run-time loop p *= z forms it (the same roundings)
+template <int I>
+struct SplineZTable {
+    double v[I + 1];
+};
+template <int S0, int C, int I>
+constexpr SplineZTable<I> spline_ztable()
+{
+    SplineZTable<I> t{};
+    for (int i = 0; i <= I; i++) t.v[i] = spline_zpow(S0 + C * i);
+    return t;
+}
+constexpr int kZSelMax = 16;  // longest table picked by selects instead of a multiply loop
 constexpr int spline_carry_terms(double zc, int maxd)
 {
     int d = 0;
@@ -546,6 +560,8 @@ __device__ __forceinline__ void block_spline_fixed(double* const* yM, double h, 
     constexpr double zN1 = spline_zpow_sq(N - 1);
     constexpr double dg = 2.0 + z, off = zN1 * (1.0 + 2.0 * z);
     constexpr double idet = 1.0 / (dg * dg - off * off);
+    constexpr int TL = 2 * KW / C;                                     // last thread with j0 <= 2 KW
+    constexpr int MH = (2 * KW >= cN + 1) ? (2 * KW - cN - 1) / C + 1 : 0;  // last m with N - jl <= 2 KW
     const int tid = threadIdx.x;
     const int j0 = tid * C;
     const bool full = j0 + C - 1 <= N;
@@ -631,11 +647,22 @@ __device__ __forceinline__ void block_spline_fixed(double* const* yM, double h, 
             const double RN = bnd[8 * k + 1] - bnd[8 * k + 4] - 2.0 * bnd[8 * k + 5];
             const double alpha = (dg * R0 - off * RN) * idet;
             const double beta = (dg * RN - off * R0) * idet;
-            // z^j and z^(N-j) of this chunk by repeated multiplication from 1
+            // z^j and z^(N-j) of this chunk by repeated multiplication from the chunk's first
+            // power: z^j0 (thread tid <= TL) and z^(N-jl) (thread tN - m, m <= MH: exponent
+            // cN + 1 + (m - 1) C) picked from compile-time tables, no serial loop
             double zlo[C], zhi[C];
+            // the thread index through an empty asm, so the picks stay here instead of being
+            // hoisted out of a caller's loop into registers held across it
+            int ts = tid;
+            asm volatile("" : "+v"(ts));
             double zp = 1.0;
-            if (j0 <= 2 * KW)
+            if constexpr (TL <= kZSelMax) {
+                constexpr auto tlo = spline_ztable<0, C, TL>();
+#pragma unroll
+                for (int t = 1; t <= TL; t++) zp = (ts == t) ? tlo.v[t] : zp;
+            } else if (j0 <= 2 * KW) {
                 for (int e = 0; e < j0; e++) zp *= z;
+            }
 #pragma unroll
             for (int c = 0; c < C; c++) {
                 zlo[c] = zp;
@@ -643,8 +670,15 @@ __device__ __forceinline__ void block_spline_fixed(double* const* yM, double h, 
             }
             const int jl = min(j0 + C - 1, N);
             zp = 1.0;
-            if (N - jl <= 2 * KW)
+            if constexpr (MH <= kZSelMax) {
+                if constexpr (MH >= 1) {
+                    constexpr auto thi = spline_ztable<cN + 1, C, MH - 1>();
+#pragma unroll
+                    for (int m = 1; m <= MH; m++) zp = (ts == tN - m) ? thi.v[m - 1] : zp;
+                }
+            } else if (N - jl <= 2 * KW) {
                 for (int e = 0; e < N - jl; e++) zp *= z;
+            }
 #pragma unroll
             for (int c = C - 1; c >= 0; c--) {
                 zhi[c] = zp;
