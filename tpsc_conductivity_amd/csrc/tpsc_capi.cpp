@@ -527,6 +527,7 @@ int stage_chi0(tpsc_ctx* c, int nq0, int nw0, bool defer = false)
     HIPCHK(c, launch_chi0_pencil(gx, dx, s));
     prof_end(c);
     if (f64) HIPCHK(c, launch_narrow(c->chirt64, (float*)c->d.chirt, (long)c->g.nwb * c->g.nk8, s));
+    HIPCHK(c, launch_sr_moments(c->g, c->d, s));
     HIPCHK(c, launch_chi0max(c->g, c->d, s));
     HIPCHK(c, hipMemcpyAsync(c->host_scalar + 19, c->d.red, sizeof(double), hipMemcpyDeviceToHost, s));
     HIPCHK(c, hipMemcpyAsync(c->host_scalar + 20, c->d.imax, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -1469,6 +1470,8 @@ int tpsc_create(const tpsc_params* p, tpsc_ctx** out)
     rc |= dalloc(c, &d.dchiqt0, nk8);
     rc |= dalloc(c, &d.chi0inf, nk8);
     rc |= dalloc(c, &d.chi0inf2, nk8);
+    rc |= dalloc(c, &d.srm, (size_t)(Nw + 1) * kSRM);
+    rc |= dalloc(c, &d.srmax, (size_t)(Nw + 1));
     rc |= dalloc(c, &d.V, (size_t)nwb * nk8, es);
     rc |= dalloc(c, &d.Stau, (size_t)(Nw + 1) * nk8, es);
     rc |= dalloc(c, &d.dSig, 2 * nk8);
@@ -2192,6 +2195,7 @@ int tpsc_queue_solve(tpsc_ctx* c, const tpsc_params* p, double* slot_dev, bool c
     HIPCHK(c, launch_chi0_tau0(gx, dx, s));
     HIPCHK(c, launch_chi0_pencil(gx, dx, s));
     if (c->chi0_fp64) HIPCHK(c, launch_narrow(c->chirt64, (float*)d.chirt, (long)g.nwb * g.nk8, s));
+    HIPCHK(c, launch_sr_moments(g, d, s));
     HIPCHK(c, launch_chi0max(g, d, s, psc + kPscChi0max));
     // Usp and Uch (stage_UspUch_grid): the same starts, brackets and host-known parts, with Uch's
     // constant formed on the device from Usp; each search settles into its own mirror in psc

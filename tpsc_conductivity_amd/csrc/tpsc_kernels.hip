@@ -520,13 +520,25 @@ __device__ __forceinline__ void sum_rule3_term(double c, double w, double sU, do
     acc[1] -= w * sgn * 0.5 * c * c * r * r;
     acc[2] += w * 0.5 * c * c * c * r * r * r;
 }
+// Rows of small chi: with y = -sU/2 and |y c| <= rho for every element of rows j >= n0,
+//   c r - c = sum_{k>=1} y^k c^(k+1),  c^2 r^2 = sum_{k>=0} (k+1) y^k c^(k+2),
+//   c^3 r^3 = sum_{k>=0} (k+1)(k+2)/2 y^k c^(k+3),   r = 1/(1 - y c),
+// so their part of the three sums is a polynomial in y over the moments
+// M_m(n0) = sum_{j>=n0} sum_q w c^m, m = 2..kSRK+3, stored once per chi0 (k_sr_moments,
+// k_sr_suffix).  rho = 1/16 and kSRK = 18: the first omitted term of the c^3 r^3 series is
+// below 210 * 16^-19 ~ 3e-21 of the moment M_3 it is measured against.  Every chi0(q,iqn) and
+// its asymptotic continuation are positive, so the moments carry no cancellation.
+constexpr double kSRRho = 1.0 / 16.0;
 // Ux (and T) come from the root search st, so that a captured pass graph serves every point;
 // a settled search leaves nothing to do.  ticket != null: the workgroup that finishes last takes
-// the Halley step.
+// the Halley step.  srm != null: rows j >= n0(Ux) (all rows whose elements satisfy
+// |Ux c/2| <= rho, from the suffix maxima srmax) are not read; workgroup (0, 0) adds their
+// part from the suffix moments srm[n0].
 __global__ void __launch_bounds__(256)
     k_sum_rule3(const double* __restrict__ chi, const double* __restrict__ inf1, const double* __restrict__ inf2,
                 const double* __restrict__ wq, int nk8, int nwmax, double Nq, double sgn, HalleyState* st,
-                double* part, unsigned* ticket, double* mirror)
+                double* part, unsigned* ticket, double* mirror, const double* __restrict__ srm,
+                const double* __restrict__ srmax)
 {
     __shared__ double sc[3 * 4];
     if (st->status != kRootRunning) return;
@@ -534,30 +546,133 @@ __global__ void __launch_bounds__(256)
     const int q = blockIdx.x * 256 + threadIdx.x;
     const int RG = gridDim.y;
     const double sU = sgn * Ux;
+    const int nrows = 2 * nwmax + 1;
+    int n0 = nrows;
+    if (srm) {
+        // srmax is non-increasing in j: n0 = the number of rows above the threshold
+        const double thr = kSRRho / (0.5 * fabs(Ux));
+        double cnt = 0.0;
+        for (int j = threadIdx.x; j < nrows; j += 256) cnt += (srmax[j] > thr) ? 1.0 : 0.0;
+        n0 = (int)block_sum(cnt, sc);
+    }
     double acc[3] = {0.0, 0.0, 0.0};
     if (q < nk8) {
         const double wb = 2.0 * wq[q] * T / Nq;
         const double a1 = inf1[q], a2 = inf2[q];
+        const int jst = min(nwmax, n0 - 1), jas = min(2 * nwmax, n0 - 1);
         int j = blockIdx.y;
-        for (; j + (kSR - 1) * RG <= nwmax; j += kSR * RG) {
+        for (; j + (kSR - 1) * RG <= jst; j += kSR * RG) {
             double c[kSR];
 #pragma unroll
             for (int u = 0; u < kSR; u++) c[u] = chi[(long)(j + u * RG) * nk8 + q];
 #pragma unroll
             for (int u = 0; u < kSR; u++) sum_rule3_term(c[u], (j + u * RG == 0) ? 0.5 * wb : wb, sU, sgn, acc);
         }
-        for (; j <= nwmax; j += RG) sum_rule3_term(chi[(long)j * nk8 + q], (j == 0) ? 0.5 * wb : wb, sU, sgn, acc);
-        for (; j <= 2 * nwmax; j += RG) {
+        for (; j <= jst; j += RG) sum_rule3_term(chi[(long)j * nk8 + q], (j == 0) ? 0.5 * wb : wb, sU, sgn, acc);
+        for (; j <= jas; j += RG) {
             const double wn = 2 * j * kPi * T;
             const double i2 = rcp_nr(wn * wn);
             sum_rule3_term((a1 + a2 * i2) * i2, wb, sU, sgn, acc);
         }
+    }
+    if (srm && n0 < nrows && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+        const double* M = srm + (long)n0 * kSRM;  // M[m - 2] = M_m(n0)
+        const double y = -0.5 * sU;
+        double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+        for (int k = kSRK; k >= 0; k--) {
+            if (k >= 1) f0 = f0 * y + M[k - 1];
+            f1 = f1 * y + (k + 1) * M[k];
+            f2 = f2 * y + 0.5 * (k + 1) * (k + 2) * M[k + 1];
+        }
+        acc[0] += y * f0;
+        acc[1] -= sgn * 0.5 * f1;
+        acc[2] += 0.5 * f2;
     }
     block_sum_k<3>(acc, sc);
     const int nblk = gridDim.x * gridDim.y, b = blockIdx.y * gridDim.x + blockIdx.x;
     if (threadIdx.x == 0)
         for (int k = 0; k < 3; k++) part[k * nblk + b] = acc[k];
     if (ticket && last_workgroup(ticket, nblk)) halley_step_block(part, nblk, st, mirror, sc);
+}
+
+// Row moments of the sum-rule terms (one workgroup per row j <= 2 nwmax, stored rows from chi,
+// the rest from the asymptote a1/wn^2 + a2/wn^4 as k_sum_rule3 forms them): R[j][m - 2] =
+// sum_q w_j wb_q c^m, m = 2..kSRK+3 (w_0 = 1/2), and rmax[j] = max_q |c|.
+__global__ void __launch_bounds__(256)
+    k_sr_moments(const double* __restrict__ chi, const double* __restrict__ inf1, const double* __restrict__ inf2,
+                 const double* __restrict__ wq, int nk8, int nwmax, double Nq, double T, double* R, double* rmax)
+{
+    __shared__ double sc[kSRM * 4];
+    const int j = blockIdx.x;
+    double acc[kSRM];
+#pragma unroll
+    for (int m = 0; m < kSRM; m++) acc[m] = 0.0;
+    double mx = 0.0;
+    const double wn = 2 * j * kPi * T;
+    const double i2 = (j > nwmax) ? rcp_nr(wn * wn) : 0.0;
+    for (int q = threadIdx.x; q < nk8; q += 256) {
+        const double wb = 2.0 * wq[q] * T / Nq * ((j == 0) ? 0.5 : 1.0);
+        const double c = (j <= nwmax) ? chi[(long)j * nk8 + q] : (inf1[q] + inf2[q] * i2) * i2;
+        double pw = wb * c * c;
+#pragma unroll
+        for (int m = 0; m < kSRM; m++) {
+            acc[m] += pw;
+            pw *= c;
+        }
+        mx = fmax(mx, fabs(c));
+    }
+    block_sum_k<kSRM>(acc, sc);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int m = 0; m < kSRM; m++) R[(long)j * kSRM + m] = acc[m];
+    // block maximum through the same scratch
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
+    if (lane == 0) sc[wave] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) rmax[j] = fmax(fmax(sc[0], sc[1]), fmax(sc[2], sc[3]));
+}
+
+// Suffix sums (columns 0..kSRM-1 of R, in place) and suffix maxima (rmax, in place) over the
+// rows 0..nrows-1: column b of one workgroup of 1024 threads, thread t owning rows
+// [t ch, t ch + ch); the thread totals are combined by a fixed-order scan, so the result does
+// not depend on timing.
+__global__ void __launch_bounds__(1024) k_sr_suffix(double* R, double* rmax, int nrows)
+{
+    __shared__ double tot[1024];
+    const int b = blockIdx.x;  // b < kSRM: a moment column; b == kSRM: the maxima
+    const bool ismax = b == kSRM;
+    const int t = threadIdx.x;
+    const int ch = (nrows + 1023) / 1024;
+    constexpr int kMaxCh = 8;  // nrows <= 8192
+    double v[kMaxCh];
+    double s = 0.0;
+    for (int i = ch - 1; i >= 0; i--) {
+        const int j = t * ch + i;
+        const double x = (j < nrows) ? (ismax ? rmax[j] : R[(long)j * kSRM + b]) : 0.0;
+        s = ismax ? fmax(s, x) : s + x;
+        v[i] = s;
+    }
+    tot[t] = s;
+    __syncthreads();
+    // inclusive suffix scan of the thread totals (Hillis-Steele, fixed order)
+    for (int o = 1; o < 1024; o <<= 1) {
+        const double a = tot[t], c = (t + o < 1024) ? tot[t + o] : 0.0;
+        __syncthreads();
+        tot[t] = ismax ? fmax(a, c) : a + c;
+        __syncthreads();
+    }
+    const double after = (t + 1 < 1024) ? tot[t + 1] : 0.0;  // rows of the threads after t
+    for (int i = 0; i < ch; i++) {
+        const int j = t * ch + i;
+        if (j < nrows) {
+            if (ismax)
+                rmax[j] = fmax(v[i], after);
+            else
+                R[(long)j * kSRM + b] = v[i] + after;
+        }
+    }
 }
 
 // wedge points per thread in k_mu0_root: 16 waves hold the 8385 points of N = 256
@@ -2069,8 +2184,20 @@ hipError_t launch_sum_rule3_pass(const Grid& g, const DevArrays& d, double sgn, 
     const int nblk = (int)(gr.x * gr.y);
     const bool fused = nblk <= kFusedStepMax;
     hipLaunchKernelGGL(k_sum_rule3, gr, dim3(256), 0, s, d.chiw, d.chi0inf, d.chi0inf2, d.wq, (int)g.nk8, g.Nw / 2,
-                       Nq, sgn, d.root, d.part, fused ? d.ticket : nullptr, mirror);
+                       Nq, sgn, d.root, d.part, fused ? d.ticket : nullptr, mirror, d.srm, d.srmax);
     if (!fused) hipLaunchKernelGGL(k_halley_step, dim3(1), dim3(256), 0, s, d.part, nblk, d.root, mirror);
+    return hipGetLastError();
+}
+
+hipError_t launch_sr_moments(const Grid& g, const DevArrays& d, hipStream_t s)
+{
+    if (!d.srm) return hipSuccess;
+    const double Nq = 4.0 * (g.nk - 1) * (g.nk - 1);
+    const int nrows = g.Nw + 1;  // 2 nwmax + 1
+    if (nrows > 8192) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_sr_moments, dim3(nrows), dim3(256), 0, s, d.chiw, d.chi0inf, d.chi0inf2, d.wq, (int)g.nk8,
+                       g.Nw / 2, Nq, g.T, d.srm, d.srmax);
+    hipLaunchKernelGGL(k_sr_suffix, dim3(kSRM + 1), dim3(1024), 0, s, d.srm, d.srmax, nrows);
     return hipGetLastError();
 }
 

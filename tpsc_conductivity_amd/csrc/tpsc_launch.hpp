@@ -26,6 +26,8 @@ struct DevArrays {
     double* dchiqt0 = nullptr;   // nk8
     double* chi0inf = nullptr;   // nk8
     double* chi0inf2 = nullptr;  // nk8
+    double* srm = nullptr;       // (Nw+1)*kSRM suffix moments of the sum-rule terms (k_sr_moments)
+    double* srmax = nullptr;     // Nw+1        suffix maxima of |chi0| per row
     double* V = nullptr;         // nwb*nk8
     double* Stau = nullptr;      // (Nw+1)*nk8
     double* dSig = nullptr;      // 2*nk8  dSigma/dtau at tau = 0, beta
@@ -155,6 +157,9 @@ constexpr int kPartCap = 8192;       // capacity of DevArrays::part
 // L2) costs more there than the launch it saves (256^2: sum-rule passes of 1056 workgroups,
 // trace passes of 2097).
 constexpr int kFusedStepMax = 640;
+// series terms of the sum-rule passes' small-chi rows and the moments they take, M_2 .. M_{K+3}
+constexpr int kSRK = 18;
+constexpr int kSRM = kSRK + 2;
 constexpr int kJJK = 16;  // k-points per Lambda workgroup for Nw > 256
 // Lambda pencil shape: up to Nw = 256 one wavefront per workgroup (its barriers are then free)
 // and 4 points per workgroup, so that small grids still spread over the CUs; beyond, 256
@@ -188,6 +193,7 @@ hipError_t launch_halley_init(HalleyState* st, const HalleyState& s0, const doub
                               double a0A = 0.0, double nsr = 0.0, double U = 0.0);
 // psc[slot], psc[3 + slot], psc[6 + slot] = root, status, evaluations of the search in st
 hipError_t launch_sum_rule3_pass(const Grid& g, const DevArrays& d, double sgn, double* mirror, hipStream_t s);
+hipError_t launch_sr_moments(const Grid& g, const DevArrays& d, hipStream_t s);
 hipError_t launch_trace_diffG3_pass(const Grid& g, const DevArrays& d, double* mirror, hipStream_t s);
 // green0::find_mu on the device (one workgroup, the whole bracketing search): out = {found, mu0, evaluations}
 hipError_t launch_mu0_root(const Grid& g, const DevArrays& d, double n, double preset, double* out, hipStream_t s);
