@@ -2183,15 +2183,17 @@ hipError_t launch_sum_rule3_pass(const Grid& g, const DevArrays& d, double sgn, 
     const dim3 gr = red_grid(g);
     const int nblk = (int)(gr.x * gr.y);
     const bool fused = nblk <= kFusedStepMax;
+    const bool series = d.srm && g.nk8 * g.nwb >= kSRMinElems;  // the moments exist (launch_sr_moments)
     hipLaunchKernelGGL(k_sum_rule3, gr, dim3(256), 0, s, d.chiw, d.chi0inf, d.chi0inf2, d.wq, (int)g.nk8, g.Nw / 2,
-                       Nq, sgn, d.root, d.part, fused ? d.ticket : nullptr, mirror, d.srm, d.srmax);
+                       Nq, sgn, d.root, d.part, fused ? d.ticket : nullptr, mirror, series ? d.srm : nullptr,
+                       series ? d.srmax : nullptr);
     if (!fused) hipLaunchKernelGGL(k_halley_step, dim3(1), dim3(256), 0, s, d.part, nblk, d.root, mirror);
     return hipGetLastError();
 }
 
 hipError_t launch_sr_moments(const Grid& g, const DevArrays& d, hipStream_t s)
 {
-    if (!d.srm) return hipSuccess;
+    if (!d.srm || g.nk8 * g.nwb < kSRMinElems) return hipSuccess;
     const double Nq = 4.0 * (g.nk - 1) * (g.nk - 1);
     const int nrows = g.Nw + 1;  // 2 nwmax + 1
     if (nrows > 8192) return hipErrorInvalidValue;

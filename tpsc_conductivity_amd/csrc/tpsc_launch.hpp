@@ -160,6 +160,9 @@ constexpr int kFusedStepMax = 640;
 // series terms of the sum-rule passes' small-chi rows and the moments they take, M_2 .. M_{K+3}
 constexpr int kSRK = 18;
 constexpr int kSRM = kSRK + 2;
+// ... used on grids of at least this many chi0(q,iqn) values (16 MB in FP64): on smaller ones a
+// pass is launch-bound, and the two moment kernels would cost more than the rows they save
+constexpr long kSRMinElems = 2L << 20;
 constexpr int kJJK = 16;  // k-points per Lambda workgroup for Nw > 256
 // Lambda pencil shape: up to Nw = 256 one wavefront per workgroup (its barriers are then free)
 // and 4 points per workgroup, so that small grids still spread over the CUs; beyond, 256
