@@ -116,9 +116,12 @@ enum tpsc_scalar {
     TPSC_S_STAGES, /* bitmask of the completed stages, TPSC_STAGE_* (no reference counterpart:
                       the reference keeps this state in its members) */
     TPSC_S_N_THETA, /* the angle bins N_theta_k of the last tpsc_chijj call */
-    TPSC_S_CHI0_FP64 /* 1 if the chi0 stage of the full grid runs in FP64: always in the FP64 path;
+    TPSC_S_CHI0_FP64, /* 1 if the chi0 stage of the full grid runs in FP64: always in the FP64 path;
                         in the mixed path once the point's spin enhancement 1/(1 - Usp chi0max/2)
                         exceeded 8 (no reference counterpart) */
+    TPSC_S_ROOTS_ON_DEVICE /* bitmask of the root searches of the last solve (1: Usp, 2: Uch, 4: mu)
+                              that settled in the device Halley iteration rather than in the host
+                              bracketed fallback (no reference counterpart) */
 };
 enum tpsc_stage {
     TPSC_STAGE_CHI0 = 1, TPSC_STAGE_USP_UCH = 2, TPSC_STAGE_VERTEX = 4, TPSC_STAGE_SIGMA = 8,

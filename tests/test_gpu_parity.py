@@ -196,6 +196,9 @@ def test_headline_config_parity_and_invariants(oracle_mod):
     print("[parity] cfg3 256x2048: HIP solve", flush=True)
     g = CondOpt(256, 2048, 0.9, 0.05, params=(-0.2, 0.1, 0.0, 5.0), coarse_pass=False)
     g.solve()
+    # the three root searches settle on the device (on this grid the sum-rule passes take the
+    # small-chi0 rows as a moment series: wrong moments would send them to the host fallback)
+    assert g.scalar("roots_on_device") == 7
     print("[parity] cfg3 256x2048: oracle solve", flush=True)
     o = oracle_mod.Oracle(256, 2048, 0.9, 0.05, 5.0, tp=-0.2, tpp=0.1, coarse_pass=False)
     o.solve()

@@ -32,6 +32,7 @@ SCALARS = {
     "trSG2": 14, "dblOcc2": 15, "kx0": 16, "chijj1_inf2": 17, "chijj2_inf2": 18,
     "chijj2_inf4": 19, "nevals_sp": 20, "nevals_ch": 21, "nevals_mu": 22, "nk": 23, "nk8": 24,
     "nwb": 25, "nq": 26, "al_N": 27, "al_mmin": 28, "stages": 29, "n_theta": 30, "chi0_fp64": 31,
+    "roots_on_device": 32,
 }
 
 

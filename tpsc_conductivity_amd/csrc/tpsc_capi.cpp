@@ -60,6 +60,7 @@ struct tpsc_ctx {
     double mu = 0.0;
     bool mu_ok = false;
     int nev_sp = 0, nev_ch = 0, nev_mu = 0;
+    int roots_dev = 0;  // searches of the last solve settled on the device: 1 Usp, 2 Uch, 4 mu
     std::vector<double> chijj1, chijj2;
     int ntheta = 1;                            // angle bins of the last Lambda_jj call
     std::vector<double> chijj1_k, chijj2_k;    // angle-binned Lambda_jj1, Lambda_jj2, [b + n*ntheta]
@@ -527,7 +528,9 @@ int stage_chi0(tpsc_ctx* c, int nq0, int nw0, bool defer = false)
     HIPCHK(c, launch_chi0_pencil(gx, dx, s));
     prof_end(c);
     if (f64) HIPCHK(c, launch_narrow(c->chirt64, (float*)c->d.chirt, (long)c->g.nwb * c->g.nk8, s));
+    prof_begin(c, "sr_moments");
     HIPCHK(c, launch_sr_moments(c->g, c->d, s));
+    prof_end(c);
     HIPCHK(c, launch_chi0max(c->g, c->d, s));
     HIPCHK(c, hipMemcpyAsync(c->host_scalar + 19, c->d.red, sizeof(double), hipMemcpyDeviceToHost, s));
     HIPCHK(c, hipMemcpyAsync(c->host_scalar + 20, c->d.imax, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -607,6 +610,7 @@ int stage_UspUch_grid(tpsc_ctx* c, int nq0, int nw0, double Usc[2], bool* ok_out
     if (!rc) rc = chi0_ready(c);
     if (rc) return rc;
     double init[3] = {U / 2.0, 0.0, 2.0 / c->chi0max - 2 * DBL_EPSILON};
+    c->roots_dev = (c->roots_dev & ~1) | (hok ? 1 : 0);
     if (!hok) hok = find_zero(fsp, init, &root, nullptr, &c->nev_sp);
     else c->nev_sp = nh;
     if (rc) return rc;
@@ -623,6 +627,7 @@ int stage_UspUch_grid(tpsc_ctx* c, int nq0, int nw0, double Usc[2], bool* ok_out
                                     n - n * n / 2.0 - (nsr + c->Usp * nsr * nsr / (2.0 * U) - nsr * nsr), 0.0),
                        &root, &nh, &hok);
     if (rc) return rc;
+    c->roots_dev = (c->roots_dev & ~2) | (hok ? 2 : 0);
     if (!hok) hok = find_zero(fch, init, &root, nullptr, &c->nev_ch);
     else c->nev_ch = nh;
     if (rc) return rc;
@@ -818,6 +823,7 @@ int stage_mu(tpsc_ctx* c)
     bool ok = false;
     rc = device_halley(c, kRootMu, pass, halley_start(init[0], lims[0], lims[1], 40, 0.0, 0.0), &root, &nh, &ok);
     if (rc) return rc;
+    c->roots_dev = (c->roots_dev & ~4) | (ok ? 4 : 0);
     if (ok)
         c->nev_mu = nh;
     else
@@ -2092,6 +2098,7 @@ int tpsc_get_scalar(tpsc_ctx* c, int which, double* v)
             break;
         case TPSC_S_N_THETA: *v = c->ntheta; break;
         case TPSC_S_CHI0_FP64: *v = (c->p.precision == TPSC_PREC_FP64 || c->chi0_fp64) ? 1.0 : 0.0; break;
+        case TPSC_S_ROOTS_ON_DEVICE: *v = c->roots_dev; break;
         default: return fail(c, TPSC_E_INVALID, "unknown scalar");
     }
     return 0;

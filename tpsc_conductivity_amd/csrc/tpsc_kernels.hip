@@ -613,12 +613,19 @@ __global__ void __launch_bounds__(256)
     for (int q = threadIdx.x; q < nk8; q += 256) {
         const double wb = 2.0 * wq[q] * T / Nq * ((j == 0) ? 0.5 : 1.0);
         const double c = (j <= nwmax) ? chi[(long)j * nk8 + q] : (inf1[q] + inf2[q] * i2) * i2;
-        double pw = wb * c * c;
+        // c^i, i < kSRM, as a product tree (c^i = c^h c^(i-h), h the largest power of two below
+        // i): five multiplies deep instead of a chain of kSRM
+        double cp[kSRM];
+        cp[0] = 1.0;
+        cp[1] = c;
 #pragma unroll
-        for (int m = 0; m < kSRM; m++) {
-            acc[m] += pw;
-            pw *= c;
+        for (int i = 2; i < kSRM; i++) {
+            const int h = 1 << (31 - __builtin_clz(i - 1));
+            cp[i] = cp[h] * cp[i - h];
         }
+        const double w2 = wb * c * c;
+#pragma unroll
+        for (int m = 0; m < kSRM; m++) acc[m] = fma(w2, cp[m], acc[m]);
         mx = fmax(mx, fabs(c));
     }
     block_sum_k<kSRM>(acc, sc);
