@@ -226,6 +226,27 @@ def test_headline_config_parity_and_invariants(oracle_mod):
     assert abs((c1[0] + c2[0] + g.scalar("kx0")) / g.scalar("kx0")) < 1e-3
 
 
+def test_sum_rule_series_grid_parity(oracle_mod):
+    """192^2 x 1024 (2.4M chi0 values: the sum-rule passes take the small-chi0 rows as a moment
+    series) at a stronger coupling than the headline point, FP64: the three roots settle on the
+    device and Usp, Uch, mu equal the oracle's (direct sums) to 1e-10."""
+    from tpsc_conductivity_amd.tpsc import CondOpt
+
+    print("[parity] series grid 192x1024 U=6: HIP solve", flush=True)
+    g = CondOpt(192, 1024, 0.85, 0.1, params=(-0.15, 0.0, 0.0, 6.0), coarse_pass=False, precision="fp64")
+    g.solve()
+    assert g.scalar("roots_on_device") == 7
+    print("[parity] series grid 192x1024 U=6: oracle solve", flush=True)
+    o = oracle_mod.Oracle(192, 1024, 0.85, 0.1, 6.0, tp=-0.15, tpp=0.0, coarse_pass=False)
+    o.solve()
+    s = o.scalars()
+    for key in ("mu0", "Usp", "Uch", "mu"):
+        assert abs(g.scalar(key) - s[key]) <= TIGHT * abs(s[key]), key
+    assert abs(g.sum_rule_sp(g.scalar("Usp"))) < 1e-13
+    assert abs(g.sum_rule_ch(g.scalar("Uch"))) < 1e-13
+    g.close()
+
+
 def test_pade_conductivity_from_the_solve(oracle_mod):
     """sigma(w) from the HIP Lambda_jj(iq_n) equals the oracle's own Pade restatement applied to
     the same values (to rounding), and agrees with the continuation of the oracle's Lambda_jj.
