@@ -229,7 +229,8 @@ def test_headline_config_parity_and_invariants(oracle_mod):
 def test_sum_rule_series_grid_parity(oracle_mod):
     """192^2 x 1024 (2.4M chi0 values: the sum-rule passes take the small-chi0 rows as a moment
     series) at a stronger coupling than the headline point, FP64: the three roots settle on the
-    device and Usp, Uch, mu equal the oracle's (direct sums) to 1e-10."""
+    device and Usp, Uch, mu equal the oracle's (direct sums) to 1e-10; the mixed path's roots
+    settle on the device as well and stay within 1e-6 of FP64."""
     from tpsc_conductivity_amd.tpsc import CondOpt
 
     print("[parity] series grid 192x1024 U=6: HIP solve", flush=True)
@@ -244,6 +245,14 @@ def test_sum_rule_series_grid_parity(oracle_mod):
         assert abs(g.scalar(key) - s[key]) <= TIGHT * abs(s[key]), key
     assert abs(g.sum_rule_sp(g.scalar("Usp"))) < 1e-13
     assert abs(g.sum_rule_ch(g.scalar("Uch"))) < 1e-13
+    # the mixed path on the same grid: its roots settle on the device too, within 1e-6 of FP64
+    print("[parity] series grid 192x1024 U=6: mixed HIP solve", flush=True)
+    gm = CondOpt(192, 1024, 0.85, 0.1, params=(-0.15, 0.0, 0.0, 6.0), coarse_pass=False, precision="mixed")
+    gm.solve()
+    assert gm.scalar("roots_on_device") == 7
+    for key in ("Usp", "Uch", "mu"):
+        assert abs(gm.scalar(key) - g.scalar(key)) <= 1e-6 * abs(g.scalar(key)), key
+    gm.close()
     g.close()
 
 
